@@ -1,0 +1,252 @@
+/*
+ * rtbvh.h — C ABI of librtbvh_amd.so, the MI355X (gfx950) implementation of
+ * meirbon/rtbvh's hot path: BVH construction (locally ordered clustering /
+ * PLOC, binned SAH, 4-wide MBVH collapse) and closest-hit / any-hit ray
+ * traversal.
+ *
+ * Part 1 is the reference's own C ABI (the cbindgen output of
+ * rtbvh_ffi/src/lib.rs, include guard RTBVH_H, rtbvh_ffi/build.rs:15-45):
+ * same structs, enum values, symbols and argument meaning, so existing C/C++
+ * callers link unchanged.  Trees are built on the GPU and mirrored to host
+ * memory owned by the library.
+ *
+ * Part 2 is additive: device-resident, stream-ordered entry points that take
+ * device pointers and a HIP stream (passed as void*).  They have no
+ * counterpart in the reference; each cites the reference behaviour it
+ * reproduces.
+ *
+ * Error convention: every entry point returns a ResultCode and never aborts.
+ * Codes 0-4 keep the reference's values (rtbvh_ffi/src/lib.rs:17-25); codes
+ * 5-9 are appended.  rtbvh_last_error() describes the last failure on the
+ * calling thread.
+ */
+#ifndef RTBVH_H
+#define RTBVH_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===================================================================== */
+/* Part 1 — rtbvh_ffi ABI                                                 */
+/* ===================================================================== */
+
+/* rtbvh_ffi/src/lib.rs:17-25 (values 0-4); 5-9 appended. */
+typedef enum ResultCode {
+    Ok = 0,
+    Error = 1,
+    NoPrimitives = 2,
+    InequalAabbsAndPrimitives = 3,
+    Nan = 4,
+    DeviceError = 5,     /* a HIP call failed */
+    OutOfMemory = 6,     /* device or host allocation failed */
+    InvalidArgument = 7, /* bad stride, size, flag or pointer kind */
+    StackOverflow = 8,   /* a ray needed more than RTBVH_MAX_STACK entries */
+    NoDevice = 9,        /* no usable gfx950 device */
+} ResultCode;
+
+/* rtbvh_ffi/src/lib.rs:129-133 (#[repr(u32)]): a 32-bit value in both
+ * languages. */
+#ifdef __cplusplus
+enum BvhType : uint32_t {
+    LocallyOrderedClustered = 0,
+    BinnedSAH = 1,
+};
+#else
+enum BvhType {
+    LocallyOrderedClustered = 0,
+    BinnedSAH = 1,
+};
+typedef uint32_t BvhType;
+#endif
+
+/* rtbvh_ffi/src/lib.rs:144-151 — 32 bytes, identical to rtbvh::Aabb<i32>. */
+typedef struct RTAabb {
+    float min[3];
+    int32_t count;
+    float max[3];
+    int32_t left_first;
+} RTAabb;
+
+/* rtbvh_ffi/src/lib.rs:175-179 — count >= 0: leaf with `count` primitives at
+ * indices[left_first..]; count == -1: interior, children at left_first and
+ * left_first + 1. */
+typedef struct RTBvhNode {
+    RTAabb aabb;
+} RTBvhNode;
+
+/* rtbvh_ffi/src/lib.rs:197-208 — 128 bytes, four child slots in SoA form.
+ * counts[i] >= 0: leaf slot (children[i] = first index); counts[i] == -1 and
+ * children[i] >= 0: interior slot (children[i] = MBVH node); both -1: empty. */
+typedef struct RTMbvhNode {
+    float min_x[4];
+    float max_x[4];
+    float min_y[4];
+    float max_y[4];
+    float min_z[4];
+    float max_z[4];
+    int32_t children[4];
+    int32_t counts[4];
+} RTMbvhNode;
+
+/* rtbvh_ffi/src/lib.rs:210-220 — passed by value; the pointers refer to
+ * library-owned memory (host memory for Part 1 handles, device memory for
+ * Part 2 handles). */
+typedef struct RTBvh {
+    uint32_t id;
+    uint32_t node_count;
+    const RTBvhNode *nodes;
+    uint32_t index_count;
+    const uint32_t *indices;
+} RTBvh;
+
+/* rtbvh_ffi/src/lib.rs:234-244 */
+typedef struct RTMbvh {
+    uint32_t id;
+    uint32_t node_count;
+    const RTMbvhNode *nodes;
+    uint32_t index_count;
+    const uint32_t *indices;
+} RTMbvh;
+
+/* rtbvh_ffi/src/lib.rs:428-493.  Builds a BVH over prim_count primitives
+ * given by their centres (center_stride 12 or 16 bytes) and, optionally, their
+ * aabbs (NULL: point boxes at the centres).  Null centres/result -> Error;
+ * prim_count 0 -> NoPrimitives; a stride other than 12 or 16 ->
+ * InvalidArgument (the reference panics).  The tree is built on the GPU and
+ * mirrored to host memory; *result points at that mirror. */
+ResultCode create_bvh(const RTAabb *aabbs, size_t prim_count, const float *centers,
+                      size_t center_stride, size_t prims_per_leaf, BvhType bvh_type,
+                      RTBvh *result);
+
+/* rtbvh_ffi/src/lib.rs:499-513.  Collapses a tree made by create_bvh into the
+ * 4-wide MBVH (on the GPU).  Unknown id / null pointers -> Error. */
+ResultCode create_mbvh(RTBvh bvh, RTMbvh *mbvh);
+
+/* rtbvh_ffi/src/lib.rs:519-538.  Recomputes every node's bounds from new
+ * per-primitive aabbs (prim_count of them), in place, keeping the topology.
+ * The reference's Bvh::refit also resets count/left_first to 0 (src/bvh.rs:
+ * 179,203); this implementation keeps them. */
+ResultCode refit(const RTAabb *aabbs, RTBvh bvh);
+
+/* rtbvh_ffi/src/lib.rs:551-581, 599-686, 700-731, 749-835.  Candidate
+ * iteration with a host callback per candidate primitive, in the reference's
+ * visit order; the callback may lower *t and returns true to stop early.
+ * These run on the host because the callback is host code. */
+ResultCode intersect(RTBvh bvh, const float *origin, const float *direction, float *t,
+                     void *user_data, bool (*intersect)(uint32_t, float *, void *));
+ResultCode intersect_packet(RTBvh bvh, const float *origin_x, const float *origin_y,
+                            const float *origin_z, const float *direction_x,
+                            const float *direction_y, const float *direction_z, float *t,
+                            void *user_data, bool (*intersect)(uint32_t, float *, void *));
+ResultCode intersect_mbvh(RTMbvh bvh, const float *origin, const float *direction, float *t,
+                          void *user_data, bool (*intersect)(uint32_t, float *, void *));
+ResultCode intersect_mbvh_packet(RTMbvh bvh, const float *origin_x, const float *origin_y,
+                                 const float *origin_z, const float *direction_x,
+                                 const float *direction_y, const float *direction_z, float *t,
+                                 void *user_data,
+                                 bool (*intersect)(uint32_t, float *, void *));
+
+/* rtbvh_ffi/src/lib.rs:838-849.  Unknown ids print to stderr, as in the
+ * reference. */
+void free_bvh(RTBvh bvh);
+void free_mbvh(RTMbvh bvh);
+
+/* ===================================================================== */
+/* Part 2 — device-resident GPU entry points (additive)                   */
+/* ===================================================================== */
+
+/* 32-byte ray: Ray::new (src/ray.rs:166-182) takes origin and direction,
+ * t_min defaults to 1e-4 and t (the current maximum) to 1e34; the FFI sets t
+ * from the caller (rtbvh_ffi/src/lib.rs:565-566). */
+typedef struct RTRay {
+    float origin[3];
+    float t_min;
+    float direction[3];
+    float t_max;
+} RTRay;
+
+/* 16-byte closest-hit record.  prim_id == RTBVH_MISS and t == t_max on a
+ * miss; u, v are the barycentrics of the Moller-Trumbore test
+ * (src/builders/spatial_sah.rs:132-163). */
+typedef struct RTHit {
+    float t;
+    float u;
+    float v;
+    uint32_t prim_id;
+} RTHit;
+
+#define RTBVH_MISS 0xFFFFFFFFu
+/* Maximum traversal stack entries per ray (the reference has 32, unchecked:
+ * src/iter.rs:25). */
+#define RTBVH_MAX_STACK 256
+
+/* Trace flags.  Default (0): the closest hit is the lexicographic minimum of
+ * (t, prim_id) over all triangles — identical to brute force.  With
+ * RTBVH_TRACE_REFERENCE_ORDER the kernel follows the reference iterator's
+ * visit order exactly (src/iter.rs:308-354 for the MBVH, :88-127 for the
+ * BVH) and, on exactly equal t, keeps the first accepted candidate as the
+ * reference's closest-hit driver does (examples/benchmark.rs:33-37). */
+#define RTBVH_TRACE_REFERENCE_ORDER 0x1u
+
+/* GPU build, same arguments as create_bvh but every pointer is device memory
+ * and the work is enqueued on `stream` (hipStream_t; NULL = default stream).
+ * *result receives device pointers owned by the library; free with
+ * rtbvh_gpu_free_bvh.  The tree is complete when the stream reaches this
+ * point.  bvh_type LocallyOrderedClustered = PLOC with search radius 14
+ * (src/builders/locb.rs), BinnedSAH = 16-bin SAH (src/builders/binned_sah.rs). */
+ResultCode rtbvh_gpu_create_bvh(const RTAabb *aabbs, size_t prim_count, const float *centers,
+                                size_t center_stride, size_t prims_per_leaf, BvhType bvh_type,
+                                void *stream, RTBvh *result);
+
+/* GPU MBVH collapse (Mbvh::construct, src/bvh.rs:381-404) of a tree made by
+ * rtbvh_gpu_create_bvh. */
+ResultCode rtbvh_gpu_create_mbvh(RTBvh bvh, void *stream, RTMbvh *result);
+
+/* GPU refit of a tree made by rtbvh_gpu_create_bvh (aabbs: device, one per
+ * primitive). */
+ResultCode rtbvh_gpu_refit(const RTAabb *aabbs, RTBvh bvh, void *stream);
+
+/* Closest-hit traversal of ray_count rays.  `bvh`/`mbvh` may be any tree whose
+ * nodes/indices point at device memory (from rtbvh_gpu_create_*, or uploaded
+ * by the caller).  vertices: device, 9 floats (v0, v1, v2) per primitive,
+ * prim_count primitives.  rays/hits: device arrays of ray_count records.
+ * Asynchronous on `stream`; call rtbvh_gpu_status() after synchronising to
+ * learn whether any ray overflowed its stack. */
+ResultCode rtbvh_gpu_intersect(RTBvh bvh, const float *vertices, size_t prim_count,
+                               const RTRay *rays, size_t ray_count, RTHit *hits, uint32_t flags,
+                               void *stream);
+ResultCode rtbvh_gpu_intersect_mbvh(RTMbvh mbvh, const float *vertices, size_t prim_count,
+                                    const RTRay *rays, size_t ray_count, RTHit *hits,
+                                    uint32_t flags, void *stream);
+
+/* Any-hit (shadow) traversal: occluded[i] = 1 when some triangle is hit with
+ * t_min < t < t_max, else 0 (the FFI's early-out callback, rtbvh_ffi/src/
+ * lib.rs:571-576). */
+ResultCode rtbvh_gpu_occluded(RTBvh bvh, const float *vertices, size_t prim_count,
+                              const RTRay *rays, size_t ray_count, uint8_t *occluded,
+                              uint32_t flags, void *stream);
+ResultCode rtbvh_gpu_occluded_mbvh(RTMbvh mbvh, const float *vertices, size_t prim_count,
+                                   const RTRay *rays, size_t ray_count, uint8_t *occluded,
+                                   uint32_t flags, void *stream);
+
+/* Synchronises `stream`, then returns and clears the device status of the
+ * current device: Ok, or StackOverflow when a traversal ran out of stack
+ * since the last call. */
+ResultCode rtbvh_gpu_status(void *stream);
+
+void rtbvh_gpu_free_bvh(RTBvh bvh);
+void rtbvh_gpu_free_mbvh(RTMbvh mbvh);
+
+/* Human-readable description of the last error on this thread ("" if none). */
+const char *rtbvh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTBVH_H */

@@ -1,0 +1,300 @@
+"""Host-side mirror of rtbvh's Builder / Bvh / Mbvh API over the C ABI.
+
+Reference interface (src/bvh.rs):
+  Builder { aabbs: Option<&[Aabb]>, primitives: &[T], primitives_per_leaf }   :48-56
+  Builder::construct_binned_sah / construct_locally_ordered_clustered         :87-137
+      -> Result<Bvh, BuildError>; BuildError::{NoPrimitives,
+         InequalAabbsAndPrimitives(aabbs, prims)}                              :26-46
+  Bvh::refit(&[Aabb])                                                          :176-205
+  Mbvh::from(Bvh) / Mbvh::construct(&Bvh)                                      :381-404,446-450
+  Bvh/Mbvh::traverse_iter* (candidate iterators)                               :258-283,418-443
+
+On the GPU the candidate iterator and the caller's triangle test are fused:
+``intersect`` returns the closest hit per ray and ``occluded`` the any-hit
+flag.  Tensors are torch tensors on a ROCm device:
+  primitives  (N, 3) or (N, 4) float32 centres (center_stride 12 / 16 bytes)
+  aabbs       (N, 8) float32 RTAabb records, or None
+  vertices    (N, 9) float32 triangles v0, v1, v2
+  rays        (R, 8) float32: origin.xyz, t_min, direction.xyz, t_max
+  hits        (R, 4) float32: t, u, v, prim id bits (RTBVH_MISS on a miss)
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import torch
+
+from . import _lib as L
+
+RAY_FLOATS = 8
+HIT_FLOATS = 4
+AABB_FLOATS = 8
+MBVH_FLOATS = 32
+
+
+class BuildError(Exception):
+    """rtbvh::BuildError (src/bvh.rs:26-46)."""
+
+    def __init__(self, kind: str, aabbs: int = 0, prims: int = 0):
+        self.kind = kind
+        self.aabbs = aabbs
+        self.prims = prims
+        if kind == "NoPrimitives":
+            msg = "No primitives supplied"
+        else:
+            msg = f"#Aabbs({aabbs}) != #Primitives({prims})"
+        super().__init__(msg)
+
+    def __eq__(self, other):
+        return isinstance(other, BuildError) and (self.kind, self.aabbs, self.prims) == (
+            other.kind, other.aabbs, other.prims)
+
+    def __hash__(self):
+        return hash((self.kind, self.aabbs, self.prims))
+
+
+def _stream(t: torch.Tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def _ptr(t: Optional[torch.Tensor]) -> ctypes.c_void_p:
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _require_device(t: torch.Tensor, name: str, dtype=torch.float32):
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise TypeError(f"{name} must be a torch tensor on a ROCm device")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+def rays_tensor(origins: torch.Tensor, directions: torch.Tensor, t_min: float = 1e-4,
+                t_max: float = 1e34) -> torch.Tensor:
+    """Pack origins/directions into (R, 8) ray records (Ray::new defaults,
+    src/ray.rs:163-182)."""
+    n = origins.shape[0]
+    r = torch.empty((n, RAY_FLOATS), dtype=torch.float32, device=origins.device)
+    r[:, 0:3] = origins
+    r[:, 3] = t_min
+    r[:, 4:7] = directions
+    r[:, 7] = t_max
+    return r
+
+
+def hits_prim(hits: torch.Tensor) -> torch.Tensor:
+    """Primitive ids of (R, 4) hit records as int64 (-1 on a miss)."""
+    p = hits[:, 3].contiguous().view(torch.int32).to(torch.int64)
+    return torch.where(p < 0, p & 0xFFFFFFFF, p).masked_fill(p == -1, -1)
+
+
+def _hip():
+    """libamdhip64 for device-to-device copies of library-owned arrays."""
+    global _HIP
+    try:
+        return _HIP
+    except NameError:
+        pass
+    _HIP = ctypes.CDLL("libamdhip64.so")
+    _HIP.hipMemcpyAsync.restype = ctypes.c_int
+    _HIP.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                    ctypes.c_int, ctypes.c_void_p]
+    return _HIP
+
+
+def _copy_out(ptr: int, count: int, floats_per: int, dtype, device) -> torch.Tensor:
+    out = torch.empty((count, floats_per), dtype=dtype, device=device)
+    if count:
+        nbytes = count * floats_per * 4
+        e = _hip().hipMemcpyAsync(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ptr),
+                                  nbytes, 3, _stream(out))  # hipMemcpyDeviceToDevice
+        if e != 0:
+            raise RuntimeError(f"hipMemcpyAsync failed ({e})")
+    return out
+
+
+class _Tree:
+    _kind = "bvh"
+
+    def __init__(self, handle, device, owned: bool, keep=None):
+        self._h = handle
+        self.device = device
+        self._owned = owned
+        self._keep = keep  # tensors that back a non-owned handle
+
+    @property
+    def node_count(self) -> int:
+        return int(self._h.node_count)
+
+    @property
+    def index_count(self) -> int:
+        return int(self._h.index_count)
+
+    def prim_count(self) -> int:
+        return self.index_count
+
+    def indices(self) -> torch.Tensor:
+        """prim_indices as an int32 device tensor (copy)."""
+        if self._keep is not None:
+            return self._keep[1].clone()
+        return _copy_out(self._h.indices, self.index_count, 1, torch.int32, self.device).view(-1)
+
+    def _trace(self, fn, vertices, rays, out, flags):
+        _require_device(rays, "rays")
+        if rays.dim() != 2 or rays.shape[1] != RAY_FLOATS:
+            raise ValueError("rays must be (R, 8) float32")
+        _require_device(vertices, "vertices")
+        verts = vertices.reshape(-1)
+        if verts.numel() % 9:
+            raise ValueError("vertices must hold 9 floats per triangle")
+        code = fn(self._h, _ptr(verts), verts.numel() // 9, _ptr(rays), rays.shape[0],
+                  _ptr(out), flags, _stream(rays))
+        L.check(code, fn.__name__)
+        return out
+
+    def status(self):
+        """Synchronise and report whether a ray overflowed its stack."""
+        L.check(L.lib().rtbvh_gpu_status(_stream(torch.empty(0, device=self.device))),
+                "rtbvh_gpu_status")
+
+
+class Bvh(_Tree):
+    """rtbvh::Bvh (src/bvh.rs:143-147) resident in device memory."""
+
+    @classmethod
+    def from_arrays(cls, nodes: torch.Tensor, indices: torch.Tensor) -> "Bvh":
+        """Wrap caller-owned device arrays (nodes (M, 8) float32 RTBvhNode
+        records, indices int32) without copying, like the FFI's from_slices."""
+        _require_device(nodes, "nodes")
+        _require_device(indices, "indices", torch.int32)
+        h = L.RTBvh(0xFFFFFFFF, nodes.shape[0], nodes.data_ptr(), indices.numel(),
+                    indices.data_ptr())
+        return cls(h, nodes.device, owned=False, keep=(nodes, indices))
+
+    def nodes(self) -> torch.Tensor:
+        """Nodes as (M, 8) float32 RTBvhNode records (copy)."""
+        if self._keep is not None:
+            return self._keep[0].clone()
+        return _copy_out(self._h.nodes, self.node_count, AABB_FLOATS, torch.float32,
+                         self.device)
+
+    def intersect(self, vertices, rays, reference_order: bool = False, out=None):
+        """Closest hit per ray -> (R, 4) hit records."""
+        out = torch.empty((rays.shape[0], HIT_FLOATS), dtype=torch.float32,
+                          device=rays.device) if out is None else out
+        flags = L.TRACE_REFERENCE_ORDER if reference_order else 0
+        return self._trace(L.lib().rtbvh_gpu_intersect, vertices, rays, out, flags)
+
+    def occluded(self, vertices, rays, reference_order: bool = False, out=None):
+        """Any hit per ray -> uint8 (R,)."""
+        out = torch.empty((rays.shape[0],), dtype=torch.uint8,
+                          device=rays.device) if out is None else out
+        flags = L.TRACE_REFERENCE_ORDER if reference_order else 0
+        return self._trace(L.lib().rtbvh_gpu_occluded, vertices, rays, out, flags)
+
+    def refit(self, aabbs: torch.Tensor):
+        """Bvh::refit (src/bvh.rs:176-205) on the GPU, in place."""
+        if not self._owned:
+            raise ValueError("refit needs a tree built by this library")
+        _require_device(aabbs, "aabbs")
+        if aabbs.shape != (self.index_count, AABB_FLOATS):
+            raise ValueError("aabbs must be (prim_count, 8)")
+        L.check(L.lib().rtbvh_gpu_refit(_ptr(aabbs), self._h, _stream(aabbs)), "rtbvh_gpu_refit")
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and L._lib is not None:
+            L._lib.rtbvh_gpu_free_bvh(self._h)
+            self._owned = False
+
+
+class Mbvh(_Tree):
+    """rtbvh::Mbvh (src/bvh.rs:320-324) resident in device memory."""
+
+    @classmethod
+    def from_bvh(cls, bvh: Bvh) -> "Mbvh":
+        """From<Bvh> for Mbvh (src/bvh.rs:446-450): GPU collapse."""
+        if not bvh._owned:
+            raise ValueError("Mbvh.from_bvh needs a tree built by this library")
+        h = L.RTMbvh()
+        s = _stream(torch.empty(0, device=bvh.device))
+        L.check(L.lib().rtbvh_gpu_create_mbvh(bvh._h, s, ctypes.byref(h)),
+                "rtbvh_gpu_create_mbvh")
+        return cls(h, bvh.device, owned=True)
+
+    @classmethod
+    def from_arrays(cls, nodes: torch.Tensor, indices: torch.Tensor) -> "Mbvh":
+        """Wrap caller-owned device arrays (nodes (M, 32) float32 RTMbvhNode
+        records, indices int32)."""
+        _require_device(nodes, "nodes")
+        _require_device(indices, "indices", torch.int32)
+        h = L.RTMbvh(0xFFFFFFFF, nodes.shape[0], nodes.data_ptr(), indices.numel(),
+                     indices.data_ptr())
+        return cls(h, nodes.device, owned=False, keep=(nodes, indices))
+
+    def quad_nodes(self) -> torch.Tensor:
+        """MBVH nodes as (M, 32) float32 RTMbvhNode records (copy)."""
+        if self._keep is not None:
+            return self._keep[0].clone()
+        return _copy_out(self._h.nodes, self.node_count, MBVH_FLOATS, torch.float32,
+                         self.device)
+
+    def intersect(self, vertices, rays, reference_order: bool = False, out=None):
+        out = torch.empty((rays.shape[0], HIT_FLOATS), dtype=torch.float32,
+                          device=rays.device) if out is None else out
+        flags = L.TRACE_REFERENCE_ORDER if reference_order else 0
+        return self._trace(L.lib().rtbvh_gpu_intersect_mbvh, vertices, rays, out, flags)
+
+    def occluded(self, vertices, rays, reference_order: bool = False, out=None):
+        out = torch.empty((rays.shape[0],), dtype=torch.uint8,
+                          device=rays.device) if out is None else out
+        flags = L.TRACE_REFERENCE_ORDER if reference_order else 0
+        return self._trace(L.lib().rtbvh_gpu_occluded_mbvh, vertices, rays, out, flags)
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and L._lib is not None:
+            L._lib.rtbvh_gpu_free_mbvh(self._h)
+            self._owned = False
+
+
+class Builder:
+    """rtbvh::Builder (src/bvh.rs:48-56): primitives are the centres."""
+
+    def __init__(self, primitives: torch.Tensor, aabbs: Optional[torch.Tensor] = None,
+                 primitives_per_leaf: Optional[int] = None):
+        self.primitives = primitives
+        self.aabbs = aabbs
+        self.primitives_per_leaf = primitives_per_leaf
+
+    def _construct(self, bvh_type: int) -> Bvh:
+        n = 0 if self.primitives is None else int(self.primitives.shape[0])
+        if n == 0:  # bvh.rs:88-90,114-116
+            raise BuildError("NoPrimitives")
+        if self.aabbs is not None and int(self.aabbs.shape[0]) != n:  # bvh.rs:92-98,118-124
+            raise BuildError("InequalAabbsAndPrimitives", int(self.aabbs.shape[0]), n)
+        _require_device(self.primitives, "primitives")
+        if self.primitives.dim() != 2 or self.primitives.shape[1] not in (3, 4):
+            raise ValueError("primitives must be (N, 3) or (N, 4) float32 centres")
+        if self.aabbs is not None:
+            _require_device(self.aabbs, "aabbs")
+            if self.aabbs.shape[1] != AABB_FLOATS:
+                raise ValueError("aabbs must be (N, 8) RTAabb records")
+        h = L.RTBvh()
+        stride = 4 * int(self.primitives.shape[1])
+        ppl = self.primitives_per_leaf or 0
+        code = L.lib().rtbvh_gpu_create_bvh(_ptr(self.aabbs), n, _ptr(self.primitives), stride,
+                                            ppl, bvh_type, _stream(self.primitives),
+                                            ctypes.byref(h))
+        if code == L.NO_PRIMITIVES:
+            raise BuildError("NoPrimitives")
+        L.check(code, "rtbvh_gpu_create_bvh")
+        return Bvh(h, self.primitives.device, owned=True)
+
+    def construct_locally_ordered_clustered(self) -> Bvh:
+        """src/bvh.rs:113-137 — PLOC (search radius 14) on the GPU."""
+        return self._construct(L.LOCALLY_ORDERED_CLUSTERED)
+
+    def construct_binned_sah(self) -> Bvh:
+        """src/bvh.rs:87-111 — 16-bin SAH on the GPU."""
+        return self._construct(L.BINNED_SAH)

@@ -1,0 +1,199 @@
+// C ABI: device-resident traversal entry points and library-wide helpers.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "internal.hpp"
+
+namespace rtbvh {
+
+namespace {
+thread_local char g_last_error[512] = {0};
+
+// One status word per device (kStatus* bits), cleared by rtbvh_gpu_status().
+__device__ uint32_t g_device_status;
+
+bool device_accessible(const void* p) {
+    if (p == nullptr) return false;
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    switch (attr.type) {
+        case hipMemoryTypeDevice:
+        case hipMemoryTypeManaged:
+        case hipMemoryTypeUnified:
+            return true;
+        case hipMemoryTypeHost:
+            return attr.devicePointer != nullptr;
+        default:
+            return false;
+    }
+}
+
+ResultCode check_device(const void* p, const char* name) {
+    if (!device_accessible(p)) {
+        set_error("%s (%p) is not device-accessible memory", name, p);
+        return InvalidArgument;
+    }
+    return Ok;
+}
+
+}  // namespace
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+    va_end(ap);
+}
+
+ResultCode hip_fail(hipError_t e, const char* what) {
+    set_error("%s: %s", what, hipGetErrorString(e));
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return OutOfMemory;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return NoDevice;
+    return DeviceError;
+}
+
+uint32_t* device_status_ptr() {
+    static uint32_t* ptrs[64] = {nullptr};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    if (ptrs[dev] == nullptr) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(g_device_status)) != hipSuccess) return nullptr;
+        ptrs[dev] = static_cast<uint32_t*>(p);
+    }
+    return ptrs[dev];
+}
+
+namespace {
+
+ResultCode trace_entry(const void* nodes, uint32_t node_count, const uint32_t* indices,
+                       uint32_t index_count, const float* vertices, size_t prim_count,
+                       const RTRay* rays, size_t ray_count, RTHit* hits, uint8_t* occluded,
+                       uint32_t flags, void* stream, bool mbvh) {
+    if (flags & ~RTBVH_TRACE_REFERENCE_ORDER) {
+        set_error("unknown trace flags 0x%x", flags);
+        return InvalidArgument;
+    }
+    if (prim_count > 0xFFFFFFFFull) {
+        set_error("prim_count %zu exceeds 2^32-1", prim_count);
+        return InvalidArgument;
+    }
+    if (ray_count == 0) return Ok;
+    ResultCode rc;
+    if ((rc = check_device(rays, "rays")) != Ok) return rc;
+    if (hits && (rc = check_device(hits, "hits")) != Ok) return rc;
+    if (occluded && (rc = check_device(occluded, "occluded")) != Ok) return rc;
+    if (node_count > 0) {
+        if ((rc = check_device(nodes, "nodes")) != Ok) return rc;
+        if (index_count > 0 && (rc = check_device(indices, "indices")) != Ok) return rc;
+        if (prim_count > 0 && (rc = check_device(vertices, "vertices")) != Ok) return rc;
+    }
+    if ((reinterpret_cast<uintptr_t>(rays) & 15) || (hits && (reinterpret_cast<uintptr_t>(hits) & 15)) ||
+        (reinterpret_cast<uintptr_t>(nodes) & 15)) {
+        set_error("rays, hits and nodes must be 16-byte aligned");
+        return InvalidArgument;
+    }
+    TraceParams p{};
+    p.mnodes = mbvh ? static_cast<const DevMbvhNode*>(nodes) : nullptr;
+    p.bnodes = mbvh ? nullptr : static_cast<const DevAabb*>(nodes);
+    p.node_count = node_count;
+    p.index_count = indices ? index_count : 0;
+    p.indices = indices;
+    p.verts = vertices;
+    p.prim_count = vertices ? uint32_t(prim_count) : 0;
+    p.rays = reinterpret_cast<const DevRay*>(rays);
+    p.ray_count = ray_count;
+    p.hits = reinterpret_cast<DevHit*>(hits);
+    p.occluded = occluded;
+    p.status = device_status_ptr();
+    if (p.status == nullptr) {
+        set_error("device status word unavailable");
+        return DeviceError;
+    }
+    const hipError_t e = launch_trace(p, mbvh, (flags & RTBVH_TRACE_REFERENCE_ORDER) != 0,
+                                      occluded != nullptr, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "trace launch");
+    return Ok;
+}
+
+}  // namespace
+}  // namespace rtbvh
+
+using namespace rtbvh;
+
+extern "C" {
+
+const char* rtbvh_last_error(void) { return g_last_error; }
+
+ResultCode rtbvh_gpu_intersect(RTBvh bvh, const float* vertices, size_t prim_count,
+                               const RTRay* rays, size_t ray_count, RTHit* hits, uint32_t flags,
+                               void* stream) {
+    if (hits == nullptr && ray_count > 0) {
+        set_error("hits is null");
+        return Error;
+    }
+    return trace_entry(bvh.nodes, bvh.node_count, bvh.indices, bvh.index_count, vertices,
+                       prim_count, rays, ray_count, hits, nullptr, flags, stream, false);
+}
+
+ResultCode rtbvh_gpu_intersect_mbvh(RTMbvh mbvh, const float* vertices, size_t prim_count,
+                                    const RTRay* rays, size_t ray_count, RTHit* hits,
+                                    uint32_t flags, void* stream) {
+    if (hits == nullptr && ray_count > 0) {
+        set_error("hits is null");
+        return Error;
+    }
+    return trace_entry(mbvh.nodes, mbvh.node_count, mbvh.indices, mbvh.index_count, vertices,
+                       prim_count, rays, ray_count, hits, nullptr, flags, stream, true);
+}
+
+ResultCode rtbvh_gpu_occluded(RTBvh bvh, const float* vertices, size_t prim_count,
+                              const RTRay* rays, size_t ray_count, uint8_t* occluded,
+                              uint32_t flags, void* stream) {
+    if (occluded == nullptr && ray_count > 0) {
+        set_error("occluded is null");
+        return Error;
+    }
+    return trace_entry(bvh.nodes, bvh.node_count, bvh.indices, bvh.index_count, vertices,
+                       prim_count, rays, ray_count, nullptr, occluded, flags, stream, false);
+}
+
+ResultCode rtbvh_gpu_occluded_mbvh(RTMbvh mbvh, const float* vertices, size_t prim_count,
+                                   const RTRay* rays, size_t ray_count, uint8_t* occluded,
+                                   uint32_t flags, void* stream) {
+    if (occluded == nullptr && ray_count > 0) {
+        set_error("occluded is null");
+        return Error;
+    }
+    return trace_entry(mbvh.nodes, mbvh.node_count, mbvh.indices, mbvh.index_count, vertices,
+                       prim_count, rays, ray_count, nullptr, occluded, flags, stream, true);
+}
+
+ResultCode rtbvh_gpu_status(void* stream) {
+    hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    uint32_t* p = device_status_ptr();
+    if (p == nullptr) {
+        set_error("device status word unavailable");
+        return DeviceError;
+    }
+    uint32_t v = 0;
+    e = hipMemcpy(&v, p, sizeof(v), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "status read");
+    if (v != 0) {
+        const uint32_t zero = 0;
+        e = hipMemcpy(p, &zero, sizeof(zero), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "status reset");
+    }
+    if (v & kStatusStackOverflow) {
+        set_error("a ray exceeded RTBVH_MAX_STACK traversal stack entries");
+        return StackOverflow;
+    }
+    return Ok;
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+// Device-side geometry for gfx950: the reference's float arithmetic restated
+// operation by operation so that results are bit-identical to rtbvh on x86-64.
+//
+// Requirements met by the build (rtbvh_amd/Makefile): -ffp-contract=off (no
+// FMA contraction; Rust never contracts), no fast-math, IEEE division
+// (hipcc's default correctly rounded fp32 divide), fp32 denormals preserved
+// (.amdhsa_float_denorm_mode_32 3).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rtbvh {
+
+// glam Vec4::min/max on SSE2 (_mm_min_ps/_mm_max_ps): `a<b?a:b`, `a>b?a:b`;
+// the second operand wins on NaN.  Used by the MBVH slab test
+// (src/mbvh_node.rs:195-203).
+__device__ __forceinline__ float ps_min(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float ps_max(float a, float b) { return a > b ? a : b; }
+
+// f32::min/max (glam Vec3 component-wise; src/aabb.rs:74-86, 252-273) as
+// LLVM lowers llvm.minnum/maxnum on x86-64: NaN-ignoring, ties return `a`.
+__device__ __forceinline__ float rs_min(float a, float b) { return (a != a) ? b : (b < a ? b : a); }
+__device__ __forceinline__ float rs_max(float a, float b) { return (a != a) ? b : (b > a ? b : a); }
+
+struct alignas(16) DevAabb {
+    float min[3];
+    int32_t extra1;  // count
+    float max[3];
+    int32_t extra2;  // left_first
+};
+
+struct alignas(16) DevMbvhNode {
+    float4 min_x, max_x, min_y, max_y, min_z, max_z;
+    int4 children, counts;
+};
+
+struct DevRay {
+    float o[3];
+    float t_min;
+    float d[3];
+    float t_max;
+};
+
+struct DevHit {
+    float t, u, v;
+    uint32_t prim;
+};
+
+// Moller-Trumbore, SpatialTriangle::intersect (src/builders/spatial_sah.rs:
+// 132-163).  glam's scalar Vec3: cross(a,b) = (a.y*b.z - b.y*a.z,
+// a.z*b.x - b.z*a.x, a.x*b.y - b.x*a.y); dot = (a.x*b.x + a.y*b.y) + a.z*b.z.
+// Returns the candidate t (and u, v) when the reference would accept it for
+// some t_cur > t; `ok` is false when it rejects regardless of t_cur.
+struct MtResult {
+    bool ok;
+    float t, u, v;
+};
+
+__device__ __forceinline__ MtResult mt_candidate(float v0x, float v0y, float v0z, float v1x,
+                                                 float v1y, float v1z, float v2x, float v2y,
+                                                 float v2z, float ox, float oy, float oz,
+                                                 float dx, float dy, float dz, float t_min) {
+    MtResult r;
+    r.ok = false;
+    r.t = 0.f;
+    r.u = 0.f;
+    r.v = 0.f;
+    const float e1x = v1x - v0x, e1y = v1y - v0y, e1z = v1z - v0z;
+    const float e2x = v2x - v0x, e2y = v2y - v0y, e2z = v2z - v0z;
+    const float hx = dy * e2z - e2y * dz;
+    const float hy = dz * e2x - e2z * dx;
+    const float hz = dx * e2y - e2x * dy;
+    const float a = (e1x * hx + e1y * hy) + e1z * hz;
+    if (a > -1e-5f && a < 1e-5f) return r;
+    const float f = 1.0f / a;
+    const float sx = ox - v0x, sy = oy - v0y, sz = oz - v0z;
+    const float u = f * ((sx * hx + sy * hy) + sz * hz);
+    if (!(0.0f <= u && u <= 1.0f)) return r;
+    const float qx = sy * e1z - e1y * sz;
+    const float qy = sz * e1x - e1z * sx;
+    const float qz = sx * e1y - e1x * sy;
+    const float v = f * ((dx * qx + dy * qy) + dz * qz);
+    if (v < 0.0f || u + v > 1.0f) return r;
+    const float t = f * ((e2x * qx + e2y * qy) + e2z * qz);
+    r.ok = t > t_min;
+    r.t = t;
+    r.u = u;
+    r.v = v;
+    return r;
+}
+
+}  // namespace rtbvh

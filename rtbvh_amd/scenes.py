@@ -1,0 +1,210 @@
+"""Seeded synthetic scenes and ray sets (host side, numpy).
+
+These play the role of the reference's benchmark harness (examples/
+benchmark.rs, shared/src/lib.rs): they produce FP32 triangle soups (9 floats
+per triangle: v0, v1, v2) and 32-byte ray records (origin, t_min, direction,
+t_max) that feed both the GPU path and the CPU oracle unchanged.
+
+Workloads (sizes as named in the benchmark configuration):
+  torus(80, 40)                   6,400 tris, R=1, r=0.4
+  load_obj(teapot.obj)            6,320 tris (the reference's fixture)
+  noisy_sphere + random_soup      1,000,000 tris in the unit cube
+"""
+from __future__ import annotations
+
+import numpy as np
+
+RAY_T_MIN = np.float32(1e-4)  # Ray::DEFAULT_T_MIN, src/ray.rs:163
+RAY_T_MAX = np.float32(1e34)  # Ray::DEFAULT_T_MAX, src/ray.rs:164
+
+
+# ------------------------------------------------------------------ meshes --
+def _quads_to_tris(p: np.ndarray) -> np.ndarray:
+    """p: (nu+1, nv+1, 3) grid of points (the u axis wraps via the caller).
+    Returns (2*nu*nv, 9) float32, two triangles per quad."""
+    a = p[:-1, :-1]
+    b = p[1:, :-1]
+    c = p[1:, 1:]
+    d = p[:-1, 1:]
+    t0 = np.concatenate([a, b, c], axis=-1)
+    t1 = np.concatenate([a, c, d], axis=-1)
+    tris = np.stack([t0, t1], axis=2)  # (nu, nv, 2, 9)
+    return np.ascontiguousarray(tris.reshape(-1, 9), dtype=np.float32)
+
+
+def torus(nu: int = 80, nv: int = 40, R: float = 1.0, r: float = 0.4) -> np.ndarray:
+    """Torus R, r with nu x nv quads split into 2*nu*nv triangles."""
+    u = np.linspace(0.0, 2.0 * np.pi, nu + 1)
+    v = np.linspace(0.0, 2.0 * np.pi, nv + 1)
+    uu, vv = np.meshgrid(u, v, indexing="ij")
+    x = (R + r * np.cos(vv)) * np.cos(uu)
+    y = (R + r * np.cos(vv)) * np.sin(uu)
+    z = r * np.sin(vv)
+    p = np.stack([x, y, z], axis=-1)
+    p[-1] = p[0]
+    p[:, -1] = p[:, 0]
+    return _quads_to_tris(p)
+
+
+def _hash3(ix, iy, iz, seed):
+    """32-bit lattice hash -> [0, 1)."""
+    h = (ix.astype(np.uint32) * np.uint32(0x8DA6B343)) ^ (iy.astype(np.uint32) * np.uint32(0xD8163841)) \
+        ^ (iz.astype(np.uint32) * np.uint32(0xCB1AB31F)) ^ np.uint32(seed * 0x9E3779B1 & 0xFFFFFFFF)
+    h ^= h >> np.uint32(16)
+    h *= np.uint32(0x7FEB352D)
+    h ^= h >> np.uint32(15)
+    h *= np.uint32(0x846CA68B)
+    h ^= h >> np.uint32(16)
+    return h.astype(np.float64) * (1.0 / 4294967296.0)
+
+
+def value_noise(p: np.ndarray, seed: int, octaves: int = 3, base_freq: float = 4.0,
+                amplitude: float = 0.1) -> np.ndarray:
+    """Lattice value noise with smoothstep trilinear interpolation, octaves
+    doubling in frequency and halving in amplitude; the amplitudes sum to
+    `amplitude`.  Returns values in [-amplitude, amplitude]."""
+    with np.errstate(over="ignore"):
+        total = np.zeros(p.shape[:-1])
+        weights = [0.5 ** o for o in range(octaves)]
+        norm = amplitude / sum(weights)
+        for o in range(octaves):
+            q = p * (base_freq * 2.0 ** o)
+            i0 = np.floor(q).astype(np.int64)
+            f = q - i0
+            s = f * f * (3.0 - 2.0 * f)
+            acc = np.zeros(p.shape[:-1])
+            for dx in (0, 1):
+                wx = s[..., 0] if dx else 1.0 - s[..., 0]
+                for dy in (0, 1):
+                    wy = s[..., 1] if dy else 1.0 - s[..., 1]
+                    for dz in (0, 1):
+                        wz = s[..., 2] if dz else 1.0 - s[..., 2]
+                        h = _hash3(i0[..., 0] + dx, i0[..., 1] + dy, i0[..., 2] + dz, seed + o)
+                        acc += wx * wy * wz * h
+            total += norm * weights[o] * (2.0 * acc - 1.0)
+    return total
+
+
+def noisy_sphere(nu: int, nv: int, center=(0.5, 0.5, 0.5), radius: float = 0.35,
+                 amplitude: float = 0.1, seed: int = 0) -> np.ndarray:
+    """UV sphere of nu (longitude) x nv (latitude) quads, 2*nu*nv triangles,
+    displaced along the normal by value_noise (3 octaves, total amplitude)."""
+    u = np.linspace(0.0, 2.0 * np.pi, nu + 1)
+    v = np.linspace(0.0, np.pi, nv + 1)
+    uu, vv = np.meshgrid(u, v, indexing="ij")
+    n = np.stack([np.sin(vv) * np.cos(uu), np.sin(vv) * np.sin(uu), np.cos(vv)], axis=-1)
+    n[-1] = n[0]
+    c = np.asarray(center, dtype=np.float64)
+    base = c + radius * n
+    disp = value_noise(base, seed, amplitude=amplitude)
+    disp[-1] = disp[0]
+    p = c + (radius + disp)[..., None] * n
+    return _quads_to_tris(p)
+
+
+def random_soup(n: int, edge: float = 0.01, seed: int = 0, lo=(0.0, 0.0, 0.0),
+                hi=(1.0, 1.0, 1.0)) -> np.ndarray:
+    """n triangles: v0 uniform in the box, v1, v2 = v0 + edge * unit vectors."""
+    rng = np.random.default_rng(seed)
+    lo = np.asarray(lo)
+    hi = np.asarray(hi)
+    v0 = lo + rng.random((n, 3)) * (hi - lo)
+    e = rng.normal(size=(n, 2, 3))
+    e /= np.linalg.norm(e, axis=-1, keepdims=True)
+    v1 = v0 + edge * e[:, 0]
+    v2 = v0 + edge * e[:, 1]
+    return np.ascontiguousarray(np.concatenate([v0, v1, v2], axis=1), dtype=np.float32)
+
+
+def scene_1m(seed: int = 2024) -> np.ndarray:
+    """1,000,000 triangles: a 750x600-quad noisy sphere (900,000 tris, centre
+    (0.5,0.5,0.5), radius 0.35, noise amplitude 0.1) plus 100,000 random
+    triangles of edge 0.01 in the unit cube."""
+    s = noisy_sphere(750, 600, seed=seed)
+    r = random_soup(100_000, 0.01, seed=seed + 1)
+    return np.ascontiguousarray(np.concatenate([s, r], axis=0))
+
+
+def load_obj(path: str) -> np.ndarray:
+    """Minimal OBJ reader: `v x y z` and triangular `f a b c` (1-based,
+    optional /vt/vn suffixes), faces in file order."""
+    verts = []
+    faces = []
+    with open(path, "r") as fh:
+        for line in fh:
+            if line.startswith("v "):
+                _, x, y, z = line.split()[:4]
+                verts.append((float(x), float(y), float(z)))
+            elif line.startswith("f "):
+                idx = [int(tok.split("/")[0]) for tok in line.split()[1:]]
+                for k in range(1, len(idx) - 1):  # fan for polygons
+                    faces.append((idx[0], idx[k], idx[k + 1]))
+    v = np.asarray(verts, dtype=np.float32)
+    f = np.asarray(faces, dtype=np.int64) - 1
+    return np.ascontiguousarray(v[f].reshape(-1, 9))
+
+
+def triangle_aabbs_centers(verts: np.ndarray):
+    """Per-triangle aabb (src/aabb.rs:252-261 grow from empty) and centre
+    (v0 + v1 + v2) * (1/3) (shared/src/lib.rs:28-30), float32.  Returns
+    (aabbs as (N, 8) float32 view of RTAabb records, centers (N, 3))."""
+    v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3, 3)
+    third = np.float32(1.0) / np.float32(3.0)
+    centers = ((v[:, 0] + v[:, 1]) + v[:, 2]) * third
+    mn = np.minimum(np.minimum(v[:, 0], v[:, 1]), v[:, 2])
+    mx = np.maximum(np.maximum(v[:, 0], v[:, 1]), v[:, 2])
+    aabbs = np.zeros((len(v), 8), np.float32)
+    aabbs[:, 0:3] = mn
+    aabbs[:, 4:7] = mx
+    return aabbs, np.ascontiguousarray(centers, dtype=np.float32)
+
+
+# -------------------------------------------------------------------- rays --
+def make_rays(origins: np.ndarray, dirs: np.ndarray, t_min=RAY_T_MIN, t_max=RAY_T_MAX):
+    n = len(origins)
+    r = np.empty((n, 8), np.float32)
+    r[:, 0:3] = origins
+    r[:, 3] = t_min
+    r[:, 4:7] = dirs
+    r[:, 7] = t_max
+    return r
+
+
+def rays_sphere_to_box(n: int, lo, hi, radius: float = 3.0, seed: int = 0) -> np.ndarray:
+    """Origins uniform on a sphere of `radius` around the origin, aimed at
+    points uniform in the box [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    o = rng.normal(size=(n, 3))
+    o = radius * o / np.linalg.norm(o, axis=1, keepdims=True)
+    tgt = np.asarray(lo) + rng.random((n, 3)) * (np.asarray(hi) - np.asarray(lo))
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return make_rays(o.astype(np.float32), d.astype(np.float32))
+
+
+def rays_in_box(n: int, lo, hi, seed: int = 0) -> np.ndarray:
+    """Origins uniform in [lo, hi], directions uniform on the unit sphere."""
+    rng = np.random.default_rng(seed)
+    o = np.asarray(lo) + rng.random((n, 3)) * (np.asarray(hi) - np.asarray(lo))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return make_rays(o.astype(np.float32), d.astype(np.float32))
+
+
+def pinhole_rays(width: int, height: int, eye=(0.5, 0.5, -1.5), fov_deg: float = 40.0,
+                 rows=None) -> np.ndarray:
+    """Pinhole camera at `eye` looking along +z; the vertical field of view
+    `fov_deg` (40 deg covers the unit cube from z = -1.5).  `rows` selects a
+    (start, stop) band of image rows."""
+    r0, r1 = (0, height) if rows is None else rows
+    ys = (np.arange(r0, r1, dtype=np.float64) + 0.5) / height
+    xs = (np.arange(width, dtype=np.float64) + 0.5) / width
+    tan_half = np.tan(np.radians(fov_deg) * 0.5)
+    aspect = width / height
+    px = (2.0 * xs - 1.0) * tan_half * aspect
+    py = (1.0 - 2.0 * ys) * tan_half
+    gx, gy = np.meshgrid(px, py)
+    d = np.stack([gx, gy, np.ones_like(gx)], axis=-1).reshape(-1, 3)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o = np.broadcast_to(np.asarray(eye, dtype=np.float64), d.shape)
+    return make_rays(o.astype(np.float32), d.astype(np.float32))
