@@ -110,7 +110,7 @@ def world_bounds(aabbs):
 def morton_codes(aabbs, centers):
     """MortonEncoder::new + encode over Aabb::union_of_list(aabbs), morton.rs:36-61."""
     aabbs = _c(aabbs, AABB)
-    centers = _c(centers, np.float32).reshape(len(aabbs), -1)
+    centers = _c(centers, np.float32).reshape(len(aabbs), -1) if len(aabbs) else np.zeros((0, 3), np.float32)
     w = np.zeros(1, AABB)
     lib().ref_world_bounds(_p(aabbs), len(aabbs), _p(w))
     codes = np.zeros(len(aabbs), np.uint32)
@@ -154,7 +154,7 @@ def triangle_aabbs_centers(verts):
 def build_locb(aabbs, centers, radius=14, return_iterations=False):
     """LocallyOrderedClusteringBuilder::build, src/builders/locb.rs:249-329."""
     aabbs = _c(aabbs, AABB)
-    centers = _c(centers, np.float32).reshape(len(aabbs), -1)
+    centers = _c(centers, np.float32).reshape(len(aabbs), -1) if len(aabbs) else np.zeros((0, 3), np.float32)
     n = len(aabbs)
     nodes = np.zeros(max(1, 2 * n - 1), AABB)
     idx = np.zeros(max(1, n), np.uint32)
@@ -169,7 +169,7 @@ def build_binned_sah(aabbs, centers, prims_per_leaf=1, contained=False):
     """BinnedSahBuilder::build, src/builders/binned_sah.rs:346-400.
     contained=True fixes the reference's Q2/Q3 box defects."""
     aabbs = _c(aabbs, AABB)
-    centers = _c(centers, np.float32).reshape(len(aabbs), -1)
+    centers = _c(centers, np.float32).reshape(len(aabbs), -1) if len(aabbs) else np.zeros((0, 3), np.float32)
     n = len(aabbs)
     nodes = np.zeros(max(1, 2 * n - 1), AABB)
     idx = np.zeros(max(1, n), np.uint32)

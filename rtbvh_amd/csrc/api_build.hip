@@ -1,0 +1,408 @@
+// C ABI: tree construction, collapse, refit and release — the rtbvh_ffi
+// entry points (create_bvh / create_mbvh / refit / free_*) and their
+// device-resident rtbvh_gpu_* counterparts.
+//
+// Trees live in a process-wide registry mirroring rtbvh_ffi's
+// StructureManager (rtbvh_ffi/src/lib.rs:8-127): ids are handed out in order
+// and never reused, a freed slot stays empty, creation and release take the
+// exclusive lock and lookups the shared lock.  Every tree's arrays live on
+// the device it was built on; handles returned by the rtbvh_ffi entry points
+// additionally carry a host mirror that their pointers refer to.
+#include <stdio.h>
+
+#include <memory>
+#include <mutex>
+#include <shared_mutex>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace rtbvh {
+
+namespace {
+
+struct BvhEntry {
+    DeviceBvh dev;
+    int device = 0;
+    std::vector<RTBvhNode> host_nodes;
+    std::vector<uint32_t> host_indices;
+};
+
+struct MbvhEntry {
+    DeviceMbvh dev;
+    int device = 0;
+    std::vector<RTMbvhNode> host_nodes;
+    std::vector<uint32_t> host_indices;
+};
+
+struct Registry {
+    std::shared_mutex mu;
+    std::vector<std::unique_ptr<BvhEntry>> bvhs;    // null = freed
+    std::vector<std::unique_ptr<MbvhEntry>> mbvhs;
+};
+
+Registry& reg() {
+    static Registry r;
+    return r;
+}
+
+void release(DeviceBvh& d) {
+    if (d.nodes) (void)hipFree(d.nodes);
+    if (d.indices) (void)hipFree(d.indices);
+    if (d.parents) (void)hipFree(d.parents);
+    d = DeviceBvh{};
+}
+
+void release(DeviceMbvh& d) {
+    if (d.nodes) (void)hipFree(d.nodes);
+    if (d.indices) (void)hipFree(d.indices);
+    d = DeviceMbvh{};
+}
+
+// Switches to `device` for the lifetime of the guard.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int device) {
+        if (hipGetDevice(&prev) == hipSuccess && prev != device) (void)hipSetDevice(device);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+ResultCode current_device(int* dev) {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        set_error("no HIP device available");
+        return NoDevice;
+    }
+    e = hipGetDevice(dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    return Ok;
+}
+
+RTBvh handle_of(uint32_t id, const BvhEntry& e, bool host) {
+    RTBvh h;
+    h.id = id;
+    h.node_count = e.dev.node_count;
+    h.index_count = e.dev.prim_count;
+    if (host) {
+        h.nodes = e.host_nodes.data();
+        h.indices = e.host_indices.data();
+    } else {
+        h.nodes = reinterpret_cast<const RTBvhNode*>(e.dev.nodes);
+        h.indices = e.dev.indices;
+    }
+    return h;
+}
+
+RTMbvh handle_of(uint32_t id, const MbvhEntry& e, bool host) {
+    RTMbvh h;
+    h.id = id;
+    h.node_count = e.dev.node_count;
+    h.index_count = e.dev.prim_count;
+    if (host) {
+        h.nodes = e.host_nodes.data();
+        h.indices = e.host_indices.data();
+    } else {
+        h.nodes = reinterpret_cast<const RTMbvhNode*>(e.dev.nodes);
+        h.indices = e.dev.indices;
+    }
+    return h;
+}
+
+// Argument checks shared by create_bvh and rtbvh_gpu_create_bvh, done before
+// any device call.
+ResultCode validate_build(size_t prim_count, size_t center_stride, BvhType bvh_type) {
+    if (center_stride != 12 && center_stride != 16) {
+        // rtbvh_ffi asserts 12 <= stride <= 16, stride % 4 == 0 (lib.rs:441-449)
+        set_error("center_stride must be 12 or 16 bytes, got %zu", center_stride);
+        return InvalidArgument;
+    }
+    if (bvh_type != LocallyOrderedClustered && bvh_type != BinnedSAH) {
+        set_error("unknown BvhType %u", unsigned(bvh_type));
+        return InvalidArgument;
+    }
+    if (prim_count == 0) {
+        set_error("no primitives");
+        return NoPrimitives;  // Builder::construct_* (src/bvh.rs:88-90,114-116)
+    }
+    return Ok;
+}
+
+ResultCode build_device(const RTAabb* aabbs, size_t prim_count, const float* centers,
+                        size_t center_stride, size_t prims_per_leaf, BvhType bvh_type,
+                        hipStream_t stream, DeviceBvh* out) {
+    ResultCode rc = validate_build(prim_count, center_stride, bvh_type);
+    if (rc != Ok) return rc;
+    if (bvh_type == LocallyOrderedClustered)
+        return gpu_build_ploc(aabbs, centers, center_stride / 4, prim_count, 14, stream, out);
+    if (bvh_type == BinnedSAH)
+        return gpu_build_binned(aabbs, centers, center_stride / 4, prim_count, prims_per_leaf,
+                                stream, out);
+    set_error("unknown BvhType %u", unsigned(bvh_type));
+    return InvalidArgument;
+}
+
+uint32_t register_bvh(std::unique_ptr<BvhEntry> e) {
+    Registry& r = reg();
+    std::unique_lock<std::shared_mutex> lk(r.mu);
+    r.bvhs.push_back(std::move(e));
+    return uint32_t(r.bvhs.size() - 1);
+}
+
+uint32_t register_mbvh(std::unique_ptr<MbvhEntry> e) {
+    Registry& r = reg();
+    std::unique_lock<std::shared_mutex> lk(r.mu);
+    r.mbvhs.push_back(std::move(e));
+    return uint32_t(r.mbvhs.size() - 1);
+}
+
+ResultCode download_bvh(BvhEntry& e) {
+    e.host_nodes.resize(e.dev.node_count);
+    e.host_indices.resize(e.dev.prim_count);
+    hipError_t err = hipSuccess;
+    if (e.dev.node_count)
+        err = hipMemcpy(e.host_nodes.data(), e.dev.nodes, size_t(e.dev.node_count) * 32,
+                        hipMemcpyDeviceToHost);
+    if (err == hipSuccess && e.dev.prim_count)
+        err = hipMemcpy(e.host_indices.data(), e.dev.indices, size_t(e.dev.prim_count) * 4,
+                        hipMemcpyDeviceToHost);
+    if (err != hipSuccess) return hip_fail(err, "download bvh");
+    return Ok;
+}
+
+ResultCode download_mbvh(MbvhEntry& e) {
+    e.host_nodes.resize(e.dev.node_count);
+    e.host_indices.resize(e.dev.prim_count);
+    hipError_t err = hipSuccess;
+    if (e.dev.node_count)
+        err = hipMemcpy(e.host_nodes.data(), e.dev.nodes, size_t(e.dev.node_count) * 128,
+                        hipMemcpyDeviceToHost);
+    if (err == hipSuccess && e.dev.prim_count)
+        err = hipMemcpy(e.host_indices.data(), e.dev.indices, size_t(e.dev.prim_count) * 4,
+                        hipMemcpyDeviceToHost);
+    if (err != hipSuccess) return hip_fail(err, "download mbvh");
+    return Ok;
+}
+
+// Finds the entry for `id`; the shared lock is held by the caller.
+BvhEntry* find_bvh(uint32_t id) {
+    Registry& r = reg();
+    if (id >= r.bvhs.size()) return nullptr;
+    return r.bvhs[id].get();
+}
+
+MbvhEntry* find_mbvh(uint32_t id) {
+    Registry& r = reg();
+    if (id >= r.mbvhs.size()) return nullptr;
+    return r.mbvhs[id].get();
+}
+
+ResultCode collapse_entry(uint32_t id, void* stream, bool host, RTMbvh* out) {
+    std::unique_ptr<MbvhEntry> me(new MbvhEntry());
+    {
+        Registry& r = reg();
+        // exclusive: the collapse may attach parent pointers to the tree
+        std::unique_lock<std::shared_mutex> lk(r.mu);
+        BvhEntry* be = find_bvh(id);
+        if (be == nullptr) {
+            set_error("unknown bvh id %u", id);
+            return Error;
+        }
+        DeviceGuard g(be->device);
+        me->device = be->device;
+        ResultCode rc = gpu_build_mbvh(be->dev, static_cast<hipStream_t>(stream), &me->dev);
+        if (rc != Ok) return rc;
+    }
+    if (host) {
+        DeviceGuard g(me->device);
+        ResultCode rc = download_mbvh(*me);
+        if (rc != Ok) {
+            release(me->dev);
+            return rc;
+        }
+    }
+    MbvhEntry* raw = me.get();
+    const uint32_t mid = register_mbvh(std::move(me));
+    *out = handle_of(mid, *raw, host);
+    return Ok;
+}
+
+}  // namespace
+
+}  // namespace rtbvh
+
+using namespace rtbvh;
+
+extern "C" {
+
+ResultCode rtbvh_gpu_create_bvh(const RTAabb* aabbs, size_t prim_count, const float* centers,
+                                size_t center_stride, size_t prims_per_leaf, BvhType bvh_type,
+                                void* stream, RTBvh* result) {
+    if (centers == nullptr || result == nullptr) {
+        set_error("centers and result must not be null");
+        return Error;  // rtbvh_ffi/src/lib.rs:437-439
+    }
+    ResultCode rc = validate_build(prim_count, center_stride, bvh_type);
+    if (rc != Ok) return rc;
+    std::unique_ptr<BvhEntry> e(new BvhEntry());
+    rc = current_device(&e->device);
+    if (rc != Ok) return rc;
+    rc = build_device(aabbs, prim_count, centers, center_stride, prims_per_leaf, bvh_type,
+                      static_cast<hipStream_t>(stream), &e->dev);
+    if (rc != Ok) return rc;
+    BvhEntry* raw = e.get();
+    const uint32_t id = register_bvh(std::move(e));
+    *result = handle_of(id, *raw, false);
+    return Ok;
+}
+
+ResultCode rtbvh_gpu_create_mbvh(RTBvh bvh, void* stream, RTMbvh* result) {
+    if (result == nullptr) {
+        set_error("result must not be null");
+        return Error;
+    }
+    return collapse_entry(bvh.id, stream, false, result);
+}
+
+ResultCode rtbvh_gpu_refit(const RTAabb* aabbs, RTBvh bvh, void* stream) {
+    if (aabbs == nullptr) {
+        set_error("aabbs must not be null");
+        return Error;
+    }
+    Registry& r = reg();
+    std::unique_lock<std::shared_mutex> lk(r.mu);
+    BvhEntry* be = find_bvh(bvh.id);
+    if (be == nullptr) {
+        set_error("unknown bvh id %u", bvh.id);
+        return Error;
+    }
+    DeviceGuard g(be->device);
+    return gpu_refit(be->dev, aabbs, static_cast<hipStream_t>(stream));
+}
+
+void rtbvh_gpu_free_bvh(RTBvh bvh) { free_bvh(bvh); }
+void rtbvh_gpu_free_mbvh(RTMbvh mbvh) { free_mbvh(mbvh); }
+
+// ---------------------------------------------------- rtbvh_ffi entry points
+
+ResultCode create_bvh(const RTAabb* aabbs, size_t prim_count, const float* centers,
+                      size_t center_stride, size_t prims_per_leaf, BvhType bvh_type,
+                      RTBvh* result) {
+    if (centers == nullptr || result == nullptr) {
+        set_error("centers and result must not be null");
+        return Error;  // rtbvh_ffi/src/lib.rs:437-439
+    }
+    ResultCode rc = validate_build(prim_count, center_stride, bvh_type);
+    if (rc != Ok) return rc;
+    std::unique_ptr<BvhEntry> e(new BvhEntry());
+    rc = current_device(&e->device);
+    if (rc != Ok) return rc;
+    // stage the caller's host arrays on the device
+    void* d_centers = nullptr;
+    void* d_aabbs = nullptr;
+    hipError_t err = hipMalloc(&d_centers, prim_count * center_stride);
+    if (err == hipSuccess)
+        err = hipMemcpy(d_centers, centers, prim_count * center_stride, hipMemcpyHostToDevice);
+    if (err == hipSuccess && aabbs) {
+        err = hipMalloc(&d_aabbs, prim_count * sizeof(RTAabb));
+        if (err == hipSuccess)
+            err = hipMemcpy(d_aabbs, aabbs, prim_count * sizeof(RTAabb), hipMemcpyHostToDevice);
+    }
+    if (err != hipSuccess) {
+        if (d_centers) (void)hipFree(d_centers);
+        if (d_aabbs) (void)hipFree(d_aabbs);
+        return hip_fail(err, "stage inputs");
+    }
+    rc = build_device(static_cast<const RTAabb*>(d_aabbs), prim_count,
+                      static_cast<const float*>(d_centers), center_stride, prims_per_leaf,
+                      bvh_type, nullptr, &e->dev);
+    (void)hipFree(d_centers);
+    if (d_aabbs) (void)hipFree(d_aabbs);
+    if (rc != Ok) return rc;
+    rc = download_bvh(*e);
+    if (rc != Ok) {
+        release(e->dev);
+        return rc;
+    }
+    BvhEntry* raw = e.get();
+    const uint32_t id = register_bvh(std::move(e));
+    *result = handle_of(id, *raw, true);
+    return Ok;
+}
+
+ResultCode create_mbvh(RTBvh bvh, RTMbvh* mbvh) {
+    if (mbvh == nullptr || bvh.nodes == nullptr || bvh.indices == nullptr) {
+        set_error("null mbvh, nodes or indices");
+        return Error;  // rtbvh_ffi/src/lib.rs:500-502
+    }
+    return collapse_entry(bvh.id, nullptr, true, mbvh);
+}
+
+ResultCode refit(const RTAabb* aabbs, RTBvh bvh) {
+    if (aabbs == nullptr || bvh.nodes == nullptr || bvh.indices == nullptr) {
+        set_error("null aabbs, nodes or indices");
+        return Error;  // rtbvh_ffi/src/lib.rs:520-522
+    }
+    Registry& r = reg();
+    std::unique_lock<std::shared_mutex> lk(r.mu);
+    BvhEntry* be = find_bvh(bvh.id);
+    if (be == nullptr) {
+        set_error("unknown bvh id %u", bvh.id);
+        return Error;
+    }
+    DeviceGuard g(be->device);
+    const size_t n = be->dev.prim_count;  // Bvh::prim_count (src/bvh.rs:172-174)
+    void* d_aabbs = nullptr;
+    hipError_t err = hipMalloc(&d_aabbs, n * sizeof(RTAabb) + 16);
+    if (err == hipSuccess && n)
+        err = hipMemcpy(d_aabbs, aabbs, n * sizeof(RTAabb), hipMemcpyHostToDevice);
+    if (err != hipSuccess) {
+        if (d_aabbs) (void)hipFree(d_aabbs);
+        return hip_fail(err, "stage aabbs");
+    }
+    ResultCode rc = gpu_refit(be->dev, static_cast<const RTAabb*>(d_aabbs), nullptr);
+    (void)hipFree(d_aabbs);
+    if (rc != Ok) return rc;
+    if (!be->host_nodes.empty()) {
+        // in place: handles keep pointing at the same host array
+        err = hipMemcpy(be->host_nodes.data(), be->dev.nodes, be->host_nodes.size() * 32,
+                        hipMemcpyDeviceToHost);
+        if (err != hipSuccess) return hip_fail(err, "download refit");
+    }
+    return Ok;
+}
+
+void free_bvh(RTBvh bvh) {
+    Registry& r = reg();
+    std::unique_lock<std::shared_mutex> lk(r.mu);
+    if (bvh.id >= r.bvhs.size()) {
+        fprintf(stderr, "Could not free bvh with id: %u\n", bvh.id);  // lib.rs:838-842
+        return;
+    }
+    if (r.bvhs[bvh.id]) {
+        DeviceGuard g(r.bvhs[bvh.id]->device);
+        release(r.bvhs[bvh.id]->dev);
+        r.bvhs[bvh.id].reset();
+    }
+}
+
+void free_mbvh(RTMbvh bvh) {
+    Registry& r = reg();
+    std::unique_lock<std::shared_mutex> lk(r.mu);
+    if (bvh.id >= r.mbvhs.size()) {
+        fprintf(stderr, "Could not free bvh with id: %u\n", bvh.id);  // lib.rs:845-849
+        return;
+    }
+    if (r.mbvhs[bvh.id]) {
+        DeviceGuard g(r.mbvhs[bvh.id]->device);
+        release(r.mbvhs[bvh.id]->dev);
+        r.mbvhs[bvh.id].reset();
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,580 @@
+// GPU binned-SAH builder for gfx950, breadth-first.
+//
+// Restates BinnedSahBuildTask::run (src/builders/binned_sah.rs:132-282) per
+// node, one tree level per step, all nodes of a level in parallel:
+//   pad the node box by 1e-4                                  :133
+//   leaf when it holds <= 1 primitive or depth >= 64          :140-143
+//   16 bins per axis over the padded node box; a primitive's
+//   bin is min(15, sat(max(c*ctb + bo, 0)))                   :119-128,145-172
+//   SAH sweep (right costs, then left + right, strict '<')    :80-114
+//   axis ties keep the lower axis                             :179-185
+//   no split / cost >= A*(n-1): approximate median on the
+//   longest axis when n > max_leaf, else leaf                 :187-210
+//   a side left empty makes a leaf                            :221-222
+//   child boxes are unions of the bins on either side         :228-247
+// Two reference defects are fixed so that every child box lies inside its
+// parent: the left box uses the split actually taken (the reference uses the
+// pre-fallback bin count, :232-235) and leaves are not padded a second time
+// (:135).  The partition is stable on both sides (the reference's Lomuto
+// partition, src/utils.rs:76-96, permutes the right side), so primitive order
+// inside multi-primitive leaves may differ; the sets, the topology and the
+// bounds do not.  Nodes are numbered breadth-first (the reference's numbering
+// depends on thread timing, builders/mod.rs:59-76).
+#include "internal.hpp"
+
+namespace rtbvh {
+
+hipError_t scan_exclusive_u32(uint32_t* data, uint32_t count, uint32_t* total, void* scratch,
+                              size_t scratch_bytes, hipStream_t stream);
+size_t scan_scratch_bytes(uint32_t count);
+
+namespace {
+
+constexpr int kBins = 16;          // binned_sah.rs:314
+constexpr uint32_t kMaxDepth = 64;  // binned_sah.rs:313
+constexpr int kBlock = 256;
+
+struct BTask {
+    float mn[3];
+    uint32_t node;
+    float mx[3];
+    uint32_t begin;
+    uint32_t end;
+    uint32_t depth;
+    uint32_t pad[2];
+};
+
+struct BinAcc {
+    uint32_t cnt;
+    uint32_t mn[3];  // order-preserving encodings of floats
+    uint32_t mx[3];
+    uint32_t pad;
+};
+
+struct BDecision {
+    float mn[3];      // padded node box
+    uint32_t split;   // 1 = split, 0 = leaf
+    float mx[3];
+    uint32_t axis;
+    float ctb[3];
+    uint32_t split_index;
+    float bo[3];
+    uint32_t left_count;
+    float lmn[3], lmx[3], rmn[3], rmx[3];
+};
+
+__device__ __forceinline__ uint32_t fenc(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float fdec(uint32_t e) {
+    return __uint_as_float((e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e);
+}
+
+__device__ __forceinline__ uint32_t bin_index(float c, float ctb, float bo) {
+    // (bin_count - 1).min((c * ctb + bo).max(0.0) as usize)
+    const float b = c * ctb + bo;
+    const float m = rs_max(b, 0.0f);
+    uint32_t u;
+    if (!(m > 0.0f))
+        u = 0;
+    else if (m >= 4294967296.0f)
+        u = 0xFFFFFFFFu;
+    else
+        u = uint32_t(m);
+    return u < uint32_t(kBins - 1) ? u : uint32_t(kBins - 1);
+}
+
+__device__ __forceinline__ float half_area6(const float* mn, const float* mx) {
+    const float d0 = mx[0] - mn[0], d1 = mx[1] - mn[1], d2 = mx[2] - mn[2];
+    return (d0 + d1) * d2 + d0 * d1;
+}
+
+// Per task: padded box, bin parameters, cleared bins.
+__global__ void level_init_kernel(const BTask* tasks, uint32_t num_tasks, BDecision* dec,
+                                  BinAcc* bins) {
+    const uint32_t t = blockIdx.x;
+    if (t >= num_tasks) return;
+    const BTask tk = tasks[t];
+    if (threadIdx.x == 0) {
+        BDecision d;
+        for (int k = 0; k < 3; ++k) {
+            d.mn[k] = tk.mn[k] - 0.0001f;  // binned_sah.rs:133
+            d.mx[k] = tk.mx[k] + 0.0001f;
+            d.ctb[k] = (1.0f / (d.mx[k] - d.mn[k])) * float(kBins);  // :146
+            d.bo[k] = (-d.mn[k]) * d.ctb[k];                        // :147
+        }
+        d.split = 0;
+        d.axis = 0;
+        d.split_index = 0;
+        d.left_count = 0;
+        dec[t] = d;
+    }
+    if (threadIdx.x < 3 * kBins) {
+        BinAcc a;
+        a.cnt = 0;
+        for (int k = 0; k < 3; ++k) {
+            a.mn[k] = fenc(1e34f);  // Aabb::empty
+            a.mx[k] = fenc(-1e34f);
+        }
+        a.pad = 0;
+        bins[size_t(t) * 3 * kBins + threadIdx.x] = a;
+    }
+}
+
+// Bin every primitive of every task with >= 2 primitives.  Blocks whose
+// primitives all belong to one task accumulate in LDS first.
+__global__ void __launch_bounds__(kBlock)
+    bin_kernel(uint32_t n, const int32_t* ptask, const uint32_t* idx, const BTask* tasks,
+               const BDecision* dec, const float* centers, uint32_t stride,
+               const DevAabb* aabbs, BinAcc* bins) {
+    __shared__ uint32_t s_cnt[3 * kBins];
+    __shared__ uint32_t s_mn[3 * kBins][3];
+    __shared__ uint32_t s_mx[3 * kBins][3];
+    __shared__ int32_t s_uniform;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t first = blockIdx.x * kBlock;
+    const uint32_t last = (first + kBlock < n ? first + kBlock : n) - 1;
+    if (threadIdx.x == 0) {
+        const int32_t a = ptask[first];
+        s_uniform = (a >= 0 && ptask[last] == a) ? a : -1;
+    }
+    if (threadIdx.x < 3 * kBins) {
+        s_cnt[threadIdx.x] = 0;
+        for (int k = 0; k < 3; ++k) {
+            s_mn[threadIdx.x][k] = fenc(1e34f);
+            s_mx[threadIdx.x][k] = fenc(-1e34f);
+        }
+    }
+    __syncthreads();
+    const int32_t uni = s_uniform;
+    if (i < n) {
+        const int32_t t = ptask[i];
+        if (t >= 0) {
+            const BTask tk = tasks[t];
+            if (tk.end - tk.begin > 1 && tk.depth < kMaxDepth) {
+                const BDecision& d = dec[t];
+                const uint32_t p = idx[i];
+                const float* c = centers + size_t(p) * stride;
+                const DevAabb a = aabbs[p];
+                for (int ax = 0; ax < 3; ++ax) {
+                    const uint32_t b = bin_index(c[ax], d.ctb[ax], d.bo[ax]);
+                    const uint32_t slot = ax * kBins + b;
+                    if (uni >= 0) {
+                        atomicAdd(&s_cnt[slot], 1u);
+                        for (int k = 0; k < 3; ++k) {
+                            if (a.min[k] == a.min[k]) atomicMin(&s_mn[slot][k], fenc(a.min[k]));
+                            if (a.max[k] == a.max[k]) atomicMax(&s_mx[slot][k], fenc(a.max[k]));
+                        }
+                    } else {
+                        BinAcc* g = bins + size_t(t) * 3 * kBins + slot;
+                        atomicAdd(&g->cnt, 1u);
+                        for (int k = 0; k < 3; ++k) {
+                            if (a.min[k] == a.min[k]) atomicMin(&g->mn[k], fenc(a.min[k]));
+                            if (a.max[k] == a.max[k]) atomicMax(&g->mx[k], fenc(a.max[k]));
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (uni >= 0 && threadIdx.x < 3 * kBins && s_cnt[threadIdx.x] > 0) {
+        BinAcc* g = bins + size_t(uni) * 3 * kBins + threadIdx.x;
+        atomicAdd(&g->cnt, s_cnt[threadIdx.x]);
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&g->mn[k], s_mn[threadIdx.x][k]);
+            atomicMax(&g->mx[k], s_mx[threadIdx.x][k]);
+        }
+    }
+}
+
+struct Box {
+    float mn[3], mx[3];
+};
+
+__device__ __forceinline__ void box_empty(Box& b) {
+    for (int k = 0; k < 3; ++k) {
+        b.mn[k] = 1e34f;
+        b.mx[k] = -1e34f;
+    }
+}
+
+__device__ __forceinline__ void box_grow(Box& b, const BinAcc& a) {
+    // grow_bb with the bin's box (src/aabb.rs:264-273)
+    for (int k = 0; k < 3; ++k) {
+        b.mn[k] = rs_min(b.mn[k], fdec(a.mn[k]));
+        b.mx[k] = rs_max(b.mx[k], fdec(a.mx[k]));
+    }
+}
+
+// find_split, binned_sah.rs:80-114
+__device__ void find_split(const BinAcc* bins, float& best_cost, uint32_t& best_count) {
+    float right_cost[kBins];
+    Box cur;
+    box_empty(cur);
+    uint64_t cc = 0;
+    for (int i = kBins - 1; i > 0; --i) {
+        box_grow(cur, bins[i]);
+        cc += bins[i].cnt;
+        right_cost[i] = half_area6(cur.mn, cur.mx) * float(cc);
+    }
+    box_empty(cur);
+    cc = 0;
+    best_cost = 3.40282347e38f;
+    best_count = kBins;
+    for (int i = 0; i < kBins - 1; ++i) {
+        box_grow(cur, bins[i]);
+        cc += bins[i].cnt;
+        const float cost = half_area6(cur.mn, cur.mx) * float(cc) + right_cost[i + 1];
+        if (cost < best_cost) {
+            best_cost = cost;
+            best_count = uint32_t(i + 1);
+        }
+    }
+}
+
+__global__ void split_kernel(const BTask* tasks, uint32_t num_tasks, BDecision* dec,
+                             const BinAcc* bins, uint32_t max_leaf, uint32_t* split_flag) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tasks) return;
+    const BTask tk = tasks[t];
+    BDecision d = dec[t];
+    const uint32_t work = tk.end - tk.begin;
+    d.split = 0;
+    if (work > 1 && tk.depth < kMaxDepth) {
+        const BinAcc* tb = bins + size_t(t) * 3 * kBins;
+        float cost[3];
+        uint32_t cnt[3];
+        for (int ax = 0; ax < 3; ++ax) find_split(tb + ax * kBins, cost[ax], cnt[ax]);
+        uint32_t axis = 0;
+        if (cost[0] > cost[1]) axis = 1;
+        if (cost[axis] > cost[2]) axis = 2;
+        uint32_t split_index = cnt[axis];
+        const float max_split_cost = half_area6(d.mn, d.mx) * (float(work) - 1.0f);
+        bool make_leaf = false;
+        if (cnt[axis] == uint32_t(kBins) || cost[axis] >= max_split_cost) {
+            if (work > max_leaf) {
+                // Aabb::longest_axis of the padded box, aabb.rs:354-363
+                axis = 0;
+                if (d.mx[1] - d.mn[1] > d.mx[0] - d.mn[0]) axis = 1;
+                if (d.mx[2] - d.mn[2] > d.mx[axis] - d.mn[axis]) axis = 2;
+                uint64_t c = 0;
+                for (int i = 0; i < kBins - 1; ++i) {
+                    c += tb[axis * kBins + i].cnt;
+                    if (c >= uint64_t(work * 2 / 5 + 1)) {
+                        split_index = uint32_t(i + 1);
+                        break;
+                    }
+                }
+            } else {
+                make_leaf = true;
+            }
+        }
+        if (!make_leaf) {
+            uint32_t left = 0;
+            const uint32_t lim = split_index < uint32_t(kBins) ? split_index : uint32_t(kBins);
+            for (uint32_t i = 0; i < lim; ++i) left += tb[axis * kBins + i].cnt;
+            if (left > 0 && left < work) {
+                Box lb, rb;
+                box_empty(lb);
+                box_empty(rb);
+                for (uint32_t i = 0; i < lim; ++i) box_grow(lb, tb[axis * kBins + i]);
+                for (uint32_t i = lim; i < uint32_t(kBins); ++i) box_grow(rb, tb[axis * kBins + i]);
+                d.split = 1;
+                d.axis = axis;
+                d.split_index = split_index;
+                d.left_count = left;
+                for (int k = 0; k < 3; ++k) {
+                    d.lmn[k] = lb.mn[k];
+                    d.lmx[k] = lb.mx[k];
+                    d.rmn[k] = rb.mn[k];
+                    d.rmx[k] = rb.mx[k];
+                }
+            }
+        }
+    }
+    dec[t] = d;
+    split_flag[t] = d.split;
+}
+
+// Writes each task's node (leaf or interior) and the next level's tasks.
+__global__ void children_kernel(const BTask* tasks, uint32_t num_tasks, const BDecision* dec,
+                                const uint32_t* split_excl, uint32_t counter, DevAabb* nodes,
+                                BTask* next) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tasks) return;
+    const BTask tk = tasks[t];
+    const BDecision& d = dec[t];
+    DevAabb nd;
+    for (int k = 0; k < 3; ++k) {
+        nd.min[k] = d.mn[k];
+        nd.max[k] = d.mx[k];
+    }
+    if (d.split) {
+        const uint32_t child = counter + 2 * split_excl[t];
+        nd.extra1 = -1;
+        nd.extra2 = int32_t(child);
+        BTask l, r;
+        for (int k = 0; k < 3; ++k) {
+            l.mn[k] = d.lmn[k];
+            l.mx[k] = d.lmx[k];
+            r.mn[k] = d.rmn[k];
+            r.mx[k] = d.rmx[k];
+        }
+        l.node = child;
+        r.node = child + 1;
+        l.begin = tk.begin;
+        l.end = tk.begin + d.left_count;
+        r.begin = l.end;
+        r.end = tk.end;
+        l.depth = r.depth = tk.depth + 1;
+        l.pad[0] = l.pad[1] = r.pad[0] = r.pad[1] = 0;
+        next[2 * split_excl[t]] = l;
+        next[2 * split_excl[t] + 1] = r;
+    } else {
+        nd.extra1 = int32_t(tk.end - tk.begin);
+        nd.extra2 = int32_t(tk.begin);
+    }
+    nodes[tk.node] = nd;
+}
+
+__global__ void flag_kernel(uint32_t n, const int32_t* ptask, const uint32_t* idx,
+                            const BDecision* dec, const float* centers, uint32_t stride,
+                            uint32_t* flag) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t t = ptask[i];
+    uint32_t f = 0;
+    if (t >= 0) {
+        const BDecision& d = dec[t];
+        if (d.split) {
+            const float c = centers[size_t(idx[i]) * stride + d.axis];
+            f = bin_index(c, d.ctb[d.axis], d.bo[d.axis]) < d.split_index ? 1u : 0u;
+        }
+    }
+    flag[i] = f;
+}
+
+__global__ void scatter_kernel(uint32_t n, const int32_t* ptask, const uint32_t* idx,
+                               const BTask* tasks, const BDecision* dec,
+                               const uint32_t* split_excl, const uint32_t* flag,
+                               const uint32_t* lscan, uint32_t* idx_out, int32_t* ptask_out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t t = ptask[i];
+    if (t < 0 || !dec[t].split) {
+        idx_out[i] = idx[i];
+        ptask_out[i] = -1;
+        return;
+    }
+    const BTask tk = tasks[t];
+    const uint32_t lefts_before = lscan[i] - lscan[tk.begin];
+    uint32_t pos;
+    int32_t nt;
+    if (flag[i]) {
+        pos = tk.begin + lefts_before;
+        nt = int32_t(2 * split_excl[t]);
+    } else {
+        pos = tk.begin + dec[t].left_count + (i - tk.begin - lefts_before);
+        nt = int32_t(2 * split_excl[t] + 1);
+    }
+    idx_out[pos] = idx[i];
+    ptask_out[pos] = nt;
+}
+
+__global__ void iota_kernel(uint32_t* idx, int32_t* ptask, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        idx[i] = i;
+        ptask[i] = 0;
+    }
+}
+
+__global__ void point_aabbs_kernel(const float* centers, uint32_t stride, uint32_t n,
+                                   DevAabb* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    DevAabb a;
+    for (int k = 0; k < 3; ++k) {
+        const float c = centers[size_t(i) * stride + k];
+        a.min[k] = rs_min(1e34f, c);
+        a.max[k] = rs_max(-1e34f, c);
+    }
+    a.extra1 = 0;
+    a.extra2 = 0;
+    out[i] = a;
+}
+
+// Root task: Aabb::union_of_list (binned_sah.rs:361), sequential fold order
+// per block then across blocks.
+__global__ void root_bounds_kernel(const DevAabb* aabbs, uint32_t n, BTask* task) {
+    __shared__ float sh[kBlock][6];
+    float acc[6] = {1e34f, 1e34f, 1e34f, -1e34f, -1e34f, -1e34f};
+    const uint32_t per = (n + kBlock - 1) / kBlock;
+    const uint32_t b = threadIdx.x * per;
+    const uint32_t e = b + per < n ? b + per : n;
+    for (uint32_t i = b; i < e; ++i) {
+        const DevAabb a = aabbs[i];
+        for (int k = 0; k < 3; ++k) {
+            acc[k] = rs_min(acc[k], a.min[k]);
+            acc[3 + k] = rs_max(acc[3 + k], a.max[k]);
+        }
+    }
+    for (int k = 0; k < 6; ++k) sh[threadIdx.x][k] = acc[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r[6] = {1e34f, 1e34f, 1e34f, -1e34f, -1e34f, -1e34f};
+        for (int t = 0; t < kBlock; ++t)
+            for (int k = 0; k < 3; ++k) {
+                r[k] = rs_min(r[k], sh[t][k]);
+                r[3 + k] = rs_max(r[3 + k], sh[t][3 + k]);
+            }
+        BTask tk;
+        for (int k = 0; k < 3; ++k) {
+            tk.mn[k] = r[k] - 0.0001f;
+            tk.mx[k] = r[3 + k] + 0.0001f;
+        }
+        tk.node = 0;
+        tk.begin = 0;
+        tk.end = n;
+        tk.depth = 0;
+        tk.pad[0] = tk.pad[1] = 0;
+        *task = tk;
+    }
+}
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t ensure(size_t count) {
+        if (count <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, count * sizeof(T) + 16);
+        if (e == hipSuccess) cap = count;
+        return e;
+    }
+};
+
+}  // namespace
+
+ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t stride_floats,
+                            size_t n64, size_t max_leaf, hipStream_t stream, DeviceBvh* out) {
+    if (n64 == 0) return NoPrimitives;
+    if (n64 > (1u << 30)) {
+        set_error("prim_count %zu too large", n64);
+        return InvalidArgument;
+    }
+    const uint32_t n = uint32_t(n64);
+    const uint32_t stride = uint32_t(stride_floats);
+    const uint32_t leaf = max_leaf == 0 ? 1u : uint32_t(max_leaf > 0xFFFFFFFFu ? 0xFFFFFFFFu : max_leaf);
+    hipError_t e;
+    DeviceBvh res;
+    const uint32_t cap_nodes = 2 * n - 1;
+    if ((e = hipMalloc(&res.nodes, size_t(cap_nodes) * sizeof(DevAabb))) != hipSuccess)
+        return hip_fail(e, "hipMalloc(nodes)");
+    if ((e = hipMalloc(&res.indices, size_t(n) * 4)) != hipSuccess) {
+        (void)hipFree(res.nodes);
+        return hip_fail(e, "hipMalloc(indices)");
+    }
+    res.prim_count = n;
+    auto fail = [&](hipError_t err, const char* what) {
+        (void)hipFree(res.nodes);
+        (void)hipFree(res.indices);
+        return hip_fail(err, what);
+    };
+    DevBuf<DevAabb> pts;
+    const DevAabb* in_aabbs = reinterpret_cast<const DevAabb*>(aabbs);
+    const uint32_t g = (n + kBlock - 1) / kBlock;
+    if (!aabbs) {
+        if ((e = pts.ensure(n)) != hipSuccess) return fail(e, "hipMalloc(aabbs)");
+        hipLaunchKernelGGL(point_aabbs_kernel, dim3(g), dim3(kBlock), 0, stream, centers, stride,
+                           n, pts.p);
+        in_aabbs = pts.p;
+    }
+    DevBuf<uint32_t> idx1, flag, lscan, split_flag, scan_scratch;
+    DevBuf<int32_t> pt0, pt1;
+    DevBuf<BTask> tasks0, tasks1;
+    DevBuf<BDecision> dec;
+    DevBuf<BinAcc> bins;
+    uint32_t* host_total = nullptr;
+    if ((e = hipHostMalloc(&host_total, 8)) != hipSuccess) return fail(e, "hipHostMalloc");
+    struct HostFree {
+        uint32_t* p;
+        ~HostFree() { (void)hipHostFree(p); }
+    } hf{host_total};
+    if ((e = idx1.ensure(n)) != hipSuccess || (e = flag.ensure(n)) != hipSuccess ||
+        (e = lscan.ensure(n)) != hipSuccess || (e = pt0.ensure(n)) != hipSuccess ||
+        (e = pt1.ensure(n)) != hipSuccess || (e = tasks0.ensure(1)) != hipSuccess ||
+        (e = scan_scratch.ensure(scan_scratch_bytes(n) / 4 + 1)) != hipSuccess)
+        return fail(e, "binned workspace");
+    uint32_t* idx_cur = res.indices;
+    uint32_t* idx_nxt = idx1.p;
+    int32_t* pt_cur = pt0.p;
+    int32_t* pt_nxt = pt1.p;
+    hipLaunchKernelGGL(iota_kernel, dim3(g), dim3(kBlock), 0, stream, idx_cur, pt_cur, n);
+    hipLaunchKernelGGL(root_bounds_kernel, dim3(1), dim3(kBlock), 0, stream, in_aabbs, n,
+                       tasks0.p);
+    BTask* t_cur = tasks0.p;
+    uint32_t num_tasks = 1;
+    uint32_t counter = 1;  // AtomicNodeStack starts at 1 (builders/mod.rs:51-56)
+    int flip = 0;
+    while (num_tasks > 0) {
+        if ((e = dec.ensure(num_tasks)) != hipSuccess ||
+            (e = bins.ensure(size_t(num_tasks) * 3 * kBins)) != hipSuccess ||
+            (e = split_flag.ensure(num_tasks)) != hipSuccess)
+            return fail(e, "binned level workspace");
+        DevBuf<BTask>& nxt_buf = flip ? tasks0 : tasks1;
+        if ((e = nxt_buf.ensure(size_t(2) * num_tasks)) != hipSuccess)
+            return fail(e, "binned tasks");
+        BTask* t_nxt = nxt_buf.p;
+        const uint32_t tg = (num_tasks + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(level_init_kernel, dim3(num_tasks), dim3(64), 0, stream, t_cur,
+                           num_tasks, dec.p, bins.p);
+        hipLaunchKernelGGL(bin_kernel, dim3(g), dim3(kBlock), 0, stream, n, pt_cur, idx_cur, t_cur,
+                           dec.p, centers, stride, in_aabbs, bins.p);
+        hipLaunchKernelGGL(split_kernel, dim3(tg), dim3(kBlock), 0, stream, t_cur, num_tasks, dec.p,
+                           bins.p, leaf, split_flag.p);
+        if ((e = scan_exclusive_u32(split_flag.p, num_tasks, host_total, scan_scratch.p,
+                                    scan_scratch.cap * 4, stream)) != hipSuccess)
+            return fail(e, "binned task scan");
+        hipLaunchKernelGGL(children_kernel, dim3(tg), dim3(kBlock), 0, stream, t_cur, num_tasks,
+                           dec.p, split_flag.p, counter, res.nodes, t_nxt);
+        hipLaunchKernelGGL(flag_kernel, dim3(g), dim3(kBlock), 0, stream, n, pt_cur, idx_cur, dec.p,
+                           centers, stride, flag.p);
+        if ((e = hipMemcpyAsync(lscan.p, flag.p, size_t(n) * 4, hipMemcpyDeviceToDevice,
+                                stream)) != hipSuccess)
+            return fail(e, "binned flag copy");
+        if ((e = scan_exclusive_u32(lscan.p, n, nullptr, scan_scratch.p, scan_scratch.cap * 4,
+                                    stream)) != hipSuccess)
+            return fail(e, "binned partition scan");
+        hipLaunchKernelGGL(scatter_kernel, dim3(g), dim3(kBlock), 0, stream, n, pt_cur, idx_cur,
+                           t_cur, dec.p, split_flag.p, flag.p, lscan.p, idx_nxt, pt_nxt);
+        if ((e = hipGetLastError()) != hipSuccess) return fail(e, "binned level");
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return fail(e, "binned level");
+        const uint32_t splits = *host_total;
+        counter += 2 * splits;
+        num_tasks = 2 * splits;
+        std::swap(idx_cur, idx_nxt);
+        std::swap(pt_cur, pt_nxt);
+        t_cur = t_nxt;
+        flip ^= 1;
+    }
+    if (idx_cur != res.indices) {
+        if ((e = hipMemcpyAsync(res.indices, idx_cur, size_t(n) * 4, hipMemcpyDeviceToDevice,
+                                stream)) != hipSuccess)
+            return fail(e, "binned indices");
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return fail(e, "binned indices");
+    }
+    res.node_count = counter;
+    *out = res;
+    return Ok;
+}
+
+}  // namespace rtbvh

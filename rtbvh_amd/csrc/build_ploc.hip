@@ -1,0 +1,790 @@
+// GPU locally-ordered clustering (PLOC) builder for gfx950.
+//
+// Restates LocallyOrderedClusteringBuilder (src/builders/locb.rs) so that the
+// node array is the one the reference produces for the same input:
+//   world bounds  Aabb::union_of_list (+1e-4)          src/aabb.rs:125-131
+//   Morton codes  MortonEncoder::new/encode             src/morton.rs:36-61
+//   stable sort   get_sorted_indices (par_sort_by)      src/morton.rs:63-102
+//   leaves        [n-1, 2n-1), bounds +1e-4, count 1    locb.rs:285-294
+//   iterations    cluster()                             locb.rs:48-245
+// The sort is an LSD radix sort (stable, 4 passes of 8 bits), so ties keep
+// their input order as rayon's stable merge sort does.
+//
+// Each clustering iteration over the N active clusters is three kernels
+// while N is large:
+//   nn_mark   nearest neighbour over |i-j| <= r (ascending j, strict '<', so
+//             the lowest j wins ties, locb.rs:92-142) for a tile plus an r
+//             halo on each side, then the mutual-pair mark at the lower index
+//             (locb.rs:159-167) and a per-tile mark count;
+//   tile_scan exclusive scan of the tile counts (the prefix_sum of
+//             locb.rs:170-176) and the next iteration's bookkeeping;
+//   merge     parents at the upper index's compacted slot, children written
+//             to their final positions end-2M.. (locb.rs:178-215).
+// Once N <= kTailMax the remaining iterations run in one workgroup with the
+// clusters resident in LDS (the long tail of small iterations costs one
+// launch).
+#include <vector>
+
+#include "internal.hpp"
+
+namespace rtbvh {
+
+namespace {
+
+constexpr int kRadius = 14;            // locb.rs:27
+constexpr int kTile = 1024;            // clusters per tile in the multi-block phase
+constexpr int kBlock = 256;
+constexpr int kTailThreads = 1024;
+constexpr int kTailMax = 4096;         // clusters held in LDS by the tail kernel
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortBlock * kSortItems;
+
+struct Bounds6 {
+    float mn[3], mx[3];
+};
+
+__device__ __forceinline__ float half_area_union(const Bounds6& a, const Bounds6& b) {
+    // half_area(a.union_of(b)), src/aabb.rs:74-86, 343-346
+    const float n0 = rs_min(a.mn[0], b.mn[0]), n1 = rs_min(a.mn[1], b.mn[1]),
+                n2 = rs_min(a.mn[2], b.mn[2]);
+    const float x0 = rs_max(a.mx[0], b.mx[0]), x1 = rs_max(a.mx[1], b.mx[1]),
+                x2 = rs_max(a.mx[2], b.mx[2]);
+    const float d0 = x0 - n0, d1 = x1 - n1, d2 = x2 - n2;
+    return (d0 + d1) * d2 + d0 * d1;
+}
+
+__device__ __forceinline__ Bounds6 bounds_of(const DevAabb& a) {
+    Bounds6 b;
+    b.mn[0] = a.min[0];
+    b.mn[1] = a.min[1];
+    b.mn[2] = a.min[2];
+    b.mx[0] = a.max[0];
+    b.mx[1] = a.max[1];
+    b.mx[2] = a.max[2];
+    return b;
+}
+
+__device__ __forceinline__ DevAabb union_node(const DevAabb& s, const DevAabb& b) {
+    DevAabb r;
+    for (int k = 0; k < 3; ++k) {
+        r.min[k] = rs_min(s.min[k], b.min[k]);
+        r.max[k] = rs_max(s.max[k], b.max[k]);
+    }
+    r.extra1 = 0;
+    r.extra2 = 0;
+    return r;
+}
+
+__device__ __forceinline__ void load_node(const DevAabb* p, DevAabb& out) {
+    const float4* q = reinterpret_cast<const float4*>(p);
+    const float4 a = q[0], b = q[1];
+    out.min[0] = a.x;
+    out.min[1] = a.y;
+    out.min[2] = a.z;
+    out.extra1 = __float_as_int(a.w);
+    out.max[0] = b.x;
+    out.max[1] = b.y;
+    out.max[2] = b.z;
+    out.extra2 = __float_as_int(b.w);
+}
+
+__device__ __forceinline__ void store_node(DevAabb* p, const DevAabb& n) {
+    float4* q = reinterpret_cast<float4*>(p);
+    q[0] = make_float4(n.min[0], n.min[1], n.min[2], __int_as_float(n.extra1));
+    q[1] = make_float4(n.max[0], n.max[1], n.max[2], __int_as_float(n.extra2));
+}
+
+// ------------------------------------------------------- input preparation --
+
+// Vector3/Vector4 Primitive::aabb (rtbvh_ffi/src/lib.rs:401-421): an empty
+// box grown by the centre.
+__global__ void point_aabbs_kernel(const float* centers, uint32_t stride, uint32_t n,
+                                   DevAabb* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    DevAabb a;
+    for (int k = 0; k < 3; ++k) {
+        const float c = centers[size_t(i) * stride + k];
+        a.min[k] = rs_min(1e34f, c);
+        a.max[k] = rs_max(-1e34f, c);
+    }
+    a.extra1 = 0;
+    a.extra2 = 0;
+    store_node(out + i, a);
+}
+
+// World bounds, first level: per-block min/max with the grow_bb operators.
+__global__ void bounds_partial_kernel(const DevAabb* aabbs, uint32_t n, Bounds6* partial) {
+    __shared__ Bounds6 sh[kBlock];
+    Bounds6 acc;
+    for (int k = 0; k < 3; ++k) {
+        acc.mn[k] = 1e34f;
+        acc.mx[k] = -1e34f;
+    }
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        DevAabb a;
+        load_node(aabbs + i, a);
+        for (int k = 0; k < 3; ++k) {
+            acc.mn[k] = rs_min(acc.mn[k], a.min[k]);
+            acc.mx[k] = rs_max(acc.mx[k], a.max[k]);
+        }
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) {
+            for (int k = 0; k < 3; ++k) {
+                sh[threadIdx.x].mn[k] = rs_min(sh[threadIdx.x].mn[k], sh[threadIdx.x + s].mn[k]);
+                sh[threadIdx.x].mx[k] = rs_max(sh[threadIdx.x].mx[k], sh[threadIdx.x + s].mx[k]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// World bounds, second level, then Aabb::with_offset(1e-4).  For n <= 2 the
+// reduction is sequential (the bounds become the root's bounds).
+__global__ void bounds_final_kernel(const Bounds6* partial, uint32_t parts, Bounds6* world) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    Bounds6 acc;
+    for (int k = 0; k < 3; ++k) {
+        acc.mn[k] = 1e34f;
+        acc.mx[k] = -1e34f;
+    }
+    for (uint32_t p = 0; p < parts; ++p)
+        for (int k = 0; k < 3; ++k) {
+            acc.mn[k] = rs_min(acc.mn[k], partial[p].mn[k]);
+            acc.mx[k] = rs_max(acc.mx[k], partial[p].mx[k]);
+        }
+    for (int k = 0; k < 3; ++k) {
+        acc.mn[k] = acc.mn[k] - 0.0001f;
+        acc.mx[k] = acc.mx[k] + 0.0001f;
+    }
+    *world = acc;
+}
+
+// morton_split for 10-bit inputs: bit b moves to bit 3b (the reference's
+// mask loop, src/morton.rs:10-25, is this spreading).
+__device__ __forceinline__ uint32_t part1by2(uint32_t x) {
+    x &= 0x3FFu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// Rust `f as i32` then max(0), min(1023) (src/morton.rs:57-59).
+__device__ __forceinline__ uint32_t grid_coord(float g) {
+    int32_t q;
+    if (g != g)
+        q = 0;
+    else if (g >= 2147483648.0f)
+        q = 2147483647;
+    else if (g <= -2147483904.0f)
+        q = -2147483647 - 1;
+    else
+        q = int32_t(g);
+    q = q < 0 ? 0 : q;
+    q = q > 1023 ? 1023 : q;
+    return uint32_t(q);
+}
+
+__global__ void morton_kernel(const float* centers, uint32_t stride, uint32_t n,
+                              const Bounds6* world, uint32_t* keys, uint32_t* vals) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Bounds6 w = *world;
+    uint32_t x[3];
+    for (int k = 0; k < 3; ++k) {
+        // MortonEncoder::new: wtg = 1024 * (1 / diag); off = (-min) * wtg
+        const float wtg = 1024.0f * (1.0f / (w.mx[k] - w.mn[k]));
+        const float off = (-w.mn[k]) * wtg;
+        const float g = centers[size_t(i) * stride + k] * wtg + off;
+        x[k] = grid_coord(g);
+    }
+    keys[i] = part1by2(x[0]) | (part1by2(x[1]) << 1) | (part1by2(x[2]) << 2);
+    vals[i] = i;
+}
+
+// ------------------------------------------------------------ radix sort --
+// Stable LSD radix sort of (key, value) pairs, 8-bit digits.
+__global__ void radix_hist_kernel(const uint32_t* keys, uint32_t n, uint32_t shift,
+                                  uint32_t tiles, uint32_t* counts) {
+    __shared__ uint32_t hist[256];
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortItems; ++r) {
+        const uint32_t i = base + r * kSortBlock + threadIdx.x;
+        if (i < n) atomicAdd(&hist[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    counts[threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
+}
+
+// Exclusive scan of `count` values by one workgroup of 1024 threads.
+__global__ void __launch_bounds__(1024) scan_exclusive_kernel(uint32_t* data, uint32_t count,
+                                                              uint32_t* total) {
+    __shared__ uint32_t sums[1024];
+    const uint32_t per = (count + 1023) / 1024;
+    const uint32_t b = threadIdx.x * per;
+    const uint32_t e = b + per < count ? b + per : count;
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += data[i];
+    sums[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = threadIdx.x >= uint32_t(off) ? sums[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sums[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? sums[threadIdx.x - 1] : 0u;
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t v = data[i];
+        data[i] = run;
+        run += v;
+    }
+    if (total && threadIdx.x == 1023) *total = sums[1023];
+}
+
+__global__ void __launch_bounds__(kSortBlock)
+    radix_scatter_kernel(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t n,
+                         uint32_t shift, uint32_t tiles, const uint32_t* offsets,
+                         uint32_t* keys_out, uint32_t* vals_out) {
+    constexpr int kWaves = kSortBlock / 64;
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[kWaves][256];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = threadIdx.x >> 6;
+    run[threadIdx.x] = offsets[threadIdx.x * tiles + blockIdx.x];
+    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    const uint32_t base = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortItems; ++r) {
+        for (int w = 0; w < kWaves; ++w) wcnt[w][threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t i = base + r * kSortBlock + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t key = valid ? keys_in[i] : 0u;
+        const uint32_t val = valid ? vals_in[i] : 0u;
+        const uint32_t d = (key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = __popcll(peers & lt_mask);
+        const uint32_t cnt = __popcll(peers);
+        if (valid && rank == 0) wcnt[wave][d] = cnt;
+        __syncthreads();
+        uint32_t pos = 0;
+        if (valid) {
+            pos = run[d] + rank;
+            for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][d];
+        }
+        __syncthreads();
+        {
+            uint32_t add = 0;
+            for (int w = 0; w < kWaves; ++w) add += wcnt[w][threadIdx.x];
+            run[threadIdx.x] += add;
+        }
+        if (valid) {
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- leaves --
+// locb.rs:289-294: leaf i = aabbs[prim_indices[i]].with_offset(1e-4), count 1,
+// left_first i.
+__global__ void leaves_kernel(const DevAabb* aabbs, const uint32_t* sorted, uint32_t n,
+                              DevAabb* clusters) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    DevAabb a;
+    load_node(aabbs + sorted[i], a);
+    for (int k = 0; k < 3; ++k) {
+        a.min[k] = a.min[k] - 0.0001f;
+        a.max[k] = a.max[k] + 0.0001f;
+    }
+    a.extra1 = 1;
+    a.extra2 = int32_t(i);
+    store_node(clusters + i, a);
+}
+
+// n <= 2: a single root leaf with the world bounds (locb.rs:258-269).
+__global__ void small_root_kernel(const Bounds6* world, uint32_t n, DevAabb* nodes,
+                                  uint32_t* indices) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    DevAabb r;
+    for (int k = 0; k < 3; ++k) {
+        r.min[k] = world->mn[k];
+        r.max[k] = world->mx[k];
+    }
+    r.extra1 = int32_t(n);
+    r.extra2 = 0;
+    store_node(nodes, r);
+    for (uint32_t i = 0; i < n; ++i) indices[i] = i;
+}
+
+// ------------------------------------------------------ clustering state --
+struct PlocState {
+    uint32_t n_active;   // clusters in [begin, end)
+    uint32_t begin;      // reference array positions of the active range
+    uint32_t end;
+    uint32_t merged;     // pairs merged by the current iteration (M)
+};
+
+__device__ __forceinline__ void nn_window(uint32_t i, uint32_t n, uint32_t& sb, uint32_t& se) {
+    // search_range, locb.rs:36-45 (relative to begin = 0)
+    sb = i > uint32_t(kRadius) ? i - kRadius : 0u;
+    se = (i + kRadius + 1) < n ? i + kRadius + 1 : n;
+}
+
+// nearest neighbour + mutual marks for one tile.
+__global__ void __launch_bounds__(kBlock)
+    nn_mark_kernel(const PlocState* state, const DevAabb* cl, uint32_t* nb_out,
+                   uint32_t* mark_out, uint32_t* tile_count) {
+    constexpr int kHalo = 2 * kRadius;
+    __shared__ Bounds6 sb_bounds[kTile + 2 * kHalo];
+    __shared__ uint32_t s_nb[kTile + 2 * kRadius];
+    __shared__ uint32_t s_cnt[kBlock];
+    const uint32_t n = state->n_active;
+    const uint32_t ts = blockIdx.x * kTile;
+    if (ts >= n) return;
+    const uint32_t te = ts + kTile < n ? ts + kTile : n;
+    const uint32_t lb = ts > uint32_t(kHalo) ? ts - kHalo : 0u;
+    const uint32_t le = te + kHalo < n ? te + kHalo : n;
+    for (uint32_t k = lb + threadIdx.x; k < le; k += kBlock) {
+        DevAabb a;
+        load_node(cl + k, a);
+        sb_bounds[k - lb] = bounds_of(a);
+    }
+    __syncthreads();
+    const uint32_t nb0 = ts > uint32_t(kRadius) ? ts - kRadius : 0u;
+    const uint32_t nb1 = te + kRadius < n ? te + kRadius : n;
+    for (uint32_t i = nb0 + threadIdx.x; i < nb1; i += kBlock) {
+        uint32_t s0, s1;
+        nn_window(i, n, s0, s1);
+        float best = 3.40282347e38f;  // f32::MAX
+        uint32_t best_j = 0xFFFFFFFFu;
+        const Bounds6 bi = sb_bounds[i - lb];
+        for (uint32_t j = s0; j < s1; ++j) {
+            if (j == i) continue;
+            const Bounds6 bj = sb_bounds[j - lb];
+            const float d = j < i ? half_area_union(bj, bi) : half_area_union(bi, bj);
+            if (d < best) {
+                best = d;
+                best_j = j;
+            }
+        }
+        s_nb[i - nb0] = best_j;
+    }
+    __syncthreads();
+    uint32_t cnt = 0;
+    for (uint32_t i = ts + threadIdx.x; i < te; i += kBlock) {
+        const uint32_t j = s_nb[i - nb0];
+        uint32_t m = 0;
+        if (j > i && j < nb1) m = (s_nb[j - nb0] == i) ? 1u : 0u;
+        nb_out[i] = j;
+        mark_out[i] = m;
+        cnt += m;
+    }
+    s_cnt[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) s_cnt[threadIdx.x] += s_cnt[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tile_count[blockIdx.x] = s_cnt[0];
+}
+
+// Exclusive scan of the tile counts; writes M into the current state and the
+// next iteration's state (locb.rs:178-185).
+__global__ void __launch_bounds__(1024)
+    tile_scan_kernel(PlocState* state, PlocState* next, uint32_t* tile_count) {
+    __shared__ uint32_t sums[1024];
+    const uint32_t n = state->n_active;
+    const uint32_t tiles = (n + kTile - 1) / kTile;
+    const uint32_t per = (tiles + 1023) / 1024;
+    const uint32_t b = threadIdx.x * per;
+    const uint32_t e = b + per < tiles ? b + per : tiles;
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += tile_count[i];
+    sums[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = threadIdx.x >= uint32_t(off) ? sums[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sums[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? sums[threadIdx.x - 1] : 0u;
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t v = tile_count[i];
+        tile_count[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023) {
+        const uint32_t m = sums[1023];
+        state->merged = m;
+        next->n_active = n - m;
+        next->begin = state->begin - m;
+        next->end = state->end - 2 * m;
+        next->merged = 0;
+    }
+}
+
+// Parents, children and compaction for one tile (locb.rs:193-215).  m[k] is
+// the inclusive prefix of marks; the tile keeps m for [ts - r, te).
+__global__ void __launch_bounds__(kBlock)
+    merge_kernel(const PlocState* state, const DevAabb* cl, const uint32_t* nb,
+                 const uint32_t* mark, const uint32_t* tile_offset, DevAabb* cl_out,
+                 DevAabb* nodes) {
+    __shared__ uint32_t s_m[kRadius + kTile];
+    __shared__ uint32_t s_scan[kBlock];
+    const uint32_t n = state->n_active;
+    const uint32_t ts = blockIdx.x * kTile;
+    if (ts >= n) return;
+    const uint32_t te = ts + kTile < n ? ts + kTile : n;
+    const uint32_t children_begin = state->end - 2 * state->merged;
+    const uint32_t excl = tile_offset[blockIdx.x];  // marks in [0, ts)
+    // inclusive prefix inside the tile: each thread scans 4 consecutive marks
+    constexpr int kPer = kTile / kBlock;
+    uint32_t v[kPer];
+    uint32_t s = 0;
+    for (int k = 0; k < kPer; ++k) {
+        const uint32_t i = ts + threadIdx.x * kPer + k;
+        v[k] = i < te ? mark[i] : 0u;
+        s += v[k];
+    }
+    s_scan[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        const uint32_t t = threadIdx.x >= uint32_t(off) ? s_scan[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s_scan[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = excl + (threadIdx.x ? s_scan[threadIdx.x - 1] : 0u);
+    for (int k = 0; k < kPer; ++k) {
+        run += v[k];
+        s_m[kRadius + threadIdx.x * kPer + k] = run;
+    }
+    // left halo: m[k] = m[ts-1] - sum(mark[k+1 .. ts-1])
+    if (threadIdx.x == 0) {
+        uint32_t acc = excl;
+        for (int h = 1; h <= kRadius; ++h) {
+            if (ts < uint32_t(h)) break;
+            const uint32_t k = ts - h;  // m[k]
+            s_m[kRadius - h] = acc;
+            acc -= mark[k];
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = ts + threadIdx.x; i < te; i += kBlock) {
+        const uint32_t j = nb[i];
+        const bool mutual = j < n && nb[j] == i;
+        const uint32_t mi = s_m[kRadius + (i - ts)];
+        if (mutual) {
+            if (j < i) {
+                // i is the upper element: the parent takes its compacted slot
+                const uint32_t mj = s_m[kRadius + j - ts];  // j >= ts - r
+                const uint32_t first_child = children_begin + (mj - 1) * 2;
+                DevAabb ci, cj;
+                load_node(cl + i, ci);
+                load_node(cl + j, cj);
+                DevAabb parent = union_node(ci, cj);
+                parent.extra1 = -1;
+                parent.extra2 = int32_t(first_child);
+                store_node(cl_out + (i - mi), parent);
+                store_node(nodes + first_child, cj);
+                store_node(nodes + first_child + 1, ci);
+            }
+        } else {
+            DevAabb ci;
+            load_node(cl + i, ci);
+            store_node(cl_out + (i - mi), ci);
+        }
+    }
+}
+
+// All remaining iterations in one workgroup, clusters resident in LDS.
+struct TailShared {
+    DevAabb cl[kTailMax];
+    uint16_t nb[kTailMax];
+    uint16_t m[kTailMax];
+    uint32_t sums[kTailThreads];
+};
+
+__global__ void __launch_bounds__(kTailThreads)
+    tail_kernel(const PlocState* state, const DevAabb* cl_in, DevAabb* nodes) {
+    extern __shared__ unsigned char smem_raw[];
+    TailShared& sh = *reinterpret_cast<TailShared*>(smem_raw);
+    uint32_t n = state->n_active;
+    uint32_t end = state->end;
+    for (uint32_t i = threadIdx.x; i < n; i += kTailThreads) load_node(cl_in + i, sh.cl[i]);
+    __syncthreads();
+    constexpr int kPer = kTailMax / kTailThreads;  // consecutive clusters per thread
+    while (n > 1) {
+        const uint32_t base = threadIdx.x * kPer;
+        // nearest neighbours
+        for (int k = 0; k < kPer; ++k) {
+            const uint32_t i = base + k;
+            if (i >= n) break;
+            uint32_t s0, s1;
+            nn_window(i, n, s0, s1);
+            float best = 3.40282347e38f;
+            uint32_t best_j = 0xFFFFu;
+            const Bounds6 bi = bounds_of(sh.cl[i]);
+            for (uint32_t j = s0; j < s1; ++j) {
+                if (j == i) continue;
+                const Bounds6 bj = bounds_of(sh.cl[j]);
+                const float d = j < i ? half_area_union(bj, bi) : half_area_union(bi, bj);
+                if (d < best) {
+                    best = d;
+                    best_j = j;
+                }
+            }
+            sh.nb[i] = uint16_t(best_j);
+        }
+        __syncthreads();
+        // marks and their inclusive prefix
+        uint32_t mk[kPer];
+        uint32_t s = 0;
+        for (int k = 0; k < kPer; ++k) {
+            const uint32_t i = base + k;
+            uint32_t mm = 0;
+            if (i < n) {
+                const uint32_t j = sh.nb[i];
+                mm = (j > i && j < n && sh.nb[j] == i) ? 1u : 0u;
+            }
+            mk[k] = mm;
+            s += mm;
+        }
+        sh.sums[threadIdx.x] = s;
+        __syncthreads();
+        for (int off = 1; off < kTailThreads; off <<= 1) {
+            const uint32_t t = threadIdx.x >= uint32_t(off) ? sh.sums[threadIdx.x - off] : 0u;
+            __syncthreads();
+            sh.sums[threadIdx.x] += t;
+            __syncthreads();
+        }
+        const uint32_t merged = sh.sums[kTailThreads - 1];
+        uint32_t run = threadIdx.x ? sh.sums[threadIdx.x - 1] : 0u;
+        for (int k = 0; k < kPer; ++k) {
+            run += mk[k];
+            if (base + k < n) sh.m[base + k] = uint16_t(run);
+        }
+        __syncthreads();
+        if (merged == 0) break;  // no mutual pair: only with NaN bounds
+        const uint32_t children_begin = end - 2 * merged;
+        // read everything this thread writes, then write after a barrier
+        DevAabb keep[kPer];
+        uint32_t dst[kPer];
+        for (int k = 0; k < kPer; ++k) {
+            const uint32_t i = base + k;
+            dst[k] = 0xFFFFFFFFu;
+            if (i >= n) continue;
+            const uint32_t j = sh.nb[i];
+            const bool mutual = j < n && sh.nb[j] == i;
+            const uint32_t mi = sh.m[i];
+            if (mutual) {
+                if (j < i) {
+                    const uint32_t first_child = children_begin + (uint32_t(sh.m[j]) - 1) * 2;
+                    const DevAabb ci = sh.cl[i];
+                    const DevAabb cj = sh.cl[j];
+                    DevAabb parent = union_node(ci, cj);
+                    parent.extra1 = -1;
+                    parent.extra2 = int32_t(first_child);
+                    store_node(nodes + first_child, cj);
+                    store_node(nodes + first_child + 1, ci);
+                    keep[k] = parent;
+                    dst[k] = i - mi;
+                }
+            } else {
+                keep[k] = sh.cl[i];
+                dst[k] = i - mi;
+            }
+        }
+        __syncthreads();
+        for (int k = 0; k < kPer; ++k)
+            if (dst[k] != 0xFFFFFFFFu) sh.cl[dst[k]] = keep[k];
+        __syncthreads();
+        n -= merged;
+        end -= 2 * merged;
+    }
+    if (threadIdx.x == 0) store_node(nodes, sh.cl[0]);
+}
+
+struct Workspace {
+    void* base = nullptr;
+    ~Workspace() {
+        if (base) (void)hipFree(base);
+    }
+};
+
+size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace
+
+ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stride_floats,
+                          size_t n64, int radius, hipStream_t stream, DeviceBvh* out) {
+    if (radius != kRadius) {
+        set_error("PLOC search radius %d not supported (built for %d)", radius, kRadius);
+        return InvalidArgument;
+    }
+    if (n64 == 0) return NoPrimitives;
+    if (n64 > (1u << 30)) {
+        set_error("prim_count %zu too large", n64);
+        return InvalidArgument;
+    }
+    const uint32_t n = uint32_t(n64);
+    const uint32_t node_count = n <= 2 ? 1u : 2 * n - 1;
+    const uint32_t stride = uint32_t(stride_floats);
+    hipError_t e;
+    DeviceBvh res;
+    if ((e = hipMalloc(&res.nodes, size_t(node_count) * sizeof(DevAabb))) != hipSuccess)
+        return hip_fail(e, "hipMalloc(nodes)");
+    if ((e = hipMalloc(&res.indices, size_t(n) * sizeof(uint32_t))) != hipSuccess) {
+        (void)hipFree(res.nodes);
+        return hip_fail(e, "hipMalloc(indices)");
+    }
+    res.node_count = node_count;
+    res.prim_count = n;
+    auto fail = [&](hipError_t err, const char* what) {
+        (void)hipFree(res.nodes);
+        (void)hipFree(res.indices);
+        return hip_fail(err, what);
+    };
+
+    const uint32_t sort_tiles = (n + kSortTile - 1) / kSortTile;
+    const uint32_t ploc_tiles = (n + kTile - 1) / kTile;
+    const uint32_t parts = 256;
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t o = off;
+        off += align_up(bytes);
+        return o;
+    };
+    const size_t o_aabb = aabbs ? 0 : carve(size_t(n) * sizeof(DevAabb));
+    const size_t o_part = carve(parts * sizeof(Bounds6));
+    const size_t o_world = carve(sizeof(Bounds6));
+    const size_t o_k0 = carve(size_t(n) * 4), o_k1 = carve(size_t(n) * 4);
+    const size_t o_v1 = carve(size_t(n) * 4);
+    const size_t o_cnt = carve(size_t(256) * sort_tiles * 4);
+    const size_t o_cl0 = carve(size_t(n) * sizeof(DevAabb));
+    const size_t o_cl1 = carve(size_t(n) * sizeof(DevAabb));
+    const size_t o_nb = carve(size_t(n) * 4), o_mark = carve(size_t(n) * 4);
+    const size_t o_tc = carve(size_t(ploc_tiles) * 4);
+    const size_t o_state = carve(2 * sizeof(PlocState));
+    Workspace ws;
+    if ((e = hipMalloc(&ws.base, off)) != hipSuccess) return fail(e, "hipMalloc(workspace)");
+    char* b = static_cast<char*>(ws.base);
+    const DevAabb* in_aabbs = aabbs ? reinterpret_cast<const DevAabb*>(aabbs)
+                                    : reinterpret_cast<DevAabb*>(b + o_aabb);
+    Bounds6* part = reinterpret_cast<Bounds6*>(b + o_part);
+    Bounds6* world = reinterpret_cast<Bounds6*>(b + o_world);
+    uint32_t* k0 = reinterpret_cast<uint32_t*>(b + o_k0);
+    uint32_t* k1 = reinterpret_cast<uint32_t*>(b + o_k1);
+    uint32_t* v0 = res.indices;  // the final sorted order lands here
+    uint32_t* v1 = reinterpret_cast<uint32_t*>(b + o_v1);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(b + o_cnt);
+    DevAabb* cl0 = reinterpret_cast<DevAabb*>(b + o_cl0);
+    DevAabb* cl1 = reinterpret_cast<DevAabb*>(b + o_cl1);
+    uint32_t* nbuf = reinterpret_cast<uint32_t*>(b + o_nb);
+    uint32_t* mbuf = reinterpret_cast<uint32_t*>(b + o_mark);
+    uint32_t* tcount = reinterpret_cast<uint32_t*>(b + o_tc);
+    PlocState* st = reinterpret_cast<PlocState*>(b + o_state);
+
+    const uint32_t g = (n + kBlock - 1) / kBlock;
+    if (!aabbs)
+        hipLaunchKernelGGL(point_aabbs_kernel, dim3(g), dim3(kBlock), 0, stream, centers, stride, n,
+                           const_cast<DevAabb*>(in_aabbs));
+    if (n <= 2) {
+        hipLaunchKernelGGL(bounds_partial_kernel, dim3(1), dim3(kBlock), 0, stream, in_aabbs, n,
+                           part);
+        hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(1), 0, stream, part, 1u, world);
+        hipLaunchKernelGGL(small_root_kernel, dim3(1), dim3(1), 0, stream, world, n, res.nodes,
+                           res.indices);
+        if ((e = hipGetLastError()) != hipSuccess) return fail(e, "small build");
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return fail(e, "small build");
+        *out = res;
+        return Ok;
+    }
+    hipLaunchKernelGGL(bounds_partial_kernel, dim3(parts), dim3(kBlock), 0, stream, in_aabbs, n,
+                       part);
+    hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(1), 0, stream, part, parts, world);
+    hipLaunchKernelGGL(morton_kernel, dim3(g), dim3(kBlock), 0, stream, centers, stride, n, world,
+                       k0, v0);
+    // 4 stable passes; after an even number of passes the result is back in
+    // (k0, v0) = (keys, res.indices).
+    uint32_t *ki = k0, *vi = v0, *ko = k1, *vo = v1;
+    for (uint32_t shift = 0; shift < 32; shift += 8) {
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(sort_tiles), dim3(256), 0, stream, ki, n, shift,
+                           sort_tiles, counts);
+        hipLaunchKernelGGL(scan_exclusive_kernel, dim3(1), dim3(1024), 0, stream, counts,
+                           256u * sort_tiles, static_cast<uint32_t*>(nullptr));
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3(sort_tiles), dim3(kSortBlock), 0, stream, ki,
+                           vi, n, shift, sort_tiles, counts, ko, vo);
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+    }
+    hipLaunchKernelGGL(leaves_kernel, dim3(g), dim3(kBlock), 0, stream, in_aabbs, res.indices, n,
+                       cl0);
+    PlocState init{n, n - 1, 2 * n - 1, 0};
+    if ((e = hipMemcpyAsync(st, &init, sizeof(init), hipMemcpyHostToDevice, stream)) != hipSuccess)
+        return fail(e, "state init");
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e, "sort launch");
+
+    // multi-block iterations while the active set is large
+    uint32_t active = n;
+    int cur = 0;
+    DevAabb* cls[2] = {cl0, cl1};
+    PlocState host_state;
+    while (active > uint32_t(kTailMax)) {
+        const uint32_t tiles = (active + kTile - 1) / kTile;
+        PlocState* s_cur = st + cur;
+        PlocState* s_nxt = st + (cur ^ 1);
+        hipLaunchKernelGGL(nn_mark_kernel, dim3(tiles), dim3(kBlock), 0, stream, s_cur, cls[cur],
+                           nbuf, mbuf, tcount);
+        hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, stream, s_cur, s_nxt, tcount);
+        hipLaunchKernelGGL(merge_kernel, dim3(tiles), dim3(kBlock), 0, stream, s_cur, cls[cur],
+                           nbuf, mbuf, tcount, cls[cur ^ 1], res.nodes);
+        if ((e = hipGetLastError()) != hipSuccess) return fail(e, "ploc iteration");
+        if ((e = hipMemcpyAsync(&host_state, s_nxt, sizeof(host_state), hipMemcpyDeviceToHost,
+                                stream)) != hipSuccess)
+            return fail(e, "state read");
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return fail(e, "ploc iteration");
+        cur ^= 1;
+        if (host_state.n_active == active) {
+            (void)hipFree(res.nodes);
+            (void)hipFree(res.indices);
+            set_error("clustering made no progress (non-finite bounds?)");
+            return Error;
+        }
+        active = host_state.n_active;
+    }
+    static bool lds_ok = false;
+    if (!lds_ok) {
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(tail_kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(TailShared)));
+        if (e != hipSuccess) return fail(e, "tail LDS size");
+        lds_ok = true;
+    }
+    hipLaunchKernelGGL(tail_kernel, dim3(1), dim3(kTailThreads), sizeof(TailShared), stream,
+                       st + cur, cls[cur], res.nodes);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e, "tail launch");
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return fail(e, "ploc tail");
+    *out = res;
+    return Ok;
+}
+
+}  // namespace rtbvh

@@ -1,0 +1,330 @@
+// GPU MBVH collapse (Mbvh::construct, src/bvh.rs:381-404 with
+// MbvhNode::merge_nodes, src/mbvh_node.rs:297-411) and GPU refit
+// (Bvh::refit, src/bvh.rs:176-205).
+//
+// merge_nodes makes one MBVH node for the root and one for every interior
+// binary node whose parent is an interior child of an MBVH node — i.e. for
+// the interior nodes at even depth.  Its four slots come from the node's two
+// children: a leaf child fills one slot and keeps the parent's bounds
+// (mbvh_node.rs:305,319-321, reproduced as is), an interior child contributes
+// its two children as slots with their own bounds.  The reference numbers
+// MBVH nodes in depth-first pre-order; here they are numbered in binary-index
+// order (one scan), which describes the same tree.
+#include "internal.hpp"
+
+namespace rtbvh {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kMaxDepth = 1u << 20;
+
+__device__ __forceinline__ void ld(const DevAabb* p, DevAabb& o) {
+    const float4* q = reinterpret_cast<const float4*>(p);
+    const float4 a = q[0], b = q[1];
+    o.min[0] = a.x;
+    o.min[1] = a.y;
+    o.min[2] = a.z;
+    o.extra1 = __float_as_int(a.w);
+    o.max[0] = b.x;
+    o.max[1] = b.y;
+    o.max[2] = b.z;
+    o.extra2 = __float_as_int(b.w);
+}
+
+__device__ __forceinline__ void st(DevAabb* p, const DevAabb& n) {
+    float4* q = reinterpret_cast<float4*>(p);
+    q[0] = make_float4(n.min[0], n.min[1], n.min[2], __int_as_float(n.extra1));
+    q[1] = make_float4(n.max[0], n.max[1], n.max[2], __int_as_float(n.extra2));
+}
+
+__global__ void fill_i32_kernel(int32_t* p, uint32_t n, int32_t v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// parent[child] for every interior node (count < 0, left_first >= 0).
+__global__ void parent_kernel(const DevAabb* nodes, uint32_t count, int32_t* parent) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    DevAabb n;
+    ld(nodes + i, n);
+    if (n.extra1 < 0 && n.extra2 >= 0 && uint32_t(n.extra2) + 1 < count) {
+        parent[n.extra2] = int32_t(i);
+        parent[n.extra2 + 1] = int32_t(i);
+    }
+}
+
+// flag[i] = 1 for the root and for interior nodes at even depth.
+__global__ void mflag_kernel(const DevAabb* nodes, uint32_t count, const int32_t* parent,
+                             uint32_t* flag, uint32_t* status) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint32_t f = 0;
+    if (i == 0) {
+        f = 1;
+    } else {
+        DevAabb n;
+        ld(nodes + i, n);
+        if (n.extra1 < 0 && n.extra2 >= 0) {
+            uint32_t depth = 0;
+            int32_t p = int32_t(i);
+            while (p > 0 && depth < kMaxDepth) {
+                p = parent[p];
+                ++depth;
+            }
+            if (p == 0)
+                f = (depth & 1u) ? 0u : 1u;
+            else if (depth >= kMaxDepth)
+                atomicOr(status, 2u);  // cycle or absurd depth
+        }
+    }
+    flag[i] = f;
+}
+
+__device__ __forceinline__ void set_slot_bounds(DevMbvhNode& m, int s, const DevAabb& b) {
+    float* mnx = reinterpret_cast<float*>(&m.min_x);
+    float* mxx = reinterpret_cast<float*>(&m.max_x);
+    float* mny = reinterpret_cast<float*>(&m.min_y);
+    float* mxy = reinterpret_cast<float*>(&m.max_y);
+    float* mnz = reinterpret_cast<float*>(&m.min_z);
+    float* mxz = reinterpret_cast<float*>(&m.max_z);
+    mnx[s] = b.min[0];
+    mny[s] = b.min[1];
+    mnz[s] = b.min[2];
+    mxx[s] = b.max[0];
+    mxy[s] = b.max[1];
+    mxz[s] = b.max[2];
+}
+
+__global__ void mfill_kernel(const DevAabb* nodes, uint32_t count, const uint32_t* flag,
+                             const uint32_t* mindex, DevMbvhNode* mnodes) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count || flag[i] == 0) return;
+    DevAabb cur;
+    ld(nodes + i, cur);
+    DevMbvhNode m;
+    int32_t* ch = reinterpret_cast<int32_t*>(&m.children);
+    int32_t* ct = reinterpret_cast<int32_t*>(&m.counts);
+    for (int s = 0; s < 4; ++s) {
+        set_slot_bounds(m, s, cur);  // mbvh_node.rs:304-306
+        ch[s] = -1;
+        ct[s] = -1;
+    }
+    const int32_t lf = cur.extra2;
+    if (lf >= 0) {
+        for (int side = 0; side < 2; ++side) {
+            const uint32_t c = uint32_t(lf) + uint32_t(side);
+            if (c >= count) continue;
+            DevAabb cn;
+            ld(nodes + c, cn);
+            if (cn.extra2 < 0) continue;
+            const int s0 = 2 * side;
+            if (cn.extra1 >= 0) {
+                // leaf child: one slot, parent's bounds kept
+                ch[s0] = cn.extra2;
+                ct[s0] = cn.extra1;
+                continue;
+            }
+            for (int g = 0; g < 2; ++g) {
+                const uint32_t gi = uint32_t(cn.extra2) + uint32_t(g);
+                if (gi >= count) continue;
+                DevAabb gn;
+                ld(nodes + gi, gn);
+                set_slot_bounds(m, s0 + g, gn);
+                if (gn.extra1 >= 0) {
+                    if (gn.extra2 >= 0) {
+                        ch[s0 + g] = gn.extra2;
+                        ct[s0 + g] = gn.extra1;
+                    }
+                } else {
+                    ch[s0 + g] = int32_t(mindex[gi]);
+                }
+            }
+        }
+    }
+    mnodes[mindex[i]] = m;
+}
+
+// ------------------------------------------------------------------ refit --
+// Leaves: grow from Aabb::new over their primitives' aabbs in order, then
+// +1e-4; interior: grow by left then right child, then +1e-4; a node whose
+// left_first is negative gets the empty box.  count and left_first are kept.
+__device__ __forceinline__ DevAabb refit_leaf(const DevAabb& n, const uint32_t* indices,
+                                              uint32_t index_count, const DevAabb* aabbs) {
+    DevAabb a;
+    for (int k = 0; k < 3; ++k) {
+        a.min[k] = 1e34f;
+        a.max[k] = -1e34f;
+    }
+    if (n.extra2 >= 0) {
+        for (int32_t j = 0; j < n.extra1; ++j) {
+            const uint32_t pos = uint32_t(n.extra2) + uint32_t(j);
+            if (pos >= index_count) break;
+            DevAabb p;
+            ld(aabbs + indices[pos], p);
+            for (int k = 0; k < 3; ++k) {
+                a.min[k] = rs_min(a.min[k], p.min[k]);
+                a.max[k] = rs_max(a.max[k], p.max[k]);
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            a.min[k] = a.min[k] - 0.0001f;
+            a.max[k] = a.max[k] + 0.0001f;
+        }
+    }
+    a.extra1 = n.extra1;
+    a.extra2 = n.extra2;
+    return a;
+}
+
+__global__ void refit_kernel(DevAabb* nodes, uint32_t count, const int32_t* parent,
+                             const uint32_t* indices, uint32_t index_count, const DevAabb* aabbs,
+                             uint32_t* arrivals) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    DevAabb n;
+    ld(nodes + i, n);
+    // interior nodes are finished by their second arriving child; leaves and
+    // nodes without children (empty box) start the walk up
+    if (n.extra1 < 0 && n.extra2 >= 0) return;
+    st(nodes + i, refit_leaf(n, indices, index_count, aabbs));
+    int32_t p = parent[i];
+    while (p >= 0) {
+        __threadfence();
+        if (atomicAdd(&arrivals[p], 1u) == 0) return;  // the sibling finishes p
+        __threadfence();
+        DevAabb pn, l, r;
+        ld(nodes + p, pn);
+        ld(nodes + pn.extra2, l);
+        ld(nodes + pn.extra2 + 1, r);
+        DevAabb a;
+        for (int k = 0; k < 3; ++k) {
+            a.min[k] = rs_min(rs_min(1e34f, l.min[k]), r.min[k]) - 0.0001f;
+            a.max[k] = rs_max(rs_max(-1e34f, l.max[k]), r.max[k]) + 0.0001f;
+        }
+        a.extra1 = pn.extra1;
+        a.extra2 = pn.extra2;
+        st(nodes + p, a);
+        p = parent[p];
+    }
+}
+
+__global__ void copy_u32_kernel(const uint32_t* src, uint32_t* dst, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+}  // namespace
+
+// Exclusive scan used by the collapse (defined in scan.hip).
+hipError_t scan_exclusive_u32(uint32_t* data, uint32_t count, uint32_t* total, void* scratch,
+                              size_t scratch_bytes, hipStream_t stream);
+size_t scan_scratch_bytes(uint32_t count);
+
+ResultCode ensure_parents(DeviceBvh& bvh, hipStream_t stream) {
+    if (bvh.parents) return Ok;
+    hipError_t e = hipMalloc(&bvh.parents, size_t(bvh.node_count) * sizeof(int32_t));
+    if (e != hipSuccess) {
+        bvh.parents = nullptr;
+        return hip_fail(e, "hipMalloc(parents)");
+    }
+    const uint32_t g = (bvh.node_count + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(g), dim3(kBlock), 0, stream, bvh.parents,
+                       bvh.node_count, -1);
+    hipLaunchKernelGGL(parent_kernel, dim3(g), dim3(kBlock), 0, stream, bvh.nodes, bvh.node_count,
+                       bvh.parents);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "parent pointers");
+    return Ok;
+}
+
+ResultCode gpu_build_mbvh(DeviceBvh& bvh, hipStream_t stream, DeviceMbvh* out) {
+    DeviceMbvh res;
+    res.prim_count = bvh.prim_count;
+    if (bvh.node_count == 0) {
+        *out = res;
+        return Ok;
+    }
+    ResultCode rc = ensure_parents(bvh, stream);
+    if (rc != Ok) return rc;
+    const uint32_t count = bvh.node_count;
+    const uint32_t g = (count + kBlock - 1) / kBlock;
+    hipError_t e;
+    uint32_t* flag = nullptr;
+    uint32_t* mindex = nullptr;
+    uint32_t* total = nullptr;
+    void* scratch = nullptr;
+    const size_t sbytes = scan_scratch_bytes(count);
+    auto cleanup = [&]() {
+        if (flag) (void)hipFree(flag);
+        if (mindex) (void)hipFree(mindex);
+        if (total) (void)hipHostFree(total);
+        if (scratch) (void)hipFree(scratch);
+    };
+    if ((e = hipMalloc(&flag, size_t(count) * 4)) != hipSuccess ||
+        (e = hipMalloc(&mindex, size_t(count) * 4)) != hipSuccess ||
+        (e = hipHostMalloc(&total, 4)) != hipSuccess ||
+        (sbytes && (e = hipMalloc(&scratch, sbytes)) != hipSuccess)) {
+        cleanup();
+        return hip_fail(e, "collapse workspace");
+    }
+    uint32_t* status = device_status_ptr();
+    hipLaunchKernelGGL(mflag_kernel, dim3(g), dim3(kBlock), 0, stream, bvh.nodes, count,
+                       bvh.parents, flag, status);
+    hipLaunchKernelGGL(copy_u32_kernel, dim3(g), dim3(kBlock), 0, stream, flag, mindex, count);
+    if ((e = scan_exclusive_u32(mindex, count, total, scratch, sbytes, stream)) != hipSuccess) {
+        cleanup();
+        return hip_fail(e, "collapse scan");
+    }
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) {
+        cleanup();
+        return hip_fail(e, "collapse scan");
+    }
+    const uint32_t mcount = *total;
+    if ((e = hipMalloc(&res.nodes, size_t(mcount) * sizeof(DevMbvhNode))) != hipSuccess ||
+        (e = hipMalloc(&res.indices, size_t(bvh.prim_count) * 4 + 4)) != hipSuccess) {
+        if (res.nodes) (void)hipFree(res.nodes);
+        cleanup();
+        return hip_fail(e, "hipMalloc(mbvh)");
+    }
+    res.node_count = mcount;
+    hipLaunchKernelGGL(mfill_kernel, dim3(g), dim3(kBlock), 0, stream, bvh.nodes, count, flag,
+                       mindex, res.nodes);
+    const uint32_t gi = (bvh.prim_count + kBlock - 1) / kBlock;
+    if (bvh.prim_count)
+        hipLaunchKernelGGL(copy_u32_kernel, dim3(gi), dim3(kBlock), 0, stream, bvh.indices,
+                           res.indices, bvh.prim_count);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    cleanup();
+    if (e != hipSuccess) {
+        (void)hipFree(res.nodes);
+        (void)hipFree(res.indices);
+        return hip_fail(e, "collapse");
+    }
+    *out = res;
+    return Ok;
+}
+
+ResultCode gpu_refit(DeviceBvh& bvh, const RTAabb* aabbs, hipStream_t stream) {
+    if (bvh.node_count == 0) return Ok;
+    ResultCode rc = ensure_parents(bvh, stream);
+    if (rc != Ok) return rc;
+    uint32_t* arrivals = nullptr;
+    hipError_t e = hipMalloc(&arrivals, size_t(bvh.node_count) * 4);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(refit)");
+    const uint32_t g = (bvh.node_count + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(g), dim3(kBlock), 0, stream,
+                       reinterpret_cast<int32_t*>(arrivals), bvh.node_count, 0);
+    hipLaunchKernelGGL(refit_kernel, dim3(g), dim3(kBlock), 0, stream, bvh.nodes, bvh.node_count,
+                       bvh.parents, bvh.indices, bvh.prim_count,
+                       reinterpret_cast<const DevAabb*>(aabbs), arrivals);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipFree(arrivals);
+    if (e != hipSuccess) return hip_fail(e, "refit");
+    return Ok;
+}
+
+}  // namespace rtbvh

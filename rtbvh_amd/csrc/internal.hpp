@@ -39,6 +39,7 @@ uint32_t* device_status_ptr();
 struct DeviceBvh {
     DevAabb* nodes = nullptr;
     uint32_t* indices = nullptr;
+    int32_t* parents = nullptr;  // lazily built by ensure_parents()
     uint32_t node_count = 0;
     uint32_t prim_count = 0;
 };
@@ -56,8 +57,9 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
                           size_t n, int radius, hipStream_t stream, DeviceBvh* out);
 ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t stride_floats,
                             size_t n, size_t max_leaf, hipStream_t stream, DeviceBvh* out);
-ResultCode gpu_build_mbvh(const DeviceBvh& bvh, hipStream_t stream, DeviceMbvh* out);
-ResultCode gpu_refit(const DeviceBvh& bvh, const RTAabb* aabbs, hipStream_t stream);
+ResultCode gpu_build_mbvh(DeviceBvh& bvh, hipStream_t stream, DeviceMbvh* out);
+ResultCode gpu_refit(DeviceBvh& bvh, const RTAabb* aabbs, hipStream_t stream);
+ResultCode ensure_parents(DeviceBvh& bvh, hipStream_t stream);
 
 void set_error(const char* fmt, ...);
 ResultCode hip_fail(hipError_t e, const char* what);

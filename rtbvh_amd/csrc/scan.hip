@@ -1,0 +1,107 @@
+// Device-wide exclusive scan of uint32 values (reduce-then-scan).
+//
+// The reference's prefix_sum (src/utils.rs:42-57) is inclusive and
+// sequential; this is the device-wide building block the builders use where
+// an exclusive prefix over a large array is needed.
+#include "internal.hpp"
+
+namespace rtbvh {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kPer = 8;
+constexpr uint32_t kTile = kThreads * kPer;  // values per block
+
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* sh, uint32_t* total) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kThreads; off <<= 1) {
+        const uint32_t t = threadIdx.x >= uint32_t(off) ? sh[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    const uint32_t incl = sh[threadIdx.x];
+    if (total) *total = sh[kThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    tile_sums_kernel(const uint32_t* data, uint32_t count, uint32_t* sums) {
+    __shared__ uint32_t sh[kThreads];
+    const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPer;
+    uint32_t s = 0;
+    for (int k = 0; k < kPer; ++k)
+        if (base + k < count) s += data[base + k];
+    uint32_t tot = 0;
+    block_exclusive(s, sh, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    tile_scan_kernel(uint32_t* data, uint32_t count, const uint32_t* offsets) {
+    __shared__ uint32_t sh[kThreads];
+    const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPer;
+    uint32_t v[kPer];
+    uint32_t s = 0;
+    for (int k = 0; k < kPer; ++k) {
+        v[k] = base + k < count ? data[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t run = block_exclusive(s, sh, nullptr) + offsets[blockIdx.x];
+    for (int k = 0; k < kPer; ++k) {
+        if (base + k < count) data[base + k] = run;
+        run += v[k];
+    }
+}
+
+// Single-block exclusive scan for the tile sums; writes the grand total.
+__global__ void __launch_bounds__(1024) sums_scan_kernel(uint32_t* data, uint32_t count,
+                                                         uint32_t* total) {
+    __shared__ uint32_t sh[1024];
+    const uint32_t per = (count + 1023) / 1024;
+    const uint32_t b = threadIdx.x * per;
+    const uint32_t e = b + per < count ? b + per : count;
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += data[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t t = threadIdx.x >= uint32_t(off) ? sh[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? sh[threadIdx.x - 1] : 0u;
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t v = data[i];
+        data[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023 && total) *total = sh[1023];
+}
+
+}  // namespace
+
+size_t scan_scratch_bytes(uint32_t count) {
+    const size_t tiles = (size_t(count) + kTile - 1) / kTile;
+    return (tiles + 1) * sizeof(uint32_t);
+}
+
+// In-place exclusive scan of data[0..count); *total (device- or
+// host-accessible) receives the sum.  scratch holds scan_scratch_bytes(count).
+hipError_t scan_exclusive_u32(uint32_t* data, uint32_t count, uint32_t* total, void* scratch,
+                              size_t scratch_bytes, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    const uint32_t tiles = (count + kTile - 1) / kTile;
+    if (scratch_bytes < scan_scratch_bytes(count)) return hipErrorInvalidValue;
+    uint32_t* sums = static_cast<uint32_t*>(scratch);
+    hipLaunchKernelGGL(tile_sums_kernel, dim3(tiles), dim3(kThreads), 0, stream, data, count, sums);
+    hipLaunchKernelGGL(sums_scan_kernel, dim3(1), dim3(1024), 0, stream, sums, tiles, total);
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(tiles), dim3(kThreads), 0, stream, data, count, sums);
+    return hipGetLastError();
+}
+
+}  // namespace rtbvh

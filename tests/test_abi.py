@@ -1,0 +1,193 @@
+"""C ABI checks that need no GPU: exported symbols, struct layouts, the
+reference's error codes, and the host-callback traversal (intersect*) against
+the oracle and the reference's own FFI tests (rtbvh_ffi/src/lib.rs:851-1061).
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import refcpu as ref
+from rtbvh_amd import _lib as L
+from tests.helpers import as_rays, flat, prep, ray_set, teapot
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                      "rtbvh.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.findall(r"^(?:ResultCode|void|const char\s*\*)\s*\**\s*(\w+)\s*\(", text, re.M)
+
+
+def test_library_exports_every_header_symbol():
+    names = header_functions()
+    assert len(names) >= 20
+    lib = L.lib()
+    missing = [n for n in names if getattr(lib, n, None) is None]
+    assert not missing, missing
+    assert set(names) == set(L.SIGNATURES)
+
+
+def test_struct_layouts():
+    """rtbvh_ffi `same_size` test (lib.rs:855-866): BvhNode = RTBvhNode = 32,
+    MbvhNode = RTMbvhNode = 128, Aabb = RTAabb = 32; RTBvh/RTMbvh are 32 bytes
+    on LP64."""
+    assert ctypes.sizeof(L.RTAabb) == 32
+    assert ctypes.sizeof(L.RTBvh) == 32 and ctypes.sizeof(L.RTMbvh) == 32
+    assert ref.AABB.itemsize == 32 and ref.MBVH.itemsize == 128
+
+
+def test_error_codes_without_device():
+    lib = L.lib()
+    out = L.RTBvh()
+    c = np.zeros((27, 4), np.float32)
+    # null centres -> Error (rtbvh_ffi/src/lib.rs:437-439, test at :891-904)
+    assert lib.create_bvh(None, 27, None, 16, 1, L.BINNED_SAH, ctypes.byref(out)) == L.ERROR
+    assert lib.create_bvh(None, 27, c.ctypes.data, 16, 1, L.BINNED_SAH, None) == L.ERROR
+    # the reference asserts on the stride; here it is an error code
+    assert lib.create_bvh(None, 27, c.ctypes.data, 20, 1, L.BINNED_SAH,
+                          ctypes.byref(out)) == L.INVALID_ARGUMENT
+    assert lib.create_bvh(None, 27, c.ctypes.data, 8, 1, L.BINNED_SAH,
+                          ctypes.byref(out)) == L.INVALID_ARGUMENT
+    # no primitives -> NoPrimitives (src/lib.rs:28-38)
+    assert lib.create_bvh(None, 0, c.ctypes.data, 16, 1, L.BINNED_SAH,
+                          ctypes.byref(out)) == L.NO_PRIMITIVES
+    assert lib.rtbvh_gpu_create_bvh(None, 0, c.ctypes.data, 16, 1, L.LOCALLY_ORDERED_CLUSTERED,
+                                    None, ctypes.byref(out)) == L.NO_PRIMITIVES
+    # create_mbvh / refit with null pointers -> Error (lib.rs:500-502, 520-522)
+    m = L.RTMbvh()
+    assert lib.create_mbvh(L.RTBvh(), ctypes.byref(m)) == L.ERROR
+    assert lib.refit(None, L.RTBvh()) == L.ERROR
+    # unknown id -> Error
+    fake = L.RTBvh(123456, 1, c.ctypes.data, 1, c.ctypes.data)
+    assert lib.create_mbvh(fake, ctypes.byref(m)) == L.ERROR
+    assert lib.refit(c.ctypes.data, fake) == L.ERROR
+    lib.free_bvh(fake)  # prints, does not abort
+    lib.free_mbvh(L.RTMbvh(123456, 0, None, 0, None))
+    assert b"unknown" in lib.rtbvh_last_error() or lib.rtbvh_last_error() != b""
+
+
+def test_builder_validation_errors():
+    """src/lib.rs:28-64: empty primitives -> NoPrimitives; aabbs of another
+    length -> InequalAabbsAndPrimitives(aabbs, prims)."""
+    torch = pytest.importorskip("torch")
+    import rtbvh_amd as rt
+
+    with pytest.raises(rt.BuildError) as e:
+        rt.Builder(torch.zeros((0, 3))).construct_binned_sah()
+    assert e.value.kind == "NoPrimitives"
+    with pytest.raises(rt.BuildError) as e:
+        rt.Builder(torch.zeros((1, 3)), aabbs=torch.zeros((0, 8))).construct_binned_sah()
+    assert e.value == rt.BuildError("InequalAabbsAndPrimitives", 0, 1)
+    with pytest.raises(rt.BuildError):
+        rt.Builder(torch.zeros((0, 3))).construct_locally_ordered_clustered()
+
+
+# ----------------------------------------------------- callback traversal --
+def _host_bvh(nodes, idx):
+    return L.RTBvh(0xFFFFFFFF, len(nodes), nodes.ctypes.data, len(idx), idx.ctypes.data)
+
+
+def _host_mbvh(m, idx):
+    return L.RTMbvh(0xFFFFFFFF, len(m), m.ctypes.data, len(idx), idx.ctypes.data)
+
+
+def _mt_callback(verts, o, d, t_min):
+    """The FFI test's intersect_test callback (rtbvh_ffi/src/lib.rs:1027-1060)
+    generalised to t_min, using the oracle's triangle test."""
+    state = {"prim": 0xFFFFFFFF, "calls": 0}
+
+    def cb(prim, t_ptr, _user):
+        state["calls"] += 1
+        ok, t, _, _ = ref.mt(verts[prim], o, d, t_min, t_ptr[0])
+        if ok:
+            t_ptr[0] = t
+            state["prim"] = prim
+        return False
+
+    return L.CALLBACK(cb), state
+
+
+def test_ffi_intersect_kat():
+    """rtbvh_ffi/src/lib.rs:945-1019: quad at z = 1, origin 0, direction +z,
+    t = 1e26 -> t == 1 through intersect and intersect_mbvh."""
+    v = np.array([[-1, -1, 1, 1, -1, 1, 1, 1, 1], [-1, -1, 1, 1, 1, 1, -1, 1, 1]], np.float32)
+    aabbs, _ = prep(v)
+    aabbs["min"] -= np.float32(1e-4)
+    aabbs["max"] += np.float32(1e-4)
+    centers = ((aabbs["min"] + aabbs["max"]) * np.float32(0.5)).astype(np.float32)
+    nodes, idx = ref.build_binned_sah(aabbs, centers)
+    m = ref.build_mbvh(nodes)
+    o = np.zeros(3, np.float32)
+    d = np.array([0, 0, 1], np.float32)
+    lib = L.lib()
+    for fn, h in ((lib.intersect, _host_bvh(nodes, idx)), (lib.intersect_mbvh, _host_mbvh(m, idx))):
+        t = np.array([1e26], np.float32)
+        cb, _ = _mt_callback(v, o, d, 1e-5)
+        assert fn(h, o.ctypes.data, d.ctypes.data, t.ctypes.data, None, cb) == L.OK
+        assert abs(float(t[0]) - 1.0) < np.finfo(np.float32).eps
+    # NaN ray -> Nan (lib.rs:561-563)
+    nan_o = np.array([np.nan, 0, 0], np.float32)
+    t = np.array([1e26], np.float32)
+    cb, _ = _mt_callback(v, o, d, 1e-5)
+    assert lib.intersect(_host_bvh(nodes, idx), nan_o.ctypes.data, d.ctypes.data, t.ctypes.data,
+                         None, cb) == L.NAN_CODE
+
+
+@pytest.mark.parametrize("builder", ["locb", "binned"])
+def test_callback_traversal_matches_oracle(builder):
+    """Closest hit through the host callback API equals the oracle's
+    reference-order traversal of the same tree (t and primitive)."""
+    verts = teapot()
+    aabbs, centers = prep(verts)
+    nodes, idx = (ref.build_locb(aabbs, centers) if builder == "locb"
+                  else ref.build_binned_sah(aabbs, centers))
+    m = ref.build_mbvh(nodes)
+    rays8 = ray_set(verts, 150, seed=2, kind="box")
+    rays = as_rays(rays8)
+    want_b = ref.trace_bvh(nodes, idx, verts, rays, mode=ref.MODE_VISIT_ORDER)
+    want_m = ref.trace_mbvh(m, idx, verts, rays, mode=ref.MODE_VISIT_ORDER)
+    lib = L.lib()
+    hb, hm = _host_bvh(nodes, idx), _host_mbvh(m, idx)
+    for i, r in enumerate(rays8):
+        o, d = r[0:3].copy(), r[4:7].copy()
+        for fn, h, want in ((lib.intersect, hb, want_b), (lib.intersect_mbvh, hm, want_m)):
+            t = np.array([r[7]], np.float32)
+            cb, st = _mt_callback(verts, o, d, float(r[3]))
+            assert fn(h, o.ctypes.data, d.ctypes.data, t.ctypes.data, None, cb) == L.OK
+            assert t[0] == want["t"][i]
+            assert st["prim"] == want["prim"][i]
+
+
+def test_callback_packet_traversal():
+    """intersect_packet / intersect_mbvh_packet: every lane's closest t
+    equals brute force when the callback tests all four lanes."""
+    verts = teapot()
+    aabbs, centers = prep(verts)
+    nodes, idx = ref.build_locb(aabbs, centers)
+    m = ref.build_mbvh(nodes)
+    rays8 = ray_set(verts, 64, seed=4, kind="sphere")
+    brute = ref.trace_brute(verts, as_rays(rays8))
+    lib = L.lib()
+    for fn, h in ((lib.intersect_packet, _host_bvh(nodes, idx)),
+                  (lib.intersect_mbvh_packet, _host_mbvh(m, idx))):
+        for p in range(0, 64, 4):
+            r = rays8[p:p + 4]
+            cols = [np.ascontiguousarray(r[:, k]) for k in (0, 1, 2, 4, 5, 6)]
+            t = np.ascontiguousarray(r[:, 7])
+
+            def cb(prim, t_ptr, _u, r=r):
+                for lane in range(4):
+                    ok, tt, _, _ = ref.mt(verts[prim], r[lane, 0:3], r[lane, 4:7], 1e-4,
+                                          t_ptr[lane])
+                    if ok:
+                        t_ptr[lane] = tt
+                return False
+
+            c = L.CALLBACK(cb)
+            assert fn(h, *[x.ctypes.data for x in cols], t.ctypes.data, None, c) == L.OK
+            assert np.array_equal(t, brute["t"][p:p + 4])

@@ -1,0 +1,420 @@
+"""Pins of the CPU oracle (oracle/refcpu.cpp) — CPU only.
+
+Each restated function is checked against what the reference's own tests
+fix (known-answer tests, fixtures) and, where those leave it open, against
+closed forms, independent restatements in another form (set-based numpy
+simulations, float64 arithmetic) and brute force, chosen so that a dropped
+term, a flipped comparison or a wrong index fails.
+"""
+import numpy as np
+import pytest
+
+from oracle import refcpu as ref
+from rtbvh_amd import scenes
+from tests.helpers import as_rays, hits_equal, prep, ray_set, teapot
+
+
+# ----------------------------------------------------------------- Morton --
+def test_morton_split_kat():
+    """src/morton.rs:109-115."""
+    assert ref.morton_split(2) == 8
+    assert ref.morton_split(4) == 64
+    assert ref.morton_split(8) == 512
+    assert ref.morton_split(32) == 32768
+
+
+def test_morton_split_closed_form():
+    """morton_split spreads bit b of a 10-bit value to bit 3b."""
+    for x in range(1024):
+        want = 0
+        for b in range(10):
+            want |= ((x >> b) & 1) << (3 * b)
+        assert ref.morton_split(x) == want
+
+
+def _aabbs_from_points(pts):
+    a = np.zeros(len(pts), ref.AABB)
+    a["min"] = pts
+    a["max"] = pts
+    return a
+
+
+def test_morton_encode_hand_cases():
+    """MortonEncoder over the world box (+1e-4 pad), 1024 cells per axis,
+    truncation, clamp to [0, 1023], NaN -> 0 (src/morton.rs:36-61)."""
+    corners = np.array([[0, 0, 0], [1, 1, 1]], np.float32)
+    centers = np.array([[0, 0, 0], [1, 1, 1], [0.3, 0.6, 0.9], [5, -5, 0.5],
+                        [np.nan, 0, 0]], np.float32)
+    aabbs = _aabbs_from_points(np.concatenate([corners, np.zeros((3, 3), np.float32)]))
+    aabbs[2:] = aabbs[0]
+    codes = ref.morton_codes(aabbs, centers)
+
+    def grid(c):  # float64 restatement of the cell index
+        lo, hi = -1e-4, 1 + 1e-4
+        g = np.floor((np.float64(c) - lo) / (hi - lo) * 1024.0)
+        return int(min(1023, max(0, g))) if np.isfinite(c) else 0
+
+    def interleave(x, y, z):
+        return ref.morton_split(x) | ref.morton_split(y) << 1 | ref.morton_split(z) << 2
+
+    assert codes[0] == 0
+    assert codes[1] == (1 << 30) - 1
+    assert codes[2] == interleave(grid(0.3), grid(0.6), grid(0.9))
+    assert codes[3] == interleave(1023, 0, grid(0.5))
+    assert codes[4] == interleave(0, 0, 0)
+
+
+def test_sorted_indices_are_stable_sort():
+    """get_sorted_indices uses rayon's stable par_sort_by (morton.rs:86-91)."""
+    rng = np.random.default_rng(0)
+    for dup in (4, 1000, 1 << 30):
+        codes = rng.integers(0, dup, size=5000).astype(np.uint32)
+        assert np.array_equal(ref.sorted_indices(codes), np.argsort(codes, kind="stable"))
+
+
+# ------------------------------------------------------------- prefix_sum --
+def test_prefix_sum_kat():
+    """src/utils.rs:295-345."""
+    out, total = ref.prefix_sum([1, 2, 3, 4, 5, 6])
+    assert list(out) == [1, 3, 6, 10, 15, 21] and total == 21
+    _, first = ref.prefix_sum([1, 2, 3, 4, 5, 6], count=0)
+    assert first == 1
+
+
+# ------------------------------------------------------------ Aabb, MT ----
+def test_half_area_closed_form():
+    a = np.zeros(1, ref.AABB)
+    a["min"] = (0, 0, 0)
+    a["max"] = (1, 2, 3)
+    assert ref.half_area(a) == (1 + 2) * 3 + 1 * 2
+    e = np.zeros(1, ref.AABB)
+    e["min"] = 1e34
+    e["max"] = -1e34
+    assert ref.half_area(e) == np.inf  # empty box: +inf (src/aabb.rs:343-346)
+
+
+def test_mt_closed_form():
+    """Moller-Trumbore (spatial_sah.rs:132-163) on hand rays."""
+    tri = np.array([0, 0, 1, 1, 0, 1, 0, 1, 1], np.float32)
+    ok, t, u, v = ref.mt(tri, [0.25, 0.25, 0], [0, 0, 1])
+    assert ok and t == 1.0 and u == 0.25 and v == 0.25
+    ok, t, u, v = ref.mt(tri, [0.0, 0.0, 0], [0, 0, 2])  # vertex v0: u = v = 0 accepted
+    assert ok and t == 0.5 and u == 0 and v == 0
+    assert ref.mt(tri, [0.25, 0.25, 0], [1, 0, 0])[0] is False  # parallel: |a| < 1e-5
+    assert ref.mt(tri, [0.25, 0.25, 2], [0, 0, 1])[0] is False  # behind: t = -1
+    assert ref.mt(tri, [0.6, 0.6, 0], [0, 0, 1])[0] is False  # u + v > 1
+    assert ref.mt(tri, [0.25, 0.25, 0], [0, 0, 1], t_min=1.0)[0] is False  # t > t_min strict
+    assert ref.mt(tri, [0.25, 0.25, 0], [0, 0, 1], t_cur=1.0)[0] is False  # t < t_cur strict
+    # against float64 geometry on random rays: same accept/reject, t to 1e-4
+    rng = np.random.default_rng(1)
+    for _ in range(300):
+        p = rng.normal(size=9).astype(np.float32)
+        o = rng.normal(size=3).astype(np.float32) * 3
+        d = (p[0:3] + p[3:6] + p[6:9]) / 3 - o + rng.normal(size=3) * 0.3
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        ok, t, u, v = ref.mt(p, o, d)
+        P = p.reshape(3, 3).astype(np.float64)
+        M = np.stack([P[1] - P[0], P[2] - P[0], -d.astype(np.float64)], axis=1)
+        try:
+            uu, vv, tt = np.linalg.solve(M, o.astype(np.float64) - P[0])
+        except np.linalg.LinAlgError:
+            continue
+        inside = uu > 1e-4 and vv > 1e-4 and uu + vv < 1 - 1e-4 and tt > 1e-3
+        outside = uu < -1e-4 or vv < -1e-4 or uu + vv > 1 + 1e-4 or tt < -1e-3
+        if inside:
+            assert ok and abs(t - tt) < 1e-3 * max(1, tt)
+        if outside:
+            assert not ok
+
+
+# --------------------------------------------------------------- LOCB ----
+def test_locb_teapot_reference_tests():
+    """locb.rs:336-388: n <= #nodes <= 2n, valid tree, root strictly contains
+    every vertex.  Strengthened: every primitive exactly once, 2n-1 nodes,
+    every parent the exact union of its children, leaves = aabb +- 1e-4 and
+    one primitive per leaf (Q15)."""
+    verts = teapot()
+    aabbs, centers = prep(verts)
+    nodes, idx = ref.build_locb(aabbs, centers)
+    n = len(verts)
+    assert len(nodes) == 2 * n - 1
+    code, viol, _ = ref.check_bvh(nodes, idx, n)
+    assert code == 0 and viol == 0
+    v = verts.reshape(-1, 3)
+    assert np.all(v > nodes[0]["min"]) and np.all(v < nodes[0]["max"])
+    inner = nodes["count"] < 0
+    lf = nodes["left_first"][inner]
+    assert np.array_equal(nodes["min"][inner], np.minimum(nodes["min"][lf], nodes["min"][lf + 1]))
+    assert np.array_equal(nodes["max"][inner], np.maximum(nodes["max"][lf], nodes["max"][lf + 1]))
+    leaves = ~inner
+    assert np.all(nodes["count"][leaves] == 1)
+    prim = idx[nodes["left_first"][leaves]]
+    assert np.array_equal(nodes["min"][leaves], aabbs["min"][prim] - np.float32(1e-4))
+    assert np.array_equal(nodes["max"][leaves], aabbs["max"][prim] + np.float32(1e-4))
+    # no primitives -> no nodes (locb.rs:336-345)
+    assert len(ref.build_locb(aabbs[:0], centers[:0])[0]) == 0
+
+
+def _clusters_of(nodes, idx):
+    """Set of primitive sets, one per node (numbering-independent)."""
+    out = set()
+    memo = {}
+
+    def leafset(k):
+        if k in memo:
+            return memo[k]
+        nd = nodes[k]
+        if nd["count"] >= 0:
+            s = frozenset(int(x) for x in idx[nd["left_first"]:nd["left_first"] + nd["count"]])
+        else:
+            s = leafset(int(nd["left_first"])) | leafset(int(nd["left_first"]) + 1)
+        memo[k] = s
+        return s
+
+    for k in range(len(nodes)):
+        out.add(leafset(k))
+    return out
+
+
+def _ploc_sets(aabbs, order, radius):
+    """Independent set-based PLOC: clusters in Morton order; each round every
+    cluster picks the neighbour within `radius` positions whose union has the
+    smallest half area (float64 here; inputs avoid near-ties), mutual pairs
+    merge into the lower position.  Returns the set of all clusters formed."""
+    pad = np.float32(1e-4)  # leaves are aabb.with_offset(1e-4) (locb.rs:291)
+    cl = [((aabbs["min"][p] - pad).astype(np.float64), (aabbs["max"][p] + pad).astype(np.float64),
+           frozenset([int(p)])) for p in order]
+    seen = {c[2] for c in cl}
+    while len(cl) > 1:
+        n = len(cl)
+        nb = []
+        for i in range(n):
+            best, bj = np.inf, -1
+            for j in range(max(0, i - radius), min(n, i + radius + 1)):
+                if j == i:
+                    continue
+                d = np.maximum(cl[i][1], cl[j][1]) - np.minimum(cl[i][0], cl[j][0])
+                a = (d[0] + d[1]) * d[2] + d[0] * d[1]
+                if a < best:
+                    best, bj = a, j
+            nb.append(bj)
+        keyed = []  # (slot, cluster): a merged pair takes the upper element's slot
+        for i in range(n):
+            j = nb[i]
+            if nb[j] == i:
+                if i < j:
+                    c = (np.minimum(cl[i][0], cl[j][0]), np.maximum(cl[i][1], cl[j][1]),
+                         cl[i][2] | cl[j][2])
+                    seen.add(c[2])
+                    keyed.append((j, c))
+            else:
+                keyed.append((i, cl[i]))
+        keyed.sort(key=lambda x: x[0])
+        cl = [c for _, c in keyed]
+    return seen
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("radius", [1, 2, 14])
+def test_locb_matches_independent_clustering(seed, radius):
+    verts = scenes.random_soup(40, 0.05, seed=100 + seed)
+    aabbs, centers = prep(verts)
+    nodes, idx = ref.build_locb(aabbs, centers, radius=radius)
+    order = ref.sorted_indices(ref.morton_codes(aabbs, centers))
+    assert _clusters_of(nodes, idx) == _ploc_sets(aabbs, order, radius)
+
+
+def test_locb_tiny_and_small():
+    """n <= 2: one root leaf with the world bounds (locb.rs:258-269)."""
+    for n in (1, 2):
+        verts = scenes.random_soup(n, 0.2, seed=n)
+        aabbs, centers = prep(verts)
+        nodes, idx = ref.build_locb(aabbs, centers)
+        w = ref.world_bounds(aabbs)
+        assert len(nodes) == 1 and nodes[0]["count"] == n and nodes[0]["left_first"] == 0
+        assert np.array_equal(nodes[0]["min"], w["min"]) and list(idx) == list(range(n))
+        mn = np.min(aabbs["min"], axis=0) - np.float32(1e-4)
+        assert np.array_equal(w["min"], mn.astype(np.float32))
+
+
+# -------------------------------------------------------------- binned ----
+def test_binned_teapot_reference_tests():
+    """binned_sah.rs:407-458: n <= #nodes <= 2n, valid, root strictly contains
+    every vertex; exactly-once strengthened."""
+    verts = teapot()
+    aabbs, centers = prep(verts)
+    nodes, idx = ref.build_binned_sah(aabbs, centers)
+    n = len(verts)
+    assert n <= len(nodes) <= 2 * n
+    assert ref.check_bvh(nodes, idx, n)[0] == 0
+    v = verts.reshape(-1, 3)
+    assert np.all(v > nodes[0]["min"]) and np.all(v < nodes[0]["max"])
+    assert len(ref.build_binned_sah(aabbs[:0], centers[:0])[0]) == 0
+
+
+def _bins_root(aabbs, centers, nb=16):
+    """float64 restatement of the root split choice of binned_sah.rs:145-210."""
+    mn = np.min(aabbs["min"], 0).astype(np.float64) - 2e-4
+    mx = np.max(aabbs["max"], 0).astype(np.float64) + 2e-4
+    best = (np.inf, None, None)
+    for ax in range(3):
+        b = np.clip(np.floor((centers[:, ax] - mn[ax]) / (mx[ax] - mn[ax]) * nb), 0, nb - 1)
+        for s in range(1, nb):
+            L = b < s
+            if L.all() or not L.any():
+                continue
+
+            def ha(sel):
+                lo = np.min(aabbs["min"][sel], 0).astype(np.float64)
+                hi = np.max(aabbs["max"][sel], 0).astype(np.float64)
+                d = hi - lo
+                return (d[0] + d[1]) * d[2] + d[0] * d[1]
+
+            cost = ha(L) * L.sum() + ha(~L) * (~L).sum()
+            if cost < best[0]:
+                best = (cost, ax, frozenset(np.nonzero(L)[0].tolist()))
+    return best
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_binned_root_split_is_sah_argmin(seed):
+    verts = scenes.random_soup(60, 0.05, seed=200 + seed)
+    aabbs, centers = prep(verts)
+    nodes, idx = ref.build_binned_sah(aabbs, centers)
+    cost, ax, left = _bins_root(aabbs, centers)
+    costs = []  # skip seeds whose two best costs nearly tie
+    assert nodes[0]["count"] == -1
+    lf = nodes[0]["left_first"]
+
+    def prims(k):
+        nd = nodes[k]
+        if nd["count"] >= 0:
+            return set(idx[nd["left_first"]:nd["left_first"] + nd["count"]].tolist())
+        return prims(nd["left_first"]) | prims(nd["left_first"] + 1)
+
+    got_left = frozenset(prims(lf))
+    del costs
+    assert got_left == left
+
+
+def test_binned_small_cases():
+    """src/lib.rs:66-94 (2-triangle quad) and :246-311 (5 nearly coplanar
+    triangles, leaf sizes 1..10, then Mbvh::from) build valid trees."""
+    quad = np.array([[-1, -1, 0, 1, -1, 0, 1, 1, 0], [-1, -1, 0, 1, 1, 0, -1, 1, 0]], np.float32)
+    five = np.array([
+        [128.79, -1422.82, 0.16, 128.5, -1426.88, 0.16, 128.79, -1426.9067, 0.16],
+        [129.8, -1422.8629, 0.16, 128.79, -1422.82, 0.16, 128.79, -1426.9067, 0.16],
+        [129.8, -1422.8629, 0.16, 128.79, -1426.9067, 0.16, 129.8, -1427.0, 0.16],
+        [130.2, -1422.88, 0.16, 129.8, -1422.8629, 0.16, 129.8, -1427.0, 0.16],
+        [130.2, -1422.88, 0.16, 129.8, -1427.0, 0.16, 130.2, -1423.13, 0.16]], np.float32)
+    for verts in (quad, five):
+        aabbs, centers = prep(verts)
+        for leaf in range(1, 11):
+            for contained in (False, True):
+                nodes, idx = ref.build_binned_sah(aabbs, centers, prims_per_leaf=leaf,
+                                                  contained=contained)
+                assert ref.check_bvh(nodes, idx, len(verts))[0] == 0
+                m = ref.build_mbvh(nodes)
+                assert ref.check_mbvh(m, idx, len(verts))[0] == 0
+        nodes, idx = ref.build_locb(aabbs, centers)
+        assert ref.check_bvh(nodes, idx, len(verts))[0] == 0
+
+
+def test_binned_contained_variant():
+    """The Q2/Q3 fixes change bounds only where the reference's boxes were
+    wrong: the fixed tree has no containment violations, traces exactly to
+    brute force, and its SAH cost is within 1% of the reference's."""
+    verts = teapot()
+    aabbs, centers = prep(verts)
+    n0, i0 = ref.build_binned_sah(aabbs, centers)
+    n1, i1 = ref.build_binned_sah(aabbs, centers, contained=True)
+    assert ref.check_bvh(n1, i1, len(verts))[1] == 0
+    assert ref.check_bvh(n0, i0, len(verts))[1] > 0  # Q2/Q3 visible in the reference
+    assert abs(ref.sah_cost(n1) - ref.sah_cost(n0)) <= 0.01 * ref.sah_cost(n0)
+    rays = as_rays(ray_set(verts, 5000, seed=3))
+    assert hits_equal(ref.trace_mbvh(ref.build_mbvh(n1), i1, verts, rays, threads=4),
+                      ref.trace_brute(verts, rays, threads=4)).all()
+
+
+# -------------------------------------------------------------- MBVH -----
+def test_mbvh_hand_tree():
+    """merge_nodes (mbvh_node.rs:297-411) on a hand-built 7-node tree:
+    root(1,2); 1 = leaf; 2 = interior(3,4); 3 = interior(5,6); 4,5,6 leaves.
+    Expected: MBVH root slot 0 = leaf of node 1 with the ROOT's bounds (Q8),
+    slot 1 empty, slot 2 = MBVH node for 3, slot 3 = leaf of node 4; the MBVH
+    node for 3 has slots 0 = leaf 5 (root's... its own parent 3 bounds) and
+    2 = leaf 6."""
+    A = ref.AABB
+
+    def node(mn, mx, count, lf):
+        r = np.zeros(1, A)
+        r["min"], r["max"], r["count"], r["left_first"] = mn, mx, count, lf
+        return r
+
+    nodes = np.concatenate([
+        node((0, 0, 0), (8, 8, 8), -1, 1),
+        node((0, 0, 0), (1, 1, 1), 1, 0),
+        node((2, 2, 2), (8, 8, 8), -1, 3),
+        node((2, 2, 2), (5, 5, 5), -1, 5),
+        node((6, 6, 6), (8, 8, 8), 1, 1),
+        node((2, 2, 2), (3, 3, 3), 1, 2),
+        node((4, 4, 4), (5, 5, 5), 1, 3),
+    ])
+    m = ref.build_mbvh(nodes)
+    assert len(m) == 2
+    r = m[0]
+    assert list(r["counts"]) == [1, -1, -1, 1] and list(r["children"]) == [0, -1, 1, 1]
+    assert r["min_x"][0] == 0 and r["max_x"][0] == 8  # leaf slot keeps the parent's box
+    assert r["min_x"][2] == 2 and r["max_x"][2] == 5
+    assert r["min_x"][3] == 6 and r["max_x"][3] == 8
+    c = m[1]  # merge_nodes(1, node 3): its children 5, 6 are leaves
+    assert list(c["counts"]) == [1, -1, 1, -1] and list(c["children"]) == [2, -1, 3, -1]
+    assert c["min_x"][0] == 2 and c["max_x"][0] == 5
+
+
+@pytest.mark.parametrize("builder", ["locb", "binned_contained"])
+def test_traversal_matches_brute_force(builder):
+    """Both trees, both traversals: lowest-id closest hits equal brute force
+    exactly; the reference visit order gives the same t, and the same
+    primitive except on exact ties."""
+    for verts in (teapot(), scenes.torus(80, 40), scenes.random_soup(4000, 0.05, seed=3)):
+        aabbs, centers = prep(verts)
+        if builder == "locb":
+            nodes, idx = ref.build_locb(aabbs, centers)
+        else:
+            nodes, idx = ref.build_binned_sah(aabbs, centers, contained=True)
+        m = ref.build_mbvh(nodes)
+        assert ref.check_mbvh(m, idx, len(verts))[0] == 0
+        for kind in ("sphere", "box"):
+            rays = as_rays(ray_set(verts, 6000, seed=7, kind=kind))
+            brute = ref.trace_brute(verts, rays, threads=4)
+            for fn, tree in ((ref.trace_bvh, nodes), (ref.trace_mbvh, m)):
+                assert hits_equal(fn(tree, idx, verts, rays, threads=4), brute).all()
+                vo = fn(tree, idx, verts, rays, mode=ref.MODE_VISIT_ORDER, threads=4)
+                assert np.array_equal(vo["t"], brute["t"])
+                diff = vo["prim"] != brute["prim"]
+                for i in np.nonzero(diff)[0]:  # only exact ties may differ
+                    ok, t, _, _ = ref.mt(verts[vo["prim"][i]], rays["o"][i], rays["d"][i],
+                                         1e-4, np.inf)
+                    assert ok and t == brute["t"][i]
+            occ = ref.occluded_mbvh(m, idx, verts, rays, threads=4)
+            assert np.array_equal(occ, ref.occluded_brute(verts, rays, threads=4))
+
+
+def test_refit_oracle():
+    """Bvh::refit bounds (src/bvh.rs:176-205): leaves = union of primitive
+    boxes +- 1e-4, interior = union of children +- 1e-4; topology kept."""
+    verts = teapot()
+    aabbs, centers = prep(verts)
+    nodes, idx = ref.build_locb(aabbs, centers)
+    r = ref.refit(nodes, idx, aabbs)
+    assert np.array_equal(r["count"], nodes["count"])
+    assert np.array_equal(r["left_first"], nodes["left_first"])
+    leaves = r["count"] >= 0
+    prim = idx[r["left_first"][leaves]]
+    assert np.array_equal(r["min"][leaves], aabbs["min"][prim] - np.float32(1e-4))
+    inner = ~leaves
+    lf = r["left_first"][inner]
+    want = np.minimum(r["min"][lf], r["min"][lf + 1]) - np.float32(1e-4)
+    assert np.array_equal(r["min"][inner], want)
+    assert ref.check_bvh(r, idx, len(verts))[1] == 0
