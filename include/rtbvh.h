@@ -224,6 +224,13 @@ ResultCode rtbvh_gpu_intersect_mbvh(RTMbvh mbvh, const float *vertices, size_t p
                                     const RTRay *rays, size_t ray_count, RTHit *hits,
                                     uint32_t flags, void *stream);
 
+/* rtbvh_gpu_intersect_mbvh that also adds {MBVH nodes visited, triangles
+ * tested} over all rays into counters[0..1] (device, uint64; not cleared) —
+ * the per-ray work behind the traversal's memory traffic. */
+ResultCode rtbvh_gpu_intersect_mbvh_stats(RTMbvh mbvh, const float *vertices, size_t prim_count,
+                                          const RTRay *rays, size_t ray_count, RTHit *hits,
+                                          uint32_t flags, uint64_t *counters, void *stream);
+
 /* Any-hit (shadow) traversal: occluded[i] = 1 when some triangle is hit with
  * t_min < t < t_max, else 0 (the FFI's early-out callback, rtbvh_ffi/src/
  * lib.rs:571-576). */

@@ -33,6 +33,10 @@ inline float rs_max(float a, float b) { return (a != a) ? b : (b > a ? b : a); }
 inline float ps_min(float a, float b) { return a < b ? a : b; }
 inline float ps_max(float a, float b) { return a > b ? a : b; }
 
+// IEEE minNum/maxNum (a NaN operand is ignored), inlined.
+inline float ieee_min(float a, float b) { return (b != b) ? a : ((a != a) ? b : (a < b ? a : b)); }
+inline float ieee_max(float a, float b) { return (b != b) ? a : ((a != a) ? b : (a > b ? a : b)); }
+
 inline bool is_nan3(const float v[3]) { return v[0] != v[0] || v[1] != v[1] || v[2] != v[2]; }
 
 // Rust `f as i32` (saturating, NaN -> 0).
@@ -314,11 +318,6 @@ inline MbvhHit mbvh_intersect(const RefMbvhNode& n, const Ray& ray, float t_cur)
 // closest-hit driver of examples/benchmark.rs:33-37.  The node's hit mask is
 // taken when the node is popped; its leaf slots are then yielded in the order
 // ids[3], ids[2], ids[1], ids[0] and interior slots pushed in that order.
-//
-// Lowest-id mode changes three things, none of which changes the set of
-// triangles that can win: boxes are accepted when tmin <= t (ties can still
-// lower the id), boxes entirely before t_min are culled, and leaf slots are
-// tested near-first and re-checked against the current t.
 void trace_mbvh_one(const RefMbvhNode* m, size_t m_count, const uint32_t* indices,
                     size_t index_count, const float* verts, size_t prim_count, const RefRay& in,
                     RefHit& out, int mode, uint64_t& nodes_visited, uint64_t& tri_tests,
@@ -334,55 +333,112 @@ void trace_mbvh_one(const RefMbvhNode* m, size_t m_count, const uint32_t* indice
     for (;;) {
         const RefMbvhNode& node = m[current];
         ++nodes_visited;
-        if (mode == REF_MODE_VISIT_ORDER) {
-            MbvhHit hit = mbvh_intersect(node, ray, h.t);
-            for (int i = 0; i < 4; ++i) {
-                const int id = hit.ids[3 - i];
-                if (!hit.result[id]) continue;
-                const int32_t count = node.counts[id];
-                const int32_t lf = node.children[id];
-                if (count > -1) {
-                    for (int32_t j = 0; j < count; ++j) {
-                        const size_t k = size_t(lf) + size_t(j);
-                        if (k >= index_count) break;
-                        try_prim(verts, prim_count, indices[k], ray, h, mode, tri_tests);
-                    }
-                } else if (lf > -1) {
-                    stack.push_back(lf);
-                }
-            }
-        } else {
-            MbvhHit hit = mbvh_intersect(node, ray, std::numeric_limits<float>::infinity());
-            bool acc[4];
-            for (int i = 0; i < 4; ++i)
-                acc[i] = (hit.tmax[i] >= hit.tmin[i]) && (hit.tmin[i] <= h.t) &&
-                         (hit.tmax[i] >= ray.t_min);
-            // leaves near-first, re-checked against the current t
-            for (int i = 0; i < 4; ++i) {
-                const int id = hit.ids[i];
-                if (!acc[id] || node.counts[id] < 0) continue;
-                if (!(hit.tmin[id] <= h.t)) continue;
-                const int32_t count = node.counts[id];
-                const int32_t lf = node.children[id];
+        MbvhHit hit = mbvh_intersect(node, ray, h.t);
+        for (int i = 0; i < 4; ++i) {
+            const int id = hit.ids[3 - i];
+            if (!hit.result[id]) continue;
+            const int32_t count = node.counts[id];
+            const int32_t lf = node.children[id];
+            if (count > -1) {
                 for (int32_t j = 0; j < count; ++j) {
                     const size_t k = size_t(lf) + size_t(j);
                     if (k >= index_count) break;
                     try_prim(verts, prim_count, indices[k], ray, h, mode, tri_tests);
                 }
-            }
-            // interior far-first so the nearest is popped first
-            for (int i = 3; i >= 0; --i) {
-                const int id = hit.ids[i];
-                if (!acc[id] || node.counts[id] >= 0) continue;
-                if (node.children[id] > -1) stack.push_back(node.children[id]);
+            } else if (lf > -1) {
+                stack.push_back(lf);
             }
         }
-        // pop, skipping nodes that the current t already excludes is not done
-        // here: the reference tests the popped node's slots, not the node.
         if (stack.empty()) break;
         current = stack.back();
         stack.pop_back();
         if (size_t(current) >= m_count) break;  // malformed tree guard
+    }
+    out = RefHit{h.t, h.u, h.v, h.prim};
+}
+
+// Lowest-(t, id) traversal of the MBVH.  The result is the brute-force
+// closest hit; the visit order is the one of the GPU kernel in
+// rtbvh_amd/csrc/trace_fast.hip, so that their work counters agree:
+// IEEE min/max slabs, slot keys (entry distance bits, 2 low bits = slot)
+// sorted with comparators (0,1),(2,3),(0,2),(1,3),(1,2), leaves near to far,
+// nearest interior child next, the rest stacked far to near with their entry
+// distance and dropped at pop when beyond the current t.
+void trace_mbvh_lowest_one(const RefMbvhNode* m, size_t m_count, const uint32_t* indices,
+                           size_t index_count, const float* verts, size_t prim_count,
+                           const RefRay& in, RefHit& out, uint64_t& nodes_visited,
+                           uint64_t& tri_tests, std::vector<std::pair<int32_t, float>>& stack) {
+    Ray ray = make_ray(in);
+    HitState h{ray.t, 0.f, 0.f, 0xFFFFFFFFu};
+    if (m_count == 0 || is_nan3(ray.o) || is_nan3(ray.d)) {
+        out = RefHit{h.t, h.u, h.v, h.prim};
+        return;
+    }
+    stack.clear();
+    int32_t cur = 0;
+    for (;;) {
+        if (cur < 0) {
+            while (!stack.empty()) {
+                const auto e = stack.back();
+                stack.pop_back();
+                if (e.second <= h.t) {
+                    cur = e.first;
+                    break;
+                }
+            }
+            if (cur < 0 || size_t(cur) >= m_count) break;
+        }
+        const RefMbvhNode& n = m[cur];
+        ++nodes_visited;
+        float tn[4];
+        uint32_t key[4];
+        for (int i = 0; i < 4; ++i) {
+            const float tx0 = (n.min_x[i] - ray.o[0]) * ray.inv[0];
+            const float tx1 = (n.max_x[i] - ray.o[0]) * ray.inv[0];
+            const float ty0 = (n.min_y[i] - ray.o[1]) * ray.inv[1];
+            const float ty1 = (n.max_y[i] - ray.o[1]) * ray.inv[1];
+            const float tz0 = (n.min_z[i] - ray.o[2]) * ray.inv[2];
+            const float tz1 = (n.max_z[i] - ray.o[2]) * ray.inv[2];
+            const float a = ieee_max(ieee_max(ieee_min(tx0, tx1), ieee_min(ty0, ty1)),
+                                     ieee_min(tz0, tz1));
+            const float b = ieee_min(ieee_min(ieee_max(tx0, tx1), ieee_max(ty0, ty1)),
+                                     ieee_max(tz0, tz1));
+            tn[i] = a;
+            const bool hit = (b >= a) && (a <= h.t) && (b >= ray.t_min);
+            float ak = a > 0.0f ? a : 0.0f;
+            uint32_t bits;
+            std::memcpy(&bits, &ak, 4);
+            key[i] = hit ? ((bits & ~3u) | uint32_t(i)) : 0xFFFFFFFFu;
+        }
+        auto cs = [&](int x, int y) {
+            if (key[x] > key[y]) std::swap(key[x], key[y]);
+        };
+        cs(0, 1);
+        cs(2, 3);
+        cs(0, 2);
+        cs(1, 3);
+        cs(1, 2);
+        for (int k = 0; k < 4; ++k) {
+            if (key[k] == 0xFFFFFFFFu) continue;
+            const int s = int(key[k] & 3u);
+            if (n.counts[s] < 0 || !(tn[s] <= h.t)) continue;
+            for (int32_t j = 0; j < n.counts[s]; ++j) {
+                const size_t pos = size_t(n.children[s]) + size_t(j);
+                if (pos >= index_count) break;
+                try_prim(verts, prim_count, indices[pos], ray, h, REF_MODE_LOWEST_ID, tri_tests);
+            }
+        }
+        int32_t next = -1;
+        float next_t = 0.f;
+        for (int k = 3; k >= 0; --k) {
+            if (key[k] == 0xFFFFFFFFu) continue;
+            const int s = int(key[k] & 3u);
+            if (n.counts[s] >= 0 || n.children[s] < 0 || !(tn[s] <= h.t)) continue;
+            if (next >= 0) stack.push_back({next, next_t});
+            next = n.children[s];
+            next_t = tn[s];
+        }
+        cur = next;
     }
     out = RefHit{h.t, h.u, h.v, h.prim};
 }
@@ -1113,20 +1169,27 @@ void ref_trace_mbvh(const RefMbvhNode* m, size_t m_count, const uint32_t* indice
                     size_t index_count, const float* verts, size_t prim_count,
                     const RefRay* rays, size_t n, RefHit* hits, int mode, int threads,
                     uint64_t* stats) {
-    std::vector<uint64_t> nv(std::max(1, threads), 0), tt(std::max(1, threads), 0);
-    parallel_rays(n, threads, [&](size_t b, size_t e, int t) {
+    std::atomic<uint64_t> nv{0}, tt{0};
+    parallel_rays(n, threads, [&](size_t b, size_t e, int) {
         std::vector<int32_t> stack;
+        std::vector<std::pair<int32_t, float>> fstack;
         stack.reserve(256);
-        for (size_t i = b; i < e; ++i)
-            trace_mbvh_one(m, m_count, indices, index_count, verts, prim_count, rays[i], hits[i],
-                           mode, nv[t], tt[t], stack);
+        fstack.reserve(256);
+        uint64_t lnv = 0, ltt = 0;  // per-chunk counters: no shared cache lines
+        for (size_t i = b; i < e; ++i) {
+            if (mode == REF_MODE_VISIT_ORDER)
+                trace_mbvh_one(m, m_count, indices, index_count, verts, prim_count, rays[i],
+                               hits[i], mode, lnv, ltt, stack);
+            else
+                trace_mbvh_lowest_one(m, m_count, indices, index_count, verts, prim_count,
+                                      rays[i], hits[i], lnv, ltt, fstack);
+        }
+        nv += lnv;
+        tt += ltt;
     });
     if (stats) {
-        stats[0] = stats[1] = 0;
-        for (size_t t = 0; t < nv.size(); ++t) {
-            stats[0] += nv[t];
-            stats[1] += tt[t];
-        }
+        stats[0] = nv.load();
+        stats[1] = tt.load();
     }
 }
 
@@ -1134,20 +1197,20 @@ void ref_trace_bvh(const RefAabb* nodes, size_t node_count, const uint32_t* indi
                    size_t index_count, const float* verts, size_t prim_count,
                    const RefRay* rays, size_t n, RefHit* hits, int mode, int threads,
                    uint64_t* stats) {
-    std::vector<uint64_t> nv(std::max(1, threads), 0), tt(std::max(1, threads), 0);
-    parallel_rays(n, threads, [&](size_t b, size_t e, int t) {
+    std::atomic<uint64_t> nv{0}, tt{0};
+    parallel_rays(n, threads, [&](size_t b, size_t e, int) {
         std::vector<int32_t> stack;
         stack.reserve(256);
+        uint64_t lnv = 0, ltt = 0;
         for (size_t i = b; i < e; ++i)
             trace_bvh_one(nodes, node_count, indices, index_count, verts, prim_count, rays[i],
-                          hits[i], mode, nv[t], tt[t], stack);
+                          hits[i], mode, lnv, ltt, stack);
+        nv += lnv;
+        tt += ltt;
     });
     if (stats) {
-        stats[0] = stats[1] = 0;
-        for (size_t t = 0; t < nv.size(); ++t) {
-            stats[0] += nv[t];
-            stats[1] += tt[t];
-        }
+        stats[0] = nv.load();
+        stats[1] = tt.load();
     }
 }
 

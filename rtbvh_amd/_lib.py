@@ -65,6 +65,7 @@ SIGNATURES = {
     "rtbvh_gpu_refit": (_rc, [_vp, RTBvh, _vp]),
     "rtbvh_gpu_intersect": (_rc, [RTBvh, _vp, _sz, _vp, _sz, _vp, _u32, _vp]),
     "rtbvh_gpu_intersect_mbvh": (_rc, [RTMbvh, _vp, _sz, _vp, _sz, _vp, _u32, _vp]),
+    "rtbvh_gpu_intersect_mbvh_stats": (_rc, [RTMbvh, _vp, _sz, _vp, _sz, _vp, _u32, _vp, _vp]),
     "rtbvh_gpu_occluded": (_rc, [RTBvh, _vp, _sz, _vp, _sz, _vp, _u32, _vp]),
     "rtbvh_gpu_occluded_mbvh": (_rc, [RTMbvh, _vp, _sz, _vp, _sz, _vp, _u32, _vp]),
     "rtbvh_gpu_status": (_rc, [_vp]),

@@ -73,7 +73,7 @@ namespace {
 ResultCode trace_entry(const void* nodes, uint32_t node_count, const uint32_t* indices,
                        uint32_t index_count, const float* vertices, size_t prim_count,
                        const RTRay* rays, size_t ray_count, RTHit* hits, uint8_t* occluded,
-                       uint32_t flags, void* stream, bool mbvh) {
+                       uint32_t flags, void* stream, bool mbvh, uint64_t* stats = nullptr) {
     if (flags & ~RTBVH_TRACE_REFERENCE_ORDER) {
         set_error("unknown trace flags 0x%x", flags);
         return InvalidArgument;
@@ -87,6 +87,7 @@ ResultCode trace_entry(const void* nodes, uint32_t node_count, const uint32_t* i
     if ((rc = check_device(rays, "rays")) != Ok) return rc;
     if (hits && (rc = check_device(hits, "hits")) != Ok) return rc;
     if (occluded && (rc = check_device(occluded, "occluded")) != Ok) return rc;
+    if (stats && (rc = check_device(stats, "stats")) != Ok) return rc;
     if (node_count > 0) {
         if ((rc = check_device(nodes, "nodes")) != Ok) return rc;
         if (index_count > 0 && (rc = check_device(indices, "indices")) != Ok) return rc;
@@ -109,6 +110,7 @@ ResultCode trace_entry(const void* nodes, uint32_t node_count, const uint32_t* i
     p.ray_count = ray_count;
     p.hits = reinterpret_cast<DevHit*>(hits);
     p.occluded = occluded;
+    p.stats = stats;
     p.status = device_status_ptr();
     if (p.status == nullptr) {
         set_error("device status word unavailable");
@@ -149,6 +151,17 @@ ResultCode rtbvh_gpu_intersect_mbvh(RTMbvh mbvh, const float* vertices, size_t p
     }
     return trace_entry(mbvh.nodes, mbvh.node_count, mbvh.indices, mbvh.index_count, vertices,
                        prim_count, rays, ray_count, hits, nullptr, flags, stream, true);
+}
+
+ResultCode rtbvh_gpu_intersect_mbvh_stats(RTMbvh mbvh, const float* vertices, size_t prim_count,
+                                          const RTRay* rays, size_t ray_count, RTHit* hits,
+                                          uint32_t flags, uint64_t* counters, void* stream) {
+    if ((hits == nullptr || counters == nullptr) && ray_count > 0) {
+        set_error("hits and counters must not be null");
+        return Error;
+    }
+    return trace_entry(mbvh.nodes, mbvh.node_count, mbvh.indices, mbvh.index_count, vertices,
+                       prim_count, rays, ray_count, hits, nullptr, flags, stream, true, counters);
 }
 
 ResultCode rtbvh_gpu_occluded(RTBvh bvh, const float* vertices, size_t prim_count,

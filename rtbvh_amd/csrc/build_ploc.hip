@@ -29,6 +29,10 @@
 
 namespace rtbvh {
 
+hipError_t scan_exclusive_u32(uint32_t* data, uint32_t count, uint32_t* total, void* scratch,
+                              size_t scratch_bytes, hipStream_t stream);
+size_t scan_scratch_bytes(uint32_t count);
+
 namespace {
 
 constexpr int kRadius = 14;            // locb.rs:27
@@ -44,13 +48,16 @@ struct Bounds6 {
     float mn[3], mx[3];
 };
 
-__device__ __forceinline__ float half_area_union(const Bounds6& a, const Bounds6& b) {
-    // half_area(a.union_of(b)), src/aabb.rs:74-86, 343-346
-    const float n0 = rs_min(a.mn[0], b.mn[0]), n1 = rs_min(a.mn[1], b.mn[1]),
-                n2 = rs_min(a.mn[2], b.mn[2]);
-    const float x0 = rs_max(a.mx[0], b.mx[0]), x1 = rs_max(a.mx[1], b.mx[1]),
-                x2 = rs_max(a.mx[2], b.mx[2]);
-    const float d0 = x0 - n0, d1 = x1 - n1, d2 = x2 - n2;
+// half_area(a.union_of(b)) (src/aabb.rs:74-86, 343-346) as a clustering
+// distance.  IEEE min/max give the same values as f32::min/max here: both
+// ignore a NaN operand, and where they differ (the sign of a zero) the
+// resulting distance compares equal.  Stored bounds use rs_min/rs_max.
+__device__ __forceinline__ float dist6(float an0, float an1, float an2, float ax0, float ax1,
+                                       float ax2, float bn0, float bn1, float bn2, float bx0,
+                                       float bx1, float bx2) {
+    const float d0 = fmaxf(ax0, bx0) - fminf(an0, bn0);
+    const float d1 = fmaxf(ax1, bx1) - fminf(an1, bn1);
+    const float d2 = fmaxf(ax2, bx2) - fminf(an2, bn2);
     return (d0 + d1) * d2 + d0 * d1;
 }
 
@@ -144,25 +151,37 @@ __global__ void bounds_partial_kernel(const DevAabb* aabbs, uint32_t n, Bounds6*
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
-// World bounds, second level, then Aabb::with_offset(1e-4).  For n <= 2 the
-// reduction is sequential (the bounds become the root's bounds).
+// World bounds, second level (one block over the partials, in order), then
+// Aabb::with_offset(1e-4).
 __global__ void bounds_final_kernel(const Bounds6* partial, uint32_t parts, Bounds6* world) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ Bounds6 sh[kBlock];
     Bounds6 acc;
     for (int k = 0; k < 3; ++k) {
         acc.mn[k] = 1e34f;
         acc.mx[k] = -1e34f;
     }
-    for (uint32_t p = 0; p < parts; ++p)
+    for (uint32_t p = threadIdx.x; p < parts; p += kBlock)
         for (int k = 0; k < 3; ++k) {
             acc.mn[k] = rs_min(acc.mn[k], partial[p].mn[k]);
             acc.mx[k] = rs_max(acc.mx[k], partial[p].mx[k]);
         }
-    for (int k = 0; k < 3; ++k) {
-        acc.mn[k] = acc.mn[k] - 0.0001f;
-        acc.mx[k] = acc.mx[k] + 0.0001f;
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s)
+            for (int k = 0; k < 3; ++k) {
+                sh[threadIdx.x].mn[k] = rs_min(sh[threadIdx.x].mn[k], sh[threadIdx.x + s].mn[k]);
+                sh[threadIdx.x].mx[k] = rs_max(sh[threadIdx.x].mx[k], sh[threadIdx.x + s].mx[k]);
+            }
+        __syncthreads();
     }
-    *world = acc;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 3; ++k) {
+            acc.mn[k] = sh[0].mn[k] - 0.0001f;
+            acc.mx[k] = sh[0].mx[k] + 0.0001f;
+        }
+        *world = acc;
+    }
 }
 
 // morton_split for 10-bit inputs: bit b moves to bit 3b (the reference's
@@ -223,32 +242,6 @@ __global__ void radix_hist_kernel(const uint32_t* keys, uint32_t n, uint32_t shi
     }
     __syncthreads();
     counts[threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
-}
-
-// Exclusive scan of `count` values by one workgroup of 1024 threads.
-__global__ void __launch_bounds__(1024) scan_exclusive_kernel(uint32_t* data, uint32_t count,
-                                                              uint32_t* total) {
-    __shared__ uint32_t sums[1024];
-    const uint32_t per = (count + 1023) / 1024;
-    const uint32_t b = threadIdx.x * per;
-    const uint32_t e = b + per < count ? b + per : count;
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += data[i];
-    sums[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = threadIdx.x >= uint32_t(off) ? sums[threadIdx.x - off] : 0u;
-        __syncthreads();
-        sums[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = threadIdx.x ? sums[threadIdx.x - 1] : 0u;
-    for (uint32_t i = b; i < e; ++i) {
-        const uint32_t v = data[i];
-        data[i] = run;
-        run += v;
-    }
-    if (total && threadIdx.x == 1023) *total = sums[1023];
 }
 
 __global__ void __launch_bounds__(kSortBlock)
@@ -335,11 +328,17 @@ __global__ void small_root_kernel(const Bounds6* world, uint32_t n, DevAabb* nod
 }
 
 // ------------------------------------------------------ clustering state --
+// One state record per iteration: iteration k reads states[k] and writes
+// states[k+1], so batches of iterations are enqueued without host round
+// trips.  An iteration whose input already fits the tail kernel passes its
+// state through unchanged.
 struct PlocState {
-    uint32_t n_active;   // clusters in [begin, end)
-    uint32_t begin;      // reference array positions of the active range
+    uint32_t n_active;  // clusters in [begin, end) of the reference's node array
+    uint32_t begin;
     uint32_t end;
-    uint32_t merged;     // pairs merged by the current iteration (M)
+    uint32_t merged;    // pairs merged by this iteration (M)
+    uint32_t buf;       // cluster buffer holding the active clusters
+    uint32_t pad[3];
 };
 
 __device__ __forceinline__ void nn_window(uint32_t i, uint32_t n, uint32_t& sb, uint32_t& se) {
@@ -348,24 +347,36 @@ __device__ __forceinline__ void nn_window(uint32_t i, uint32_t n, uint32_t& sb, 
     se = (i + kRadius + 1) < n ? i + kRadius + 1 : n;
 }
 
-// nearest neighbour + mutual marks for one tile.
+// Nearest neighbour + mutual marks for one tile.  Bounds are staged in LDS
+// as six arrays so that lanes reading consecutive clusters hit consecutive
+// banks.
 __global__ void __launch_bounds__(kBlock)
-    nn_mark_kernel(const PlocState* state, const DevAabb* cl, uint32_t* nb_out,
-                   uint32_t* mark_out, uint32_t* tile_count) {
+    nn_mark_kernel(const PlocState* state, const DevAabb* cl0, const DevAabb* cl1,
+                   uint32_t* nb_out, uint32_t* mark_out, uint32_t* tile_count) {
     constexpr int kHalo = 2 * kRadius;
-    __shared__ Bounds6 sb_bounds[kTile + 2 * kHalo];
+    constexpr int kSpan = kTile + 2 * kHalo;
+    __shared__ float s_b[6][kSpan];
     __shared__ uint32_t s_nb[kTile + 2 * kRadius];
-    __shared__ uint32_t s_cnt[kBlock];
-    const uint32_t n = state->n_active;
+    __shared__ uint32_t s_total;
+    const PlocState stt = *state;
+    if (stt.n_active <= uint32_t(kTailMax)) return;
+    const uint32_t n = stt.n_active;
+    const DevAabb* cl = stt.buf ? cl1 : cl0;
     const uint32_t ts = blockIdx.x * kTile;
     if (ts >= n) return;
     const uint32_t te = ts + kTile < n ? ts + kTile : n;
     const uint32_t lb = ts > uint32_t(kHalo) ? ts - kHalo : 0u;
     const uint32_t le = te + kHalo < n ? te + kHalo : n;
+    if (threadIdx.x == 0) s_total = 0;
     for (uint32_t k = lb + threadIdx.x; k < le; k += kBlock) {
         DevAabb a;
         load_node(cl + k, a);
-        sb_bounds[k - lb] = bounds_of(a);
+        s_b[0][k - lb] = a.min[0];
+        s_b[1][k - lb] = a.min[1];
+        s_b[2][k - lb] = a.min[2];
+        s_b[3][k - lb] = a.max[0];
+        s_b[4][k - lb] = a.max[1];
+        s_b[5][k - lb] = a.max[2];
     }
     __syncthreads();
     const uint32_t nb0 = ts > uint32_t(kRadius) ? ts - kRadius : 0u;
@@ -373,13 +384,16 @@ __global__ void __launch_bounds__(kBlock)
     for (uint32_t i = nb0 + threadIdx.x; i < nb1; i += kBlock) {
         uint32_t s0, s1;
         nn_window(i, n, s0, s1);
-        float best = 3.40282347e38f;  // f32::MAX
+        const uint32_t li = i - lb;
+        const float a0 = s_b[0][li], a1 = s_b[1][li], a2 = s_b[2][li];
+        const float a3 = s_b[3][li], a4 = s_b[4][li], a5 = s_b[5][li];
+        float best = 3.40282347e38f;  // f32::MAX (locb.rs:95)
         uint32_t best_j = 0xFFFFFFFFu;
-        const Bounds6 bi = sb_bounds[i - lb];
         for (uint32_t j = s0; j < s1; ++j) {
             if (j == i) continue;
-            const Bounds6 bj = sb_bounds[j - lb];
-            const float d = j < i ? half_area_union(bj, bi) : half_area_union(bi, bj);
+            const uint32_t lj = j - lb;
+            const float d = dist6(a0, a1, a2, a3, a4, a5, s_b[0][lj], s_b[1][lj], s_b[2][lj],
+                                  s_b[3][lj], s_b[4][lj], s_b[5][lj]);
             if (d < best) {
                 best = d;
                 best_j = j;
@@ -397,21 +411,25 @@ __global__ void __launch_bounds__(kBlock)
         mark_out[i] = m;
         cnt += m;
     }
-    s_cnt[threadIdx.x] = cnt;
+    if (cnt) atomicAdd(&s_total, cnt);
     __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if (int(threadIdx.x) < s) s_cnt[threadIdx.x] += s_cnt[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) tile_count[blockIdx.x] = s_cnt[0];
+    if (threadIdx.x == 0) tile_count[blockIdx.x] = s_total;
 }
 
-// Exclusive scan of the tile counts; writes M into the current state and the
-// next iteration's state (locb.rs:178-185).
-__global__ void __launch_bounds__(1024)
-    tile_scan_kernel(PlocState* state, PlocState* next, uint32_t* tile_count) {
+// Exclusive scan of the tile counts; writes M into states[k] and the next
+// iteration's bookkeeping into states[k+1] (locb.rs:178-185).
+__global__ void __launch_bounds__(1024) tile_scan_kernel(PlocState* states, uint32_t* tile_count) {
     __shared__ uint32_t sums[1024];
-    const uint32_t n = state->n_active;
+    const PlocState stt = states[0];
+    if (stt.n_active <= uint32_t(kTailMax)) {
+        if (threadIdx.x == 0) {
+            PlocState nx = stt;
+            nx.merged = 0;
+            states[1] = nx;
+        }
+        return;
+    }
+    const uint32_t n = stt.n_active;
     const uint32_t tiles = (n + kTile - 1) / kTile;
     const uint32_t per = (tiles + 1023) / 1024;
     const uint32_t b = threadIdx.x * per;
@@ -434,29 +452,35 @@ __global__ void __launch_bounds__(1024)
     }
     if (threadIdx.x == 1023) {
         const uint32_t m = sums[1023];
-        state->merged = m;
-        next->n_active = n - m;
-        next->begin = state->begin - m;
-        next->end = state->end - 2 * m;
-        next->merged = 0;
+        states[0].merged = m;
+        PlocState nx;
+        nx.n_active = n - m;
+        nx.begin = stt.begin - m;
+        nx.end = stt.end - 2 * m;
+        nx.merged = 0;
+        nx.buf = stt.buf ^ 1u;
+        nx.pad[0] = nx.pad[1] = nx.pad[2] = 0;
+        states[1] = nx;
     }
 }
 
 // Parents, children and compaction for one tile (locb.rs:193-215).  m[k] is
 // the inclusive prefix of marks; the tile keeps m for [ts - r, te).
 __global__ void __launch_bounds__(kBlock)
-    merge_kernel(const PlocState* state, const DevAabb* cl, const uint32_t* nb,
-                 const uint32_t* mark, const uint32_t* tile_offset, DevAabb* cl_out,
-                 DevAabb* nodes) {
+    merge_kernel(const PlocState* state, const DevAabb* cl0, DevAabb* cl1, const uint32_t* nb,
+                 const uint32_t* mark, const uint32_t* tile_offset, DevAabb* nodes) {
     __shared__ uint32_t s_m[kRadius + kTile];
     __shared__ uint32_t s_scan[kBlock];
-    const uint32_t n = state->n_active;
+    const PlocState stt = *state;
+    if (stt.n_active <= uint32_t(kTailMax)) return;
+    const uint32_t n = stt.n_active;
+    const DevAabb* cl = stt.buf ? cl1 : cl0;
+    DevAabb* cl_out = stt.buf ? const_cast<DevAabb*>(cl0) : cl1;
     const uint32_t ts = blockIdx.x * kTile;
     if (ts >= n) return;
     const uint32_t te = ts + kTile < n ? ts + kTile : n;
-    const uint32_t children_begin = state->end - 2 * state->merged;
+    const uint32_t children_begin = stt.end - 2 * stt.merged;
     const uint32_t excl = tile_offset[blockIdx.x];  // marks in [0, ts)
-    // inclusive prefix inside the tile: each thread scans 4 consecutive marks
     constexpr int kPer = kTile / kBlock;
     uint32_t v[kPer];
     uint32_t s = 0;
@@ -483,9 +507,8 @@ __global__ void __launch_bounds__(kBlock)
         uint32_t acc = excl;
         for (int h = 1; h <= kRadius; ++h) {
             if (ts < uint32_t(h)) break;
-            const uint32_t k = ts - h;  // m[k]
             s_m[kRadius - h] = acc;
-            acc -= mark[k];
+            acc -= mark[ts - h];
         }
     }
     __syncthreads();
@@ -501,7 +524,7 @@ __global__ void __launch_bounds__(kBlock)
                 DevAabb ci, cj;
                 load_node(cl + i, ci);
                 load_node(cl + j, cj);
-                DevAabb parent = union_node(ci, cj);
+                DevAabb parent = union_node(ci, cj);  // input[j].union_of(input[i]), locb.rs:202
                 parent.extra1 = -1;
                 parent.extra2 = int32_t(first_child);
                 store_node(cl_out + (i - mi), parent);
@@ -516,38 +539,80 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-// All remaining iterations in one workgroup, clusters resident in LDS.
+// All remaining iterations in one workgroup, clusters resident in LDS as
+// six bound arrays plus the two node fields.  Thread t owns clusters
+// t, t + 1024, t + 2048, t + 3072, so lanes touch consecutive LDS words; the
+// mark prefix comes from wave ballots.
+constexpr int kTailPer = kTailMax / kTailThreads;
+constexpr int kTailWaves = kTailThreads / 64;
+
 struct TailShared {
-    DevAabb cl[kTailMax];
+    float b[6][kTailMax];
+    int32_t e1[kTailMax];
+    int32_t e2[kTailMax];
     uint16_t nb[kTailMax];
     uint16_t m[kTailMax];
-    uint32_t sums[kTailThreads];
+    uint32_t wsum[kTailPer * kTailWaves];
 };
 
+__device__ __forceinline__ DevAabb tail_node(const TailShared& sh, uint32_t i) {
+    DevAabb a;
+    a.min[0] = sh.b[0][i];
+    a.min[1] = sh.b[1][i];
+    a.min[2] = sh.b[2][i];
+    a.max[0] = sh.b[3][i];
+    a.max[1] = sh.b[4][i];
+    a.max[2] = sh.b[5][i];
+    a.extra1 = sh.e1[i];
+    a.extra2 = sh.e2[i];
+    return a;
+}
+
+__device__ __forceinline__ void tail_store(TailShared& sh, uint32_t i, const DevAabb& a) {
+    sh.b[0][i] = a.min[0];
+    sh.b[1][i] = a.min[1];
+    sh.b[2][i] = a.min[2];
+    sh.b[3][i] = a.max[0];
+    sh.b[4][i] = a.max[1];
+    sh.b[5][i] = a.max[2];
+    sh.e1[i] = a.extra1;
+    sh.e2[i] = a.extra2;
+}
+
 __global__ void __launch_bounds__(kTailThreads)
-    tail_kernel(const PlocState* state, const DevAabb* cl_in, DevAabb* nodes) {
+    tail_kernel(const PlocState* state, const DevAabb* cl0, const DevAabb* cl1, DevAabb* nodes) {
     extern __shared__ unsigned char smem_raw[];
     TailShared& sh = *reinterpret_cast<TailShared*>(smem_raw);
-    uint32_t n = state->n_active;
-    uint32_t end = state->end;
-    for (uint32_t i = threadIdx.x; i < n; i += kTailThreads) load_node(cl_in + i, sh.cl[i]);
+    const PlocState stt = *state;
+    uint32_t n = stt.n_active;
+    uint32_t end = stt.end;
+    const DevAabb* cl_in = stt.buf ? cl1 : cl0;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = tid >> 6;
+    const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+    for (uint32_t i = tid; i < n; i += kTailThreads) {
+        DevAabb a;
+        load_node(cl_in + i, a);
+        tail_store(sh, i, a);
+    }
     __syncthreads();
-    constexpr int kPer = kTailMax / kTailThreads;  // consecutive clusters per thread
     while (n > 1) {
-        const uint32_t base = threadIdx.x * kPer;
         // nearest neighbours
-        for (int k = 0; k < kPer; ++k) {
-            const uint32_t i = base + k;
-            if (i >= n) break;
+#pragma unroll
+        for (int k = 0; k < kTailPer; ++k) {
+            const uint32_t i = tid + uint32_t(k) * kTailThreads;
+            if (i >= n) continue;
             uint32_t s0, s1;
             nn_window(i, n, s0, s1);
+            const float a0 = sh.b[0][i], a1 = sh.b[1][i], a2 = sh.b[2][i];
+            const float a3 = sh.b[3][i], a4 = sh.b[4][i], a5 = sh.b[5][i];
             float best = 3.40282347e38f;
             uint32_t best_j = 0xFFFFu;
-            const Bounds6 bi = bounds_of(sh.cl[i]);
             for (uint32_t j = s0; j < s1; ++j) {
                 if (j == i) continue;
-                const Bounds6 bj = bounds_of(sh.cl[j]);
-                const float d = j < i ? half_area_union(bj, bi) : half_area_union(bi, bj);
+                const float d = dist6(a0, a1, a2, a3, a4, a5, sh.b[0][j], sh.b[1][j], sh.b[2][j],
+                                      sh.b[3][j], sh.b[4][j], sh.b[5][j]);
                 if (d < best) {
                     best = d;
                     best_j = j;
@@ -556,41 +621,44 @@ __global__ void __launch_bounds__(kTailThreads)
             sh.nb[i] = uint16_t(best_j);
         }
         __syncthreads();
-        // marks and their inclusive prefix
-        uint32_t mk[kPer];
-        uint32_t s = 0;
-        for (int k = 0; k < kPer; ++k) {
-            const uint32_t i = base + k;
-            uint32_t mm = 0;
+        // mutual marks at the lower index and their inclusive prefix
+        uint32_t mk[kTailPer], pre[kTailPer];
+#pragma unroll
+        for (int k = 0; k < kTailPer; ++k) {
+            const uint32_t i = tid + uint32_t(k) * kTailThreads;
+            bool mark = false;
             if (i < n) {
                 const uint32_t j = sh.nb[i];
-                mm = (j > i && j < n && sh.nb[j] == i) ? 1u : 0u;
+                mark = j > i && j < n && sh.nb[j] == i;
             }
-            mk[k] = mm;
-            s += mm;
+            const uint64_t bal = __ballot(mark);
+            mk[k] = mark ? 1u : 0u;
+            pre[k] = __popcll(bal & lt);
+            if (lane == 0) sh.wsum[k * kTailWaves + wave] = __popcll(bal);
         }
-        sh.sums[threadIdx.x] = s;
         __syncthreads();
-        for (int off = 1; off < kTailThreads; off <<= 1) {
-            const uint32_t t = threadIdx.x >= uint32_t(off) ? sh.sums[threadIdx.x - off] : 0u;
-            __syncthreads();
-            sh.sums[threadIdx.x] += t;
-            __syncthreads();
+        uint32_t off[kTailPer];
+        uint32_t running = 0;
+        for (int c = 0; c < kTailPer * kTailWaves; ++c) {
+#pragma unroll
+            for (int k = 0; k < kTailPer; ++k)
+                if (c == k * kTailWaves + int(wave)) off[k] = running;
+            running += sh.wsum[c];
         }
-        const uint32_t merged = sh.sums[kTailThreads - 1];
-        uint32_t run = threadIdx.x ? sh.sums[threadIdx.x - 1] : 0u;
-        for (int k = 0; k < kPer; ++k) {
-            run += mk[k];
-            if (base + k < n) sh.m[base + k] = uint16_t(run);
+        const uint32_t merged = running;
+#pragma unroll
+        for (int k = 0; k < kTailPer; ++k) {
+            const uint32_t i = tid + uint32_t(k) * kTailThreads;
+            if (i < n) sh.m[i] = uint16_t(off[k] + pre[k] + mk[k]);
         }
         __syncthreads();
         if (merged == 0) break;  // no mutual pair: only with NaN bounds
         const uint32_t children_begin = end - 2 * merged;
-        // read everything this thread writes, then write after a barrier
-        DevAabb keep[kPer];
-        uint32_t dst[kPer];
-        for (int k = 0; k < kPer; ++k) {
-            const uint32_t i = base + k;
+        DevAabb keep[kTailPer];
+        uint32_t dst[kTailPer];
+#pragma unroll
+        for (int k = 0; k < kTailPer; ++k) {
+            const uint32_t i = tid + uint32_t(k) * kTailThreads;
             dst[k] = 0xFFFFFFFFu;
             if (i >= n) continue;
             const uint32_t j = sh.nb[i];
@@ -599,8 +667,8 @@ __global__ void __launch_bounds__(kTailThreads)
             if (mutual) {
                 if (j < i) {
                     const uint32_t first_child = children_begin + (uint32_t(sh.m[j]) - 1) * 2;
-                    const DevAabb ci = sh.cl[i];
-                    const DevAabb cj = sh.cl[j];
+                    const DevAabb ci = tail_node(sh, i);
+                    const DevAabb cj = tail_node(sh, j);
                     DevAabb parent = union_node(ci, cj);
                     parent.extra1 = -1;
                     parent.extra2 = int32_t(first_child);
@@ -610,18 +678,19 @@ __global__ void __launch_bounds__(kTailThreads)
                     dst[k] = i - mi;
                 }
             } else {
-                keep[k] = sh.cl[i];
+                keep[k] = tail_node(sh, i);
                 dst[k] = i - mi;
             }
         }
         __syncthreads();
-        for (int k = 0; k < kPer; ++k)
-            if (dst[k] != 0xFFFFFFFFu) sh.cl[dst[k]] = keep[k];
+#pragma unroll
+        for (int k = 0; k < kTailPer; ++k)
+            if (dst[k] != 0xFFFFFFFFu) tail_store(sh, dst[k], keep[k]);
         __syncthreads();
         n -= merged;
         end -= 2 * merged;
     }
-    if (threadIdx.x == 0) store_node(nodes, sh.cl[0]);
+    if (tid == 0) store_node(nodes, tail_node(sh, 0));
 }
 
 struct Workspace {
@@ -632,6 +701,9 @@ struct Workspace {
 };
 
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+constexpr uint32_t kMaxIterations = 1u << 14;
+constexpr int kBatch = 6;  // iterations enqueued between host checks
 
 }  // namespace
 
@@ -659,9 +731,12 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
     }
     res.node_count = node_count;
     res.prim_count = n;
-    auto fail = [&](hipError_t err, const char* what) {
+    auto release_result = [&]() {
         (void)hipFree(res.nodes);
         (void)hipFree(res.indices);
+    };
+    auto fail = [&](hipError_t err, const char* what) {
+        release_result();
         return hip_fail(err, what);
     };
 
@@ -679,12 +754,15 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
     const size_t o_world = carve(sizeof(Bounds6));
     const size_t o_k0 = carve(size_t(n) * 4), o_k1 = carve(size_t(n) * 4);
     const size_t o_v1 = carve(size_t(n) * 4);
-    const size_t o_cnt = carve(size_t(256) * sort_tiles * 4);
+    const uint32_t count_entries = 256u * sort_tiles;
+    const size_t o_cnt = carve(size_t(count_entries) * 4);
+    const size_t scan_bytes = scan_scratch_bytes(count_entries);
+    const size_t o_scan = carve(scan_bytes);
     const size_t o_cl0 = carve(size_t(n) * sizeof(DevAabb));
     const size_t o_cl1 = carve(size_t(n) * sizeof(DevAabb));
     const size_t o_nb = carve(size_t(n) * 4), o_mark = carve(size_t(n) * 4);
     const size_t o_tc = carve(size_t(ploc_tiles) * 4);
-    const size_t o_state = carve(2 * sizeof(PlocState));
+    const size_t o_state = carve(size_t(kMaxIterations) * sizeof(PlocState));
     Workspace ws;
     if ((e = hipMalloc(&ws.base, off)) != hipSuccess) return fail(e, "hipMalloc(workspace)");
     char* b = static_cast<char*>(ws.base);
@@ -697,6 +775,7 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
     uint32_t* v0 = res.indices;  // the final sorted order lands here
     uint32_t* v1 = reinterpret_cast<uint32_t*>(b + o_v1);
     uint32_t* counts = reinterpret_cast<uint32_t*>(b + o_cnt);
+    void* scan_scratch = b + o_scan;
     DevAabb* cl0 = reinterpret_cast<DevAabb*>(b + o_cl0);
     DevAabb* cl1 = reinterpret_cast<DevAabb*>(b + o_cl1);
     uint32_t* nbuf = reinterpret_cast<uint32_t*>(b + o_nb);
@@ -711,7 +790,7 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
     if (n <= 2) {
         hipLaunchKernelGGL(bounds_partial_kernel, dim3(1), dim3(kBlock), 0, stream, in_aabbs, n,
                            part);
-        hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(1), 0, stream, part, 1u, world);
+        hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(kBlock), 0, stream, part, 1u, world);
         hipLaunchKernelGGL(small_root_kernel, dim3(1), dim3(1), 0, stream, world, n, res.nodes,
                            res.indices);
         if ((e = hipGetLastError()) != hipSuccess) return fail(e, "small build");
@@ -721,7 +800,7 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
     }
     hipLaunchKernelGGL(bounds_partial_kernel, dim3(parts), dim3(kBlock), 0, stream, in_aabbs, n,
                        part);
-    hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(1), 0, stream, part, parts, world);
+    hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(kBlock), 0, stream, part, parts, world);
     hipLaunchKernelGGL(morton_kernel, dim3(g), dim3(kBlock), 0, stream, centers, stride, n, world,
                        k0, v0);
     // 4 stable passes; after an even number of passes the result is back in
@@ -730,8 +809,9 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
     for (uint32_t shift = 0; shift < 32; shift += 8) {
         hipLaunchKernelGGL(radix_hist_kernel, dim3(sort_tiles), dim3(256), 0, stream, ki, n, shift,
                            sort_tiles, counts);
-        hipLaunchKernelGGL(scan_exclusive_kernel, dim3(1), dim3(1024), 0, stream, counts,
-                           256u * sort_tiles, static_cast<uint32_t*>(nullptr));
+        if ((e = scan_exclusive_u32(counts, count_entries, nullptr, scan_scratch, scan_bytes,
+                                    stream)) != hipSuccess)
+            return fail(e, "radix scan");
         hipLaunchKernelGGL(radix_scatter_kernel, dim3(sort_tiles), dim3(kSortBlock), 0, stream, ki,
                            vi, n, shift, sort_tiles, counts, ko, vo);
         std::swap(ki, ko);
@@ -739,38 +819,41 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
     }
     hipLaunchKernelGGL(leaves_kernel, dim3(g), dim3(kBlock), 0, stream, in_aabbs, res.indices, n,
                        cl0);
-    PlocState init{n, n - 1, 2 * n - 1, 0};
+    PlocState init{n, n - 1, 2 * n - 1, 0, 0, {0, 0, 0}};
     if ((e = hipMemcpyAsync(st, &init, sizeof(init), hipMemcpyHostToDevice, stream)) != hipSuccess)
         return fail(e, "state init");
     if ((e = hipGetLastError()) != hipSuccess) return fail(e, "sort launch");
 
-    // multi-block iterations while the active set is large
-    uint32_t active = n;
-    int cur = 0;
-    DevAabb* cls[2] = {cl0, cl1};
+    // multi-block iterations while the active set is large, in batches
+    uint32_t known = n;
+    uint32_t iter = 0;
     PlocState host_state;
-    while (active > uint32_t(kTailMax)) {
-        const uint32_t tiles = (active + kTile - 1) / kTile;
-        PlocState* s_cur = st + cur;
-        PlocState* s_nxt = st + (cur ^ 1);
-        hipLaunchKernelGGL(nn_mark_kernel, dim3(tiles), dim3(kBlock), 0, stream, s_cur, cls[cur],
-                           nbuf, mbuf, tcount);
-        hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, stream, s_cur, s_nxt, tcount);
-        hipLaunchKernelGGL(merge_kernel, dim3(tiles), dim3(kBlock), 0, stream, s_cur, cls[cur],
-                           nbuf, mbuf, tcount, cls[cur ^ 1], res.nodes);
+    while (known > uint32_t(kTailMax)) {
+        const uint32_t tiles = (known + kTile - 1) / kTile;
+        for (int bt = 0; bt < kBatch; ++bt) {
+            if (iter + 1 >= kMaxIterations) {
+                release_result();
+                set_error("clustering needed more than %u iterations", kMaxIterations);
+                return Error;
+            }
+            hipLaunchKernelGGL(nn_mark_kernel, dim3(tiles), dim3(kBlock), 0, stream, st + iter, cl0,
+                               cl1, nbuf, mbuf, tcount);
+            hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, stream, st + iter, tcount);
+            hipLaunchKernelGGL(merge_kernel, dim3(tiles), dim3(kBlock), 0, stream, st + iter, cl0,
+                               cl1, nbuf, mbuf, tcount, res.nodes);
+            ++iter;
+        }
         if ((e = hipGetLastError()) != hipSuccess) return fail(e, "ploc iteration");
-        if ((e = hipMemcpyAsync(&host_state, s_nxt, sizeof(host_state), hipMemcpyDeviceToHost,
+        if ((e = hipMemcpyAsync(&host_state, st + iter, sizeof(host_state), hipMemcpyDeviceToHost,
                                 stream)) != hipSuccess)
             return fail(e, "state read");
         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return fail(e, "ploc iteration");
-        cur ^= 1;
-        if (host_state.n_active == active) {
-            (void)hipFree(res.nodes);
-            (void)hipFree(res.indices);
+        if (host_state.n_active >= known) {
+            release_result();
             set_error("clustering made no progress (non-finite bounds?)");
             return Error;
         }
-        active = host_state.n_active;
+        known = host_state.n_active;
     }
     static bool lds_ok = false;
     if (!lds_ok) {
@@ -780,7 +863,7 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
         lds_ok = true;
     }
     hipLaunchKernelGGL(tail_kernel, dim3(1), dim3(kTailThreads), sizeof(TailShared), stream,
-                       st + cur, cls[cur], res.nodes);
+                       st + iter, cl0, cl1, res.nodes);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e, "tail launch");
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return fail(e, "ploc tail");
     *out = res;

@@ -26,10 +26,13 @@ struct TraceParams {
     DevHit* hits;
     uint8_t* occluded;
     uint32_t* status;
+    uint64_t* stats;  // non-null: {node visits, triangle tests} accumulated (MBVH closest hit)
 };
 
 hipError_t launch_trace(const TraceParams& p, bool mbvh, bool ref_order, bool any_hit,
                         hipStream_t stream);
+// Lowest-(t, id) MBVH traversal (trace_fast.hip).
+hipError_t launch_trace_fast(const TraceParams& p, bool any_hit, hipStream_t stream);
 
 // Device status word of the current device (kStatus* bits).
 uint32_t* device_status_ptr();

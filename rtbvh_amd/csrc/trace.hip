@@ -141,9 +141,10 @@ __device__ __forceinline__ void write_result(const TraceParams& p, uint64_t rid,
         hit[b] = h_;                        \
     }
 
-template <bool REF, bool ANY>
+template <bool REF, bool ANY, bool STATS = false>
 __global__ void __launch_bounds__(kBlock) mbvh_trace_kernel(TraceParams p) {
     __shared__ int32_t lds_stack[kLdsStack * kBlock];
+    uint64_t n_nodes = 0, n_tris = 0;  // STATS only
     Stack st;
     st.lds = lds_stack + threadIdx.x;
     st.sp = 0;
@@ -166,6 +167,7 @@ __global__ void __launch_bounds__(kBlock) mbvh_trace_kernel(TraceParams p) {
             st.sp = 0;
         }
         const DevMbvhNode nd = p.mnodes[cur];
+        if (STATS) ++n_nodes;
         float tn[4], tf[4];
         int32_t ch[4], ct[4];
         bool hit[4];
@@ -216,6 +218,7 @@ __global__ void __launch_bounds__(kBlock) mbvh_trace_kernel(TraceParams p) {
             for (int k = 3; k >= 0; --k) {
                 if (!hit[k] || found || overflow) continue;
                 if (ct[k] > -1) {
+                    if (STATS) n_tris += uint64_t(ct[k]);
                     found = test_leaf<REF, ANY>(p, ch[k], ct[k], s);
                 } else if (ch[k] > -1) {
                     overflow = !st.push(ch[k]);
@@ -230,6 +233,7 @@ __global__ void __launch_bounds__(kBlock) mbvh_trace_kernel(TraceParams p) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 if (!hit[k] || found || ct[k] < 0 || !(tn[k] <= s.bt)) continue;
+                if (STATS) n_tris += uint64_t(ct[k]);
                 found = test_leaf<REF, ANY>(p, ch[k], ct[k], s);
             }
 #pragma unroll
@@ -253,6 +257,10 @@ __global__ void __launch_bounds__(kBlock) mbvh_trace_kernel(TraceParams p) {
             active = false;
             rid += stride;
         }
+    }
+    if (STATS) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(p.stats), (unsigned long long)n_nodes);
+        atomicAdd(reinterpret_cast<unsigned long long*>(p.stats) + 1, (unsigned long long)n_tris);
     }
 }
 
@@ -374,9 +382,9 @@ int persistent_grid(K kernel, uint64_t work_items) {
     return int(need < uint64_t(blocks) ? need : uint64_t(blocks));
 }
 
-template <bool REF, bool ANY>
+template <bool REF, bool ANY, bool STATS = false>
 hipError_t launch_mbvh(const TraceParams& p, hipStream_t stream) {
-    auto k = mbvh_trace_kernel<REF, ANY>;
+    auto k = mbvh_trace_kernel<REF, ANY, STATS>;
     const int grid = persistent_grid(k, p.ray_count);
     if (grid == 0) return hipSuccess;
     hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, stream, p);
@@ -396,6 +404,12 @@ hipError_t launch_bvh(const TraceParams& p, hipStream_t stream) {
 
 hipError_t launch_trace(const TraceParams& p, bool mbvh, bool ref_order, bool any_hit,
                         hipStream_t stream) {
+    if (mbvh && !ref_order) return launch_trace_fast(p, any_hit, stream);
+    if (p.stats) {
+        if (!mbvh || any_hit) return hipErrorInvalidValue;
+        return ref_order ? launch_mbvh<true, false, true>(p, stream)
+                         : launch_mbvh<false, false, true>(p, stream);
+    }
     if (mbvh) {
         if (ref_order)
             return any_hit ? launch_mbvh<true, true>(p, stream) : launch_mbvh<true, false>(p, stream);

@@ -1,0 +1,278 @@
+// MBVH closest-hit / any-hit traversal for the default tie rule (the
+// lexicographic minimum of (t, prim id) — identical to brute force), tuned
+// for gfx950.  The reference-order kernel in trace.hip reproduces
+// MbvhIterator exactly; this kernel keeps its triangle test and box
+// arithmetic but is free to choose the visit order, because the lowest
+// (t, prim) over all triangles does not depend on it:
+//
+//   * slab test with IEEE min/max (v_min3/v_max3): a NaN slab (zero
+//     direction component, origin on the plane) drops out of the test, which
+//     can only accept more boxes, never fewer;
+//   * a slot is visited when t_entry <= t_best and t_exit >= t_min;
+//   * slots are ordered by a 32-bit key (entry distance bits with the slot
+//     id in the two low bits), sorted with five integer min/max pairs;
+//   * leaves are tested near to far, each re-checked against the current t;
+//   * the nearest interior child is visited next without a stack round trip;
+//     the others are pushed far to near together with their entry distance,
+//     and a popped entry whose distance now exceeds t_best is dropped without
+//     fetching its node.
+// The oracle's lowest-id mode (oracle/refcpu.cpp) follows the same order, so
+// the kernel's work counters (STATS) are checked against it.
+#include "internal.hpp"
+
+namespace rtbvh {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kLdsStack = 12;
+constexpr int kDeepStack = RTBVH_MAX_STACK - kLdsStack;
+
+struct FastStack {
+    int32_t* lds_i;  // lane-interleaved: entry e at [e * kBlock]
+    float* lds_t;
+    int32_t deep_i[kDeepStack];
+    float deep_t[kDeepStack];
+    int sp;
+
+    __device__ __forceinline__ bool push(int32_t node, float t) {
+        if (sp < kLdsStack) {
+            lds_i[sp * kBlock] = node;
+            lds_t[sp * kBlock] = t;
+        } else if (sp < RTBVH_MAX_STACK) {
+            deep_i[sp - kLdsStack] = node;
+            deep_t[sp - kLdsStack] = t;
+        } else {
+            return false;
+        }
+        ++sp;
+        return true;
+    }
+    __device__ __forceinline__ void pop(int32_t& node, float& t) {
+        --sp;
+        if (sp < kLdsStack) {
+            node = lds_i[sp * kBlock];
+            t = lds_t[sp * kBlock];
+        } else {
+            node = deep_i[sp - kLdsStack];
+            t = deep_t[sp - kLdsStack];
+        }
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ T sel4(uint32_t k, T a, T b, T c, T d) {
+    return k == 0 ? a : (k == 1 ? b : (k == 2 ? c : d));
+}
+
+__device__ __forceinline__ void csort(uint32_t& a, uint32_t& b) {
+    const uint32_t lo = a < b ? a : b;
+    const uint32_t hi = a < b ? b : a;
+    a = lo;
+    b = hi;
+}
+
+template <bool ANY, bool STATS>
+__global__ void __launch_bounds__(kBlock) mbvh_fast_kernel(TraceParams p) {
+    __shared__ int32_t s_idx[kLdsStack * kBlock];
+    __shared__ float s_t[kLdsStack * kBlock];
+    FastStack st;
+    st.lds_i = s_idx + threadIdx.x;
+    st.lds_t = s_t + threadIdx.x;
+    st.sp = 0;
+    uint64_t n_nodes = 0, n_tris = 0;
+    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+    uint64_t rid = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f,
+          iz = 0.f, tmin = 0.f;
+    float bt = 0.f, bu = 0.f, bv = 0.f;
+    uint32_t bp = RTBVH_MISS;
+    int32_t cur = -1;
+    bool active = false;
+    for (;;) {
+        if (!active) {
+            if (rid >= p.ray_count) break;
+            const float4* r4 = reinterpret_cast<const float4*>(p.rays + rid);
+            const float4 ra = r4[0];
+            const float4 rb = r4[1];
+            ox = ra.x;
+            oy = ra.y;
+            oz = ra.z;
+            tmin = ra.w;
+            dx = rb.x;
+            dy = rb.y;
+            dz = rb.z;
+            bt = rb.w;
+            ix = 1.0f / dx;  // Ray::new, src/ray.rs:166-182
+            iy = 1.0f / dy;
+            iz = 1.0f / dz;
+            bu = 0.f;
+            bv = 0.f;
+            bp = RTBVH_MISS;
+            const bool nan = (ox != ox) || (oy != oy) || (oz != oz) || (dx != dx) ||
+                             (dy != dy) || (dz != dz);
+            if (nan || p.node_count == 0) {
+                if (ANY) {
+                    p.occluded[rid] = 0;
+                } else {
+                    reinterpret_cast<float4*>(p.hits)[rid] =
+                        make_float4(bt, 0.f, 0.f, __uint_as_float(RTBVH_MISS));
+                }
+                rid += stride;
+                continue;
+            }
+            active = true;
+            cur = 0;
+            st.sp = 0;
+        }
+        bool done = false;
+        bool overflow = false;
+        bool found = false;
+        if (cur < 0) {
+            // pop until an entry can still hold a hit at or before bt
+            while (st.sp > 0) {
+                int32_t c;
+                float t;
+                st.pop(c, t);
+                if (t <= bt) {
+                    cur = c;
+                    break;
+                }
+            }
+            if (cur < 0 || uint32_t(cur) >= p.node_count) done = true;
+        }
+        if (!done) {
+            const DevMbvhNode nd = p.mnodes[cur];
+            if (STATS) ++n_nodes;
+            const float mnx[4] = {nd.min_x.x, nd.min_x.y, nd.min_x.z, nd.min_x.w};
+            const float mxx[4] = {nd.max_x.x, nd.max_x.y, nd.max_x.z, nd.max_x.w};
+            const float mny[4] = {nd.min_y.x, nd.min_y.y, nd.min_y.z, nd.min_y.w};
+            const float mxy[4] = {nd.max_y.x, nd.max_y.y, nd.max_y.z, nd.max_y.w};
+            const float mnz[4] = {nd.min_z.x, nd.min_z.y, nd.min_z.z, nd.min_z.w};
+            const float mxz[4] = {nd.max_z.x, nd.max_z.y, nd.max_z.z, nd.max_z.w};
+            float tn[4];
+            uint32_t key[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float tx0 = (mnx[i] - ox) * ix, tx1 = (mxx[i] - ox) * ix;
+                const float ty0 = (mny[i] - oy) * iy, ty1 = (mxy[i] - oy) * iy;
+                const float tz0 = (mnz[i] - oz) * iz, tz1 = (mxz[i] - oz) * iz;
+                const float a = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+                const float b = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+                tn[i] = a;
+                const bool h = (b >= a) && (a <= bt) && (b >= tmin);
+                key[i] = h ? ((__float_as_uint(a > 0.0f ? a : 0.0f) & ~3u) | uint32_t(i))
+                           : 0xFFFFFFFFu;
+            }
+            csort(key[0], key[1]);
+            csort(key[2], key[3]);
+            csort(key[0], key[2]);
+            csort(key[1], key[3]);
+            csort(key[1], key[2]);
+            const int32_t c0 = nd.children.x, c1 = nd.children.y, c2 = nd.children.z,
+                          c3 = nd.children.w;
+            const int32_t n0 = nd.counts.x, n1 = nd.counts.y, n2 = nd.counts.z, n3 = nd.counts.w;
+            // leaves, near to far
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (key[k] == 0xFFFFFFFFu || found) continue;
+                const uint32_t s = key[k] & 3u;
+                const int32_t cnt = sel4(s, n0, n1, n2, n3);
+                if (cnt < 0) continue;
+                if (!(sel4(s, tn[0], tn[1], tn[2], tn[3]) <= bt)) continue;
+                const int32_t first = sel4(s, c0, c1, c2, c3);
+                if (STATS) n_tris += uint64_t(cnt);
+                for (int32_t j = 0; j < cnt; ++j) {
+                    const uint32_t pos = uint32_t(first) + uint32_t(j);
+                    if (pos >= p.index_count) break;
+                    const uint32_t prim = p.indices[pos];
+                    if (prim >= p.prim_count) continue;
+                    const float* v = p.verts + size_t(prim) * 9;
+                    const MtResult m = mt_candidate(v[0], v[1], v[2], v[3], v[4], v[5], v[6],
+                                                    v[7], v[8], ox, oy, oz, dx, dy, dz, tmin);
+                    if (!m.ok) continue;
+                    if (ANY) {
+                        if (m.t < bt) {
+                            bp = prim;
+                            found = true;
+                            break;
+                        }
+                    } else if ((m.t < bt) || (m.t == bt && prim < bp)) {
+                        bt = m.t;
+                        bu = m.u;
+                        bv = m.v;
+                        bp = prim;
+                    }
+                }
+            }
+            // interior children, far to near; the nearest becomes the next node
+            int32_t next = -1;
+            float next_t = 0.f;
+#pragma unroll
+            for (int k = 3; k >= 0; --k) {
+                if (key[k] == 0xFFFFFFFFu || found || overflow) continue;
+                const uint32_t s = key[k] & 3u;
+                if (sel4(s, n0, n1, n2, n3) >= 0) continue;
+                const int32_t c = sel4(s, c0, c1, c2, c3);
+                const float t = sel4(s, tn[0], tn[1], tn[2], tn[3]);
+                if (c < 0 || !(t <= bt)) continue;
+                if (next >= 0) overflow = !st.push(next, next_t);
+                next = c;
+                next_t = t;
+            }
+            cur = next;
+            if (found || overflow) done = true;
+        }
+        if (overflow) atomicOr(p.status, kStatusStackOverflow);
+        if (done) {
+            if (ANY) {
+                p.occluded[rid] = (bp != RTBVH_MISS) ? 1 : 0;
+            } else {
+                reinterpret_cast<float4*>(p.hits)[rid] =
+                    make_float4(overflow ? __builtin_nanf("") : bt, bu, bv,
+                                __uint_as_float(overflow ? RTBVH_MISS : bp));
+            }
+            active = false;
+            cur = -1;
+            rid += stride;
+        }
+    }
+    if (STATS) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(p.stats), (unsigned long long)n_nodes);
+        atomicAdd(reinterpret_cast<unsigned long long*>(p.stats) + 1, (unsigned long long)n_tris);
+    }
+}
+
+template <bool ANY, bool STATS>
+hipError_t launch_fast(const TraceParams& p, hipStream_t stream) {
+    auto k = mbvh_fast_kernel<ANY, STATS>;
+    static int blocks[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (blocks[dev] == 0) {
+        int cus = 0, per_cu = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, 0) != hipSuccess ||
+            per_cu <= 0)
+            per_cu = 1;
+        blocks[dev] = cus * per_cu;
+    }
+    const uint64_t need = (p.ray_count + kBlock - 1) / kBlock;
+    const int grid = int(need < uint64_t(blocks[dev]) ? need : uint64_t(blocks[dev]));
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, stream, p);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_trace_fast(const TraceParams& p, bool any_hit, hipStream_t stream) {
+    if (p.stats) {
+        if (any_hit) return hipErrorInvalidValue;
+        return launch_fast<false, true>(p, stream);
+    }
+    return any_hit ? launch_fast<true, false>(p, stream) : launch_fast<false, false>(p, stream);
+}
+
+}  // namespace rtbvh

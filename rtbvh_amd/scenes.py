@@ -208,3 +208,91 @@ def pinhole_rays(width: int, height: int, eye=(0.5, 0.5, -1.5), fov_deg: float =
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     o = np.broadcast_to(np.asarray(eye, dtype=np.float64), d.shape)
     return make_rays(o.astype(np.float32), d.astype(np.float32))
+
+
+# ------------------------------------------------------- device ray sets --
+def pinhole_rays_torch(width: int, height: int, device, eye=(0.5, 0.5, -1.5),
+                       fov_deg: float = 40.0, row0: int = 0, rows: int = None):
+    """pinhole_rays() on a torch device: rows [row0, row0 + rows) of the image,
+    as (rows * width, 8) ray records."""
+    import torch
+
+    rows = height - row0 if rows is None else rows
+    ys = (torch.arange(row0, row0 + rows, device=device, dtype=torch.float64) + 0.5) / height
+    xs = (torch.arange(width, device=device, dtype=torch.float64) + 0.5) / width
+    tan_half = float(np.tan(np.radians(fov_deg) * 0.5))
+    px = (2.0 * xs - 1.0) * tan_half * (width / height)
+    py = (1.0 - 2.0 * ys) * tan_half
+    gy, gx = torch.meshgrid(py, px, indexing="ij")
+    d = torch.stack([gx, gy, torch.ones_like(gx)], dim=-1).reshape(-1, 3)
+    d = d / torch.linalg.norm(d, dim=1, keepdim=True)
+    r = torch.empty((d.shape[0], 8), dtype=torch.float32, device=device)
+    r[:, 0:3] = torch.tensor(eye, dtype=torch.float32, device=device)
+    r[:, 3] = float(RAY_T_MIN)
+    r[:, 4:7] = d.to(torch.float32)
+    r[:, 7] = float(RAY_T_MAX)
+    return r
+
+
+def _onb(n):
+    """Orthonormal basis around unit normals n (Duff et al. 2017)."""
+    import torch
+
+    s = torch.where(n[:, 2] >= 0, torch.ones_like(n[:, 2]), -torch.ones_like(n[:, 2]))
+    a = -1.0 / (s + n[:, 2])
+    b = n[:, 0] * n[:, 1] * a
+    t = torch.stack([1.0 + s * n[:, 0] * n[:, 0] * a, s * b, -s * n[:, 0]], dim=1)
+    bb = torch.stack([b, s + n[:, 1] * n[:, 1] * a, -n[:, 1]], dim=1)
+    return t, bb
+
+
+def bounce_rays_torch(verts, primary, hits, generator, offset: float = 1e-4):
+    """Cosine-weighted bounce rays.  For a primary hit the origin is the hit
+    point moved `offset` along the geometric normal facing the incoming ray;
+    for a primary miss it is a uniformly sampled surface point (area-weighted
+    triangle, uniform barycentrics, random side).  verts: (N, 9) float32
+    device tensor; primary (R, 8) and hits (R, 4) device tensors."""
+    import torch
+
+    dev = primary.device
+    R = primary.shape[0]
+    v = verts.view(-1, 3, 3).to(torch.float64)
+    prim = hits[:, 3].contiguous().view(torch.int32).to(torch.int64)
+    hit = prim >= 0
+    e1 = v[:, 1] - v[:, 0]
+    e2 = v[:, 2] - v[:, 0]
+    nrm = torch.cross(e1, e2, dim=1)
+    area = torch.linalg.norm(nrm, dim=1)
+    nrm = nrm / area.clamp_min(1e-30)[:, None]
+    cdf = torch.cumsum(area, 0)
+    u = torch.rand(R, generator=generator, device=dev, dtype=torch.float64) * cdf[-1]
+    rand_tri = torch.searchsorted(cdf, u).clamp_max(v.shape[0] - 1)
+    tri = torch.where(hit, prim, rand_tri)
+    n = nrm[tri]
+    d_in = primary[:, 4:7].to(torch.float64)
+    r1 = torch.rand(R, generator=generator, device=dev, dtype=torch.float64)
+    r2 = torch.rand(R, generator=generator, device=dev, dtype=torch.float64)
+    sq = torch.sqrt(r1)
+    vt = v[tri]
+    surf = (1 - sq)[:, None] * vt[:, 0] + (sq * (1 - r2))[:, None] * vt[:, 1] + (sq * r2)[:, None] * vt[:, 2]
+    p_hit = primary[:, 0:3].to(torch.float64) + hits[:, 0:1].to(torch.float64) * d_in
+    p = torch.where(hit[:, None], p_hit, surf)
+    side = torch.where(hit, -torch.sign((n * d_in).sum(1)),
+                       torch.where(torch.rand(R, generator=generator, device=dev) < 0.5, -1.0, 1.0)
+                       .to(torch.float64))
+    side = torch.where(side == 0, torch.ones_like(side), side)
+    n = n * side[:, None]
+    o = p + offset * n
+    phi = 2 * np.pi * torch.rand(R, generator=generator, device=dev, dtype=torch.float64)
+    r = torch.sqrt(torch.rand(R, generator=generator, device=dev, dtype=torch.float64))
+    lx, ly = r * torch.cos(phi), r * torch.sin(phi)
+    lz = torch.sqrt(torch.clamp(1 - r * r, min=0))
+    t, b = _onb(n)
+    d = lx[:, None] * t + ly[:, None] * b + lz[:, None] * n
+    d = d / torch.linalg.norm(d, dim=1, keepdim=True)
+    out = torch.empty((R, 8), dtype=torch.float32, device=dev)
+    out[:, 0:3] = o.to(torch.float32)
+    out[:, 3] = float(RAY_T_MIN)
+    out[:, 4:7] = d.to(torch.float32)
+    out[:, 7] = float(RAY_T_MAX)
+    return out
