@@ -36,6 +36,117 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def run_batched(args, dev, rt, scenes, torch):
+    """Many scenes of uneven size: per-scene launches (plain) against one
+    batched build and one batched launch.  Whole job = PLOC build + MBVH
+    collapse + leaf-ordered triangles + closest hit of every scene's rays."""
+    log(f"generating {args.batch_scenes} batch scenes")
+    vl = scenes.batch_scenes(args.batch_scenes, seed=args.seed)
+    counts = [len(v) for v in vl]
+    offs = [0]
+    for c in counts:
+        offs.append(offs[-1] + c)
+    verts = np.concatenate(vl)
+    del vl
+    aabbs, centers = scenes.triangle_aabbs_centers(verts)
+    lo = aabbs.view(np.float32).reshape(-1, 8)
+    v_d = torch.from_numpy(verts).to(dev)
+    a_d = torch.from_numpy(aabbs).to(dev)
+    c_d = torch.from_numpy(centers).to(dev)
+    box_lo = np.stack([lo[offs[s]:offs[s + 1], 0:3].min(0) for s in range(len(counts))])
+    box_hi = np.stack([lo[offs[s]:offs[s + 1], 4:7].max(0) for s in range(len(counts))])
+    del verts, aabbs, centers, lo
+    ray_counts = [16 * c for c in counts]
+    roffs = [0]
+    for c in ray_counts:
+        roffs.append(roffs[-1] + c)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed + 1)
+    rays = scenes.batch_rays_torch(box_lo, box_hi, ray_counts, dev, gen)
+    roff_d = rt.SceneBatch.ray_offsets(ray_counts, dev)
+    n_rays = rays.shape[0]
+    hits_plain = torch.empty((n_rays, 4), dtype=torch.float32, device=dev)
+    hits_batch = torch.empty_like(hits_plain)
+    log(f"batch: {len(counts)} scenes, {offs[-1]} tris, {n_rays} rays")
+
+    def plain(first, last, keep):
+        trees = []
+        for s in range(first, last):
+            p0, p1, r0, r1 = offs[s], offs[s + 1], roffs[s], roffs[s + 1]
+            bvh = rt.Builder(c_d[p0:p1], aabbs=a_d[p0:p1]).construct_locally_ordered_clustered()
+            m = rt.Mbvh.from_bvh(bvh)
+            m.attach_triangles(v_d[p0:p1])
+            m.intersect_prepared(rays[r0:r1], out=hits_plain[r0:r1])
+            if keep:
+                trees.append((bvh, m))
+        torch.cuda.synchronize()
+        return trees
+
+    def batched():
+        b = rt.SceneBatch(c_d, offs, aabbs=a_d)
+        b.attach_triangles(v_d)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        b.intersect(rays, roff_d, out=hits_batch)
+        torch.cuda.synchronize()
+        return b, t1
+
+    # warm-up: kernels loaded, allocator primed
+    plain(0, 4, False)
+    b, _ = batched()
+    del b
+    plain_ms, batch_ms, batch_build_ms, batch_trace_ms = [], [], [], []
+    trees = None
+    for _ in range(2):
+        trees = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        trees = plain(0, len(counts), True)
+        plain_ms.append((time.perf_counter() - t0) * 1e3)
+    # per-scene trace launches alone, trees already built
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s, (_, m) in enumerate(trees):
+        m.intersect_prepared(rays[roffs[s]:roffs[s + 1]], out=hits_plain[roffs[s]:roffs[s + 1]])
+    torch.cuda.synchronize()
+    plain_trace_ms = (time.perf_counter() - t0) * 1e3
+    trees[-1][1].status()
+    del trees
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b, t1 = batched()
+        t2 = time.perf_counter()
+        batch_ms.append((t2 - t0) * 1e3)
+        batch_build_ms.append((t1 - t0) * 1e3)
+        batch_trace_ms.append((t2 - t1) * 1e3)
+        del b
+    same = bool(torch.equal(hits_plain.view(torch.int32), hits_batch.view(torch.int32)))
+    p_ms, b_ms = statistics.median(plain_ms), statistics.median(batch_ms)
+    bt_ms = statistics.median(batch_trace_ms)
+    res = {
+        "scenes": len(counts),
+        "tris": offs[-1],
+        "rays": n_rays,
+        "plain_ms": round(p_ms, 2),
+        "batched_ms": round(b_ms, 2),
+        "plain_trace_ms": round(plain_trace_ms, 2),
+        "batched_build_ms": round(statistics.median(batch_build_ms), 2),
+        "batched_trace_ms": round(bt_ms, 2),
+        "batched_mrays_s": round(n_rays / (bt_ms * 1e-3) / 1e6, 1),
+        "plain_mrays_s": round(n_rays / (plain_trace_ms * 1e-3) / 1e6, 1),
+        "same_hits": same,
+        "workload": "scene sizes round(exp(U(ln 1e3, ln 5e5))), alternating noisy sphere / "
+                    "soup; 16 rays per triangle, origin uniform in the scene box, uniform "
+                    "direction; whole job = PLOC + collapse + triangles + closest hit",
+    }
+    log(f"batched {b_ms:.1f} ms vs plain {p_ms:.1f} ms (x{p_ms / b_ms:.2f}); trace "
+        f"{bt_ms:.1f} vs {plain_trace_ms:.1f} ms; same hits {same}")
+    del rays, hits_plain, hits_batch, v_d, a_d, c_d
+    torch.cuda.empty_cache()
+    return res, p_ms / b_ms
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -47,13 +158,16 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--quick", action="store_true",
                     help="one build of each kind and no binned SAH (for profiled runs)")
+    ap.add_argument("--batch-scenes", type=int, default=256,
+                    help="scenes in the batched-vs-plain comparison (0 skips it)")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="GiB copied by one plain device copy at the end (scales rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays and scenes "
-              "shard with no exchange, see DESIGN.md)", flush=True)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build runs "
+              "on a single GPU (rays and scenes would shard across GPUs with no exchange)",
+              flush=True)
         sys.exit(2)
 
     import torch
@@ -126,17 +240,32 @@ def main():
         bounce[s:e] = scenes.bounce_rays_torch(v_d, primary[s:e], hits_p[s:e], gen)
     torch.cuda.synchronize()
 
-    def timed(rays, steps, warmup):
+    attach_ms = []
+    for i in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        mbvh.attach_triangles(v_d)
+        torch.cuda.synchronize()
+        if i > 0:
+            attach_ms.append((time.perf_counter() - t0) * 1e3)
+
+    def trace_vertex(rays, out):
+        mbvh.intersect(v_d, rays, out=out)
+
+    def trace_prepared(rays, out):
+        mbvh.intersect_prepared(rays, out=out)
+
+    def timed(rays, steps, warmup, fn=trace_prepared):
         out = torch.empty((rays.shape[0], 4), dtype=torch.float32, device=dev)
         for _ in range(warmup):
-            mbvh.intersect(v_d, rays, out=out)
+            fn(rays, out)
         torch.cuda.synchronize()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
               for _ in range(steps)]
         t0 = time.perf_counter()
         for a, b in ev:
             a.record(stream)
-            mbvh.intersect(v_d, rays, out=out)
+            fn(rays, out)
             b.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
@@ -154,6 +283,11 @@ def main():
     ms_per_step = sum(per) / len(per)
     value = n_rays / (ms_per_step * 1e-3) / 1e6
     log(f"bounce {value:.1f} Mrays/s ({ms_per_step:.2f} ms/step), primary {primary_mrays:.1f} Mrays/s")
+    hits_v, per_v, _ = timed(bounce, 3, 1, fn=trace_vertex)
+    vertex_mrays = n_rays / (statistics.median(per_v) * 1e-3) / 1e6
+    same_vertex_hits = bool(torch.equal(hits_v.view(torch.int32), hits.view(torch.int32)))
+    del hits_v
+    log(f"vertex-array path {vertex_mrays:.1f} Mrays/s; same hits {same_vertex_hits}")
 
     # ---- kernel work counters (MBVH nodes and triangles per ray) -----------
     counters = torch.zeros(2, dtype=torch.int64, device=dev)
@@ -169,7 +303,8 @@ def main():
     tris_per_ray = counters[1].item() / n_rays
     same_stats_hits = bool(torch.equal(stats_out.view(torch.int32), hits.view(torch.int32)))
     del stats_out
-    b_ray = 32 + 16 + 128 * nodes_per_ray + 40 * tris_per_ray
+    # ray 32 B + hit 16 B + 128 B per MBVH node + 48 B per leaf-ordered triangle
+    b_ray = 32 + 16 + 128 * nodes_per_ray + 48 * tris_per_ray
     achieved = value * 1e6 * b_ray / 1e9
     traffic = None
     if os.path.exists(PMC_SUMMARY):
@@ -212,6 +347,12 @@ def main():
                          f"1M scene {cpu_build_ms:.0f} ms on 1 thread"}
         log(f"CPU oracle {cpu_mrays:.2f} Mrays/s on {threads} threads; parity {parity}")
 
+    batch_info, batch_ratio = None, None
+    if args.batch_scenes > 0:
+        del bounce
+        torch.cuda.empty_cache()
+        batch_info, batch_ratio = run_batched(args, dev, rt, scenes, torch)
+
     result = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -226,7 +367,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "1M-tri noisy sphere + soup; GPU PLOC (r=14) + MBVH; closest hit of "
+            "workload": "1M-tri noisy sphere + soup; GPU PLOC (r=14) + MBVH + leaf-ordered triangles; closest hit of "
                         "8192^2 cosine-bounce rays",
             "tris": n_tris,
             "rays_per_step": n_rays,
@@ -234,13 +375,17 @@ def main():
         },
         "ploc_build_ms": round(statistics.median(build_ms), 3),
         "mbvh_collapse_ms": round(statistics.median(collapse_ms), 3),
+        "attach_triangles_ms": round(statistics.median(attach_ms), 3),
         "binned_sah_build_ms": round(statistics.median(binned_ms), 3),
         "primary_mrays_s": round(primary_mrays, 2),
+        "vertex_path_mrays_s": round(vertex_mrays, 2),
+        "vertex_path_same_hits": same_vertex_hits,
         "wall_s_timed": round(wall, 4),
         "nodes_per_ray": round(nodes_per_ray, 3),
         "tris_per_ray": round(tris_per_ray, 3),
         "stats_kernel_same_hits": same_stats_hits,
-        "batched_vs_plain": None,
+        "batched_vs_plain": None if batch_ratio is None else round(batch_ratio, 3),
+        "batched": batch_info,
         "parity_vs_cpu_oracle": parity,
         "roofline": {
             "bound": "hbm",

@@ -231,6 +231,82 @@ ResultCode rtbvh_gpu_intersect_mbvh_stats(RTMbvh mbvh, const float *vertices, si
                                           const RTRay *rays, size_t ray_count, RTHit *hits,
                                           uint32_t flags, uint64_t *counters, void *stream);
 
+/* Attaches to an MBVH made by rtbvh_gpu_create_mbvh a device copy of its
+ * triangles in leaf order (v0 and the edges v1 - v0, v2 - v0 per index
+ * entry, 48 bytes each), taken from `vertices` (device, 9 floats per
+ * primitive).  The copy is a snapshot: attach again after the vertices
+ * change.  Freed with the MBVH. */
+ResultCode rtbvh_gpu_mbvh_attach_triangles(RTMbvh mbvh, const float *vertices, size_t prim_count,
+                                           void *stream);
+
+/* rtbvh_gpu_intersect_mbvh / rtbvh_gpu_occluded_mbvh over the attached
+ * triangles: identical results, one dependent memory access less per
+ * triangle test.  Error if none are attached. */
+ResultCode rtbvh_gpu_intersect_mbvh_prepared(RTMbvh mbvh, const RTRay *rays, size_t ray_count,
+                                             RTHit *hits, uint32_t flags, void *stream);
+ResultCode rtbvh_gpu_occluded_mbvh_prepared(RTMbvh mbvh, const RTRay *rays, size_t ray_count,
+                                            uint8_t *occluded, uint32_t flags, void *stream);
+
+/* ---- Adjoint of the hit ----------------------------------------------------
+ * Gradients of a loss L through closest hits made by the rtbvh_gpu_intersect*
+ * calls (no counterpart in the reference, whose triangle test is forward
+ * only).  The hit triangle is fixed by the forward pass; (t, u, v) solves
+ * o + t d = (1-u-v) v0 + u v1 + v v2.  Inputs (device): vertices (9 floats per
+ * primitive), the rays and hits of the forward pass, grad_tuv (3 floats per
+ * ray: dL/dt, dL/du, dL/dv).  Outputs (device, each may be NULL):
+ * grad_vertices (9 floats per primitive) is ACCUMULATED into (zero it first),
+ * grad_origin / grad_direction (3 floats per ray) are written; rays that
+ * missed get zeros. */
+ResultCode rtbvh_gpu_adjoint(const float *vertices, size_t prim_count, const RTRay *rays,
+                             const RTHit *hits, size_t ray_count, const float *grad_tuv,
+                             float *grad_vertices, float *grad_origin, float *grad_direction,
+                             void *stream);
+
+/* ---- Batches of scenes ---------------------------------------------------
+ * Many scenes of uneven size built and traced together (no counterpart in
+ * the reference, which builds and traces one scene per call).  The scenes'
+ * primitives are concatenated; scene s owns primitives
+ * [scene_offsets[s], scene_offsets[s+1]).  Each scene's tree is the PLOC tree
+ * a build of that scene alone makes; the trees share one node array. */
+#define RTBVH_MAX_BATCH_SCENES 4096u
+
+typedef struct RTBatch {
+    uint32_t id;
+    uint32_t scene_count;
+    uint32_t bvh_node_count;      /* 2 * index_count - scene_count */
+    uint32_t node_count;          /* MBVH nodes */
+    const RTBvhNode *bvh_nodes;   /* device; node s is the root of scene s */
+    const RTMbvhNode *nodes;      /* device; node s is the root of scene s */
+    uint32_t index_count;
+    uint32_t pad_;
+    const uint32_t *indices;      /* device; batch-wide primitive ids in leaf order */
+} RTBatch;
+
+/* Builds (PLOC, radius 14) and collapses every scene of the batch.
+ * aabbs/centers: device, one per primitive of the batch (aabbs may be
+ * NULL); scene_offsets: HOST array of scene_count + 1 entries, starting at 0,
+ * every scene at least 3 primitives, scene_count <= RTBVH_MAX_BATCH_SCENES. */
+ResultCode rtbvh_gpu_create_batch(const RTAabb *aabbs, const float *centers, size_t center_stride,
+                                  const uint32_t *scene_offsets, size_t scene_count,
+                                  void *stream, RTBatch *result);
+
+/* Leaf-ordered triangles for the batch (as rtbvh_gpu_mbvh_attach_triangles),
+ * vertices: device, 9 floats per primitive of the batch. */
+ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch batch, const float *vertices,
+                                            size_t prim_count, void *stream);
+
+/* One persistent launch over the rays of every scene: ray i belongs to the
+ * scene s with ray_offsets[s] <= i < ray_offsets[s+1] (ray_offsets: DEVICE
+ * array of scene_count + 1 entries, ray_offsets[0] = 0, ray_offsets[scene_count]
+ * = ray_count).  Hits carry scene-local primitive ids and equal, bit for bit,
+ * those of tracing each scene on its own.  Lowest-(t, id) rule only (flags 0). */
+ResultCode rtbvh_gpu_intersect_batch(RTBatch batch, const RTRay *rays, const uint32_t *ray_offsets,
+                                     size_t ray_count, RTHit *hits, uint32_t flags, void *stream);
+ResultCode rtbvh_gpu_occluded_batch(RTBatch batch, const RTRay *rays, const uint32_t *ray_offsets,
+                                    size_t ray_count, uint8_t *occluded, uint32_t flags,
+                                    void *stream);
+void rtbvh_gpu_free_batch(RTBatch batch);
+
 /* Any-hit (shadow) traversal: occluded[i] = 1 when some triangle is hit with
  * t_min < t < t_max, else 0 (the FFI's early-out callback, rtbvh_ffi/src/
  * lib.rs:571-576). */

@@ -786,9 +786,9 @@ struct BinnedBuilder {
     size_t max_leaf_size = 1;  // binned_sah.rs:315
     float traversal_cost = 1.0f;  // binned_sah.rs:316
     // contained = true fixes two reference defects so that every child box
-    // lies inside its parent: Q2 (left box from best[best_axis].count after
-    // the fallback replaced best_axis, binned_sah.rs:188,196,232-235) and Q3
-    // (leaves padded a second time, binned_sah.rs:133,135).
+    // lies inside its parent: the stale left-box count (left box from best[best_axis].count after
+    // the fallback replaced best_axis, binned_sah.rs:188,196,232-235) and the
+    // double leaf pad (leaves padded a second time, binned_sah.rs:133,135).
     bool contained = false;
     std::vector<RefAabb>* nodes;
     uint32_t* prim_indices;
@@ -901,7 +901,7 @@ struct BinnedBuilder {
             node.extra2 = int32_t(l);
             node.extra1 = -1;
             RefAabb left_box = aabb_new(), right_box = aabb_new();
-            // Q2: the left box spans bins [0, best[best_axis].count) where
+            // Stale left-box count: the left box spans bins [0, best[best_axis].count) where
             // best_axis may have been replaced by the fallback axis.
             const size_t left_end = contained ? split_index : size_t(best[best_axis].count);
             for (size_t i = 0; i < left_end; ++i) grow_bb(left_box, bins[best_axis][i].aabb);
@@ -988,7 +988,7 @@ void merge_nodes(size_t m_index, size_t cur, const RefAabb* pool, size_t pool_le
             if (c.extra2 < 0) continue;  // get_left_first() is None
             const int32_t lf = c.extra2;
             if (c.extra1 >= 0) {
-                // leaf child: one slot keeps the parent's bounds (Q8)
+                // leaf child: one slot keeps the parent's bounds (mbvh_node.rs:305,319-321)
                 nodes[s0] = lf;
                 leafs[s0] = c.extra1;
             } else {

@@ -84,8 +84,9 @@ size_t ref_build_locb_radius(const RefAabb* aabbs, const float* centers,
 size_t ref_build_binned_sah(const RefAabb* aabbs, const float* centers,
                             size_t center_stride_floats, size_t n, size_t prims_per_leaf,
                             RefAabb* nodes, uint32_t* indices);
-/* The same with the reference's Q2/Q3 box defects fixed (every child box
- * inside its parent); the topology rule is unchanged. */
+/* The same with the reference's two box defects fixed (stale left-box count,
+ * binned_sah.rs:188-235; double leaf pad, :133,135), so every child box lies
+ * inside its parent; the topology rule is unchanged. */
 size_t ref_build_binned_sah_contained(const RefAabb* aabbs, const float* centers,
                                       size_t center_stride_floats, size_t n,
                                       size_t prims_per_leaf, RefAabb* nodes, uint32_t* indices);

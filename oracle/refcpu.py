@@ -167,7 +167,8 @@ def build_locb(aabbs, centers, radius=14, return_iterations=False):
 
 def build_binned_sah(aabbs, centers, prims_per_leaf=1, contained=False):
     """BinnedSahBuilder::build, src/builders/binned_sah.rs:346-400.
-    contained=True fixes the reference's Q2/Q3 box defects."""
+    contained=True fixes the reference's two box defects (stale left-box
+    count, binned_sah.rs:188-235; double leaf pad, binned_sah.rs:133,135)."""
     aabbs = _c(aabbs, AABB)
     centers = _c(centers, np.float32).reshape(len(aabbs), -1) if len(aabbs) else np.zeros((0, 3), np.float32)
     n = len(aabbs)

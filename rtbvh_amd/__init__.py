@@ -9,13 +9,17 @@ Host-side mirror of the reference's Rust API (src/bvh.rs):
     bvh.refit(aabbs)
     bvh / mbvh .intersect(vertices, rays)      closest hit on the GPU
     bvh / mbvh .occluded(vertices, rays)       any hit on the GPU
+    mbvh.attach_triangles(vertices); mbvh.intersect_prepared(rays)
+                                               leaf-ordered triangle copy
+    SceneBatch(centres, scene_offsets, aabbs)  many scenes, one build + one launch
 
 Data lives in device memory (torch tensors on a ROCm device); the work runs
 in librtbvh_amd.so through the C ABI of include/rtbvh.h.
 """
 from . import _lib
-from .api import (Bvh, BuildError, Builder, Mbvh, RAY_FLOATS, HIT_FLOATS, hits_prim,
-                  rays_tensor)
+from ._lib import RTBvhError
+from .api import (Bvh, BuildError, Builder, Mbvh, SceneBatch, RAY_FLOATS, HIT_FLOATS,
+                  hits_prim, rays_tensor)
 
-__all__ = ["Builder", "Bvh", "Mbvh", "BuildError", "RAY_FLOATS", "HIT_FLOATS", "hits_prim",
-           "rays_tensor", "_lib"]
+__all__ = ["Builder", "Bvh", "Mbvh", "SceneBatch", "BuildError", "RAY_FLOATS", "HIT_FLOATS", "hits_prim",
+           "rays_tensor", "RTBvhError", "_lib"]

@@ -25,6 +25,7 @@ BINNED_SAH = 1
 TRACE_REFERENCE_ORDER = 0x1
 MISS = 0xFFFFFFFF
 MAX_STACK = 256
+MAX_BATCH_SCENES = 4096
 
 
 class RTAabb(ctypes.Structure):
@@ -41,6 +42,14 @@ class RTBvh(ctypes.Structure):
 class RTMbvh(ctypes.Structure):
     _fields_ = [("id", ctypes.c_uint32), ("node_count", ctypes.c_uint32),
                 ("nodes", ctypes.c_void_p), ("index_count", ctypes.c_uint32),
+                ("indices", ctypes.c_void_p)]
+
+
+class RTBatch(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint32), ("scene_count", ctypes.c_uint32),
+                ("bvh_node_count", ctypes.c_uint32), ("node_count", ctypes.c_uint32),
+                ("bvh_nodes", ctypes.c_void_p), ("nodes", ctypes.c_void_p),
+                ("index_count", ctypes.c_uint32), ("pad_", ctypes.c_uint32),
                 ("indices", ctypes.c_void_p)]
 
 
@@ -66,6 +75,15 @@ SIGNATURES = {
     "rtbvh_gpu_intersect": (_rc, [RTBvh, _vp, _sz, _vp, _sz, _vp, _u32, _vp]),
     "rtbvh_gpu_intersect_mbvh": (_rc, [RTMbvh, _vp, _sz, _vp, _sz, _vp, _u32, _vp]),
     "rtbvh_gpu_intersect_mbvh_stats": (_rc, [RTMbvh, _vp, _sz, _vp, _sz, _vp, _u32, _vp, _vp]),
+    "rtbvh_gpu_mbvh_attach_triangles": (_rc, [RTMbvh, _vp, _sz, _vp]),
+    "rtbvh_gpu_intersect_mbvh_prepared": (_rc, [RTMbvh, _vp, _sz, _vp, _u32, _vp]),
+    "rtbvh_gpu_occluded_mbvh_prepared": (_rc, [RTMbvh, _vp, _sz, _vp, _u32, _vp]),
+    "rtbvh_gpu_adjoint": (_rc, [_vp, _sz, _vp, _vp, _sz, _vp, _vp, _vp, _vp, _vp]),
+    "rtbvh_gpu_create_batch": (_rc, [_vp, _vp, _sz, _vp, _sz, _vp, ctypes.POINTER(RTBatch)]),
+    "rtbvh_gpu_batch_attach_triangles": (_rc, [RTBatch, _vp, _sz, _vp]),
+    "rtbvh_gpu_intersect_batch": (_rc, [RTBatch, _vp, _vp, _sz, _vp, _u32, _vp]),
+    "rtbvh_gpu_occluded_batch": (_rc, [RTBatch, _vp, _vp, _sz, _vp, _u32, _vp]),
+    "rtbvh_gpu_free_batch": (None, [RTBatch]),
     "rtbvh_gpu_occluded": (_rc, [RTBvh, _vp, _sz, _vp, _sz, _vp, _u32, _vp]),
     "rtbvh_gpu_occluded_mbvh": (_rc, [RTMbvh, _vp, _sz, _vp, _sz, _vp, _u32, _vp]),
     "rtbvh_gpu_status": (_rc, [_vp]),

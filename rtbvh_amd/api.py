@@ -252,9 +252,150 @@ class Mbvh(_Tree):
         flags = L.TRACE_REFERENCE_ORDER if reference_order else 0
         return self._trace(L.lib().rtbvh_gpu_occluded_mbvh, vertices, rays, out, flags)
 
+    def attach_triangles(self, vertices):
+        """Device copy of the triangles in leaf order, used by the
+        *_prepared traversals (a snapshot of `vertices`)."""
+        if not self._owned:
+            raise ValueError("attach_triangles needs an MBVH built by this library")
+        _require_device(vertices, "vertices")
+        verts = vertices.reshape(-1)
+        L.check(L.lib().rtbvh_gpu_mbvh_attach_triangles(self._h, _ptr(verts), verts.numel() // 9,
+                                                        _stream(verts)),
+                 "rtbvh_gpu_mbvh_attach_triangles")
+
+    def intersect_prepared(self, rays, reference_order: bool = False, out=None):
+        """intersect() over the attached triangles."""
+        _require_device(rays, "rays")
+        out = torch.empty((rays.shape[0], HIT_FLOATS), dtype=torch.float32,
+                          device=rays.device) if out is None else out
+        flags = L.TRACE_REFERENCE_ORDER if reference_order else 0
+        L.check(L.lib().rtbvh_gpu_intersect_mbvh_prepared(self._h, _ptr(rays), rays.shape[0],
+                                                          _ptr(out), flags, _stream(rays)),
+                "rtbvh_gpu_intersect_mbvh_prepared")
+        return out
+
+    def occluded_prepared(self, rays, reference_order: bool = False, out=None):
+        """occluded() over the attached triangles."""
+        _require_device(rays, "rays")
+        out = torch.empty((rays.shape[0],), dtype=torch.uint8,
+                          device=rays.device) if out is None else out
+        flags = L.TRACE_REFERENCE_ORDER if reference_order else 0
+        L.check(L.lib().rtbvh_gpu_occluded_mbvh_prepared(self._h, _ptr(rays), rays.shape[0],
+                                                         _ptr(out), flags, _stream(rays)),
+                "rtbvh_gpu_occluded_mbvh_prepared")
+        return out
+
     def __del__(self):
         if getattr(self, "_owned", False) and L._lib is not None:
             L._lib.rtbvh_gpu_free_mbvh(self._h)
+            self._owned = False
+
+
+class SceneBatch:
+    """Many scenes of uneven size built and traced together (no counterpart in
+    the reference, which builds one scene per Builder).  The scenes'
+    primitives are concatenated; scene s owns [scene_offsets[s],
+    scene_offsets[s+1]).  Each scene's tree is the PLOC tree of that scene on
+    its own, and intersect() returns the hits of tracing each scene on its
+    own, with scene-local primitive ids."""
+
+    def __init__(self, primitives: torch.Tensor, scene_offsets, aabbs: Optional[torch.Tensor] = None):
+        _require_device(primitives, "primitives")
+        if primitives.dim() != 2 or primitives.shape[1] not in (3, 4):
+            raise ValueError("primitives must be (N, 3) or (N, 4) float32 centres")
+        if aabbs is not None:
+            _require_device(aabbs, "aabbs")
+            if aabbs.shape != (primitives.shape[0], AABB_FLOATS):
+                raise ValueError("aabbs must be (N, 8) RTAabb records")
+        offs = [int(x) for x in scene_offsets]
+        if len(offs) < 2 or offs[-1] != primitives.shape[0]:
+            raise ValueError("scene_offsets must have scene_count + 1 entries ending at N")
+        self.scene_offsets = offs
+        host = (ctypes.c_uint32 * len(offs))(*offs)
+        self._h = L.RTBatch()
+        self.device = primitives.device
+        self._owned = False
+        L.check(L.lib().rtbvh_gpu_create_batch(_ptr(aabbs), _ptr(primitives),
+                                               4 * int(primitives.shape[1]), host, len(offs) - 1,
+                                               _stream(primitives), ctypes.byref(self._h)),
+                "rtbvh_gpu_create_batch")
+        self._owned = True
+
+    @property
+    def scene_count(self) -> int:
+        return int(self._h.scene_count)
+
+    @property
+    def node_count(self) -> int:
+        return int(self._h.node_count)
+
+    @property
+    def bvh_node_count(self) -> int:
+        return int(self._h.bvh_node_count)
+
+    @property
+    def index_count(self) -> int:
+        return int(self._h.index_count)
+
+    def bvh_nodes(self) -> torch.Tensor:
+        """Binary forest as (M, 8) float32 RTBvhNode records (copy); node s is
+        the root of scene s."""
+        return _copy_out(self._h.bvh_nodes, self.bvh_node_count, AABB_FLOATS, torch.float32,
+                         self.device)
+
+    def quad_nodes(self) -> torch.Tensor:
+        """MBVH forest as (M, 32) float32 records (copy); node s is the root of
+        scene s."""
+        return _copy_out(self._h.nodes, self.node_count, MBVH_FLOATS, torch.float32, self.device)
+
+    def indices(self) -> torch.Tensor:
+        """Batch-wide primitive ids in leaf order (int32 copy)."""
+        return _copy_out(self._h.indices, self.index_count, 1, torch.int32, self.device).view(-1)
+
+    def attach_triangles(self, vertices: torch.Tensor):
+        _require_device(vertices, "vertices")
+        verts = vertices.reshape(-1)
+        L.check(L.lib().rtbvh_gpu_batch_attach_triangles(self._h, _ptr(verts), verts.numel() // 9,
+                                                         _stream(verts)),
+                "rtbvh_gpu_batch_attach_triangles")
+
+    @staticmethod
+    def ray_offsets(counts, device) -> torch.Tensor:
+        """Device offsets (scene_count + 1 uint32 words, held in an int32
+        tensor) from per-scene ray counts."""
+        import numpy as np
+
+        off = np.zeros(len(counts) + 1, dtype=np.uint64)
+        off[1:] = np.cumsum(np.asarray(list(counts), dtype=np.uint64))
+        if int(off[-1]) > 0xFFFFFFFF:
+            raise ValueError("a batch traces at most 2^32-1 rays")
+        return torch.from_numpy(off.astype(np.uint32).view(np.int32)).to(device)
+
+    def _trace(self, fn, rays, ray_offsets, out):
+        _require_device(rays, "rays")
+        if rays.dim() != 2 or rays.shape[1] != RAY_FLOATS:
+            raise ValueError("rays must be (R, 8) float32")
+        if not isinstance(ray_offsets, torch.Tensor) or ray_offsets.device != rays.device \
+                or ray_offsets.element_size() != 4 or ray_offsets.numel() != self.scene_count + 1:
+            raise ValueError("ray_offsets must be a 32-bit device tensor of scene_count + 1 entries")
+        L.check(fn(self._h, _ptr(rays), _ptr(ray_offsets.contiguous()), rays.shape[0], _ptr(out),
+                   0, _stream(rays)), fn.__name__)
+        return out
+
+    def intersect(self, rays: torch.Tensor, ray_offsets: torch.Tensor, out=None):
+        """Closest hit per ray (lowest (t, id) rule), scene-local prim ids."""
+        out = torch.empty((rays.shape[0], HIT_FLOATS), dtype=torch.float32,
+                          device=rays.device) if out is None else out
+        return self._trace(L.lib().rtbvh_gpu_intersect_batch, rays, ray_offsets, out)
+
+    def occluded(self, rays: torch.Tensor, ray_offsets: torch.Tensor, out=None):
+        out = torch.empty((rays.shape[0],), dtype=torch.uint8,
+                          device=rays.device) if out is None else out
+        return self._trace(L.lib().rtbvh_gpu_occluded_batch, rays, ray_offsets, out)
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and L._lib is not None:
+            L._lib.rtbvh_gpu_free_batch(self._h)
             self._owned = False
 
 

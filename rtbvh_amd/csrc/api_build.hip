@@ -35,10 +35,19 @@ struct MbvhEntry {
     std::vector<uint32_t> host_indices;
 };
 
+struct BatchEntry {
+    DeviceBvh bvh;    // forest, roots [0, scenes)
+    DeviceMbvh mbvh;  // forest, roots [0, scenes)
+    uint32_t* scene_off = nullptr;  // device, scenes + 1 primitive offsets
+    uint32_t scenes = 0;
+    int device = 0;
+};
+
 struct Registry {
     std::shared_mutex mu;
     std::vector<std::unique_ptr<BvhEntry>> bvhs;    // null = freed
     std::vector<std::unique_ptr<MbvhEntry>> mbvhs;
+    std::vector<std::unique_ptr<BatchEntry>> batches;
 };
 
 Registry& reg() {
@@ -56,7 +65,15 @@ void release(DeviceBvh& d) {
 void release(DeviceMbvh& d) {
     if (d.nodes) (void)hipFree(d.nodes);
     if (d.indices) (void)hipFree(d.indices);
+    if (d.tris) (void)hipFree(d.tris);
     d = DeviceMbvh{};
+}
+
+void release(BatchEntry& b) {
+    release(b.bvh);
+    release(b.mbvh);
+    if (b.scene_off) (void)hipFree(b.scene_off);
+    b.scene_off = nullptr;
 }
 
 // Switches to `device` for the lifetime of the guard.
@@ -202,6 +219,35 @@ MbvhEntry* find_mbvh(uint32_t id) {
     return r.mbvhs[id].get();
 }
 
+BatchEntry* find_batch(uint32_t id) {
+    Registry& r = reg();
+    if (id >= r.batches.size()) return nullptr;
+    return r.batches[id].get();
+}
+
+RTBatch handle_of(uint32_t id, const BatchEntry& e) {
+    RTBatch h{};
+    h.id = id;
+    h.scene_count = e.scenes;
+    h.bvh_node_count = e.bvh.node_count;
+    h.node_count = e.mbvh.node_count;
+    h.bvh_nodes = reinterpret_cast<const RTBvhNode*>(e.bvh.nodes);
+    h.nodes = reinterpret_cast<const RTMbvhNode*>(e.mbvh.nodes);
+    h.index_count = e.mbvh.prim_count;
+    h.indices = e.mbvh.indices;
+    return h;
+}
+
+bool is_device_pointer(const void* p) {
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
+           attr.type == hipMemoryTypeUnified;
+}
+
 ResultCode collapse_entry(uint32_t id, void* stream, bool host, RTMbvh* out) {
     std::unique_ptr<MbvhEntry> me(new MbvhEntry());
     {
@@ -234,6 +280,34 @@ ResultCode collapse_entry(uint32_t id, void* stream, bool host, RTMbvh* out) {
 
 }  // namespace
 
+// Attached leaf-ordered triangles of an MBVH made by this library (null if
+// none); *count receives the number of index entries they cover.
+const float4* attached_triangles(uint32_t id, uint32_t* count, int* device) {
+    Registry& r = reg();
+    std::shared_lock<std::shared_mutex> lk(r.mu);
+    MbvhEntry* me = find_mbvh(id);
+    if (me == nullptr || me->dev.tris == nullptr) return nullptr;
+    *count = me->dev.prim_count;
+    *device = me->device;
+    return me->dev.tris;
+}
+
+// The arrays a batched traversal needs (false if the id is unknown).
+bool batch_view(uint32_t id, BatchView* out) {
+    Registry& r = reg();
+    std::shared_lock<std::shared_mutex> lk(r.mu);
+    BatchEntry* be = find_batch(id);
+    if (be == nullptr) return false;
+    out->nodes = be->mbvh.nodes;
+    out->node_count = be->mbvh.node_count;
+    out->indices = be->mbvh.indices;
+    out->index_count = be->mbvh.prim_count;
+    out->tris = be->mbvh.tris;
+    out->scenes = be->scenes;
+    out->device = be->device;
+    return true;
+}
+
 }  // namespace rtbvh
 
 using namespace rtbvh;
@@ -252,6 +326,10 @@ ResultCode rtbvh_gpu_create_bvh(const RTAabb* aabbs, size_t prim_count, const fl
     std::unique_ptr<BvhEntry> e(new BvhEntry());
     rc = current_device(&e->device);
     if (rc != Ok) return rc;
+    if (!is_device_pointer(centers) || (aabbs && !is_device_pointer(aabbs))) {
+        set_error("centers and aabbs must be device memory");
+        return InvalidArgument;
+    }
     rc = build_device(aabbs, prim_count, centers, center_stride, prims_per_leaf, bvh_type,
                       static_cast<hipStream_t>(stream), &e->dev);
     if (rc != Ok) return rc;
@@ -283,6 +361,152 @@ ResultCode rtbvh_gpu_refit(const RTAabb* aabbs, RTBvh bvh, void* stream) {
     }
     DeviceGuard g(be->device);
     return gpu_refit(be->dev, aabbs, static_cast<hipStream_t>(stream));
+}
+
+ResultCode rtbvh_gpu_mbvh_attach_triangles(RTMbvh mbvh, const float* vertices, size_t prim_count,
+                                           void* stream) {
+    if (vertices == nullptr && prim_count > 0) {
+        set_error("vertices must not be null");
+        return Error;
+    }
+    if (prim_count > 0xFFFFFFFFull) {
+        set_error("prim_count %zu exceeds 2^32-1", prim_count);
+        return InvalidArgument;
+    }
+    Registry& r = reg();
+    std::unique_lock<std::shared_mutex> lk(r.mu);
+    MbvhEntry* me = find_mbvh(mbvh.id);
+    if (me == nullptr) {
+        set_error("unknown mbvh id %u", mbvh.id);
+        return Error;
+    }
+    DeviceGuard g(me->device);
+    hipPointerAttribute_t attr;
+    if (prim_count > 0 && (hipPointerGetAttributes(&attr, vertices) != hipSuccess ||
+                           !(attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
+                             attr.type == hipMemoryTypeUnified))) {
+        (void)hipGetLastError();
+        set_error("vertices (%p) is not device memory", static_cast<const void*>(vertices));
+        return InvalidArgument;
+    }
+    const uint32_t entries = me->dev.prim_count;
+    if (me->dev.tris == nullptr && entries > 0) {
+        hipError_t e = hipMalloc(&me->dev.tris, size_t(entries) * 3 * sizeof(float4));
+        if (e != hipSuccess) {
+            me->dev.tris = nullptr;
+            return hip_fail(e, "hipMalloc(triangles)");
+        }
+    }
+    hipError_t e = launch_prepare_triangles(me->dev.indices, entries, vertices,
+                                            uint32_t(prim_count), me->dev.tris,
+                                            static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "prepare triangles");
+    return Ok;
+}
+
+ResultCode rtbvh_gpu_create_batch(const RTAabb* aabbs, const float* centers, size_t center_stride,
+                                  const uint32_t* scene_offsets, size_t scene_count, void* stream,
+                                  RTBatch* result) {
+    if (centers == nullptr || result == nullptr || scene_offsets == nullptr) {
+        set_error("centers, scene_offsets and result must not be null");
+        return Error;
+    }
+    if (center_stride != 12 && center_stride != 16) {
+        set_error("center_stride must be 12 or 16 bytes, got %zu", center_stride);
+        return InvalidArgument;
+    }
+    if (scene_count == 0) {
+        set_error("no scenes");
+        return NoPrimitives;
+    }
+    if (scene_count > RTBVH_MAX_BATCH_SCENES) {
+        set_error("a batch holds at most %u scenes (got %zu)", RTBVH_MAX_BATCH_SCENES,
+                  scene_count);
+        return InvalidArgument;
+    }
+    std::unique_ptr<BatchEntry> be(new BatchEntry());
+    ResultCode rc = current_device(&be->device);
+    if (rc != Ok) return rc;
+    if (!is_device_pointer(centers) || (aabbs && !is_device_pointer(aabbs))) {
+        set_error("centers and aabbs must be device memory");
+        return InvalidArgument;
+    }
+    const uint32_t scenes = uint32_t(scene_count);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    rc = gpu_build_ploc_batch(aabbs, centers, center_stride / 4, scene_offsets, scenes, st,
+                              &be->bvh);
+    if (rc != Ok) return rc;
+    be->scenes = scenes;
+    rc = gpu_build_mbvh(be->bvh, st, &be->mbvh);
+    if (rc != Ok) {
+        release(*be);
+        return rc;
+    }
+    hipError_t e = hipMalloc(&be->scene_off, size_t(scenes + 1) * 4);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(be->scene_off, scene_offsets, size_t(scenes + 1) * 4,
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        release(*be);
+        return hip_fail(e, "scene offsets");
+    }
+    Registry& r = reg();
+    std::unique_lock<std::shared_mutex> lk(r.mu);
+    r.batches.push_back(std::move(be));
+    const uint32_t id = uint32_t(r.batches.size() - 1);
+    *result = handle_of(id, *r.batches.back());
+    return Ok;
+}
+
+ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch batch, const float* vertices,
+                                            size_t prim_count, void* stream) {
+    if (vertices == nullptr) {
+        set_error("vertices must not be null");
+        return Error;
+    }
+    Registry& r = reg();
+    std::unique_lock<std::shared_mutex> lk(r.mu);
+    BatchEntry* be = find_batch(batch.id);
+    if (be == nullptr) {
+        set_error("unknown batch id %u", batch.id);
+        return Error;
+    }
+    if (prim_count != be->mbvh.prim_count) {
+        set_error("batch has %u primitives, got vertices for %zu", be->mbvh.prim_count,
+                  prim_count);
+        return InvalidArgument;
+    }
+    DeviceGuard g(be->device);
+    if (!is_device_pointer(vertices)) {
+        set_error("vertices (%p) is not device memory", static_cast<const void*>(vertices));
+        return InvalidArgument;
+    }
+    const uint32_t entries = be->mbvh.prim_count;
+    if (be->mbvh.tris == nullptr) {
+        hipError_t e = hipMalloc(&be->mbvh.tris, size_t(entries) * 3 * sizeof(float4));
+        if (e != hipSuccess) {
+            be->mbvh.tris = nullptr;
+            return hip_fail(e, "hipMalloc(triangles)");
+        }
+    }
+    hipError_t e = launch_prepare_triangles(be->mbvh.indices, entries, vertices, entries,
+                                            be->mbvh.tris, static_cast<hipStream_t>(stream),
+                                            be->scene_off, be->scenes);
+    if (e != hipSuccess) return hip_fail(e, "prepare triangles");
+    return Ok;
+}
+
+void rtbvh_gpu_free_batch(RTBatch batch) {
+    Registry& r = reg();
+    std::unique_lock<std::shared_mutex> lk(r.mu);
+    if (batch.id >= r.batches.size() || !r.batches[batch.id]) {
+        fprintf(stderr, "Could not free batch with id: %u\n", batch.id);
+        return;
+    }
+    DeviceGuard g(r.batches[batch.id]->device);
+    release(*r.batches[batch.id]);
+    r.batches[batch.id].reset();
 }
 
 void rtbvh_gpu_free_bvh(RTBvh bvh) { free_bvh(bvh); }

@@ -73,7 +73,8 @@ namespace {
 ResultCode trace_entry(const void* nodes, uint32_t node_count, const uint32_t* indices,
                        uint32_t index_count, const float* vertices, size_t prim_count,
                        const RTRay* rays, size_t ray_count, RTHit* hits, uint8_t* occluded,
-                       uint32_t flags, void* stream, bool mbvh, uint64_t* stats = nullptr) {
+                       uint32_t flags, void* stream, bool mbvh, uint64_t* stats = nullptr,
+                       const float4* tris = nullptr) {
     if (flags & ~RTBVH_TRACE_REFERENCE_ORDER) {
         set_error("unknown trace flags 0x%x", flags);
         return InvalidArgument;
@@ -91,7 +92,7 @@ ResultCode trace_entry(const void* nodes, uint32_t node_count, const uint32_t* i
     if (node_count > 0) {
         if ((rc = check_device(nodes, "nodes")) != Ok) return rc;
         if (index_count > 0 && (rc = check_device(indices, "indices")) != Ok) return rc;
-        if (prim_count > 0 && (rc = check_device(vertices, "vertices")) != Ok) return rc;
+        if (!tris && prim_count > 0 && (rc = check_device(vertices, "vertices")) != Ok) return rc;
     }
     if ((reinterpret_cast<uintptr_t>(rays) & 15) || (hits && (reinterpret_cast<uintptr_t>(hits) & 15)) ||
         (reinterpret_cast<uintptr_t>(nodes) & 15)) {
@@ -111,6 +112,7 @@ ResultCode trace_entry(const void* nodes, uint32_t node_count, const uint32_t* i
     p.hits = reinterpret_cast<DevHit*>(hits);
     p.occluded = occluded;
     p.stats = stats;
+    p.tris = tris;
     p.status = device_status_ptr();
     if (p.status == nullptr) {
         set_error("device status word unavailable");
@@ -119,6 +121,90 @@ ResultCode trace_entry(const void* nodes, uint32_t node_count, const uint32_t* i
     const hipError_t e = launch_trace(p, mbvh, (flags & RTBVH_TRACE_REFERENCE_ORDER) != 0,
                                       occluded != nullptr, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "trace launch");
+    return Ok;
+}
+
+ResultCode prepared_entry(RTMbvh mbvh, const RTRay* rays, size_t ray_count, RTHit* hits,
+                          uint8_t* occluded, uint32_t flags, void* stream) {
+    uint32_t count = 0;
+    int device = 0;
+    const float4* tris = attached_triangles(mbvh.id, &count, &device);
+    if (tris == nullptr) {
+        set_error("mbvh %u has no attached triangles (rtbvh_gpu_mbvh_attach_triangles)", mbvh.id);
+        return Error;
+    }
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != device) {
+        set_error("mbvh %u lives on device %d, current device is %d", mbvh.id, device, cur);
+        return InvalidArgument;
+    }
+    if (count != mbvh.index_count) {
+        set_error("handle does not match the attached triangles");
+        return InvalidArgument;
+    }
+    return trace_entry(mbvh.nodes, mbvh.node_count, mbvh.indices, mbvh.index_count, nullptr,
+                       count, rays, ray_count, hits, occluded, flags, stream, true, nullptr, tris);
+}
+
+ResultCode batch_entry(RTBatch batch, const RTRay* rays, const uint32_t* ray_offsets,
+                       size_t ray_count, RTHit* hits, uint8_t* occluded, uint32_t flags,
+                       void* stream) {
+    if (flags != 0) {
+        set_error("batched traversal supports the lowest-(t, id) rule only (flags 0)");
+        return InvalidArgument;
+    }
+    if (ray_count > 0xFFFFFFFFull) {
+        set_error("a batch traces at most 2^32-1 rays (got %zu)", ray_count);
+        return InvalidArgument;
+    }
+    BatchView v;
+    if (!batch_view(batch.id, &v)) {
+        set_error("unknown batch id %u", batch.id);
+        return Error;
+    }
+    if (v.tris == nullptr) {
+        set_error("batch %u has no attached triangles (rtbvh_gpu_batch_attach_triangles)",
+                  batch.id);
+        return Error;
+    }
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != v.device) {
+        set_error("batch %u lives on device %d, current device is %d", batch.id, v.device, cur);
+        return InvalidArgument;
+    }
+    if (ray_count == 0) return Ok;
+    ResultCode rc;
+    if (ray_offsets == nullptr) {
+        set_error("ray_offsets must not be null");
+        return Error;
+    }
+    if ((rc = check_device(ray_offsets, "ray_offsets")) != Ok) return rc;
+    if ((rc = check_device(rays, "rays")) != Ok) return rc;
+    if (hits && (rc = check_device(hits, "hits")) != Ok) return rc;
+    if (occluded && (rc = check_device(occluded, "occluded")) != Ok) return rc;
+    if ((reinterpret_cast<uintptr_t>(rays) & 15) || (hits && (reinterpret_cast<uintptr_t>(hits) & 15))) {
+        set_error("rays and hits must be 16-byte aligned");
+        return InvalidArgument;
+    }
+    TraceParams p{};
+    p.mnodes = v.nodes;
+    p.node_count = v.node_count;
+    p.index_count = v.index_count;
+    p.indices = v.indices;
+    p.tris = v.tris;
+    p.rays = reinterpret_cast<const DevRay*>(rays);
+    p.ray_count = ray_count;
+    p.hits = reinterpret_cast<DevHit*>(hits);
+    p.occluded = occluded;
+    p.ray_offsets = ray_offsets;
+    p.scene_count = v.scenes;
+    p.status = device_status_ptr();
+    if (p.status == nullptr) {
+        set_error("device status word unavailable");
+        return DeviceError;
+    }
+    const hipError_t e = launch_trace_fast(p, occluded != nullptr, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "batch trace launch");
     return Ok;
 }
 
@@ -162,6 +248,81 @@ ResultCode rtbvh_gpu_intersect_mbvh_stats(RTMbvh mbvh, const float* vertices, si
     }
     return trace_entry(mbvh.nodes, mbvh.node_count, mbvh.indices, mbvh.index_count, vertices,
                        prim_count, rays, ray_count, hits, nullptr, flags, stream, true, counters);
+}
+
+ResultCode rtbvh_gpu_intersect_mbvh_prepared(RTMbvh mbvh, const RTRay* rays, size_t ray_count,
+                                             RTHit* hits, uint32_t flags, void* stream) {
+    if (hits == nullptr && ray_count > 0) {
+        set_error("hits is null");
+        return Error;
+    }
+    return prepared_entry(mbvh, rays, ray_count, hits, nullptr, flags, stream);
+}
+
+ResultCode rtbvh_gpu_adjoint(const float* vertices, size_t prim_count, const RTRay* rays,
+                             const RTHit* hits, size_t ray_count, const float* grad_tuv,
+                             float* grad_vertices, float* grad_origin, float* grad_direction,
+                             void* stream) {
+    if (prim_count > 0xFFFFFFFFull) {
+        set_error("prim_count %zu exceeds 2^32-1", prim_count);
+        return InvalidArgument;
+    }
+    if (ray_count == 0) return Ok;
+    if (rays == nullptr || hits == nullptr || grad_tuv == nullptr ||
+        (vertices == nullptr && prim_count > 0)) {
+        set_error("vertices, rays, hits and grad_tuv must not be null");
+        return Error;
+    }
+    ResultCode rc;
+    if ((rc = check_device(rays, "rays")) != Ok || (rc = check_device(hits, "hits")) != Ok ||
+        (rc = check_device(grad_tuv, "grad_tuv")) != Ok)
+        return rc;
+    if (prim_count > 0 && (rc = check_device(vertices, "vertices")) != Ok) return rc;
+    if (grad_vertices && (rc = check_device(grad_vertices, "grad_vertices")) != Ok) return rc;
+    if (grad_origin && (rc = check_device(grad_origin, "grad_origin")) != Ok) return rc;
+    if (grad_direction && (rc = check_device(grad_direction, "grad_direction")) != Ok) return rc;
+    if ((reinterpret_cast<uintptr_t>(rays) & 15) || (reinterpret_cast<uintptr_t>(hits) & 15)) {
+        set_error("rays and hits must be 16-byte aligned");
+        return InvalidArgument;
+    }
+    if (ray_count > (uint64_t(0x7FFFFFFF) * 256)) {
+        set_error("too many rays (%zu)", ray_count);
+        return InvalidArgument;
+    }
+    const hipError_t e = launch_adjoint(
+        vertices, uint32_t(prim_count), reinterpret_cast<const DevRay*>(rays),
+        reinterpret_cast<const DevHit*>(hits), ray_count, grad_tuv, grad_vertices, grad_origin,
+        grad_direction, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "adjoint launch");
+    return Ok;
+}
+
+ResultCode rtbvh_gpu_intersect_batch(RTBatch batch, const RTRay* rays, const uint32_t* ray_offsets,
+                                     size_t ray_count, RTHit* hits, uint32_t flags, void* stream) {
+    if (hits == nullptr && ray_count > 0) {
+        set_error("hits is null");
+        return Error;
+    }
+    return batch_entry(batch, rays, ray_offsets, ray_count, hits, nullptr, flags, stream);
+}
+
+ResultCode rtbvh_gpu_occluded_batch(RTBatch batch, const RTRay* rays, const uint32_t* ray_offsets,
+                                    size_t ray_count, uint8_t* occluded, uint32_t flags,
+                                    void* stream) {
+    if (occluded == nullptr && ray_count > 0) {
+        set_error("occluded is null");
+        return Error;
+    }
+    return batch_entry(batch, rays, ray_offsets, ray_count, nullptr, occluded, flags, stream);
+}
+
+ResultCode rtbvh_gpu_occluded_mbvh_prepared(RTMbvh mbvh, const RTRay* rays, size_t ray_count,
+                                            uint8_t* occluded, uint32_t flags, void* stream) {
+    if (occluded == nullptr && ray_count > 0) {
+        set_error("occluded is null");
+        return Error;
+    }
+    return prepared_entry(mbvh, rays, ray_count, nullptr, occluded, flags, stream);
 }
 
 ResultCode rtbvh_gpu_occluded(RTBvh bvh, const float* vertices, size_t prim_count,

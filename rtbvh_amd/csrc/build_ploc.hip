@@ -153,8 +153,11 @@ __global__ void bounds_partial_kernel(const DevAabb* aabbs, uint32_t n, Bounds6*
 
 // World bounds, second level (one block over the partials, in order), then
 // Aabb::with_offset(1e-4).
+// One block per scene: block b reduces partial[b*parts, (b+1)*parts).
 __global__ void bounds_final_kernel(const Bounds6* partial, uint32_t parts, Bounds6* world) {
     __shared__ Bounds6 sh[kBlock];
+    partial += size_t(blockIdx.x) * parts;
+    world += blockIdx.x;
     Bounds6 acc;
     for (int k = 0; k < 3; ++k) {
         acc.mn[k] = 1e34f;
@@ -211,11 +214,13 @@ __device__ __forceinline__ uint32_t grid_coord(float g) {
     return uint32_t(q);
 }
 
+template <bool SEG>
 __global__ void morton_kernel(const float* centers, uint32_t stride, uint32_t n,
-                              const Bounds6* world, uint32_t* keys, uint32_t* vals) {
+                              const Bounds6* world, const uint32_t* prim_scene, uint32_t* keys,
+                              uint32_t* vals) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const Bounds6 w = *world;
+    const Bounds6 w = world[SEG ? prim_scene[i] : 0u];
     uint32_t x[3];
     for (int k = 0; k < 3; ++k) {
         // MortonEncoder::new: wtg = 1024 * (1 / diag); off = (-min) * wtg
@@ -226,6 +231,75 @@ __global__ void morton_kernel(const float* centers, uint32_t stride, uint32_t n,
     }
     keys[i] = part1by2(x[0]) | (part1by2(x[1]) << 1) | (part1by2(x[2]) << 2);
     vals[i] = i;
+}
+
+// ------------------------------------------------------------ scene batches --
+// A batch is scene_count scenes whose primitives are concatenated; scene s
+// owns primitives [off[s], off[s+1]).  Each scene gets its own world bounds
+// and Morton grid, the sort orders by (scene, code), and clustering never
+// pairs clusters of different scenes, so every scene's tree is the one a
+// build of that scene alone produces (node numbering aside).
+
+// prim_scene[i] = s with off[s] <= i < off[s+1].
+__global__ void scene_of_kernel(const uint32_t* off, uint32_t scenes, uint32_t n,
+                                uint32_t* prim_scene) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t lo = 0, hi = scenes;  // off[lo] <= i < off[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    prim_scene[i] = lo;
+}
+
+// Per-scene world bounds, first level: blockIdx.y is the scene, blockIdx.x
+// one of gridDim.x slices of it.
+__global__ void bounds_seg_partial_kernel(const DevAabb* aabbs, const uint32_t* off,
+                                          Bounds6* partial) {
+    __shared__ Bounds6 sh[kBlock];
+    Bounds6 acc;
+    for (int k = 0; k < 3; ++k) {
+        acc.mn[k] = 1e34f;
+        acc.mx[k] = -1e34f;
+    }
+    const uint32_t s = blockIdx.y;
+    const uint32_t b = off[s], e = off[s + 1];
+    for (uint32_t i = b + blockIdx.x * kBlock + threadIdx.x; i < e; i += gridDim.x * kBlock) {
+        DevAabb a;
+        load_node(aabbs + i, a);
+        for (int k = 0; k < 3; ++k) {
+            acc.mn[k] = rs_min(acc.mn[k], a.min[k]);
+            acc.mx[k] = rs_max(acc.mx[k], a.max[k]);
+        }
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if (int(threadIdx.x) < h) {
+            for (int k = 0; k < 3; ++k) {
+                sh[threadIdx.x].mn[k] = rs_min(sh[threadIdx.x].mn[k], sh[threadIdx.x + h].mn[k]);
+                sh[threadIdx.x].mx[k] = rs_max(sh[threadIdx.x].mx[k], sh[threadIdx.x + h].mx[k]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[size_t(s) * gridDim.x + blockIdx.x] = sh[0];
+}
+
+// Scene keys for the final sort passes: keys[i] = scene of vals[i].
+__global__ void scene_keys_kernel(const uint32_t* vals, const uint32_t* prim_scene, uint32_t n,
+                                  uint32_t* keys) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = prim_scene[vals[i]];
+}
+
+__global__ void copy_words_kernel(const uint32_t* src, uint32_t* dst, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 // ------------------------------------------------------------ radix sort --
@@ -297,12 +371,15 @@ __global__ void __launch_bounds__(kSortBlock)
 // ---------------------------------------------------------------- leaves --
 // locb.rs:289-294: leaf i = aabbs[prim_indices[i]].with_offset(1e-4), count 1,
 // left_first i.
+template <bool SEG>
 __global__ void leaves_kernel(const DevAabb* aabbs, const uint32_t* sorted, uint32_t n,
-                              DevAabb* clusters) {
+                              const uint32_t* prim_scene, DevAabb* clusters,
+                              uint32_t* cluster_scene) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const uint32_t prim = sorted[i];
     DevAabb a;
-    load_node(aabbs + sorted[i], a);
+    load_node(aabbs + prim, a);
     for (int k = 0; k < 3; ++k) {
         a.min[k] = a.min[k] - 0.0001f;
         a.max[k] = a.max[k] + 0.0001f;
@@ -310,6 +387,7 @@ __global__ void leaves_kernel(const DevAabb* aabbs, const uint32_t* sorted, uint
     a.extra1 = 1;
     a.extra2 = int32_t(i);
     store_node(clusters + i, a);
+    if (SEG) cluster_scene[i] = prim_scene[prim];
 }
 
 // n <= 2: a single root leaf with the world bounds (locb.rs:258-269).
@@ -350,18 +428,22 @@ __device__ __forceinline__ void nn_window(uint32_t i, uint32_t n, uint32_t& sb, 
 // Nearest neighbour + mutual marks for one tile.  Bounds are staged in LDS
 // as six arrays so that lanes reading consecutive clusters hit consecutive
 // banks.
+template <bool SEG>
 __global__ void __launch_bounds__(kBlock)
     nn_mark_kernel(const PlocState* state, const DevAabb* cl0, const DevAabb* cl1,
-                   uint32_t* nb_out, uint32_t* mark_out, uint32_t* tile_count) {
+                   const uint32_t* cs0, const uint32_t* cs1, uint32_t* nb_out,
+                   uint32_t* mark_out, uint32_t* tile_count) {
     constexpr int kHalo = 2 * kRadius;
     constexpr int kSpan = kTile + 2 * kHalo;
     __shared__ float s_b[6][kSpan];
+    __shared__ uint32_t s_sc[SEG ? kSpan : 1];
     __shared__ uint32_t s_nb[kTile + 2 * kRadius];
     __shared__ uint32_t s_total;
     const PlocState stt = *state;
     if (stt.n_active <= uint32_t(kTailMax)) return;
     const uint32_t n = stt.n_active;
     const DevAabb* cl = stt.buf ? cl1 : cl0;
+    const uint32_t* cs = stt.buf ? cs1 : cs0;
     const uint32_t ts = blockIdx.x * kTile;
     if (ts >= n) return;
     const uint32_t te = ts + kTile < n ? ts + kTile : n;
@@ -377,6 +459,7 @@ __global__ void __launch_bounds__(kBlock)
         s_b[3][k - lb] = a.max[0];
         s_b[4][k - lb] = a.max[1];
         s_b[5][k - lb] = a.max[2];
+        if (SEG) s_sc[k - lb] = cs[k];
     }
     __syncthreads();
     const uint32_t nb0 = ts > uint32_t(kRadius) ? ts - kRadius : 0u;
@@ -392,6 +475,7 @@ __global__ void __launch_bounds__(kBlock)
         for (uint32_t j = s0; j < s1; ++j) {
             if (j == i) continue;
             const uint32_t lj = j - lb;
+            if (SEG && s_sc[lj] != s_sc[li]) continue;  // other scene
             const float d = dist6(a0, a1, a2, a3, a4, a5, s_b[0][lj], s_b[1][lj], s_b[2][lj],
                                   s_b[3][lj], s_b[4][lj], s_b[5][lj]);
             if (d < best) {
@@ -466,9 +550,11 @@ __global__ void __launch_bounds__(1024) tile_scan_kernel(PlocState* states, uint
 
 // Parents, children and compaction for one tile (locb.rs:193-215).  m[k] is
 // the inclusive prefix of marks; the tile keeps m for [ts - r, te).
+template <bool SEG>
 __global__ void __launch_bounds__(kBlock)
-    merge_kernel(const PlocState* state, const DevAabb* cl0, DevAabb* cl1, const uint32_t* nb,
-                 const uint32_t* mark, const uint32_t* tile_offset, DevAabb* nodes) {
+    merge_kernel(const PlocState* state, const DevAabb* cl0, DevAabb* cl1, const uint32_t* cs0,
+                 uint32_t* cs1, const uint32_t* nb, const uint32_t* mark,
+                 const uint32_t* tile_offset, DevAabb* nodes) {
     __shared__ uint32_t s_m[kRadius + kTile];
     __shared__ uint32_t s_scan[kBlock];
     const PlocState stt = *state;
@@ -476,6 +562,8 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t n = stt.n_active;
     const DevAabb* cl = stt.buf ? cl1 : cl0;
     DevAabb* cl_out = stt.buf ? const_cast<DevAabb*>(cl0) : cl1;
+    const uint32_t* cs = stt.buf ? cs1 : cs0;
+    uint32_t* cs_out = stt.buf ? const_cast<uint32_t*>(cs0) : cs1;
     const uint32_t ts = blockIdx.x * kTile;
     if (ts >= n) return;
     const uint32_t te = ts + kTile < n ? ts + kTile : n;
@@ -528,6 +616,7 @@ __global__ void __launch_bounds__(kBlock)
                 parent.extra1 = -1;
                 parent.extra2 = int32_t(first_child);
                 store_node(cl_out + (i - mi), parent);
+                if (SEG) cs_out[i - mi] = cs[i];
                 store_node(nodes + first_child, cj);
                 store_node(nodes + first_child + 1, ci);
             }
@@ -535,6 +624,7 @@ __global__ void __launch_bounds__(kBlock)
             DevAabb ci;
             load_node(cl + i, ci);
             store_node(cl_out + (i - mi), ci);
+            if (SEG) cs_out[i - mi] = cs[i];
         }
     }
 }
@@ -546,16 +636,19 @@ __global__ void __launch_bounds__(kBlock)
 constexpr int kTailPer = kTailMax / kTailThreads;
 constexpr int kTailWaves = kTailThreads / 64;
 
+template <bool SEG>
 struct TailShared {
     float b[6][kTailMax];
     int32_t e1[kTailMax];
     int32_t e2[kTailMax];
     uint16_t nb[kTailMax];
     uint16_t m[kTailMax];
+    uint16_t sc[SEG ? kTailMax : 2];  // scene of each cluster (batches)
     uint32_t wsum[kTailPer * kTailWaves];
 };
 
-__device__ __forceinline__ DevAabb tail_node(const TailShared& sh, uint32_t i) {
+template <typename TS>
+__device__ __forceinline__ DevAabb tail_node(const TS& sh, uint32_t i) {
     DevAabb a;
     a.min[0] = sh.b[0][i];
     a.min[1] = sh.b[1][i];
@@ -568,7 +661,8 @@ __device__ __forceinline__ DevAabb tail_node(const TailShared& sh, uint32_t i) {
     return a;
 }
 
-__device__ __forceinline__ void tail_store(TailShared& sh, uint32_t i, const DevAabb& a) {
+template <typename TS>
+__device__ __forceinline__ void tail_store(TS& sh, uint32_t i, const DevAabb& a) {
     sh.b[0][i] = a.min[0];
     sh.b[1][i] = a.min[1];
     sh.b[2][i] = a.min[2];
@@ -579,14 +673,24 @@ __device__ __forceinline__ void tail_store(TailShared& sh, uint32_t i, const Dev
     sh.e2[i] = a.extra2;
 }
 
+// Runs until n_final clusters remain (1, or one per scene of a batch), stores
+// them at nodes[0, n_final) and reports the count it reached in *result.
+template <bool SEG>
 __global__ void __launch_bounds__(kTailThreads)
-    tail_kernel(const PlocState* state, const DevAabb* cl0, const DevAabb* cl1, DevAabb* nodes) {
+    tail_kernel(const PlocState* state, const DevAabb* cl0, const DevAabb* cl1,
+                const uint32_t* cs0, const uint32_t* cs1, DevAabb* nodes, uint32_t n_final,
+                PlocState* result) {
     extern __shared__ unsigned char smem_raw[];
-    TailShared& sh = *reinterpret_cast<TailShared*>(smem_raw);
+    TailShared<SEG>& sh = *reinterpret_cast<TailShared<SEG>*>(smem_raw);
     const PlocState stt = *state;
     uint32_t n = stt.n_active;
     uint32_t end = stt.end;
+    if (n > uint32_t(kTailMax)) {  // the iterations before did not get here
+        if (threadIdx.x == 0) *result = stt;
+        return;
+    }
     const DevAabb* cl_in = stt.buf ? cl1 : cl0;
+    const uint32_t* cs_in = stt.buf ? cs1 : cs0;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
@@ -595,9 +699,10 @@ __global__ void __launch_bounds__(kTailThreads)
         DevAabb a;
         load_node(cl_in + i, a);
         tail_store(sh, i, a);
+        if (SEG) sh.sc[i] = uint16_t(cs_in[i]);
     }
     __syncthreads();
-    while (n > 1) {
+    while (n > n_final) {
         // nearest neighbours
 #pragma unroll
         for (int k = 0; k < kTailPer; ++k) {
@@ -607,10 +712,12 @@ __global__ void __launch_bounds__(kTailThreads)
             nn_window(i, n, s0, s1);
             const float a0 = sh.b[0][i], a1 = sh.b[1][i], a2 = sh.b[2][i];
             const float a3 = sh.b[3][i], a4 = sh.b[4][i], a5 = sh.b[5][i];
+            const uint32_t si = SEG ? sh.sc[i] : 0u;
             float best = 3.40282347e38f;
             uint32_t best_j = 0xFFFFu;
             for (uint32_t j = s0; j < s1; ++j) {
                 if (j == i) continue;
+                if (SEG && sh.sc[j] != si) continue;  // other scene
                 const float d = dist6(a0, a1, a2, a3, a4, a5, sh.b[0][j], sh.b[1][j], sh.b[2][j],
                                       sh.b[3][j], sh.b[4][j], sh.b[5][j]);
                 if (d < best) {
@@ -656,11 +763,13 @@ __global__ void __launch_bounds__(kTailThreads)
         const uint32_t children_begin = end - 2 * merged;
         DevAabb keep[kTailPer];
         uint32_t dst[kTailPer];
+        uint16_t keep_sc[kTailPer];
 #pragma unroll
         for (int k = 0; k < kTailPer; ++k) {
             const uint32_t i = tid + uint32_t(k) * kTailThreads;
             dst[k] = 0xFFFFFFFFu;
             if (i >= n) continue;
+            keep_sc[k] = SEG ? sh.sc[i] : uint16_t(0);
             const uint32_t j = sh.nb[i];
             const bool mutual = j < n && sh.nb[j] == i;
             const uint32_t mi = sh.m[i];
@@ -685,12 +794,22 @@ __global__ void __launch_bounds__(kTailThreads)
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < kTailPer; ++k)
-            if (dst[k] != 0xFFFFFFFFu) tail_store(sh, dst[k], keep[k]);
+            if (dst[k] != 0xFFFFFFFFu) {
+                tail_store(sh, dst[k], keep[k]);
+                if (SEG) sh.sc[dst[k]] = keep_sc[k];
+            }
         __syncthreads();
         n -= merged;
         end -= 2 * merged;
     }
-    if (tid == 0) store_node(nodes, tail_node(sh, 0));
+    if (n == n_final)
+        for (uint32_t i = tid; i < n; i += kTailThreads) store_node(nodes + i, tail_node(sh, i));
+    if (tid == 0) {
+        PlocState r = stt;
+        r.n_active = n;
+        r.end = end;
+        *result = r;
+    }
 }
 
 struct Workspace {
@@ -707,19 +826,15 @@ constexpr int kBatch = 6;  // iterations enqueued between host checks
 
 }  // namespace
 
-ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stride_floats,
-                          size_t n64, int radius, hipStream_t stream, DeviceBvh* out) {
-    if (radius != kRadius) {
-        set_error("PLOC search radius %d not supported (built for %d)", radius, kRadius);
-        return InvalidArgument;
-    }
-    if (n64 == 0) return NoPrimitives;
-    if (n64 > (1u << 30)) {
-        set_error("prim_count %zu too large", n64);
-        return InvalidArgument;
-    }
-    const uint32_t n = uint32_t(n64);
-    const uint32_t node_count = n <= 2 ? 1u : 2 * n - 1;
+namespace {
+
+// Shared driver of the single-scene build (scenes == 1, scene_off null) and
+// the batched build (scene_off: scenes + 1 host offsets, scenes >= 2).
+template <bool SEG>
+ResultCode build_ploc_impl(const RTAabb* aabbs, const float* centers, size_t stride_floats,
+                           uint32_t n, const uint32_t* scene_off, uint32_t scenes,
+                           hipStream_t stream, DeviceBvh* out) {
+    const uint32_t node_count = SEG ? 2 * n - scenes : (n <= 2 ? 1u : 2 * n - 1);
     const uint32_t stride = uint32_t(stride_floats);
     hipError_t e;
     DeviceBvh res;
@@ -731,6 +846,7 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
     }
     res.node_count = node_count;
     res.prim_count = n;
+    res.root_count = SEG ? scenes : 1u;
     auto release_result = [&]() {
         (void)hipFree(res.nodes);
         (void)hipFree(res.indices);
@@ -742,7 +858,7 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
 
     const uint32_t sort_tiles = (n + kSortTile - 1) / kSortTile;
     const uint32_t ploc_tiles = (n + kTile - 1) / kTile;
-    const uint32_t parts = 256;
+    const uint32_t parts = SEG ? 16u : 256u;  // bounds partials (per scene when SEG)
     size_t off = 0;
     auto carve = [&](size_t bytes) {
         const size_t o = off;
@@ -750,8 +866,8 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
         return o;
     };
     const size_t o_aabb = aabbs ? 0 : carve(size_t(n) * sizeof(DevAabb));
-    const size_t o_part = carve(parts * sizeof(Bounds6));
-    const size_t o_world = carve(sizeof(Bounds6));
+    const size_t o_part = carve(size_t(parts) * scenes * sizeof(Bounds6));
+    const size_t o_world = carve(size_t(scenes) * sizeof(Bounds6));
     const size_t o_k0 = carve(size_t(n) * 4), o_k1 = carve(size_t(n) * 4);
     const size_t o_v1 = carve(size_t(n) * 4);
     const uint32_t count_entries = 256u * sort_tiles;
@@ -762,7 +878,11 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
     const size_t o_cl1 = carve(size_t(n) * sizeof(DevAabb));
     const size_t o_nb = carve(size_t(n) * 4), o_mark = carve(size_t(n) * 4);
     const size_t o_tc = carve(size_t(ploc_tiles) * 4);
-    const size_t o_state = carve(size_t(kMaxIterations) * sizeof(PlocState));
+    const size_t o_state = carve(size_t(kMaxIterations + 1) * sizeof(PlocState));
+    const size_t o_soff = SEG ? carve(size_t(scenes + 1) * 4) : 0;
+    const size_t o_pscene = SEG ? carve(size_t(n) * 4) : 0;
+    const size_t o_cs0 = SEG ? carve(size_t(n) * 4) : 0;
+    const size_t o_cs1 = SEG ? carve(size_t(n) * 4) : 0;
     Workspace ws;
     if ((e = hipMalloc(&ws.base, off)) != hipSuccess) return fail(e, "hipMalloc(workspace)");
     char* b = static_cast<char*>(ws.base);
@@ -782,12 +902,16 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
     uint32_t* mbuf = reinterpret_cast<uint32_t*>(b + o_mark);
     uint32_t* tcount = reinterpret_cast<uint32_t*>(b + o_tc);
     PlocState* st = reinterpret_cast<PlocState*>(b + o_state);
+    uint32_t* soff = SEG ? reinterpret_cast<uint32_t*>(b + o_soff) : nullptr;
+    uint32_t* pscene = SEG ? reinterpret_cast<uint32_t*>(b + o_pscene) : nullptr;
+    uint32_t* cs0 = SEG ? reinterpret_cast<uint32_t*>(b + o_cs0) : nullptr;
+    uint32_t* cs1 = SEG ? reinterpret_cast<uint32_t*>(b + o_cs1) : nullptr;
 
     const uint32_t g = (n + kBlock - 1) / kBlock;
     if (!aabbs)
         hipLaunchKernelGGL(point_aabbs_kernel, dim3(g), dim3(kBlock), 0, stream, centers, stride, n,
                            const_cast<DevAabb*>(in_aabbs));
-    if (n <= 2) {
+    if (!SEG && n <= 2) {
         hipLaunchKernelGGL(bounds_partial_kernel, dim3(1), dim3(kBlock), 0, stream, in_aabbs, n,
                            part);
         hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(kBlock), 0, stream, part, 1u, world);
@@ -798,15 +922,36 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
         *out = res;
         return Ok;
     }
-    hipLaunchKernelGGL(bounds_partial_kernel, dim3(parts), dim3(kBlock), 0, stream, in_aabbs, n,
-                       part);
-    hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(kBlock), 0, stream, part, parts, world);
-    hipLaunchKernelGGL(morton_kernel, dim3(g), dim3(kBlock), 0, stream, centers, stride, n, world,
-                       k0, v0);
-    // 4 stable passes; after an even number of passes the result is back in
-    // (k0, v0) = (keys, res.indices).
+    if (SEG) {
+        if ((e = hipMemcpyAsync(soff, scene_off, size_t(scenes + 1) * 4, hipMemcpyHostToDevice,
+                                stream)) != hipSuccess)
+            return fail(e, "scene offsets");
+        hipLaunchKernelGGL(scene_of_kernel, dim3(g), dim3(kBlock), 0, stream, soff, scenes, n,
+                           pscene);
+        hipLaunchKernelGGL(bounds_seg_partial_kernel, dim3(parts, scenes), dim3(kBlock), 0, stream,
+                           in_aabbs, soff, part);
+        hipLaunchKernelGGL(bounds_final_kernel, dim3(scenes), dim3(kBlock), 0, stream, part, parts,
+                           world);
+    } else {
+        hipLaunchKernelGGL(bounds_partial_kernel, dim3(parts), dim3(kBlock), 0, stream, in_aabbs,
+                           n, part);
+        hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(kBlock), 0, stream, part, parts,
+                           world);
+    }
+    hipLaunchKernelGGL(morton_kernel<SEG>, dim3(g), dim3(kBlock), 0, stream, centers, stride, n,
+                       world, pscene, k0, v0);
+    // 4 stable passes over the codes, then (batches) 1 or 2 passes over the
+    // scene ids; the sorted order must end in res.indices.
     uint32_t *ki = k0, *vi = v0, *ko = k1, *vo = v1;
-    for (uint32_t shift = 0; shift < 32; shift += 8) {
+    const uint32_t scene_bits = SEG ? 32u - uint32_t(__builtin_clz(scenes - 1)) : 0u;
+    const uint32_t scene_passes = (scene_bits + 7) / 8;
+    for (uint32_t pass = 0; pass < 4 + scene_passes; ++pass) {
+        uint32_t shift = pass * 8;
+        if (pass == 4) {
+            hipLaunchKernelGGL(scene_keys_kernel, dim3(g), dim3(kBlock), 0, stream, vi, pscene, n,
+                               ki);
+        }
+        if (pass >= 4) shift = (pass - 4) * 8;
         hipLaunchKernelGGL(radix_hist_kernel, dim3(sort_tiles), dim3(256), 0, stream, ki, n, shift,
                            sort_tiles, counts);
         if ((e = scan_exclusive_u32(counts, count_entries, nullptr, scan_scratch, scan_bytes,
@@ -817,9 +962,13 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
         std::swap(ki, ko);
         std::swap(vi, vo);
     }
-    hipLaunchKernelGGL(leaves_kernel, dim3(g), dim3(kBlock), 0, stream, in_aabbs, res.indices, n,
-                       cl0);
-    PlocState init{n, n - 1, 2 * n - 1, 0, 0, {0, 0, 0}};
+    if (vi != res.indices)
+        hipLaunchKernelGGL(copy_words_kernel, dim3(g), dim3(kBlock), 0, stream, vi, res.indices, n);
+    hipLaunchKernelGGL(leaves_kernel<SEG>, dim3(g), dim3(kBlock), 0, stream, in_aabbs, res.indices,
+                       n, pscene, cl0, cs0);
+    // the reference's node array: leaves conceptually at [n - S, 2n - S), the
+    // S roots end at [0, S)
+    PlocState init{n, n - res.root_count, 2 * n - res.root_count, 0, 0, {0, 0, 0}};
     if ((e = hipMemcpyAsync(st, &init, sizeof(init), hipMemcpyHostToDevice, stream)) != hipSuccess)
         return fail(e, "state init");
     if ((e = hipGetLastError()) != hipSuccess) return fail(e, "sort launch");
@@ -836,11 +985,11 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
                 set_error("clustering needed more than %u iterations", kMaxIterations);
                 return Error;
             }
-            hipLaunchKernelGGL(nn_mark_kernel, dim3(tiles), dim3(kBlock), 0, stream, st + iter, cl0,
-                               cl1, nbuf, mbuf, tcount);
+            hipLaunchKernelGGL(nn_mark_kernel<SEG>, dim3(tiles), dim3(kBlock), 0, stream, st + iter,
+                               cl0, cl1, cs0, cs1, nbuf, mbuf, tcount);
             hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, stream, st + iter, tcount);
-            hipLaunchKernelGGL(merge_kernel, dim3(tiles), dim3(kBlock), 0, stream, st + iter, cl0,
-                               cl1, nbuf, mbuf, tcount, res.nodes);
+            hipLaunchKernelGGL(merge_kernel<SEG>, dim3(tiles), dim3(kBlock), 0, stream, st + iter,
+                               cl0, cl1, cs0, cs1, nbuf, mbuf, tcount, res.nodes);
             ++iter;
         }
         if ((e = hipGetLastError()) != hipSuccess) return fail(e, "ploc iteration");
@@ -855,19 +1004,75 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
         }
         known = host_state.n_active;
     }
-    static bool lds_ok = false;
-    if (!lds_ok) {
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(tail_kernel),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(TailShared)));
+    static bool lds_ok[2] = {false, false};
+    if (!lds_ok[SEG]) {
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(tail_kernel<SEG>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                int(sizeof(TailShared<SEG>)));
         if (e != hipSuccess) return fail(e, "tail LDS size");
-        lds_ok = true;
+        lds_ok[SEG] = true;
     }
-    hipLaunchKernelGGL(tail_kernel, dim3(1), dim3(kTailThreads), sizeof(TailShared), stream,
-                       st + iter, cl0, cl1, res.nodes);
+    hipLaunchKernelGGL(tail_kernel<SEG>, dim3(1), dim3(kTailThreads), sizeof(TailShared<SEG>),
+                       stream, st + iter, cl0, cl1, cs0, cs1, res.nodes, res.root_count,
+                       st + iter + 1);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e, "tail launch");
+    if ((e = hipMemcpyAsync(&host_state, st + iter + 1, sizeof(host_state), hipMemcpyDeviceToHost,
+                            stream)) != hipSuccess)
+        return fail(e, "state read");
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return fail(e, "ploc tail");
+    if (host_state.n_active != res.root_count) {
+        release_result();
+        set_error("clustering stopped at %u clusters (non-finite bounds?)", host_state.n_active);
+        return Error;
+    }
     *out = res;
     return Ok;
+}
+
+}  // namespace
+
+ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stride_floats,
+                          size_t n64, int radius, hipStream_t stream, DeviceBvh* out) {
+    if (radius != kRadius) {
+        set_error("PLOC search radius %d not supported (built for %d)", radius, kRadius);
+        return InvalidArgument;
+    }
+    if (n64 == 0) return NoPrimitives;
+    if (n64 > (1u << 30)) {
+        set_error("prim_count %zu too large", n64);
+        return InvalidArgument;
+    }
+    return build_ploc_impl<false>(aabbs, centers, stride_floats, uint32_t(n64), nullptr, 1u,
+                                  stream, out);
+}
+
+ResultCode gpu_build_ploc_batch(const RTAabb* aabbs, const float* centers, size_t stride_floats,
+                                const uint32_t* scene_off, uint32_t scenes, hipStream_t stream,
+                                DeviceBvh* out) {
+    if (scenes == 0 || scene_off == nullptr) return NoPrimitives;
+    if (scenes > kMaxBatchScenes) {
+        set_error("a batch holds at most %u scenes (got %u)", kMaxBatchScenes, scenes);
+        return InvalidArgument;
+    }
+    if (scene_off[0] != 0) {
+        set_error("scene_offsets[0] must be 0");
+        return InvalidArgument;
+    }
+    for (uint32_t s = 0; s < scenes; ++s) {
+        if (scene_off[s + 1] < scene_off[s] + 3) {
+            set_error("scene %u has %d primitives; a batched scene needs at least 3", s,
+                      int(scene_off[s + 1]) - int(scene_off[s]));
+            return InvalidArgument;
+        }
+    }
+    const uint32_t n = scene_off[scenes];
+    if (n > (1u << 30)) {
+        set_error("batch of %u primitives too large", n);
+        return InvalidArgument;
+    }
+    if (scenes == 1)
+        return build_ploc_impl<false>(aabbs, centers, stride_floats, n, nullptr, 1u, stream, out);
+    return build_ploc_impl<true>(aabbs, centers, stride_floats, n, scene_off, scenes, stream, out);
 }
 
 }  // namespace rtbvh

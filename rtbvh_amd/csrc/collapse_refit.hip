@@ -55,13 +55,14 @@ __global__ void parent_kernel(const DevAabb* nodes, uint32_t count, int32_t* par
     }
 }
 
-// flag[i] = 1 for the root and for interior nodes at even depth.
-__global__ void mflag_kernel(const DevAabb* nodes, uint32_t count, const int32_t* parent,
-                             uint32_t* flag, uint32_t* status) {
+// flag[i] = 1 for the roots (nodes [0, roots)) and for interior nodes at
+// even depth below them.
+__global__ void mflag_kernel(const DevAabb* nodes, uint32_t count, uint32_t roots,
+                             const int32_t* parent, uint32_t* flag, uint32_t* status) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     uint32_t f = 0;
-    if (i == 0) {
+    if (i < roots) {
         f = 1;
     } else {
         DevAabb n;
@@ -69,11 +70,11 @@ __global__ void mflag_kernel(const DevAabb* nodes, uint32_t count, const int32_t
         if (n.extra1 < 0 && n.extra2 >= 0) {
             uint32_t depth = 0;
             int32_t p = int32_t(i);
-            while (p > 0 && depth < kMaxDepth) {
+            while (p >= int32_t(roots) && depth < kMaxDepth) {
                 p = parent[p];
                 ++depth;
             }
-            if (p == 0)
+            if (p >= 0 && p < int32_t(roots))
                 f = (depth & 1u) ? 0u : 1u;
             else if (depth >= kMaxDepth)
                 atomicOr(status, 2u);  // cycle or absurd depth
@@ -242,6 +243,7 @@ ResultCode ensure_parents(DeviceBvh& bvh, hipStream_t stream) {
 ResultCode gpu_build_mbvh(DeviceBvh& bvh, hipStream_t stream, DeviceMbvh* out) {
     DeviceMbvh res;
     res.prim_count = bvh.prim_count;
+    res.root_count = bvh.root_count;
     if (bvh.node_count == 0) {
         *out = res;
         return Ok;
@@ -271,7 +273,7 @@ ResultCode gpu_build_mbvh(DeviceBvh& bvh, hipStream_t stream, DeviceMbvh* out) {
     }
     uint32_t* status = device_status_ptr();
     hipLaunchKernelGGL(mflag_kernel, dim3(g), dim3(kBlock), 0, stream, bvh.nodes, count,
-                       bvh.parents, flag, status);
+                       bvh.root_count, bvh.parents, flag, status);
     hipLaunchKernelGGL(copy_u32_kernel, dim3(g), dim3(kBlock), 0, stream, flag, mindex, count);
     if ((e = scan_exclusive_u32(mindex, count, total, scratch, sbytes, stream)) != hipSuccess) {
         cleanup();

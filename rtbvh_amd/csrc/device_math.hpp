@@ -57,17 +57,18 @@ struct MtResult {
     float t, u, v;
 };
 
-__device__ __forceinline__ MtResult mt_candidate(float v0x, float v0y, float v0z, float v1x,
-                                                 float v1y, float v1z, float v2x, float v2y,
-                                                 float v2z, float ox, float oy, float oz,
-                                                 float dx, float dy, float dz, float t_min) {
+// The same test with the edges e1 = v1 - v0, e2 = v2 - v0 already formed
+// (identical values: the subtraction is the same IEEE operation wherever it
+// is done).
+__device__ __forceinline__ MtResult mt_candidate_e(float v0x, float v0y, float v0z, float e1x,
+                                                   float e1y, float e1z, float e2x, float e2y,
+                                                   float e2z, float ox, float oy, float oz,
+                                                   float dx, float dy, float dz, float t_min) {
     MtResult r;
     r.ok = false;
     r.t = 0.f;
     r.u = 0.f;
     r.v = 0.f;
-    const float e1x = v1x - v0x, e1y = v1y - v0y, e1z = v1z - v0z;
-    const float e2x = v2x - v0x, e2y = v2y - v0y, e2z = v2z - v0z;
     const float hx = dy * e2z - e2y * dz;
     const float hy = dz * e2x - e2z * dx;
     const float hz = dx * e2y - e2x * dy;
@@ -88,6 +89,27 @@ __device__ __forceinline__ MtResult mt_candidate(float v0x, float v0y, float v0z
     r.u = u;
     r.v = v;
     return r;
+}
+
+__device__ __forceinline__ MtResult mt_candidate(float v0x, float v0y, float v0z, float v1x,
+                                                 float v1y, float v1z, float v2x, float v2y,
+                                                 float v2z, float ox, float oy, float oz,
+                                                 float dx, float dy, float dz, float t_min) {
+    return mt_candidate_e(v0x, v0y, v0z, v1x - v0x, v1y - v0y, v1z - v0z, v2x - v0x, v2y - v0y,
+                          v2z - v0z, ox, oy, oz, dx, dy, dz, t_min);
+}
+
+// Leaf-ordered triangle record (three float4 per index entry): v0 with the
+// primitive id in .w (RTBVH_MISS: no triangle), e1, e2.
+__device__ __forceinline__ MtResult mt_prepared(const float4* tris, uint32_t pos, float ox,
+                                                float oy, float oz, float dx, float dy, float dz,
+                                                float t_min, uint32_t* prim) {
+    const float4 a = tris[size_t(pos) * 3];
+    const float4 e1 = tris[size_t(pos) * 3 + 1];
+    const float4 e2 = tris[size_t(pos) * 3 + 2];
+    *prim = __float_as_uint(a.w);
+    return mt_candidate_e(a.x, a.y, a.z, e1.x, e1.y, e1.z, e2.x, e2.y, e2.z, ox, oy, oz, dx, dy,
+                          dz, t_min);
 }
 
 }  // namespace rtbvh

@@ -28,27 +28,28 @@ constexpr int kBlock = 256;
 constexpr int kLdsStack = 24;
 constexpr int kDeepStack = RTBVH_MAX_STACK - kLdsStack;
 
-struct Stack {
-    int32_t* lds;  // lane-interleaved: entry e of this lane at lds[e * kBlock]
-    int32_t deep[kDeepStack];
-    int sp;
-
-    __device__ __forceinline__ bool push(int32_t v) {
-        if (sp < kLdsStack) {
-            lds[sp * kBlock] = v;
-        } else if (sp < RTBVH_MAX_STACK) {
-            deep[sp - kLdsStack] = v;
-        } else {
-            return false;
-        }
-        ++sp;
-        return true;
-    }
-    __device__ __forceinline__ int32_t pop() {
-        --sp;
-        return sp < kLdsStack ? lds[sp * kBlock] : deep[sp - kLdsStack];
-    }
+// The stack pointer is a separate local so that it stays in a register; the
+// private overflow array is dynamically indexed and lives in scratch.
+struct DeepStack {
+    int32_t v[kDeepStack];
 };
+
+__device__ __forceinline__ bool stack_push(int32_t* lds, DeepStack& deep, int& sp, int32_t v) {
+    if (sp < kLdsStack) {
+        lds[sp * kBlock] = v;  // lane-interleaved: entry e of this lane at lds[e * kBlock]
+    } else if (sp < RTBVH_MAX_STACK) {
+        deep.v[sp - kLdsStack] = v;
+    } else {
+        return false;
+    }
+    ++sp;
+    return true;
+}
+
+__device__ __forceinline__ int32_t stack_pop(const int32_t* lds, const DeepStack& deep, int& sp) {
+    --sp;
+    return sp < kLdsStack ? lds[sp * kBlock] : deep.v[sp - kLdsStack];
+}
 
 struct RayState {
     float ox, oy, oz, dx, dy, dz, ix, iy, iz, t_min;
@@ -88,11 +89,18 @@ __device__ __forceinline__ bool test_leaf(const TraceParams& p, int32_t first, i
     for (int32_t j = 0; j < count; ++j) {
         const uint32_t pos = uint32_t(first) + uint32_t(j);
         if (pos >= p.index_count) break;
-        const uint32_t prim = p.indices[pos];
-        if (prim >= p.prim_count) continue;  // iter.rs:337 skips missing primitives
-        const float* v = p.verts + size_t(prim) * 9;
-        const MtResult m = mt_candidate(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8],
-                                        s.ox, s.oy, s.oz, s.dx, s.dy, s.dz, s.t_min);
+        uint32_t prim;
+        MtResult m;
+        if (p.tris) {
+            m = mt_prepared(p.tris, pos, s.ox, s.oy, s.oz, s.dx, s.dy, s.dz, s.t_min, &prim);
+            if (prim == RTBVH_MISS) continue;
+        } else {
+            prim = p.indices[pos];
+            if (prim >= p.prim_count) continue;  // iter.rs:337 skips missing primitives
+            const float* v = p.verts + size_t(prim) * 9;
+            m = mt_candidate(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], s.ox, s.oy,
+                             s.oz, s.dx, s.dy, s.dz, s.t_min);
+        }
         if (!m.ok) continue;
         bool take;
         if (REF || ANY)
@@ -145,9 +153,9 @@ template <bool REF, bool ANY, bool STATS = false>
 __global__ void __launch_bounds__(kBlock) mbvh_trace_kernel(TraceParams p) {
     __shared__ int32_t lds_stack[kLdsStack * kBlock];
     uint64_t n_nodes = 0, n_tris = 0;  // STATS only
-    Stack st;
-    st.lds = lds_stack + threadIdx.x;
-    st.sp = 0;
+    int32_t* lds = lds_stack + threadIdx.x;
+    DeepStack deep;
+    int sp = 0;
     const uint64_t stride = uint64_t(gridDim.x) * kBlock;
     uint64_t rid = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
     RayState s;
@@ -164,7 +172,7 @@ __global__ void __launch_bounds__(kBlock) mbvh_trace_kernel(TraceParams p) {
             }
             active = true;
             cur = 0;
-            st.sp = 0;
+            sp = 0;
         }
         const DevMbvhNode nd = p.mnodes[cur];
         if (STATS) ++n_nodes;
@@ -221,7 +229,7 @@ __global__ void __launch_bounds__(kBlock) mbvh_trace_kernel(TraceParams p) {
                     if (STATS) n_tris += uint64_t(ct[k]);
                     found = test_leaf<REF, ANY>(p, ch[k], ct[k], s);
                 } else if (ch[k] > -1) {
-                    overflow = !st.push(ch[k]);
+                    overflow = !stack_push(lds, deep, sp, ch[k]);
                 }
             }
         } else {
@@ -239,15 +247,15 @@ __global__ void __launch_bounds__(kBlock) mbvh_trace_kernel(TraceParams p) {
 #pragma unroll
             for (int k = 3; k >= 0; --k) {
                 if (!hit[k] || found || overflow || ct[k] >= 0 || ch[k] < 0) continue;
-                overflow = !st.push(ch[k]);
+                overflow = !stack_push(lds, deep, sp, ch[k]);
             }
         }
         bool done = found || overflow;
         if (!done) {
-            if (st.sp == 0) {
+            if (sp == 0) {
                 done = true;
             } else {
-                cur = st.pop();
+                cur = stack_pop(lds, deep, sp);
                 if (uint32_t(cur) >= p.node_count) done = true;  // malformed tree
             }
         }
@@ -294,9 +302,9 @@ __device__ __forceinline__ bool bvh_slab(const DevAabb& b, const RayState& s, fl
 template <bool REF, bool ANY>
 __global__ void __launch_bounds__(kBlock) bvh_trace_kernel(TraceParams p) {
     __shared__ int32_t lds_stack[kLdsStack * kBlock];
-    Stack st;
-    st.lds = lds_stack + threadIdx.x;
-    st.sp = 0;
+    int32_t* lds = lds_stack + threadIdx.x;
+    DeepStack deep;
+    int sp = 0;
     const uint64_t stride = uint64_t(gridDim.x) * kBlock;
     uint64_t rid = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
     RayState s;
@@ -313,7 +321,7 @@ __global__ void __launch_bounds__(kBlock) bvh_trace_kernel(TraceParams p) {
             }
             active = true;
             cur = 0;  // the root is visited without a box test (iter.rs:36-44)
-            st.sp = 0;
+            sp = 0;
         }
         const DevAabb nd = p.bnodes[cur];
         bool found = false;
@@ -334,22 +342,22 @@ __global__ void __launch_bounds__(kBlock) bvh_trace_kernel(TraceParams p) {
             // sort_nodes, bvh_node.rs:150-177
             if (hl && hr) {
                 if (tl < tr) {
-                    overflow = !st.push(lf) || !st.push(lf + 1);
+                    overflow = !stack_push(lds, deep, sp, lf) || !stack_push(lds, deep, sp, lf + 1);
                 } else {
-                    overflow = !st.push(lf + 1) || !st.push(lf);
+                    overflow = !stack_push(lds, deep, sp, lf + 1) || !stack_push(lds, deep, sp, lf);
                 }
             } else if (hl) {
-                overflow = !st.push(lf);
+                overflow = !stack_push(lds, deep, sp, lf);
             } else if (hr) {
-                overflow = !st.push(lf + 1);
+                overflow = !stack_push(lds, deep, sp, lf + 1);
             }
         }
         bool done = found || overflow;
         if (!done) {
-            if (st.sp == 0) {
+            if (sp == 0) {
                 done = true;
             } else {
-                cur = st.pop();
+                cur = stack_pop(lds, deep, sp);
                 if (uint32_t(cur) >= p.node_count) done = true;
             }
         }

@@ -28,37 +28,41 @@ constexpr int kBlock = 256;
 constexpr int kLdsStack = 12;
 constexpr int kDeepStack = RTBVH_MAX_STACK - kLdsStack;
 
-struct FastStack {
-    int32_t* lds_i;  // lane-interleaved: entry e at [e * kBlock]
-    float* lds_t;
-    int32_t deep_i[kDeepStack];
-    float deep_t[kDeepStack];
-    int sp;
-
-    __device__ __forceinline__ bool push(int32_t node, float t) {
-        if (sp < kLdsStack) {
-            lds_i[sp * kBlock] = node;
-            lds_t[sp * kBlock] = t;
-        } else if (sp < RTBVH_MAX_STACK) {
-            deep_i[sp - kLdsStack] = node;
-            deep_t[sp - kLdsStack] = t;
-        } else {
-            return false;
-        }
-        ++sp;
-        return true;
-    }
-    __device__ __forceinline__ void pop(int32_t& node, float& t) {
-        --sp;
-        if (sp < kLdsStack) {
-            node = lds_i[sp * kBlock];
-            t = lds_t[sp * kBlock];
-        } else {
-            node = deep_i[sp - kLdsStack];
-            t = deep_t[sp - kLdsStack];
-        }
-    }
+// Stack entries beyond the LDS part live in per-lane private arrays.  The
+// stack pointer is kept apart from those arrays so that it stays in a
+// register (a struct holding both would be placed in scratch as a whole).
+struct DeepStack {
+    int32_t i[kDeepStack];
+    float t[kDeepStack];
 };
+
+__device__ __forceinline__ bool stack_push(int32_t* lds_i, float* lds_t, DeepStack& deep, int& sp,
+                                           int32_t node, float t) {
+    if (sp < kLdsStack) {
+        lds_i[sp * kBlock] = node;
+        lds_t[sp * kBlock] = t;
+    } else if (sp < RTBVH_MAX_STACK) {
+        deep.i[sp - kLdsStack] = node;
+        deep.t[sp - kLdsStack] = t;
+    } else {
+        return false;
+    }
+    ++sp;
+    return true;
+}
+
+__device__ __forceinline__ void stack_pop(const int32_t* lds_i, const float* lds_t,
+                                          const DeepStack& deep, int& sp, int32_t& node,
+                                          float& t) {
+    --sp;
+    if (sp < kLdsStack) {
+        node = lds_i[sp * kBlock];
+        t = lds_t[sp * kBlock];
+    } else {
+        node = deep.i[sp - kLdsStack];
+        t = deep.t[sp - kLdsStack];
+    }
+}
 
 template <typename T>
 __device__ __forceinline__ T sel4(uint32_t k, T a, T b, T c, T d) {
@@ -72,14 +76,30 @@ __device__ __forceinline__ void csort(uint32_t& a, uint32_t& b) {
     b = hi;
 }
 
-template <bool ANY, bool STATS>
+// Scene of ray rid in a batch: the s with off[s] <= rid < off[s+1].
+__device__ __forceinline__ uint32_t scene_of_ray(const uint32_t* off, uint32_t scenes,
+                                                 uint32_t rid) {
+    uint32_t lo = 0, hi = scenes;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= rid)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// BATCH: the MBVH is a forest whose node s is the root of scene s, and each
+// ray starts at the root of its scene.
+template <bool ANY, bool STATS, bool PREP, bool BATCH>
 __global__ void __launch_bounds__(kBlock) mbvh_fast_kernel(TraceParams p) {
     __shared__ int32_t s_idx[kLdsStack * kBlock];
     __shared__ float s_t[kLdsStack * kBlock];
-    FastStack st;
-    st.lds_i = s_idx + threadIdx.x;
-    st.lds_t = s_t + threadIdx.x;
-    st.sp = 0;
+    int32_t* lds_i = s_idx + threadIdx.x;
+    float* lds_t = s_t + threadIdx.x;
+    DeepStack deep;
+    int sp = 0;
     uint64_t n_nodes = 0, n_tris = 0;
     const uint64_t stride = uint64_t(gridDim.x) * kBlock;
     uint64_t rid = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -122,18 +142,18 @@ __global__ void __launch_bounds__(kBlock) mbvh_fast_kernel(TraceParams p) {
                 continue;
             }
             active = true;
-            cur = 0;
-            st.sp = 0;
+            cur = BATCH ? int32_t(scene_of_ray(p.ray_offsets, p.scene_count, uint32_t(rid))) : 0;
+            sp = 0;
         }
         bool done = false;
         bool overflow = false;
         bool found = false;
         if (cur < 0) {
             // pop until an entry can still hold a hit at or before bt
-            while (st.sp > 0) {
+            while (sp > 0) {
                 int32_t c;
                 float t;
-                st.pop(c, t);
+                stack_pop(lds_i, lds_t, deep, sp, c, t);
                 if (t <= bt) {
                     cur = c;
                     break;
@@ -185,11 +205,18 @@ __global__ void __launch_bounds__(kBlock) mbvh_fast_kernel(TraceParams p) {
                 for (int32_t j = 0; j < cnt; ++j) {
                     const uint32_t pos = uint32_t(first) + uint32_t(j);
                     if (pos >= p.index_count) break;
-                    const uint32_t prim = p.indices[pos];
-                    if (prim >= p.prim_count) continue;
-                    const float* v = p.verts + size_t(prim) * 9;
-                    const MtResult m = mt_candidate(v[0], v[1], v[2], v[3], v[4], v[5], v[6],
-                                                    v[7], v[8], ox, oy, oz, dx, dy, dz, tmin);
+                    uint32_t prim;
+                    MtResult m;
+                    if (PREP) {
+                        m = mt_prepared(p.tris, pos, ox, oy, oz, dx, dy, dz, tmin, &prim);
+                        if (prim == RTBVH_MISS) continue;
+                    } else {
+                        prim = p.indices[pos];
+                        if (prim >= p.prim_count) continue;
+                        const float* v = p.verts + size_t(prim) * 9;
+                        m = mt_candidate(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], ox,
+                                         oy, oz, dx, dy, dz, tmin);
+                    }
                     if (!m.ok) continue;
                     if (ANY) {
                         if (m.t < bt) {
@@ -216,7 +243,7 @@ __global__ void __launch_bounds__(kBlock) mbvh_fast_kernel(TraceParams p) {
                 const int32_t c = sel4(s, c0, c1, c2, c3);
                 const float t = sel4(s, tn[0], tn[1], tn[2], tn[3]);
                 if (c < 0 || !(t <= bt)) continue;
-                if (next >= 0) overflow = !st.push(next, next_t);
+                if (next >= 0) overflow = !stack_push(lds_i, lds_t, deep, sp, next, next_t);
                 next = c;
                 next_t = t;
             }
@@ -243,9 +270,9 @@ __global__ void __launch_bounds__(kBlock) mbvh_fast_kernel(TraceParams p) {
     }
 }
 
-template <bool ANY, bool STATS>
+template <bool ANY, bool STATS, bool PREP, bool BATCH = false>
 hipError_t launch_fast(const TraceParams& p, hipStream_t stream) {
-    auto k = mbvh_fast_kernel<ANY, STATS>;
+    auto k = mbvh_fast_kernel<ANY, STATS, PREP, BATCH>;
     static int blocks[64] = {0};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
@@ -268,11 +295,56 @@ hipError_t launch_fast(const TraceParams& p, hipStream_t stream) {
 }  // namespace
 
 hipError_t launch_trace_fast(const TraceParams& p, bool any_hit, hipStream_t stream) {
+    if (p.ray_offsets) {
+        if (p.stats || !p.tris) return hipErrorInvalidValue;
+        return any_hit ? launch_fast<true, false, true, true>(p, stream)
+                       : launch_fast<false, false, true, true>(p, stream);
+    }
     if (p.stats) {
         if (any_hit) return hipErrorInvalidValue;
-        return launch_fast<false, true>(p, stream);
+        return p.tris ? launch_fast<false, true, true>(p, stream)
+                      : launch_fast<false, true, false>(p, stream);
     }
-    return any_hit ? launch_fast<true, false>(p, stream) : launch_fast<false, false>(p, stream);
+    if (p.tris)
+        return any_hit ? launch_fast<true, false, true>(p, stream)
+                       : launch_fast<false, false, true>(p, stream);
+    return any_hit ? launch_fast<true, false, false>(p, stream)
+                   : launch_fast<false, false, false>(p, stream);
+}
+
+namespace {
+// scene_off (batches, device, scenes + 1 entries): the id stored is the
+// primitive's index within its scene.
+__global__ void prepare_tris_kernel(const uint32_t* indices, uint32_t index_count,
+                                    const float* verts, uint32_t prim_count, float4* tris,
+                                    const uint32_t* scene_off, uint32_t scenes) {
+    const uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= index_count) return;
+    const uint32_t prim = indices[pos];
+    float4 a = make_float4(0.f, 0.f, 0.f, __uint_as_float(RTBVH_MISS));
+    float4 e1 = make_float4(0.f, 0.f, 0.f, 0.f), e2 = e1;
+    if (prim < prim_count) {
+        const float* v = verts + size_t(prim) * 9;
+        const uint32_t id = scene_off ? prim - scene_off[scene_of_ray(scene_off, scenes, prim)]
+                                      : prim;
+        a = make_float4(v[0], v[1], v[2], __uint_as_float(id));
+        e1 = make_float4(v[3] - v[0], v[4] - v[1], v[5] - v[2], 0.f);
+        e2 = make_float4(v[6] - v[0], v[7] - v[1], v[8] - v[2], 0.f);
+    }
+    tris[size_t(pos) * 3] = a;
+    tris[size_t(pos) * 3 + 1] = e1;
+    tris[size_t(pos) * 3 + 2] = e2;
+}
+}  // namespace
+
+hipError_t launch_prepare_triangles(const uint32_t* indices, uint32_t index_count,
+                                    const float* verts, uint32_t prim_count, float4* tris,
+                                    hipStream_t stream, const uint32_t* scene_off,
+                                    uint32_t scenes) {
+    if (index_count == 0) return hipSuccess;
+    hipLaunchKernelGGL(prepare_tris_kernel, dim3((index_count + 255) / 256), dim3(256), 0, stream,
+                       indices, index_count, verts, prim_count, tris, scene_off, scenes);
+    return hipGetLastError();
 }
 
 }  // namespace rtbvh

@@ -296,3 +296,50 @@ def bounce_rays_torch(verts, primary, hits, generator, offset: float = 1e-4):
     out[:, 4:7] = d.to(torch.float32)
     out[:, 7] = float(RAY_T_MAX)
     return out
+
+
+def batch_scene_sizes(count: int = 256, seed: int = 7, lo: float = 1e3, hi: float = 5e5):
+    """Triangle counts log-uniform in [lo, hi]: round(exp(U(ln lo, ln hi)))."""
+    rng = np.random.default_rng(seed)
+    return np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(np.int64)
+
+
+def batch_scenes(count: int = 256, seed: int = 7):
+    """Scenes of uneven size for batched builds: even indices are noise-
+    displaced spheres (about N triangles: 2*nu*nv with nu/nv = 1.25), odd
+    ones random soups of exactly N triangles with edge 0.01, each in the unit
+    cube and seeded by its index."""
+    out = []
+    for s, n in enumerate(batch_scene_sizes(count, seed)):
+        if s % 2 == 0:
+            nv = max(3, int(round(np.sqrt(n / 2.5))))
+            nu = max(4, int(round(n / (2 * nv))))
+            out.append(noisy_sphere(nu, nv, seed=seed * 1000 + s))
+        else:
+            out.append(random_soup(int(n), 0.01, seed=seed * 1000 + s))
+    return out
+
+
+def batch_rays_torch(lo, hi, counts, device, generator):
+    """counts[s] rays for scene s: origin uniform in the scene's box [lo[s],
+    hi[s]], direction uniform on the unit sphere; (R, 8) float32 records in
+    scene order."""
+    import torch
+
+    counts_t = torch.as_tensor(np.asarray(counts, dtype=np.int64), device=device)
+    scene = torch.repeat_interleave(torch.arange(len(counts), device=device), counts_t)
+    lo_t = torch.as_tensor(np.asarray(lo, dtype=np.float32), device=device)
+    hi_t = torch.as_tensor(np.asarray(hi, dtype=np.float32), device=device)
+    n = scene.numel()
+    out = torch.empty((n, 8), dtype=torch.float32, device=device)
+    chunk = 1 << 24
+    for b in range(0, n, chunk):
+        e = min(n, b + chunk)
+        sc = scene[b:e]
+        u = torch.rand((e - b, 3), device=device, generator=generator)
+        out[b:e, 0:3] = lo_t[sc] + u * (hi_t[sc] - lo_t[sc])
+        d = torch.randn((e - b, 3), device=device, generator=generator)
+        out[b:e, 4:7] = d / d.norm(dim=1, keepdim=True).clamp_min(1e-12)
+        out[b:e, 3] = RAY_T_MIN
+        out[b:e, 7] = RAY_T_MAX
+    return out

@@ -1,0 +1,148 @@
+"""Batches of scenes (rtbvh_gpu_create_batch / rtbvh_gpu_intersect_batch).
+
+No reference counterpart: the reference builds and traces one scene per
+call.  The bar is that a batch is exactly the per-scene work done together:
+  * every scene's subtree of the batch forest equals, node for node (bounds
+    bit for bit, same child order, same leaf primitives), the PLOC tree of
+    that scene built alone (which is itself bit-identical to the oracle's
+    LOCB);
+  * the batched launch returns, bit for bit, the hits and occlusion flags of
+    tracing each scene alone, and those equal the oracle's brute force.
+"""
+import numpy as np
+import pytest
+
+from oracle import refcpu as ref
+from rtbvh_amd import scenes
+from tests.helpers import as_rays, flat, hits_equal, prep, ray_set, teapot
+
+torch = pytest.importorskip("torch")
+import rtbvh_amd as rt  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a, dtype=torch.float32):
+    return torch.from_numpy(np.ascontiguousarray(a)).view(dtype).cuda()
+
+
+def _scene_list():
+    return [
+        scenes.random_soup(3, 0.2, seed=1),
+        teapot(),
+        scenes.random_soup(100, 0.1, seed=4),
+        scenes.noisy_sphere(60, 40, seed=5),
+        np.repeat(scenes.random_soup(400, 0.05, seed=6), 3, axis=0),
+        scenes.torus(120, 60),
+        scenes.random_soup(7, 0.3, seed=8),
+    ]
+
+
+def _subtree_equal(na, ia, ra, off_a, nb, ib, rb):
+    """Node-for-node equality of the subtree of (na, ia) at ra, whose leaves
+    hold batch-wide ids offset by off_a, and the tree (nb, ib) at rb."""
+    ua = flat(na).view(np.uint32).reshape(-1, 8)
+    ub = flat(nb).view(np.uint32).reshape(-1, 8)
+    stack = [(ra, rb)]
+    count = 0
+    while stack:
+        a, b = stack.pop()
+        count += 1
+        if not np.array_equal(ua[a][[0, 1, 2, 4, 5, 6]], ub[b][[0, 1, 2, 4, 5, 6]]):
+            return False, f"bounds differ at {a}/{b}"
+        ca, cb = int(na[a]["count"]), int(nb[b]["count"])
+        if ca != cb:
+            return False, f"count differs at {a}/{b}"
+        la, lb = int(na[a]["left_first"]), int(nb[b]["left_first"])
+        if ca >= 0:
+            pa = ia[la:la + ca].astype(np.int64) - off_a
+            pb = ib[lb:lb + cb].astype(np.int64)
+            if not np.array_equal(pa, pb):
+                return False, f"leaf prims differ at {a}/{b}"
+        else:
+            stack.append((la + 1, lb + 1))
+            stack.append((la, lb))
+    return True, count
+
+
+def _build_batch(verts_list):
+    counts = [len(v) for v in verts_list]
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    verts = np.concatenate(verts_list)
+    aabbs, centers = prep(verts)
+    batch = rt.SceneBatch(_dev(centers), offs.tolist(), aabbs=_dev(flat(aabbs)))
+    return batch, verts, offs
+
+
+def test_batch_trees_equal_single_scene_builds():
+    print("[batch_trees]", flush=True)
+    vl = _scene_list()
+    batch, verts, offs = _build_batch(vl)
+    assert batch.scene_count == len(vl)
+    assert batch.bvh_node_count == 2 * len(verts) - len(vl)
+    nodes = batch.bvh_nodes().cpu().numpy().view(ref.AABB).reshape(-1)
+    idx = batch.indices().cpu().numpy().view(np.uint32)
+    assert sorted(idx.tolist()) == list(range(len(verts)))
+    for s, v in enumerate(vl):
+        aabbs, centers = prep(v)
+        want_n, want_i = ref.build_locb(aabbs, centers)
+        ok, info = _subtree_equal(nodes, idx, s, int(offs[s]), want_n, want_i, 0)
+        assert ok, f"scene {s}: {info}"
+        assert info == 2 * len(v) - 1
+
+
+def test_batch_trace_equals_per_scene_trace():
+    print("[batch_trace]", flush=True)
+    vl = _scene_list()
+    batch, verts, offs = _build_batch(vl)
+    batch.attach_triangles(_dev(verts))
+    rays_l = [ray_set(v, 3000 + 500 * s, seed=20 + s, kind="box") for s, v in enumerate(vl)]
+    rays_l[2] = rays_l[2][:0]  # a scene without rays
+    for r in rays_l[::2]:
+        if len(r):
+            r[::4, 7] = np.float32(0.5)
+    counts = [len(r) for r in rays_l]
+    rays = np.concatenate(rays_l)
+    roff = rt.SceneBatch.ray_offsets(counts, "cuda")
+    got = batch.intersect(_dev(rays), roff).cpu().numpy()
+    got_occ = batch.occluded(_dev(rays), roff).cpu().numpy()
+    torch.cuda.synchronize()
+    start = 0
+    for s, v in enumerate(vl):
+        r = rays_l[s]
+        g = got[start:start + len(r)]
+        go = got_occ[start:start + len(r)]
+        start += len(r)
+        if not len(r):
+            continue
+        aabbs, centers = prep(v)
+        bvh = rt.Builder(_dev(centers), aabbs=_dev(flat(aabbs))).construct_locally_ordered_clustered()
+        m = rt.Mbvh.from_bvh(bvh)
+        m.attach_triangles(_dev(v))
+        want = m.intersect_prepared(_dev(r)).cpu().numpy()
+        want_occ = m.occluded_prepared(_dev(r)).cpu().numpy()
+        assert np.array_equal(g.view(np.uint32), want.view(np.uint32)), f"scene {s}"
+        assert np.array_equal(go, want_occ), f"scene {s} occluded"
+        brute = ref.trace_brute(v, as_rays(r), threads=8)
+        assert hits_equal(g.view(ref.HIT).reshape(-1), brute).all(), f"scene {s} vs brute force"
+    m.status()
+
+
+def test_batch_errors():
+    print("[batch_errors]", flush=True)
+    v = scenes.random_soup(50, 0.1, seed=3)
+    aabbs, centers = prep(v)
+    with pytest.raises(rt.RTBvhError, match="InvalidArgument"):
+        rt.SceneBatch(_dev(centers), [0, 2, 50], aabbs=_dev(flat(aabbs)))  # scene of 2
+    batch = rt.SceneBatch(_dev(centers), [0, 20, 50], aabbs=_dev(flat(aabbs)))
+    rays = _dev(ray_set(v, 64, seed=1))
+    roff = rt.SceneBatch.ray_offsets([32, 32], "cuda")
+    with pytest.raises(rt.RTBvhError):
+        batch.intersect(rays, roff)  # no triangles attached
+    batch.attach_triangles(_dev(v))
+    out = batch.intersect(rays, roff)
+    torch.cuda.synchronize()
+    assert out.shape == (64, 4)
+    with pytest.raises(rt.RTBvhError, match="InvalidArgument"):
+        rt._lib.check(rt._lib.lib().rtbvh_gpu_intersect_batch(
+            batch._h, rays.data_ptr(), roff.data_ptr(), 64, out.data_ptr(), 1, None), "flags")

@@ -4,7 +4,7 @@
       prim_indices bit-identical to the oracle's LOCB restatement.
   MBVH collapse: structurally identical to the oracle's Mbvh::construct.
   Binned SAH: same topology and bounds as the oracle's binned builder with
-      the Q2/Q3 box defects fixed; every child box inside its parent; SAH cost
+      the reference's two box defects fixed; every child box inside its parent; SAH cost
       within 5% of the reference builder's.
   Refit: bit-identical to the oracle's Bvh::refit.
   GPU-built trees trace to the brute-force closest hits.
@@ -179,3 +179,38 @@ def test_refit_matches_oracle(kind):
     rays8 = ray_set(moved, 4000, seed=6, kind="box")
     h = bvh.intersect(_dev(moved), _dev(rays8)).cpu().numpy().view(ref.HIT).reshape(-1)
     assert hits_equal(h, ref.trace_brute(moved, as_rays(rays8), threads=8)).all()
+
+
+@pytest.mark.parametrize("kind", ["ploc", "binned"])
+def test_prepared_triangles_match_vertex_path(kind):
+    """rtbvh_gpu_*_mbvh_prepared over attached leaf-ordered triangles returns
+    the same hit records and occlusion flags, bit for bit, as the vertex path,
+    in both tie modes."""
+    print(f"[prepared] kind={kind}", flush=True)
+    verts = np.concatenate([scenes.noisy_sphere(90, 60, seed=12),
+                            scenes.random_soup(3000, 0.03, seed=13)])
+    bvh, _, _ = _build(verts, kind)
+    m = rt.Mbvh.from_bvh(bvh)
+    rays8 = ray_set(verts, 20000, seed=14, kind="box")
+    rays8[::3, 7] = np.random.default_rng(1).uniform(0.0, 2.0, len(rays8[::3])).astype(np.float32)
+    dv, dr = _dev(verts), _dev(rays8)
+    with pytest.raises(rt.RTBvhError):
+        m.intersect_prepared(dr)
+    m.attach_triangles(dv)
+    for ref_order in (False, True):
+        a = m.intersect(dv, dr, reference_order=ref_order).cpu().numpy()
+        b = m.intersect_prepared(dr, reference_order=ref_order).cpu().numpy()
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), ref_order
+        oa = m.occluded(dv, dr, reference_order=ref_order).cpu().numpy()
+        ob = m.occluded_prepared(dr, reference_order=ref_order).cpu().numpy()
+        assert np.array_equal(oa, ob), ref_order
+    m.status()
+    want = ref.trace_brute(verts, as_rays(rays8), threads=8)
+    got = m.intersect_prepared(dr).cpu().numpy().view(ref.HIT).reshape(-1)
+    if kind == "ploc":
+        assert hits_equal(got, want).all()
+    # the attached copy is a snapshot: moved vertices need a new attach
+    moved = verts + np.float32(0.5)
+    m.attach_triangles(_dev(moved))
+    got2 = m.intersect_prepared(dr).cpu().numpy()
+    assert not np.array_equal(got2.view(np.uint32), got.view(np.uint32).reshape(-1, 4))

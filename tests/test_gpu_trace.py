@@ -85,7 +85,7 @@ def test_occluded_parity(builder):
     # the reference order is bit-exact with the oracle's traversal of the
     # same tree; on trees whose boxes contain their primitives (LOCB) every
     # mode equals brute force.  Binned-SAH trees built with the reference's
-    # Q2 defect can lose hits, so only the reference-order result is pinned.
+    # stale left-box count (binned_sah.rs:188-235) can lose hits, so only the reference-order result is pinned.
     assert np.array_equal(mbvh.occluded(dv, _dev(rays8), reference_order=True).cpu().numpy(),
                           want_m)
     assert np.array_equal(bvh.occluded(dv, _dev(rays8), reference_order=True).cpu().numpy(),

@@ -132,7 +132,7 @@ def test_locb_teapot_reference_tests():
     """locb.rs:336-388: n <= #nodes <= 2n, valid tree, root strictly contains
     every vertex.  Strengthened: every primitive exactly once, 2n-1 nodes,
     every parent the exact union of its children, leaves = aabb +- 1e-4 and
-    one primitive per leaf (Q15)."""
+    one primitive per leaf (bvh.rs:126,135 ignore primitives_per_leaf)."""
     verts = teapot()
     aabbs, centers = prep(verts)
     nodes, idx = ref.build_locb(aabbs, centers)
@@ -321,7 +321,7 @@ def test_binned_small_cases():
 
 
 def test_binned_contained_variant():
-    """The Q2/Q3 fixes change bounds only where the reference's boxes were
+    """The two box-defect fixes (stale left-box count, double leaf pad) change bounds only where the reference's boxes were
     wrong: the fixed tree has no containment violations, traces exactly to
     brute force, and its SAH cost is within 1% of the reference's."""
     verts = teapot()
@@ -329,7 +329,7 @@ def test_binned_contained_variant():
     n0, i0 = ref.build_binned_sah(aabbs, centers)
     n1, i1 = ref.build_binned_sah(aabbs, centers, contained=True)
     assert ref.check_bvh(n1, i1, len(verts))[1] == 0
-    assert ref.check_bvh(n0, i0, len(verts))[1] > 0  # Q2/Q3 visible in the reference
+    assert ref.check_bvh(n0, i0, len(verts))[1] > 0  # defects visible in the reference
     assert abs(ref.sah_cost(n1) - ref.sah_cost(n0)) <= 0.01 * ref.sah_cost(n0)
     rays = as_rays(ray_set(verts, 5000, seed=3))
     assert hits_equal(ref.trace_mbvh(ref.build_mbvh(n1), i1, verts, rays, threads=4),
@@ -340,7 +340,7 @@ def test_binned_contained_variant():
 def test_mbvh_hand_tree():
     """merge_nodes (mbvh_node.rs:297-411) on a hand-built 7-node tree:
     root(1,2); 1 = leaf; 2 = interior(3,4); 3 = interior(5,6); 4,5,6 leaves.
-    Expected: MBVH root slot 0 = leaf of node 1 with the ROOT's bounds (Q8),
+    Expected: MBVH root slot 0 = leaf of node 1 with the ROOT's bounds (mbvh_node.rs:305,319-321),
     slot 1 empty, slot 2 = MBVH node for 3, slot 3 = leaf of node 4; the MBVH
     node for 3 has slots 0 = leaf 5 (root's... its own parent 3 bounds) and
     2 = leaf 6."""
