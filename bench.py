@@ -122,7 +122,22 @@ def run_batched(args, dev, rt, scenes, torch):
         batch_trace_ms.append((t2 - t1) * 1e3)
         del b
     same = bool(torch.equal(hits_plain.view(torch.int32), hits_batch.view(torch.int32)))
+    # the same frame as one hipGraph (build + collapse + triangles + trace)
+    plan = rt.BatchPlan(c_d, offs, v_d, aabbs=a_d)
+    hits_graph = torch.empty_like(hits_plain)
+    graph_ms = []
+    for k in range(4):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.launch(c_d, v_d, rays, roff_d, aabbs=a_d, out=hits_graph)
+        torch.cuda.synchronize()
+        if k > 0:  # the first call records the graph
+            graph_ms.append((time.perf_counter() - t0) * 1e3)
+    plan.status()
+    same_graph = bool(torch.equal(hits_graph.view(torch.int32), hits_batch.view(torch.int32)))
+    del hits_graph, plan
     p_ms, b_ms = statistics.median(plain_ms), statistics.median(batch_ms)
+    g_ms = statistics.median(graph_ms)
     bt_ms = statistics.median(batch_trace_ms)
     res = {
         "scenes": len(counts),
@@ -136,15 +151,91 @@ def run_batched(args, dev, rt, scenes, torch):
         "batched_mrays_s": round(n_rays / (bt_ms * 1e-3) / 1e6, 1),
         "plain_mrays_s": round(n_rays / (plain_trace_ms * 1e-3) / 1e6, 1),
         "same_hits": same,
+        "graph_ms": round(g_ms, 2),
+        "graph_same_hits": same_graph,
+        "graph_vs_plain": round(p_ms / g_ms, 3),
         "workload": "scene sizes round(exp(U(ln 1e3, ln 5e5))), alternating noisy sphere / "
                     "soup; 16 rays per triangle, origin uniform in the scene box, uniform "
                     "direction; whole job = PLOC + collapse + triangles + closest hit",
     }
     log(f"batched {b_ms:.1f} ms vs plain {p_ms:.1f} ms (x{p_ms / b_ms:.2f}); trace "
-        f"{bt_ms:.1f} vs {plain_trace_ms:.1f} ms; same hits {same}")
+        f"{bt_ms:.1f} vs {plain_trace_ms:.1f} ms; same hits {same}; graph {g_ms:.1f} ms "
+        f"(same {same_graph})")
     del rays, hits_plain, hits_batch, v_d, a_d, c_d
     torch.cuda.empty_cache()
-    return res, p_ms / b_ms
+    # the batched path proper is the recorded frame (one graph: build + trace)
+    return res, p_ms / g_ms
+
+
+def run_large(args, dev, rt, scenes, torch):
+    """64 x 64 displaced spheres (19,988,480 triangles) animated per frame:
+    each frame moves the vertices, recomputes boxes and centres, rebuilds
+    (PLOC + collapse + leaf-ordered triangles) and traces 2^28 any-hit shadow
+    rays to a point light — the whole frame as one recorded graph."""
+    log("generating the 20M-triangle sphere grid")
+    base = torch.from_numpy(scenes.sphere_grid()).to(dev)
+    n = base.shape[0]
+    verts = torch.empty_like(base)
+    aabbs = torch.empty((n, 8), dtype=torch.float32, device=dev)
+    centers = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    scenes.animate_y_torch(base, 0, verts)
+    scenes.triangle_aabbs_centers_torch(verts, aabbs, centers)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed + 2)
+    n_rays = 1 << args.large_rays_log2
+    rays = scenes.shadow_rays_torch(verts, n_rays, (32.0, 20.0, 32.0), dev, gen)
+    occ = torch.empty((n_rays,), dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bvh = rt.Builder(centers, aabbs=aabbs).construct_locally_ordered_clustered()
+    torch.cuda.synchronize()
+    eager_build_ms = (time.perf_counter() - t0) * 1e3
+    del bvh
+    plan = rt.BatchPlan(centers, [0, n], verts, aabbs=aabbs)
+    frame_ms = []
+    for f in range(1, 5):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        scenes.animate_y_torch(base, f, verts)
+        scenes.triangle_aabbs_centers_torch(verts, aabbs, centers)
+        plan.launch(centers, verts, rays, None, aabbs=aabbs, any_hit=True, out=occ)
+        torch.cuda.synchronize()
+        if f > 1:  # the first launch records the graph
+            frame_ms.append((time.perf_counter() - t0) * 1e3)
+    plan.status()
+    occluded = float(occ.float().mean().item())
+    # the last frame against an eager build + any-hit of the same vertices
+    bvh = rt.Builder(centers, aabbs=aabbs).construct_locally_ordered_clustered()
+    m = rt.Mbvh.from_bvh(bvh)
+    m.attach_triangles(verts)
+    want = m.occluded_prepared(rays)
+    same = bool(torch.equal(want, occ))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m.occluded_prepared(rays, out=want)
+    torch.cuda.synchronize()
+    trace_ms = (time.perf_counter() - t0) * 1e3
+    fm = statistics.median(frame_ms)
+    res = {
+        "tris": n,
+        "rays": n_rays,
+        "frame_ms": round(fm, 2),
+        "frame_mrays_s": round(n_rays / (fm * 1e-3) / 1e6, 1),
+        "eager_ploc_build_ms": round(eager_build_ms, 2),
+        "any_hit_trace_ms": round(trace_ms, 2),
+        "any_hit_mrays_s": round(n_rays / (trace_ms * 1e-3) / 1e6, 1),
+        "occluded_fraction": round(occluded, 4),
+        "plan_iterations": plan.iterations,
+        "same_as_eager": same,
+        "workload": "64x64 seeded displaced spheres (61x40 quads each); per frame y += "
+                    "0.02 sin(2 pi (x + f/60)), boxes/centres, PLOC rebuild, collapse, "
+                    "triangles and 2^28 any-hit shadow rays to a point light, one graph",
+    }
+    log(f"large: frame {fm:.1f} ms ({res['frame_mrays_s']} Mrays/s incl. rebuild), any-hit "
+        f"{trace_ms:.1f} ms, eager PLOC {eager_build_ms:.1f} ms, same {same}")
+    del plan, m, bvh, rays, occ, want, base, verts, aabbs, centers
+    torch.cuda.empty_cache()
+    return res
 
 
 def main():
@@ -158,8 +249,13 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--quick", action="store_true",
                     help="one build of each kind and no binned SAH (for profiled runs)")
+    ap.add_argument("--max-leaf", type=int, default=4,
+                    help="primitives per leaf of the traversal tree")
     ap.add_argument("--batch-scenes", type=int, default=256,
                     help="scenes in the batched-vs-plain comparison (0 skips it)")
+    ap.add_argument("--large", type=int, default=1,
+                    help="1: also run the animated 20M-triangle any-hit frame (0 skips it)")
+    ap.add_argument("--large-rays-log2", type=int, default=28)
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="GiB copied by one plain device copy at the end (scales rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -249,13 +345,27 @@ def main():
         if i > 0:
             attach_ms.append((time.perf_counter() - t0) * 1e3)
 
+    tree_ms = []
+    ttree = None
+    for i in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ttree = bvh.trace_tree(v_d, max_leaf=args.max_leaf)
+        torch.cuda.synchronize()
+        if i > 0:
+            tree_ms.append((time.perf_counter() - t0) * 1e3)
+    log(f"traversal tree: {ttree.node_count} nodes, {statistics.median(tree_ms):.2f} ms")
+
     def trace_vertex(rays, out):
         mbvh.intersect(v_d, rays, out=out)
 
     def trace_prepared(rays, out):
         mbvh.intersect_prepared(rays, out=out)
 
-    def timed(rays, steps, warmup, fn=trace_prepared):
+    def trace_tree(rays, out):
+        ttree.intersect(rays, out=out)
+
+    def timed(rays, steps, warmup, fn=trace_tree):
         out = torch.empty((rays.shape[0], 4), dtype=torch.float32, device=dev)
         for _ in range(warmup):
             fn(rays, out)
@@ -283,27 +393,42 @@ def main():
     ms_per_step = sum(per) / len(per)
     value = n_rays / (ms_per_step * 1e-3) / 1e6
     log(f"bounce {value:.1f} Mrays/s ({ms_per_step:.2f} ms/step), primary {primary_mrays:.1f} Mrays/s")
+    # the reference-layout MBVH (the collapse of src/bvh.rs:381-404) traced over
+    # leaf-ordered triangles and over the caller's vertex array: same hits
+    hits_m, per_m, _ = timed(bounce, 3, 1, fn=trace_prepared)
+    mbvh_mrays = n_rays / (statistics.median(per_m) * 1e-3) / 1e6
+    same_mbvh_hits = bool(torch.equal(hits_m.view(torch.int32), hits.view(torch.int32)))
+    del hits_m
     hits_v, per_v, _ = timed(bounce, 3, 1, fn=trace_vertex)
     vertex_mrays = n_rays / (statistics.median(per_v) * 1e-3) / 1e6
     same_vertex_hits = bool(torch.equal(hits_v.view(torch.int32), hits.view(torch.int32)))
     del hits_v
-    log(f"vertex-array path {vertex_mrays:.1f} Mrays/s; same hits {same_vertex_hits}")
+    log(f"MBVH + leaf-ordered triangles {mbvh_mrays:.1f} Mrays/s (same {same_mbvh_hits}), "
+        f"MBVH + vertex array {vertex_mrays:.1f} Mrays/s (same {same_vertex_hits})")
 
-    # ---- kernel work counters (MBVH nodes and triangles per ray) -----------
-    counters = torch.zeros(2, dtype=torch.int64, device=dev)
-    stats_out = torch.empty_like(hits)
+    # ---- kernel work counters (nodes and triangles per ray) ----------------
     import ctypes
 
-    _lib.check(_lib.lib().rtbvh_gpu_intersect_mbvh_stats(
-        mbvh._h, ctypes.c_void_p(v_d.data_ptr()), n_tris, ctypes.c_void_p(bounce.data_ptr()),
-        n_rays, ctypes.c_void_p(stats_out.data_ptr()), 0, ctypes.c_void_p(counters.data_ptr()),
-        ctypes.c_void_p(stream.cuda_stream)), "rtbvh_gpu_intersect_mbvh_stats")
-    torch.cuda.synchronize()
-    nodes_per_ray = counters[0].item() / n_rays
-    tris_per_ray = counters[1].item() / n_rays
-    same_stats_hits = bool(torch.equal(stats_out.view(torch.int32), hits.view(torch.int32)))
-    del stats_out
-    # ray 32 B + hit 16 B + 128 B per MBVH node + 48 B per leaf-ordered triangle
+    def work_counts(call):
+        counters = torch.zeros(2, dtype=torch.int64, device=dev)
+        stats_out = torch.empty_like(hits)
+        _lib.check(call(stats_out, counters), "stats")
+        torch.cuda.synchronize()
+        same = bool(torch.equal(stats_out.view(torch.int32), hits.view(torch.int32)))
+        return counters[0].item() / n_rays, counters[1].item() / n_rays, same
+
+    nodes_per_ray, tris_per_ray, same_stats_hits = work_counts(
+        lambda out, cnt: _lib.lib().rtbvh_gpu_trace_tree_intersect_stats(
+            ttree._h, ctypes.c_void_p(bounce.data_ptr()), n_rays, ctypes.c_void_p(out.data_ptr()),
+            ctypes.c_void_p(cnt.data_ptr()), ctypes.c_void_p(stream.cuda_stream)))
+    m_nodes_per_ray, m_tris_per_ray, _ = work_counts(
+        lambda out, cnt: _lib.lib().rtbvh_gpu_intersect_mbvh_stats(
+            mbvh._h, ctypes.c_void_p(v_d.data_ptr()), n_tris, ctypes.c_void_p(bounce.data_ptr()),
+            n_rays, ctypes.c_void_p(out.data_ptr()), 0, ctypes.c_void_p(cnt.data_ptr()),
+            ctypes.c_void_p(stream.cuda_stream)))
+    log(f"per ray: traversal tree {nodes_per_ray:.2f} nodes / {tris_per_ray:.2f} triangles, "
+        f"MBVH {m_nodes_per_ray:.2f} / {m_tris_per_ray:.2f}")
+    # ray 32 B + hit 16 B + 128 B per node + 48 B per leaf-ordered triangle
     b_ray = 32 + 16 + 128 * nodes_per_ray + 48 * tris_per_ray
     achieved = value * 1e6 * b_ray / 1e9
     traffic = None
@@ -353,6 +478,8 @@ def main():
         torch.cuda.empty_cache()
         batch_info, batch_ratio = run_batched(args, dev, rt, scenes, torch)
 
+    large_info = run_large(args, dev, rt, scenes, torch) if args.large else None
+
     result = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -367,7 +494,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "1M-tri noisy sphere + soup; GPU PLOC (r=14) + MBVH + leaf-ordered triangles; closest hit of "
+            "workload": "1M-tri noisy sphere + soup; GPU PLOC (r=14) + 4-wide traversal tree; closest hit of "
                         "8192^2 cosine-bounce rays",
             "tris": n_tris,
             "rays_per_step": n_rays,
@@ -376,16 +503,23 @@ def main():
         "ploc_build_ms": round(statistics.median(build_ms), 3),
         "mbvh_collapse_ms": round(statistics.median(collapse_ms), 3),
         "attach_triangles_ms": round(statistics.median(attach_ms), 3),
+        "trace_tree_build_ms": round(statistics.median(tree_ms), 3),
+        "trace_tree_max_leaf": args.max_leaf,
         "binned_sah_build_ms": round(statistics.median(binned_ms), 3),
         "primary_mrays_s": round(primary_mrays, 2),
-        "vertex_path_mrays_s": round(vertex_mrays, 2),
-        "vertex_path_same_hits": same_vertex_hits,
+        "mbvh_prepared_mrays_s": round(mbvh_mrays, 2),
+        "mbvh_prepared_same_hits": same_mbvh_hits,
+        "mbvh_vertex_mrays_s": round(vertex_mrays, 2),
+        "mbvh_vertex_same_hits": same_vertex_hits,
+        "mbvh_nodes_per_ray": round(m_nodes_per_ray, 3),
+        "mbvh_tris_per_ray": round(m_tris_per_ray, 3),
         "wall_s_timed": round(wall, 4),
         "nodes_per_ray": round(nodes_per_ray, 3),
         "tris_per_ray": round(tris_per_ray, 3),
         "stats_kernel_same_hits": same_stats_hits,
         "batched_vs_plain": None if batch_ratio is None else round(batch_ratio, 3),
         "batched": batch_info,
+        "large_animated": large_info,
         "parity_vs_cpu_oracle": parity,
         "roofline": {
             "bound": "hbm",

@@ -247,6 +247,55 @@ ResultCode rtbvh_gpu_intersect_mbvh_prepared(RTMbvh mbvh, const RTRay *rays, siz
 ResultCode rtbvh_gpu_occluded_mbvh_prepared(RTMbvh mbvh, const RTRay *rays, size_t ray_count,
                                             uint8_t *occluded, uint32_t flags, void *stream);
 
+/* ---- Traversal trees --------------------------------------------------------
+ * A 4-wide tree built from a binary tree for the lowest-(t, id) traversal
+ * only (no counterpart in the reference): each node opens the child with the
+ * largest surface area until it has four slots, slot bounds are the
+ * children's own, and subtrees of at most max_leaf primitives become one
+ * leaf whose triangles are stored contiguously (48 bytes each).  Hits are
+ * identical to rtbvh_gpu_intersect_mbvh's default mode on the same
+ * triangles; the tree is a snapshot of `vertices`. */
+typedef struct RTTraceTree {
+    uint32_t id;
+    uint32_t node_count;
+    uint32_t prim_count;
+    uint32_t max_leaf;
+} RTTraceTree;
+
+/* bvh: made by rtbvh_gpu_create_bvh; vertices: device, 9 floats per
+ * primitive; max_leaf 1..8 (0: 4). */
+ResultCode rtbvh_gpu_create_trace_tree(RTBvh bvh, const float *vertices, size_t prim_count,
+                                       uint32_t max_leaf, void *stream, RTTraceTree *result);
+ResultCode rtbvh_gpu_trace_tree_intersect(RTTraceTree tree, const RTRay *rays, size_t ray_count,
+                                          RTHit *hits, uint32_t flags, void *stream);
+/* rtbvh_gpu_trace_tree_intersect that also adds {nodes visited, triangles
+ * tested} into counters[0..1] (device, uint64; not cleared). */
+ResultCode rtbvh_gpu_trace_tree_intersect_stats(RTTraceTree tree, const RTRay *rays,
+                                                size_t ray_count, RTHit *hits, uint64_t *counters,
+                                                void *stream);
+ResultCode rtbvh_gpu_trace_tree_occluded(RTTraceTree tree, const RTRay *rays, size_t ray_count,
+                                         uint8_t *occluded, uint32_t flags, void *stream);
+void rtbvh_gpu_free_trace_tree(RTTraceTree tree);
+
+/* ---- Launch shape of the lowest-id MBVH traversal ---------------------------
+ * (no counterpart in the reference).  Per device; the default is
+ * {256, 12, 100}.  Results never depend on it, only speed. */
+typedef struct RTTraceConfig {
+    uint32_t block;          /* threads per block: 64, 128 or 256 */
+    uint32_t lds_stack;      /* traversal stack entries per ray kept in LDS, 0..32 */
+    uint32_t occupancy_pct;  /* persistent grid as a share of resident blocks, 10..400 */
+} RTTraceConfig;
+
+ResultCode rtbvh_gpu_set_trace_config(const RTTraceConfig *config); /* NULL: default */
+ResultCode rtbvh_gpu_get_trace_config(RTTraceConfig *config);
+/* Autotuner: times 54 launch shapes (block 64/128/256 x LDS stack 0..24 x
+ * occupancy 50/100/200 %) of the closest-hit traversal of `rays` against
+ * `mbvh`'s attached triangles, writing `hits` (device, ray_count records),
+ * and makes the fastest the current device's shape (also returned in *best).
+ * Synchronises the stream. */
+ResultCode rtbvh_gpu_autotune_mbvh(RTMbvh mbvh, const RTRay *rays, size_t ray_count, RTHit *hits,
+                                   void *stream, RTTraceConfig *best);
+
 /* ---- Adjoint of the hit ----------------------------------------------------
  * Gradients of a loss L through closest hits made by the rtbvh_gpu_intersect*
  * calls (no counterpart in the reference, whose triangle test is forward
@@ -290,8 +339,9 @@ ResultCode rtbvh_gpu_create_batch(const RTAabb *aabbs, const float *centers, siz
                                   const uint32_t *scene_offsets, size_t scene_count,
                                   void *stream, RTBatch *result);
 
-/* Leaf-ordered triangles for the batch (as rtbvh_gpu_mbvh_attach_triangles),
- * vertices: device, 9 floats per primitive of the batch. */
+/* Builds the batch's traversal tree (as rtbvh_gpu_create_trace_tree, leaves
+ * of up to 4 primitives, scene-local ids) from vertices: device, 9 floats per
+ * primitive of the batch.  Needed before tracing the batch. */
 ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch batch, const float *vertices,
                                             size_t prim_count, void *stream);
 
@@ -306,6 +356,52 @@ ResultCode rtbvh_gpu_occluded_batch(RTBatch batch, const RTRay *rays, const uint
                                     size_t ray_count, uint8_t *occluded, uint32_t flags,
                                     void *stream);
 void rtbvh_gpu_free_batch(RTBatch batch);
+
+/* ---- Capturable batch build + trace ----------------------------------------
+ * A plan holds every buffer for one batch shape (scene offsets fixed at
+ * creation) and enqueues PLOC build, traversal tree (as
+ * rtbvh_gpu_create_trace_tree, leaves of up to 4 primitives) and traversal
+ * without allocating and without waiting for the device, so the whole frame
+ * can live in a hipGraph (no counterpart in the reference).  Clustering
+ * iterations and tree levels are fixed at creation by an eager build of the
+ * given inputs with headroom; when later inputs need more, the traversal of
+ * that frame does nothing and rtbvh_gpu_batch_plan_status returns Error.
+ * Calls on one plan must not overlap. */
+typedef struct RTBatchPlan {
+    uint32_t id;
+    uint32_t scene_count;
+    uint32_t prim_count;
+    uint32_t iterations;  /* multi-block clustering iterations per build */
+} RTBatchPlan;
+
+/* aabbs (nullable), centers, vertices: device, as rtbvh_gpu_create_batch
+ * and rtbvh_gpu_batch_attach_triangles (an eager build of these sizes the
+ * budgets); scene_offsets: host, scene_count + 1 entries (scene_count may be
+ * 1). */
+ResultCode rtbvh_gpu_batch_plan_create(const RTAabb *aabbs, const float *centers,
+                                       size_t center_stride, const float *vertices,
+                                       const uint32_t *scene_offsets, size_t scene_count,
+                                       void *stream, RTBatchPlan *result);
+/* Enqueues build + collapse + triangles + traversal on `stream`.  Exactly one
+ * of hits (closest hit, scene-local ids) and occluded (any hit) is non-NULL;
+ * ray_offsets (device, scene_count + 1 entries) may be NULL for one scene.
+ * aabbs may be NULL only if it was NULL at creation. */
+ResultCode rtbvh_gpu_batch_plan_enqueue(RTBatchPlan plan, const RTAabb *aabbs,
+                                        const float *centers, const float *vertices,
+                                        const RTRay *rays, const uint32_t *ray_offsets,
+                                        size_t ray_count, RTHit *hits, uint8_t *occluded,
+                                        void *stream);
+/* The same work as one hipGraph: recorded on the first call for a set of
+ * argument pointers (after one eager pass), replayed by later calls with the
+ * same pointers; ordered after prior work on `stream` and before later work. */
+ResultCode rtbvh_gpu_batch_plan_launch(RTBatchPlan plan, const RTAabb *aabbs, const float *centers,
+                                       const float *vertices, const RTRay *rays,
+                                       const uint32_t *ray_offsets, size_t ray_count, RTHit *hits,
+                                       uint8_t *occluded, void *stream);
+/* Synchronises; Error if the last build ran out of clustering iterations,
+ * else rtbvh_gpu_status. */
+ResultCode rtbvh_gpu_batch_plan_status(RTBatchPlan plan, void *stream);
+void rtbvh_gpu_batch_plan_free(RTBatchPlan plan);
 
 /* Any-hit (shadow) traversal: occluded[i] = 1 when some triangle is hit with
  * t_min < t < t_max, else 0 (the FFI's early-out callback, rtbvh_ffi/src/

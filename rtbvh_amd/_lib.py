@@ -53,6 +53,21 @@ class RTBatch(ctypes.Structure):
                 ("indices", ctypes.c_void_p)]
 
 
+class RTBatchPlan(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint32), ("scene_count", ctypes.c_uint32),
+                ("prim_count", ctypes.c_uint32), ("iterations", ctypes.c_uint32)]
+
+
+class RTTraceConfig(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_uint32), ("lds_stack", ctypes.c_uint32),
+                ("occupancy_pct", ctypes.c_uint32)]
+
+
+class RTTraceTree(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint32), ("node_count", ctypes.c_uint32),
+                ("prim_count", ctypes.c_uint32), ("max_leaf", ctypes.c_uint32)]
+
+
 CALLBACK = ctypes.CFUNCTYPE(ctypes.c_bool, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float),
                             ctypes.c_void_p)
 
@@ -78,7 +93,23 @@ SIGNATURES = {
     "rtbvh_gpu_mbvh_attach_triangles": (_rc, [RTMbvh, _vp, _sz, _vp]),
     "rtbvh_gpu_intersect_mbvh_prepared": (_rc, [RTMbvh, _vp, _sz, _vp, _u32, _vp]),
     "rtbvh_gpu_occluded_mbvh_prepared": (_rc, [RTMbvh, _vp, _sz, _vp, _u32, _vp]),
+    "rtbvh_gpu_create_trace_tree": (_rc, [RTBvh, _vp, _sz, _u32, _vp, ctypes.POINTER(RTTraceTree)]),
+    "rtbvh_gpu_trace_tree_intersect": (_rc, [RTTraceTree, _vp, _sz, _vp, _u32, _vp]),
+    "rtbvh_gpu_trace_tree_intersect_stats": (_rc, [RTTraceTree, _vp, _sz, _vp, _vp, _vp]),
+    "rtbvh_gpu_trace_tree_occluded": (_rc, [RTTraceTree, _vp, _sz, _vp, _u32, _vp]),
+    "rtbvh_gpu_free_trace_tree": (None, [RTTraceTree]),
+    "rtbvh_gpu_set_trace_config": (_rc, [ctypes.POINTER(RTTraceConfig)]),
+    "rtbvh_gpu_get_trace_config": (_rc, [ctypes.POINTER(RTTraceConfig)]),
+    "rtbvh_gpu_autotune_mbvh": (_rc, [RTMbvh, _vp, _sz, _vp, _vp, ctypes.POINTER(RTTraceConfig)]),
     "rtbvh_gpu_adjoint": (_rc, [_vp, _sz, _vp, _vp, _sz, _vp, _vp, _vp, _vp, _vp]),
+    "rtbvh_gpu_batch_plan_create": (_rc, [_vp, _vp, _sz, _vp, _vp, _sz, _vp,
+                                         ctypes.POINTER(RTBatchPlan)]),
+    "rtbvh_gpu_batch_plan_enqueue": (_rc, [RTBatchPlan, _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp,
+                                          _vp]),
+    "rtbvh_gpu_batch_plan_launch": (_rc, [RTBatchPlan, _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp,
+                                         _vp]),
+    "rtbvh_gpu_batch_plan_status": (_rc, [RTBatchPlan, _vp]),
+    "rtbvh_gpu_batch_plan_free": (None, [RTBatchPlan]),
     "rtbvh_gpu_create_batch": (_rc, [_vp, _vp, _sz, _vp, _sz, _vp, ctypes.POINTER(RTBatch)]),
     "rtbvh_gpu_batch_attach_triangles": (_rc, [RTBatch, _vp, _sz, _vp]),
     "rtbvh_gpu_intersect_batch": (_rc, [RTBatch, _vp, _vp, _sz, _vp, _u32, _vp]),

@@ -203,9 +203,64 @@ class Bvh(_Tree):
             raise ValueError("aabbs must be (prim_count, 8)")
         L.check(L.lib().rtbvh_gpu_refit(_ptr(aabbs), self._h, _stream(aabbs)), "rtbvh_gpu_refit")
 
+    def trace_tree(self, vertices: torch.Tensor, max_leaf: int = 4) -> "TraceTree":
+        """Traversal tree over `vertices` (see TraceTree)."""
+        return TraceTree(self, vertices, max_leaf)
+
     def __del__(self):
         if getattr(self, "_owned", False) and L._lib is not None:
             L._lib.rtbvh_gpu_free_bvh(self._h)
+            self._owned = False
+
+
+class TraceTree:
+    """4-wide traversal tree with leaf-ordered triangles, built from a Bvh for
+    the lowest-(t, id) traversal (rtbvh_gpu_create_trace_tree); hits equal
+    Mbvh.intersect's default mode on the same triangles."""
+
+    def __init__(self, bvh: "Bvh", vertices: torch.Tensor, max_leaf: int = 4):
+        if not bvh._owned:
+            raise ValueError("a trace tree needs a Bvh built by this library")
+        _require_device(vertices, "vertices")
+        verts = vertices.reshape(-1)
+        self.device = vertices.device
+        self._owned = False
+        self._h = L.RTTraceTree()
+        L.check(L.lib().rtbvh_gpu_create_trace_tree(bvh._h, _ptr(verts), verts.numel() // 9,
+                                                    max_leaf, _stream(verts),
+                                                    ctypes.byref(self._h)),
+                "rtbvh_gpu_create_trace_tree")
+        self._owned = True
+
+    @property
+    def node_count(self) -> int:
+        return int(self._h.node_count)
+
+    def intersect(self, rays: torch.Tensor, out=None):
+        _require_device(rays, "rays")
+        out = torch.empty((rays.shape[0], HIT_FLOATS), dtype=torch.float32,
+                          device=rays.device) if out is None else out
+        L.check(L.lib().rtbvh_gpu_trace_tree_intersect(self._h, _ptr(rays), rays.shape[0],
+                                                       _ptr(out), 0, _stream(rays)),
+                "rtbvh_gpu_trace_tree_intersect")
+        return out
+
+    def occluded(self, rays: torch.Tensor, out=None):
+        _require_device(rays, "rays")
+        out = torch.empty((rays.shape[0],), dtype=torch.uint8,
+                          device=rays.device) if out is None else out
+        L.check(L.lib().rtbvh_gpu_trace_tree_occluded(self._h, _ptr(rays), rays.shape[0],
+                                                      _ptr(out), 0, _stream(rays)),
+                "rtbvh_gpu_trace_tree_occluded")
+        return out
+
+    def status(self):
+        L.check(L.lib().rtbvh_gpu_status(_stream(torch.empty(0, device=self.device))),
+                "rtbvh_gpu_status")
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and L._lib is not None:
+            L._lib.rtbvh_gpu_free_trace_tree(self._h)
             self._owned = False
 
 
@@ -399,6 +454,86 @@ class SceneBatch:
             self._owned = False
 
 
+class BatchPlan:
+    """A batch shape (scene offsets fixed) whose whole frame — PLOC build,
+    MBVH collapse, leaf-ordered triangles, traversal — is enqueued without
+    allocation or host synchronisation (rtbvh_gpu_batch_plan_*), so that it
+    can run as one hipGraph.  No counterpart in the reference."""
+
+    def __init__(self, primitives: torch.Tensor, scene_offsets, vertices: torch.Tensor,
+                 aabbs: Optional[torch.Tensor] = None):
+        _require_device(primitives, "primitives")
+        _require_device(vertices, "vertices")
+        if aabbs is not None:
+            _require_device(aabbs, "aabbs")
+        offs = [int(x) for x in scene_offsets]
+        if len(offs) < 2 or offs[-1] != primitives.shape[0]:
+            raise ValueError("scene_offsets must have scene_count + 1 entries ending at N")
+        self.scene_offsets = offs
+        self.device = primitives.device
+        self._h = L.RTBatchPlan()
+        self._owned = False
+        host = (ctypes.c_uint32 * len(offs))(*offs)
+        L.check(L.lib().rtbvh_gpu_batch_plan_create(_ptr(aabbs), _ptr(primitives),
+                                                    4 * int(primitives.shape[1]),
+                                                    _ptr(vertices.reshape(-1)), host,
+                                                    len(offs) - 1, _stream(primitives),
+                                                    ctypes.byref(self._h)),
+                "rtbvh_gpu_batch_plan_create")
+        self._owned = True
+
+    @property
+    def iterations(self) -> int:
+        return int(self._h.iterations)
+
+    def _run(self, fn, primitives, aabbs, vertices, rays, ray_offsets, out, any_hit):
+        _require_device(primitives, "primitives")
+        _require_device(vertices, "vertices")
+        _require_device(rays, "rays")
+        if aabbs is not None:
+            _require_device(aabbs, "aabbs")
+        if ray_offsets is not None and (ray_offsets.element_size() != 4 or
+                                        ray_offsets.numel() != len(self.scene_offsets)):
+            raise ValueError("ray_offsets must be a 32-bit device tensor of scene_count + 1 entries")
+        hits = None if any_hit else out
+        occ = out if any_hit else None
+        L.check(fn(self._h, _ptr(aabbs), _ptr(primitives), _ptr(vertices.reshape(-1)), _ptr(rays),
+                   _ptr(ray_offsets), rays.shape[0], _ptr(hits), _ptr(occ), _stream(rays)),
+                fn.__name__)
+        return out
+
+    def _out(self, rays, any_hit, out):
+        if out is not None:
+            return out
+        if any_hit:
+            return torch.empty((rays.shape[0],), dtype=torch.uint8, device=rays.device)
+        return torch.empty((rays.shape[0], HIT_FLOATS), dtype=torch.float32, device=rays.device)
+
+    def enqueue(self, primitives, vertices, rays, ray_offsets=None, aabbs=None, any_hit=False,
+                out=None):
+        """Build + trace on the current stream (stream-ordered, eager launches)."""
+        out = self._out(rays, any_hit, out)
+        return self._run(L.lib().rtbvh_gpu_batch_plan_enqueue, primitives, aabbs, vertices, rays,
+                         ray_offsets, out, any_hit)
+
+    def launch(self, primitives, vertices, rays, ray_offsets=None, aabbs=None, any_hit=False,
+               out=None):
+        """The same frame as one hipGraph (recorded on first use of these
+        tensors, replayed afterwards; keep passing the same tensors)."""
+        out = self._out(rays, any_hit, out)
+        return self._run(L.lib().rtbvh_gpu_batch_plan_launch, primitives, aabbs, vertices, rays,
+                         ray_offsets, out, any_hit)
+
+    def status(self):
+        L.check(L.lib().rtbvh_gpu_batch_plan_status(self._h, _stream(torch.empty(0, device=self.device))),
+                "rtbvh_gpu_batch_plan_status")
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and L._lib is not None:
+            L._lib.rtbvh_gpu_batch_plan_free(self._h)
+            self._owned = False
+
+
 class Builder:
     """rtbvh::Builder (src/bvh.rs:48-56): primitives are the centres."""
 
@@ -439,3 +574,31 @@ class Builder:
     def construct_binned_sah(self) -> Bvh:
         """src/bvh.rs:87-111 — 16-bin SAH on the GPU."""
         return self._construct(L.BINNED_SAH)
+
+
+def get_trace_config() -> dict:
+    """Launch shape of the lowest-id MBVH traversal on the current device."""
+    c = L.RTTraceConfig()
+    L.check(L.lib().rtbvh_gpu_get_trace_config(ctypes.byref(c)), "rtbvh_gpu_get_trace_config")
+    return {"block": c.block, "lds_stack": c.lds_stack, "occupancy_pct": c.occupancy_pct}
+
+
+def set_trace_config(block: Optional[int] = None, lds_stack: int = 12, occupancy_pct: int = 100):
+    """Sets the traversal launch shape (block=None restores the default)."""
+    if block is None:
+        L.check(L.lib().rtbvh_gpu_set_trace_config(None), "rtbvh_gpu_set_trace_config")
+        return
+    c = L.RTTraceConfig(block, lds_stack, occupancy_pct)
+    L.check(L.lib().rtbvh_gpu_set_trace_config(ctypes.byref(c)), "rtbvh_gpu_set_trace_config")
+
+
+def autotune(mbvh: "Mbvh", rays: torch.Tensor) -> dict:
+    """Times the candidate launch shapes on `rays` against `mbvh` (which needs
+    attached triangles) and keeps the fastest for the current device."""
+    _require_device(rays, "rays")
+    hits = torch.empty((rays.shape[0], HIT_FLOATS), dtype=torch.float32, device=rays.device)
+    c = L.RTTraceConfig()
+    L.check(L.lib().rtbvh_gpu_autotune_mbvh(mbvh._h, _ptr(rays), rays.shape[0], _ptr(hits),
+                                            _stream(rays), ctypes.byref(c)),
+            "rtbvh_gpu_autotune_mbvh")
+    return {"block": c.block, "lds_stack": c.lds_stack, "occupancy_pct": c.occupancy_pct}

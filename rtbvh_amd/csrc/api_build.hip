@@ -35,9 +35,15 @@ struct MbvhEntry {
     std::vector<uint32_t> host_indices;
 };
 
+struct TraceTreeEntry {
+    DeviceMbvh t;  // nodes + leaf-ordered triangles
+    int device = 0;
+};
+
 struct BatchEntry {
     DeviceBvh bvh;    // forest, roots [0, scenes)
     DeviceMbvh mbvh;  // forest, roots [0, scenes)
+    DeviceMbvh ttree; // traversal tree over the batch (after attach)
     uint32_t* scene_off = nullptr;  // device, scenes + 1 primitive offsets
     uint32_t scenes = 0;
     int device = 0;
@@ -48,6 +54,7 @@ struct Registry {
     std::vector<std::unique_ptr<BvhEntry>> bvhs;    // null = freed
     std::vector<std::unique_ptr<MbvhEntry>> mbvhs;
     std::vector<std::unique_ptr<BatchEntry>> batches;
+    std::vector<std::unique_ptr<TraceTreeEntry>> ttrees;
 };
 
 Registry& reg() {
@@ -72,6 +79,7 @@ void release(DeviceMbvh& d) {
 void release(BatchEntry& b) {
     release(b.bvh);
     release(b.mbvh);
+    release(b.ttree);
     if (b.scene_off) (void)hipFree(b.scene_off);
     b.scene_off = nullptr;
 }
@@ -292,19 +300,35 @@ const float4* attached_triangles(uint32_t id, uint32_t* count, int* device) {
     return me->dev.tris;
 }
 
-// The arrays a batched traversal needs (false if the id is unknown).
+// The arrays a batched traversal needs (false if the id is unknown); tris
+// is null until triangles are attached.
 bool batch_view(uint32_t id, BatchView* out) {
     Registry& r = reg();
     std::shared_lock<std::shared_mutex> lk(r.mu);
     BatchEntry* be = find_batch(id);
     if (be == nullptr) return false;
-    out->nodes = be->mbvh.nodes;
-    out->node_count = be->mbvh.node_count;
-    out->indices = be->mbvh.indices;
-    out->index_count = be->mbvh.prim_count;
-    out->tris = be->mbvh.tris;
+    out->nodes = be->ttree.nodes;
+    out->node_count = be->ttree.node_count;
+    out->indices = nullptr;
+    out->index_count = be->ttree.prim_count;
+    out->tris = be->ttree.tris;
     out->scenes = be->scenes;
     out->device = be->device;
+    return true;
+}
+
+bool trace_tree_view(uint32_t id, BatchView* out) {
+    Registry& r = reg();
+    std::shared_lock<std::shared_mutex> lk(r.mu);
+    if (id >= r.ttrees.size() || !r.ttrees[id]) return false;
+    const TraceTreeEntry& te = *r.ttrees[id];
+    out->nodes = te.t.nodes;
+    out->node_count = te.t.node_count;
+    out->indices = nullptr;
+    out->index_count = te.t.prim_count;
+    out->tris = te.t.tris;
+    out->scenes = te.t.root_count;
+    out->device = te.device;
     return true;
 }
 
@@ -472,8 +496,8 @@ ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch batch, const float* vertices
         set_error("unknown batch id %u", batch.id);
         return Error;
     }
-    if (prim_count != be->mbvh.prim_count) {
-        set_error("batch has %u primitives, got vertices for %zu", be->mbvh.prim_count,
+    if (prim_count != be->bvh.prim_count) {
+        set_error("batch has %u primitives, got vertices for %zu", be->bvh.prim_count,
                   prim_count);
         return InvalidArgument;
     }
@@ -482,19 +506,82 @@ ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch batch, const float* vertices
         set_error("vertices (%p) is not device memory", static_cast<const void*>(vertices));
         return InvalidArgument;
     }
-    const uint32_t entries = be->mbvh.prim_count;
-    if (be->mbvh.tris == nullptr) {
-        hipError_t e = hipMalloc(&be->mbvh.tris, size_t(entries) * 3 * sizeof(float4));
-        if (e != hipSuccess) {
-            be->mbvh.tris = nullptr;
-            return hip_fail(e, "hipMalloc(triangles)");
+    // the traversal tree of the batch (tight 4-wide forest, leaves of up to
+    // 4 triangles, scene-local ids)
+    release(be->ttree);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    ResultCode rc = ensure_parents(be->bvh, st);
+    if (rc != Ok) return rc;
+    TraceTreePlan* plan = nullptr;
+    rc = trace_tree_plan_create(be->bvh, kDefaultMaxLeaf, &plan, &be->ttree);
+    if (rc != Ok) return rc;
+    rc = trace_tree_build(plan, be->bvh, vertices, uint32_t(prim_count), be->scene_off,
+                          be->scenes, &be->ttree, st);
+    trace_tree_plan_destroy(plan);
+    if (rc != Ok) release(be->ttree);
+    return rc;
+}
+
+ResultCode rtbvh_gpu_create_trace_tree(RTBvh bvh, const float* vertices, size_t prim_count,
+                                       uint32_t max_leaf, void* stream, RTTraceTree* result) {
+    if (vertices == nullptr || result == nullptr) {
+        set_error("vertices and result must not be null");
+        return Error;
+    }
+    if (prim_count > 0xFFFFFFFFull) {
+        set_error("prim_count %zu exceeds 2^32-1", prim_count);
+        return InvalidArgument;
+    }
+    if (max_leaf == 0) max_leaf = kDefaultMaxLeaf;
+    std::unique_ptr<TraceTreeEntry> te(new TraceTreeEntry());
+    {
+        Registry& r = reg();
+        std::unique_lock<std::shared_mutex> lk(r.mu);  // may attach parent links
+        BvhEntry* be = find_bvh(bvh.id);
+        if (be == nullptr) {
+            set_error("unknown bvh id %u", bvh.id);
+            return Error;
+        }
+        DeviceGuard g(be->device);
+        if (!is_device_pointer(vertices)) {
+            set_error("vertices (%p) is not device memory", static_cast<const void*>(vertices));
+            return InvalidArgument;
+        }
+        te->device = be->device;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        ResultCode rc = ensure_parents(be->dev, st);
+        if (rc != Ok) return rc;
+        TraceTreePlan* plan = nullptr;
+        rc = trace_tree_plan_create(be->dev, max_leaf, &plan, &te->t);
+        if (rc != Ok) return rc;
+        rc = trace_tree_build(plan, be->dev, vertices, uint32_t(prim_count), nullptr, 0, &te->t,
+                              st);
+        trace_tree_plan_destroy(plan);
+        if (rc != Ok) {
+            release(te->t);
+            return rc;
         }
     }
-    hipError_t e = launch_prepare_triangles(be->mbvh.indices, entries, vertices, entries,
-                                            be->mbvh.tris, static_cast<hipStream_t>(stream),
-                                            be->scene_off, be->scenes);
-    if (e != hipSuccess) return hip_fail(e, "prepare triangles");
+    Registry& r = reg();
+    std::unique_lock<std::shared_mutex> lk(r.mu);
+    r.ttrees.push_back(std::move(te));
+    result->id = uint32_t(r.ttrees.size() - 1);
+    result->node_count = r.ttrees.back()->t.node_count;
+    result->prim_count = r.ttrees.back()->t.prim_count;
+    result->max_leaf = max_leaf;
     return Ok;
+}
+
+void rtbvh_gpu_free_trace_tree(RTTraceTree tree) {
+    Registry& r = reg();
+    std::unique_lock<std::shared_mutex> lk(r.mu);
+    if (tree.id >= r.ttrees.size() || !r.ttrees[tree.id]) {
+        fprintf(stderr, "Could not free trace tree with id: %u\n", tree.id);
+        return;
+    }
+    DeviceGuard g(r.ttrees[tree.id]->device);
+    release(r.ttrees[tree.id]->t);
+    r.ttrees[tree.id].reset();
 }
 
 void rtbvh_gpu_free_batch(RTBatch batch) {

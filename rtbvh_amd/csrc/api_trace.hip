@@ -146,51 +146,50 @@ ResultCode prepared_entry(RTMbvh mbvh, const RTRay* rays, size_t ray_count, RTHi
                        count, rays, ray_count, hits, occluded, flags, stream, true, nullptr, tris);
 }
 
-ResultCode batch_entry(RTBatch batch, const RTRay* rays, const uint32_t* ray_offsets,
-                       size_t ray_count, RTHit* hits, uint8_t* occluded, uint32_t flags,
-                       void* stream) {
+// Lowest-id traversal of a traversal tree or a batch's tree (leaf-ordered
+// triangles, no index array).
+ResultCode view_entry(const BatchView& v, const char* what, uint32_t id, const RTRay* rays,
+                      const uint32_t* ray_offsets, size_t ray_count, RTHit* hits,
+                      uint8_t* occluded, uint32_t flags, void* stream,
+                      uint64_t* stats = nullptr) {
     if (flags != 0) {
-        set_error("batched traversal supports the lowest-(t, id) rule only (flags 0)");
+        set_error("this traversal supports the lowest-(t, id) rule only (flags 0)");
         return InvalidArgument;
     }
     if (ray_count > 0xFFFFFFFFull) {
-        set_error("a batch traces at most 2^32-1 rays (got %zu)", ray_count);
+        set_error("at most 2^32-1 rays per call (got %zu)", ray_count);
         return InvalidArgument;
     }
-    BatchView v;
-    if (!batch_view(batch.id, &v)) {
-        set_error("unknown batch id %u", batch.id);
-        return Error;
-    }
     if (v.tris == nullptr) {
-        set_error("batch %u has no attached triangles (rtbvh_gpu_batch_attach_triangles)",
-                  batch.id);
+        set_error("%s %u has no attached triangles", what, id);
         return Error;
     }
     int cur = -1;
     if (hipGetDevice(&cur) != hipSuccess || cur != v.device) {
-        set_error("batch %u lives on device %d, current device is %d", batch.id, v.device, cur);
+        set_error("%s %u lives on device %d, current device is %d", what, id, v.device, cur);
         return InvalidArgument;
     }
     if (ray_count == 0) return Ok;
     ResultCode rc;
-    if (ray_offsets == nullptr) {
-        set_error("ray_offsets must not be null");
-        return Error;
-    }
-    if ((rc = check_device(ray_offsets, "ray_offsets")) != Ok) return rc;
+    if (ray_offsets && (rc = check_device(ray_offsets, "ray_offsets")) != Ok) return rc;
     if ((rc = check_device(rays, "rays")) != Ok) return rc;
     if (hits && (rc = check_device(hits, "hits")) != Ok) return rc;
     if (occluded && (rc = check_device(occluded, "occluded")) != Ok) return rc;
+    if (stats && (rc = check_device(stats, "counters")) != Ok) return rc;
     if ((reinterpret_cast<uintptr_t>(rays) & 15) || (hits && (reinterpret_cast<uintptr_t>(hits) & 15))) {
         set_error("rays and hits must be 16-byte aligned");
         return InvalidArgument;
     }
+    if (stats && (occluded || ray_offsets)) {
+        set_error("counters are available for single-tree closest hit only");
+        return InvalidArgument;
+    }
     TraceParams p{};
+    p.stats = stats;
     p.mnodes = v.nodes;
     p.node_count = v.node_count;
     p.index_count = v.index_count;
-    p.indices = v.indices;
+    p.indices = nullptr;
     p.tris = v.tris;
     p.rays = reinterpret_cast<const DevRay*>(rays);
     p.ray_count = ray_count;
@@ -204,8 +203,36 @@ ResultCode batch_entry(RTBatch batch, const RTRay* rays, const uint32_t* ray_off
         return DeviceError;
     }
     const hipError_t e = launch_trace_fast(p, occluded != nullptr, static_cast<hipStream_t>(stream));
-    if (e != hipSuccess) return hip_fail(e, "batch trace launch");
+    if (e != hipSuccess) return hip_fail(e, "trace launch");
     return Ok;
+}
+
+ResultCode batch_entry(RTBatch batch, const RTRay* rays, const uint32_t* ray_offsets,
+                       size_t ray_count, RTHit* hits, uint8_t* occluded, uint32_t flags,
+                       void* stream) {
+    BatchView v;
+    if (!batch_view(batch.id, &v)) {
+        set_error("unknown batch id %u", batch.id);
+        return Error;
+    }
+    if (ray_offsets == nullptr && ray_count > 0) {
+        set_error("ray_offsets must not be null");
+        return Error;
+    }
+    return view_entry(v, "batch", batch.id, rays, ray_offsets, ray_count, hits, occluded, flags,
+                      stream);
+}
+
+ResultCode tree_entry(RTTraceTree tree, const RTRay* rays, size_t ray_count, RTHit* hits,
+                      uint8_t* occluded, uint32_t flags, void* stream, uint64_t* stats = nullptr) {
+    BatchView v;
+    if (!trace_tree_view(tree.id, &v)) {
+        set_error("unknown trace tree id %u", tree.id);
+        return Error;
+    }
+    v.scenes = 1;
+    return view_entry(v, "trace tree", tree.id, rays, nullptr, ray_count, hits, occluded, flags,
+                      stream, stats);
 }
 
 }  // namespace
@@ -297,6 +324,34 @@ ResultCode rtbvh_gpu_adjoint(const float* vertices, size_t prim_count, const RTR
     return Ok;
 }
 
+ResultCode rtbvh_gpu_trace_tree_intersect(RTTraceTree tree, const RTRay* rays, size_t ray_count,
+                                          RTHit* hits, uint32_t flags, void* stream) {
+    if (hits == nullptr && ray_count > 0) {
+        set_error("hits is null");
+        return Error;
+    }
+    return tree_entry(tree, rays, ray_count, hits, nullptr, flags, stream);
+}
+
+ResultCode rtbvh_gpu_trace_tree_intersect_stats(RTTraceTree tree, const RTRay* rays,
+                                                size_t ray_count, RTHit* hits, uint64_t* counters,
+                                                void* stream) {
+    if ((hits == nullptr && ray_count > 0) || counters == nullptr) {
+        set_error("hits and counters must not be null");
+        return Error;
+    }
+    return tree_entry(tree, rays, ray_count, hits, nullptr, 0, stream, counters);
+}
+
+ResultCode rtbvh_gpu_trace_tree_occluded(RTTraceTree tree, const RTRay* rays, size_t ray_count,
+                                         uint8_t* occluded, uint32_t flags, void* stream) {
+    if (occluded == nullptr && ray_count > 0) {
+        set_error("occluded is null");
+        return Error;
+    }
+    return tree_entry(tree, rays, ray_count, nullptr, occluded, flags, stream);
+}
+
 ResultCode rtbvh_gpu_intersect_batch(RTBatch batch, const RTRay* rays, const uint32_t* ray_offsets,
                                      size_t ray_count, RTHit* hits, uint32_t flags, void* stream) {
     if (hits == nullptr && ray_count > 0) {
@@ -345,6 +400,98 @@ ResultCode rtbvh_gpu_occluded_mbvh(RTMbvh mbvh, const float* vertices, size_t pr
     }
     return trace_entry(mbvh.nodes, mbvh.node_count, mbvh.indices, mbvh.index_count, vertices,
                        prim_count, rays, ray_count, nullptr, occluded, flags, stream, true);
+}
+
+ResultCode rtbvh_gpu_set_trace_config(const RTTraceConfig* config) {
+    if (config == nullptr) {
+        set_trace_config(nullptr);
+        return Ok;
+    }
+    const TraceConfig c{config->block, config->lds_stack, config->occupancy_pct};
+    if (!valid_trace_config(c)) {
+        set_error("trace config: block 64/128/256, lds_stack <= 32, occupancy_pct 10..400");
+        return InvalidArgument;
+    }
+    set_trace_config(&c);
+    return Ok;
+}
+
+ResultCode rtbvh_gpu_get_trace_config(RTTraceConfig* config) {
+    if (config == nullptr) {
+        set_error("config must not be null");
+        return Error;
+    }
+    const TraceConfig c = get_trace_config();
+    config->block = c.block;
+    config->lds_stack = c.lds_depth;
+    config->occupancy_pct = c.occupancy_pct;
+    return Ok;
+}
+
+// Times every candidate launch shape on the caller's rays (closest hit over
+// the attached triangles, best of two runs each) and keeps the fastest as the
+// device's configuration.
+ResultCode rtbvh_gpu_autotune_mbvh(RTMbvh mbvh, const RTRay* rays, size_t ray_count, RTHit* hits,
+                                   void* stream, RTTraceConfig* best) {
+    if (rays == nullptr || hits == nullptr || ray_count == 0) {
+        set_error("autotuning needs rays and hits");
+        return Error;
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return hip_fail(e, "autotune events");
+    }
+    const TraceConfig before = get_trace_config();
+    static const uint32_t blocks[] = {64, 128, 256};
+    static const uint32_t depths[] = {0, 4, 8, 12, 16, 24};
+    static const uint32_t occ[] = {50, 100, 200};
+    TraceConfig top = before;
+    float top_ms = 3.4e38f;
+    ResultCode rc = Ok;
+    for (uint32_t b : blocks) {
+        for (uint32_t d : depths) {
+            for (uint32_t o : occ) {
+                const TraceConfig c{b, d, o};
+                set_trace_config(&c);
+                float ms_best = 3.4e38f;
+                for (int rep = 0; rep < 3 && rc == Ok; ++rep) {
+                    if ((e = hipEventRecord(e0, st)) != hipSuccess) break;
+                    rc = prepared_entry(mbvh, rays, ray_count, hits, nullptr, 0, stream);
+                    if (rc != Ok) break;
+                    if ((e = hipEventRecord(e1, st)) != hipSuccess ||
+                        (e = hipEventSynchronize(e1)) != hipSuccess)
+                        break;
+                    float ms = 0.f;
+                    if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+                    if (rep > 0 && ms < ms_best) ms_best = ms;  // first run warms up
+                }
+                if (rc != Ok || e != hipSuccess) break;
+                if (ms_best < top_ms) {
+                    top_ms = ms_best;
+                    top = c;
+                }
+            }
+            if (rc != Ok || e != hipSuccess) break;
+        }
+        if (rc != Ok || e != hipSuccess) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != Ok || e != hipSuccess) {
+        set_trace_config(&before);
+        return rc != Ok ? rc : hip_fail(e, "autotune");
+    }
+    set_trace_config(&top);
+    if (best) {
+        best->block = top.block;
+        best->lds_stack = top.lds_depth;
+        best->occupancy_pct = top.occupancy_pct;
+    }
+    return Ok;
 }
 
 ResultCode rtbvh_gpu_status(void* stream) {

@@ -812,224 +812,407 @@ __global__ void __launch_bounds__(kTailThreads)
     }
 }
 
-struct Workspace {
-    void* base = nullptr;
-    ~Workspace() {
-        if (base) (void)hipFree(base);
-    }
-};
-
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 constexpr uint32_t kMaxIterations = 1u << 14;
 constexpr int kBatch = 6;  // iterations enqueued between host checks
 
+// The first iteration's state (a kernel, so that the whole build can be
+// recorded into a graph).
+__global__ void init_state_kernel(PlocState* st, uint32_t n, uint32_t roots) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    PlocState s;
+    s.n_active = n;
+    s.begin = n - roots;
+    s.end = 2 * n - roots;
+    s.merged = 0;
+    s.buf = 0;
+    s.pad[0] = s.pad[1] = s.pad[2] = 0;
+    st[0] = s;
+}
+
+// fail = 1 when clustering stopped before one cluster per scene remained.
+__global__ void check_kernel(const PlocState* result, uint32_t n_final, uint32_t* fail) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    fail[0] = result->n_active == n_final ? 0u : 1u;
+}
+
 }  // namespace
+
+// Every buffer of one PLOC build of n primitives in `scenes` scenes.  The
+// tree (nodes, indices) has allocations of its own so that an eager build
+// can hand it over and free the rest.
+struct PlocPlanImpl {
+    uint32_t n = 0, scenes = 1, stride = 3;
+    bool seg = false, point_aabbs = false;
+    uint32_t iterations = 0;  // clustering iterations recorded by enqueue
+    DevAabb* nodes = nullptr;
+    uint32_t* indices = nullptr;
+    void* ws = nullptr;
+    // carved from ws
+    DevAabb* in_aabbs = nullptr;  // point boxes (aabbs == null)
+    Bounds6* part = nullptr;
+    Bounds6* world = nullptr;
+    uint32_t *k0 = nullptr, *k1 = nullptr, *v1 = nullptr, *counts = nullptr;
+    void* scan_scratch = nullptr;
+    size_t scan_bytes = 0;
+    DevAabb *cl0 = nullptr, *cl1 = nullptr;
+    uint32_t *nbuf = nullptr, *mbuf = nullptr, *tcount = nullptr;
+    PlocState* st = nullptr;
+    uint32_t *soff = nullptr, *pscene = nullptr, *cs0 = nullptr, *cs1 = nullptr;
+    uint32_t* fail = nullptr;
+    uint32_t parts = 256, sort_tiles = 0, ploc_tiles = 0, count_entries = 0;
+
+    uint32_t node_count() const { return seg ? 2 * n - scenes : (n <= 2 ? 1u : 2 * n - 1); }
+};
 
 namespace {
 
-// Shared driver of the single-scene build (scenes == 1, scene_off null) and
-// the batched build (scene_off: scenes + 1 host offsets, scenes >= 2).
-template <bool SEG>
-ResultCode build_ploc_impl(const RTAabb* aabbs, const float* centers, size_t stride_floats,
-                           uint32_t n, const uint32_t* scene_off, uint32_t scenes,
-                           hipStream_t stream, DeviceBvh* out) {
-    const uint32_t node_count = SEG ? 2 * n - scenes : (n <= 2 ? 1u : 2 * n - 1);
-    const uint32_t stride = uint32_t(stride_floats);
-    hipError_t e;
-    DeviceBvh res;
-    if ((e = hipMalloc(&res.nodes, size_t(node_count) * sizeof(DevAabb))) != hipSuccess)
-        return hip_fail(e, "hipMalloc(nodes)");
-    if ((e = hipMalloc(&res.indices, size_t(n) * sizeof(uint32_t))) != hipSuccess) {
-        (void)hipFree(res.nodes);
-        return hip_fail(e, "hipMalloc(indices)");
+void free_plan(PlocPlanImpl* p, bool keep_tree) {
+    if (!p) return;
+    if (!keep_tree) {
+        if (p->nodes) (void)hipFree(p->nodes);
+        if (p->indices) (void)hipFree(p->indices);
     }
-    res.node_count = node_count;
-    res.prim_count = n;
-    res.root_count = SEG ? scenes : 1u;
-    auto release_result = [&]() {
-        (void)hipFree(res.nodes);
-        (void)hipFree(res.indices);
-    };
-    auto fail = [&](hipError_t err, const char* what) {
-        release_result();
-        return hip_fail(err, what);
-    };
+    if (p->ws) (void)hipFree(p->ws);
+    delete p;
+}
 
-    const uint32_t sort_tiles = (n + kSortTile - 1) / kSortTile;
-    const uint32_t ploc_tiles = (n + kTile - 1) / kTile;
-    const uint32_t parts = SEG ? 16u : 256u;  // bounds partials (per scene when SEG)
+ResultCode alloc_plan(uint32_t n, uint32_t scenes, bool point_aabbs, size_t stride_floats,
+                      PlocPlanImpl** out) {
+    PlocPlanImpl* p = new PlocPlanImpl();
+    p->n = n;
+    p->scenes = scenes;
+    p->seg = scenes > 1;
+    p->point_aabbs = point_aabbs;
+    p->stride = uint32_t(stride_floats);
+    p->sort_tiles = (n + kSortTile - 1) / kSortTile;
+    p->ploc_tiles = (n + kTile - 1) / kTile;
+    p->parts = p->seg ? 16u : 256u;  // bounds partials (per scene when batched)
+    p->count_entries = 256u * p->sort_tiles;
+    p->scan_bytes = scan_scratch_bytes(p->count_entries);
     size_t off = 0;
     auto carve = [&](size_t bytes) {
         const size_t o = off;
         off += align_up(bytes);
         return o;
     };
-    const size_t o_aabb = aabbs ? 0 : carve(size_t(n) * sizeof(DevAabb));
-    const size_t o_part = carve(size_t(parts) * scenes * sizeof(Bounds6));
+    const size_t o_aabb = point_aabbs ? carve(size_t(n) * sizeof(DevAabb)) : 0;
+    const size_t o_part = carve(size_t(p->parts) * scenes * sizeof(Bounds6));
     const size_t o_world = carve(size_t(scenes) * sizeof(Bounds6));
     const size_t o_k0 = carve(size_t(n) * 4), o_k1 = carve(size_t(n) * 4);
     const size_t o_v1 = carve(size_t(n) * 4);
-    const uint32_t count_entries = 256u * sort_tiles;
-    const size_t o_cnt = carve(size_t(count_entries) * 4);
-    const size_t scan_bytes = scan_scratch_bytes(count_entries);
-    const size_t o_scan = carve(scan_bytes);
+    const size_t o_cnt = carve(size_t(p->count_entries) * 4);
+    const size_t o_scan = carve(p->scan_bytes);
     const size_t o_cl0 = carve(size_t(n) * sizeof(DevAabb));
     const size_t o_cl1 = carve(size_t(n) * sizeof(DevAabb));
     const size_t o_nb = carve(size_t(n) * 4), o_mark = carve(size_t(n) * 4);
-    const size_t o_tc = carve(size_t(ploc_tiles) * 4);
-    const size_t o_state = carve(size_t(kMaxIterations + 1) * sizeof(PlocState));
-    const size_t o_soff = SEG ? carve(size_t(scenes + 1) * 4) : 0;
-    const size_t o_pscene = SEG ? carve(size_t(n) * 4) : 0;
-    const size_t o_cs0 = SEG ? carve(size_t(n) * 4) : 0;
-    const size_t o_cs1 = SEG ? carve(size_t(n) * 4) : 0;
-    Workspace ws;
-    if ((e = hipMalloc(&ws.base, off)) != hipSuccess) return fail(e, "hipMalloc(workspace)");
-    char* b = static_cast<char*>(ws.base);
-    const DevAabb* in_aabbs = aabbs ? reinterpret_cast<const DevAabb*>(aabbs)
-                                    : reinterpret_cast<DevAabb*>(b + o_aabb);
-    Bounds6* part = reinterpret_cast<Bounds6*>(b + o_part);
-    Bounds6* world = reinterpret_cast<Bounds6*>(b + o_world);
-    uint32_t* k0 = reinterpret_cast<uint32_t*>(b + o_k0);
-    uint32_t* k1 = reinterpret_cast<uint32_t*>(b + o_k1);
-    uint32_t* v0 = res.indices;  // the final sorted order lands here
-    uint32_t* v1 = reinterpret_cast<uint32_t*>(b + o_v1);
-    uint32_t* counts = reinterpret_cast<uint32_t*>(b + o_cnt);
-    void* scan_scratch = b + o_scan;
-    DevAabb* cl0 = reinterpret_cast<DevAabb*>(b + o_cl0);
-    DevAabb* cl1 = reinterpret_cast<DevAabb*>(b + o_cl1);
-    uint32_t* nbuf = reinterpret_cast<uint32_t*>(b + o_nb);
-    uint32_t* mbuf = reinterpret_cast<uint32_t*>(b + o_mark);
-    uint32_t* tcount = reinterpret_cast<uint32_t*>(b + o_tc);
-    PlocState* st = reinterpret_cast<PlocState*>(b + o_state);
-    uint32_t* soff = SEG ? reinterpret_cast<uint32_t*>(b + o_soff) : nullptr;
-    uint32_t* pscene = SEG ? reinterpret_cast<uint32_t*>(b + o_pscene) : nullptr;
-    uint32_t* cs0 = SEG ? reinterpret_cast<uint32_t*>(b + o_cs0) : nullptr;
-    uint32_t* cs1 = SEG ? reinterpret_cast<uint32_t*>(b + o_cs1) : nullptr;
+    const size_t o_tc = carve(size_t(p->ploc_tiles) * 4);
+    const size_t o_state = carve(size_t(kMaxIterations + 2) * sizeof(PlocState));
+    const size_t o_soff = carve(size_t(scenes + 1) * 4);
+    const size_t o_pscene = p->seg ? carve(size_t(n) * 4) : 0;
+    const size_t o_cs0 = p->seg ? carve(size_t(n) * 4) : 0;
+    const size_t o_cs1 = p->seg ? carve(size_t(n) * 4) : 0;
+    const size_t o_fail = carve(4);
+    hipError_t e;
+    if ((e = hipMalloc(&p->nodes, size_t(p->node_count()) * sizeof(DevAabb))) != hipSuccess ||
+        (e = hipMalloc(&p->indices, size_t(n) * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&p->ws, off)) != hipSuccess) {
+        free_plan(p, false);
+        return hip_fail(e, "hipMalloc(ploc)");
+    }
+    char* b = static_cast<char*>(p->ws);
+    p->in_aabbs = point_aabbs ? reinterpret_cast<DevAabb*>(b + o_aabb) : nullptr;
+    p->part = reinterpret_cast<Bounds6*>(b + o_part);
+    p->world = reinterpret_cast<Bounds6*>(b + o_world);
+    p->k0 = reinterpret_cast<uint32_t*>(b + o_k0);
+    p->k1 = reinterpret_cast<uint32_t*>(b + o_k1);
+    p->v1 = reinterpret_cast<uint32_t*>(b + o_v1);
+    p->counts = reinterpret_cast<uint32_t*>(b + o_cnt);
+    p->scan_scratch = b + o_scan;
+    p->cl0 = reinterpret_cast<DevAabb*>(b + o_cl0);
+    p->cl1 = reinterpret_cast<DevAabb*>(b + o_cl1);
+    p->nbuf = reinterpret_cast<uint32_t*>(b + o_nb);
+    p->mbuf = reinterpret_cast<uint32_t*>(b + o_mark);
+    p->tcount = reinterpret_cast<uint32_t*>(b + o_tc);
+    p->st = reinterpret_cast<PlocState*>(b + o_state);
+    p->soff = reinterpret_cast<uint32_t*>(b + o_soff);
+    p->pscene = p->seg ? reinterpret_cast<uint32_t*>(b + o_pscene) : nullptr;
+    p->cs0 = p->seg ? reinterpret_cast<uint32_t*>(b + o_cs0) : nullptr;
+    p->cs1 = p->seg ? reinterpret_cast<uint32_t*>(b + o_cs1) : nullptr;
+    p->fail = reinterpret_cast<uint32_t*>(b + o_fail);
+    *out = p;
+    return Ok;
+}
 
-    const uint32_t g = (n + kBlock - 1) / kBlock;
+// Point boxes, world bounds, Morton codes, sort, leaves and the first state.
+// Needs p->soff on the device for batches.
+template <bool SEG>
+hipError_t enqueue_front(const PlocPlanImpl* p, const RTAabb* aabbs, const float* centers,
+                         hipStream_t stream) {
+    const uint32_t n = p->n, g = (n + kBlock - 1) / kBlock;
+    const DevAabb* in_aabbs = aabbs ? reinterpret_cast<const DevAabb*>(aabbs) : p->in_aabbs;
     if (!aabbs)
-        hipLaunchKernelGGL(point_aabbs_kernel, dim3(g), dim3(kBlock), 0, stream, centers, stride, n,
-                           const_cast<DevAabb*>(in_aabbs));
-    if (!SEG && n <= 2) {
-        hipLaunchKernelGGL(bounds_partial_kernel, dim3(1), dim3(kBlock), 0, stream, in_aabbs, n,
-                           part);
-        hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(kBlock), 0, stream, part, 1u, world);
-        hipLaunchKernelGGL(small_root_kernel, dim3(1), dim3(1), 0, stream, world, n, res.nodes,
-                           res.indices);
-        if ((e = hipGetLastError()) != hipSuccess) return fail(e, "small build");
-        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return fail(e, "small build");
-        *out = res;
-        return Ok;
-    }
+        hipLaunchKernelGGL(point_aabbs_kernel, dim3(g), dim3(kBlock), 0, stream, centers,
+                           p->stride, n, p->in_aabbs);
     if (SEG) {
-        if ((e = hipMemcpyAsync(soff, scene_off, size_t(scenes + 1) * 4, hipMemcpyHostToDevice,
-                                stream)) != hipSuccess)
-            return fail(e, "scene offsets");
-        hipLaunchKernelGGL(scene_of_kernel, dim3(g), dim3(kBlock), 0, stream, soff, scenes, n,
-                           pscene);
-        hipLaunchKernelGGL(bounds_seg_partial_kernel, dim3(parts, scenes), dim3(kBlock), 0, stream,
-                           in_aabbs, soff, part);
-        hipLaunchKernelGGL(bounds_final_kernel, dim3(scenes), dim3(kBlock), 0, stream, part, parts,
-                           world);
+        hipLaunchKernelGGL(scene_of_kernel, dim3(g), dim3(kBlock), 0, stream, p->soff, p->scenes,
+                           n, p->pscene);
+        hipLaunchKernelGGL(bounds_seg_partial_kernel, dim3(p->parts, p->scenes), dim3(kBlock), 0,
+                           stream, in_aabbs, p->soff, p->part);
+        hipLaunchKernelGGL(bounds_final_kernel, dim3(p->scenes), dim3(kBlock), 0, stream, p->part,
+                           p->parts, p->world);
     } else {
-        hipLaunchKernelGGL(bounds_partial_kernel, dim3(parts), dim3(kBlock), 0, stream, in_aabbs,
-                           n, part);
-        hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(kBlock), 0, stream, part, parts,
-                           world);
+        hipLaunchKernelGGL(bounds_partial_kernel, dim3(p->parts), dim3(kBlock), 0, stream,
+                           in_aabbs, n, p->part);
+        hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(kBlock), 0, stream, p->part,
+                           p->parts, p->world);
     }
-    hipLaunchKernelGGL(morton_kernel<SEG>, dim3(g), dim3(kBlock), 0, stream, centers, stride, n,
-                       world, pscene, k0, v0);
+    hipLaunchKernelGGL(morton_kernel<SEG>, dim3(g), dim3(kBlock), 0, stream, centers, p->stride,
+                       n, p->world, p->pscene, p->k0, p->indices);
     // 4 stable passes over the codes, then (batches) 1 or 2 passes over the
-    // scene ids; the sorted order must end in res.indices.
-    uint32_t *ki = k0, *vi = v0, *ko = k1, *vo = v1;
-    const uint32_t scene_bits = SEG ? 32u - uint32_t(__builtin_clz(scenes - 1)) : 0u;
+    // scene ids; the sorted order must end in p->indices.
+    uint32_t *ki = p->k0, *vi = p->indices, *ko = p->k1, *vo = p->v1;
+    const uint32_t scene_bits = SEG ? 32u - uint32_t(__builtin_clz(p->scenes - 1)) : 0u;
     const uint32_t scene_passes = (scene_bits + 7) / 8;
+    hipError_t e;
     for (uint32_t pass = 0; pass < 4 + scene_passes; ++pass) {
         uint32_t shift = pass * 8;
-        if (pass == 4) {
-            hipLaunchKernelGGL(scene_keys_kernel, dim3(g), dim3(kBlock), 0, stream, vi, pscene, n,
-                               ki);
-        }
+        if (pass == 4)
+            hipLaunchKernelGGL(scene_keys_kernel, dim3(g), dim3(kBlock), 0, stream, vi, p->pscene,
+                               n, ki);
         if (pass >= 4) shift = (pass - 4) * 8;
-        hipLaunchKernelGGL(radix_hist_kernel, dim3(sort_tiles), dim3(256), 0, stream, ki, n, shift,
-                           sort_tiles, counts);
-        if ((e = scan_exclusive_u32(counts, count_entries, nullptr, scan_scratch, scan_bytes,
-                                    stream)) != hipSuccess)
-            return fail(e, "radix scan");
-        hipLaunchKernelGGL(radix_scatter_kernel, dim3(sort_tiles), dim3(kSortBlock), 0, stream, ki,
-                           vi, n, shift, sort_tiles, counts, ko, vo);
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(p->sort_tiles), dim3(256), 0, stream, ki, n,
+                           shift, p->sort_tiles, p->counts);
+        if ((e = scan_exclusive_u32(p->counts, p->count_entries, nullptr, p->scan_scratch,
+                                    p->scan_bytes, stream)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3(p->sort_tiles), dim3(kSortBlock), 0, stream,
+                           ki, vi, n, shift, p->sort_tiles, p->counts, ko, vo);
         std::swap(ki, ko);
         std::swap(vi, vo);
     }
-    if (vi != res.indices)
-        hipLaunchKernelGGL(copy_words_kernel, dim3(g), dim3(kBlock), 0, stream, vi, res.indices, n);
-    hipLaunchKernelGGL(leaves_kernel<SEG>, dim3(g), dim3(kBlock), 0, stream, in_aabbs, res.indices,
-                       n, pscene, cl0, cs0);
-    // the reference's node array: leaves conceptually at [n - S, 2n - S), the
-    // S roots end at [0, S)
-    PlocState init{n, n - res.root_count, 2 * n - res.root_count, 0, 0, {0, 0, 0}};
-    if ((e = hipMemcpyAsync(st, &init, sizeof(init), hipMemcpyHostToDevice, stream)) != hipSuccess)
-        return fail(e, "state init");
-    if ((e = hipGetLastError()) != hipSuccess) return fail(e, "sort launch");
+    if (vi != p->indices)
+        hipLaunchKernelGGL(copy_words_kernel, dim3(g), dim3(kBlock), 0, stream, vi, p->indices, n);
+    hipLaunchKernelGGL(leaves_kernel<SEG>, dim3(g), dim3(kBlock), 0, stream, in_aabbs, p->indices,
+                       n, p->pscene, p->cl0, p->cs0);
+    hipLaunchKernelGGL(init_state_kernel, dim3(1), dim3(1), 0, stream, p->st, n,
+                       SEG ? p->scenes : 1u);
+    return hipGetLastError();
+}
 
-    // multi-block iterations while the active set is large, in batches
-    uint32_t known = n;
+// One multi-block clustering iteration reading states[iter]; a no-op once
+// the active set fits the tail kernel.
+template <bool SEG>
+void enqueue_iteration(const PlocPlanImpl* p, uint32_t iter, uint32_t tiles, hipStream_t stream) {
+    hipLaunchKernelGGL(nn_mark_kernel<SEG>, dim3(tiles), dim3(kBlock), 0, stream, p->st + iter,
+                       p->cl0, p->cl1, p->cs0, p->cs1, p->nbuf, p->mbuf, p->tcount);
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, stream, p->st + iter, p->tcount);
+    hipLaunchKernelGGL(merge_kernel<SEG>, dim3(tiles), dim3(kBlock), 0, stream, p->st + iter,
+                       p->cl0, p->cl1, p->cs0, p->cs1, p->nbuf, p->mbuf, p->tcount, p->nodes);
+}
+
+template <bool SEG>
+hipError_t set_tail_lds() {
+    static bool done[64] = {false};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (done[dev]) return hipSuccess;
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(tail_kernel<SEG>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             int(sizeof(TailShared<SEG>)));
+    if (e == hipSuccess) done[dev] = true;
+    return e;
+}
+
+// The remaining iterations in one workgroup from states[iter]; the final
+// state lands in states[iter + 1] and p->fail tells whether it is complete.
+template <bool SEG>
+hipError_t enqueue_tail(const PlocPlanImpl* p, uint32_t iter, hipStream_t stream) {
+    const uint32_t roots = SEG ? p->scenes : 1u;
+    hipLaunchKernelGGL(tail_kernel<SEG>, dim3(1), dim3(kTailThreads), sizeof(TailShared<SEG>),
+                       stream, p->st + iter, p->cl0, p->cl1, p->cs0, p->cs1, p->nodes, roots,
+                       p->st + iter + 1);
+    hipLaunchKernelGGL(check_kernel, dim3(1), dim3(1), 0, stream, p->st + iter + 1, roots,
+                       p->fail);
+    return hipGetLastError();
+}
+
+// Eager build with host checks between batches of iterations; *iterations
+// receives the number of multi-block iterations the data needed.
+template <bool SEG>
+ResultCode run_eager(PlocPlanImpl* p, const RTAabb* aabbs, const float* centers,
+                     const uint32_t* scene_off_host, hipStream_t stream, uint32_t* iterations) {
+    hipError_t e;
+    if (SEG) {
+        if ((e = hipMemcpyAsync(p->soff, scene_off_host, size_t(p->scenes + 1) * 4,
+                                hipMemcpyHostToDevice, stream)) != hipSuccess)
+            return hip_fail(e, "scene offsets");
+    }
+    if ((e = set_tail_lds<SEG>()) != hipSuccess) return hip_fail(e, "tail LDS size");
+    if ((e = enqueue_front<SEG>(p, aabbs, centers, stream)) != hipSuccess)
+        return hip_fail(e, "ploc sort");
+    uint32_t known = p->n;
     uint32_t iter = 0;
     PlocState host_state;
     while (known > uint32_t(kTailMax)) {
         const uint32_t tiles = (known + kTile - 1) / kTile;
         for (int bt = 0; bt < kBatch; ++bt) {
             if (iter + 1 >= kMaxIterations) {
-                release_result();
                 set_error("clustering needed more than %u iterations", kMaxIterations);
                 return Error;
             }
-            hipLaunchKernelGGL(nn_mark_kernel<SEG>, dim3(tiles), dim3(kBlock), 0, stream, st + iter,
-                               cl0, cl1, cs0, cs1, nbuf, mbuf, tcount);
-            hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, stream, st + iter, tcount);
-            hipLaunchKernelGGL(merge_kernel<SEG>, dim3(tiles), dim3(kBlock), 0, stream, st + iter,
-                               cl0, cl1, cs0, cs1, nbuf, mbuf, tcount, res.nodes);
+            enqueue_iteration<SEG>(p, iter, tiles, stream);
             ++iter;
         }
-        if ((e = hipGetLastError()) != hipSuccess) return fail(e, "ploc iteration");
-        if ((e = hipMemcpyAsync(&host_state, st + iter, sizeof(host_state), hipMemcpyDeviceToHost,
-                                stream)) != hipSuccess)
-            return fail(e, "state read");
-        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return fail(e, "ploc iteration");
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "ploc iteration");
+        if ((e = hipMemcpyAsync(&host_state, p->st + iter, sizeof(host_state),
+                                hipMemcpyDeviceToHost, stream)) != hipSuccess)
+            return hip_fail(e, "state read");
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "ploc iteration");
         if (host_state.n_active >= known) {
-            release_result();
             set_error("clustering made no progress (non-finite bounds?)");
             return Error;
         }
         known = host_state.n_active;
     }
-    static bool lds_ok[2] = {false, false};
-    if (!lds_ok[SEG]) {
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(tail_kernel<SEG>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                int(sizeof(TailShared<SEG>)));
-        if (e != hipSuccess) return fail(e, "tail LDS size");
-        lds_ok[SEG] = true;
-    }
-    hipLaunchKernelGGL(tail_kernel<SEG>, dim3(1), dim3(kTailThreads), sizeof(TailShared<SEG>),
-                       stream, st + iter, cl0, cl1, cs0, cs1, res.nodes, res.root_count,
-                       st + iter + 1);
-    if ((e = hipGetLastError()) != hipSuccess) return fail(e, "tail launch");
-    if ((e = hipMemcpyAsync(&host_state, st + iter + 1, sizeof(host_state), hipMemcpyDeviceToHost,
-                            stream)) != hipSuccess)
-        return fail(e, "state read");
-    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return fail(e, "ploc tail");
-    if (host_state.n_active != res.root_count) {
-        release_result();
+    if ((e = enqueue_tail<SEG>(p, iter, stream)) != hipSuccess) return hip_fail(e, "tail launch");
+    uint32_t fail = 1;
+    if ((e = hipMemcpyAsync(&fail, p->fail, 4, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(&host_state, p->st + iter + 1, sizeof(host_state),
+                            hipMemcpyDeviceToHost, stream)) != hipSuccess)
+        return hip_fail(e, "state read");
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "ploc tail");
+    if (fail) {
         set_error("clustering stopped at %u clusters (non-finite bounds?)", host_state.n_active);
         return Error;
     }
-    *out = res;
+    if (iterations) {
+        // iterations that did work: the first state small enough for the tail
+        uint32_t need = 0;
+        if (iter > 0) {
+            std::vector<PlocState> states(iter + 1);
+            if ((e = hipMemcpy(states.data(), p->st, sizeof(PlocState) * (iter + 1),
+                               hipMemcpyDeviceToHost)) != hipSuccess)
+                return hip_fail(e, "state read");
+            while (need < iter && states[need].n_active > uint32_t(kTailMax)) ++need;
+        }
+        *iterations = need;
+    }
+    return Ok;
+}
+
+// n <= 2, single scene: one root leaf with the world bounds.
+ResultCode build_small(const RTAabb* aabbs, const float* centers, size_t stride_floats,
+                       uint32_t n, hipStream_t stream, DeviceBvh* out) {
+    PlocPlanImpl* p = nullptr;
+    ResultCode rc = alloc_plan(n, 1, aabbs == nullptr, stride_floats, &p);
+    if (rc != Ok) return rc;
+    const DevAabb* in_aabbs = aabbs ? reinterpret_cast<const DevAabb*>(aabbs) : p->in_aabbs;
+    if (!aabbs)
+        hipLaunchKernelGGL(point_aabbs_kernel, dim3(1), dim3(kBlock), 0, stream, centers,
+                           p->stride, n, p->in_aabbs);
+    hipLaunchKernelGGL(bounds_partial_kernel, dim3(1), dim3(kBlock), 0, stream, in_aabbs, n,
+                       p->part);
+    hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(kBlock), 0, stream, p->part, 1u,
+                       p->world);
+    hipLaunchKernelGGL(small_root_kernel, dim3(1), dim3(1), 0, stream, p->world, n, p->nodes,
+                       p->indices);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        free_plan(p, false);
+        return hip_fail(e, "small build");
+    }
+    out->nodes = p->nodes;
+    out->indices = p->indices;
+    out->node_count = 1;
+    out->prim_count = n;
+    out->root_count = 1;
+    free_plan(p, true);
+    return Ok;
+}
+
+// Eager build of one scene (scene_off null) or a batch (scene_off: host
+// offsets, scenes >= 2); the tree's arrays pass to *out.
+ResultCode build_eager(const RTAabb* aabbs, const float* centers, size_t stride_floats,
+                       uint32_t n, const uint32_t* scene_off, uint32_t scenes, hipStream_t stream,
+                       DeviceBvh* out) {
+    if (scenes == 1 && n <= 2) return build_small(aabbs, centers, stride_floats, n, stream, out);
+    PlocPlanImpl* p = nullptr;
+    ResultCode rc = alloc_plan(n, scenes, aabbs == nullptr, stride_floats, &p);
+    if (rc != Ok) return rc;
+    rc = p->seg ? run_eager<true>(p, aabbs, centers, scene_off, stream, nullptr)
+                : run_eager<false>(p, aabbs, centers, nullptr, stream, nullptr);
+    if (rc != Ok) {
+        free_plan(p, false);
+        return rc;
+    }
+    out->nodes = p->nodes;
+    out->indices = p->indices;
+    out->node_count = p->node_count();
+    out->prim_count = n;
+    out->root_count = p->seg ? scenes : 1u;
+    free_plan(p, true);
     return Ok;
 }
 
 }  // namespace
+
+// ---- capturable builds -------------------------------------------------------
+
+ResultCode ploc_plan_create(const RTAabb* aabbs, const float* centers, size_t stride_floats,
+                            const uint32_t* scene_off, uint32_t scenes, hipStream_t stream,
+                            PlocPlanImpl** out, DeviceBvh* tree) {
+    const uint32_t n = scene_off[scenes];
+    PlocPlanImpl* p = nullptr;
+    ResultCode rc = alloc_plan(n, scenes, aabbs == nullptr, stride_floats, &p);
+    if (rc != Ok) return rc;
+    uint32_t need = 0;
+    hipError_t e = hipMemcpy(p->soff, scene_off, size_t(scenes + 1) * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_plan(p, false);
+        return hip_fail(e, "scene offsets");
+    }
+    rc = p->seg ? run_eager<true>(p, aabbs, centers, scene_off, stream, &need)
+                : run_eager<false>(p, aabbs, centers, nullptr, stream, &need);
+    if (rc != Ok) {
+        free_plan(p, false);
+        return rc;
+    }
+    // headroom for later inputs of the same sizes (animated geometry)
+    p->iterations = need + (need / 4 > 4 ? need / 4 : 4);
+    if (p->iterations + 2 > kMaxIterations) p->iterations = kMaxIterations - 2;
+    tree->nodes = p->nodes;
+    tree->indices = p->indices;
+    tree->parents = nullptr;
+    tree->node_count = p->node_count();
+    tree->prim_count = n;
+    tree->root_count = p->seg ? scenes : 1u;
+    *out = p;
+    return Ok;
+}
+
+hipError_t ploc_plan_enqueue(const PlocPlanImpl* p, const RTAabb* aabbs, const float* centers,
+                             hipStream_t stream) {
+    if (aabbs == nullptr && !p->point_aabbs) return hipErrorInvalidValue;
+    hipError_t e;
+    if (p->seg) {
+        if ((e = enqueue_front<true>(p, aabbs, centers, stream)) != hipSuccess) return e;
+        for (uint32_t k = 0; k < p->iterations; ++k)
+            enqueue_iteration<true>(p, k, p->ploc_tiles, stream);
+        return enqueue_tail<true>(p, p->iterations, stream);
+    }
+    if ((e = enqueue_front<false>(p, aabbs, centers, stream)) != hipSuccess) return e;
+    for (uint32_t k = 0; k < p->iterations; ++k)
+        enqueue_iteration<false>(p, k, p->ploc_tiles, stream);
+    return enqueue_tail<false>(p, p->iterations, stream);
+}
+
+const uint32_t* ploc_plan_fail_word(const PlocPlanImpl* p) { return p->fail; }
+const uint32_t* ploc_plan_scene_offsets(const PlocPlanImpl* p) { return p->soff; }
+uint32_t ploc_plan_iterations(const PlocPlanImpl* p) { return p->iterations; }
+void ploc_plan_destroy(PlocPlanImpl* p) { free_plan(p, false); }
 
 ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stride_floats,
                           size_t n64, int radius, hipStream_t stream, DeviceBvh* out) {
@@ -1042,14 +1225,14 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
         set_error("prim_count %zu too large", n64);
         return InvalidArgument;
     }
-    return build_ploc_impl<false>(aabbs, centers, stride_floats, uint32_t(n64), nullptr, 1u,
-                                  stream, out);
+    return build_eager(aabbs, centers, stride_floats, uint32_t(n64), nullptr, 1u, stream, out);
 }
 
-ResultCode gpu_build_ploc_batch(const RTAabb* aabbs, const float* centers, size_t stride_floats,
-                                const uint32_t* scene_off, uint32_t scenes, hipStream_t stream,
-                                DeviceBvh* out) {
-    if (scenes == 0 || scene_off == nullptr) return NoPrimitives;
+ResultCode check_batch_offsets(const uint32_t* scene_off, uint32_t scenes) {
+    if (scenes == 0 || scene_off == nullptr) {
+        set_error("no scenes");
+        return NoPrimitives;
+    }
     if (scenes > kMaxBatchScenes) {
         set_error("a batch holds at most %u scenes (got %u)", kMaxBatchScenes, scenes);
         return InvalidArgument;
@@ -1059,20 +1242,27 @@ ResultCode gpu_build_ploc_batch(const RTAabb* aabbs, const float* centers, size_
         return InvalidArgument;
     }
     for (uint32_t s = 0; s < scenes; ++s) {
-        if (scene_off[s + 1] < scene_off[s] + 3) {
-            set_error("scene %u has %d primitives; a batched scene needs at least 3", s,
-                      int(scene_off[s + 1]) - int(scene_off[s]));
+        if (scene_off[s + 1] < scene_off[s] || scene_off[s + 1] - scene_off[s] < 3) {
+            set_error("scene %u has %lld primitives; a batched scene needs at least 3", s,
+                      (long long)scene_off[s + 1] - (long long)scene_off[s]);
             return InvalidArgument;
         }
     }
-    const uint32_t n = scene_off[scenes];
-    if (n > (1u << 30)) {
-        set_error("batch of %u primitives too large", n);
+    if (scene_off[scenes] > (1u << 30)) {
+        set_error("batch of %u primitives too large", scene_off[scenes]);
         return InvalidArgument;
     }
-    if (scenes == 1)
-        return build_ploc_impl<false>(aabbs, centers, stride_floats, n, nullptr, 1u, stream, out);
-    return build_ploc_impl<true>(aabbs, centers, stride_floats, n, scene_off, scenes, stream, out);
+    return Ok;
+}
+
+ResultCode gpu_build_ploc_batch(const RTAabb* aabbs, const float* centers, size_t stride_floats,
+                                const uint32_t* scene_off, uint32_t scenes, hipStream_t stream,
+                                DeviceBvh* out) {
+    ResultCode rc = check_batch_offsets(scene_off, scenes);
+    if (rc != Ok) return rc;
+    const uint32_t n = scene_off[scenes];
+    return build_eager(aabbs, centers, stride_floats, n, scenes > 1 ? scene_off : nullptr, scenes,
+                       stream, out);
 }
 
 }  // namespace rtbvh

@@ -99,9 +99,9 @@ __device__ __forceinline__ void set_slot_bounds(DevMbvhNode& m, int s, const Dev
 }
 
 __global__ void mfill_kernel(const DevAabb* nodes, uint32_t count, const uint32_t* flag,
-                             const uint32_t* mindex, DevMbvhNode* mnodes) {
+                             const uint32_t* mindex, DevMbvhNode* mnodes, uint32_t mcap) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count || flag[i] == 0) return;
+    if (i >= count || flag[i] == 0 || mindex[i] >= mcap) return;
     DevAabb cur;
     ld(nodes + i, cur);
     DevMbvhNode m;
@@ -223,6 +223,13 @@ hipError_t scan_exclusive_u32(uint32_t* data, uint32_t count, uint32_t* total, v
                               size_t scratch_bytes, hipStream_t stream);
 size_t scan_scratch_bytes(uint32_t count);
 
+hipError_t launch_parent_links(const DevAabb* nodes, uint32_t count, int32_t* parents,
+                               hipStream_t stream) {
+    const uint32_t g = (count + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(parent_kernel, dim3(g), dim3(kBlock), 0, stream, nodes, count, parents);
+    return hipGetLastError();
+}
+
 ResultCode ensure_parents(DeviceBvh& bvh, hipStream_t stream) {
     if (bvh.parents) return Ok;
     hipError_t e = hipMalloc(&bvh.parents, size_t(bvh.node_count) * sizeof(int32_t));
@@ -292,7 +299,7 @@ ResultCode gpu_build_mbvh(DeviceBvh& bvh, hipStream_t stream, DeviceMbvh* out) {
     }
     res.node_count = mcount;
     hipLaunchKernelGGL(mfill_kernel, dim3(g), dim3(kBlock), 0, stream, bvh.nodes, count, flag,
-                       mindex, res.nodes);
+                       mindex, res.nodes, mcount);
     const uint32_t gi = (bvh.prim_count + kBlock - 1) / kBlock;
     if (bvh.prim_count)
         hipLaunchKernelGGL(copy_u32_kernel, dim3(gi), dim3(kBlock), 0, stream, bvh.indices,
@@ -307,6 +314,71 @@ ResultCode gpu_build_mbvh(DeviceBvh& bvh, hipStream_t stream, DeviceMbvh* out) {
     }
     *out = res;
     return Ok;
+}
+
+struct CollapsePlanImpl {
+    uint32_t count = 0, mcap = 0, roots = 1;
+    int32_t* parents = nullptr;
+    uint32_t* flag = nullptr;
+    uint32_t* mindex = nullptr;
+    void* scratch = nullptr;
+    size_t sbytes = 0;
+    DevMbvhNode* mnodes = nullptr;
+};
+
+void collapse_plan_destroy(CollapsePlanImpl* c) {
+    if (!c) return;
+    if (c->parents) (void)hipFree(c->parents);
+    if (c->flag) (void)hipFree(c->flag);
+    if (c->mindex) (void)hipFree(c->mindex);
+    if (c->scratch) (void)hipFree(c->scratch);
+    if (c->mnodes) (void)hipFree(c->mnodes);
+    delete c;
+}
+
+// A tree of n primitives and r roots has n - r interior nodes, which bounds
+// the MBVH node count (the roots and the interior nodes at even depth).
+ResultCode collapse_plan_create(const DeviceBvh& tree, CollapsePlanImpl** out, DeviceMbvh* view) {
+    CollapsePlanImpl* c = new CollapsePlanImpl();
+    c->count = tree.node_count;
+    c->roots = tree.root_count;
+    c->mcap = tree.prim_count > tree.root_count ? tree.prim_count - tree.root_count : 1u;
+    c->sbytes = scan_scratch_bytes(c->count);
+    hipError_t e;
+    if ((e = hipMalloc(&c->parents, size_t(c->count) * 4)) != hipSuccess ||
+        (e = hipMalloc(&c->flag, size_t(c->count) * 4)) != hipSuccess ||
+        (e = hipMalloc(&c->mindex, size_t(c->count) * 4)) != hipSuccess ||
+        (c->sbytes && (e = hipMalloc(&c->scratch, c->sbytes)) != hipSuccess) ||
+        (e = hipMalloc(&c->mnodes, size_t(c->mcap) * sizeof(DevMbvhNode))) != hipSuccess) {
+        collapse_plan_destroy(c);
+        return hip_fail(e, "collapse plan");
+    }
+    view->nodes = c->mnodes;
+    view->indices = tree.indices;  // shared with the tree
+    view->tris = nullptr;
+    view->node_count = c->mcap;
+    view->prim_count = tree.prim_count;
+    view->root_count = tree.root_count;
+    *out = c;
+    return Ok;
+}
+
+hipError_t collapse_plan_enqueue(const CollapsePlanImpl* c, const DeviceBvh& tree,
+                                 hipStream_t stream) {
+    const uint32_t count = c->count;
+    const uint32_t g = (count + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(g), dim3(kBlock), 0, stream, c->parents, count, -1);
+    hipLaunchKernelGGL(parent_kernel, dim3(g), dim3(kBlock), 0, stream, tree.nodes, count,
+                       c->parents);
+    hipLaunchKernelGGL(mflag_kernel, dim3(g), dim3(kBlock), 0, stream, tree.nodes, count, c->roots,
+                       c->parents, c->flag, device_status_ptr());
+    hipLaunchKernelGGL(copy_u32_kernel, dim3(g), dim3(kBlock), 0, stream, c->flag, c->mindex,
+                       count);
+    hipError_t e = scan_exclusive_u32(c->mindex, count, nullptr, c->scratch, c->sbytes, stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(mfill_kernel, dim3(g), dim3(kBlock), 0, stream, tree.nodes, count, c->flag,
+                       c->mindex, c->mnodes, c->mcap);
+    return hipGetLastError();
 }
 
 ResultCode gpu_refit(DeviceBvh& bvh, const RTAabb* aabbs, hipStream_t stream) {

@@ -30,7 +30,20 @@ struct TraceParams {
     uint64_t* stats;  // non-null: {node visits, triangle tests} accumulated (MBVH closest hit)
     const uint32_t* ray_offsets;  // non-null: batch of scene_count scenes (MBVH forest)
     uint32_t scene_count;
+    const uint32_t* gate;  // non-null: the launch does nothing while *gate != 0
+    uint32_t lds_depth;    // set by the launcher: stack entries per lane held in LDS
 };
+
+// Launch shape of the lowest-id MBVH traversal (per device).
+struct TraceConfig {
+    uint32_t block;          // threads per block: 64, 128 or 256
+    uint32_t lds_depth;      // stack entries per lane in LDS (0..32)
+    uint32_t occupancy_pct;  // persistent grid as a share of resident blocks (10..400)
+};
+TraceConfig default_trace_config();
+bool valid_trace_config(const TraceConfig& c);
+void set_trace_config(const TraceConfig* c);  // null: back to the default
+TraceConfig get_trace_config();
 
 hipError_t launch_trace(const TraceParams& p, bool mbvh, bool ref_order, bool any_hit,
                         hipStream_t stream);
@@ -73,6 +86,52 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
 ResultCode gpu_build_ploc_batch(const RTAabb* aabbs, const float* centers, size_t stride_floats,
                                 const uint32_t* scene_off, uint32_t scenes, hipStream_t stream,
                                 DeviceBvh* out);
+ResultCode check_batch_offsets(const uint32_t* scene_off, uint32_t scenes);
+
+// Capturable PLOC build: every buffer allocated up front, a fixed number of
+// clustering iterations (sized by one eager build at creation), no host
+// synchronisation in ploc_plan_enqueue.  The tree's arrays stay owned by the
+// plan.  The fail word is nonzero after an enqueued build whose iteration
+// budget ran out (the tree is then incomplete).
+struct PlocPlanImpl;
+ResultCode ploc_plan_create(const RTAabb* aabbs, const float* centers, size_t stride_floats,
+                            const uint32_t* scene_off, uint32_t scenes, hipStream_t stream,
+                            PlocPlanImpl** out, DeviceBvh* tree);
+hipError_t ploc_plan_enqueue(const PlocPlanImpl* p, const RTAabb* aabbs, const float* centers,
+                             hipStream_t stream);
+const uint32_t* ploc_plan_fail_word(const PlocPlanImpl* p);
+const uint32_t* ploc_plan_scene_offsets(const PlocPlanImpl* p);
+uint32_t ploc_plan_iterations(const PlocPlanImpl* p);
+void ploc_plan_destroy(PlocPlanImpl* p);
+
+// Capturable collapse of a plan's tree into at most mcap MBVH nodes.
+struct CollapsePlanImpl;
+ResultCode collapse_plan_create(const DeviceBvh& tree, CollapsePlanImpl** out, DeviceMbvh* view);
+hipError_t collapse_plan_enqueue(const CollapsePlanImpl* c, const DeviceBvh& tree,
+                                 hipStream_t stream);
+void collapse_plan_destroy(CollapsePlanImpl* c);
+
+// parent[child] for every interior node (parents pre-filled with -1).
+hipError_t launch_parent_links(const DevAabb* nodes, uint32_t count, int32_t* parents,
+                               hipStream_t stream);
+
+// Traversal tree (trace_tree.hip): a 4-wide tree with tight slot bounds and
+// leaves of up to max_leaf primitives whose triangles are stored in leaf
+// order; traced by the lowest-id kernel with `tris`.  *out receives newly
+// allocated nodes (capacity prim_count + roots) and tris; indices is null.
+struct TraceTreePlan;
+ResultCode trace_tree_plan_create(const DeviceBvh& tree, uint32_t max_leaf,
+                                  TraceTreePlan** out_plan, DeviceMbvh* out);
+ResultCode trace_tree_build(TraceTreePlan* t, const DeviceBvh& tree, const float* verts,
+                            uint32_t prim_count, const uint32_t* scene_off, uint32_t scenes,
+                            DeviceMbvh* out, hipStream_t stream);
+hipError_t trace_tree_enqueue(const TraceTreePlan* t, const DeviceBvh& tree, const float* verts,
+                              uint32_t prim_count, const uint32_t* scene_off, uint32_t scenes,
+                              DeviceMbvh* out, uint32_t levels, uint32_t* fail,
+                              hipStream_t stream);
+uint32_t trace_tree_levels(const TraceTreePlan* t);
+void trace_tree_plan_destroy(TraceTreePlan* t);
+
 ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t stride_floats,
                             size_t n, size_t max_leaf, hipStream_t stream, DeviceBvh* out);
 ResultCode gpu_build_mbvh(DeviceBvh& bvh, hipStream_t stream, DeviceMbvh* out);
@@ -103,6 +162,9 @@ struct BatchView {
     int device;
 };
 bool batch_view(uint32_t id, BatchView* out);
+bool trace_tree_view(uint32_t id, BatchView* out);
+
+constexpr uint32_t kDefaultMaxLeaf = 4;
 
 void set_error(const char* fmt, ...);
 ResultCode hip_fail(hipError_t e, const char* what);

@@ -24,26 +24,33 @@ namespace rtbvh {
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kLdsStack = 12;
-constexpr int kDeepStack = RTBVH_MAX_STACK - kLdsStack;
+constexpr int kMaxBlock = 256;
 
-// Stack entries beyond the LDS part live in per-lane private arrays.  The
-// stack pointer is kept apart from those arrays so that it stays in a
-// register (a struct holding both would be placed in scratch as a whole).
+// The stack's first `depth` entries live in LDS (entry k of lane l at
+// lds[k * stride + l], stride = block size: consecutive lanes, consecutive
+// banks), the rest in per-lane private arrays.  The stack pointer is kept
+// apart from those arrays so that it stays in a register (a struct holding
+// both would be placed in scratch as a whole).
 struct DeepStack {
-    int32_t i[kDeepStack];
-    float t[kDeepStack];
+    int32_t i[RTBVH_MAX_STACK];
+    float t[RTBVH_MAX_STACK];
 };
 
-__device__ __forceinline__ bool stack_push(int32_t* lds_i, float* lds_t, DeepStack& deep, int& sp,
+struct LdsStack {
+    int32_t* i;
+    float* t;
+    int depth;
+    int stride;
+};
+
+__device__ __forceinline__ bool stack_push(const LdsStack& ls, DeepStack& deep, int& sp,
                                            int32_t node, float t) {
-    if (sp < kLdsStack) {
-        lds_i[sp * kBlock] = node;
-        lds_t[sp * kBlock] = t;
+    if (sp < ls.depth) {
+        ls.i[sp * ls.stride] = node;
+        ls.t[sp * ls.stride] = t;
     } else if (sp < RTBVH_MAX_STACK) {
-        deep.i[sp - kLdsStack] = node;
-        deep.t[sp - kLdsStack] = t;
+        deep.i[sp - ls.depth] = node;
+        deep.t[sp - ls.depth] = t;
     } else {
         return false;
     }
@@ -51,16 +58,15 @@ __device__ __forceinline__ bool stack_push(int32_t* lds_i, float* lds_t, DeepSta
     return true;
 }
 
-__device__ __forceinline__ void stack_pop(const int32_t* lds_i, const float* lds_t,
-                                          const DeepStack& deep, int& sp, int32_t& node,
-                                          float& t) {
+__device__ __forceinline__ void stack_pop(const LdsStack& ls, const DeepStack& deep, int& sp,
+                                          int32_t& node, float& t) {
     --sp;
-    if (sp < kLdsStack) {
-        node = lds_i[sp * kBlock];
-        t = lds_t[sp * kBlock];
+    if (sp < ls.depth) {
+        node = ls.i[sp * ls.stride];
+        t = ls.t[sp * ls.stride];
     } else {
-        node = deep.i[sp - kLdsStack];
-        t = deep.t[sp - kLdsStack];
+        node = deep.i[sp - ls.depth];
+        t = deep.t[sp - ls.depth];
     }
 }
 
@@ -93,16 +99,17 @@ __device__ __forceinline__ uint32_t scene_of_ray(const uint32_t* off, uint32_t s
 // BATCH: the MBVH is a forest whose node s is the root of scene s, and each
 // ray starts at the root of its scene.
 template <bool ANY, bool STATS, bool PREP, bool BATCH>
-__global__ void __launch_bounds__(kBlock) mbvh_fast_kernel(TraceParams p) {
-    __shared__ int32_t s_idx[kLdsStack * kBlock];
-    __shared__ float s_t[kLdsStack * kBlock];
-    int32_t* lds_i = s_idx + threadIdx.x;
-    float* lds_t = s_t + threadIdx.x;
+__global__ void __launch_bounds__(kMaxBlock) mbvh_fast_kernel(TraceParams p) {
+    if (p.gate && *p.gate) return;  // the tree this launch would read is incomplete
+    extern __shared__ int32_t s_stack[];  // p.lds_depth entries per lane: ids, then distances
+    const LdsStack ls{s_stack + threadIdx.x,
+                      reinterpret_cast<float*>(s_stack + p.lds_depth * blockDim.x) + threadIdx.x,
+                      int(p.lds_depth), int(blockDim.x)};
     DeepStack deep;
     int sp = 0;
     uint64_t n_nodes = 0, n_tris = 0;
-    const uint64_t stride = uint64_t(gridDim.x) * kBlock;
-    uint64_t rid = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    uint64_t rid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f,
           iz = 0.f, tmin = 0.f;
     float bt = 0.f, bu = 0.f, bv = 0.f;
@@ -153,7 +160,7 @@ __global__ void __launch_bounds__(kBlock) mbvh_fast_kernel(TraceParams p) {
             while (sp > 0) {
                 int32_t c;
                 float t;
-                stack_pop(lds_i, lds_t, deep, sp, c, t);
+                stack_pop(ls, deep, sp, c, t);
                 if (t <= bt) {
                     cur = c;
                     break;
@@ -243,7 +250,7 @@ __global__ void __launch_bounds__(kBlock) mbvh_fast_kernel(TraceParams p) {
                 const int32_t c = sel4(s, c0, c1, c2, c3);
                 const float t = sel4(s, tn[0], tn[1], tn[2], tn[3]);
                 if (c < 0 || !(t <= bt)) continue;
-                if (next >= 0) overflow = !stack_push(lds_i, lds_t, deep, sp, next, next_t);
+                if (next >= 0) overflow = !stack_push(ls, deep, sp, next, next_t);
                 next = c;
                 next_t = t;
             }
@@ -270,29 +277,68 @@ __global__ void __launch_bounds__(kBlock) mbvh_fast_kernel(TraceParams p) {
     }
 }
 
-template <bool ANY, bool STATS, bool PREP, bool BATCH = false>
-hipError_t launch_fast(const TraceParams& p, hipStream_t stream) {
-    auto k = mbvh_fast_kernel<ANY, STATS, PREP, BATCH>;
-    static int blocks[64] = {0};
+// Launch shape per device: block size, LDS stack depth and the share of the
+// resident-block capacity used by the persistent grid (percent).
+TraceConfig g_config[64];
+bool g_config_set[64] = {false};
+
+int current_dev() {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-    if (blocks[dev] == 0) {
+    return dev;
+}
+
+template <bool ANY, bool STATS, bool PREP, bool BATCH = false>
+hipError_t launch_fast(const TraceParams& p0, hipStream_t stream) {
+    auto k = mbvh_fast_kernel<ANY, STATS, PREP, BATCH>;
+    const int dev = current_dev();
+    const TraceConfig cfg = g_config_set[dev] ? g_config[dev] : default_trace_config();
+    // resident blocks per CU for this shape, cached per (device, shape)
+    static int cache_key[64] = {0};
+    static int cache_blocks[64] = {0};
+    const size_t lds = size_t(cfg.lds_depth) * cfg.block * 8;
+    const int key = int(cfg.block) * 1000 + int(cfg.lds_depth) + 1;
+    if (cache_key[dev] != key) {
         int cus = 0, per_cu = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             cus = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, int(cfg.block), lds) !=
+                hipSuccess ||
             per_cu <= 0)
             per_cu = 1;
-        blocks[dev] = cus * per_cu;
+        cache_blocks[dev] = cus * per_cu;
+        cache_key[dev] = key;
     }
-    const uint64_t need = (p.ray_count + kBlock - 1) / kBlock;
-    const int grid = int(need < uint64_t(blocks[dev]) ? need : uint64_t(blocks[dev]));
+    uint64_t blocks = uint64_t(cache_blocks[dev]) * cfg.occupancy_pct / 100;
+    if (blocks == 0) blocks = 1;
+    const uint64_t need = (p0.ray_count + cfg.block - 1) / cfg.block;
+    const int grid = int(need < blocks ? need : blocks);
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, stream, p);
+    TraceParams p = p0;
+    p.lds_depth = cfg.lds_depth;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(cfg.block), lds, stream, p);
     return hipGetLastError();
 }
 
 }  // namespace
+
+TraceConfig default_trace_config() { return TraceConfig{256, 12, 100}; }
+
+bool valid_trace_config(const TraceConfig& c) {
+    return (c.block == 64 || c.block == 128 || c.block == 256) && c.lds_depth <= 32 &&
+           c.occupancy_pct >= 10 && c.occupancy_pct <= 400;
+}
+
+void set_trace_config(const TraceConfig* c) {
+    const int dev = current_dev();
+    if (c) g_config[dev] = *c;
+    g_config_set[dev] = c != nullptr;
+}
+
+TraceConfig get_trace_config() {
+    const int dev = current_dev();
+    return g_config_set[dev] ? g_config[dev] : default_trace_config();
+}
 
 hipError_t launch_trace_fast(const TraceParams& p, bool any_hit, hipStream_t stream) {
     if (p.ray_offsets) {

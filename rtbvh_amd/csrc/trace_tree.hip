@@ -1,0 +1,481 @@
+// Traversal tree: a 4-wide tree made for the lowest-id traversal from any
+// binary BVH (no counterpart in the reference, whose MBVH collapse,
+// src/mbvh_node.rs:297-411, always merges exactly two binary levels and gives
+// a leaf slot its parent's bounds).
+//
+// Each wide node starts from a binary node's two children and repeatedly
+// opens the child with the largest surface area, until it has four slots or
+// nothing is left to open.  A subtree of at most `max_leaf` primitives is not
+// opened: it becomes one leaf slot with its own (tight) bounds, and its
+// triangles are stored contiguously, in leaf order, as three float4 each
+// (v0 with the primitive id, e1, e2).  The closest hit under the lowest
+// (t, id) rule does not depend on the tree, so tracing this tree gives the
+// same hit records as tracing the reference-layout MBVH.
+//
+// Construction is breadth-first: one kernel per level over a queue of
+// (binary node, wide node) pairs held in device memory; the level kernels
+// read the queue length on the device, so a fixed number of them can also be
+// recorded into a graph.
+#include <vector>
+
+#include "internal.hpp"
+
+namespace rtbvh {
+
+hipError_t scan_exclusive_u32(uint32_t* data, uint32_t count, uint32_t* total, void* scratch,
+                              size_t scratch_bytes, hipStream_t stream);
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kLevelGrid = 2048;  // persistent blocks per level kernel
+constexpr int kMaxLeafLimit = 8;
+
+struct QItem {
+    uint32_t b;  // binary node
+    uint32_t w;  // wide node it becomes
+};
+
+__device__ __forceinline__ void ldn(const DevAabb* p, DevAabb& o) {
+    const float4* q = reinterpret_cast<const float4*>(p);
+    const float4 a = q[0], b = q[1];
+    o.min[0] = a.x;
+    o.min[1] = a.y;
+    o.min[2] = a.z;
+    o.extra1 = __float_as_int(a.w);
+    o.max[0] = b.x;
+    o.max[1] = b.y;
+    o.max[2] = b.z;
+    o.extra2 = __float_as_int(b.w);
+}
+
+__device__ __forceinline__ float half_area(const DevAabb& a) {
+    const float dx = a.max[0] - a.min[0], dy = a.max[1] - a.min[1], dz = a.max[2] - a.min[2];
+    return (dx + dy) * dz + dx * dy;
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Primitives below every node: leaves start, the second child to arrive at
+// a parent adds both (the same arrival scheme as the refit).
+__global__ void subtree_counts_kernel(const DevAabb* nodes, uint32_t count, const int32_t* parent,
+                                      uint32_t* cnt, uint32_t* arrivals) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    DevAabb n;
+    ldn(nodes + i, n);
+    if (n.extra1 < 0 && n.extra2 >= 0) return;
+    cnt[i] = n.extra1 > 0 ? uint32_t(n.extra1) : 0u;
+    int32_t p = parent[i];
+    // bounded walk: a tree left incomplete by a failed build may hold cycles
+    for (uint32_t steps = 0; p >= 0 && uint32_t(p) < count && steps < count; ++steps) {
+        __threadfence();
+        if (atomicAdd(&arrivals[p], 1u) == 0) return;
+        __threadfence();
+        DevAabb pn;
+        ldn(nodes + p, pn);
+        const uint32_t l = uint32_t(pn.extra2);
+        if (l + 1 >= count) return;
+        cnt[p] = cnt[l] + cnt[l + 1];
+        p = parent[p];
+    }
+}
+
+__device__ __forceinline__ void write_tri(float4* tris, uint32_t pos, const float* verts,
+                                          uint32_t prim_count, uint32_t prim,
+                                          const uint32_t* scene_off, uint32_t scenes) {
+    float4 a = make_float4(0.f, 0.f, 0.f, __uint_as_float(RTBVH_MISS));
+    float4 e1 = make_float4(0.f, 0.f, 0.f, 0.f), e2 = e1;
+    if (prim < prim_count) {
+        const float* v = verts + size_t(prim) * 9;
+        uint32_t id = prim;
+        if (scene_off) {
+            uint32_t lo = 0, hi = scenes;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (scene_off[mid] <= prim)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            id = prim - scene_off[lo];
+        }
+        a = make_float4(v[0], v[1], v[2], __uint_as_float(id));
+        e1 = make_float4(v[3] - v[0], v[4] - v[1], v[5] - v[2], 0.f);
+        e2 = make_float4(v[6] - v[0], v[7] - v[1], v[8] - v[2], 0.f);
+    }
+    tris[size_t(pos) * 3] = a;
+    tris[size_t(pos) * 3 + 1] = e1;
+    tris[size_t(pos) * 3 + 2] = e2;
+}
+
+struct LevelArgs {
+    const DevAabb* nodes;
+    uint32_t node_count;
+    const uint32_t* cnt;
+    const uint32_t* indices;
+    uint32_t index_count;
+    const float* verts;
+    uint32_t prim_count;
+    const uint32_t* scene_off;  // batches: scene-local ids
+    uint32_t scenes;
+    uint32_t max_leaf;
+    DevMbvhNode* wnodes;
+    uint32_t wcap;
+    float4* tris;
+    uint32_t* counters;  // [0] wide nodes, [1] triangles, [2] fail
+};
+
+// Writes the triangles of the binary subtree at `root` from position pos.
+__device__ bool gather_leaf(const LevelArgs& a, int32_t root, uint32_t pos, uint32_t end) {
+    int32_t st[2 * kMaxLeafLimit + 2];
+    int sp = 0;
+    st[sp++] = root;
+    while (sp > 0) {
+        const int32_t i = st[--sp];
+        if (i < 0 || uint32_t(i) >= a.node_count) return false;
+        DevAabb n;
+        ldn(a.nodes + i, n);
+        if (n.extra1 >= 0) {
+            for (int32_t j = 0; j < n.extra1; ++j) {
+                const uint32_t ip = uint32_t(n.extra2) + uint32_t(j);
+                if (pos >= end || n.extra2 < 0 || ip >= a.index_count) return false;
+                write_tri(a.tris, pos++, a.verts, a.prim_count, a.indices[ip], a.scene_off,
+                          a.scenes);
+            }
+        } else {
+            if (sp + 2 > 2 * kMaxLeafLimit + 2 || n.extra2 < 0) return false;
+            st[sp++] = n.extra2 + 1;
+            st[sp++] = n.extra2;
+        }
+    }
+    return pos == end;
+}
+
+__device__ __forceinline__ bool opens(const LevelArgs& a, const DevAabb& n, uint32_t i) {
+    return n.extra1 < 0 && n.extra2 >= 0 && a.cnt[i] > a.max_leaf;
+}
+
+// One wide node from queue item (b, w): pick its slots, allocate wide
+// children and triangle ranges, queue the wide children.
+__device__ void make_node(const LevelArgs& a, uint32_t b, uint32_t w, QItem* qout,
+                          uint32_t* qout_count) {
+    int32_t slot[4];
+    int ns = 0;
+    DevAabb bn;
+    ldn(a.nodes + b, bn);
+    const auto bad = [&](int32_t i) { return i < 0 || uint32_t(i) >= a.node_count; };
+    if (bn.extra1 >= 0) {  // a root that is a leaf: one leaf slot
+        slot[ns++] = int32_t(b);
+    } else {
+        slot[ns++] = bn.extra2;
+        slot[ns++] = bn.extra2 + 1;
+        if (bad(slot[0]) || bad(slot[1])) {  // only in a tree left incomplete
+            atomicOr(&a.counters[2], 1u);
+            return;
+        }
+        while (ns < 4) {
+            int best = -1;
+            float best_area = -1.0f;
+            for (int k = 0; k < ns; ++k) {
+                DevAabb sn;
+                ldn(a.nodes + slot[k], sn);
+                if (!opens(a, sn, uint32_t(slot[k]))) continue;
+                const float ar = half_area(sn);
+                if (!(ar <= best_area)) {  // larger, or the first (NaN areas still open)
+                    best_area = ar;
+                    best = k;
+                }
+            }
+            if (best < 0) break;
+            DevAabb en;
+            ldn(a.nodes + slot[best], en);
+            if (bad(en.extra2) || bad(en.extra2 + 1)) {
+                atomicOr(&a.counters[2], 1u);
+                return;
+            }
+            slot[best] = en.extra2;
+            slot[ns++] = en.extra2 + 1;
+        }
+    }
+    DevAabb sn[4];
+    bool inner[4];
+    uint32_t n_inner = 0, n_tris = 0;
+    for (int k = 0; k < ns; ++k) {
+        ldn(a.nodes + slot[k], sn[k]);
+        inner[k] = bn.extra1 < 0 && opens(a, sn[k], uint32_t(slot[k]));
+        if (inner[k])
+            ++n_inner;
+        else
+            n_tris += sn[k].extra1 >= 0 ? uint32_t(sn[k].extra1) : a.cnt[slot[k]];
+    }
+    const uint32_t wbase = n_inner ? atomicAdd(&a.counters[0], n_inner) : 0u;
+    const uint32_t tbase = n_tris ? atomicAdd(&a.counters[1], n_tris) : 0u;
+    const uint32_t qbase = n_inner ? atomicAdd(qout_count, n_inner) : 0u;
+    if (wbase + n_inner > a.wcap || tbase + n_tris > a.index_count) {
+        atomicOr(&a.counters[2], 1u);
+        return;
+    }
+    DevMbvhNode m;
+    float* mnx = reinterpret_cast<float*>(&m.min_x);
+    float* mxx = reinterpret_cast<float*>(&m.max_x);
+    float* mny = reinterpret_cast<float*>(&m.min_y);
+    float* mxy = reinterpret_cast<float*>(&m.max_y);
+    float* mnz = reinterpret_cast<float*>(&m.min_z);
+    float* mxz = reinterpret_cast<float*>(&m.max_z);
+    int32_t* ch = reinterpret_cast<int32_t*>(&m.children);
+    int32_t* ct = reinterpret_cast<int32_t*>(&m.counts);
+    uint32_t wi = wbase, ti = tbase, qi = qbase;
+    for (int k = 0; k < 4; ++k) {
+        if (k >= ns) {
+            // an unused slot: a point box beyond any ray's reach, never entered
+            mnx[k] = mxx[k] = mny[k] = mxy[k] = mnz[k] = mxz[k] = 3.0e38f;
+            ch[k] = -1;
+            ct[k] = -1;
+            continue;
+        }
+        mnx[k] = sn[k].min[0];
+        mny[k] = sn[k].min[1];
+        mnz[k] = sn[k].min[2];
+        mxx[k] = sn[k].max[0];
+        mxy[k] = sn[k].max[1];
+        mxz[k] = sn[k].max[2];
+        if (inner[k]) {
+            ch[k] = int32_t(wi);
+            ct[k] = -1;
+            qout[qi].b = uint32_t(slot[k]);
+            qout[qi].w = wi;
+            ++wi;
+            ++qi;
+        } else {
+            const uint32_t c = sn[k].extra1 >= 0 ? uint32_t(sn[k].extra1) : a.cnt[slot[k]];
+            ch[k] = int32_t(ti);
+            ct[k] = int32_t(c);
+            if (!gather_leaf(a, slot[k], ti, ti + c)) atomicOr(&a.counters[2], 1u);
+            ti += c;
+        }
+    }
+    a.wnodes[w] = m;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    level_kernel(LevelArgs a, const QItem* qin, const uint32_t* qin_count, QItem* qout,
+                 uint32_t* qout_count) {
+    const uint32_t q = *qin_count;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < q; i += gridDim.x * kBlock) {
+        const QItem it = qin[i];
+        if (it.b >= a.node_count || it.w >= a.wcap) {
+            atomicOr(&a.counters[2], 1u);
+            continue;
+        }
+        make_node(a, it.b, it.w, qout, qout_count);
+    }
+}
+
+// Roots [0, roots) become wide nodes [0, roots); counters start after them.
+__global__ void roots_kernel(uint32_t roots, QItem* q, uint32_t* q_count, uint32_t* counters) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        *q_count = roots;
+        counters[0] = roots;
+        counters[1] = 0;
+        counters[2] = 0;
+    }
+    if (i < roots) {
+        q[i].b = i;
+        q[i].w = i;
+    }
+}
+
+__global__ void clear_word_kernel(uint32_t* w) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *w = 0;
+}
+
+// fail when the last level still queued work (the level budget ran out) or
+// the triangles do not add up to every index entry.
+__global__ void finish_kernel(const uint32_t* last_q, uint32_t* counters, uint32_t index_count,
+                              uint32_t* fail_out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (*last_q != 0 || counters[1] != index_count) counters[2] |= 1u;
+    if (fail_out) *fail_out |= counters[2];
+}
+
+}  // namespace
+
+struct TraceTreePlan {
+    uint32_t count = 0, roots = 1, n = 0, wcap = 0, max_leaf = 1, levels = 0;
+    uint32_t* cnt = nullptr;
+    uint32_t* arrivals = nullptr;
+    int32_t* parents = nullptr;  // owned when the tree had none
+    bool own_parents = false;
+    QItem* q[2] = {nullptr, nullptr};
+    uint32_t* words = nullptr;  // [0..1] queue counts, [2..4] counters
+};
+
+void trace_tree_plan_destroy(TraceTreePlan* t) {
+    if (!t) return;
+    if (t->cnt) (void)hipFree(t->cnt);
+    if (t->arrivals) (void)hipFree(t->arrivals);
+    if (t->own_parents && t->parents) (void)hipFree(t->parents);
+    if (t->q[0]) (void)hipFree(t->q[0]);
+    if (t->q[1]) (void)hipFree(t->q[1]);
+    if (t->words) (void)hipFree(t->words);
+    delete t;
+}
+
+namespace {
+
+LevelArgs level_args(const TraceTreePlan* t, const DeviceBvh& tree, const float* verts,
+                     uint32_t prim_count, const uint32_t* scene_off, uint32_t scenes,
+                     DeviceMbvh* out) {
+    LevelArgs a;
+    a.nodes = tree.nodes;
+    a.node_count = tree.node_count;
+    a.cnt = t->cnt;
+    a.indices = tree.indices;
+    a.index_count = tree.prim_count;
+    a.verts = verts;
+    a.prim_count = prim_count;
+    a.scene_off = scene_off;
+    a.scenes = scenes;
+    a.max_leaf = t->max_leaf;
+    a.wnodes = out->nodes;
+    a.wcap = t->wcap;
+    a.tris = out->tris;
+    a.counters = t->words + 2;
+    return a;
+}
+
+// Parent links, subtree counts and the root queue.
+hipError_t enqueue_setup(const TraceTreePlan* t, const DeviceBvh& tree, hipStream_t stream) {
+    const uint32_t count = t->count;
+    const uint32_t g = (count + kBlock - 1) / kBlock;
+    if (t->own_parents) {
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(g), dim3(kBlock), 0, stream,
+                           reinterpret_cast<uint32_t*>(t->parents), count, 0xFFFFFFFFu);
+        hipError_t e = launch_parent_links(tree.nodes, count, t->parents, stream);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(g), dim3(kBlock), 0, stream, t->arrivals, count, 0u);
+    hipLaunchKernelGGL(subtree_counts_kernel, dim3(g), dim3(kBlock), 0, stream, tree.nodes, count,
+                       t->parents, t->cnt, t->arrivals);
+    const uint32_t gr = (t->roots + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(roots_kernel, dim3(gr), dim3(kBlock), 0, stream, t->roots, t->q[0],
+                       t->words + 0, t->words + 2);
+    return hipGetLastError();
+}
+
+void enqueue_level(const TraceTreePlan* t, const LevelArgs& a, uint32_t level, uint32_t grid,
+                   hipStream_t stream) {
+    const int in = level & 1, outq = in ^ 1;
+    hipLaunchKernelGGL(clear_word_kernel, dim3(1), dim3(1), 0, stream, t->words + outq);
+    hipLaunchKernelGGL(level_kernel, dim3(grid), dim3(kBlock), 0, stream, a, t->q[in],
+                       t->words + in, t->q[outq], t->words + outq);
+}
+
+}  // namespace
+
+ResultCode trace_tree_plan_create(const DeviceBvh& tree, uint32_t max_leaf,
+                                  TraceTreePlan** out_plan, DeviceMbvh* out) {
+    if (max_leaf < 1 || max_leaf > uint32_t(kMaxLeafLimit)) {
+        set_error("max_leaf must be 1..%d", kMaxLeafLimit);
+        return InvalidArgument;
+    }
+    TraceTreePlan* t = new TraceTreePlan();
+    t->count = tree.node_count;
+    t->roots = tree.root_count;
+    t->n = tree.prim_count;
+    t->max_leaf = max_leaf;
+    t->wcap = tree.prim_count + tree.root_count;
+    hipError_t e;
+    if ((e = hipMalloc(&t->cnt, size_t(t->count) * 4)) != hipSuccess ||
+        (e = hipMalloc(&t->arrivals, size_t(t->count) * 4)) != hipSuccess ||
+        (e = hipMalloc(&t->q[0], size_t(t->wcap) * sizeof(QItem))) != hipSuccess ||
+        (e = hipMalloc(&t->q[1], size_t(t->wcap) * sizeof(QItem))) != hipSuccess ||
+        (e = hipMalloc(&t->words, 8 * 4)) != hipSuccess) {
+        trace_tree_plan_destroy(t);
+        return hip_fail(e, "trace tree workspace");
+    }
+    if (tree.parents) {
+        t->parents = tree.parents;
+    } else {
+        if ((e = hipMalloc(&t->parents, size_t(t->count) * 4)) != hipSuccess) {
+            trace_tree_plan_destroy(t);
+            return hip_fail(e, "trace tree parents");
+        }
+        t->own_parents = true;
+    }
+    DeviceMbvh res;
+    if ((e = hipMalloc(&res.nodes, size_t(t->wcap) * sizeof(DevMbvhNode))) != hipSuccess ||
+        (e = hipMalloc(&res.tris, size_t(t->n) * 3 * sizeof(float4))) != hipSuccess) {
+        if (res.nodes) (void)hipFree(res.nodes);
+        trace_tree_plan_destroy(t);
+        return hip_fail(e, "trace tree");
+    }
+    res.indices = nullptr;
+    res.node_count = t->wcap;
+    res.prim_count = t->n;
+    res.root_count = t->roots;
+    *out = res;
+    *out_plan = t;
+    return Ok;
+}
+
+// Eager: levels until the queue is empty (host check every few levels).
+ResultCode trace_tree_build(TraceTreePlan* t, const DeviceBvh& tree, const float* verts,
+                            uint32_t prim_count, const uint32_t* scene_off, uint32_t scenes,
+                            DeviceMbvh* out, hipStream_t stream) {
+    hipError_t e = enqueue_setup(t, tree, stream);
+    if (e != hipSuccess) return hip_fail(e, "trace tree setup");
+    const LevelArgs a = level_args(t, tree, verts, prim_count, scene_off, scenes, out);
+    uint32_t level = 0;
+    uint32_t host_words[8];
+    constexpr uint32_t kChunk = 4;
+    for (;;) {
+        for (uint32_t k = 0; k < kChunk; ++k, ++level) enqueue_level(t, a, level, kLevelGrid, stream);
+        if ((e = hipMemcpyAsync(host_words, t->words, sizeof(host_words), hipMemcpyDeviceToHost,
+                                stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(stream)) != hipSuccess)
+            return hip_fail(e, "trace tree level");
+        if (host_words[level & 1] == 0) break;  // nothing queued for the next level
+        if (level > 4 * 4096) {
+            set_error("trace tree deeper than expected");
+            return Error;
+        }
+    }
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1), 0, stream, t->words + (level & 1),
+                       t->words + 2, t->n, nullptr);
+    if ((e = hipMemcpyAsync(host_words, t->words, sizeof(host_words), hipMemcpyDeviceToHost,
+                            stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(stream)) != hipSuccess)
+        return hip_fail(e, "trace tree finish");
+    if (host_words[4] != 0) {
+        set_error("trace tree construction failed (inconsistent binary tree?)");
+        return Error;
+    }
+    t->levels = level;
+    out->node_count = host_words[2];
+    return Ok;
+}
+
+// Capturable: a fixed number of levels (the eager build's depth plus
+// headroom), failure reported in *fail.
+hipError_t trace_tree_enqueue(const TraceTreePlan* t, const DeviceBvh& tree, const float* verts,
+                              uint32_t prim_count, const uint32_t* scene_off, uint32_t scenes,
+                              DeviceMbvh* out, uint32_t levels, uint32_t* fail,
+                              hipStream_t stream) {
+    hipError_t e = enqueue_setup(t, tree, stream);
+    if (e != hipSuccess) return e;
+    const LevelArgs a = level_args(t, tree, verts, prim_count, scene_off, scenes, out);
+    for (uint32_t level = 0; level < levels; ++level) enqueue_level(t, a, level, kLevelGrid, stream);
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1), 0, stream, t->words + (levels & 1),
+                       t->words + 2, t->n, fail);
+    return hipGetLastError();
+}
+
+uint32_t trace_tree_levels(const TraceTreePlan* t) { return t->levels; }
+
+}  // namespace rtbvh

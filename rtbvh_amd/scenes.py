@@ -343,3 +343,71 @@ def batch_rays_torch(lo, hi, counts, device, generator):
         out[b:e, 3] = RAY_T_MIN
         out[b:e, 7] = RAY_T_MAX
     return out
+
+
+def sphere_grid(nx: int = 64, nz: int = 64, nu: int = 61, nv: int = 40, seed: int = 11):
+    """nx * nz noise-displaced spheres (2 * nu * nv triangles each, radius 0.35,
+    amplitude 0.1) centred at (i + 0.5, 0.5, k + 0.5), each seeded by its
+    index: 64 x 64 x 4,880 = 19,988,480 triangles by default."""
+    out = np.empty((nx * nz, 2 * nu * nv, 9), dtype=np.float32)
+    for i in range(nx):
+        for k in range(nz):
+            s = i * nz + k
+            out[s] = noisy_sphere(nu, nv, center=(i + 0.5, 0.5, k + 0.5), seed=seed * 100003 + s)
+    return out.reshape(-1, 9)
+
+
+def animate_y_torch(base, frame: int, out):
+    """Per-frame animation of a (N, 9) vertex tensor: y += 0.02 sin(2 pi (x +
+    frame / 60)), written into `out` (same shape)."""
+    import math
+
+    import torch
+
+    out.copy_(base)
+    out[:, 1::3] += 0.02 * torch.sin(2.0 * math.pi * (base[:, 0::3] + frame / 60.0))
+    return out
+
+
+def triangle_aabbs_centers_torch(verts, aabbs_out, centers_out):
+    """RTAabb records (N, 8) and centres (N, 3) of (N, 9) triangles on the
+    device: boxes grown from the empty box by v0, v1, v2, centre
+    ((v0 + v1) + v2) * (1 / 3)."""
+    import torch
+
+    v = verts.view(-1, 3, 3)
+    aabbs_out[:, 0:3] = torch.minimum(torch.minimum(v[:, 0], v[:, 1]), v[:, 2])
+    aabbs_out[:, 4:7] = torch.maximum(torch.maximum(v[:, 0], v[:, 1]), v[:, 2])
+    aabbs_out[:, 3] = 0.0
+    aabbs_out[:, 7] = 0.0
+    centers_out.copy_(((v[:, 0] + v[:, 1]) + v[:, 2]) * np.float32(1.0 / 3.0))
+
+
+def shadow_rays_torch(verts, n: int, light, device, generator):
+    """n rays from uniformly chosen points on uniformly chosen triangles
+    (offset 1e-4 along the normal facing the light) towards a point light,
+    t_max = (1 - 1e-4) * distance."""
+    import torch
+
+    tri = torch.randint(0, verts.shape[0], (n,), device=device, generator=generator)
+    out = torch.empty((n, 8), dtype=torch.float32, device=device)
+    lt = torch.as_tensor(light, dtype=torch.float32, device=device)
+    chunk = 1 << 24
+    for b in range(0, n, chunk):
+        e = min(n, b + chunk)
+        v = verts[tri[b:e]].view(-1, 3, 3)
+        r1 = torch.rand((e - b, 1), device=device, generator=generator).sqrt()
+        r2 = torch.rand((e - b, 1), device=device, generator=generator)
+        p = (1 - r1) * v[:, 0] + r1 * (1 - r2) * v[:, 1] + r1 * r2 * v[:, 2]
+        nrm = torch.linalg.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+        nrm = nrm / nrm.norm(dim=1, keepdim=True).clamp_min(1e-20)
+        to_l = lt - p
+        nrm = torch.where(((nrm * to_l).sum(1, keepdim=True) < 0), -nrm, nrm)
+        o = p + 1e-4 * nrm
+        d = lt - o
+        dist = d.norm(dim=1, keepdim=True).clamp_min(1e-20)
+        out[b:e, 0:3] = o
+        out[b:e, 3] = RAY_T_MIN
+        out[b:e, 4:7] = d / dist
+        out[b:e, 7] = (1.0 - 1e-4) * dist[:, 0]
+    return out

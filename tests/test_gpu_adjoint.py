@@ -7,13 +7,16 @@ the gradient is pinned by mathematics, not by reference outputs:
     closed form in FP32: relative error (norm of the difference over the norm
     of the FP64 gradient) <= 1e-4 for vertex, origin and direction
     gradients;
-  * central finite differences, h = 1e-3, of the FP64 forward with the hit
-    triangles fixed, against the FP64 gradient: relative error <= 1e-5;
+  * central finite differences of the FP64 forward with the hit triangles
+    fixed, against the FP64 gradient: relative error <= 2e-2 at h = 1e-3
+    (the truncation error is ~(h / edge)^2 with edges ~0.03) and <= 1e-5 at
+    h = 1e-5;
   * central finite differences, h = 1e-3, of the GPU forward itself (full
     re-trace, shared vertex positions moved together so the mesh stays
-    watertight), against the GPU gradient: relative error <= 5e-2 (FP32
-    forward, and rays near an edge may change triangle).
-Loss: L = sum over hit rays of w_t t + w_u u + w_v v, w ~ N(0, 1).
+    watertight), against the GPU gradient of L = sum of w_t t: relative
+    error <= 5e-2 (FP32 forward, and rays near an edge may change triangle;
+    t is continuous there, u and v are not, so they are left out).
+Loss otherwise: L = sum over hit rays of w_t t + w_u u + w_v v, w ~ N(0, 1).
 """
 import numpy as np
 import pytest
@@ -118,36 +121,43 @@ def test_fp64_gradient_matches_central_differences():
     d64 = rays[hit, 4:7].astype(np.float64)
     p, wh = prim[hit], w[hit]
     gv, go, gd, _ = _grad_f64(verts64, o64, d64, p, wh)
-    rng = np.random.default_rng(0)
-    h = 1e-3
-    fd, an = [], []
-    for tri in rng.choice(np.unique(p), 12, replace=False):
-        for c in rng.choice(9, 3, replace=False):
-            vp, vm = verts64.copy(), verts64.copy()
-            vp[tri, c] += h
-            vm[tri, c] -= h
-            fd.append((_loss_f64(vp, o64, d64, p, wh) - _loss_f64(vm, o64, d64, p, wh)) / (2 * h))
-            an.append(gv[tri, c])
-    for r in rng.choice(len(p), 12, replace=False):
-        for arr, g in ((o64, go), (d64, gd)):
-            c = int(rng.integers(3))
-            ap, am = arr.copy(), arr.copy()
-            ap[r, c] += h
-            am[r, c] -= h
-            if arr is o64:
-                lp, lm = _loss_f64(verts64, ap, d64, p, wh), _loss_f64(verts64, am, d64, p, wh)
-            else:
-                lp, lm = _loss_f64(verts64, o64, ap, p, wh), _loss_f64(verts64, o64, am, p, wh)
-            fd.append((lp - lm) / (2 * h))
-            an.append(g[r, c])
-    err = _rel(np.asarray(an), np.asarray(fd))
-    print(f"[adjoint_fd64] relative error {err:.2e} over {len(fd)} coordinates", flush=True)
-    assert err <= 1e-5
+    # h = 1e-3: truncation error ~ (h / edge)^2 with edges ~0.03, so ~1e-2;
+    # h = 1e-5 shrinks it to ~1e-7 and pins the formula tightly
+    for h, tol in ((1e-3, 2e-2), (1e-5, 1e-5)):
+        rng = np.random.default_rng(0)
+        fd, an = [], []
+        for tri in rng.choice(np.unique(p), 12, replace=False):
+            for c in rng.choice(9, 3, replace=False):
+                vp, vm = verts64.copy(), verts64.copy()
+                vp[tri, c] += h
+                vm[tri, c] -= h
+                fd.append((_loss_f64(vp, o64, d64, p, wh) - _loss_f64(vm, o64, d64, p, wh))
+                          / (2 * h))
+                an.append(gv[tri, c])
+        for r in rng.choice(len(p), 12, replace=False):
+            for arr, g in ((o64, go), (d64, gd)):
+                c = int(rng.integers(3))
+                ap, am = arr.copy(), arr.copy()
+                ap[r, c] += h
+                am[r, c] -= h
+                if arr is o64:
+                    lp, lm = _loss_f64(verts64, ap, d64, p, wh), _loss_f64(verts64, am, d64, p, wh)
+                else:
+                    lp, lm = _loss_f64(verts64, o64, ap, p, wh), _loss_f64(verts64, o64, am, p, wh)
+                fd.append((lp - lm) / (2 * h))
+                an.append(g[r, c])
+        err = _rel(np.asarray(an), np.asarray(fd))
+        print(f"[adjoint_fd64] h={h:g}: relative error {err:.2e} over {len(fd)} coordinates",
+              flush=True)
+        assert err <= tol, (h, err)
 
 
 def test_gpu_gradient_matches_gpu_central_differences():
     print("[adjoint_fd_gpu]", flush=True)
     verts, rays, tree, w, dev = _setup(seed=5, n_rays=20000)
+    # t alone: with shared positions moved together t stays continuous, while
+    # u and v jump where a ray changes triangle
+    w[:, 1:] = 0.0
     _, _, _, prim, g_v, _, _ = _forward_backward(verts, rays, tree, w, dev)
     corners = verts.reshape(-1, 3)
     keys, inverse = np.unique(corners, axis=0, return_inverse=True)

@@ -214,3 +214,41 @@ def test_prepared_triangles_match_vertex_path(kind):
     m.attach_triangles(_dev(moved))
     got2 = m.intersect_prepared(dr).cpu().numpy()
     assert not np.array_equal(got2.view(np.uint32), got.view(np.uint32).reshape(-1, 4))
+
+
+@pytest.mark.parametrize("kind", ["ploc", "binned"])
+def test_trace_tree_matches_mbvh(kind):
+    """rtbvh_gpu_create_trace_tree: a different 4-wide tree over the same
+    binary tree and triangles; every leaf size gives, bit for bit, the hit
+    records and occlusion flags of the MBVH's lowest-id traversal (and so of
+    brute force)."""
+    print(f"[trace_tree] kind={kind}", flush=True)
+    verts = np.concatenate([scenes.noisy_sphere(90, 60, seed=15),
+                            scenes.random_soup(4000, 0.03, seed=16)])
+    bvh, _, _ = _build(verts, kind)
+    m = rt.Mbvh.from_bvh(bvh)
+    rays8 = ray_set(verts, 30000, seed=17, kind="box")
+    rays8[::3, 7] = np.random.default_rng(2).uniform(0.0, 2.0, len(rays8[::3])).astype(np.float32)
+    dv, dr = _dev(verts), _dev(rays8)
+    want = m.intersect(dv, dr).cpu().numpy()
+    want_occ = m.occluded(dv, dr).cpu().numpy()
+    brute = ref.trace_brute(verts, as_rays(rays8), threads=8)
+    assert hits_equal(want.view(ref.HIT).reshape(-1), brute).all()
+    for max_leaf in (1, 2, 4, 8):
+        tt = bvh.trace_tree(dv, max_leaf=max_leaf)
+        assert 0 < tt.node_count <= len(verts)
+        got = tt.intersect(dr).cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), max_leaf
+        assert np.array_equal(tt.occluded(dr).cpu().numpy(), want_occ), max_leaf
+        tt.status()
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 17])
+def test_trace_tree_small(n):
+    print(f"[trace_tree_small] n={n}", flush=True)
+    verts = scenes.random_soup(n, 0.3, seed=n)
+    bvh, _, _ = _build(verts, "ploc")
+    rays8 = ray_set(verts, 2000, seed=n, kind="box")
+    got = bvh.trace_tree(_dev(verts)).intersect(_dev(rays8)).cpu().numpy()
+    want = ref.trace_brute(verts, as_rays(rays8), threads=8)
+    assert hits_equal(got.view(ref.HIT).reshape(-1), want).all()

@@ -176,3 +176,37 @@ def test_trace_brute_force_large():
     got = _gpu_hits(mbvh, verts, big, False)
     want = ref.trace_mbvh(m, idx, verts, as_rays(big), threads=8)
     assert hits_equal(got, want).all()
+
+
+def test_launch_shapes_and_autotune_keep_results():
+    """The traversal's launch shape (block size, LDS stack depth, persistent
+    grid size) changes speed only: every shape gives the same hit records,
+    the autotuner's pick included, with deep stacks spilling from LDS to
+    private memory at depth 0."""
+    print("[launch_shapes]", flush=True)
+    verts = scenes.noisy_sphere(100, 80, seed=11)
+    aabbs, centers = prep(verts)
+    bvh = rt.Builder(_dev(centers), aabbs=_dev(flat(aabbs))).construct_locally_ordered_clustered()
+    m = rt.Mbvh.from_bvh(bvh)
+    dv = _dev(verts)
+    m.attach_triangles(dv)
+    rays = _dev(ray_set(verts, 50000, seed=21, kind="box"))
+    default = rt.get_trace_config()
+    want = m.intersect_prepared(rays).cpu().numpy()
+    try:
+        for block, depth, occ in ((64, 0, 100), (128, 4, 50), (256, 24, 200), (64, 32, 400)):
+            rt.set_trace_config(block, depth, occ)
+            got = m.intersect_prepared(rays).cpu().numpy()
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (block, depth, occ)
+            occ_got = m.occluded_prepared(rays).cpu().numpy()
+            assert np.array_equal(occ_got, (want[:, 3].view(np.uint32) != 0xFFFFFFFF).astype(np.uint8))
+        with pytest.raises(rt.RTBvhError, match="InvalidArgument"):
+            rt.set_trace_config(96, 4, 100)
+        best = rt.autotune(m, rays)
+        assert rt.get_trace_config() == best
+        got = m.intersect_prepared(rays).cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+        m.status()
+    finally:
+        rt.set_trace_config(None)
+    assert rt.get_trace_config() == default
