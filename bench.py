@@ -1,17 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark of the GPU hot path on one MI355X.
 
-Workload (1M-triangle scene, the benchmark configuration's second case):
+Headline workload (1M-triangle scene):
   1,000,000 FP32 triangles — a 750x600-quad noisy sphere (900k) plus 100k
   random triangles (edge 0.01) in the unit cube; GPU PLOC build (search
-  radius 14) + MBVH collapse; 8192^2 = 67,108,864 pinhole primary rays and
-  the same number of cosine-weighted bounce rays from the primary hits.
+  radius 14), the reference-layout MBVH collapse and the 4-wide traversal
+  tree; 8192^2 = 67,108,864 pinhole primary rays and the same number of
+  cosine-weighted bounce rays from the primary hits.
 
-A step is one closest-hit pass over the 67,108,864 bounce rays (the
-incoherent set), inputs resident in HBM.  The headline value is
-bounce-ray Mrays/s; the JSON line also carries the PLOC build time, the MBVH
-collapse time, the primary-ray rate, the roofline of the trace kernel and
-the CPU oracle's rate on a sample of the same rays (host threads stated).
+A step is one closest-hit launch over the 67,108,864 bounce rays (the
+incoherent set), inputs resident in HBM, lowest-(t, id) rule (hit records
+identical to brute force; a sample is checked against the CPU oracle).  The
+headline value is bounce-ray Mrays/s of the traversal tree; the JSON line
+also carries the build times (PLOC, collapse, traversal tree, binned SAH),
+the primary-ray rate, the reference-layout MBVH's rate on the same rays, the
+roofline of the trace kernel, the CPU oracle's rate on a sample of the same
+rays (host threads stated), the batched-vs-plain comparison over 256 scenes
+of uneven size, and the animated 20M-triangle any-hit frame.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -249,7 +254,7 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--quick", action="store_true",
                     help="one build of each kind and no binned SAH (for profiled runs)")
-    ap.add_argument("--max-leaf", type=int, default=4,
+    ap.add_argument("--max-leaf", type=int, default=1,
                     help="primitives per leaf of the traversal tree")
     ap.add_argument("--batch-scenes", type=int, default=256,
                     help="scenes in the batched-vs-plain comparison (0 skips it)")
@@ -383,6 +388,11 @@ def main():
         per = [a.elapsed_time(b) for a, b in ev]
         return out, per, wall
 
+    # launch shape chosen by the autotuner on a 16M-ray sample of the bounce
+    # rays (untimed set-up, like the builds)
+    tuned = rt.autotune(ttree, bounce[: min(n_rays, 1 << 24)])
+    log(f"autotuned launch shape {tuned}")
+
     log("timing primary rays")
     _, per_p, _ = timed(primary, 3, 1)
     primary_mrays = n_rays / (statistics.median(per_p) * 1e-3) / 1e6
@@ -505,6 +515,7 @@ def main():
         "attach_triangles_ms": round(statistics.median(attach_ms), 3),
         "trace_tree_build_ms": round(statistics.median(tree_ms), 3),
         "trace_tree_max_leaf": args.max_leaf,
+        "trace_launch_shape": tuned,
         "binned_sah_build_ms": round(statistics.median(binned_ms), 3),
         "primary_mrays_s": round(primary_mrays, 2),
         "mbvh_prepared_mrays_s": round(mbvh_mrays, 2),

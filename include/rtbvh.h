@@ -263,7 +263,7 @@ typedef struct RTTraceTree {
 } RTTraceTree;
 
 /* bvh: made by rtbvh_gpu_create_bvh; vertices: device, 9 floats per
- * primitive; max_leaf 1..8 (0: 4). */
+ * primitive; max_leaf 1..8 (0: 1, the fastest on the benchmark scenes). */
 ResultCode rtbvh_gpu_create_trace_tree(RTBvh bvh, const float *vertices, size_t prim_count,
                                        uint32_t max_leaf, void *stream, RTTraceTree *result);
 ResultCode rtbvh_gpu_trace_tree_intersect(RTTraceTree tree, const RTRay *rays, size_t ray_count,
@@ -279,22 +279,27 @@ void rtbvh_gpu_free_trace_tree(RTTraceTree tree);
 
 /* ---- Launch shape of the lowest-id MBVH traversal ---------------------------
  * (no counterpart in the reference).  Per device; the default is
- * {256, 12, 100}.  Results never depend on it, only speed. */
+ * {256, 16, 100, 0}.  Results never depend on it, only speed. */
 typedef struct RTTraceConfig {
     uint32_t block;          /* threads per block: 64, 128 or 256 */
     uint32_t lds_stack;      /* traversal stack entries per ray kept in LDS, 0..32 */
     uint32_t occupancy_pct;  /* persistent grid as a share of resident blocks, 10..400 */
+    uint32_t kernel;         /* with leaf-ordered triangles: 0 one node or one triangle per
+                                ray per iteration, 1 a node and all its leaf slots */
 } RTTraceConfig;
 
 ResultCode rtbvh_gpu_set_trace_config(const RTTraceConfig *config); /* NULL: default */
 ResultCode rtbvh_gpu_get_trace_config(RTTraceConfig *config);
-/* Autotuner: times 54 launch shapes (block 64/128/256 x LDS stack 0..24 x
- * occupancy 50/100/200 %) of the closest-hit traversal of `rays` against
+/* Autotuner: times 60 launch shapes (kernel 0/1 x block 64/128/256 x LDS
+ * stack 4..24 x occupancy 100/200 %) of the closest-hit traversal of `rays` against
  * `mbvh`'s attached triangles, writing `hits` (device, ray_count records),
  * and makes the fastest the current device's shape (also returned in *best).
  * Synchronises the stream. */
 ResultCode rtbvh_gpu_autotune_mbvh(RTMbvh mbvh, const RTRay *rays, size_t ray_count, RTHit *hits,
                                    void *stream, RTTraceConfig *best);
+/* The same over a traversal tree. */
+ResultCode rtbvh_gpu_autotune_trace_tree(RTTraceTree tree, const RTRay *rays, size_t ray_count,
+                                         RTHit *hits, void *stream, RTTraceConfig *best);
 
 /* ---- Adjoint of the hit ----------------------------------------------------
  * Gradients of a loss L through closest hits made by the rtbvh_gpu_intersect*
@@ -339,9 +344,9 @@ ResultCode rtbvh_gpu_create_batch(const RTAabb *aabbs, const float *centers, siz
                                   const uint32_t *scene_offsets, size_t scene_count,
                                   void *stream, RTBatch *result);
 
-/* Builds the batch's traversal tree (as rtbvh_gpu_create_trace_tree, leaves
- * of up to 4 primitives, scene-local ids) from vertices: device, 9 floats per
- * primitive of the batch.  Needed before tracing the batch. */
+/* Builds the batch's traversal tree (as rtbvh_gpu_create_trace_tree with
+ * max_leaf 0, scene-local ids) from vertices: device, 9 floats per primitive
+ * of the batch.  Needed before tracing the batch. */
 ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch batch, const float *vertices,
                                             size_t prim_count, void *stream);
 
@@ -360,7 +365,7 @@ void rtbvh_gpu_free_batch(RTBatch batch);
 /* ---- Capturable batch build + trace ----------------------------------------
  * A plan holds every buffer for one batch shape (scene offsets fixed at
  * creation) and enqueues PLOC build, traversal tree (as
- * rtbvh_gpu_create_trace_tree, leaves of up to 4 primitives) and traversal
+ * rtbvh_gpu_create_trace_tree with max_leaf 0) and traversal
  * without allocating and without waiting for the device, so the whole frame
  * can live in a hipGraph (no counterpart in the reference).  Clustering
  * iterations and tree levels are fixed at creation by an eager build of the

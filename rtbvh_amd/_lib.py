@@ -60,7 +60,7 @@ class RTBatchPlan(ctypes.Structure):
 
 class RTTraceConfig(ctypes.Structure):
     _fields_ = [("block", ctypes.c_uint32), ("lds_stack", ctypes.c_uint32),
-                ("occupancy_pct", ctypes.c_uint32)]
+                ("occupancy_pct", ctypes.c_uint32), ("kernel", ctypes.c_uint32)]
 
 
 class RTTraceTree(ctypes.Structure):
@@ -101,6 +101,8 @@ SIGNATURES = {
     "rtbvh_gpu_set_trace_config": (_rc, [ctypes.POINTER(RTTraceConfig)]),
     "rtbvh_gpu_get_trace_config": (_rc, [ctypes.POINTER(RTTraceConfig)]),
     "rtbvh_gpu_autotune_mbvh": (_rc, [RTMbvh, _vp, _sz, _vp, _vp, ctypes.POINTER(RTTraceConfig)]),
+    "rtbvh_gpu_autotune_trace_tree": (_rc, [RTTraceTree, _vp, _sz, _vp, _vp,
+                                           ctypes.POINTER(RTTraceConfig)]),
     "rtbvh_gpu_adjoint": (_rc, [_vp, _sz, _vp, _vp, _sz, _vp, _vp, _vp, _vp, _vp]),
     "rtbvh_gpu_batch_plan_create": (_rc, [_vp, _vp, _sz, _vp, _vp, _sz, _vp,
                                          ctypes.POINTER(RTBatchPlan)]),

@@ -203,7 +203,7 @@ class Bvh(_Tree):
             raise ValueError("aabbs must be (prim_count, 8)")
         L.check(L.lib().rtbvh_gpu_refit(_ptr(aabbs), self._h, _stream(aabbs)), "rtbvh_gpu_refit")
 
-    def trace_tree(self, vertices: torch.Tensor, max_leaf: int = 4) -> "TraceTree":
+    def trace_tree(self, vertices: torch.Tensor, max_leaf: int = 1) -> "TraceTree":
         """Traversal tree over `vertices` (see TraceTree)."""
         return TraceTree(self, vertices, max_leaf)
 
@@ -218,7 +218,7 @@ class TraceTree:
     the lowest-(t, id) traversal (rtbvh_gpu_create_trace_tree); hits equal
     Mbvh.intersect's default mode on the same triangles."""
 
-    def __init__(self, bvh: "Bvh", vertices: torch.Tensor, max_leaf: int = 4):
+    def __init__(self, bvh: "Bvh", vertices: torch.Tensor, max_leaf: int = 1):
         if not bvh._owned:
             raise ValueError("a trace tree needs a Bvh built by this library")
         _require_device(vertices, "vertices")
@@ -580,25 +580,30 @@ def get_trace_config() -> dict:
     """Launch shape of the lowest-id MBVH traversal on the current device."""
     c = L.RTTraceConfig()
     L.check(L.lib().rtbvh_gpu_get_trace_config(ctypes.byref(c)), "rtbvh_gpu_get_trace_config")
-    return {"block": c.block, "lds_stack": c.lds_stack, "occupancy_pct": c.occupancy_pct}
+    return {"block": c.block, "lds_stack": c.lds_stack, "occupancy_pct": c.occupancy_pct,
+            "kernel": c.kernel}
 
 
-def set_trace_config(block: Optional[int] = None, lds_stack: int = 12, occupancy_pct: int = 100):
+def set_trace_config(block: Optional[int] = None, lds_stack: int = 16, occupancy_pct: int = 100,
+                     kernel: int = 0):
     """Sets the traversal launch shape (block=None restores the default)."""
     if block is None:
         L.check(L.lib().rtbvh_gpu_set_trace_config(None), "rtbvh_gpu_set_trace_config")
         return
-    c = L.RTTraceConfig(block, lds_stack, occupancy_pct)
+    c = L.RTTraceConfig(block, lds_stack, occupancy_pct, kernel)
     L.check(L.lib().rtbvh_gpu_set_trace_config(ctypes.byref(c)), "rtbvh_gpu_set_trace_config")
 
 
-def autotune(mbvh: "Mbvh", rays: torch.Tensor) -> dict:
-    """Times the candidate launch shapes on `rays` against `mbvh` (which needs
-    attached triangles) and keeps the fastest for the current device."""
+def autotune(tree, rays: torch.Tensor) -> dict:
+    """Times the candidate launch shapes on `rays` against `tree` (an Mbvh with
+    attached triangles, or a TraceTree) and keeps the fastest for the current
+    device."""
     _require_device(rays, "rays")
     hits = torch.empty((rays.shape[0], HIT_FLOATS), dtype=torch.float32, device=rays.device)
     c = L.RTTraceConfig()
-    L.check(L.lib().rtbvh_gpu_autotune_mbvh(mbvh._h, _ptr(rays), rays.shape[0], _ptr(hits),
-                                            _stream(rays), ctypes.byref(c)),
-            "rtbvh_gpu_autotune_mbvh")
-    return {"block": c.block, "lds_stack": c.lds_stack, "occupancy_pct": c.occupancy_pct}
+    fn = (L.lib().rtbvh_gpu_autotune_trace_tree if isinstance(tree, TraceTree)
+          else L.lib().rtbvh_gpu_autotune_mbvh)
+    L.check(fn(tree._h, _ptr(rays), rays.shape[0], _ptr(hits), _stream(rays), ctypes.byref(c)),
+            fn.__name__)
+    return {"block": c.block, "lds_stack": c.lds_stack, "occupancy_pct": c.occupancy_pct,
+            "kernel": c.kernel}

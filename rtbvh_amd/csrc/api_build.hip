@@ -84,6 +84,10 @@ void release(BatchEntry& b) {
     b.scene_off = nullptr;
 }
 
+// Frees are ordered after all work already enqueued on the device, which may
+// still read the arrays (hipFree alone is not relied on for that).
+void drain_device() { (void)hipDeviceSynchronize(); }
+
 // Switches to `device` for the lifetime of the guard.
 struct DeviceGuard {
     int prev = -1;
@@ -508,7 +512,10 @@ ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch batch, const float* vertices
     }
     // the traversal tree of the batch (tight 4-wide forest, leaves of up to
     // 4 triangles, scene-local ids)
-    release(be->ttree);
+    if (be->ttree.nodes) {
+        drain_device();  // traversals of the previous tree may be in flight
+        release(be->ttree);
+    }
     hipStream_t st = static_cast<hipStream_t>(stream);
     ResultCode rc = ensure_parents(be->bvh, st);
     if (rc != Ok) return rc;
@@ -580,6 +587,7 @@ void rtbvh_gpu_free_trace_tree(RTTraceTree tree) {
         return;
     }
     DeviceGuard g(r.ttrees[tree.id]->device);
+    drain_device();
     release(r.ttrees[tree.id]->t);
     r.ttrees[tree.id].reset();
 }
@@ -592,6 +600,7 @@ void rtbvh_gpu_free_batch(RTBatch batch) {
         return;
     }
     DeviceGuard g(r.batches[batch.id]->device);
+    drain_device();
     release(*r.batches[batch.id]);
     r.batches[batch.id].reset();
 }
@@ -697,6 +706,7 @@ void free_bvh(RTBvh bvh) {
     }
     if (r.bvhs[bvh.id]) {
         DeviceGuard g(r.bvhs[bvh.id]->device);
+        drain_device();
         release(r.bvhs[bvh.id]->dev);
         r.bvhs[bvh.id].reset();
     }
@@ -711,6 +721,7 @@ void free_mbvh(RTMbvh bvh) {
     }
     if (r.mbvhs[bvh.id]) {
         DeviceGuard g(r.mbvhs[bvh.id]->device);
+        drain_device();
         release(r.mbvhs[bvh.id]->dev);
         r.mbvhs[bvh.id].reset();
     }

@@ -407,9 +407,10 @@ ResultCode rtbvh_gpu_set_trace_config(const RTTraceConfig* config) {
         set_trace_config(nullptr);
         return Ok;
     }
-    const TraceConfig c{config->block, config->lds_stack, config->occupancy_pct};
+    const TraceConfig c{config->block, config->lds_stack, config->occupancy_pct, config->kernel};
     if (!valid_trace_config(c)) {
-        set_error("trace config: block 64/128/256, lds_stack <= 32, occupancy_pct 10..400");
+        set_error("trace config: block 64/128/256, lds_stack <= 32, occupancy_pct 10..400, "
+                  "kernel 0/1");
         return InvalidArgument;
     }
     set_trace_config(&c);
@@ -425,14 +426,21 @@ ResultCode rtbvh_gpu_get_trace_config(RTTraceConfig* config) {
     config->block = c.block;
     config->lds_stack = c.lds_depth;
     config->occupancy_pct = c.occupancy_pct;
+    config->kernel = c.kernel;
     return Ok;
 }
 
-// Times every candidate launch shape on the caller's rays (closest hit over
-// the attached triangles, best of two runs each) and keeps the fastest as the
+}  // extern "C"
+
+namespace rtbvh {
+namespace {
+
+// Times every candidate launch shape of `trace` (closest hit of the caller's
+// rays, best of two runs after a warm-up) and keeps the fastest as the
 // device's configuration.
-ResultCode rtbvh_gpu_autotune_mbvh(RTMbvh mbvh, const RTRay* rays, size_t ray_count, RTHit* hits,
-                                   void* stream, RTTraceConfig* best) {
+template <typename Trace>
+ResultCode autotune(const RTRay* rays, size_t ray_count, RTHit* hits, void* stream,
+                    RTTraceConfig* best, Trace trace) {
     if (rays == nullptr || hits == nullptr || ray_count == 0) {
         set_error("autotuning needs rays and hits");
         return Error;
@@ -446,21 +454,23 @@ ResultCode rtbvh_gpu_autotune_mbvh(RTMbvh mbvh, const RTRay* rays, size_t ray_co
         return hip_fail(e, "autotune events");
     }
     const TraceConfig before = get_trace_config();
+    static const uint32_t kernels[] = {0, 1};
     static const uint32_t blocks[] = {64, 128, 256};
-    static const uint32_t depths[] = {0, 4, 8, 12, 16, 24};
-    static const uint32_t occ[] = {50, 100, 200};
+    static const uint32_t depths[] = {4, 8, 12, 16, 24};
+    static const uint32_t occ[] = {100, 200};
     TraceConfig top = before;
     float top_ms = 3.4e38f;
     ResultCode rc = Ok;
+    for (uint32_t kn : kernels) {
     for (uint32_t b : blocks) {
         for (uint32_t d : depths) {
             for (uint32_t o : occ) {
-                const TraceConfig c{b, d, o};
+                const TraceConfig c{b, d, o, kn};
                 set_trace_config(&c);
                 float ms_best = 3.4e38f;
                 for (int rep = 0; rep < 3 && rc == Ok; ++rep) {
                     if ((e = hipEventRecord(e0, st)) != hipSuccess) break;
-                    rc = prepared_entry(mbvh, rays, ray_count, hits, nullptr, 0, stream);
+                    rc = trace();
                     if (rc != Ok) break;
                     if ((e = hipEventRecord(e1, st)) != hipSuccess ||
                         (e = hipEventSynchronize(e1)) != hipSuccess)
@@ -479,6 +489,8 @@ ResultCode rtbvh_gpu_autotune_mbvh(RTMbvh mbvh, const RTRay* rays, size_t ray_co
         }
         if (rc != Ok || e != hipSuccess) break;
     }
+        if (rc != Ok || e != hipSuccess) break;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc != Ok || e != hipSuccess) {
@@ -490,8 +502,28 @@ ResultCode rtbvh_gpu_autotune_mbvh(RTMbvh mbvh, const RTRay* rays, size_t ray_co
         best->block = top.block;
         best->lds_stack = top.lds_depth;
         best->occupancy_pct = top.occupancy_pct;
+        best->kernel = top.kernel;
     }
     return Ok;
+}
+
+}  // namespace
+}  // namespace rtbvh
+
+extern "C" {
+
+ResultCode rtbvh_gpu_autotune_mbvh(RTMbvh mbvh, const RTRay* rays, size_t ray_count, RTHit* hits,
+                                   void* stream, RTTraceConfig* best) {
+    return autotune(rays, ray_count, hits, stream, best, [&]() {
+        return prepared_entry(mbvh, rays, ray_count, hits, nullptr, 0, stream);
+    });
+}
+
+ResultCode rtbvh_gpu_autotune_trace_tree(RTTraceTree tree, const RTRay* rays, size_t ray_count,
+                                         RTHit* hits, void* stream, RTTraceConfig* best) {
+    return autotune(rays, ray_count, hits, stream, best, [&]() {
+        return tree_entry(tree, rays, ray_count, hits, nullptr, 0, stream);
+    });
 }
 
 ResultCode rtbvh_gpu_status(void* stream) {

@@ -334,6 +334,7 @@ void rtbvh_gpu_batch_plan_free(RTBatchPlan plan) {
         return;
     }
     (void)hipStreamSynchronize(p.v[plan.id]->stream);
+    (void)hipDeviceSynchronize();  // eager enqueues may still read the buffers
     p.v[plan.id].reset();
 }
 

@@ -39,6 +39,8 @@ struct TraceConfig {
     uint32_t block;          // threads per block: 64, 128 or 256
     uint32_t lds_depth;      // stack entries per lane in LDS (0..32)
     uint32_t occupancy_pct;  // persistent grid as a share of resident blocks (10..400)
+    uint32_t kernel;         // with leaf-ordered triangles: 0 one item per iteration,
+                             // 1 a node and all its leaf slots per iteration
 };
 TraceConfig default_trace_config();
 bool valid_trace_config(const TraceConfig& c);
@@ -164,7 +166,7 @@ struct BatchView {
 bool batch_view(uint32_t id, BatchView* out);
 bool trace_tree_view(uint32_t id, BatchView* out);
 
-constexpr uint32_t kDefaultMaxLeaf = 4;
+constexpr uint32_t kDefaultMaxLeaf = 1;
 
 void set_error(const char* fmt, ...);
 ResultCode hip_fail(hipError_t e, const char* what);

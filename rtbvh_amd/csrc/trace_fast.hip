@@ -277,6 +277,240 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_fast_kernel(TraceParams p) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// One item per lane per iteration.  The kernel above tests a node's leaf
+// slots inside the node step, so every iteration of a wave pays for up to
+// four triangle tests and their loops whenever any lane has a leaf there
+// (most of its issue slots then run with few lanes active).  Here a lane
+// holds one item — a node, or a leaf's remaining triangles — and each
+// iteration does at most one node visit and one triangle test per lane:
+// the node step makes the nearest child slot (node or leaf) the item in
+// hand and pushes the others, far to near, with their entry distance; the
+// triangle step tests one triangle of the leaf in hand.  Leaves travel on
+// the stack as tagged entries: bit 31 set, bits 27..30 count - 1, bits 0..26
+// the first triangle, so this kernel needs fewer than 2^27 index entries.
+// Culling and the lowest-(t, id) rule are those of the kernel above.
+constexpr uint32_t kLeafFlag = 0x80000000u;
+constexpr uint32_t kLeafFirstMask = (1u << 27) - 1u;
+constexpr uint32_t kLeafChunk = 16;
+
+__device__ __forceinline__ int32_t sel4i(uint32_t s, int32_t a, int32_t b, int32_t c, int32_t d) {
+    const int32_t lo = (s & 1u) ? b : a;
+    const int32_t hi = (s & 1u) ? d : c;
+    return (s & 2u) ? hi : lo;
+}
+
+__device__ __forceinline__ float sel4f(uint32_t s, float a, float b, float c, float d) {
+    const float lo = (s & 1u) ? b : a;
+    const float hi = (s & 1u) ? d : c;
+    return (s & 2u) ? hi : lo;
+}
+
+// Pushes a node (count < 0) or a leaf range, split into entries of at most
+// kLeafChunk triangles.
+__device__ __forceinline__ bool push_item(const LdsStack& ls, DeepStack& deep, int& sp,
+                                          int32_t node, uint32_t first, int32_t count, float t) {
+    if (count < 0) return stack_push(ls, deep, sp, node, t);
+    uint32_t left = uint32_t(count);
+    while (left > 0) {
+        const uint32_t c = left < kLeafChunk ? left : kLeafChunk;
+        if (!stack_push(ls, deep, sp, int32_t(kLeafFlag | ((c - 1u) << 27) | first), t))
+            return false;
+        first += c;
+        left -= c;
+    }
+    return true;
+}
+
+template <bool ANY, bool STATS, bool BATCH>
+__global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
+    if (p.gate && *p.gate) return;  // the tree this launch would read is incomplete
+    extern __shared__ int32_t s_stack[];
+    const LdsStack ls{s_stack + threadIdx.x,
+                      reinterpret_cast<float*>(s_stack + p.lds_depth * blockDim.x) + threadIdx.x,
+                      int(p.lds_depth), int(blockDim.x)};
+    DeepStack deep;
+    int sp = 0;
+    uint64_t n_nodes = 0, n_tris = 0;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    uint64_t rid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f,
+          iz = 0.f, tmin = 0.f;
+    float bt = 0.f, bu = 0.f, bv = 0.f;
+    uint32_t bp = RTBVH_MISS;
+    int32_t node = -1;     // node in hand, or -1
+    uint32_t lfirst = 0;   // triangles in hand: [lfirst, lfirst + lcount)
+    int32_t lcount = 0;
+    bool active = false;
+    for (;;) {
+        if (!active) {
+            if (rid >= p.ray_count) break;
+            const float4* r4 = reinterpret_cast<const float4*>(p.rays + rid);
+            const float4 ra = r4[0];
+            const float4 rb = r4[1];
+            ox = ra.x;
+            oy = ra.y;
+            oz = ra.z;
+            tmin = ra.w;
+            dx = rb.x;
+            dy = rb.y;
+            dz = rb.z;
+            bt = rb.w;
+            ix = 1.0f / dx;  // Ray::new, src/ray.rs:166-182
+            iy = 1.0f / dy;
+            iz = 1.0f / dz;
+            bu = 0.f;
+            bv = 0.f;
+            bp = RTBVH_MISS;
+            const bool nan = (ox != ox) || (oy != oy) || (oz != oz) || (dx != dx) ||
+                             (dy != dy) || (dz != dz);
+            if (nan || p.node_count == 0) {
+                if (ANY) {
+                    p.occluded[rid] = 0;
+                } else {
+                    reinterpret_cast<float4*>(p.hits)[rid] =
+                        make_float4(bt, 0.f, 0.f, __uint_as_float(RTBVH_MISS));
+                }
+                rid += stride;
+                continue;
+            }
+            active = true;
+            node = BATCH ? int32_t(scene_of_ray(p.ray_offsets, p.scene_count, uint32_t(rid))) : 0;
+            lcount = 0;
+            sp = 0;
+        }
+        bool overflow = false;
+        bool found = false;
+        // nothing in hand: the next entry that can still hold a hit at or before bt
+        if (node < 0 && lcount == 0) {
+            while (sp > 0) {
+                int32_t e;
+                float t;
+                stack_pop(ls, deep, sp, e, t);
+                if (!(t <= bt)) continue;
+                if (e >= 0) {
+                    node = e;
+                } else {
+                    lfirst = uint32_t(e) & kLeafFirstMask;
+                    lcount = int32_t((uint32_t(e) >> 27) & 15u) + 1;
+                }
+                break;
+            }
+        }
+        // node step
+        if (node >= 0) {
+            const uint32_t cur = uint32_t(node);
+            node = -1;
+            if (cur < p.node_count) {
+                const DevMbvhNode nd = p.mnodes[cur];
+                if (STATS) ++n_nodes;
+                const float mnx[4] = {nd.min_x.x, nd.min_x.y, nd.min_x.z, nd.min_x.w};
+                const float mxx[4] = {nd.max_x.x, nd.max_x.y, nd.max_x.z, nd.max_x.w};
+                const float mny[4] = {nd.min_y.x, nd.min_y.y, nd.min_y.z, nd.min_y.w};
+                const float mxy[4] = {nd.max_y.x, nd.max_y.y, nd.max_y.z, nd.max_y.w};
+                const float mnz[4] = {nd.min_z.x, nd.min_z.y, nd.min_z.z, nd.min_z.w};
+                const float mxz[4] = {nd.max_z.x, nd.max_z.y, nd.max_z.z, nd.max_z.w};
+                const int32_t ch[4] = {nd.children.x, nd.children.y, nd.children.z,
+                                       nd.children.w};
+                const int32_t ct[4] = {nd.counts.x, nd.counts.y, nd.counts.z, nd.counts.w};
+                float tn[4];
+                uint32_t key[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float tx0 = (mnx[i] - ox) * ix, tx1 = (mxx[i] - ox) * ix;
+                    const float ty0 = (mny[i] - oy) * iy, ty1 = (mxy[i] - oy) * iy;
+                    const float tz0 = (mnz[i] - oz) * iz, tz1 = (mxz[i] - oz) * iz;
+                    const float a =
+                        fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+                    const float b =
+                        fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+                    tn[i] = a;
+                    // an empty slot (no child, no triangles) is never entered
+                    const bool used = ct[i] >= 0 || ch[i] >= 0;
+                    const bool h = used && (b >= a) && (a <= bt) && (b >= tmin);
+                    key[i] = h ? ((__float_as_uint(a > 0.0f ? a : 0.0f) & ~3u) | uint32_t(i))
+                               : 0xFFFFFFFFu;
+                }
+                csort(key[0], key[1]);
+                csort(key[2], key[3]);
+                csort(key[0], key[2]);
+                csort(key[1], key[3]);
+                csort(key[1], key[2]);
+                // far to near: each new (nearer) child pushes the one held before
+                bool have = false;
+                int32_t hnode = -1, hcount = -1;
+                uint32_t hfirst = 0;
+                float ht = 0.f;
+#pragma unroll
+                for (int k = 3; k >= 0; --k) {
+                    if (key[k] == 0xFFFFFFFFu) continue;
+                    const uint32_t sl = key[k] & 3u;
+                    const int32_t c = sel4i(sl, ch[0], ch[1], ch[2], ch[3]);
+                    const int32_t n = sel4i(sl, ct[0], ct[1], ct[2], ct[3]);
+                    const float t = sel4f(sl, tn[0], tn[1], tn[2], tn[3]);
+                    if (have && !overflow)
+                        overflow = !push_item(ls, deep, sp, hnode, hfirst, hcount, ht);
+                    have = true;
+                    ht = t;
+                    hcount = n;
+                    hnode = n < 0 ? c : -1;
+                    hfirst = uint32_t(c);
+                }
+                if (have) {
+                    if (hcount < 0)
+                        node = hnode;
+                    else {
+                        lfirst = hfirst;
+                        lcount = hcount;
+                    }
+                }
+            }
+        }
+        // triangle step: one triangle of the leaf in hand
+        if (lcount > 0) {
+            const uint32_t pos = lfirst;
+            ++lfirst;
+            --lcount;
+            if (pos < p.index_count) {
+                if (STATS) ++n_tris;
+                uint32_t prim;
+                const MtResult m = mt_prepared(p.tris, pos, ox, oy, oz, dx, dy, dz, tmin, &prim);
+                if (prim != RTBVH_MISS && m.ok) {
+                    if (ANY) {
+                        if (m.t < bt) {
+                            bp = prim;
+                            found = true;
+                        }
+                    } else if ((m.t < bt) || (m.t == bt && prim < bp)) {
+                        bt = m.t;
+                        bu = m.u;
+                        bv = m.v;
+                        bp = prim;
+                    }
+                }
+            }
+        }
+        if (overflow) atomicOr(p.status, kStatusStackOverflow);
+        if (found || overflow || (node < 0 && lcount == 0 && sp == 0)) {
+            if (ANY) {
+                p.occluded[rid] = (bp != RTBVH_MISS) ? 1 : 0;
+            } else {
+                reinterpret_cast<float4*>(p.hits)[rid] =
+                    make_float4(overflow ? __builtin_nanf("") : bt, bu, bv,
+                                __uint_as_float(overflow ? RTBVH_MISS : bp));
+            }
+            active = false;
+            node = -1;
+            lcount = 0;
+            rid += stride;
+        }
+    }
+    if (STATS) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(p.stats), (unsigned long long)n_nodes);
+        atomicAdd(reinterpret_cast<unsigned long long*>(p.stats) + 1, (unsigned long long)n_tris);
+    }
+}
+
 // Launch shape per device: block size, LDS stack depth and the share of the
 // resident-block capacity used by the persistent grid (percent).
 TraceConfig g_config[64];
@@ -288,9 +522,9 @@ int current_dev() {
     return dev;
 }
 
-template <bool ANY, bool STATS, bool PREP, bool BATCH = false>
+template <bool ANY, bool STATS, bool PREP, bool BATCH = false, bool ITEM = false>
 hipError_t launch_fast(const TraceParams& p0, hipStream_t stream) {
-    auto k = mbvh_fast_kernel<ANY, STATS, PREP, BATCH>;
+    auto k = ITEM ? mbvh_item_kernel<ANY, STATS, BATCH> : mbvh_fast_kernel<ANY, STATS, PREP, BATCH>;
     const int dev = current_dev();
     const TraceConfig cfg = g_config_set[dev] ? g_config[dev] : default_trace_config();
     // resident blocks per CU for this shape, cached per (device, shape)
@@ -322,11 +556,11 @@ hipError_t launch_fast(const TraceParams& p0, hipStream_t stream) {
 
 }  // namespace
 
-TraceConfig default_trace_config() { return TraceConfig{256, 12, 100}; }
+TraceConfig default_trace_config() { return TraceConfig{256, 16, 100, 0}; }
 
 bool valid_trace_config(const TraceConfig& c) {
     return (c.block == 64 || c.block == 128 || c.block == 256) && c.lds_depth <= 32 &&
-           c.occupancy_pct >= 10 && c.occupancy_pct <= 400;
+           c.occupancy_pct >= 10 && c.occupancy_pct <= 400 && c.kernel <= 1;
 }
 
 void set_trace_config(const TraceConfig* c) {
@@ -341,16 +575,26 @@ TraceConfig get_trace_config() {
 }
 
 hipError_t launch_trace_fast(const TraceParams& p, bool any_hit, hipStream_t stream) {
+    // the one-item kernel over leaf-ordered triangles, unless the leaf tags
+    // cannot address every index entry
+    const bool item = p.tris && p.index_count <= kLeafFirstMask && get_trace_config().kernel == 0;
     if (p.ray_offsets) {
         if (p.stats || !p.tris) return hipErrorInvalidValue;
+        if (item)
+            return any_hit ? launch_fast<true, false, true, true, true>(p, stream)
+                           : launch_fast<false, false, true, true, true>(p, stream);
         return any_hit ? launch_fast<true, false, true, true>(p, stream)
                        : launch_fast<false, false, true, true>(p, stream);
     }
     if (p.stats) {
         if (any_hit) return hipErrorInvalidValue;
+        if (item) return launch_fast<false, true, true, false, true>(p, stream);
         return p.tris ? launch_fast<false, true, true>(p, stream)
                       : launch_fast<false, true, false>(p, stream);
     }
+    if (item)
+        return any_hit ? launch_fast<true, false, true, false, true>(p, stream)
+                       : launch_fast<false, false, true, false, true>(p, stream);
     if (p.tris)
         return any_hit ? launch_fast<true, false, true>(p, stream)
                        : launch_fast<false, false, true>(p, stream);

@@ -340,8 +340,8 @@ def batch_rays_torch(lo, hi, counts, device, generator):
         out[b:e, 0:3] = lo_t[sc] + u * (hi_t[sc] - lo_t[sc])
         d = torch.randn((e - b, 3), device=device, generator=generator)
         out[b:e, 4:7] = d / d.norm(dim=1, keepdim=True).clamp_min(1e-12)
-        out[b:e, 3] = RAY_T_MIN
-        out[b:e, 7] = RAY_T_MAX
+        out[b:e, 3] = float(RAY_T_MIN)
+        out[b:e, 7] = float(RAY_T_MAX)
     return out
 
 
@@ -407,7 +407,7 @@ def shadow_rays_torch(verts, n: int, light, device, generator):
         d = lt - o
         dist = d.norm(dim=1, keepdim=True).clamp_min(1e-20)
         out[b:e, 0:3] = o
-        out[b:e, 3] = RAY_T_MIN
+        out[b:e, 3] = float(RAY_T_MIN)
         out[b:e, 4:7] = d / dist
         out[b:e, 7] = (1.0 - 1e-4) * dist[:, 0]
     return out

@@ -180,9 +180,9 @@ def test_trace_brute_force_large():
 
 def test_launch_shapes_and_autotune_keep_results():
     """The traversal's launch shape (block size, LDS stack depth, persistent
-    grid size) changes speed only: every shape gives the same hit records,
-    the autotuner's pick included, with deep stacks spilling from LDS to
-    private memory at depth 0."""
+    grid size, one-item or per-node-leaves kernel) changes speed only: every
+    shape gives the same hit records, the autotuner's pick included, with deep
+    stacks spilling from LDS to private memory at depth 0."""
     print("[launch_shapes]", flush=True)
     verts = scenes.noisy_sphere(100, 80, seed=11)
     aabbs, centers = prep(verts)
@@ -194,10 +194,13 @@ def test_launch_shapes_and_autotune_keep_results():
     default = rt.get_trace_config()
     want = m.intersect_prepared(rays).cpu().numpy()
     try:
-        for block, depth, occ in ((64, 0, 100), (128, 4, 50), (256, 24, 200), (64, 32, 400)):
-            rt.set_trace_config(block, depth, occ)
+        shapes = ((64, 0, 100, 0), (128, 4, 50, 0), (256, 24, 200, 0), (64, 32, 400, 0),
+                  (64, 0, 100, 1), (256, 12, 100, 1), (128, 32, 200, 1))
+        for block, depth, occ, kernel in shapes:
+            rt.set_trace_config(block, depth, occ, kernel)
             got = m.intersect_prepared(rays).cpu().numpy()
-            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (block, depth, occ)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (block, depth, occ,
+                                                                              kernel)
             occ_got = m.occluded_prepared(rays).cpu().numpy()
             assert np.array_equal(occ_got, (want[:, 3].view(np.uint32) != 0xFFFFFFFF).astype(np.uint8))
         with pytest.raises(rt.RTBvhError, match="InvalidArgument"):
