@@ -34,7 +34,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s closest-hit (1M tris, incoherent); PLOC build ms; batched vs plain"
 HBM_PEAK_GBS = 8000.0
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "trace_mbvh_bounce_1m_traffic.json")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "trace_tree_bounce_1m_traffic.json")
 
 
 def log(msg):
@@ -172,6 +172,20 @@ def run_batched(args, dev, rt, scenes, torch):
     return res, p_ms / g_ms
 
 
+def copy_calibration(args, torch, dev):
+    """One plain device copy of known size, so that rocprofv3's FETCH_SIZE /
+    WRITE_SIZE can be scaled to bytes."""
+    if args.copy_calibration <= 0:
+        return
+    nfl = int(args.copy_calibration * (1 << 30)) // 4
+    src = torch.ones(nfl, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {nfl * 4} bytes read and written")
+
+
 def run_large(args, dev, rt, scenes, torch):
     """64 x 64 displaced spheres (19,988,480 triangles) animated per frame:
     each frame moves the vertices, recomputes boxes and centres, rebuilds
@@ -261,6 +275,11 @@ def main():
     ap.add_argument("--large", type=int, default=1,
                     help="1: also run the animated 20M-triangle any-hit frame (0 skips it)")
     ap.add_argument("--large-rays-log2", type=int, default=28)
+    ap.add_argument("--trace-only", action="store_true",
+                    help="build, then only the timed bounce-ray steps of the headline trace "
+                         "(and the copy calibration) for profiler runs; no JSON line")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shape instead of autotuning")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="GiB copied by one plain device copy at the end (scales rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -390,8 +409,18 @@ def main():
 
     # launch shape chosen by the autotuner on a 16M-ray sample of the bounce
     # rays (untimed set-up, like the builds)
-    tuned = rt.autotune(ttree, bounce[: min(n_rays, 1 << 24)])
-    log(f"autotuned launch shape {tuned}")
+    if args.no_autotune:
+        tuned = rt.get_trace_config()
+    else:
+        tuned = rt.autotune(ttree, bounce[: min(n_rays, 1 << 24)])
+    log(f"launch shape {tuned}")
+
+    if args.trace_only:
+        del hits_p
+        _, per, _ = timed(bounce, args.steps, args.warmup)
+        log(f"trace only: {n_rays / (sum(per) / len(per) * 1e-3) / 1e6:.1f} Mrays/s")
+        copy_calibration(args, torch, dev)
+        return
 
     log("timing primary rays")
     _, per_p, _ = timed(primary, 3, 1)
@@ -544,14 +573,7 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)
-    if args.copy_calibration > 0:
-        nfl = int(args.copy_calibration * (1 << 30)) // 4
-        src = torch.ones(nfl, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copy calibration: {nfl * 4} bytes read and written")
+    copy_calibration(args, torch, dev)
 
 
 if __name__ == "__main__":

@@ -279,7 +279,8 @@ void rtbvh_gpu_free_trace_tree(RTTraceTree tree);
 
 /* ---- Launch shape of the lowest-id MBVH traversal ---------------------------
  * (no counterpart in the reference).  Per device; the default is
- * {256, 16, 100, 0}.  Results never depend on it, only speed. */
+ * {64, 8, 200, 0} (the fastest on the benchmark scenes).  Results never depend on
+ * it, only speed. */
 typedef struct RTTraceConfig {
     uint32_t block;          /* threads per block: 64, 128 or 256 */
     uint32_t lds_stack;      /* traversal stack entries per ray kept in LDS, 0..32 */
@@ -328,15 +329,16 @@ typedef struct RTBatch {
     uint32_t id;
     uint32_t scene_count;
     uint32_t bvh_node_count;      /* 2 * index_count - scene_count */
-    uint32_t node_count;          /* MBVH nodes */
+    uint32_t node_count;          /* traversal-tree nodes (0 until triangles are attached) */
     const RTBvhNode *bvh_nodes;   /* device; node s is the root of scene s */
-    const RTMbvhNode *nodes;      /* device; node s is the root of scene s */
+    const RTMbvhNode *nodes;      /* device traversal tree, node s the root of scene s; NULL
+                                     in the handle returned by rtbvh_gpu_create_batch */
     uint32_t index_count;
     uint32_t pad_;
     const uint32_t *indices;      /* device; batch-wide primitive ids in leaf order */
 } RTBatch;
 
-/* Builds (PLOC, radius 14) and collapses every scene of the batch.
+/* Builds (PLOC, radius 14) every scene of the batch.
  * aabbs/centers: device, one per primitive of the batch (aabbs may be
  * NULL); scene_offsets: HOST array of scene_count + 1 entries, starting at 0,
  * every scene at least 3 primitives, scene_count <= RTBVH_MAX_BATCH_SCENES. */
@@ -346,8 +348,8 @@ ResultCode rtbvh_gpu_create_batch(const RTAabb *aabbs, const float *centers, siz
 
 /* Builds the batch's traversal tree (as rtbvh_gpu_create_trace_tree with
  * max_leaf 0, scene-local ids) from vertices: device, 9 floats per primitive
- * of the batch.  Needed before tracing the batch. */
-ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch batch, const float *vertices,
+ * of the batch, and updates *batch.  Needed before tracing the batch. */
+ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch *batch, const float *vertices,
                                             size_t prim_count, void *stream);
 
 /* One persistent launch over the rays of every scene: ray i belongs to the

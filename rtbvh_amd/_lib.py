@@ -113,7 +113,7 @@ SIGNATURES = {
     "rtbvh_gpu_batch_plan_status": (_rc, [RTBatchPlan, _vp]),
     "rtbvh_gpu_batch_plan_free": (None, [RTBatchPlan]),
     "rtbvh_gpu_create_batch": (_rc, [_vp, _vp, _sz, _vp, _sz, _vp, ctypes.POINTER(RTBatch)]),
-    "rtbvh_gpu_batch_attach_triangles": (_rc, [RTBatch, _vp, _sz, _vp]),
+    "rtbvh_gpu_batch_attach_triangles": (_rc, [ctypes.POINTER(RTBatch), _vp, _sz, _vp]),
     "rtbvh_gpu_intersect_batch": (_rc, [RTBatch, _vp, _vp, _sz, _vp, _u32, _vp]),
     "rtbvh_gpu_occluded_batch": (_rc, [RTBatch, _vp, _vp, _sz, _vp, _u32, _vp]),
     "rtbvh_gpu_free_batch": (None, [RTBatch]),

@@ -399,8 +399,8 @@ class SceneBatch:
                          self.device)
 
     def quad_nodes(self) -> torch.Tensor:
-        """MBVH forest as (M, 32) float32 records (copy); node s is the root of
-        scene s."""
+        """Traversal-tree forest as (M, 32) float32 records (copy, after
+        attach_triangles); node s is the root of scene s."""
         return _copy_out(self._h.nodes, self.node_count, MBVH_FLOATS, torch.float32, self.device)
 
     def indices(self) -> torch.Tensor:
@@ -410,8 +410,8 @@ class SceneBatch:
     def attach_triangles(self, vertices: torch.Tensor):
         _require_device(vertices, "vertices")
         verts = vertices.reshape(-1)
-        L.check(L.lib().rtbvh_gpu_batch_attach_triangles(self._h, _ptr(verts), verts.numel() // 9,
-                                                         _stream(verts)),
+        L.check(L.lib().rtbvh_gpu_batch_attach_triangles(ctypes.byref(self._h), _ptr(verts),
+                                                         verts.numel() // 9, _stream(verts)),
                 "rtbvh_gpu_batch_attach_triangles")
 
     @staticmethod
@@ -584,7 +584,7 @@ def get_trace_config() -> dict:
             "kernel": c.kernel}
 
 
-def set_trace_config(block: Optional[int] = None, lds_stack: int = 16, occupancy_pct: int = 100,
+def set_trace_config(block: Optional[int] = None, lds_stack: int = 8, occupancy_pct: int = 200,
                      kernel: int = 0):
     """Sets the traversal launch shape (block=None restores the default)."""
     if block is None:

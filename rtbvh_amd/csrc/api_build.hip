@@ -42,7 +42,6 @@ struct TraceTreeEntry {
 
 struct BatchEntry {
     DeviceBvh bvh;    // forest, roots [0, scenes)
-    DeviceMbvh mbvh;  // forest, roots [0, scenes)
     DeviceMbvh ttree; // traversal tree over the batch (after attach)
     uint32_t* scene_off = nullptr;  // device, scenes + 1 primitive offsets
     uint32_t scenes = 0;
@@ -66,6 +65,7 @@ void release(DeviceBvh& d) {
     if (d.nodes) (void)hipFree(d.nodes);
     if (d.indices) (void)hipFree(d.indices);
     if (d.parents) (void)hipFree(d.parents);
+    if (d.order) (void)hipFree(d.order);
     d = DeviceBvh{};
 }
 
@@ -78,7 +78,6 @@ void release(DeviceMbvh& d) {
 
 void release(BatchEntry& b) {
     release(b.bvh);
-    release(b.mbvh);
     release(b.ttree);
     if (b.scene_off) (void)hipFree(b.scene_off);
     b.scene_off = nullptr;
@@ -242,11 +241,11 @@ RTBatch handle_of(uint32_t id, const BatchEntry& e) {
     h.id = id;
     h.scene_count = e.scenes;
     h.bvh_node_count = e.bvh.node_count;
-    h.node_count = e.mbvh.node_count;
+    h.node_count = e.ttree.node_count;
     h.bvh_nodes = reinterpret_cast<const RTBvhNode*>(e.bvh.nodes);
-    h.nodes = reinterpret_cast<const RTMbvhNode*>(e.mbvh.nodes);
-    h.index_count = e.mbvh.prim_count;
-    h.indices = e.mbvh.indices;
+    h.nodes = reinterpret_cast<const RTMbvhNode*>(e.ttree.nodes);
+    h.index_count = e.bvh.prim_count;
+    h.indices = e.bvh.indices;
     return h;
 }
 
@@ -465,11 +464,6 @@ ResultCode rtbvh_gpu_create_batch(const RTAabb* aabbs, const float* centers, siz
                               &be->bvh);
     if (rc != Ok) return rc;
     be->scenes = scenes;
-    rc = gpu_build_mbvh(be->bvh, st, &be->mbvh);
-    if (rc != Ok) {
-        release(*be);
-        return rc;
-    }
     hipError_t e = hipMalloc(&be->scene_off, size_t(scenes + 1) * 4);
     if (e == hipSuccess)
         e = hipMemcpyAsync(be->scene_off, scene_offsets, size_t(scenes + 1) * 4,
@@ -487,17 +481,17 @@ ResultCode rtbvh_gpu_create_batch(const RTAabb* aabbs, const float* centers, siz
     return Ok;
 }
 
-ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch batch, const float* vertices,
+ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch* batch, const float* vertices,
                                             size_t prim_count, void* stream) {
-    if (vertices == nullptr) {
-        set_error("vertices must not be null");
+    if (vertices == nullptr || batch == nullptr) {
+        set_error("batch and vertices must not be null");
         return Error;
     }
     Registry& r = reg();
     std::unique_lock<std::shared_mutex> lk(r.mu);
-    BatchEntry* be = find_batch(batch.id);
+    BatchEntry* be = find_batch(batch->id);
     if (be == nullptr) {
-        set_error("unknown batch id %u", batch.id);
+        set_error("unknown batch id %u", batch->id);
         return Error;
     }
     if (prim_count != be->bvh.prim_count) {
@@ -517,16 +511,18 @@ ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch batch, const float* vertices
         release(be->ttree);
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
-    ResultCode rc = ensure_parents(be->bvh, st);
-    if (rc != Ok) return rc;
     TraceTreePlan* plan = nullptr;
-    rc = trace_tree_plan_create(be->bvh, kDefaultMaxLeaf, &plan, &be->ttree);
+    ResultCode rc = trace_tree_plan_create(be->bvh, kDefaultMaxLeaf, &plan, &be->ttree);
     if (rc != Ok) return rc;
     rc = trace_tree_build(plan, be->bvh, vertices, uint32_t(prim_count), be->scene_off,
                           be->scenes, &be->ttree, st);
     trace_tree_plan_destroy(plan);
-    if (rc != Ok) release(be->ttree);
-    return rc;
+    if (rc != Ok) {
+        release(be->ttree);
+        return rc;
+    }
+    *batch = handle_of(batch->id, *be);
+    return Ok;
 }
 
 ResultCode rtbvh_gpu_create_trace_tree(RTBvh bvh, const float* vertices, size_t prim_count,
@@ -543,7 +539,7 @@ ResultCode rtbvh_gpu_create_trace_tree(RTBvh bvh, const float* vertices, size_t 
     std::unique_ptr<TraceTreeEntry> te(new TraceTreeEntry());
     {
         Registry& r = reg();
-        std::unique_lock<std::shared_mutex> lk(r.mu);  // may attach parent links
+        std::shared_lock<std::shared_mutex> lk(r.mu);
         BvhEntry* be = find_bvh(bvh.id);
         if (be == nullptr) {
             set_error("unknown bvh id %u", bvh.id);
@@ -556,10 +552,8 @@ ResultCode rtbvh_gpu_create_trace_tree(RTBvh bvh, const float* vertices, size_t 
         }
         te->device = be->device;
         hipStream_t st = static_cast<hipStream_t>(stream);
-        ResultCode rc = ensure_parents(be->dev, st);
-        if (rc != Ok) return rc;
         TraceTreePlan* plan = nullptr;
-        rc = trace_tree_plan_create(be->dev, max_leaf, &plan, &te->t);
+        ResultCode rc = trace_tree_plan_create(be->dev, max_leaf, &plan, &te->t);
         if (rc != Ok) return rc;
         rc = trace_tree_build(plan, be->dev, vertices, uint32_t(prim_count), nullptr, 0, &te->t,
                               st);

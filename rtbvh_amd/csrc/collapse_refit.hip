@@ -151,6 +151,9 @@ __global__ void mfill_kernel(const DevAabb* nodes, uint32_t count, const uint32_
 // Leaves: grow from Aabb::new over their primitives' aabbs in order, then
 // +1e-4; interior: grow by left then right child, then +1e-4; a node whose
 // left_first is negative gets the empty box.  count and left_first are kept.
+// Bottom-up by breadth-first level, deepest level first (the reference walks
+// the node array in reverse, src/bvh.rs:176-205; children always come after
+// their parent, so both orders finish every child before its parent).
 __device__ __forceinline__ DevAabb refit_leaf(const DevAabb& n, const uint32_t* indices,
                                               uint32_t index_count, const DevAabb* aabbs) {
     DevAabb a;
@@ -179,36 +182,54 @@ __device__ __forceinline__ DevAabb refit_leaf(const DevAabb& n, const uint32_t* 
     return a;
 }
 
-__global__ void refit_kernel(DevAabb* nodes, uint32_t count, const int32_t* parent,
-                             const uint32_t* indices, uint32_t index_count, const DevAabb* aabbs,
-                             uint32_t* arrivals) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
+// Breadth-first order, one level per launch: the children of
+// order[begin, end) are appended from *next_end on.
+__global__ void bfs_level_kernel(const DevAabb* nodes, uint32_t count, uint32_t* order,
+                                 uint32_t begin, uint32_t end, uint32_t* next_end) {
+    const uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= end) return;
     DevAabb n;
-    ld(nodes + i, n);
-    // interior nodes are finished by their second arriving child; leaves and
-    // nodes without children (empty box) start the walk up
-    if (n.extra1 < 0 && n.extra2 >= 0) return;
-    st(nodes + i, refit_leaf(n, indices, index_count, aabbs));
-    int32_t p = parent[i];
-    while (p >= 0) {
-        __threadfence();
-        if (atomicAdd(&arrivals[p], 1u) == 0) return;  // the sibling finishes p
-        __threadfence();
-        DevAabb pn, l, r;
-        ld(nodes + p, pn);
-        ld(nodes + pn.extra2, l);
-        ld(nodes + pn.extra2 + 1, r);
-        DevAabb a;
-        for (int k = 0; k < 3; ++k) {
-            a.min[k] = rs_min(rs_min(1e34f, l.min[k]), r.min[k]) - 0.0001f;
-            a.max[k] = rs_max(rs_max(-1e34f, l.max[k]), r.max[k]) + 0.0001f;
+    ld(nodes + order[i], n);
+    if (n.extra1 < 0 && n.extra2 >= 0 && uint32_t(n.extra2) + 1 < count) {
+        const uint32_t pos = atomicAdd(next_end, 2u);
+        if (pos + 1 < count) {
+            order[pos] = uint32_t(n.extra2);
+            order[pos + 1] = uint32_t(n.extra2) + 1;
         }
-        a.extra1 = pn.extra1;
-        a.extra2 = pn.extra2;
-        st(nodes + p, a);
-        p = parent[p];
     }
+}
+
+__global__ void iota_roots_kernel(uint32_t* order, uint32_t roots, uint32_t* end) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < roots) order[i] = i;
+    if (i == 0) *end = roots;
+}
+
+// One depth level of the refit, deepest first: every child is final before
+// its parent's launch, so no thread waits for another.
+__global__ void refit_level_kernel(DevAabb* nodes, const uint32_t* order, uint32_t begin,
+                                   uint32_t end, const uint32_t* indices, uint32_t index_count,
+                                   const DevAabb* aabbs) {
+    const uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= end) return;
+    const uint32_t k = order[i];
+    DevAabb n;
+    ld(nodes + k, n);
+    if (!(n.extra1 < 0 && n.extra2 >= 0)) {
+        st(nodes + k, refit_leaf(n, indices, index_count, aabbs));
+        return;
+    }
+    DevAabb l, r;
+    ld(nodes + n.extra2, l);
+    ld(nodes + n.extra2 + 1, r);
+    DevAabb a;
+    for (int c = 0; c < 3; ++c) {
+        a.min[c] = rs_min(rs_min(1e34f, l.min[c]), r.min[c]) - 0.0001f;
+        a.max[c] = rs_max(rs_max(-1e34f, l.max[c]), r.max[c]) + 0.0001f;
+    }
+    a.extra1 = n.extra1;
+    a.extra2 = n.extra2;
+    st(nodes + k, a);
 }
 
 __global__ void copy_u32_kernel(const uint32_t* src, uint32_t* dst, uint32_t n) {
@@ -381,22 +402,59 @@ hipError_t collapse_plan_enqueue(const CollapsePlanImpl* c, const DeviceBvh& tre
     return hipGetLastError();
 }
 
+// Breadth-first levels of the tree (topology only, so computed once).
+ResultCode ensure_order(DeviceBvh& bvh, hipStream_t stream) {
+    if (bvh.order) return Ok;
+    const uint32_t count = bvh.node_count;
+    uint32_t* word = nullptr;
+    hipError_t e = hipMalloc(&bvh.order, size_t(count) * 4);
+    if (e == hipSuccess) e = hipMalloc(&word, 4);
+    if (e != hipSuccess) {
+        if (bvh.order) (void)hipFree(bvh.order);
+        bvh.order = nullptr;
+        return hip_fail(e, "hipMalloc(order)");
+    }
+    std::vector<uint32_t> off{0};
+    const uint32_t roots = bvh.root_count < count ? bvh.root_count : count;
+    hipLaunchKernelGGL(iota_roots_kernel, dim3((roots + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       stream, bvh.order, roots, word);
+    uint32_t begin = 0, end = roots;
+    off.push_back(end);
+    while (end > begin && off.size() <= count + 1) {
+        const uint32_t g = (end - begin + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(bfs_level_kernel, dim3(g), dim3(kBlock), 0, stream, bvh.nodes, count,
+                           bvh.order, begin, end, word);
+        uint32_t next = 0;
+        if ((e = hipMemcpyAsync(&next, word, 4, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(stream)) != hipSuccess)
+            break;
+        if (next > count) next = count;
+        begin = end;
+        end = next;
+        if (end > begin) off.push_back(end);
+    }
+    (void)hipFree(word);
+    if (e != hipSuccess) {
+        (void)hipFree(bvh.order);
+        bvh.order = nullptr;
+        return hip_fail(e, "breadth-first order");
+    }
+    bvh.level_off = off;
+    return Ok;
+}
+
 ResultCode gpu_refit(DeviceBvh& bvh, const RTAabb* aabbs, hipStream_t stream) {
     if (bvh.node_count == 0) return Ok;
-    ResultCode rc = ensure_parents(bvh, stream);
+    ResultCode rc = ensure_order(bvh, stream);
     if (rc != Ok) return rc;
-    uint32_t* arrivals = nullptr;
-    hipError_t e = hipMalloc(&arrivals, size_t(bvh.node_count) * 4);
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc(refit)");
-    const uint32_t g = (bvh.node_count + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(g), dim3(kBlock), 0, stream,
-                       reinterpret_cast<int32_t*>(arrivals), bvh.node_count, 0);
-    hipLaunchKernelGGL(refit_kernel, dim3(g), dim3(kBlock), 0, stream, bvh.nodes, bvh.node_count,
-                       bvh.parents, bvh.indices, bvh.prim_count,
-                       reinterpret_cast<const DevAabb*>(aabbs), arrivals);
-    e = hipGetLastError();
+    for (size_t L = bvh.level_off.size() - 1; L-- > 0;) {
+        const uint32_t b = bvh.level_off[L], e = bvh.level_off[L + 1];
+        hipLaunchKernelGGL(refit_level_kernel, dim3((e - b + kBlock - 1) / kBlock), dim3(kBlock),
+                           0, stream, bvh.nodes, bvh.order, b, e, bvh.indices, bvh.prim_count,
+                           reinterpret_cast<const DevAabb*>(aabbs));
+    }
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    (void)hipFree(arrivals);
     if (e != hipSuccess) return hip_fail(e, "refit");
     return Ok;
 }

@@ -1,6 +1,8 @@
 // Internal interfaces shared by the translation units of librtbvh_amd.so.
 #pragma once
 
+#include <vector>
+
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -63,9 +65,11 @@ struct DeviceBvh {
     DevAabb* nodes = nullptr;
     uint32_t* indices = nullptr;
     int32_t* parents = nullptr;  // lazily built by ensure_parents()
+    uint32_t* order = nullptr;   // breadth-first node order, lazily built for the refit
     uint32_t node_count = 0;
     uint32_t prim_count = 0;
     uint32_t root_count = 1;
+    std::vector<uint32_t> level_off;  // order[level_off[L], level_off[L+1]) is depth L
 };
 
 struct DeviceMbvh {

@@ -556,7 +556,7 @@ hipError_t launch_fast(const TraceParams& p0, hipStream_t stream) {
 
 }  // namespace
 
-TraceConfig default_trace_config() { return TraceConfig{256, 16, 100, 0}; }
+TraceConfig default_trace_config() { return TraceConfig{64, 8, 200, 0}; }
 
 bool valid_trace_config(const TraceConfig& c) {
     return (c.block == 64 || c.block == 128 || c.block == 256) && c.lds_depth <= 32 &&

@@ -54,34 +54,40 @@ __device__ __forceinline__ float half_area(const DevAabb& a) {
     return (dx + dy) * dz + dx * dy;
 }
 
-__global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-// Primitives below every node: leaves start, the second child to arrive at
-// a parent adds both (the same arrival scheme as the refit).
-__global__ void subtree_counts_kernel(const DevAabb* nodes, uint32_t count, const int32_t* parent,
-                                      uint32_t* cnt, uint32_t* arrivals) {
+// Primitives below every node, capped at cap + 1: a depth-first count of
+// the node's subtree that stops once it exceeds the cap (only "at most
+// max_leaf, and how many" matters).  Each node is counted on its own, so no
+// thread waits for another and no cross-workgroup ordering is needed.
+__global__ void capped_counts_kernel(const DevAabb* nodes, uint32_t count, uint32_t cap,
+                                     uint32_t* cnt) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
-    DevAabb n;
-    ldn(nodes + i, n);
-    if (n.extra1 < 0 && n.extra2 >= 0) return;
-    cnt[i] = n.extra1 > 0 ? uint32_t(n.extra1) : 0u;
-    int32_t p = parent[i];
-    // bounded walk: a tree left incomplete by a failed build may hold cycles
-    for (uint32_t steps = 0; p >= 0 && uint32_t(p) < count && steps < count; ++steps) {
-        __threadfence();
-        if (atomicAdd(&arrivals[p], 1u) == 0) return;
-        __threadfence();
-        DevAabb pn;
-        ldn(nodes + p, pn);
-        const uint32_t l = uint32_t(pn.extra2);
-        if (l + 1 >= count) return;
-        cnt[p] = cnt[l] + cnt[l + 1];
-        p = parent[p];
+    int32_t st[2 * kMaxLeafLimit + 4];
+    int sp = 0;
+    st[sp++] = int32_t(i);
+    uint32_t sum = 0;
+    uint32_t steps = 0;
+    const uint32_t max_steps = 4 * cap + 8;  // bounds a chain of empty leaves too
+    while (sp > 0 && sum <= cap) {
+        const int32_t k = st[--sp];
+        if (k < 0 || uint32_t(k) >= count || ++steps > max_steps) {
+            sum = cap + 1;
+            break;
+        }
+        DevAabb n;
+        ldn(nodes + k, n);
+        if (n.extra1 >= 0) {
+            sum += uint32_t(n.extra1);
+        } else if (n.extra2 >= 0) {
+            if (sp + 2 > 2 * kMaxLeafLimit + 4) {
+                sum = cap + 1;
+                break;
+            }
+            st[sp++] = n.extra2 + 1;
+            st[sp++] = n.extra2;
+        }
     }
+    cnt[i] = sum <= cap ? sum : cap + 1;
 }
 
 __device__ __forceinline__ void write_tri(float4* tris, uint32_t pos, const float* verts,
@@ -308,9 +314,6 @@ __global__ void finish_kernel(const uint32_t* last_q, uint32_t* counters, uint32
 struct TraceTreePlan {
     uint32_t count = 0, roots = 1, n = 0, wcap = 0, max_leaf = 1, levels = 0;
     uint32_t* cnt = nullptr;
-    uint32_t* arrivals = nullptr;
-    int32_t* parents = nullptr;  // owned when the tree had none
-    bool own_parents = false;
     QItem* q[2] = {nullptr, nullptr};
     uint32_t* words = nullptr;  // [0..1] queue counts, [2..4] counters
 };
@@ -318,8 +321,6 @@ struct TraceTreePlan {
 void trace_tree_plan_destroy(TraceTreePlan* t) {
     if (!t) return;
     if (t->cnt) (void)hipFree(t->cnt);
-    if (t->arrivals) (void)hipFree(t->arrivals);
-    if (t->own_parents && t->parents) (void)hipFree(t->parents);
     if (t->q[0]) (void)hipFree(t->q[0]);
     if (t->q[1]) (void)hipFree(t->q[1]);
     if (t->words) (void)hipFree(t->words);
@@ -349,19 +350,12 @@ LevelArgs level_args(const TraceTreePlan* t, const DeviceBvh& tree, const float*
     return a;
 }
 
-// Parent links, subtree counts and the root queue.
+// Capped subtree counts and the root queue.
 hipError_t enqueue_setup(const TraceTreePlan* t, const DeviceBvh& tree, hipStream_t stream) {
     const uint32_t count = t->count;
     const uint32_t g = (count + kBlock - 1) / kBlock;
-    if (t->own_parents) {
-        hipLaunchKernelGGL(fill_u32_kernel, dim3(g), dim3(kBlock), 0, stream,
-                           reinterpret_cast<uint32_t*>(t->parents), count, 0xFFFFFFFFu);
-        hipError_t e = launch_parent_links(tree.nodes, count, t->parents, stream);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(g), dim3(kBlock), 0, stream, t->arrivals, count, 0u);
-    hipLaunchKernelGGL(subtree_counts_kernel, dim3(g), dim3(kBlock), 0, stream, tree.nodes, count,
-                       t->parents, t->cnt, t->arrivals);
+    hipLaunchKernelGGL(capped_counts_kernel, dim3(g), dim3(kBlock), 0, stream, tree.nodes, count,
+                       t->max_leaf, t->cnt);
     const uint32_t gr = (t->roots + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(roots_kernel, dim3(gr), dim3(kBlock), 0, stream, t->roots, t->q[0],
                        t->words + 0, t->words + 2);
@@ -392,21 +386,11 @@ ResultCode trace_tree_plan_create(const DeviceBvh& tree, uint32_t max_leaf,
     t->wcap = tree.prim_count + tree.root_count;
     hipError_t e;
     if ((e = hipMalloc(&t->cnt, size_t(t->count) * 4)) != hipSuccess ||
-        (e = hipMalloc(&t->arrivals, size_t(t->count) * 4)) != hipSuccess ||
         (e = hipMalloc(&t->q[0], size_t(t->wcap) * sizeof(QItem))) != hipSuccess ||
         (e = hipMalloc(&t->q[1], size_t(t->wcap) * sizeof(QItem))) != hipSuccess ||
         (e = hipMalloc(&t->words, 8 * 4)) != hipSuccess) {
         trace_tree_plan_destroy(t);
         return hip_fail(e, "trace tree workspace");
-    }
-    if (tree.parents) {
-        t->parents = tree.parents;
-    } else {
-        if ((e = hipMalloc(&t->parents, size_t(t->count) * 4)) != hipSuccess) {
-            trace_tree_plan_destroy(t);
-            return hip_fail(e, "trace tree parents");
-        }
-        t->own_parents = true;
     }
     DeviceMbvh res;
     if ((e = hipMalloc(&res.nodes, size_t(t->wcap) * sizeof(DevMbvhNode))) != hipSuccess ||
