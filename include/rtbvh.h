@@ -247,6 +247,22 @@ ResultCode rtbvh_gpu_intersect_mbvh_prepared(RTMbvh mbvh, const RTRay *rays, siz
 ResultCode rtbvh_gpu_occluded_mbvh_prepared(RTMbvh mbvh, const RTRay *rays, size_t ray_count,
                                             uint8_t *occluded, uint32_t flags, void *stream);
 
+/* ---- Ray packets ------------------------------------------------------------
+ * RayPacket4 (src/ray.rs:44-59): four rays as ten 4-wide vectors.  On the GPU
+ * a wavefront is the packet, so packets are unpacked into ray records and
+ * traced like any other rays; hit i belongs to packet i / 4, lane i % 4. */
+typedef struct RTRayPacket4 {
+    float origin_x[4], origin_y[4], origin_z[4];
+    float direction_x[4], direction_y[4], direction_z[4];
+    float inv_direction_x[4], inv_direction_y[4], inv_direction_z[4];
+    float t[4];
+} RTRayPacket4;
+
+/* rays[4 p + i] = RayPacket4::ray(i) of packets[p] (src/ray.rs:118-146): origin,
+ * direction, t_min 1e-4, t_max = t[i].  packets, rays: device, 16-byte aligned. */
+ResultCode rtbvh_gpu_rays_from_packets(const RTRayPacket4 *packets, size_t packet_count,
+                                       RTRay *rays, void *stream);
+
 /* ---- Traversal trees --------------------------------------------------------
  * A 4-wide tree built from a binary tree for the lowest-(t, id) traversal
  * only (no counterpart in the reference): each node opens the child with the
@@ -279,20 +295,23 @@ void rtbvh_gpu_free_trace_tree(RTTraceTree tree);
 
 /* ---- Launch shape of the lowest-id MBVH traversal ---------------------------
  * (no counterpart in the reference).  Per device; the default is
- * {64, 8, 200, 0} (the fastest on the benchmark scenes).  Results never depend on
+ * {64, 8, 200, 0, 1}.  Results never depend on
  * it, only speed. */
 typedef struct RTTraceConfig {
     uint32_t block;          /* threads per block: 64, 128 or 256 */
     uint32_t lds_stack;      /* traversal stack entries per ray kept in LDS, 0..32 */
-    uint32_t occupancy_pct;  /* persistent grid as a share of resident blocks, 10..400 */
+    uint32_t occupancy_pct;  /* persistent grid as a share of resident blocks, 10..3200 */
     uint32_t kernel;         /* with leaf-ordered triangles: 0 one node or one triangle per
                                 ray per iteration, 1 a node and all its leaf slots */
+    uint32_t refill;         /* kernel 0: idle lanes of a wavefront that take new rays
+                                together, 1..64 */
 } RTTraceConfig;
 
 ResultCode rtbvh_gpu_set_trace_config(const RTTraceConfig *config); /* NULL: default */
 ResultCode rtbvh_gpu_get_trace_config(RTTraceConfig *config);
-/* Autotuner: times 60 launch shapes (kernel 0/1 x block 64/128/256 x LDS
- * stack 4..24 x occupancy 100/200 %) of the closest-hit traversal of `rays` against
+/* Autotuner: times 120 launch shapes (block 64/128/256 x LDS stack 4..16 x
+ * occupancy 100/200 % x {kernel 0 with refill 1/8/16/32, kernel 1}) of the
+ * closest-hit traversal of `rays` against
  * `mbvh`'s attached triangles, writing `hits` (device, ray_count records),
  * and makes the fastest the current device's shape (also returned in *best).
  * Synchronises the stream. */

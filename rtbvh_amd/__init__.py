@@ -21,8 +21,9 @@ in librtbvh_amd.so through the C ABI of include/rtbvh.h.
 from . import _lib
 from ._lib import RTBvhError
 from .api import (BatchPlan, Bvh, BuildError, Builder, Mbvh, SceneBatch, TraceTree, RAY_FLOATS, HIT_FLOATS,
-                  autotune, get_trace_config, hits_prim, rays_tensor, set_trace_config)
+                  autotune, get_trace_config, hits_prim, rays_from_packets, rays_tensor,
+                  set_trace_config)
 
 __all__ = ["Builder", "Bvh", "Mbvh", "SceneBatch", "BatchPlan", "TraceTree", "BuildError", "RAY_FLOATS", "HIT_FLOATS", "hits_prim",
-           "rays_tensor", "RTBvhError", "autotune", "get_trace_config",
+           "rays_tensor", "rays_from_packets", "RTBvhError", "autotune", "get_trace_config",
            "set_trace_config", "_lib"]

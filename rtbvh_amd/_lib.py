@@ -60,7 +60,8 @@ class RTBatchPlan(ctypes.Structure):
 
 class RTTraceConfig(ctypes.Structure):
     _fields_ = [("block", ctypes.c_uint32), ("lds_stack", ctypes.c_uint32),
-                ("occupancy_pct", ctypes.c_uint32), ("kernel", ctypes.c_uint32)]
+                ("occupancy_pct", ctypes.c_uint32), ("kernel", ctypes.c_uint32),
+                ("refill", ctypes.c_uint32)]
 
 
 class RTTraceTree(ctypes.Structure):
@@ -98,6 +99,7 @@ SIGNATURES = {
     "rtbvh_gpu_trace_tree_intersect_stats": (_rc, [RTTraceTree, _vp, _sz, _vp, _vp, _vp]),
     "rtbvh_gpu_trace_tree_occluded": (_rc, [RTTraceTree, _vp, _sz, _vp, _u32, _vp]),
     "rtbvh_gpu_free_trace_tree": (None, [RTTraceTree]),
+    "rtbvh_gpu_rays_from_packets": (_rc, [_vp, _sz, _vp, _vp]),
     "rtbvh_gpu_set_trace_config": (_rc, [ctypes.POINTER(RTTraceConfig)]),
     "rtbvh_gpu_get_trace_config": (_rc, [ctypes.POINTER(RTTraceConfig)]),
     "rtbvh_gpu_autotune_mbvh": (_rc, [RTMbvh, _vp, _sz, _vp, _vp, ctypes.POINTER(RTTraceConfig)]),

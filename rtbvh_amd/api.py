@@ -84,6 +84,20 @@ def rays_tensor(origins: torch.Tensor, directions: torch.Tensor, t_min: float = 
     return r
 
 
+def rays_from_packets(packets: torch.Tensor) -> torch.Tensor:
+    """(P, 40) float32 RayPacket4 records (src/ray.rs:44-59) -> (4P, 8) ray
+    records on the device; ray 4p + i is lane i of packet p."""
+    _require_device(packets, "packets")
+    if packets.dim() != 2 or packets.shape[1] != 40:
+        raise ValueError("packets must be (P, 40) float32 RayPacket4 records")
+    rays = torch.empty((packets.shape[0] * 4, RAY_FLOATS), dtype=torch.float32,
+                       device=packets.device)
+    L.check(L.lib().rtbvh_gpu_rays_from_packets(_ptr(packets), packets.shape[0], _ptr(rays),
+                                                _stream(packets)),
+            "rtbvh_gpu_rays_from_packets")
+    return rays
+
+
 def hits_prim(hits: torch.Tensor) -> torch.Tensor:
     """Primitive ids of (R, 4) hit records as int64 (-1 on a miss)."""
     p = hits[:, 3].contiguous().view(torch.int32).to(torch.int64)
@@ -581,16 +595,16 @@ def get_trace_config() -> dict:
     c = L.RTTraceConfig()
     L.check(L.lib().rtbvh_gpu_get_trace_config(ctypes.byref(c)), "rtbvh_gpu_get_trace_config")
     return {"block": c.block, "lds_stack": c.lds_stack, "occupancy_pct": c.occupancy_pct,
-            "kernel": c.kernel}
+            "kernel": c.kernel, "refill": c.refill}
 
 
 def set_trace_config(block: Optional[int] = None, lds_stack: int = 8, occupancy_pct: int = 200,
-                     kernel: int = 0):
+                     kernel: int = 0, refill: int = 1):
     """Sets the traversal launch shape (block=None restores the default)."""
     if block is None:
         L.check(L.lib().rtbvh_gpu_set_trace_config(None), "rtbvh_gpu_set_trace_config")
         return
-    c = L.RTTraceConfig(block, lds_stack, occupancy_pct, kernel)
+    c = L.RTTraceConfig(block, lds_stack, occupancy_pct, kernel, refill)
     L.check(L.lib().rtbvh_gpu_set_trace_config(ctypes.byref(c)), "rtbvh_gpu_set_trace_config")
 
 
@@ -606,4 +620,4 @@ def autotune(tree, rays: torch.Tensor) -> dict:
     L.check(fn(tree._h, _ptr(rays), rays.shape[0], _ptr(hits), _stream(rays), ctypes.byref(c)),
             fn.__name__)
     return {"block": c.block, "lds_stack": c.lds_stack, "occupancy_pct": c.occupancy_pct,
-            "kernel": c.kernel}
+            "kernel": c.kernel, "refill": c.refill}

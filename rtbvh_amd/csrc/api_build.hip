@@ -293,13 +293,14 @@ ResultCode collapse_entry(uint32_t id, void* stream, bool host, RTMbvh* out) {
 
 // Attached leaf-ordered triangles of an MBVH made by this library (null if
 // none); *count receives the number of index entries they cover.
-const float4* attached_triangles(uint32_t id, uint32_t* count, int* device) {
+const float4* attached_triangles(uint32_t id, uint32_t* count, int* device, bool* small) {
     Registry& r = reg();
     std::shared_lock<std::shared_mutex> lk(r.mu);
     MbvhEntry* me = find_mbvh(id);
     if (me == nullptr || me->dev.tris == nullptr) return nullptr;
     *count = me->dev.prim_count;
     *device = me->device;
+    *small = me->dev.max_leaf <= 16;
     return me->dev.tris;
 }
 
@@ -428,6 +429,9 @@ ResultCode rtbvh_gpu_mbvh_attach_triangles(RTMbvh mbvh, const float* vertices, s
                                             uint32_t(prim_count), me->dev.tris,
                                             static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "prepare triangles");
+    e = mbvh_max_leaf(me->dev.nodes, me->dev.node_count, &me->dev.max_leaf,
+                      static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "largest leaf");
     return Ok;
 }
 

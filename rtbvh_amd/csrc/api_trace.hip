@@ -3,6 +3,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <vector>
+
 #include "internal.hpp"
 
 namespace rtbvh {
@@ -74,7 +76,7 @@ ResultCode trace_entry(const void* nodes, uint32_t node_count, const uint32_t* i
                        uint32_t index_count, const float* vertices, size_t prim_count,
                        const RTRay* rays, size_t ray_count, RTHit* hits, uint8_t* occluded,
                        uint32_t flags, void* stream, bool mbvh, uint64_t* stats = nullptr,
-                       const float4* tris = nullptr) {
+                       const float4* tris = nullptr, bool small_leaves = false) {
     if (flags & ~RTBVH_TRACE_REFERENCE_ORDER) {
         set_error("unknown trace flags 0x%x", flags);
         return InvalidArgument;
@@ -113,6 +115,7 @@ ResultCode trace_entry(const void* nodes, uint32_t node_count, const uint32_t* i
     p.occluded = occluded;
     p.stats = stats;
     p.tris = tris;
+    p.small_leaves = small_leaves;
     p.status = device_status_ptr();
     if (p.status == nullptr) {
         set_error("device status word unavailable");
@@ -128,7 +131,8 @@ ResultCode prepared_entry(RTMbvh mbvh, const RTRay* rays, size_t ray_count, RTHi
                           uint8_t* occluded, uint32_t flags, void* stream) {
     uint32_t count = 0;
     int device = 0;
-    const float4* tris = attached_triangles(mbvh.id, &count, &device);
+    bool small = false;
+    const float4* tris = attached_triangles(mbvh.id, &count, &device, &small);
     if (tris == nullptr) {
         set_error("mbvh %u has no attached triangles (rtbvh_gpu_mbvh_attach_triangles)", mbvh.id);
         return Error;
@@ -143,7 +147,8 @@ ResultCode prepared_entry(RTMbvh mbvh, const RTRay* rays, size_t ray_count, RTHi
         return InvalidArgument;
     }
     return trace_entry(mbvh.nodes, mbvh.node_count, mbvh.indices, mbvh.index_count, nullptr,
-                       count, rays, ray_count, hits, occluded, flags, stream, true, nullptr, tris);
+                       count, rays, ray_count, hits, occluded, flags, stream, true, nullptr, tris,
+                       small);
 }
 
 // Lowest-id traversal of a traversal tree or a batch's tree (leaf-ordered
@@ -186,6 +191,7 @@ ResultCode view_entry(const BatchView& v, const char* what, uint32_t id, const R
     }
     TraceParams p{};
     p.stats = stats;
+    p.small_leaves = true;  // traversal-tree leaves hold at most 8 triangles
     p.mnodes = v.nodes;
     p.node_count = v.node_count;
     p.index_count = v.index_count;
@@ -233,6 +239,21 @@ ResultCode tree_entry(RTTraceTree tree, const RTRay* rays, size_t ray_count, RTH
     v.scenes = 1;
     return view_entry(v, "trace tree", tree.id, rays, nullptr, ray_count, hits, occluded, flags,
                       stream, stats);
+}
+
+// RayPacket4 (src/ray.rs:44-59): ten Vec4 — origin x/y/z, direction x/y/z,
+// inverse direction x/y/z, t — for four rays; RayPacket4::ray (src/ray.rs:
+// 118-146) gives lane i the ray (origin_i, direction_i) with t_min 1e-4 and
+// t_i.
+__global__ void packets_to_rays_kernel(const float4* packets, uint64_t packet_count,
+                                       float4* rays) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;  // one ray per thread
+    if (i >= packet_count * 4) return;
+    const uint64_t pk = i >> 2;
+    const uint32_t lane = uint32_t(i & 3u);
+    const float* q = reinterpret_cast<const float*>(packets + pk * 10);
+    rays[i * 2] = make_float4(q[0 + lane], q[4 + lane], q[8 + lane], 1e-4f);
+    rays[i * 2 + 1] = make_float4(q[12 + lane], q[16 + lane], q[20 + lane], q[36 + lane]);
 }
 
 }  // namespace
@@ -402,15 +423,43 @@ ResultCode rtbvh_gpu_occluded_mbvh(RTMbvh mbvh, const float* vertices, size_t pr
                        prim_count, rays, ray_count, nullptr, occluded, flags, stream, true);
 }
 
+ResultCode rtbvh_gpu_rays_from_packets(const RTRayPacket4* packets, size_t packet_count,
+                                       RTRay* rays, void* stream) {
+    if (packet_count == 0) return Ok;
+    if (packets == nullptr || rays == nullptr) {
+        set_error("packets and rays must not be null");
+        return Error;
+    }
+    ResultCode rc;
+    if ((rc = check_device(packets, "packets")) != Ok || (rc = check_device(rays, "rays")) != Ok)
+        return rc;
+    if ((reinterpret_cast<uintptr_t>(packets) & 15) || (reinterpret_cast<uintptr_t>(rays) & 15)) {
+        set_error("packets and rays must be 16-byte aligned");
+        return InvalidArgument;
+    }
+    if (packet_count > (uint64_t(0x7FFFFFFF) * 64)) {
+        set_error("too many packets (%zu)", packet_count);
+        return InvalidArgument;
+    }
+    const uint64_t n = uint64_t(packet_count) * 4;
+    hipLaunchKernelGGL(packets_to_rays_kernel, dim3(uint32_t((n + 255) / 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), reinterpret_cast<const float4*>(packets),
+                       uint64_t(packet_count), reinterpret_cast<float4*>(rays));
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "packets to rays");
+    return Ok;
+}
+
 ResultCode rtbvh_gpu_set_trace_config(const RTTraceConfig* config) {
     if (config == nullptr) {
         set_trace_config(nullptr);
         return Ok;
     }
-    const TraceConfig c{config->block, config->lds_stack, config->occupancy_pct, config->kernel};
+    const TraceConfig c{config->block, config->lds_stack, config->occupancy_pct, config->kernel,
+                        config->refill};
     if (!valid_trace_config(c)) {
-        set_error("trace config: block 64/128/256, lds_stack <= 32, occupancy_pct 10..400, "
-                  "kernel 0/1");
+        set_error("trace config: block 64/128/256, lds_stack <= 32, occupancy_pct 10..3200, "
+                  "kernel 0/1, refill 1..64");
         return InvalidArgument;
     }
     set_trace_config(&c);
@@ -427,6 +476,7 @@ ResultCode rtbvh_gpu_get_trace_config(RTTraceConfig* config) {
     config->lds_stack = c.lds_depth;
     config->occupancy_pct = c.occupancy_pct;
     config->kernel = c.kernel;
+    config->refill = c.refill;
     return Ok;
 }
 
@@ -454,42 +504,37 @@ ResultCode autotune(const RTRay* rays, size_t ray_count, RTHit* hits, void* stre
         return hip_fail(e, "autotune events");
     }
     const TraceConfig before = get_trace_config();
-    static const uint32_t kernels[] = {0, 1};
-    static const uint32_t blocks[] = {64, 128, 256};
-    static const uint32_t depths[] = {4, 8, 12, 16, 24};
-    static const uint32_t occ[] = {100, 200};
+    // one-item kernel: block x LDS depth x grid x refill; per-node-leaves
+    // kernel: block x LDS depth x grid (refill does not apply)
+    std::vector<TraceConfig> shapes;
+    for (uint32_t b : {64u, 128u, 256u})
+        for (uint32_t d : {4u, 8u, 12u, 16u})
+            for (uint32_t o : {100u, 200u}) {
+                for (uint32_t rf : {1u, 8u, 16u, 32u}) shapes.push_back(TraceConfig{b, d, o, 0, rf});
+                shapes.push_back(TraceConfig{b, d, o, 1, 1});
+            }
     TraceConfig top = before;
     float top_ms = 3.4e38f;
     ResultCode rc = Ok;
-    for (uint32_t kn : kernels) {
-    for (uint32_t b : blocks) {
-        for (uint32_t d : depths) {
-            for (uint32_t o : occ) {
-                const TraceConfig c{b, d, o, kn};
-                set_trace_config(&c);
-                float ms_best = 3.4e38f;
-                for (int rep = 0; rep < 3 && rc == Ok; ++rep) {
-                    if ((e = hipEventRecord(e0, st)) != hipSuccess) break;
-                    rc = trace();
-                    if (rc != Ok) break;
-                    if ((e = hipEventRecord(e1, st)) != hipSuccess ||
-                        (e = hipEventSynchronize(e1)) != hipSuccess)
-                        break;
-                    float ms = 0.f;
-                    if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
-                    if (rep > 0 && ms < ms_best) ms_best = ms;  // first run warms up
-                }
-                if (rc != Ok || e != hipSuccess) break;
-                if (ms_best < top_ms) {
-                    top_ms = ms_best;
-                    top = c;
-                }
-            }
-            if (rc != Ok || e != hipSuccess) break;
+    for (const TraceConfig& c : shapes) {
+        set_trace_config(&c);
+        float ms_best = 3.4e38f;
+        for (int rep = 0; rep < 3 && rc == Ok; ++rep) {
+            if ((e = hipEventRecord(e0, st)) != hipSuccess) break;
+            rc = trace();
+            if (rc != Ok) break;
+            if ((e = hipEventRecord(e1, st)) != hipSuccess ||
+                (e = hipEventSynchronize(e1)) != hipSuccess)
+                break;
+            float ms = 0.f;
+            if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+            if (rep > 0 && ms < ms_best) ms_best = ms;  // first run warms up
         }
         if (rc != Ok || e != hipSuccess) break;
-    }
-        if (rc != Ok || e != hipSuccess) break;
+        if (ms_best < top_ms) {
+            top_ms = ms_best;
+            top = c;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -503,6 +548,7 @@ ResultCode autotune(const RTRay* rays, size_t ray_count, RTHit* hits, void* stre
         best->lds_stack = top.lds_depth;
         best->occupancy_pct = top.occupancy_pct;
         best->kernel = top.kernel;
+        best->refill = top.refill;
     }
     return Ok;
 }

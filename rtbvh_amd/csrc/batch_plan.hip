@@ -153,6 +153,7 @@ hipError_t enqueue_all(PlanEntry& pe, const RTAabb* aabbs, const float* centers,
     p.ray_offsets = pe.scenes > 1 ? ray_offsets : nullptr;
     p.scene_count = pe.scenes;
     p.gate = fail;
+    p.small_leaves = true;  // traversal-tree leaves
     p.status = device_status_ptr();
     if (p.status == nullptr) return hipErrorInvalidValue;
     return launch_trace_fast(p, occluded != nullptr, stream);

@@ -34,15 +34,19 @@ struct TraceParams {
     uint32_t scene_count;
     const uint32_t* gate;  // non-null: the launch does nothing while *gate != 0
     uint32_t lds_depth;    // set by the launcher: stack entries per lane held in LDS
+    uint32_t refill;       // set by the launcher: idle lanes that trigger taking new rays
+    bool small_leaves;     // every leaf holds at most 16 triangles (one-item kernel)
 };
 
 // Launch shape of the lowest-id MBVH traversal (per device).
 struct TraceConfig {
     uint32_t block;          // threads per block: 64, 128 or 256
     uint32_t lds_depth;      // stack entries per lane in LDS (0..32)
-    uint32_t occupancy_pct;  // persistent grid as a share of resident blocks (10..400)
+    uint32_t occupancy_pct;  // persistent grid as a share of resident blocks (10..3200)
     uint32_t kernel;         // with leaf-ordered triangles: 0 one item per iteration,
                              // 1 a node and all its leaf slots per iteration
+    uint32_t refill;         // one-item kernel: idle lanes of a wave that take new rays
+                             // together (1..64)
 };
 TraceConfig default_trace_config();
 bool valid_trace_config(const TraceConfig& c);
@@ -76,6 +80,7 @@ struct DeviceMbvh {
     DevMbvhNode* nodes = nullptr;
     uint32_t* indices = nullptr;
     float4* tris = nullptr;  // attached leaf-ordered triangles (optional)
+    uint32_t max_leaf = 0;   // largest leaf, measured when triangles are attached
     uint32_t node_count = 0;
     uint32_t prim_count = 0;
     uint32_t root_count = 1;  // MBVH node s is the root of scene s
@@ -156,7 +161,11 @@ hipError_t launch_adjoint(const float* verts, uint32_t prim_count, const DevRay*
                           float* grad_verts, float* grad_origin, float* grad_dir,
                           hipStream_t stream);
 
-const float4* attached_triangles(uint32_t id, uint32_t* count, int* device);
+hipError_t mbvh_max_leaf(const DevMbvhNode* nodes, uint32_t count, uint32_t* result,
+                         hipStream_t stream);
+
+// small: every leaf of the MBVH holds at most 16 triangles.
+const float4* attached_triangles(uint32_t id, uint32_t* count, int* device, bool* small);
 
 struct BatchView {
     const DevMbvhNode* nodes;

@@ -288,11 +288,16 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_fast_kernel(TraceParams p) {
 // hand and pushes the others, far to near, with their entry distance; the
 // triangle step tests one triangle of the leaf in hand.  Leaves travel on
 // the stack as tagged entries: bit 31 set, bits 27..30 count - 1, bits 0..26
-// the first triangle, so this kernel needs fewer than 2^27 index entries.
-// Culling and the lowest-(t, id) rule are those of the kernel above.
+// the first triangle, so this kernel needs fewer than 2^27 index entries and
+// leaves of at most 16 triangles (the launcher checks both).
+//
+// Lanes whose ray is done take new rays together, once at least p.refill of
+// them are idle (or no lane has work left): loading a ray and its three IEEE
+// reciprocals then runs for many lanes at once instead of on almost every
+// iteration for one.  Culling and the lowest-(t, id) rule are those of the
+// kernel above.
 constexpr uint32_t kLeafFlag = 0x80000000u;
 constexpr uint32_t kLeafFirstMask = (1u << 27) - 1u;
-constexpr uint32_t kLeafChunk = 16;
 
 __device__ __forceinline__ int32_t sel4i(uint32_t s, int32_t a, int32_t b, int32_t c, int32_t d) {
     const int32_t lo = (s & 1u) ? b : a;
@@ -306,20 +311,9 @@ __device__ __forceinline__ float sel4f(uint32_t s, float a, float b, float c, fl
     return (s & 2u) ? hi : lo;
 }
 
-// Pushes a node (count < 0) or a leaf range, split into entries of at most
-// kLeafChunk triangles.
-__device__ __forceinline__ bool push_item(const LdsStack& ls, DeepStack& deep, int& sp,
-                                          int32_t node, uint32_t first, int32_t count, float t) {
-    if (count < 0) return stack_push(ls, deep, sp, node, t);
-    uint32_t left = uint32_t(count);
-    while (left > 0) {
-        const uint32_t c = left < kLeafChunk ? left : kLeafChunk;
-        if (!stack_push(ls, deep, sp, int32_t(kLeafFlag | ((c - 1u) << 27) | first), t))
-            return false;
-        first += c;
-        left -= c;
-    }
-    return true;
+// Stack entry of slot s: the child node, or the tagged leaf.
+__device__ __forceinline__ int32_t slot_entry(int32_t c, int32_t n) {
+    return n < 0 ? c : int32_t(kLeafFlag | (uint32_t(n - 1) << 27) | uint32_t(c));
 }
 
 template <bool ANY, bool STATS, bool BATCH>
@@ -334,6 +328,7 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
     uint64_t n_nodes = 0, n_tris = 0;
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     uint64_t rid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint32_t refill = p.refill ? p.refill : 1u;
     float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f,
           iz = 0.f, tmin = 0.f;
     float bt = 0.f, bu = 0.f, bv = 0.f;
@@ -343,8 +338,12 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
     int32_t lcount = 0;
     bool active = false;
     for (;;) {
-        if (!active) {
-            if (rid >= p.ray_count) break;
+        // ---- refill: idle lanes that still have rays take them together
+        const bool can_take = !active && rid < p.ray_count;
+        const uint64_t takers = __ballot(can_take);
+        const uint64_t busy = __ballot(active);
+        if (takers == 0 && busy == 0) break;  // wave-uniform: no ray left for any lane
+        if (takers != 0 && (uint32_t(__popcll(takers)) >= refill || busy == 0) && can_take) {
             const float4* r4 = reinterpret_cast<const float4*>(p.rays + rid);
             const float4 ra = r4[0];
             const float4 rb = r4[1];
@@ -372,16 +371,19 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
                         make_float4(bt, 0.f, 0.f, __uint_as_float(RTBVH_MISS));
                 }
                 rid += stride;
-                continue;
+            } else {
+                active = true;
+                node = BATCH ? int32_t(scene_of_ray(p.ray_offsets, p.scene_count,
+                                                    uint32_t(rid)))
+                             : 0;
+                lcount = 0;
+                sp = 0;
             }
-            active = true;
-            node = BATCH ? int32_t(scene_of_ray(p.ray_offsets, p.scene_count, uint32_t(rid))) : 0;
-            lcount = 0;
-            sp = 0;
         }
+        if (!active) continue;
         bool overflow = false;
         bool found = false;
-        // nothing in hand: the next entry that can still hold a hit at or before bt
+        // ---- nothing in hand: the next entry that can still hold a hit at or before bt
         if (node < 0 && lcount == 0) {
             while (sp > 0) {
                 int32_t e;
@@ -397,7 +399,7 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
                 break;
             }
         }
-        // node step
+        // ---- node step
         if (node >= 0) {
             const uint32_t cur = uint32_t(node);
             node = -1;
@@ -425,8 +427,8 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
                     const float b =
                         fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
                     tn[i] = a;
-                    // an empty slot (no child, no triangles) is never entered
-                    const bool used = ct[i] >= 0 || ch[i] >= 0;
+                    // a slot holds an interior child or a non-empty leaf
+                    const bool used = (ct[i] < 0 && ch[i] >= 0) || ct[i] > 0;
                     const bool h = used && (b >= a) && (a <= bt) && (b >= tmin);
                     key[i] = h ? ((__float_as_uint(a > 0.0f ? a : 0.0f) & ~3u) | uint32_t(i))
                                : 0xFFFFFFFFu;
@@ -436,37 +438,59 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
                 csort(key[0], key[2]);
                 csort(key[1], key[3]);
                 csort(key[1], key[2]);
-                // far to near: each new (nearer) child pushes the one held before
-                bool have = false;
-                int32_t hnode = -1, hcount = -1;
-                uint32_t hfirst = 0;
-                float ht = 0.f;
+                // key[0] (the nearest) goes in hand; key[nv-1] .. key[1] are pushed,
+                // farthest first, so the nearer ones pop first
+                const int nv = (key[0] != 0xFFFFFFFFu) + (key[1] != 0xFFFFFFFFu) +
+                               (key[2] != 0xFFFFFFFFu) + (key[3] != 0xFFFFFFFFu);
+                int32_t ent[3];
+                float ent_t[3];
 #pragma unroll
-                for (int k = 3; k >= 0; --k) {
-                    if (key[k] == 0xFFFFFFFFu) continue;
-                    const uint32_t sl = key[k] & 3u;
+                for (int j = 1; j < 4; ++j) {
+                    const uint32_t sl = key[j] & 3u;
+                    ent[j - 1] = slot_entry(sel4i(sl, ch[0], ch[1], ch[2], ch[3]),
+                                            sel4i(sl, ct[0], ct[1], ct[2], ct[3]));
+                    ent_t[j - 1] = sel4f(sl, tn[0], tn[1], tn[2], tn[3]);
+                }
+                const int np = nv > 1 ? nv - 1 : 0;
+                if (sp + np <= ls.depth) {
+                    // all pushes land in LDS: entry k of this node goes to sp + k,
+                    // holding key[np - k] (farthest at the bottom)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        if (k < np) {
+                            const int j = np - 1 - k;  // ent index of key[np - k]
+                            const int32_t e = j == 0 ? ent[0] : (j == 1 ? ent[1] : ent[2]);
+                            const float t = j == 0 ? ent_t[0] : (j == 1 ? ent_t[1] : ent_t[2]);
+                            ls.i[(sp + k) * ls.stride] = e;
+                            ls.t[(sp + k) * ls.stride] = t;
+                        }
+                    }
+                    sp += np;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        if (k < np && !overflow) {
+                            const int j = np - 1 - k;
+                            const int32_t e = j == 0 ? ent[0] : (j == 1 ? ent[1] : ent[2]);
+                            const float t = j == 0 ? ent_t[0] : (j == 1 ? ent_t[1] : ent_t[2]);
+                            overflow = !stack_push(ls, deep, sp, e, t);
+                        }
+                    }
+                }
+                if (nv > 0) {
+                    const uint32_t sl = key[0] & 3u;
                     const int32_t c = sel4i(sl, ch[0], ch[1], ch[2], ch[3]);
                     const int32_t n = sel4i(sl, ct[0], ct[1], ct[2], ct[3]);
-                    const float t = sel4f(sl, tn[0], tn[1], tn[2], tn[3]);
-                    if (have && !overflow)
-                        overflow = !push_item(ls, deep, sp, hnode, hfirst, hcount, ht);
-                    have = true;
-                    ht = t;
-                    hcount = n;
-                    hnode = n < 0 ? c : -1;
-                    hfirst = uint32_t(c);
-                }
-                if (have) {
-                    if (hcount < 0)
-                        node = hnode;
-                    else {
-                        lfirst = hfirst;
-                        lcount = hcount;
+                    if (n < 0) {
+                        node = c;
+                    } else {
+                        lfirst = uint32_t(c);
+                        lcount = n;
                     }
                 }
             }
         }
-        // triangle step: one triangle of the leaf in hand
+        // ---- triangle step: one triangle of the leaf in hand
         if (lcount > 0) {
             const uint32_t pos = lfirst;
             ++lfirst;
@@ -550,17 +574,19 @@ hipError_t launch_fast(const TraceParams& p0, hipStream_t stream) {
     if (grid == 0) return hipSuccess;
     TraceParams p = p0;
     p.lds_depth = cfg.lds_depth;
+    p.refill = cfg.refill;
     hipLaunchKernelGGL(k, dim3(grid), dim3(cfg.block), lds, stream, p);
     return hipGetLastError();
 }
 
 }  // namespace
 
-TraceConfig default_trace_config() { return TraceConfig{64, 8, 200, 0}; }
+TraceConfig default_trace_config() { return TraceConfig{64, 8, 200, 0, 1}; }
 
 bool valid_trace_config(const TraceConfig& c) {
     return (c.block == 64 || c.block == 128 || c.block == 256) && c.lds_depth <= 32 &&
-           c.occupancy_pct >= 10 && c.occupancy_pct <= 400 && c.kernel <= 1;
+           c.occupancy_pct >= 10 && c.occupancy_pct <= 3200 && c.kernel <= 1 && c.refill >= 1 &&
+           c.refill <= 64;
 }
 
 void set_trace_config(const TraceConfig* c) {
@@ -577,7 +603,8 @@ TraceConfig get_trace_config() {
 hipError_t launch_trace_fast(const TraceParams& p, bool any_hit, hipStream_t stream) {
     // the one-item kernel over leaf-ordered triangles, unless the leaf tags
     // cannot address every index entry
-    const bool item = p.tris && p.index_count <= kLeafFirstMask && get_trace_config().kernel == 0;
+    const bool item = p.tris && p.small_leaves && p.index_count <= kLeafFirstMask &&
+                      get_trace_config().kernel == 0;
     if (p.ray_offsets) {
         if (p.stats || !p.tris) return hipErrorInvalidValue;
         if (item)
@@ -626,6 +653,36 @@ __global__ void prepare_tris_kernel(const uint32_t* indices, uint32_t index_coun
     tris[size_t(pos) * 3 + 2] = e2;
 }
 }  // namespace
+
+namespace {
+__global__ void max_leaf_kernel(const DevMbvhNode* nodes, uint32_t count, uint32_t* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int4 c = nodes[i].counts;
+    const int32_t m = max(max(c.x, c.y), max(c.z, c.w));
+    if (m > 0) atomicMax(out, uint32_t(m));
+}
+}  // namespace
+
+// Largest leaf (triangles per slot) of an MBVH; synchronises the stream.
+hipError_t mbvh_max_leaf(const DevMbvhNode* nodes, uint32_t count, uint32_t* result,
+                         hipStream_t stream) {
+    *result = 0;
+    if (count == 0) return hipSuccess;
+    uint32_t* d = nullptr;
+    hipError_t e = hipMalloc(&d, 4);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(d, result, 4, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(max_leaf_kernel, dim3((count + 255) / 256), dim3(256), 0, stream, nodes,
+                           count, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(result, d, 4, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipFree(d);
+    return e;
+}
 
 hipError_t launch_prepare_triangles(const uint32_t* indices, uint32_t index_count,
                                     const float* verts, uint32_t prim_count, float4* tris,

@@ -194,13 +194,14 @@ def test_launch_shapes_and_autotune_keep_results():
     default = rt.get_trace_config()
     want = m.intersect_prepared(rays).cpu().numpy()
     try:
-        shapes = ((64, 0, 100, 0), (128, 4, 50, 0), (256, 24, 200, 0), (64, 32, 400, 0),
-                  (64, 0, 100, 1), (256, 12, 100, 1), (128, 32, 200, 1))
-        for block, depth, occ, kernel in shapes:
-            rt.set_trace_config(block, depth, occ, kernel)
+        shapes = ((64, 0, 100, 0, 1), (128, 4, 50, 0, 8), (256, 24, 200, 0, 64),
+                  (64, 32, 400, 0, 16), (64, 8, 200, 0, 32), (64, 0, 100, 1, 1),
+                  (256, 12, 100, 1, 1), (128, 32, 200, 1, 1))
+        for block, depth, occ, kernel, refill in shapes:
+            rt.set_trace_config(block, depth, occ, kernel, refill)
             got = m.intersect_prepared(rays).cpu().numpy()
             assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (block, depth, occ,
-                                                                              kernel)
+                                                                              kernel, refill)
             occ_got = m.occluded_prepared(rays).cpu().numpy()
             assert np.array_equal(occ_got, (want[:, 3].view(np.uint32) != 0xFFFFFFFF).astype(np.uint8))
         with pytest.raises(rt.RTBvhError, match="InvalidArgument"):
@@ -213,3 +214,32 @@ def test_launch_shapes_and_autotune_keep_results():
     finally:
         rt.set_trace_config(None)
     assert rt.get_trace_config() == default
+
+
+def test_ray_packets_unpack_and_trace():
+    """RayPacket4 records (src/ray.rs:44-59) unpack to the rays
+    RayPacket4::ray gives (src/ray.rs:118-146: t_min 1e-4, t_max = t[i]), and
+    tracing them gives lane-for-lane the hits of those single rays."""
+    print("[packets]", flush=True)
+    verts = teapot()
+    rays8 = ray_set(verts, 4000, seed=31, kind="box")
+    rays8[:, 3] = np.float32(1e-4)
+    rays8[::5, 7] = np.float32(0.75)
+    p = len(rays8) // 4
+    r = rays8[: 4 * p].reshape(p, 4, 8)
+    packets = np.empty((p, 40), np.float32)
+    for k, col in enumerate((0, 1, 2, 4, 5, 6)):
+        packets[:, 4 * k: 4 * k + 4] = r[:, :, col]
+    with np.errstate(divide="ignore"):
+        for k, col in enumerate((4, 5, 6)):
+            packets[:, 24 + 4 * k: 28 + 4 * k] = np.float32(1.0) / r[:, :, col]
+    packets[:, 36:40] = r[:, :, 7]
+    got = rt.rays_from_packets(_dev(packets))
+    assert np.array_equal(got.cpu().numpy().view(np.uint32),
+                          rays8[: 4 * p].view(np.uint32))
+    nodes, idx, m = _trees(verts, "locb")
+    mbvh = rt.Mbvh.from_arrays(_dev(flat(m)), _dev(idx, torch.int32))
+    hp = mbvh.intersect(_dev(verts.reshape(-1, 9)), got).cpu().numpy()
+    want = ref.trace_mbvh(m, idx, verts, as_rays(rays8[: 4 * p]), mode=ref.MODE_LOWEST_ID,
+                          threads=8)
+    assert hits_equal(hp.view(ref.HIT).reshape(-1), want).all()
