@@ -172,6 +172,97 @@ def run_batched(args, dev, rt, scenes, torch):
     return res, p_ms / g_ms
 
 
+def run_adjoint(args, dev, rt, scenes, torch):
+    """100k-triangle noisy sphere, 16M rays, L = sum w_i t_i: forward closest
+    hit and the adjoint kernel's backward to vertices, origins and
+    directions, timed; gradients of a ray sample checked against a float64
+    solve on the host."""
+    from rtbvh_amd.diff import trace_tuv
+
+    log("differentiable: 100k-triangle sphere, 16M rays")
+    verts_np = scenes.noisy_sphere(250, 200, seed=args.seed)
+    aabbs, centers = scenes.triangle_aabbs_centers(verts_np)
+    verts = torch.from_numpy(verts_np).to(dev)
+    bvh = rt.Builder(torch.from_numpy(centers).to(dev),
+                     aabbs=torch.from_numpy(aabbs).to(dev)).construct_locally_ordered_clustered()
+    tree = bvh.trace_tree(verts)
+    n = 1 << 24
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed + 3)
+    o = torch.randn((n, 3), device=dev, generator=gen)
+    o = 3.0 * o / o.norm(dim=1, keepdim=True)
+    tgt = 0.25 + 0.5 * torch.rand((n, 3), device=dev, generator=gen)
+    d = tgt - o
+    d = (d / d.norm(dim=1, keepdim=True)).contiguous()
+    o = o.contiguous()
+    w = torch.randn(n, device=dev, generator=gen)
+
+    def step():
+        vt = verts.detach().requires_grad_(True)
+        oo = o.detach().requires_grad_(True)
+        dd = d.detach().requires_grad_(True)
+        t, u, v, prim = trace_tuv(tree, vt, oo, dd)
+        loss = (w * torch.where(prim >= 0, t, torch.zeros_like(t))).sum()
+        return loss, vt, oo, dd
+
+    step()[0].backward()
+    torch.cuda.synchronize()
+    fwd, tot = [], []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss, vt, oo, dd = step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fwd.append((t1 - t0) * 1e3)
+        tot.append((t2 - t0) * 1e3)
+    # float64 check on the first 200k rays: per-ray gradients by linear solves
+    k = 200_000
+    vt = verts.detach().requires_grad_(True)
+    oo = o[:k].detach().requires_grad_(True)
+    dd = d[:k].detach().requires_grad_(True)
+    t, u, v, prim = trace_tuv(tree, vt, oo, dd)
+    (w[:k] * torch.where(prim >= 0, t, torch.zeros_like(t))).sum().backward()
+    p_ = prim.cpu().numpy()
+    hit = p_ >= 0
+    v64 = verts_np.astype(np.float64)[p_[hit]].reshape(-1, 3, 3)
+    o64 = o[:k].cpu().numpy().astype(np.float64)[hit]
+    d64 = d[:k].cpu().numpy().astype(np.float64)[hit]
+    m = np.stack([-d64, v64[:, 1] - v64[:, 0], v64[:, 2] - v64[:, 0]], axis=-1)
+    x = np.linalg.solve(m, (o64 - v64[:, 0])[..., None])[..., 0]
+    g = np.zeros_like(x)
+    g[:, 0] = w[:k].cpu().numpy().astype(np.float64)[hit]
+    lam = np.linalg.solve(np.transpose(m, (0, 2, 1)), g[..., None])[..., 0]
+    gv = np.zeros((len(verts_np), 9))
+    uu, vv = x[:, 1], x[:, 2]
+    np.add.at(gv[:, 0:3], p_[hit], -(1.0 - uu - vv)[:, None] * lam)
+    np.add.at(gv[:, 3:6], p_[hit], -uu[:, None] * lam)
+    np.add.at(gv[:, 6:9], p_[hit], -vv[:, None] * lam)
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+    errs = {"vertices": rel(vt.grad.cpu().numpy(), gv),
+            "origins": rel(oo.grad.cpu().numpy()[hit], lam),
+            "directions": rel(dd.grad.cpu().numpy()[hit], x[:, 0:1] * lam)}
+    fm, tm = statistics.median(fwd), statistics.median(tot)
+    res = {"tris": len(verts_np), "rays": n, "forward_ms": round(fm, 2),
+           "forward_backward_ms": round(tm, 2),
+           "forward_backward_mrays_s": round(n / (tm * 1e-3) / 1e6, 1),
+           "hit_fraction": round(float(hit.mean()), 4),
+           "fp64_rel_error_200k_rays": {a: float(f"{b:.3e}") for a, b in errs.items()},
+           "workload": "noisy sphere 250x200 quads (100k tris), 2^24 rays from radius 3 at the "
+                       "cube [0.25, 0.75]^3, L = sum w t, w ~ N(0,1); forward = trace tree closest "
+                       "hit, backward = adjoint kernel (vertex grads by float atomics)"}
+    log(f"differentiable: forward {fm:.1f} ms, forward+backward {tm:.1f} ms, errors {errs}")
+    del tree, bvh, verts, o, d, w
+    torch.cuda.empty_cache()
+    return res
+
+
 def copy_calibration(args, torch, dev):
     """One plain device copy of known size, so that rocprofv3's FETCH_SIZE /
     WRITE_SIZE can be scaled to bytes."""
@@ -275,6 +366,8 @@ def main():
     ap.add_argument("--large", type=int, default=1,
                     help="1: also run the animated 20M-triangle any-hit frame (0 skips it)")
     ap.add_argument("--large-rays-log2", type=int, default=28)
+    ap.add_argument("--adjoint", type=int, default=1,
+                    help="1: also time forward + adjoint backward on 100k triangles (0 skips it)")
     ap.add_argument("--trace-only", action="store_true",
                     help="build, then only the timed bounce-ray steps of the headline trace "
                          "(and the copy calibration) for profiler runs; no JSON line")
@@ -407,12 +500,12 @@ def main():
         per = [a.elapsed_time(b) for a, b in ev]
         return out, per, wall
 
-    # launch shape chosen by the autotuner on a 16M-ray sample of the bounce
+    # launch shape chosen by the autotuner on an 8M-ray sample of the bounce
     # rays (untimed set-up, like the builds)
     if args.no_autotune:
         tuned = rt.get_trace_config()
     else:
-        tuned = rt.autotune(ttree, bounce[: min(n_rays, 1 << 24)])
+        tuned = rt.autotune(ttree, bounce[: min(n_rays, 1 << 23)])
     log(f"launch shape {tuned}")
 
     if args.trace_only:
@@ -518,6 +611,7 @@ def main():
         batch_info, batch_ratio = run_batched(args, dev, rt, scenes, torch)
 
     large_info = run_large(args, dev, rt, scenes, torch) if args.large else None
+    adjoint_info = run_adjoint(args, dev, rt, scenes, torch) if args.adjoint else None
 
     result = {
         "metric": METRIC,
@@ -560,6 +654,7 @@ def main():
         "batched_vs_plain": None if batch_ratio is None else round(batch_ratio, 3),
         "batched": batch_info,
         "large_animated": large_info,
+        "differentiable": adjoint_info,
         "parity_vs_cpu_oracle": parity,
         "roofline": {
             "bound": "hbm",

@@ -295,7 +295,7 @@ void rtbvh_gpu_free_trace_tree(RTTraceTree tree);
 
 /* ---- Launch shape of the lowest-id MBVH traversal ---------------------------
  * (no counterpart in the reference).  Per device; the default is
- * {64, 8, 200, 0, 1}.  Results never depend on
+ * {64, 8, 400, 0, 1, 0}.  Results never depend on
  * it, only speed. */
 typedef struct RTTraceConfig {
     uint32_t block;          /* threads per block: 64, 128 or 256 */
@@ -305,13 +305,15 @@ typedef struct RTTraceConfig {
                                 ray per iteration, 1 a node and all its leaf slots */
     uint32_t refill;         /* kernel 0: idle lanes of a wavefront that take new rays
                                 together, 1..64 */
+    uint32_t schedule;       /* kernel 0: 0 each lane walks a fixed stride of rays, 1 waves
+                                take runs of rays from per-chunk counters as lanes free up */
 } RTTraceConfig;
 
 ResultCode rtbvh_gpu_set_trace_config(const RTTraceConfig *config); /* NULL: default */
 ResultCode rtbvh_gpu_get_trace_config(RTTraceConfig *config);
-/* Autotuner: times 120 launch shapes (block 64/128/256 x LDS stack 4..16 x
- * occupancy 100/200 % x {kernel 0 with refill 1/8/16/32, kernel 1}) of the
- * closest-hit traversal of `rays` against
+/* Autotuner: times 252 launch shapes (block 64/128/256 x LDS stack 4..16 x
+ * grid 200/400/800 % x {kernel 0 with refill 1/4/16 and schedule 0/1,
+ * kernel 1}) of the closest-hit traversal of `rays` against
  * `mbvh`'s attached triangles, writing `hits` (device, ray_count records),
  * and makes the fastest the current device's shape (also returned in *best).
  * Synchronises the stream. */

@@ -595,16 +595,16 @@ def get_trace_config() -> dict:
     c = L.RTTraceConfig()
     L.check(L.lib().rtbvh_gpu_get_trace_config(ctypes.byref(c)), "rtbvh_gpu_get_trace_config")
     return {"block": c.block, "lds_stack": c.lds_stack, "occupancy_pct": c.occupancy_pct,
-            "kernel": c.kernel, "refill": c.refill}
+            "kernel": c.kernel, "refill": c.refill, "schedule": c.schedule}
 
 
-def set_trace_config(block: Optional[int] = None, lds_stack: int = 8, occupancy_pct: int = 200,
-                     kernel: int = 0, refill: int = 1):
+def set_trace_config(block: Optional[int] = None, lds_stack: int = 8, occupancy_pct: int = 400,
+                     kernel: int = 0, refill: int = 1, schedule: int = 0):
     """Sets the traversal launch shape (block=None restores the default)."""
     if block is None:
         L.check(L.lib().rtbvh_gpu_set_trace_config(None), "rtbvh_gpu_set_trace_config")
         return
-    c = L.RTTraceConfig(block, lds_stack, occupancy_pct, kernel, refill)
+    c = L.RTTraceConfig(block, lds_stack, occupancy_pct, kernel, refill, schedule)
     L.check(L.lib().rtbvh_gpu_set_trace_config(ctypes.byref(c)), "rtbvh_gpu_set_trace_config")
 
 
@@ -620,4 +620,4 @@ def autotune(tree, rays: torch.Tensor) -> dict:
     L.check(fn(tree._h, _ptr(rays), rays.shape[0], _ptr(hits), _stream(rays), ctypes.byref(c)),
             fn.__name__)
     return {"block": c.block, "lds_stack": c.lds_stack, "occupancy_pct": c.occupancy_pct,
-            "kernel": c.kernel, "refill": c.refill}
+            "kernel": c.kernel, "refill": c.refill, "schedule": c.schedule}

@@ -456,10 +456,10 @@ ResultCode rtbvh_gpu_set_trace_config(const RTTraceConfig* config) {
         return Ok;
     }
     const TraceConfig c{config->block, config->lds_stack, config->occupancy_pct, config->kernel,
-                        config->refill};
+                        config->refill, config->schedule};
     if (!valid_trace_config(c)) {
         set_error("trace config: block 64/128/256, lds_stack <= 32, occupancy_pct 10..3200, "
-                  "kernel 0/1, refill 1..64");
+                  "kernel 0/1, refill 1..64, schedule 0/1");
         return InvalidArgument;
     }
     set_trace_config(&c);
@@ -477,6 +477,7 @@ ResultCode rtbvh_gpu_get_trace_config(RTTraceConfig* config) {
     config->occupancy_pct = c.occupancy_pct;
     config->kernel = c.kernel;
     config->refill = c.refill;
+    config->schedule = c.schedule;
     return Ok;
 }
 
@@ -504,14 +505,15 @@ ResultCode autotune(const RTRay* rays, size_t ray_count, RTHit* hits, void* stre
         return hip_fail(e, "autotune events");
     }
     const TraceConfig before = get_trace_config();
-    // one-item kernel: block x LDS depth x grid x refill; per-node-leaves
-    // kernel: block x LDS depth x grid (refill does not apply)
+    // one-item kernel: block x LDS depth x grid x refill x schedule;
+    // per-node-leaves kernel: block x LDS depth x grid
     std::vector<TraceConfig> shapes;
     for (uint32_t b : {64u, 128u, 256u})
         for (uint32_t d : {4u, 8u, 12u, 16u})
-            for (uint32_t o : {100u, 200u}) {
-                for (uint32_t rf : {1u, 8u, 16u, 32u}) shapes.push_back(TraceConfig{b, d, o, 0, rf});
-                shapes.push_back(TraceConfig{b, d, o, 1, 1});
+            for (uint32_t o : {200u, 400u, 800u}) {
+                for (uint32_t rf : {1u, 4u, 16u})
+                    for (uint32_t sc : {0u, 1u}) shapes.push_back(TraceConfig{b, d, o, 0, rf, sc});
+                shapes.push_back(TraceConfig{b, d, o, 1, 1, 0});
             }
     TraceConfig top = before;
     float top_ms = 3.4e38f;
@@ -549,6 +551,7 @@ ResultCode autotune(const RTRay* rays, size_t ray_count, RTHit* hits, void* stre
         best->occupancy_pct = top.occupancy_pct;
         best->kernel = top.kernel;
         best->refill = top.refill;
+        best->schedule = top.schedule;
     }
     return Ok;
 }

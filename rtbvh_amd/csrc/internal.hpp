@@ -36,7 +36,11 @@ struct TraceParams {
     uint32_t lds_depth;    // set by the launcher: stack entries per lane held in LDS
     uint32_t refill;       // set by the launcher: idle lanes that trigger taking new rays
     bool small_leaves;     // every leaf holds at most 16 triangles (one-item kernel)
+    uint32_t* work;        // dynamic schedule: work_chunks ray counters (launcher-cleared)
+    uint32_t work_chunks;
 };
+
+constexpr uint32_t kPlanWorkWords = 256;  // counter block a plan owns for its traversal
 
 // Launch shape of the lowest-id MBVH traversal (per device).
 struct TraceConfig {
@@ -47,6 +51,8 @@ struct TraceConfig {
                              // 1 a node and all its leaf slots per iteration
     uint32_t refill;         // one-item kernel: idle lanes of a wave that take new rays
                              // together (1..64)
+    uint32_t schedule;       // one-item kernel: 0 fixed ray stride per lane, 1 rays taken
+                             // from per-chunk counters as lanes free up
 };
 TraceConfig default_trace_config();
 bool valid_trace_config(const TraceConfig& c);

@@ -18,6 +18,8 @@
 //     fetching its node.
 // The oracle's lowest-id mode (oracle/refcpu.cpp) follows the same order, so
 // the kernel's work counters (STATS) are checked against it.
+#include <atomic>
+
 #include "internal.hpp"
 
 namespace rtbvh {
@@ -329,6 +331,15 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     uint64_t rid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const uint32_t refill = p.refill ? p.refill : 1u;
+    // dynamic schedule: the rays are cut into p.work_chunks chunks with one
+    // counter each; a wave takes rays from its home chunk, then from the next
+    // ones once that is used up
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lanes_below = lane ? ((~0ull) >> (64u - lane)) : 0ull;
+    const uint32_t chunks = p.work ? p.work_chunks : 1u;
+    const uint64_t chunk_len = (p.ray_count + chunks - 1) / chunks;
+    uint32_t home = uint32_t(((uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6) % chunks);
+    bool drained = false;  // wave-uniform: every chunk used up
     float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f,
           iz = 0.f, tmin = 0.f;
     float bt = 0.f, bu = 0.f, bv = 0.f;
@@ -339,11 +350,46 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
     bool active = false;
     for (;;) {
         // ---- refill: idle lanes that still have rays take them together
-        const bool can_take = !active && rid < p.ray_count;
+        const bool can_take = !active && (p.work ? !drained : rid < p.ray_count);
         const uint64_t takers = __ballot(can_take);
         const uint64_t busy = __ballot(active);
         if (takers == 0 && busy == 0) break;  // wave-uniform: no ray left for any lane
-        if (takers != 0 && (uint32_t(__popcll(takers)) >= refill || busy == 0) && can_take) {
+        bool take = false;
+        if (takers != 0 && (uint32_t(__popcll(takers)) >= refill || busy == 0)) {
+            if (p.work) {
+                // one atomic per wave: the first taker claims a run of rays for all
+                const int leader = __ffsll((unsigned long long)takers) - 1;
+                const uint32_t n = uint32_t(__popcll(takers));
+                bool got = false;
+                uint64_t first = 0, end = 0;
+                for (uint32_t tries = 0; tries < chunks && !got; ++tries) {
+                    uint32_t old = 0;
+                    if (int(lane) == leader) old = atomicAdd(&p.work[home], n);
+                    old = uint32_t(__shfl(int(old), leader));
+                    const uint64_t cb = uint64_t(home) * chunk_len;
+                    const uint64_t ce = cb + chunk_len < p.ray_count ? cb + chunk_len : p.ray_count;
+                    if (cb + old < ce) {
+                        got = true;
+                        first = cb + old;
+                        end = ce;
+                    } else {
+                        home = home + 1 == chunks ? 0u : home + 1;
+                    }
+                }
+                if (!got) {
+                    drained = true;
+                } else if (can_take) {
+                    const uint64_t r = first + uint64_t(__popcll(takers & lanes_below));
+                    if (r < end) {
+                        rid = r;
+                        take = true;
+                    }
+                }
+            } else {
+                take = can_take;
+            }
+        }
+        if (take) {
             const float4* r4 = reinterpret_cast<const float4*>(p.rays + rid);
             const float4 ra = r4[0];
             const float4 rb = r4[1];
@@ -370,7 +416,7 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
                     reinterpret_cast<float4*>(p.hits)[rid] =
                         make_float4(bt, 0.f, 0.f, __uint_as_float(RTBVH_MISS));
                 }
-                rid += stride;
+                if (!p.work) rid += stride;
             } else {
                 active = true;
                 node = BATCH ? int32_t(scene_of_ray(p.ray_offsets, p.scene_count,
@@ -526,13 +572,38 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
             active = false;
             node = -1;
             lcount = 0;
-            rid += stride;
+            if (!p.work) rid += stride;
         }
     }
     if (STATS) {
         atomicAdd(reinterpret_cast<unsigned long long*>(p.stats), (unsigned long long)n_nodes);
         atomicAdd(reinterpret_cast<unsigned long long*>(p.stats) + 1, (unsigned long long)n_tris);
     }
+}
+
+// Counter blocks of the dynamic schedule: a ring per device, one block per
+// launch (cleared by a kernel in the launch's stream).  A launch that brings
+// its own block (p.work) uses that instead.
+constexpr uint32_t kWorkChunks = 256;
+constexpr uint32_t kWorkSlots = 256;
+__device__ uint32_t g_work_ring[kWorkSlots * kWorkChunks];
+std::atomic<uint32_t> g_work_next{0};
+
+uint32_t* work_ring_ptr() {
+    static uint32_t* ptrs[64] = {nullptr};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    if (ptrs[dev] == nullptr) {
+        void* q = nullptr;
+        if (hipGetSymbolAddress(&q, HIP_SYMBOL(g_work_ring)) != hipSuccess) return nullptr;
+        ptrs[dev] = static_cast<uint32_t*>(q);
+    }
+    return ptrs[dev];
+}
+
+__global__ void clear_words_kernel(uint32_t* w, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = 0;
 }
 
 // Launch shape per device: block size, LDS stack depth and the share of the
@@ -575,18 +646,30 @@ hipError_t launch_fast(const TraceParams& p0, hipStream_t stream) {
     TraceParams p = p0;
     p.lds_depth = cfg.lds_depth;
     p.refill = cfg.refill;
+    if (ITEM && cfg.schedule == 1) {
+        if (!p.work) {
+            uint32_t* ring = work_ring_ptr();
+            if (ring == nullptr) return hipErrorInvalidSymbol;
+            p.work = ring + size_t(g_work_next.fetch_add(1) % kWorkSlots) * kWorkChunks;
+        }
+        p.work_chunks = kWorkChunks;
+        hipLaunchKernelGGL(clear_words_kernel, dim3((kWorkChunks + 255) / 256), dim3(256), 0,
+                           stream, p.work, kWorkChunks);
+    } else {
+        p.work = nullptr;
+    }
     hipLaunchKernelGGL(k, dim3(grid), dim3(cfg.block), lds, stream, p);
     return hipGetLastError();
 }
 
 }  // namespace
 
-TraceConfig default_trace_config() { return TraceConfig{64, 8, 200, 0, 1}; }
+TraceConfig default_trace_config() { return TraceConfig{64, 8, 400, 0, 1, 0}; }
 
 bool valid_trace_config(const TraceConfig& c) {
     return (c.block == 64 || c.block == 128 || c.block == 256) && c.lds_depth <= 32 &&
            c.occupancy_pct >= 10 && c.occupancy_pct <= 3200 && c.kernel <= 1 && c.refill >= 1 &&
-           c.refill <= 64;
+           c.refill <= 64 && c.schedule <= 1;
 }
 
 void set_trace_config(const TraceConfig* c) {

@@ -13,7 +13,7 @@ import ctypes
 import torch
 
 from . import _lib as L
-from .api import RAY_FLOATS, _ptr, _require_device, _stream, hits_prim, rays_tensor
+from .api import RAY_FLOATS, TraceTree, _ptr, _require_device, _stream, hits_prim, rays_tensor
 
 
 def adjoint(vertices: torch.Tensor, rays: torch.Tensor, hits: torch.Tensor,
@@ -46,7 +46,10 @@ class TraceTUV(torch.autograd.Function):
         verts = vertices.detach().contiguous()
         rays = rays_tensor(origins.detach().contiguous(), directions.detach().contiguous(),
                            t_min, t_max)
-        hits = tree.intersect(verts.reshape(-1, 9), rays)
+        if isinstance(tree, TraceTree):  # built over these vertices
+            hits = tree.intersect(rays)
+        else:
+            hits = tree.intersect(verts.reshape(-1, 9), rays)
         prim = hits_prim(hits)
         ctx.save_for_backward(verts, rays, hits)
         ctx.mark_non_differentiable(prim)
@@ -69,8 +72,9 @@ class TraceTUV(torch.autograd.Function):
 
 def trace_tuv(tree, vertices: torch.Tensor, origins: torch.Tensor, directions: torch.Tensor,
               t_min: float = 1e-4, t_max: float = 1e34):
-    """Closest hit (t, u, v, prim id) of rays against `tree` (a Bvh or Mbvh
-    over `vertices`), differentiable in vertices, origins and directions.
+    """Closest hit (t, u, v, prim id) of rays against `tree` (a Bvh, Mbvh or
+    TraceTree over `vertices`), differentiable in vertices, origins and
+    directions.
     Misses return t = t_max, u = v = 0, prim = -1 and zero gradients."""
     return TraceTUV.apply(vertices, origins, directions, tree, t_min, t_max)
 

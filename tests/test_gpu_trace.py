@@ -194,14 +194,16 @@ def test_launch_shapes_and_autotune_keep_results():
     default = rt.get_trace_config()
     want = m.intersect_prepared(rays).cpu().numpy()
     try:
-        shapes = ((64, 0, 100, 0, 1), (128, 4, 50, 0, 8), (256, 24, 200, 0, 64),
-                  (64, 32, 400, 0, 16), (64, 8, 200, 0, 32), (64, 0, 100, 1, 1),
-                  (256, 12, 100, 1, 1), (128, 32, 200, 1, 1))
-        for block, depth, occ, kernel, refill in shapes:
-            rt.set_trace_config(block, depth, occ, kernel, refill)
+        shapes = ((64, 0, 100, 0, 1, 0), (128, 4, 50, 0, 8, 0), (256, 24, 200, 0, 64, 0),
+                  (64, 32, 400, 0, 16, 0), (64, 8, 200, 0, 32, 1), (64, 8, 800, 0, 1, 1),
+                  (256, 4, 3200, 0, 4, 1), (64, 0, 100, 1, 1, 0), (256, 12, 100, 1, 1, 0),
+                  (128, 32, 200, 1, 1, 0))
+        for block, depth, occ, kernel, refill, sched in shapes:
+            rt.set_trace_config(block, depth, occ, kernel, refill, sched)
             got = m.intersect_prepared(rays).cpu().numpy()
             assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (block, depth, occ,
-                                                                              kernel, refill)
+                                                                              kernel, refill,
+                                                                              sched)
             occ_got = m.occluded_prepared(rays).cpu().numpy()
             assert np.array_equal(occ_got, (want[:, 3].view(np.uint32) != 0xFFFFFFFF).astype(np.uint8))
         with pytest.raises(rt.RTBvhError, match="InvalidArgument"):
