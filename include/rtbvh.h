@@ -276,12 +276,26 @@ typedef struct RTTraceTree {
     uint32_t node_count;
     uint32_t prim_count;
     uint32_t max_leaf;
+    uint32_t width;     /* 4, or 8: the compressed tree of rtbvh_gpu_create_trace_tree8 */
+    uint32_t pad_;
+    const void *nodes;       /* device: width 4, 128 B RTMbvhNode-layout records; width 8,
+                                80 B compressed records (rtbvh_amd/csrc/device_math.hpp) */
+    const float *triangles;  /* device: 12 floats per primitive, leaf order: v0 with the
+                                primitive id's bits in [3], e1 = v1 - v0, e2 = v2 - v0 */
 } RTTraceTree;
 
 /* bvh: made by rtbvh_gpu_create_bvh; vertices: device, 9 floats per
  * primitive; max_leaf 1..8 (0: 1, the fastest on the benchmark scenes). */
 ResultCode rtbvh_gpu_create_trace_tree(RTBvh bvh, const float *vertices, size_t prim_count,
                                        uint32_t max_leaf, void *stream, RTTraceTree *result);
+/* The compressed 8-wide traversal tree: 80-byte nodes whose eight slot boxes
+ * are 8-bit steps on a per-node power-of-two grid, rounded outward with one
+ * step to spare; slots placed by direction octant so that a ray visits the
+ * near side first; one triangle per leaf slot.  Same hits, bit for bit, as
+ * the 4-wide tree; traced and freed by the same calls.  Fails with Error on
+ * non-finite bounds (the 4-wide tree accepts them). */
+ResultCode rtbvh_gpu_create_trace_tree8(RTBvh bvh, const float *vertices, size_t prim_count,
+                                        void *stream, RTTraceTree *result);
 ResultCode rtbvh_gpu_trace_tree_intersect(RTTraceTree tree, const RTRay *rays, size_t ray_count,
                                           RTHit *hits, uint32_t flags, void *stream);
 /* rtbvh_gpu_trace_tree_intersect that also adds {nodes visited, triangles

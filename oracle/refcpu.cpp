@@ -263,9 +263,12 @@ inline void try_prim(const float* verts, size_t prim_count, uint32_t prim, const
     } else {
         // accept t == h.t when the id is lower: test against nextafter(h.t)
         // would change the comparison form, so evaluate with t_cur = +inf and
-        // compare here.
+        // compare here.  The tie only counts against an accepted hit: a
+        // candidate exactly at the ray's t_max is rejected, as the
+        // reference's `t < ray.t` (src/builders/spatial_sah.rs:157) and brute
+        // force do.
         MT m = mt_test(p, ray.o, ray.d, ray.t_min, std::numeric_limits<float>::infinity());
-        if (m.ok && (m.t < h.t || (m.t == h.t && prim < h.prim))) {
+        if (m.ok && (m.t < h.t || (m.t == h.t && h.prim != 0xFFFFFFFFu && prim < h.prim))) {
             h.t = m.t;
             h.u = m.u;
             h.v = m.v;

@@ -66,7 +66,9 @@ class RTTraceConfig(ctypes.Structure):
 
 class RTTraceTree(ctypes.Structure):
     _fields_ = [("id", ctypes.c_uint32), ("node_count", ctypes.c_uint32),
-                ("prim_count", ctypes.c_uint32), ("max_leaf", ctypes.c_uint32)]
+                ("prim_count", ctypes.c_uint32), ("max_leaf", ctypes.c_uint32),
+                ("width", ctypes.c_uint32), ("pad_", ctypes.c_uint32),
+                ("nodes", ctypes.c_void_p), ("triangles", ctypes.c_void_p)]
 
 
 CALLBACK = ctypes.CFUNCTYPE(ctypes.c_bool, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float),
@@ -95,6 +97,7 @@ SIGNATURES = {
     "rtbvh_gpu_intersect_mbvh_prepared": (_rc, [RTMbvh, _vp, _sz, _vp, _u32, _vp]),
     "rtbvh_gpu_occluded_mbvh_prepared": (_rc, [RTMbvh, _vp, _sz, _vp, _u32, _vp]),
     "rtbvh_gpu_create_trace_tree": (_rc, [RTBvh, _vp, _sz, _u32, _vp, ctypes.POINTER(RTTraceTree)]),
+    "rtbvh_gpu_create_trace_tree8": (_rc, [RTBvh, _vp, _sz, _vp, ctypes.POINTER(RTTraceTree)]),
     "rtbvh_gpu_trace_tree_intersect": (_rc, [RTTraceTree, _vp, _sz, _vp, _u32, _vp]),
     "rtbvh_gpu_trace_tree_intersect_stats": (_rc, [RTTraceTree, _vp, _sz, _vp, _vp, _vp]),
     "rtbvh_gpu_trace_tree_occluded": (_rc, [RTTraceTree, _vp, _sz, _vp, _u32, _vp]),

@@ -217,9 +217,10 @@ class Bvh(_Tree):
             raise ValueError("aabbs must be (prim_count, 8)")
         L.check(L.lib().rtbvh_gpu_refit(_ptr(aabbs), self._h, _stream(aabbs)), "rtbvh_gpu_refit")
 
-    def trace_tree(self, vertices: torch.Tensor, max_leaf: int = 1) -> "TraceTree":
+    def trace_tree(self, vertices: torch.Tensor, max_leaf: int = 1,
+                   width: int = 4) -> "TraceTree":
         """Traversal tree over `vertices` (see TraceTree)."""
-        return TraceTree(self, vertices, max_leaf)
+        return TraceTree(self, vertices, max_leaf, width)
 
     def __del__(self):
         if getattr(self, "_owned", False) and L._lib is not None:
@@ -227,28 +228,80 @@ class Bvh(_Tree):
             self._owned = False
 
 
-class TraceTree:
-    """4-wide traversal tree with leaf-ordered triangles, built from a Bvh for
-    the lowest-(t, id) traversal (rtbvh_gpu_create_trace_tree); hits equal
-    Mbvh.intersect's default mode on the same triangles."""
+class _DeviceSpan:
+    """A device byte range presented through __cuda_array_interface__."""
 
-    def __init__(self, bvh: "Bvh", vertices: torch.Tensor, max_leaf: int = 1):
+    def __init__(self, ptr: int, nbytes: int):
+        self.__cuda_array_interface__ = {"shape": (nbytes,), "typestr": "|u1",
+                                         "data": (ptr, False), "version": 2}
+
+
+def _device_bytes(ptr, nbytes: int, device) -> torch.Tensor:
+    if not ptr or nbytes == 0:
+        return torch.empty(0, dtype=torch.uint8, device=device)
+    torch.cuda.synchronize(device)
+    return torch.as_tensor(_DeviceSpan(int(ptr), nbytes), device=device).clone()
+
+
+class TraceTree:
+    """Traversal tree with leaf-ordered triangles, built from a Bvh for the
+    lowest-(t, id) traversal; hits equal Mbvh.intersect's default mode on the
+    same triangles.  width 4: rtbvh_gpu_create_trace_tree (leaves of up to
+    max_leaf triangles); width 8: rtbvh_gpu_create_trace_tree8 (compressed
+    8-wide nodes, one triangle per leaf slot)."""
+
+    def __init__(self, bvh: "Bvh", vertices: torch.Tensor, max_leaf: int = 1, width: int = 4):
         if not bvh._owned:
             raise ValueError("a trace tree needs a Bvh built by this library")
+        if width not in (4, 8):
+            raise ValueError("width must be 4 or 8")
         _require_device(vertices, "vertices")
         verts = vertices.reshape(-1)
         self.device = vertices.device
         self._owned = False
         self._h = L.RTTraceTree()
-        L.check(L.lib().rtbvh_gpu_create_trace_tree(bvh._h, _ptr(verts), verts.numel() // 9,
-                                                    max_leaf, _stream(verts),
-                                                    ctypes.byref(self._h)),
-                "rtbvh_gpu_create_trace_tree")
+        if width == 8:
+            L.check(L.lib().rtbvh_gpu_create_trace_tree8(bvh._h, _ptr(verts), verts.numel() // 9,
+                                                         _stream(verts), ctypes.byref(self._h)),
+                    "rtbvh_gpu_create_trace_tree8")
+        else:
+            L.check(L.lib().rtbvh_gpu_create_trace_tree(bvh._h, _ptr(verts), verts.numel() // 9,
+                                                        max_leaf, _stream(verts),
+                                                        ctypes.byref(self._h)),
+                    "rtbvh_gpu_create_trace_tree")
         self._owned = True
 
     @property
     def node_count(self) -> int:
         return int(self._h.node_count)
+
+    @property
+    def width(self) -> int:
+        return int(self._h.width)
+
+    def node_bytes(self) -> torch.Tensor:
+        """Copy of the node array as bytes (node_count x 128 for width 4,
+        node_count x 80 for width 8)."""
+        rec = 80 if self.width == 8 else 128
+        return _device_bytes(self._h.nodes, self.node_count * rec, self.device)
+
+    def triangles(self) -> torch.Tensor:
+        """Copy of the leaf-ordered triangle records, (prim_count, 12) float32."""
+        b = _device_bytes(self._h.triangles, int(self._h.prim_count) * 48, self.device)
+        return b.view(torch.float32).reshape(-1, 12)
+
+    def intersect_stats(self, rays: torch.Tensor, out=None):
+        """intersect() plus (nodes visited, triangles tested) over all rays."""
+        _require_device(rays, "rays")
+        out = torch.empty((rays.shape[0], HIT_FLOATS), dtype=torch.float32,
+                          device=rays.device) if out is None else out
+        counters = torch.zeros(2, dtype=torch.int64, device=rays.device)
+        L.check(L.lib().rtbvh_gpu_trace_tree_intersect_stats(self._h, _ptr(rays), rays.shape[0],
+                                                             _ptr(out), _ptr(counters),
+                                                             _stream(rays)),
+                "rtbvh_gpu_trace_tree_intersect_stats")
+        c = counters.cpu()
+        return out, int(c[0]), int(c[1])
 
     def intersect(self, rays: torch.Tensor, out=None):
         _require_device(rays, "rays")

@@ -71,6 +71,7 @@ void release(DeviceBvh& d) {
 
 void release(DeviceMbvh& d) {
     if (d.nodes) (void)hipFree(d.nodes);
+    if (d.cnodes) (void)hipFree(d.cnodes);
     if (d.indices) (void)hipFree(d.indices);
     if (d.tris) (void)hipFree(d.tris);
     d = DeviceMbvh{};
@@ -312,6 +313,7 @@ bool batch_view(uint32_t id, BatchView* out) {
     BatchEntry* be = find_batch(id);
     if (be == nullptr) return false;
     out->nodes = be->ttree.nodes;
+    out->cnodes = be->ttree.cnodes;
     out->node_count = be->ttree.node_count;
     out->indices = nullptr;
     out->index_count = be->ttree.prim_count;
@@ -327,6 +329,7 @@ bool trace_tree_view(uint32_t id, BatchView* out) {
     if (id >= r.ttrees.size() || !r.ttrees[id]) return false;
     const TraceTreeEntry& te = *r.ttrees[id];
     out->nodes = te.t.nodes;
+    out->cnodes = te.t.cnodes;
     out->node_count = te.t.node_count;
     out->indices = nullptr;
     out->index_count = te.t.prim_count;
@@ -510,13 +513,14 @@ ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch* batch, const float* vertice
     }
     // the traversal tree of the batch (tight 4-wide forest, leaves of up to
     // 4 triangles, scene-local ids)
-    if (be->ttree.nodes) {
+    if (be->ttree.nodes || be->ttree.cnodes) {
         drain_device();  // traversals of the previous tree may be in flight
         release(be->ttree);
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
     TraceTreePlan* plan = nullptr;
-    ResultCode rc = trace_tree_plan_create(be->bvh, kDefaultMaxLeaf, &plan, &be->ttree);
+    ResultCode rc =
+        trace_tree_plan_create(be->bvh, kDefaultMaxLeaf, kBatchTreeWidth, &plan, &be->ttree);
     if (rc != Ok) return rc;
     rc = trace_tree_build(plan, be->bvh, vertices, uint32_t(prim_count), be->scene_off,
                           be->scenes, &be->ttree, st);
@@ -529,8 +533,9 @@ ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch* batch, const float* vertice
     return Ok;
 }
 
-ResultCode rtbvh_gpu_create_trace_tree(RTBvh bvh, const float* vertices, size_t prim_count,
-                                       uint32_t max_leaf, void* stream, RTTraceTree* result) {
+static ResultCode create_trace_tree(RTBvh bvh, const float* vertices, size_t prim_count,
+                                    uint32_t max_leaf, uint32_t width, void* stream,
+                                    RTTraceTree* result) {
     if (vertices == nullptr || result == nullptr) {
         set_error("vertices and result must not be null");
         return Error;
@@ -557,7 +562,7 @@ ResultCode rtbvh_gpu_create_trace_tree(RTBvh bvh, const float* vertices, size_t 
         te->device = be->device;
         hipStream_t st = static_cast<hipStream_t>(stream);
         TraceTreePlan* plan = nullptr;
-        ResultCode rc = trace_tree_plan_create(be->dev, max_leaf, &plan, &te->t);
+        ResultCode rc = trace_tree_plan_create(be->dev, max_leaf, width, &plan, &te->t);
         if (rc != Ok) return rc;
         rc = trace_tree_build(plan, be->dev, vertices, uint32_t(prim_count), nullptr, 0, &te->t,
                               st);
@@ -574,7 +579,22 @@ ResultCode rtbvh_gpu_create_trace_tree(RTBvh bvh, const float* vertices, size_t 
     result->node_count = r.ttrees.back()->t.node_count;
     result->prim_count = r.ttrees.back()->t.prim_count;
     result->max_leaf = max_leaf;
+    result->width = width;
+    result->pad_ = 0;
+    result->nodes = width == 8 ? static_cast<const void*>(r.ttrees.back()->t.cnodes)
+                               : static_cast<const void*>(r.ttrees.back()->t.nodes);
+    result->triangles = reinterpret_cast<const float*>(r.ttrees.back()->t.tris);
     return Ok;
+}
+
+ResultCode rtbvh_gpu_create_trace_tree(RTBvh bvh, const float* vertices, size_t prim_count,
+                                       uint32_t max_leaf, void* stream, RTTraceTree* result) {
+    return create_trace_tree(bvh, vertices, prim_count, max_leaf, 4, stream, result);
+}
+
+ResultCode rtbvh_gpu_create_trace_tree8(RTBvh bvh, const float* vertices, size_t prim_count,
+                                        void* stream, RTTraceTree* result) {
+    return create_trace_tree(bvh, vertices, prim_count, 1, 8, stream, result);
 }
 
 void rtbvh_gpu_free_trace_tree(RTTraceTree tree) {

@@ -193,6 +193,7 @@ ResultCode view_entry(const BatchView& v, const char* what, uint32_t id, const R
     p.stats = stats;
     p.small_leaves = true;  // traversal-tree leaves hold at most 8 triangles
     p.mnodes = v.nodes;
+    p.cnodes = v.cnodes;
     p.node_count = v.node_count;
     p.index_count = v.index_count;
     p.indices = nullptr;

@@ -58,6 +58,7 @@ struct PlanEntry {
         if (ev_out) (void)hipEventDestroy(ev_out);
         if (stream) (void)hipStreamDestroy(stream);
         if (ttree.nodes) (void)hipFree(ttree.nodes);
+        if (ttree.cnodes) (void)hipFree(ttree.cnodes);
         if (ttree.tris) (void)hipFree(ttree.tris);
         trace_tree_plan_destroy(tt);
         ploc_plan_destroy(ploc);
@@ -144,6 +145,7 @@ hipError_t enqueue_all(PlanEntry& pe, const RTAabb* aabbs, const float* centers,
     if (ray_count == 0) return hipSuccess;
     TraceParams p{};
     p.mnodes = pe.ttree.nodes;
+    p.cnodes = pe.ttree.cnodes;
     p.node_count = pe.ttree.node_count;
     p.index_count = pe.prims;
     p.indices = nullptr;
@@ -209,7 +211,7 @@ ResultCode rtbvh_gpu_batch_plan_create(const RTAabb* aabbs, const float* centers
                           &pe->tree);
     if (rc != Ok) return rc;
     // the traversal tree, built once eagerly to size its level budget
-    rc = trace_tree_plan_create(pe->tree, kDefaultMaxLeaf, &pe->tt, &pe->ttree);
+    rc = trace_tree_plan_create(pe->tree, kDefaultMaxLeaf, kBatchTreeWidth, &pe->tt, &pe->ttree);
     if (rc != Ok) return rc;
     const uint32_t capacity = pe->ttree.node_count;
     if (vertices == nullptr) {

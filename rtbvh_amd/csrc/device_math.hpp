@@ -35,6 +35,32 @@ struct alignas(16) DevMbvhNode {
     int4 children, counts;
 };
 
+// Compressed 8-wide traversal-tree node (80 B).  Slot s of the node covers
+// [p + qlo[s] * 2^(e-127), p + qhi[s] * 2^(e-127)] per axis, a box that holds
+// the slot's subtree with at least one grid step to spare on every side.
+// Interior children are stored from child_base on, triangles (one per leaf
+// slot, 48 B records) from tri_base on, both in slot order.
+struct alignas(16) DevCw8Node {
+    float px, py, pz;
+    uint32_t emask;       // bytes 0..2: biased exponents of x, y, z; byte 3: interior slots
+    uint32_t child_base;  // first interior child
+    uint32_t tri_base;    // first triangle record
+    uint32_t lmask;       // byte 0: triangle slots
+    uint32_t pad;
+    uint32_t qlo_x[2], qlo_y[2], qlo_z[2];  // byte s (of the pair) = slot s
+    uint32_t qhi_x[2], qhi_y[2], qhi_z[2];
+};
+static_assert(sizeof(DevCw8Node) == 80, "compressed node is 80 bytes");
+
+// Lowest-(t, id) rule: a candidate (t, prim) replaces the best (bt, bp) when
+// it is nearer, or as near with a lower id -- the latter only against an
+// accepted hit, never against the ray's t_max (bp still RTBVH_MISS), which
+// the reference's test (t < ray.t, src/builders/spatial_sah.rs:157)
+// excludes.  This is what brute force in id order gives.
+__device__ __forceinline__ bool lowest_first(float t, uint32_t prim, float bt, uint32_t bp) {
+    return (t < bt) || (t == bt && bp != 0xFFFFFFFFu && prim < bp);
+}
+
 struct DevRay {
     float o[3];
     float t_min;

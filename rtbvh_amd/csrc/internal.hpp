@@ -16,6 +16,7 @@ constexpr uint32_t kStatusStackOverflow = 1u;
 
 struct TraceParams {
     const DevMbvhNode* mnodes;
+    const DevCw8Node* cnodes;  // non-null: compressed 8-wide tree (with tris)
     const DevAabb* bnodes;
     uint32_t node_count;
     uint32_t index_count;
@@ -84,6 +85,7 @@ struct DeviceBvh {
 
 struct DeviceMbvh {
     DevMbvhNode* nodes = nullptr;
+    DevCw8Node* cnodes = nullptr;  // compressed 8-wide traversal tree (nodes is then null)
     uint32_t* indices = nullptr;
     float4* tris = nullptr;  // attached leaf-ordered triangles (optional)
     uint32_t max_leaf = 0;   // largest leaf, measured when triangles are attached
@@ -132,12 +134,14 @@ void collapse_plan_destroy(CollapsePlanImpl* c);
 hipError_t launch_parent_links(const DevAabb* nodes, uint32_t count, int32_t* parents,
                                hipStream_t stream);
 
-// Traversal tree (trace_tree.hip): a 4-wide tree with tight slot bounds and
-// leaves of up to max_leaf primitives whose triangles are stored in leaf
-// order; traced by the lowest-id kernel with `tris`.  *out receives newly
-// allocated nodes (capacity prim_count + roots) and tris; indices is null.
+// Traversal tree (trace_tree.hip): width 4, a tree with tight slot bounds and
+// leaves of up to max_leaf primitives; width 8, the compressed 8-wide tree
+// with one triangle per leaf slot (max_leaf must be 1).  Triangles are stored
+// in leaf order; traced by the lowest-id kernels with `tris`.  *out receives
+// newly allocated nodes or cnodes (capacity prim_count + roots) and tris;
+// indices is null.
 struct TraceTreePlan;
-ResultCode trace_tree_plan_create(const DeviceBvh& tree, uint32_t max_leaf,
+ResultCode trace_tree_plan_create(const DeviceBvh& tree, uint32_t max_leaf, uint32_t width,
                                   TraceTreePlan** out_plan, DeviceMbvh* out);
 ResultCode trace_tree_build(TraceTreePlan* t, const DeviceBvh& tree, const float* verts,
                             uint32_t prim_count, const uint32_t* scene_off, uint32_t scenes,
@@ -175,6 +179,7 @@ const float4* attached_triangles(uint32_t id, uint32_t* count, int* device, bool
 
 struct BatchView {
     const DevMbvhNode* nodes;
+    const DevCw8Node* cnodes;
     uint32_t node_count;
     const uint32_t* indices;
     uint32_t index_count;
@@ -186,6 +191,8 @@ bool batch_view(uint32_t id, BatchView* out);
 bool trace_tree_view(uint32_t id, BatchView* out);
 
 constexpr uint32_t kDefaultMaxLeaf = 1;
+// Traversal trees of batches and batch plans: 4 or 8 (the compressed tree).
+constexpr uint32_t kBatchTreeWidth = 4;
 
 void set_error(const char* fmt, ...);
 ResultCode hip_fail(hipError_t e, const char* what);

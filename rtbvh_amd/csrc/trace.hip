@@ -106,7 +106,7 @@ __device__ __forceinline__ bool test_leaf(const TraceParams& p, int32_t first, i
         if (REF || ANY)
             take = m.t < s.bt;
         else
-            take = (m.t < s.bt) || (m.t == s.bt && prim < s.bp);
+            take = lowest_first(m.t, prim, s.bt, s.bp);
         if (take) {
             s.bt = m.t;
             s.bu = m.u;

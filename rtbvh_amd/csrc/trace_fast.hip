@@ -5,9 +5,9 @@
 // arithmetic but is free to choose the visit order, because the lowest
 // (t, prim) over all triangles does not depend on it:
 //
-//   * slab test with IEEE min/max (v_min3/v_max3): a NaN slab (zero
-//     direction component, origin on the plane) drops out of the test, which
-//     can only accept more boxes, never fewer;
+//   * slab test with IEEE min/max (v_min3/v_max3); along a zero direction
+//     component the planes are measured from the floats next to the origin
+//     (slab_origin), so that a ray inside a face plane touches the box;
 //   * a slot is visited when t_entry <= t_best and t_exit >= t_min;
 //   * slots are ordered by a 32-bit key (entry distance bits with the slot
 //     id in the two low bits), sorted with five integer min/max pairs;
@@ -72,6 +72,21 @@ __device__ __forceinline__ void stack_pop(const LdsStack& ls, const DeepStack& d
     }
 }
 
+// Origins of the slab tests of the lowest-id 4-wide kernels, per axis: the
+// ray's own, except where 1/d is infinite (a zero or tiny direction
+// component), where the low planes are measured from the next float above it
+// and the high planes from the next float below.  A ray that runs inside a box's face
+// plane then gets -inf / +inf for that slab (it touches the box) instead of
+// 0 * inf = NaN, which min/max would turn into a closed slab and miss hits on
+// that face that brute force finds; a ray one float or more outside still
+// gets an empty slab.  Nonzero components keep the exact arithmetic of
+// MbvhNode::intersect (src/mbvh_node.rs:177-240).
+__device__ __forceinline__ void slab_origin(float o, float inv, float& lo, float& hi) {
+    const bool flat = fabsf(inv) > 3.4028235e38f;
+    lo = flat ? nextafterf(o, 3.4028235e38f) : o;
+    hi = flat ? nextafterf(o, -3.4028235e38f) : o;
+}
+
 template <typename T>
 __device__ __forceinline__ T sel4(uint32_t k, T a, T b, T c, T d) {
     return k == 0 ? a : (k == 1 ? b : (k == 2 ? c : d));
@@ -114,6 +129,7 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_fast_kernel(TraceParams p) {
     uint64_t rid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f,
           iz = 0.f, tmin = 0.f;
+    float lox = 0.f, loy = 0.f, loz = 0.f, hox = 0.f, hoy = 0.f, hoz = 0.f;  // slab origins
     float bt = 0.f, bu = 0.f, bv = 0.f;
     uint32_t bp = RTBVH_MISS;
     int32_t cur = -1;
@@ -135,6 +151,9 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_fast_kernel(TraceParams p) {
             ix = 1.0f / dx;  // Ray::new, src/ray.rs:166-182
             iy = 1.0f / dy;
             iz = 1.0f / dz;
+            slab_origin(ox, ix, lox, hox);
+            slab_origin(oy, iy, loy, hoy);
+            slab_origin(oz, iz, loz, hoz);
             bu = 0.f;
             bv = 0.f;
             bp = RTBVH_MISS;
@@ -183,9 +202,9 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_fast_kernel(TraceParams p) {
             uint32_t key[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float tx0 = (mnx[i] - ox) * ix, tx1 = (mxx[i] - ox) * ix;
-                const float ty0 = (mny[i] - oy) * iy, ty1 = (mxy[i] - oy) * iy;
-                const float tz0 = (mnz[i] - oz) * iz, tz1 = (mxz[i] - oz) * iz;
+                const float tx0 = (mnx[i] - lox) * ix, tx1 = (mxx[i] - hox) * ix;
+                const float ty0 = (mny[i] - loy) * iy, ty1 = (mxy[i] - hoy) * iy;
+                const float tz0 = (mnz[i] - loz) * iz, tz1 = (mxz[i] - hoz) * iz;
                 const float a = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
                 const float b = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
                 tn[i] = a;
@@ -233,7 +252,7 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_fast_kernel(TraceParams p) {
                             found = true;
                             break;
                         }
-                    } else if ((m.t < bt) || (m.t == bt && prim < bp)) {
+                    } else if (lowest_first(m.t, prim, bt, bp)) {
                         bt = m.t;
                         bu = m.u;
                         bv = m.v;
@@ -342,6 +361,7 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
     bool drained = false;  // wave-uniform: every chunk used up
     float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f,
           iz = 0.f, tmin = 0.f;
+    float lox = 0.f, loy = 0.f, loz = 0.f, hox = 0.f, hoy = 0.f, hoz = 0.f;  // slab origins
     float bt = 0.f, bu = 0.f, bv = 0.f;
     uint32_t bp = RTBVH_MISS;
     int32_t node = -1;     // node in hand, or -1
@@ -404,6 +424,9 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
             ix = 1.0f / dx;  // Ray::new, src/ray.rs:166-182
             iy = 1.0f / dy;
             iz = 1.0f / dz;
+            slab_origin(ox, ix, lox, hox);
+            slab_origin(oy, iy, loy, hoy);
+            slab_origin(oz, iz, loz, hoz);
             bu = 0.f;
             bv = 0.f;
             bp = RTBVH_MISS;
@@ -465,9 +488,9 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
                 uint32_t key[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const float tx0 = (mnx[i] - ox) * ix, tx1 = (mxx[i] - ox) * ix;
-                    const float ty0 = (mny[i] - oy) * iy, ty1 = (mxy[i] - oy) * iy;
-                    const float tz0 = (mnz[i] - oz) * iz, tz1 = (mxz[i] - oz) * iz;
+                    const float tx0 = (mnx[i] - lox) * ix, tx1 = (mxx[i] - hox) * ix;
+                    const float ty0 = (mny[i] - loy) * iy, ty1 = (mxy[i] - hoy) * iy;
+                    const float tz0 = (mnz[i] - loz) * iz, tz1 = (mxz[i] - hoz) * iz;
                     const float a =
                         fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
                     const float b =
@@ -551,7 +574,7 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
                             bp = prim;
                             found = true;
                         }
-                    } else if ((m.t < bt) || (m.t == bt && prim < bp)) {
+                    } else if (lowest_first(m.t, prim, bt, bp)) {
                         bt = m.t;
                         bu = m.u;
                         bv = m.v;
@@ -573,6 +596,276 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
             node = -1;
             lcount = 0;
             if (!p.work) rid += stride;
+        }
+    }
+    if (STATS) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(p.stats), (unsigned long long)n_nodes);
+        atomicAdd(reinterpret_cast<unsigned long long*>(p.stats) + 1, (unsigned long long)n_tris);
+    }
+}
+
+// ---- compressed 8-wide tree (DevCw8Node) ----------------------------------
+//
+// The stack holds groups rather than single nodes:
+//   node group      y >= 2^24: bits 24..31 the node's interior hits not yet
+//                   visited, by key (slot ^ ray octant); bits 16..23 its
+//                   interior-slot mask; bits 0..15 a lower bound on their
+//                   entry distances (a group beyond t_best is dropped
+//                   unvisited); x its first child;
+//   triangle group  y < 2^24: bits 0..7 the triangle hits not yet tested, by
+//                   slot; bits 8..15 the triangle-slot mask; x the node's
+//                   first triangle.
+// Each iteration visits at most one node (the lowest key of the node group
+// in hand, which pushes the rest of that group) and tests at most one
+// triangle.  A slot is entered when max(t_near, t_min) <= min(t_far, t_best)
+// with t_near lowered and t_far raised by 2^-20 of their magnitude; with the
+// grid step kept free around every box, this never rejects a box that the
+// exact slab test would enter.  Where a grid step times 1/d overflows, the
+// plane distances are infinite or NaN (0 * inf); a NaN drops out of the
+// min/max, which can only enter more slots.  Which triangles are tested can therefore
+// only grow, and the lowest (t, id) is that of brute force.
+constexpr uint32_t kNodeGroup = 1u << 24;
+constexpr float kBoxEps = 0x1p-20f;
+
+// bit s of an 8-bit mask moves to bit s ^ oct
+__device__ __forceinline__ uint32_t oct_permute(uint32_t m, uint32_t oct) {
+    const uint32_t a = ((m & 0x55u) << 1) | ((m >> 1) & 0x55u);
+    m = (oct & 1u) ? a : m;
+    const uint32_t b = ((m & 0x33u) << 2) | ((m >> 2) & 0x33u);
+    m = (oct & 2u) ? b : m;
+    const uint32_t c = ((m & 0x0Fu) << 4) | ((m >> 4) & 0x0Fu);
+    return (oct & 4u) ? c : m;
+}
+
+// The LDS part of the group stack is accessed with explicit DS instructions,
+// so that the compiler cannot fold them with the private-array accesses into
+// generic (flat) ones.  addr: LDS byte address, 8-byte aligned.
+__device__ __forceinline__ void lds_put(uint32_t addr, uint32_t x, uint32_t y) {
+    const uint64_t v = (uint64_t(y) << 32) | x;
+    asm volatile("ds_write_b64 %0, %1" : : "v"(addr), "v"(v));
+}
+
+__device__ __forceinline__ uint2 lds_get(uint32_t addr) {
+    uint64_t v;
+    asm volatile("ds_read_b64 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(v) : "v"(addr));
+    return make_uint2(uint32_t(v), uint32_t(v >> 32));
+}
+
+// The low 16 bits of a node group: the upper half of the nearest entry
+// distance among its children (non-negative, so the truncation rounds down
+// and the bound stays a lower bound).
+__device__ __forceinline__ uint32_t bound_bits(float t) {
+    return __float_as_uint(fmaxf(t, 0.0f)) >> 16;
+}
+
+__device__ __forceinline__ float group_bound(uint32_t y) { return __uint_as_float(y << 16); }
+
+__device__ __forceinline__ float qbyte(uint32_t w, int i) {
+    return float((w >> ((i & 3) * 8)) & 0xFFu);
+}
+
+template <bool ANY, bool STATS, bool BATCH>
+__global__ void __launch_bounds__(kMaxBlock) cw8_kernel(TraceParams p) {
+    if (p.gate && *p.gate) return;  // the tree this launch would read is incomplete
+    extern __shared__ int32_t s_stack[];  // p.lds_depth groups (8 B) per lane
+    // LDS byte address of this lane's first entry (the low half of a shared
+    // pointer) and the step between its entries
+    const uint32_t lbase = uint32_t(reinterpret_cast<uintptr_t>(s_stack)) + threadIdx.x * 8u;
+    const uint32_t lstep = blockDim.x * 8u;
+    const int ldepth = int(p.lds_depth);
+    uint2 deep[RTBVH_MAX_STACK];
+    int sp = 0;
+    uint64_t n_nodes = 0, n_tris = 0;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    uint64_t rid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint32_t refill = p.refill ? p.refill : 1u;
+    float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f,
+          iz = 0.f, tmin = 0.f;
+    float bt = 0.f, bu = 0.f, bv = 0.f;
+    uint32_t bp = RTBVH_MISS, oct = 0;
+    uint32_t ngx = 0, ngy = 0;  // node group in hand
+    uint32_t tgx = 0, tgy = 0;  // triangle group in hand
+    bool active = false;
+    const auto push = [&](uint32_t x, uint32_t y) -> bool {
+        if (sp < ldepth) {
+            lds_put(lbase + uint32_t(sp) * lstep, x, y);
+        } else if (sp < RTBVH_MAX_STACK) {
+            deep[sp - ldepth] = make_uint2(x, y);
+        } else {
+            return false;
+        }
+        ++sp;
+        return true;
+    };
+    for (;;) {
+        // ---- refill: idle lanes that still have rays take them together
+        const bool can_take = !active && rid < p.ray_count;
+        const uint64_t takers = __ballot(can_take);
+        const uint64_t busy = __ballot(active);
+        if (takers == 0 && busy == 0) break;  // wave-uniform: no ray left for any lane
+        if (can_take && (uint32_t(__popcll(takers)) >= refill || busy == 0)) {
+            const float4* r4 = reinterpret_cast<const float4*>(p.rays + rid);
+            const float4 ra = r4[0];
+            const float4 rb = r4[1];
+            ox = ra.x;
+            oy = ra.y;
+            oz = ra.z;
+            tmin = ra.w;
+            dx = rb.x;
+            dy = rb.y;
+            dz = rb.z;
+            bt = rb.w;
+            ix = 1.0f / dx;  // Ray::new, src/ray.rs:166-182
+            iy = 1.0f / dy;
+            iz = 1.0f / dz;
+            bu = 0.f;
+            bv = 0.f;
+            bp = RTBVH_MISS;
+            const bool nan = (ox != ox) || (oy != oy) || (oz != oz) || (dx != dx) ||
+                             (dy != dy) || (dz != dz);
+            if (nan || p.node_count == 0) {
+                if (ANY) {
+                    p.occluded[rid] = 0;
+                } else {
+                    reinterpret_cast<float4*>(p.hits)[rid] =
+                        make_float4(bt, 0.f, 0.f, __uint_as_float(RTBVH_MISS));
+                }
+                rid += stride;
+            } else {
+                active = true;
+                oct = (__float_as_uint(ix) >> 31) | ((__float_as_uint(iy) >> 31) << 1) |
+                      ((__float_as_uint(iz) >> 31) << 2);
+                // the root as a node group of one interior slot (slot 0), bound 0
+                ngx = BATCH ? scene_of_ray(p.ray_offsets, p.scene_count, uint32_t(rid)) : 0u;
+                ngy = (1u << (24 + oct)) | (1u << 16);
+                tgy = 0;
+                sp = 0;
+            }
+        }
+        if (!active) continue;
+        bool overflow = false;
+        bool found = false;
+        // ---- a group in hand whose children all start beyond t_best is done
+        if (ngy >= kNodeGroup && group_bound(ngy) > bt) ngy = 0;
+        // ---- no node group in hand: pop until one that can still hold a
+        // hit at or before t_best, taking a triangle group on the way when
+        // none is in hand (one that is stops the search)
+        while (ngy < kNodeGroup && sp > 0) {
+            uint2 g;
+            if (sp - 1 < ldepth)
+                g = lds_get(lbase + uint32_t(sp - 1) * lstep);
+            else
+                g = deep[sp - 1 - ldepth];
+            if (g.y >= kNodeGroup) {
+                --sp;
+                if (group_bound(g.y) <= bt) {
+                    ngx = g.x;
+                    ngy = g.y;
+                }
+            } else if ((tgy & 0xFFu) == 0u) {
+                tgx = g.x;
+                tgy = g.y;
+                --sp;
+            } else {
+                break;
+            }
+        }
+        // ---- node step: the nearest-keyed child of the group in hand
+        if (ngy >= kNodeGroup) {
+            const uint32_t key = uint32_t(__builtin_ctz(ngy >> 24));
+            ngy &= ~(1u << (24 + key));
+            const uint32_t slot = key ^ oct;
+            const uint32_t idx = ngx + uint32_t(__popc((ngy >> 16) & ((1u << slot) - 1u)));
+            if (ngy >= kNodeGroup) overflow = !push(ngx, ngy);
+            ngy = 0;
+            if (idx < p.node_count) {
+                if (STATS) ++n_nodes;
+                const uint4* q = reinterpret_cast<const uint4*>(p.cnodes + idx);
+                const uint4 h0 = q[0], h1 = q[1], qa = q[2], qb = q[3], qc = q[4];
+                // grid step times 1/d, and the grid origin's distance
+                const float sx = __uint_as_float((h0.w & 0xFFu) << 23) * ix;
+                const float sy = __uint_as_float(((h0.w >> 8) & 0xFFu) << 23) * iy;
+                const float sz = __uint_as_float(((h0.w >> 16) & 0xFFu) << 23) * iz;
+                const float bx = (__uint_as_float(h0.x) - ox) * ix;
+                const float by = (__uint_as_float(h0.y) - oy) * iy;
+                const float bz = (__uint_as_float(h0.z) - oz) * iz;
+                // near and far planes: low planes first along a positive direction
+                const bool nx = (oct & 1u) != 0u, ny = (oct & 2u) != 0u, nz = (oct & 4u) != 0u;
+                const uint32_t nx0 = nx ? qb.z : qa.x, nx1 = nx ? qb.w : qa.y;
+                const uint32_t fx0 = nx ? qa.x : qb.z, fx1 = nx ? qa.y : qb.w;
+                const uint32_t ny0 = ny ? qc.x : qa.z, ny1 = ny ? qc.y : qa.w;
+                const uint32_t fy0 = ny ? qa.z : qc.x, fy1 = ny ? qa.w : qc.y;
+                const uint32_t nz0 = nz ? qc.z : qb.x, nz1 = nz ? qc.w : qb.y;
+                const uint32_t fz0 = nz ? qb.x : qc.z, fz1 = nz ? qb.y : qc.w;
+                const uint32_t imask = h0.w >> 24, lmask = h1.z & 0xFFu;
+                uint32_t hm = 0;
+                float gmin = 3.0e38f;  // nearest entry among the interior hits
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const float tnx = fmaf(qbyte(i < 4 ? nx0 : nx1, i), sx, bx);
+                    const float tfx = fmaf(qbyte(i < 4 ? fx0 : fx1, i), sx, bx);
+                    const float tny = fmaf(qbyte(i < 4 ? ny0 : ny1, i), sy, by);
+                    const float tfy = fmaf(qbyte(i < 4 ? fy0 : fy1, i), sy, by);
+                    const float tnz = fmaf(qbyte(i < 4 ? nz0 : nz1, i), sz, bz);
+                    const float tfz = fmaf(qbyte(i < 4 ? fz0 : fz1, i), sz, bz);
+                    const float tn = fmaxf(fmaxf(tnx, tny), tnz);
+                    const float tf = fminf(fminf(tfx, tfy), tfz);
+                    const float tnc = fmaf(fabsf(tn), -kBoxEps, tn);
+                    const float tfc = fmaf(fabsf(tf), kBoxEps, tf);
+                    const float en = fmaxf(tnc, tmin);
+                    const bool hit = en <= fminf(tfc, bt);
+                    hm |= hit ? (1u << i) : 0u;
+                    gmin = (hit && ((imask >> i) & 1u)) ? fminf(gmin, en) : gmin;
+                }
+                const uint32_t ih = hm & imask, th = hm & lmask;
+                if (ih) {
+                    ngx = h1.x;
+                    ngy = (oct_permute(ih, oct) << 24) | (imask << 16) | bound_bits(gmin);
+                }
+                if (th) {
+                    if ((tgy & 0xFFu) != 0u && !overflow) overflow = !push(tgx, tgy);
+                    tgx = h1.y;
+                    tgy = (lmask << 8) | th;
+                }
+            }
+        }
+        // ---- triangle step: one triangle of the group in hand
+        if ((tgy & 0xFFu) != 0u) {
+            const uint32_t slot = uint32_t(__builtin_ctz(tgy & 0xFFu));
+            tgy &= ~(1u << slot);
+            const uint32_t pos = tgx + uint32_t(__popc((tgy >> 8) & ((1u << slot) - 1u)));
+            if (pos < p.index_count) {
+                if (STATS) ++n_tris;
+                uint32_t prim;
+                const MtResult m = mt_prepared(p.tris, pos, ox, oy, oz, dx, dy, dz, tmin, &prim);
+                if (prim != RTBVH_MISS && m.ok) {
+                    if (ANY) {
+                        if (m.t < bt) {
+                            bp = prim;
+                            found = true;
+                        }
+                    } else if (lowest_first(m.t, prim, bt, bp)) {
+                        bt = m.t;
+                        bu = m.u;
+                        bv = m.v;
+                        bp = prim;
+                    }
+                }
+            }
+        }
+        if (overflow) atomicOr(p.status, kStatusStackOverflow);
+        if (found || overflow || (ngy < kNodeGroup && (tgy & 0xFFu) == 0u && sp == 0)) {
+            if (ANY) {
+                p.occluded[rid] = (bp != RTBVH_MISS) ? 1 : 0;
+            } else {
+                reinterpret_cast<float4*>(p.hits)[rid] =
+                    make_float4(overflow ? __builtin_nanf("") : bt, bu, bv,
+                                __uint_as_float(overflow ? RTBVH_MISS : bp));
+            }
+            active = false;
+            ngy = 0;
+            tgy = 0;
+            rid += stride;
         }
     }
     if (STATS) {
@@ -617,9 +910,11 @@ int current_dev() {
     return dev;
 }
 
-template <bool ANY, bool STATS, bool PREP, bool BATCH = false, bool ITEM = false>
+template <bool ANY, bool STATS, bool PREP, bool BATCH = false, bool ITEM = false, bool CW8 = false>
 hipError_t launch_fast(const TraceParams& p0, hipStream_t stream) {
-    auto k = ITEM ? mbvh_item_kernel<ANY, STATS, BATCH> : mbvh_fast_kernel<ANY, STATS, PREP, BATCH>;
+    auto k = CW8 ? cw8_kernel<ANY, STATS, BATCH>
+                 : (ITEM ? mbvh_item_kernel<ANY, STATS, BATCH>
+                         : mbvh_fast_kernel<ANY, STATS, PREP, BATCH>);
     const int dev = current_dev();
     const TraceConfig cfg = g_config_set[dev] ? g_config[dev] : default_trace_config();
     // resident blocks per CU for this shape, cached per (device, shape)
@@ -646,7 +941,7 @@ hipError_t launch_fast(const TraceParams& p0, hipStream_t stream) {
     TraceParams p = p0;
     p.lds_depth = cfg.lds_depth;
     p.refill = cfg.refill;
-    if (ITEM && cfg.schedule == 1) {
+    if (ITEM && !CW8 && cfg.schedule == 1) {
         if (!p.work) {
             uint32_t* ring = work_ring_ptr();
             if (ring == nullptr) return hipErrorInvalidSymbol;
@@ -684,6 +979,15 @@ TraceConfig get_trace_config() {
 }
 
 hipError_t launch_trace_fast(const TraceParams& p, bool any_hit, hipStream_t stream) {
+    if (p.cnodes) {  // compressed 8-wide tree
+        if (!p.tris || (p.stats && (any_hit || p.ray_offsets))) return hipErrorInvalidValue;
+        if (p.ray_offsets)
+            return any_hit ? launch_fast<true, false, true, true, false, true>(p, stream)
+                           : launch_fast<false, false, true, true, false, true>(p, stream);
+        if (p.stats) return launch_fast<false, true, true, false, false, true>(p, stream);
+        return any_hit ? launch_fast<true, false, true, false, false, true>(p, stream)
+                       : launch_fast<false, false, true, false, false, true>(p, stream);
+    }
     // the one-item kernel over leaf-ordered triangles, unless the leaf tags
     // cannot address every index entry
     const bool item = p.tris && p.small_leaves && p.index_count <= kLeafFirstMask &&

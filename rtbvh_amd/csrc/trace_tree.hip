@@ -30,6 +30,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr uint32_t kLevelGrid = 2048;  // persistent blocks per level kernel
 constexpr int kMaxLeafLimit = 8;
+constexpr int kWide = 8;  // slots of a compressed node
 
 struct QItem {
     uint32_t b;  // binary node
@@ -130,6 +131,7 @@ struct LevelArgs {
     uint32_t scenes;
     uint32_t max_leaf;
     DevMbvhNode* wnodes;
+    DevCw8Node* cnodes;  // width 8: compressed nodes instead of wnodes
     uint32_t wcap;
     float4* tris;
     uint32_t* counters;  // [0] wide nodes, [1] triangles, [2] fail
@@ -267,6 +269,267 @@ __device__ void make_node(const LevelArgs& a, uint32_t b, uint32_t w, QItem* qou
     a.wnodes[w] = m;
 }
 
+// ---- compressed 8-wide nodes -------------------------------------------
+
+// A slot candidate: a binary interior node, a binary leaf of several
+// primitives (opened into them when they fit), or one primitive (an index
+// position, with the bounds of its triangle).
+struct Cand {
+    int32_t id;
+    int32_t kind;  // 0 interior, 1 leaf of n > 1 primitives, 2 primitive
+    uint32_t n;
+    float lo[3], hi[3];
+};
+
+__device__ bool cand_prim(const LevelArgs& a, uint32_t pos, Cand& c) {
+    if (pos >= a.index_count) return false;
+    c.id = int32_t(pos);
+    c.kind = 2;
+    c.n = 1;
+    const uint32_t prim = a.indices[pos];
+    if (prim < a.prim_count) {
+        const float* v = a.verts + size_t(prim) * 9;
+        for (int k = 0; k < 3; ++k) {
+            c.lo[k] = fminf(fminf(v[k], v[3 + k]), v[6 + k]);
+            c.hi[k] = fmaxf(fmaxf(v[k], v[3 + k]), v[6 + k]);
+        }
+    } else {  // stored as a triangle that no ray hits
+        for (int k = 0; k < 3; ++k) c.lo[k] = c.hi[k] = 0.0f;
+    }
+    return true;
+}
+
+// Appends binary node i as a candidate (an empty leaf adds nothing).
+__device__ bool cand_binary(const LevelArgs& a, int32_t i, Cand* c, int& ns) {
+    if (i < 0 || uint32_t(i) >= a.node_count) return false;
+    DevAabb n;
+    ldn(a.nodes + i, n);
+    if (n.extra1 == 0) return true;
+    if (ns >= kWide) return false;
+    if (n.extra1 == 1) return cand_prim(a, uint32_t(n.extra2), c[ns++]);
+    Cand& d = c[ns++];
+    d.id = i;
+    d.kind = n.extra1 > 1 ? 1 : 0;
+    d.n = n.extra1 > 1 ? uint32_t(n.extra1) : 0u;
+    for (int k = 0; k < 3; ++k) {
+        d.lo[k] = n.min[k];
+        d.hi[k] = n.max[k];
+    }
+    return n.extra2 >= 0;
+}
+
+__device__ __forceinline__ float cand_area(const Cand& c) {
+    const float dx = c.hi[0] - c.lo[0], dy = c.hi[1] - c.lo[1], dz = c.hi[2] - c.lo[2];
+    return (dx + dy) * dz + dx * dy;
+}
+
+// Grid of one axis: origin p at least one step below lo, step 2^(e-127) a
+// power of two with (hi - lo) / step <= 252 and step >= 2^-22 * max(|lo|,
+// |hi|) (so the spare step outlasts float rounding in the traversal).
+__device__ bool grid_axis(float lo, float hi, float& p, uint32_t& ebyte, double& step) {
+    const double ext = double(hi) - double(lo);
+    const double mag = fmax(fabs(double(lo)), fabs(double(hi)));
+    double need = fmax(ext / 252.0, mag * 0x1p-22);
+    need = fmax(need, 0x1p-100);
+    int k = 0;
+    const double m = frexp(need, &k);  // need = m 2^k, m in [0.5, 1)
+    if (m == 0.5) --k;                 // need is itself a power of two
+    if (k + 127 < 1 || k + 127 > 254) return false;
+    ebyte = uint32_t(k + 127);
+    step = ldexp(1.0, k);
+    const double pd = double(lo) - step;
+    float pf = float(pd);
+    if (double(pf) > pd) pf = nextafterf(pf, -3.4028235e38f);
+    p = pf;
+    return true;
+}
+
+__device__ __forceinline__ uint32_t quant_lo(float v, float p, double step) {
+    const double q = floor((double(v) - double(p)) / step) - 1.0;
+    return q <= 0.0 ? 0u : (q >= 255.0 ? 255u : uint32_t(q));
+}
+
+__device__ __forceinline__ uint32_t quant_hi(float v, float p, double step) {
+    const double q = ceil((double(v) - double(p)) / step) + 1.0;
+    return q <= 0.0 ? 0u : (q >= 255.0 ? 255u : uint32_t(q));
+}
+
+// One compressed node from queue item (b, w).  Slots are opened by largest
+// area (as the 4-wide tree does) up to eight, then placed so that slot s
+// lies towards the octant s (bit k of s set: the + side of axis k) of the
+// node's centre, greedily by alignment; a ray whose direction has signs O
+// visits keys s ^ O in increasing order, the near side first.
+__device__ void make_node8(const LevelArgs& a, uint32_t b, uint32_t w, QItem* qout,
+                           uint32_t* qout_count) {
+    Cand c[kWide];
+    int ns = 0;
+    DevAabb bn;
+    ldn(a.nodes + b, bn);
+    bool ok = true;
+    if (bn.extra1 >= 0) {  // a root that is a leaf, or a leaf of several primitives
+        if (bn.extra1 > kWide || bn.extra2 < 0) ok = false;
+        for (int32_t j = 0; ok && j < bn.extra1; ++j)
+            ok = cand_prim(a, uint32_t(bn.extra2 + j), c[ns++]);
+    } else {
+        ok = cand_binary(a, bn.extra2, c, ns) && cand_binary(a, bn.extra2 + 1, c, ns);
+        while (ok) {
+            int best = -1;
+            float best_area = -1.0f;
+            for (int k = 0; k < ns; ++k) {
+                const bool fits = c[k].kind == 0 ? ns + 1 <= kWide
+                                                 : (c[k].kind == 1 && ns - 1 + int(c[k].n) <= kWide);
+                if (!fits) continue;
+                const float ar = cand_area(c[k]);
+                if (!(ar <= best_area)) {
+                    best_area = ar;
+                    best = k;
+                }
+            }
+            if (best < 0) break;
+            const Cand open = c[best];
+            c[best] = c[ns - 1];
+            --ns;
+            DevAabb on;
+            ldn(a.nodes + open.id, on);
+            if (open.kind == 0) {
+                ok = cand_binary(a, on.extra2, c, ns) && cand_binary(a, on.extra2 + 1, c, ns);
+            } else {
+                for (uint32_t j = 0; ok && j < open.n; ++j)
+                    ok = ns < kWide && cand_prim(a, uint32_t(on.extra2) + j, c[ns++]);
+            }
+        }
+    }
+    if (!ok || ns == 0) {
+        atomicOr(&a.counters[2], 1u);
+        return;
+    }
+    // bounds of the node and the centre of every candidate
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = c[0].lo[k];
+        hi[k] = c[0].hi[k];
+    }
+    for (int j = 1; j < ns; ++j)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fminf(lo[k], c[j].lo[k]);
+            hi[k] = fmaxf(hi[k], c[j].hi[k]);
+        }
+    for (int k = 0; k < 3; ++k)
+        if (!(isfinite(lo[k]) && isfinite(hi[k]) && lo[k] <= hi[k])) {
+            atomicOr(&a.counters[2], 2u);
+            return;
+        }
+    // octant placement
+    int slot_cand[kWide];
+    for (int s = 0; s < kWide; ++s) slot_cand[s] = -1;
+    uint32_t placed = 0;
+    for (int it = 0; it < ns; ++it) {
+        float best = -3.4e38f;
+        int bk = -1, bs = -1;
+        for (int k = 0; k < ns; ++k) {
+            if (placed & (1u << k)) continue;
+            float off[3];
+            for (int ax = 0; ax < 3; ++ax)
+                off[ax] = (c[k].lo[ax] + c[k].hi[ax]) - (lo[ax] + hi[ax]);
+            for (int s = 0; s < kWide; ++s) {
+                if (slot_cand[s] >= 0) continue;
+                const float sc = ((s & 1) ? off[0] : -off[0]) + ((s & 2) ? off[1] : -off[1]) +
+                                 ((s & 4) ? off[2] : -off[2]);
+                if (bk < 0 || sc > best) {
+                    best = sc;
+                    bk = k;
+                    bs = s;
+                }
+            }
+        }
+        slot_cand[bs] = bk;
+        placed |= 1u << bk;
+    }
+    // grid, children and triangles
+    float p[3];
+    uint32_t eb[3];
+    double step[3];
+    for (int k = 0; k < 3; ++k)
+        if (!grid_axis(lo[k], hi[k], p[k], eb[k], step[k])) {
+            atomicOr(&a.counters[2], 2u);
+            return;
+        }
+    uint32_t n_inner = 0, n_tris = 0;
+    for (int j = 0; j < ns; ++j) {
+        if (c[j].kind == 2)
+            ++n_tris;
+        else
+            ++n_inner;
+    }
+    const uint32_t wbase = n_inner ? atomicAdd(&a.counters[0], n_inner) : 0u;
+    const uint32_t tbase = n_tris ? atomicAdd(&a.counters[1], n_tris) : 0u;
+    const uint32_t qbase = n_inner ? atomicAdd(qout_count, n_inner) : 0u;
+    if (wbase + n_inner > a.wcap || tbase + n_tris > a.index_count) {
+        atomicOr(&a.counters[2], 1u);
+        return;
+    }
+    DevCw8Node m;
+    m.px = p[0];
+    m.py = p[1];
+    m.pz = p[2];
+    uint32_t imask = 0, lmask = 0;
+    uint32_t qlo[3][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}};
+    uint32_t qhi[3][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}};
+    uint32_t wi = wbase, ti = tbase, qi = qbase;
+    for (int s = 0; s < kWide; ++s) {
+        const int k = slot_cand[s];
+        const int word = s >> 2, sh = (s & 3) * 8;
+        if (k < 0) {  // empty: an inverted box, never entered
+            for (int ax = 0; ax < 3; ++ax) qlo[ax][word] |= 255u << sh;
+            continue;
+        }
+        for (int ax = 0; ax < 3; ++ax) {
+            qlo[ax][word] |= quant_lo(c[k].lo[ax], p[ax], step[ax]) << sh;
+            qhi[ax][word] |= quant_hi(c[k].hi[ax], p[ax], step[ax]) << sh;
+        }
+        if (c[k].kind == 2) {
+            lmask |= 1u << s;
+            write_tri(a.tris, ti, a.verts, a.prim_count, a.indices[c[k].id], a.scene_off,
+                      a.scenes);
+            ++ti;
+        } else {
+            imask |= 1u << s;
+            qout[qi].b = uint32_t(c[k].id);
+            qout[qi].w = wi;
+            ++wi;
+            ++qi;
+        }
+    }
+    m.emask = eb[0] | (eb[1] << 8) | (eb[2] << 16) | (imask << 24);
+    m.child_base = wbase;
+    m.tri_base = tbase;
+    m.lmask = lmask;
+    m.pad = 0;
+    for (int word = 0; word < 2; ++word) {
+        m.qlo_x[word] = qlo[0][word];
+        m.qlo_y[word] = qlo[1][word];
+        m.qlo_z[word] = qlo[2][word];
+        m.qhi_x[word] = qhi[0][word];
+        m.qhi_y[word] = qhi[1][word];
+        m.qhi_z[word] = qhi[2][word];
+    }
+    a.cnodes[w] = m;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    level8_kernel(LevelArgs a, const QItem* qin, const uint32_t* qin_count, QItem* qout,
+                  uint32_t* qout_count) {
+    const uint32_t q = *qin_count;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < q; i += gridDim.x * kBlock) {
+        const QItem it = qin[i];
+        if (it.b >= a.node_count || it.w >= a.wcap) {
+            atomicOr(&a.counters[2], 1u);
+            continue;
+        }
+        make_node8(a, it.b, it.w, qout, qout_count);
+    }
+}
+
 __global__ void __launch_bounds__(kBlock)
     level_kernel(LevelArgs a, const QItem* qin, const uint32_t* qin_count, QItem* qout,
                  uint32_t* qout_count) {
@@ -312,7 +575,7 @@ __global__ void finish_kernel(const uint32_t* last_q, uint32_t* counters, uint32
 }  // namespace
 
 struct TraceTreePlan {
-    uint32_t count = 0, roots = 1, n = 0, wcap = 0, max_leaf = 1, levels = 0;
+    uint32_t count = 0, roots = 1, n = 0, wcap = 0, max_leaf = 1, levels = 0, width = 4;
     uint32_t* cnt = nullptr;
     QItem* q[2] = {nullptr, nullptr};
     uint32_t* words = nullptr;  // [0..1] queue counts, [2..4] counters
@@ -344,6 +607,7 @@ LevelArgs level_args(const TraceTreePlan* t, const DeviceBvh& tree, const float*
     a.scenes = scenes;
     a.max_leaf = t->max_leaf;
     a.wnodes = out->nodes;
+    a.cnodes = out->cnodes;
     a.wcap = t->wcap;
     a.tris = out->tris;
     a.counters = t->words + 2;
@@ -354,8 +618,9 @@ LevelArgs level_args(const TraceTreePlan* t, const DeviceBvh& tree, const float*
 hipError_t enqueue_setup(const TraceTreePlan* t, const DeviceBvh& tree, hipStream_t stream) {
     const uint32_t count = t->count;
     const uint32_t g = (count + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(capped_counts_kernel, dim3(g), dim3(kBlock), 0, stream, tree.nodes, count,
-                       t->max_leaf, t->cnt);
+    if (t->width == 4)  // the compressed tree never collapses subtrees into leaves
+        hipLaunchKernelGGL(capped_counts_kernel, dim3(g), dim3(kBlock), 0, stream, tree.nodes,
+                           count, t->max_leaf, t->cnt);
     const uint32_t gr = (t->roots + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(roots_kernel, dim3(gr), dim3(kBlock), 0, stream, t->roots, t->q[0],
                        t->words + 0, t->words + 2);
@@ -366,16 +631,24 @@ void enqueue_level(const TraceTreePlan* t, const LevelArgs& a, uint32_t level, u
                    hipStream_t stream) {
     const int in = level & 1, outq = in ^ 1;
     hipLaunchKernelGGL(clear_word_kernel, dim3(1), dim3(1), 0, stream, t->words + outq);
-    hipLaunchKernelGGL(level_kernel, dim3(grid), dim3(kBlock), 0, stream, a, t->q[in],
-                       t->words + in, t->q[outq], t->words + outq);
+    if (t->width == 8)
+        hipLaunchKernelGGL(level8_kernel, dim3(grid), dim3(kBlock), 0, stream, a, t->q[in],
+                           t->words + in, t->q[outq], t->words + outq);
+    else
+        hipLaunchKernelGGL(level_kernel, dim3(grid), dim3(kBlock), 0, stream, a, t->q[in],
+                           t->words + in, t->q[outq], t->words + outq);
 }
 
 }  // namespace
 
-ResultCode trace_tree_plan_create(const DeviceBvh& tree, uint32_t max_leaf,
+ResultCode trace_tree_plan_create(const DeviceBvh& tree, uint32_t max_leaf, uint32_t width,
                                   TraceTreePlan** out_plan, DeviceMbvh* out) {
-    if (max_leaf < 1 || max_leaf > uint32_t(kMaxLeafLimit)) {
-        set_error("max_leaf must be 1..%d", kMaxLeafLimit);
+    if (width != 4 && width != 8) {
+        set_error("trace tree width must be 4 or 8");
+        return InvalidArgument;
+    }
+    if (max_leaf < 1 || max_leaf > uint32_t(kMaxLeafLimit) || (width == 8 && max_leaf != 1)) {
+        set_error("max_leaf must be 1..%d (1 for the 8-wide tree)", kMaxLeafLimit);
         return InvalidArgument;
     }
     TraceTreePlan* t = new TraceTreePlan();
@@ -383,6 +656,7 @@ ResultCode trace_tree_plan_create(const DeviceBvh& tree, uint32_t max_leaf,
     t->roots = tree.root_count;
     t->n = tree.prim_count;
     t->max_leaf = max_leaf;
+    t->width = width;
     t->wcap = tree.prim_count + tree.root_count;
     hipError_t e;
     if ((e = hipMalloc(&t->cnt, size_t(t->count) * 4)) != hipSuccess ||
@@ -393,9 +667,13 @@ ResultCode trace_tree_plan_create(const DeviceBvh& tree, uint32_t max_leaf,
         return hip_fail(e, "trace tree workspace");
     }
     DeviceMbvh res;
-    if ((e = hipMalloc(&res.nodes, size_t(t->wcap) * sizeof(DevMbvhNode))) != hipSuccess ||
+    if ((width == 4 &&
+         (e = hipMalloc(&res.nodes, size_t(t->wcap) * sizeof(DevMbvhNode))) != hipSuccess) ||
+        (width == 8 &&
+         (e = hipMalloc(&res.cnodes, size_t(t->wcap) * sizeof(DevCw8Node))) != hipSuccess) ||
         (e = hipMalloc(&res.tris, size_t(t->n) * 3 * sizeof(float4))) != hipSuccess) {
         if (res.nodes) (void)hipFree(res.nodes);
+        if (res.cnodes) (void)hipFree(res.cnodes);
         trace_tree_plan_destroy(t);
         return hip_fail(e, "trace tree");
     }
@@ -437,7 +715,8 @@ ResultCode trace_tree_build(TraceTreePlan* t, const DeviceBvh& tree, const float
         (e = hipStreamSynchronize(stream)) != hipSuccess)
         return hip_fail(e, "trace tree finish");
     if (host_words[4] != 0) {
-        set_error("trace tree construction failed (inconsistent binary tree?)");
+        set_error(host_words[4] & 2u ? "trace tree construction failed (non-finite bounds)"
+                                     : "trace tree construction failed (inconsistent binary tree?)");
         return Error;
     }
     t->levels = level;

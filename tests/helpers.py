@@ -59,3 +59,38 @@ def to_device(arr, torch, dtype=None):
     if dtype is not None:
         t = t.view(dtype)
     return t.to("cuda")
+
+
+def flat_grid(g=40, cell=0.05, offset=0.0):
+    """2 g^2 triangles tiling [0, g cell]^2 in the plane z = offset, shifted by
+    offset in x and y (a scene with one axis of zero extent)."""
+    xs, ys = np.meshgrid(np.arange(g, dtype=np.float32), np.arange(g, dtype=np.float32))
+    xs, ys = xs.reshape(-1) * np.float32(cell), ys.reshape(-1) * np.float32(cell)
+    z = np.zeros_like(xs)
+    d = np.float32(cell)
+    t1 = np.stack([xs, ys, z, xs + d, ys, z, xs, ys + d, z], 1)
+    t2 = np.stack([xs + d, ys, z, xs + d, ys + d, z, xs, ys + d, z], 1)
+    verts = (np.concatenate([t1, t2]) + np.float32(offset)).astype(np.float32)
+    verts[:, 2::3] = np.float32(offset)
+    return verts
+
+
+def flat_grid_rays(verts, n, seed=11):
+    """Rays for flat_grid scenes: a quarter straight down or up (zero x, y
+    direction components), a quarter parallel to the plane, the rest oblique
+    (some with a zero x component); every 5th with t_min 0.5, every 7th with
+    t_max 1.0 -- exactly the distance of the axis rays to the plane."""
+    rng = np.random.default_rng(seed)
+    lo, hi = verts.reshape(-1, 3).min(0), verts.reshape(-1, 3).max(0)
+    plane = np.float32(verts[0, 2])
+    o = rng.uniform(lo - 0.2, hi + 0.2, (n, 3)).astype(np.float32)
+    o[:, 2] = plane + rng.choice([-1.0, 1.0], n).astype(np.float32)
+    dirs = rng.normal(size=(n, 3)).astype(np.float32)
+    dirs[0::4] = np.float32(0.0)
+    dirs[0::4, 2] = -np.sign(o[0::4, 2] - plane)
+    dirs[1::4, 2] = np.float32(0.0)
+    dirs[2::8, 0] = np.float32(0.0)
+    dirs[2::8, 2] = -np.sign(o[2::8, 2] - plane) * np.abs(dirs[2::8, 2])
+    tmin = np.where(np.arange(n) % 5 == 0, np.float32(0.5), np.float32(1e-4)).astype(np.float32)
+    tmax = np.where(np.arange(n) % 7 == 0, np.float32(1.0), np.float32(1e30)).astype(np.float32)
+    return np.concatenate([o, tmin[:, None], dirs, tmax[:, None]], 1).astype(np.float32)

@@ -14,7 +14,8 @@ import pytest
 
 from oracle import refcpu as ref
 from rtbvh_amd import scenes
-from tests.helpers import as_rays, flat, hits_equal, prep, ray_set, teapot
+from tests.helpers import (as_rays, flat, flat_grid, flat_grid_rays, hits_equal, prep,
+                           ray_set, teapot)
 
 torch = pytest.importorskip("torch")
 import rtbvh_amd as rt  # noqa: E402
@@ -252,3 +253,144 @@ def test_trace_tree_small(n):
     got = bvh.trace_tree(_dev(verts)).intersect(_dev(rays8)).cpu().numpy()
     want = ref.trace_brute(verts, as_rays(rays8), threads=8)
     assert hits_equal(got.view(ref.HIT).reshape(-1), want).all()
+
+
+# ---- compressed 8-wide traversal tree ----------------------------------------
+
+def _decode_cw8(tt):
+    """Host decode of the 80-byte records: grid origin, step per axis,
+    interior / triangle slot masks, bases and the per-slot boxes (float64)."""
+    raw = tt.node_bytes().cpu().numpy().reshape(-1, 80)
+    p = raw[:, 0:12].copy().view(np.float32).astype(np.float64)
+    e = raw[:, 12:15].astype(np.int64)
+    step = np.ldexp(1.0, e - 127)
+    imask = raw[:, 15].astype(np.uint32)
+    child_base = raw[:, 16:20].copy().view(np.uint32)[:, 0]
+    tri_base = raw[:, 20:24].copy().view(np.uint32)[:, 0]
+    lmask = raw[:, 24].astype(np.uint32)
+    q = raw[:, 32:80].reshape(-1, 6, 8).astype(np.float64)  # qlo x y z, qhi x y z
+    lo = p[:, :, None] + q[:, 0:3, :] * step[:, :, None]
+    hi = p[:, :, None] + q[:, 3:6, :] * step[:, :, None]
+    return dict(lo=lo, hi=hi, step=step, imask=imask, lmask=lmask, child_base=child_base,
+                tri_base=tri_base, q=q)
+
+
+def _check_cw8(tt, verts):
+    """Every primitive stored once; every slot box holds its subtree's
+    triangles with at least one grid step to spare; empty slots inverted."""
+    d = _decode_cw8(tt)
+    tri = tt.triangles().cpu().numpy()
+    ids = tri[:, 3].copy().view(np.uint32)
+    assert np.array_equal(np.sort(ids), np.arange(len(verts), dtype=np.uint32))
+    v = verts.reshape(-1, 3, 3).astype(np.float64)
+    tlo, thi = v.min(1), v.max(1)  # per primitive
+    n_nodes = tt.node_count
+    seen_nodes = np.zeros(n_nodes, bool)
+    seen_tris = np.zeros(len(verts), bool)
+
+    def walk(i):  # returns (lo, hi) of the subtree's triangles
+        assert 0 <= i < n_nodes and not seen_nodes[i]
+        seen_nodes[i] = True
+        im, lm = int(d["imask"][i]), int(d["lmask"][i])
+        assert im & lm == 0 and (im | lm) != 0
+        slo, shi = np.full(3, np.inf), np.full(3, -np.inf)
+        ci, ti = int(d["child_base"][i]), int(d["tri_base"][i])
+        for s in range(8):
+            box_lo, box_hi = d["lo"][i, :, s], d["hi"][i, :, s]
+            if im >> s & 1:
+                clo, chi = walk(ci)
+                ci += 1
+            elif lm >> s & 1:
+                assert not seen_tris[ti]
+                seen_tris[ti] = True
+                prim = ids[ti]
+                clo, chi = tlo[prim], thi[prim]
+                ti += 1
+            else:
+                assert np.all(d["q"][i, 0:3, s] == 255) and np.all(d["q"][i, 3:6, s] == 0)
+                continue
+            assert np.all(box_lo <= clo - d["step"][i]), (i, s)
+            assert np.all(box_hi >= chi + d["step"][i]), (i, s)
+            slo, shi = np.minimum(slo, clo), np.maximum(shi, chi)
+        return slo, shi
+
+    import sys
+    sys.setrecursionlimit(max(10000, sys.getrecursionlimit()))
+    walk(0)
+    assert seen_tris.all() and seen_nodes.all()
+
+
+@pytest.mark.parametrize("kind,ppl", [("ploc", None), ("binned", None), ("binned", 4)])
+def test_trace_tree8_structure_and_hits(kind, ppl):
+    """rtbvh_gpu_create_trace_tree8: decoded on the host, the compressed tree
+    holds every primitive once and every slot box contains its subtree with
+    a grid step to spare; traced, it returns the MBVH's lowest-id hit records
+    and occlusion flags bit for bit (= brute force), with fewer node visits
+    per ray than the 4-wide tree's."""
+    print(f"[trace_tree8] kind={kind} ppl={ppl}", flush=True)
+    verts = np.concatenate([scenes.noisy_sphere(90, 60, seed=25),
+                            scenes.random_soup(4000, 0.03, seed=26)])
+    bvh, _, _ = _build(verts, kind, ppl=ppl)
+    m = rt.Mbvh.from_bvh(bvh)
+    rays8 = ray_set(verts, 30000, seed=27, kind="box")
+    rays8[::3, 7] = np.random.default_rng(3).uniform(0.0, 2.0, len(rays8[::3])).astype(np.float32)
+    dv, dr = _dev(verts), _dev(rays8)
+    want = m.intersect(dv, dr).cpu().numpy()
+    want_occ = m.occluded(dv, dr).cpu().numpy()
+    if kind == "ploc":
+        brute = ref.trace_brute(verts, as_rays(rays8), threads=8)
+        assert hits_equal(want.view(ref.HIT).reshape(-1), brute).all()
+    tt = bvh.trace_tree(dv, width=8)
+    assert tt.width == 8 and 0 < tt.node_count <= len(verts)
+    _check_cw8(tt, verts)
+    got, n8, t8 = tt.intersect_stats(dr)
+    got = got.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert np.array_equal(tt.occluded(dr).cpu().numpy(), want_occ)
+    assert np.array_equal(tt.intersect(dr).cpu().numpy().view(np.uint32), want.view(np.uint32))
+    tt.status()
+    _, n4, t4 = bvh.trace_tree(dv).intersect_stats(dr)
+    print(f"  nodes/ray 8-wide {n8 / len(rays8):.2f} vs 4-wide {n4 / len(rays8):.2f}; "
+          f"tris/ray {t8 / len(rays8):.2f} vs {t4 / len(rays8):.2f}", flush=True)
+    assert 0 < n8 < n4
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 9, 17, 100])
+def test_trace_tree8_small(n):
+    print(f"[trace_tree8_small] n={n}", flush=True)
+    verts = scenes.random_soup(n, 0.3, seed=40 + n)
+    bvh, _, _ = _build(verts, "ploc")
+    tt = bvh.trace_tree(_dev(verts), width=8)
+    _check_cw8(tt, verts)
+    rays8 = ray_set(verts, 3000, seed=n, kind="box")
+    got = tt.intersect(_dev(rays8)).cpu().numpy()
+    want = ref.trace_brute(verts, as_rays(rays8), threads=8)
+    assert hits_equal(got.view(ref.HIT).reshape(-1), want).all()
+
+
+@pytest.mark.parametrize("offset", [0.0, 3.0e4])
+def test_trace_tree8_flat_scene_axis_rays(offset):
+    """A flat scene (one axis of zero extent) at or far from the origin, hit
+    by axis-parallel rays (zero direction components: infinite 1/d), rays
+    parallel to the plane and oblique rays, with t_min > 0 and with t_max
+    exactly at the hit distance on some: compressed-tree hits equal brute
+    force bit for bit (a hit at exactly t_max is no hit)."""
+    print(f"[trace_tree8_flat] offset={offset}", flush=True)
+    verts = flat_grid(offset=offset)
+    rays8 = flat_grid_rays(verts, 6000)
+    n = len(rays8)
+    bvh, _, _ = _build(verts, "ploc")
+    dv, dr = _dev(verts), _dev(rays8)
+    tt = bvh.trace_tree(dv, width=8)
+    _check_cw8(tt, verts)
+    want = ref.trace_brute(verts, as_rays(rays8), threads=8)
+    m = rt.Mbvh.from_bvh(bvh)
+    results = {"8-wide": tt.intersect(dr), "4-wide": bvh.trace_tree(dv).intersect(dr),
+               "mbvh": m.intersect(dv, dr)}
+    for name, got in results.items():
+        got = got.cpu().numpy().view(ref.HIT).reshape(-1)
+        same = hits_equal(got, want)
+        for i in np.flatnonzero(~same)[:4]:
+            print(f"  {name} ray {i}: {rays8[i].tolist()} got {got[i]} want {want[i]}", flush=True)
+        assert same.all(), name
+    assert (want["prim"] != 0xFFFFFFFF).sum() > n // 8

@@ -11,7 +11,7 @@ import pytest
 
 from oracle import refcpu as ref
 from rtbvh_amd import scenes
-from tests.helpers import as_rays, hits_equal, prep, ray_set, teapot
+from tests.helpers import as_rays, flat_grid, flat_grid_rays, hits_equal, prep, ray_set, teapot
 
 
 # ----------------------------------------------------------------- Morton --
@@ -399,6 +399,27 @@ def test_traversal_matches_brute_force(builder):
                     assert ok and t == brute["t"][i]
             occ = ref.occluded_mbvh(m, idx, verts, rays, threads=4)
             assert np.array_equal(occ, ref.occluded_brute(verts, rays, threads=4))
+
+
+def test_lowest_id_traversal_rejects_hits_at_t_max():
+    """A hit exactly at the ray's t_max is no hit (the reference's
+    `t < ray.t`, src/builders/spatial_sah.rs:157): the lowest-id traversals
+    of both trees equal brute force on a flat grid whose axis rays land on
+    the plane at exactly t_max, and whose shared edges give exact t ties."""
+    verts = flat_grid()
+    rays = as_rays(flat_grid_rays(verts, 4000))
+    brute = ref.trace_brute(verts, rays, threads=4)
+    open_rays = rays.copy()
+    open_rays["t_max"] = np.float32(np.inf)
+    unbounded = ref.trace_brute(verts, open_rays, threads=4)
+    exact = (rays["t_max"] == 1.0) & (unbounded["t"] == 1.0)  # the nearest hit is at t_max
+    assert exact.sum() > 5 and np.all(brute["prim"][exact] == 0xFFFFFFFF)
+    assert (brute["prim"] != 0xFFFFFFFF).sum() > 500
+    aabbs, centers = prep(verts)
+    nodes, idx = ref.build_locb(aabbs, centers)
+    m = ref.build_mbvh(nodes)
+    assert hits_equal(ref.trace_mbvh(m, idx, verts, rays, threads=4), brute).all()
+    assert hits_equal(ref.trace_bvh(nodes, idx, verts, rays, threads=4), brute).all()
 
 
 def test_refit_oracle():
