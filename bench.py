@@ -4,17 +4,17 @@
 Headline workload (1M-triangle scene):
   1,000,000 FP32 triangles — a 750x600-quad noisy sphere (900k) plus 100k
   random triangles (edge 0.01) in the unit cube; GPU PLOC build (search
-  radius 14), the reference-layout MBVH collapse and the 4-wide traversal
-  tree; 8192^2 = 67,108,864 pinhole primary rays and the same number of
-  cosine-weighted bounce rays from the primary hits.
+  radius 14), the reference-layout MBVH collapse and the compressed 8-wide
+  traversal tree; 8192^2 = 67,108,864 pinhole primary rays and the same
+  number of cosine-weighted bounce rays from the primary hits.
 
 A step is one closest-hit launch over the 67,108,864 bounce rays (the
 incoherent set), inputs resident in HBM, lowest-(t, id) rule (hit records
 identical to brute force; a sample is checked against the CPU oracle).  The
 headline value is bounce-ray Mrays/s of the traversal tree; the JSON line
 also carries the build times (PLOC, collapse, traversal tree, binned SAH),
-the primary-ray rate, the reference-layout MBVH's rate on the same rays, the
-roofline of the trace kernel, the CPU oracle's rate on a sample of the same
+the primary-ray rate, the 4-wide traversal tree's and the reference-layout
+MBVH's rates on the same rays (same hits), the roofline of the trace kernel, the CPU oracle's rate on a sample of the same
 rays (host threads stated), the batched-vs-plain comparison over 256 scenes
 of uneven size, and the animated 20M-triangle any-hit frame.
 
@@ -34,7 +34,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s closest-hit (1M tris, incoherent); PLOC build ms; batched vs plain"
 HBM_PEAK_GBS = 8000.0
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "trace_tree_bounce_1m_traffic.json")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "cw8_bounce_1m_traffic.json")
 
 
 def log(msg):
@@ -43,8 +43,9 @@ def log(msg):
 
 def run_batched(args, dev, rt, scenes, torch):
     """Many scenes of uneven size: per-scene launches (plain) against one
-    batched build and one batched launch.  Whole job = PLOC build + MBVH
-    collapse + leaf-ordered triangles + closest hit of every scene's rays."""
+    batched build and one batched launch.  Whole job = PLOC build + compressed
+    traversal tree + closest hit of every scene's rays, the same tree and
+    kernel either way."""
     log(f"generating {args.batch_scenes} batch scenes")
     vl = scenes.batch_scenes(args.batch_scenes, seed=args.seed)
     counts = [len(v) for v in vl]
@@ -79,11 +80,10 @@ def run_batched(args, dev, rt, scenes, torch):
         for s in range(first, last):
             p0, p1, r0, r1 = offs[s], offs[s + 1], roffs[s], roffs[s + 1]
             bvh = rt.Builder(c_d[p0:p1], aabbs=a_d[p0:p1]).construct_locally_ordered_clustered()
-            m = rt.Mbvh.from_bvh(bvh)
-            m.attach_triangles(v_d[p0:p1])
-            m.intersect_prepared(rays[r0:r1], out=hits_plain[r0:r1])
+            tree = bvh.trace_tree(v_d[p0:p1])
+            tree.intersect(rays[r0:r1], out=hits_plain[r0:r1])
             if keep:
-                trees.append((bvh, m))
+                trees.append((bvh, tree))
         torch.cuda.synchronize()
         return trees
 
@@ -111,8 +111,8 @@ def run_batched(args, dev, rt, scenes, torch):
     # per-scene trace launches alone, trees already built
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s, (_, m) in enumerate(trees):
-        m.intersect_prepared(rays[roffs[s]:roffs[s + 1]], out=hits_plain[roffs[s]:roffs[s + 1]])
+    for s, (_, tree) in enumerate(trees):
+        tree.intersect(rays[roffs[s]:roffs[s + 1]], out=hits_plain[roffs[s]:roffs[s + 1]])
     torch.cuda.synchronize()
     plain_trace_ms = (time.perf_counter() - t0) * 1e3
     trees[-1][1].status()
@@ -127,7 +127,7 @@ def run_batched(args, dev, rt, scenes, torch):
         batch_trace_ms.append((t2 - t1) * 1e3)
         del b
     same = bool(torch.equal(hits_plain.view(torch.int32), hits_batch.view(torch.int32)))
-    # the same frame as one hipGraph (build + collapse + triangles + trace)
+    # the same frame as one hipGraph (build + traversal tree + trace)
     plan = rt.BatchPlan(c_d, offs, v_d, aabbs=a_d)
     hits_graph = torch.empty_like(hits_plain)
     graph_ms = []
@@ -161,7 +161,8 @@ def run_batched(args, dev, rt, scenes, torch):
         "graph_vs_plain": round(p_ms / g_ms, 3),
         "workload": "scene sizes round(exp(U(ln 1e3, ln 5e5))), alternating noisy sphere / "
                     "soup; 16 rays per triangle, origin uniform in the scene box, uniform "
-                    "direction; whole job = PLOC + collapse + triangles + closest hit",
+                    "direction; whole job = PLOC + compressed traversal tree + closest hit "
+                    "(plain: per-scene calls; batched: one build and one launch)",
     }
     log(f"batched {b_ms:.1f} ms vs plain {p_ms:.1f} ms (x{p_ms / b_ms:.2f}); trace "
         f"{bt_ms:.1f} vs {plain_trace_ms:.1f} ms; same hits {same}; graph {g_ms:.1f} ms "
@@ -280,8 +281,9 @@ def copy_calibration(args, torch, dev):
 def run_large(args, dev, rt, scenes, torch):
     """64 x 64 displaced spheres (19,988,480 triangles) animated per frame:
     each frame moves the vertices, recomputes boxes and centres, rebuilds
-    (PLOC + collapse + leaf-ordered triangles) and traces 2^28 any-hit shadow
-    rays to a point light — the whole frame as one recorded graph."""
+    (PLOC + compressed traversal tree) and traces 2^28 any-hit shadow rays to
+    a point light — the whole frame as one recorded graph.  For comparison,
+    the same rays are also traced eagerly against the reference-layout MBVH."""
     log("generating the 20M-triangle sphere grid")
     base = torch.from_numpy(scenes.sphere_grid()).to(dev)
     n = base.shape[0]
@@ -332,17 +334,17 @@ def run_large(args, dev, rt, scenes, torch):
         "frame_ms": round(fm, 2),
         "frame_mrays_s": round(n_rays / (fm * 1e-3) / 1e6, 1),
         "eager_ploc_build_ms": round(eager_build_ms, 2),
-        "any_hit_trace_ms": round(trace_ms, 2),
-        "any_hit_mrays_s": round(n_rays / (trace_ms * 1e-3) / 1e6, 1),
+        "eager_mbvh_any_hit_ms": round(trace_ms, 2),
+        "eager_mbvh_any_hit_mrays_s": round(n_rays / (trace_ms * 1e-3) / 1e6, 1),
         "occluded_fraction": round(occluded, 4),
         "plan_iterations": plan.iterations,
         "same_as_eager": same,
         "workload": "64x64 seeded displaced spheres (61x40 quads each); per frame y += "
-                    "0.02 sin(2 pi (x + f/60)), boxes/centres, PLOC rebuild, collapse, "
-                    "triangles and 2^28 any-hit shadow rays to a point light, one graph",
+                    "0.02 sin(2 pi (x + f/60)), boxes/centres, PLOC rebuild, compressed "
+                    "traversal tree and 2^28 any-hit shadow rays to a point light, one graph",
     }
-    log(f"large: frame {fm:.1f} ms ({res['frame_mrays_s']} Mrays/s incl. rebuild), any-hit "
-        f"{trace_ms:.1f} ms, eager PLOC {eager_build_ms:.1f} ms, same {same}")
+    log(f"large: frame {fm:.1f} ms ({res['frame_mrays_s']} Mrays/s incl. rebuild), eager "
+        f"MBVH any-hit {trace_ms:.1f} ms, eager PLOC {eager_build_ms:.1f} ms, same {same}")
     del plan, m, bvh, rays, occ, want, base, verts, aabbs, centers
     torch.cuda.empty_cache()
     return res
@@ -360,7 +362,9 @@ def main():
     ap.add_argument("--quick", action="store_true",
                     help="one build of each kind and no binned SAH (for profiled runs)")
     ap.add_argument("--max-leaf", type=int, default=1,
-                    help="primitives per leaf of the traversal tree")
+                    help="primitives per leaf of the 4-wide traversal tree")
+    ap.add_argument("--width", type=int, default=8, choices=(4, 8),
+                    help="traversal tree of the headline: 8 compressed (default) or 4-wide")
     ap.add_argument("--batch-scenes", type=int, default=256,
                     help="scenes in the batched-vs-plain comparison (0 skips it)")
     ap.add_argument("--large", type=int, default=1,
@@ -467,11 +471,16 @@ def main():
     for i in range(3):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        ttree = bvh.trace_tree(v_d, max_leaf=args.max_leaf)
+        ttree = bvh.trace_tree(v_d, max_leaf=args.max_leaf if args.width == 4 else 1,
+                               width=args.width)
         torch.cuda.synchronize()
         if i > 0:
             tree_ms.append((time.perf_counter() - t0) * 1e3)
-    log(f"traversal tree: {ttree.node_count} nodes, {statistics.median(tree_ms):.2f} ms")
+    log(f"traversal tree (width {args.width}): {ttree.node_count} nodes, "
+        f"{statistics.median(tree_ms):.2f} ms")
+    # the other traversal tree, for comparison on the same rays
+    other_width = 4 if args.width == 8 else 8
+    otree = bvh.trace_tree(v_d, width=other_width)
 
     def trace_vertex(rays, out):
         mbvh.intersect(v_d, rays, out=out)
@@ -481,6 +490,9 @@ def main():
 
     def trace_tree(rays, out):
         ttree.intersect(rays, out=out)
+
+    def trace_other(rays, out):
+        otree.intersect(rays, out=out)
 
     def timed(rays, steps, warmup, fn=trace_tree):
         out = torch.empty((rays.shape[0], 4), dtype=torch.float32, device=dev)
@@ -500,12 +512,13 @@ def main():
         per = [a.elapsed_time(b) for a, b in ev]
         return out, per, wall
 
-    # launch shape chosen by the autotuner on an 8M-ray sample of the bounce
-    # rays (untimed set-up, like the builds)
+    # launch shape chosen by the autotuner on the bounce rays themselves
+    # (untimed set-up, like the builds; a smaller sample favours smaller
+    # persistent grids than the full launch wants)
     if args.no_autotune:
         tuned = rt.get_trace_config()
     else:
-        tuned = rt.autotune(ttree, bounce[: min(n_rays, 1 << 23)])
+        tuned = rt.autotune(ttree, bounce)
     log(f"launch shape {tuned}")
 
     if args.trace_only:
@@ -537,6 +550,11 @@ def main():
     del hits_v
     log(f"MBVH + leaf-ordered triangles {mbvh_mrays:.1f} Mrays/s (same {same_mbvh_hits}), "
         f"MBVH + vertex array {vertex_mrays:.1f} Mrays/s (same {same_vertex_hits})")
+    hits_o, per_o, _ = timed(bounce, 3, 1, fn=trace_other)
+    other_mrays = n_rays / (statistics.median(per_o) * 1e-3) / 1e6
+    same_other_hits = bool(torch.equal(hits_o.view(torch.int32), hits.view(torch.int32)))
+    del hits_o
+    log(f"width-{other_width} traversal tree {other_mrays:.1f} Mrays/s (same {same_other_hits})")
 
     # ---- kernel work counters (nodes and triangles per ray) ----------------
     import ctypes
@@ -560,8 +578,10 @@ def main():
             ctypes.c_void_p(stream.cuda_stream)))
     log(f"per ray: traversal tree {nodes_per_ray:.2f} nodes / {tris_per_ray:.2f} triangles, "
         f"MBVH {m_nodes_per_ray:.2f} / {m_tris_per_ray:.2f}")
-    # ray 32 B + hit 16 B + 128 B per node + 48 B per leaf-ordered triangle
-    b_ray = 32 + 16 + 128 * nodes_per_ray + 48 * tris_per_ray
+    # ray 32 B + hit 16 B + one node record (80 B compressed, 128 B 4-wide)
+    # per visit + 48 B per leaf-ordered triangle
+    node_bytes = 80 if args.width == 8 else 128
+    b_ray = 32 + 16 + node_bytes * nodes_per_ray + 48 * tris_per_ray
     achieved = value * 1e6 * b_ray / 1e9
     traffic = None
     if os.path.exists(PMC_SUMMARY):
@@ -627,8 +647,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "1M-tri noisy sphere + soup; GPU PLOC (r=14) + 4-wide traversal tree; closest hit of "
-                        "8192^2 cosine-bounce rays",
+            "workload": f"1M-tri noisy sphere + soup; GPU PLOC (r=14) + "
+                        f"{'compressed 8-wide' if args.width == 8 else '4-wide'} traversal tree; "
+                        f"closest hit of 8192^2 cosine-bounce rays",
             "tris": n_tris,
             "rays_per_step": n_rays,
             "tie_rule": "lowest (t, prim id)",
@@ -637,7 +658,11 @@ def main():
         "mbvh_collapse_ms": round(statistics.median(collapse_ms), 3),
         "attach_triangles_ms": round(statistics.median(attach_ms), 3),
         "trace_tree_build_ms": round(statistics.median(tree_ms), 3),
-        "trace_tree_max_leaf": args.max_leaf,
+        "trace_tree_width": args.width,
+        "trace_tree_max_leaf": args.max_leaf if args.width == 4 else 1,
+        "trace_tree_nodes": ttree.node_count,
+        f"width{other_width}_tree_mrays_s": round(other_mrays, 2),
+        f"width{other_width}_tree_same_hits": same_other_hits,
         "trace_launch_shape": tuned,
         "binned_sah_build_ms": round(statistics.median(binned_ms), 3),
         "primary_mrays_s": round(primary_mrays, 2),

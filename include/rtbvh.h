@@ -309,18 +309,23 @@ void rtbvh_gpu_free_trace_tree(RTTraceTree tree);
 
 /* ---- Launch shape of the lowest-id MBVH traversal ---------------------------
  * (no counterpart in the reference).  Per device; the default is
- * {64, 8, 400, 0, 1, 0}.  Results never depend on
+ * {64, 8, 800, 0, 1, 0, 0}.  Results never depend on
  * it, only speed. */
 typedef struct RTTraceConfig {
     uint32_t block;          /* threads per block: 64, 128 or 256 */
     uint32_t lds_stack;      /* traversal stack entries per ray kept in LDS, 0..32 */
     uint32_t occupancy_pct;  /* persistent grid as a share of resident blocks, 10..3200 */
-    uint32_t kernel;         /* with leaf-ordered triangles: 0 one node or one triangle per
-                                ray per iteration, 1 a node and all its leaf slots */
-    uint32_t refill;         /* kernel 0: idle lanes of a wavefront that take new rays
-                                together, 1..64 */
-    uint32_t schedule;       /* kernel 0: 0 each lane walks a fixed stride of rays, 1 waves
-                                take runs of rays from per-chunk counters as lanes free up */
+    uint32_t kernel;         /* 4-wide trees with leaf-ordered triangles: 0 one node or one
+                                triangle per ray per iteration, 1 a node and all its leaf
+                                slots (compressed trees have one kernel) */
+    uint32_t refill;         /* kernel 0 and compressed trees: idle lanes of a wavefront
+                                that take new rays together, 1..64 */
+    uint32_t schedule;       /* 4-wide kernel 0: 0 each lane walks a fixed stride of rays,
+                                1 waves take runs of rays from per-chunk counters as lanes
+                                free up (compressed trees: always 0) */
+    uint32_t postpone;       /* compressed trees: a wavefront tests triangles only once this
+                                many of its lanes have one waiting, or a lane has nothing
+                                else to do; 0..64 (0: whenever any lane has one) */
 } RTTraceConfig;
 
 ResultCode rtbvh_gpu_set_trace_config(const RTTraceConfig *config); /* NULL: default */
@@ -333,7 +338,8 @@ ResultCode rtbvh_gpu_get_trace_config(RTTraceConfig *config);
  * Synchronises the stream. */
 ResultCode rtbvh_gpu_autotune_mbvh(RTMbvh mbvh, const RTRay *rays, size_t ray_count, RTHit *hits,
                                    void *stream, RTTraceConfig *best);
-/* The same over a traversal tree. */
+/* The same over a traversal tree; for a compressed (width 8) tree, 72 shapes:
+ * block 64/128/256 x LDS stack 4/8/12 x grid 200..1600 % x refill 1/8. */
 ResultCode rtbvh_gpu_autotune_trace_tree(RTTraceTree tree, const RTRay *rays, size_t ray_count,
                                          RTHit *hits, void *stream, RTTraceConfig *best);
 
@@ -366,7 +372,8 @@ typedef struct RTBatch {
     uint32_t bvh_node_count;      /* 2 * index_count - scene_count */
     uint32_t node_count;          /* traversal-tree nodes (0 until triangles are attached) */
     const RTBvhNode *bvh_nodes;   /* device; node s is the root of scene s */
-    const RTMbvhNode *nodes;      /* device traversal tree, node s the root of scene s; NULL
+    const void *nodes;            /* device traversal tree (compressed 8-wide records, as
+                                     RTTraceTree width 8), node s the root of scene s; NULL
                                      in the handle returned by rtbvh_gpu_create_batch */
     uint32_t index_count;
     uint32_t pad_;
@@ -381,9 +388,10 @@ ResultCode rtbvh_gpu_create_batch(const RTAabb *aabbs, const float *centers, siz
                                   const uint32_t *scene_offsets, size_t scene_count,
                                   void *stream, RTBatch *result);
 
-/* Builds the batch's traversal tree (as rtbvh_gpu_create_trace_tree with
- * max_leaf 0, scene-local ids) from vertices: device, 9 floats per primitive
- * of the batch, and updates *batch.  Needed before tracing the batch. */
+/* Builds the batch's traversal tree (as rtbvh_gpu_create_trace_tree8, one
+ * tree per scene, scene-local ids) from vertices: device, 9 floats per
+ * primitive of the batch, and updates *batch.  Needed before tracing the
+ * batch. */
 ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch *batch, const float *vertices,
                                             size_t prim_count, void *stream);
 

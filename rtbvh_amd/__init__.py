@@ -11,7 +11,7 @@ Host-side mirror of the reference's Rust API (src/bvh.rs):
     bvh / mbvh .occluded(vertices, rays)       any hit on the GPU
     mbvh.attach_triangles(vertices); mbvh.intersect_prepared(rays)
                                                leaf-ordered triangle copy
-    bvh.trace_tree(vertices, max_leaf)         4-wide traversal tree (fastest path)
+    bvh.trace_tree(vertices)                   compressed 8-wide traversal tree (fastest path)
     SceneBatch(centres, scene_offsets, aabbs)  many scenes, one build + one launch
     BatchPlan(centres, scene_offsets, aabbs)   build + trace as one hipGraph
 

@@ -61,7 +61,8 @@ class RTBatchPlan(ctypes.Structure):
 class RTTraceConfig(ctypes.Structure):
     _fields_ = [("block", ctypes.c_uint32), ("lds_stack", ctypes.c_uint32),
                 ("occupancy_pct", ctypes.c_uint32), ("kernel", ctypes.c_uint32),
-                ("refill", ctypes.c_uint32), ("schedule", ctypes.c_uint32)]
+                ("refill", ctypes.c_uint32), ("schedule", ctypes.c_uint32),
+                ("postpone", ctypes.c_uint32)]
 
 
 class RTTraceTree(ctypes.Structure):

@@ -218,7 +218,7 @@ class Bvh(_Tree):
         L.check(L.lib().rtbvh_gpu_refit(_ptr(aabbs), self._h, _stream(aabbs)), "rtbvh_gpu_refit")
 
     def trace_tree(self, vertices: torch.Tensor, max_leaf: int = 1,
-                   width: int = 4) -> "TraceTree":
+                   width: int = 8) -> "TraceTree":
         """Traversal tree over `vertices` (see TraceTree)."""
         return TraceTree(self, vertices, max_leaf, width)
 
@@ -250,7 +250,7 @@ class TraceTree:
     max_leaf triangles); width 8: rtbvh_gpu_create_trace_tree8 (compressed
     8-wide nodes, one triangle per leaf slot)."""
 
-    def __init__(self, bvh: "Bvh", vertices: torch.Tensor, max_leaf: int = 1, width: int = 4):
+    def __init__(self, bvh: "Bvh", vertices: torch.Tensor, max_leaf: int = 1, width: int = 8):
         if not bvh._owned:
             raise ValueError("a trace tree needs a Bvh built by this library")
         if width not in (4, 8):
@@ -648,16 +648,17 @@ def get_trace_config() -> dict:
     c = L.RTTraceConfig()
     L.check(L.lib().rtbvh_gpu_get_trace_config(ctypes.byref(c)), "rtbvh_gpu_get_trace_config")
     return {"block": c.block, "lds_stack": c.lds_stack, "occupancy_pct": c.occupancy_pct,
-            "kernel": c.kernel, "refill": c.refill, "schedule": c.schedule}
+            "kernel": c.kernel, "refill": c.refill, "schedule": c.schedule,
+            "postpone": c.postpone}
 
 
-def set_trace_config(block: Optional[int] = None, lds_stack: int = 8, occupancy_pct: int = 400,
-                     kernel: int = 0, refill: int = 1, schedule: int = 0):
+def set_trace_config(block: Optional[int] = None, lds_stack: int = 8, occupancy_pct: int = 800,
+                     kernel: int = 0, refill: int = 1, schedule: int = 0, postpone: int = 0):
     """Sets the traversal launch shape (block=None restores the default)."""
     if block is None:
         L.check(L.lib().rtbvh_gpu_set_trace_config(None), "rtbvh_gpu_set_trace_config")
         return
-    c = L.RTTraceConfig(block, lds_stack, occupancy_pct, kernel, refill, schedule)
+    c = L.RTTraceConfig(block, lds_stack, occupancy_pct, kernel, refill, schedule, postpone)
     L.check(L.lib().rtbvh_gpu_set_trace_config(ctypes.byref(c)), "rtbvh_gpu_set_trace_config")
 
 
@@ -673,4 +674,5 @@ def autotune(tree, rays: torch.Tensor) -> dict:
     L.check(fn(tree._h, _ptr(rays), rays.shape[0], _ptr(hits), _stream(rays), ctypes.byref(c)),
             fn.__name__)
     return {"block": c.block, "lds_stack": c.lds_stack, "occupancy_pct": c.occupancy_pct,
-            "kernel": c.kernel, "refill": c.refill, "schedule": c.schedule}
+            "kernel": c.kernel, "refill": c.refill, "schedule": c.schedule,
+            "postpone": c.postpone}

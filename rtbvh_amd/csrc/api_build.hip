@@ -244,7 +244,8 @@ RTBatch handle_of(uint32_t id, const BatchEntry& e) {
     h.bvh_node_count = e.bvh.node_count;
     h.node_count = e.ttree.node_count;
     h.bvh_nodes = reinterpret_cast<const RTBvhNode*>(e.bvh.nodes);
-    h.nodes = reinterpret_cast<const RTMbvhNode*>(e.ttree.nodes);
+    h.nodes = e.ttree.cnodes ? static_cast<const void*>(e.ttree.cnodes)
+                             : static_cast<const void*>(e.ttree.nodes);
     h.index_count = e.bvh.prim_count;
     h.indices = e.bvh.indices;
     return h;
@@ -511,8 +512,7 @@ ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch* batch, const float* vertice
         set_error("vertices (%p) is not device memory", static_cast<const void*>(vertices));
         return InvalidArgument;
     }
-    // the traversal tree of the batch (tight 4-wide forest, leaves of up to
-    // 4 triangles, scene-local ids)
+    // the traversal tree of the batch (a forest, scene-local ids)
     if (be->ttree.nodes || be->ttree.cnodes) {
         drain_device();  // traversals of the previous tree may be in flight
         release(be->ttree);

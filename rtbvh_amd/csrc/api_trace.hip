@@ -457,10 +457,10 @@ ResultCode rtbvh_gpu_set_trace_config(const RTTraceConfig* config) {
         return Ok;
     }
     const TraceConfig c{config->block, config->lds_stack, config->occupancy_pct, config->kernel,
-                        config->refill, config->schedule};
+                        config->refill, config->schedule, config->postpone};
     if (!valid_trace_config(c)) {
         set_error("trace config: block 64/128/256, lds_stack <= 32, occupancy_pct 10..3200, "
-                  "kernel 0/1, refill 1..64, schedule 0/1");
+                  "kernel 0/1, refill 1..64, schedule 0/1, postpone 0..64");
         return InvalidArgument;
     }
     set_trace_config(&c);
@@ -479,6 +479,7 @@ ResultCode rtbvh_gpu_get_trace_config(RTTraceConfig* config) {
     config->kernel = c.kernel;
     config->refill = c.refill;
     config->schedule = c.schedule;
+    config->postpone = c.postpone;
     return Ok;
 }
 
@@ -492,7 +493,7 @@ namespace {
 // device's configuration.
 template <typename Trace>
 ResultCode autotune(const RTRay* rays, size_t ray_count, RTHit* hits, void* stream,
-                    RTTraceConfig* best, Trace trace) {
+                    RTTraceConfig* best, bool compressed, Trace trace) {
     if (rays == nullptr || hits == nullptr || ray_count == 0) {
         set_error("autotuning needs rays and hits");
         return Error;
@@ -506,12 +507,19 @@ ResultCode autotune(const RTRay* rays, size_t ray_count, RTHit* hits, void* stre
         return hip_fail(e, "autotune events");
     }
     const TraceConfig before = get_trace_config();
-    // one-item kernel: block x LDS depth x grid x refill x schedule;
-    // per-node-leaves kernel: block x LDS depth x grid
+    // compressed tree: block x LDS depth x grid x refill (one kernel, static
+    // schedule); 4-wide trees, one-item kernel: block x LDS depth x grid x
+    // refill x schedule; per-node-leaves kernel: block x LDS depth x grid
     std::vector<TraceConfig> shapes;
     for (uint32_t b : {64u, 128u, 256u})
         for (uint32_t d : {4u, 8u, 12u, 16u})
-            for (uint32_t o : {200u, 400u, 800u}) {
+            for (uint32_t o : {200u, 400u, 800u, 1600u}) {
+                if (compressed) {
+                    if (d == 16u) continue;
+                    for (uint32_t rf : {1u, 8u}) shapes.push_back(TraceConfig{b, d, o, 0, rf, 0});
+                    continue;
+                }
+                if (o == 1600u) continue;
                 for (uint32_t rf : {1u, 4u, 16u})
                     for (uint32_t sc : {0u, 1u}) shapes.push_back(TraceConfig{b, d, o, 0, rf, sc});
                 shapes.push_back(TraceConfig{b, d, o, 1, 1, 0});
@@ -553,6 +561,7 @@ ResultCode autotune(const RTRay* rays, size_t ray_count, RTHit* hits, void* stre
         best->kernel = top.kernel;
         best->refill = top.refill;
         best->schedule = top.schedule;
+        best->postpone = top.postpone;
     }
     return Ok;
 }
@@ -564,14 +573,19 @@ extern "C" {
 
 ResultCode rtbvh_gpu_autotune_mbvh(RTMbvh mbvh, const RTRay* rays, size_t ray_count, RTHit* hits,
                                    void* stream, RTTraceConfig* best) {
-    return autotune(rays, ray_count, hits, stream, best, [&]() {
+    return autotune(rays, ray_count, hits, stream, best, false, [&]() {
         return prepared_entry(mbvh, rays, ray_count, hits, nullptr, 0, stream);
     });
 }
 
 ResultCode rtbvh_gpu_autotune_trace_tree(RTTraceTree tree, const RTRay* rays, size_t ray_count,
                                          RTHit* hits, void* stream, RTTraceConfig* best) {
-    return autotune(rays, ray_count, hits, stream, best, [&]() {
+    BatchView v;
+    if (!trace_tree_view(tree.id, &v)) {
+        set_error("unknown trace tree id %u", tree.id);
+        return Error;
+    }
+    return autotune(rays, ray_count, hits, stream, best, v.cnodes != nullptr, [&]() {
         return tree_entry(tree, rays, ray_count, hits, nullptr, 0, stream);
     });
 }

@@ -36,6 +36,8 @@ struct TraceParams {
     const uint32_t* gate;  // non-null: the launch does nothing while *gate != 0
     uint32_t lds_depth;    // set by the launcher: stack entries per lane held in LDS
     uint32_t refill;       // set by the launcher: idle lanes that trigger taking new rays
+    uint32_t postpone;     // set by the launcher (compressed tree): triangle tests wait until
+                           // this many lanes of the wave have one (0: never wait)
     bool small_leaves;     // every leaf holds at most 16 triangles (one-item kernel)
     uint32_t* work;        // dynamic schedule: work_chunks ray counters (launcher-cleared)
     uint32_t work_chunks;
@@ -54,6 +56,8 @@ struct TraceConfig {
                              // together (1..64)
     uint32_t schedule;       // one-item kernel: 0 fixed ray stride per lane, 1 rays taken
                              // from per-chunk counters as lanes free up
+    uint32_t postpone;       // compressed tree: triangle tests wait until this many lanes
+                             // of a wave have one, unless a lane has nothing else (0..64)
 };
 TraceConfig default_trace_config();
 bool valid_trace_config(const TraceConfig& c);
@@ -192,7 +196,7 @@ bool trace_tree_view(uint32_t id, BatchView* out);
 
 constexpr uint32_t kDefaultMaxLeaf = 1;
 // Traversal trees of batches and batch plans: 4 or 8 (the compressed tree).
-constexpr uint32_t kBatchTreeWidth = 4;
+constexpr uint32_t kBatchTreeWidth = 8;
 
 void set_error(const char* fmt, ...);
 ResultCode hip_fail(hipError_t e, const char* what);

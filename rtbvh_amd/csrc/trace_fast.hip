@@ -617,7 +617,9 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
 //                   first triangle.
 // Each iteration visits at most one node (the lowest key of the node group
 // in hand, which pushes the rest of that group) and tests at most one
-// triangle.  A slot is entered when max(t_near, t_min) <= min(t_far, t_best)
+// triangle; with `postpone` set, the triangle test waits until that many
+// lanes of the wave have one or a lane has no node group left to visit
+// (a newer triangle group then pushes the waiting one).  A slot is entered when max(t_near, t_min) <= min(t_far, t_best)
 // with t_near lowered and t_far raised by 2^-20 of their magnitude; with the
 // grid step kept free around every box, this never rejects a box that the
 // exact slab test would enter.  Where a grid step times 1/d overflows, the
@@ -659,6 +661,13 @@ __device__ __forceinline__ uint32_t bound_bits(float t) {
 }
 
 __device__ __forceinline__ float group_bound(uint32_t y) { return __uint_as_float(y << 16); }
+
+// 1/d for the compressed tree's box tests: the hardware reciprocal (within
+// 1 ulp, which the 2^-20 widening covers).  A zero component gives an
+// infinity of its sign; a component beyond 2^126 gives 0 or a denormal, so
+// its plane distances are 0 and only t_min <= 0 <= t_best decides, which
+// enters at least the slots the exact distances would.
+__device__ __forceinline__ float box_inv(float d) { return __builtin_amdgcn_rcpf(d); }
 
 __device__ __forceinline__ float qbyte(uint32_t w, int i) {
     return float((w >> ((i & 3) * 8)) & 0xFFu);
@@ -715,9 +724,9 @@ __global__ void __launch_bounds__(kMaxBlock) cw8_kernel(TraceParams p) {
             dy = rb.y;
             dz = rb.z;
             bt = rb.w;
-            ix = 1.0f / dx;  // Ray::new, src/ray.rs:166-182
-            iy = 1.0f / dy;
-            iz = 1.0f / dz;
+            ix = box_inv(dx);
+            iy = box_inv(dy);
+            iz = box_inv(dz);
             bu = 0.f;
             bv = 0.f;
             bp = RTBVH_MISS;
@@ -769,6 +778,15 @@ __global__ void __launch_bounds__(kMaxBlock) cw8_kernel(TraceParams p) {
             } else {
                 break;
             }
+        }
+        // ---- triangle tests of this iteration: when enough lanes have one
+        // waiting, or some lane has no node group to visit instead
+        bool tri_turn = true;
+        if (p.postpone > 1u) {
+            const bool waiting = (tgy & 0xFFu) != 0u;
+            const uint64_t w = __ballot(waiting);
+            tri_turn = uint32_t(__popcll(w)) >= p.postpone ||
+                       __ballot(waiting && ngy < kNodeGroup) != 0;
         }
         // ---- node step: the nearest-keyed child of the group in hand
         if (ngy >= kNodeGroup) {
@@ -830,7 +848,7 @@ __global__ void __launch_bounds__(kMaxBlock) cw8_kernel(TraceParams p) {
             }
         }
         // ---- triangle step: one triangle of the group in hand
-        if ((tgy & 0xFFu) != 0u) {
+        if (tri_turn && (tgy & 0xFFu) != 0u) {
             const uint32_t slot = uint32_t(__builtin_ctz(tgy & 0xFFu));
             tgy &= ~(1u << slot);
             const uint32_t pos = tgx + uint32_t(__popc((tgy >> 8) & ((1u << slot) - 1u)));
@@ -941,6 +959,7 @@ hipError_t launch_fast(const TraceParams& p0, hipStream_t stream) {
     TraceParams p = p0;
     p.lds_depth = cfg.lds_depth;
     p.refill = cfg.refill;
+    p.postpone = cfg.postpone;
     if (ITEM && !CW8 && cfg.schedule == 1) {
         if (!p.work) {
             uint32_t* ring = work_ring_ptr();
@@ -959,12 +978,12 @@ hipError_t launch_fast(const TraceParams& p0, hipStream_t stream) {
 
 }  // namespace
 
-TraceConfig default_trace_config() { return TraceConfig{64, 8, 400, 0, 1, 0}; }
+TraceConfig default_trace_config() { return TraceConfig{64, 8, 800, 0, 1, 0, 0}; }
 
 bool valid_trace_config(const TraceConfig& c) {
     return (c.block == 64 || c.block == 128 || c.block == 256) && c.lds_depth <= 32 &&
            c.occupancy_pct >= 10 && c.occupancy_pct <= 3200 && c.kernel <= 1 && c.refill >= 1 &&
-           c.refill <= 64 && c.schedule <= 1;
+           c.refill <= 64 && c.schedule <= 1 && c.postpone <= 64;
 }
 
 void set_trace_config(const TraceConfig* c) {

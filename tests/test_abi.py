@@ -39,7 +39,7 @@ def test_struct_layouts():
     assert ctypes.sizeof(L.RTAabb) == 32
     assert ctypes.sizeof(L.RTBvh) == 32 and ctypes.sizeof(L.RTMbvh) == 32
     assert ref.AABB.itemsize == 32 and ref.MBVH.itemsize == 128
-    assert ctypes.sizeof(L.RTTraceTree) == 40 and ctypes.sizeof(L.RTTraceConfig) == 24
+    assert ctypes.sizeof(L.RTTraceTree) == 40 and ctypes.sizeof(L.RTTraceConfig) == 28
 
 
 def test_error_codes_without_device():

@@ -236,7 +236,7 @@ def test_trace_tree_matches_mbvh(kind):
     brute = ref.trace_brute(verts, as_rays(rays8), threads=8)
     assert hits_equal(want.view(ref.HIT).reshape(-1), brute).all()
     for max_leaf in (1, 2, 4, 8):
-        tt = bvh.trace_tree(dv, max_leaf=max_leaf)
+        tt = bvh.trace_tree(dv, max_leaf=max_leaf, width=4)
         assert 0 < tt.node_count <= len(verts)
         got = tt.intersect(dr).cpu().numpy()
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), max_leaf
@@ -250,7 +250,7 @@ def test_trace_tree_small(n):
     verts = scenes.random_soup(n, 0.3, seed=n)
     bvh, _, _ = _build(verts, "ploc")
     rays8 = ray_set(verts, 2000, seed=n, kind="box")
-    got = bvh.trace_tree(_dev(verts)).intersect(_dev(rays8)).cpu().numpy()
+    got = bvh.trace_tree(_dev(verts), width=4).intersect(_dev(rays8)).cpu().numpy()
     want = ref.trace_brute(verts, as_rays(rays8), threads=8)
     assert hits_equal(got.view(ref.HIT).reshape(-1), want).all()
 
@@ -349,7 +349,7 @@ def test_trace_tree8_structure_and_hits(kind, ppl):
     assert np.array_equal(tt.occluded(dr).cpu().numpy(), want_occ)
     assert np.array_equal(tt.intersect(dr).cpu().numpy().view(np.uint32), want.view(np.uint32))
     tt.status()
-    _, n4, t4 = bvh.trace_tree(dv).intersect_stats(dr)
+    _, n4, t4 = bvh.trace_tree(dv, width=4).intersect_stats(dr)
     print(f"  nodes/ray 8-wide {n8 / len(rays8):.2f} vs 4-wide {n4 / len(rays8):.2f}; "
           f"tris/ray {t8 / len(rays8):.2f} vs {t4 / len(rays8):.2f}", flush=True)
     assert 0 < n8 < n4
@@ -385,7 +385,7 @@ def test_trace_tree8_flat_scene_axis_rays(offset):
     _check_cw8(tt, verts)
     want = ref.trace_brute(verts, as_rays(rays8), threads=8)
     m = rt.Mbvh.from_bvh(bvh)
-    results = {"8-wide": tt.intersect(dr), "4-wide": bvh.trace_tree(dv).intersect(dr),
+    results = {"8-wide": tt.intersect(dr), "4-wide": bvh.trace_tree(dv, width=4).intersect(dr),
                "mbvh": m.intersect(dv, dr)}
     for name, got in results.items():
         got = got.cpu().numpy().view(ref.HIT).reshape(-1)
