@@ -309,7 +309,7 @@ void rtbvh_gpu_free_trace_tree(RTTraceTree tree);
 
 /* ---- Launch shape of the lowest-id MBVH traversal ---------------------------
  * (no counterpart in the reference).  Per device; the default is
- * {64, 8, 800, 0, 1, 0, 0}.  Results never depend on
+ * {64, 8, 800, 0, 16, 1, 0}.  Results never depend on
  * it, only speed. */
 typedef struct RTTraceConfig {
     uint32_t block;          /* threads per block: 64, 128 or 256 */
@@ -320,9 +320,12 @@ typedef struct RTTraceConfig {
                                 slots (compressed trees have one kernel) */
     uint32_t refill;         /* kernel 0 and compressed trees: idle lanes of a wavefront
                                 that take new rays together, 1..64 */
-    uint32_t schedule;       /* 4-wide kernel 0: 0 each lane walks a fixed stride of rays,
-                                1 waves take runs of rays from per-chunk counters as lanes
-                                free up (compressed trees: always 0) */
+    uint32_t schedule;       /* 4-wide kernel 0 and compressed trees: 0 each lane walks a
+                                fixed stride of rays, 1 each wavefront owns a contiguous
+                                range of rays and its lanes draw the next one as they
+                                free up; the launcher lowers refill and the grid when
+                                the launch has few rays per resident lane, and runs
+                                schedule 0 with refill 1 below 2 rays per lane */
     uint32_t postpone;       /* compressed trees: a wavefront tests triangles only once this
                                 many of its lanes have one waiting, or a lane has nothing
                                 else to do; 0..64 (0: whenever any lane has one) */
@@ -338,8 +341,9 @@ ResultCode rtbvh_gpu_get_trace_config(RTTraceConfig *config);
  * Synchronises the stream. */
 ResultCode rtbvh_gpu_autotune_mbvh(RTMbvh mbvh, const RTRay *rays, size_t ray_count, RTHit *hits,
                                    void *stream, RTTraceConfig *best);
-/* The same over a traversal tree; for a compressed (width 8) tree, 72 shapes:
- * block 64/128/256 x LDS stack 4/8/12 x grid 200..1600 % x refill 1/8. */
+/* The same over a traversal tree; for a compressed (width 8) tree, 55 shapes:
+ * schedule 1 with block 64/128 x LDS stack 6/8/10 x grid 400/800/1600 % x
+ * refill 8/16/32, and the static schedule at 64/8/800 %. */
 ResultCode rtbvh_gpu_autotune_trace_tree(RTTraceTree tree, const RTRay *rays, size_t ray_count,
                                          RTHit *hits, void *stream, RTTraceConfig *best);
 

@@ -653,7 +653,7 @@ def get_trace_config() -> dict:
 
 
 def set_trace_config(block: Optional[int] = None, lds_stack: int = 8, occupancy_pct: int = 800,
-                     kernel: int = 0, refill: int = 1, schedule: int = 0, postpone: int = 0):
+                     kernel: int = 0, refill: int = 16, schedule: int = 1, postpone: int = 0):
     """Sets the traversal launch shape (block=None restores the default)."""
     if block is None:
         L.check(L.lib().rtbvh_gpu_set_trace_config(None), "rtbvh_gpu_set_trace_config")

@@ -507,23 +507,28 @@ ResultCode autotune(const RTRay* rays, size_t ray_count, RTHit* hits, void* stre
         return hip_fail(e, "autotune events");
     }
     const TraceConfig before = get_trace_config();
-    // compressed tree: block x LDS depth x grid x refill (one kernel, static
-    // schedule); 4-wide trees, one-item kernel: block x LDS depth x grid x
-    // refill x schedule; per-node-leaves kernel: block x LDS depth x grid
+    // compressed tree (one kernel): wave-range schedule with block x LDS
+    // depth x grid x refill, and the static schedule at the default shape;
+    // 4-wide trees, one-item kernel: block x LDS depth x grid x refill x
+    // schedule; per-node-leaves kernel: block x LDS depth x grid
     std::vector<TraceConfig> shapes;
-    for (uint32_t b : {64u, 128u, 256u})
-        for (uint32_t d : {4u, 8u, 12u, 16u})
-            for (uint32_t o : {200u, 400u, 800u, 1600u}) {
-                if (compressed) {
-                    if (d == 16u) continue;
-                    for (uint32_t rf : {1u, 8u}) shapes.push_back(TraceConfig{b, d, o, 0, rf, 0});
-                    continue;
+    if (compressed) {
+        for (uint32_t b : {64u, 128u})
+            for (uint32_t d : {6u, 8u, 10u})
+                for (uint32_t o : {400u, 800u, 1600u})
+                    for (uint32_t rf : {8u, 16u, 32u})
+                        shapes.push_back(TraceConfig{b, d, o, 0, rf, 1, 0});
+        shapes.push_back(TraceConfig{64, 8, 800, 0, 1, 0, 0});
+    } else {
+        for (uint32_t b : {64u, 128u, 256u})
+            for (uint32_t d : {4u, 8u, 12u, 16u})
+                for (uint32_t o : {200u, 400u, 800u}) {
+                    for (uint32_t rf : {1u, 4u, 16u})
+                        for (uint32_t sc : {0u, 1u})
+                            shapes.push_back(TraceConfig{b, d, o, 0, rf, sc});
+                    shapes.push_back(TraceConfig{b, d, o, 1, 1, 0});
                 }
-                if (o == 1600u) continue;
-                for (uint32_t rf : {1u, 4u, 16u})
-                    for (uint32_t sc : {0u, 1u}) shapes.push_back(TraceConfig{b, d, o, 0, rf, sc});
-                shapes.push_back(TraceConfig{b, d, o, 1, 1, 0});
-            }
+    }
     TraceConfig top = before;
     float top_ms = 3.4e38f;
     ResultCode rc = Ok;

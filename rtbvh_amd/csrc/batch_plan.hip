@@ -46,14 +46,12 @@ struct PlanEntry {
     TraceTreePlan* tt = nullptr;
     DeviceBvh tree;    // view into the PLOC plan
     DeviceMbvh ttree;  // traversal tree: nodes (capacity) and triangles, owned
-    uint32_t* work = nullptr;      // ray counters of the traversal's dynamic schedule
     hipStream_t stream = nullptr;  // graph launches
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipGraphExec_t exec = nullptr;
     GraphKey key{};
     ~PlanEntry() {
         if (exec) (void)hipGraphExecDestroy(exec);
-        if (work) (void)hipFree(work);
         if (ev_in) (void)hipEventDestroy(ev_in);
         if (ev_out) (void)hipEventDestroy(ev_out);
         if (stream) (void)hipStreamDestroy(stream);
@@ -158,7 +156,6 @@ hipError_t enqueue_all(PlanEntry& pe, const RTAabb* aabbs, const float* centers,
     p.scene_count = pe.scenes;
     p.gate = fail;
     p.small_leaves = true;  // traversal-tree leaves
-    p.work = pe.work;
     p.status = device_status_ptr();
     if (p.status == nullptr) return hipErrorInvalidValue;
     return launch_trace_fast(p, occluded != nullptr, stream);
@@ -225,8 +222,7 @@ ResultCode rtbvh_gpu_batch_plan_create(const RTAabb* aabbs, const float* centers
     pe->ttree.node_count = capacity;  // replays may need more nodes than this build
     const uint32_t lv = trace_tree_levels(pe->tt);
     pe->levels = lv + (lv / 4 > 4 ? lv / 4 : 4);
-    hipError_t e = hipMalloc(&pe->work, kPlanWorkWords * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&pe->stream, hipStreamNonBlocking);
+    hipError_t e = hipStreamCreateWithFlags(&pe->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&pe->ev_in, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&pe->ev_out, hipEventDisableTiming);
     if (e != hipSuccess) return hip_fail(e, "batch plan");

@@ -39,11 +39,9 @@ struct TraceParams {
     uint32_t postpone;     // set by the launcher (compressed tree): triangle tests wait until
                            // this many lanes of the wave have one (0: never wait)
     bool small_leaves;     // every leaf holds at most 16 triangles (one-item kernel)
-    uint32_t* work;        // dynamic schedule: work_chunks ray counters (launcher-cleared)
-    uint32_t work_chunks;
+    uint32_t wave_ranges;  // set by the launcher: 1 each wave draws from its own ray range
 };
 
-constexpr uint32_t kPlanWorkWords = 256;  // counter block a plan owns for its traversal
 
 // Launch shape of the lowest-id MBVH traversal (per device).
 struct TraceConfig {
@@ -54,8 +52,8 @@ struct TraceConfig {
                              // 1 a node and all its leaf slots per iteration
     uint32_t refill;         // one-item kernel: idle lanes of a wave that take new rays
                              // together (1..64)
-    uint32_t schedule;       // one-item kernel: 0 fixed ray stride per lane, 1 rays taken
-                             // from per-chunk counters as lanes free up
+    uint32_t schedule;       // one-item and compressed kernels: 0 fixed ray stride per lane,
+                             // 1 each wave's lanes draw from the wave's own ray range
     uint32_t postpone;       // compressed tree: triangle tests wait until this many lanes
                              // of a wave have one, unless a lane has nothing else (0..64)
 };

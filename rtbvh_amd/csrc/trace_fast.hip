@@ -1,9 +1,11 @@
-// MBVH closest-hit / any-hit traversal for the default tie rule (the
+// Closest-hit / any-hit traversal for the default tie rule (the
 // lexicographic minimum of (t, prim id) — identical to brute force), tuned
-// for gfx950.  The reference-order kernel in trace.hip reproduces
-// MbvhIterator exactly; this kernel keeps its triangle test and box
-// arithmetic but is free to choose the visit order, because the lowest
-// (t, prim) over all triangles does not depend on it:
+// for gfx950: the 4-wide kernels (reference-layout MBVH and 4-wide
+// traversal tree) and, further down, the compressed 8-wide tree's kernel.
+// The reference-order kernel in trace.hip reproduces MbvhIterator exactly;
+// these keep its triangle test but are free to choose the visit order,
+// because the lowest (t, prim) over all triangles does not depend on it.
+// The 4-wide kernels keep its box arithmetic:
 //
 //   * slab test with IEEE min/max (v_min3/v_max3); along a zero direction
 //     component the planes are measured from the floats next to the origin
@@ -16,10 +18,8 @@
 //     the others are pushed far to near together with their entry distance,
 //     and a popped entry whose distance now exceeds t_best is dropped without
 //     fetching its node.
-// The oracle's lowest-id mode (oracle/refcpu.cpp) follows the same order, so
-// the kernel's work counters (STATS) are checked against it.
-#include <atomic>
-
+// STATS variants count node visits and triangle tests (the bytes-per-ray
+// figure of the benchmark).
 #include "internal.hpp"
 
 namespace rtbvh {
@@ -337,6 +337,57 @@ __device__ __forceinline__ int32_t slot_entry(int32_t c, int32_t n) {
     return n < 0 ? c : int32_t(kLeafFlag | (uint32_t(n - 1) << 27) | uint32_t(c));
 }
 
+// Ray hand-out of the persistent kernels.  Static (schedule 0): lane k of
+// the grid takes rays k, k + stride, ...  Wave ranges (schedule 1): wave w
+// owns the contiguous rays [w L, (w + 1) L), L = ceil(rays / waves), and its
+// idle lanes draw the next ones in lane order as they free up, so a wave's
+// lanes stay busy until its range is used up (no atomics: the cursor is
+// wave-uniform).
+struct RaySchedule {
+    uint64_t rid;
+    uint64_t stride;
+    uint64_t cursor;  // wave ranges: next ray of the wave's range (wave-uniform)
+    uint64_t end;
+    bool ranges;
+
+    __device__ void init(const TraceParams& p) {
+        stride = uint64_t(gridDim.x) * blockDim.x;
+        const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+        rid = tid;
+        ranges = p.wave_ranges != 0;
+        const uint64_t waves = (stride + 63) >> 6;
+        const uint64_t per = (p.ray_count + waves - 1) / waves;
+        const uint64_t b = (tid >> 6) * per;
+        cursor = b < p.ray_count ? b : p.ray_count;
+        end = b + per < p.ray_count ? b + per : p.ray_count;
+    }
+
+    // This lane is idle and rays may be left for it.
+    __device__ bool can_take(const TraceParams& p, bool active) const {
+        return !active && (ranges ? cursor < end : rid < p.ray_count);
+    }
+
+    // Called by the whole wave when it hands out rays (takers = ballot of
+    // can_take); true when this lane got ray `rid`.
+    __device__ bool take(bool me, uint64_t takers) {
+        if (!ranges) return me;
+        // takers in the lanes below this one (v_mbcnt)
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+            uint32_t(takers >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(takers), 0u));
+        const uint64_t r = cursor + below;
+        cursor += uint64_t(__popcll(takers));
+        if (cursor > end) cursor = end;
+        if (!me || r >= end) return false;
+        rid = r;
+        return true;
+    }
+
+    // The lane's ray is finished (or skipped).
+    __device__ void next() {
+        if (!ranges) rid += stride;
+    }
+};
+
 template <bool ANY, bool STATS, bool BATCH>
 __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
     if (p.gate && *p.gate) return;  // the tree this launch would read is incomplete
@@ -347,18 +398,9 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
     DeepStack deep;
     int sp = 0;
     uint64_t n_nodes = 0, n_tris = 0;
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    uint64_t rid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    RaySchedule sched;
+    sched.init(p);
     const uint32_t refill = p.refill ? p.refill : 1u;
-    // dynamic schedule: the rays are cut into p.work_chunks chunks with one
-    // counter each; a wave takes rays from its home chunk, then from the next
-    // ones once that is used up
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t lanes_below = lane ? ((~0ull) >> (64u - lane)) : 0ull;
-    const uint32_t chunks = p.work ? p.work_chunks : 1u;
-    const uint64_t chunk_len = (p.ray_count + chunks - 1) / chunks;
-    uint32_t home = uint32_t(((uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6) % chunks);
-    bool drained = false;  // wave-uniform: every chunk used up
     float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f,
           iz = 0.f, tmin = 0.f;
     float lox = 0.f, loy = 0.f, loz = 0.f, hox = 0.f, hoy = 0.f, hoz = 0.f;  // slab origins
@@ -370,45 +412,14 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
     bool active = false;
     for (;;) {
         // ---- refill: idle lanes that still have rays take them together
-        const bool can_take = !active && (p.work ? !drained : rid < p.ray_count);
+        const bool can_take = sched.can_take(p, active);
         const uint64_t takers = __ballot(can_take);
         const uint64_t busy = __ballot(active);
         if (takers == 0 && busy == 0) break;  // wave-uniform: no ray left for any lane
         bool take = false;
-        if (takers != 0 && (uint32_t(__popcll(takers)) >= refill || busy == 0)) {
-            if (p.work) {
-                // one atomic per wave: the first taker claims a run of rays for all
-                const int leader = __ffsll((unsigned long long)takers) - 1;
-                const uint32_t n = uint32_t(__popcll(takers));
-                bool got = false;
-                uint64_t first = 0, end = 0;
-                for (uint32_t tries = 0; tries < chunks && !got; ++tries) {
-                    uint32_t old = 0;
-                    if (int(lane) == leader) old = atomicAdd(&p.work[home], n);
-                    old = uint32_t(__shfl(int(old), leader));
-                    const uint64_t cb = uint64_t(home) * chunk_len;
-                    const uint64_t ce = cb + chunk_len < p.ray_count ? cb + chunk_len : p.ray_count;
-                    if (cb + old < ce) {
-                        got = true;
-                        first = cb + old;
-                        end = ce;
-                    } else {
-                        home = home + 1 == chunks ? 0u : home + 1;
-                    }
-                }
-                if (!got) {
-                    drained = true;
-                } else if (can_take) {
-                    const uint64_t r = first + uint64_t(__popcll(takers & lanes_below));
-                    if (r < end) {
-                        rid = r;
-                        take = true;
-                    }
-                }
-            } else {
-                take = can_take;
-            }
-        }
+        if (takers != 0 && (uint32_t(__popcll(takers)) >= refill || busy == 0))
+            take = sched.take(can_take, takers);
+        const uint64_t rid = sched.rid;
         if (take) {
             const float4* r4 = reinterpret_cast<const float4*>(p.rays + rid);
             const float4 ra = r4[0];
@@ -439,7 +450,7 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
                     reinterpret_cast<float4*>(p.hits)[rid] =
                         make_float4(bt, 0.f, 0.f, __uint_as_float(RTBVH_MISS));
                 }
-                if (!p.work) rid += stride;
+                sched.next();
             } else {
                 active = true;
                 node = BATCH ? int32_t(scene_of_ray(p.ray_offsets, p.scene_count,
@@ -595,7 +606,7 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
             active = false;
             node = -1;
             lcount = 0;
-            if (!p.work) rid += stride;
+            sched.next();
         }
     }
     if (STATS) {
@@ -622,12 +633,25 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
 // (a newer triangle group then pushes the waiting one).  A slot is entered when max(t_near, t_min) <= min(t_far, t_best)
 // with t_near lowered and t_far raised by 2^-20 of their magnitude; with the
 // grid step kept free around every box, this never rejects a box that the
-// exact slab test would enter.  Where a grid step times 1/d overflows, the
-// plane distances are infinite or NaN (0 * inf); a NaN drops out of the
-// min/max, which can only enter more slots.  Which triangles are tested can therefore
+// exact slab test would enter.  Along an axis whose 1/d is infinite (a zero
+// direction component) the plane distances would be infinite or NaN (0 * inf),
+// and a NaN dropping out of min/max would leave that axis unculled; such an
+// axis uses the distance H (q - u) instead, u the ray's origin in grid steps
+// and H = 2^118: a slot whose planes lie a step or more beyond the origin is
+// out of reach (entry >= H, beyond any t_max up to 3e35), one that holds the
+// origin is open along that axis (entry <= -H, exit >= H).  The spare grid
+// step makes every decision within a step of a plane safe either way.  The
+// offset -H u is clamped to +-2^126, so that H q - H u stays finite (an
+// infinite distance would turn NaN in the 2^-20 widening).  Which triangles are tested can therefore
 // only grow, and the lowest (t, id) is that of brute force.
 constexpr uint32_t kNodeGroup = 1u << 24;
 constexpr float kBoxEps = 0x1p-20f;
+constexpr float kFlatScale = 0x1p118f;
+
+// -H u of a flat axis, clamped so that H q - H u (q <= 255) stays finite.
+__device__ __forceinline__ float flat_offset(float u) {
+    return fminf(fmaxf(-kFlatScale * u, -0x1p126f), 0x1p126f);
+}
 
 // bit s of an 8-bit mask moves to bit s ^ oct
 __device__ __forceinline__ uint32_t oct_permute(uint32_t m, uint32_t oct) {
@@ -674,7 +698,8 @@ __device__ __forceinline__ float qbyte(uint32_t w, int i) {
 }
 
 template <bool ANY, bool STATS, bool BATCH>
-__global__ void __launch_bounds__(kMaxBlock) cw8_kernel(TraceParams p) {
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(8)))
+cw8_kernel(TraceParams p) {
     if (p.gate && *p.gate) return;  // the tree this launch would read is incomplete
     extern __shared__ int32_t s_stack[];  // p.lds_depth groups (8 B) per lane
     // LDS byte address of this lane's first entry (the low half of a shared
@@ -685,13 +710,14 @@ __global__ void __launch_bounds__(kMaxBlock) cw8_kernel(TraceParams p) {
     uint2 deep[RTBVH_MAX_STACK];
     int sp = 0;
     uint64_t n_nodes = 0, n_tris = 0;
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    uint64_t rid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    RaySchedule sched;
+    sched.init(p);
     const uint32_t refill = p.refill ? p.refill : 1u;
     float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f,
           iz = 0.f, tmin = 0.f;
     float bt = 0.f, bu = 0.f, bv = 0.f;
     uint32_t bp = RTBVH_MISS, oct = 0;
+    uint32_t flat = 0;          // axes whose 1/d is infinite
     uint32_t ngx = 0, ngy = 0;  // node group in hand
     uint32_t tgx = 0, tgy = 0;  // triangle group in hand
     bool active = false;
@@ -708,11 +734,15 @@ __global__ void __launch_bounds__(kMaxBlock) cw8_kernel(TraceParams p) {
     };
     for (;;) {
         // ---- refill: idle lanes that still have rays take them together
-        const bool can_take = !active && rid < p.ray_count;
+        const bool can_take = sched.can_take(p, active);
         const uint64_t takers = __ballot(can_take);
         const uint64_t busy = __ballot(active);
         if (takers == 0 && busy == 0) break;  // wave-uniform: no ray left for any lane
-        if (can_take && (uint32_t(__popcll(takers)) >= refill || busy == 0)) {
+        bool take = false;
+        if (takers != 0 && (uint32_t(__popcll(takers)) >= refill || busy == 0))
+            take = sched.take(can_take, takers);
+        const uint64_t rid = sched.rid;
+        if (take) {
             const float4* r4 = reinterpret_cast<const float4*>(p.rays + rid);
             const float4 ra = r4[0];
             const float4 rb = r4[1];
@@ -739,11 +769,15 @@ __global__ void __launch_bounds__(kMaxBlock) cw8_kernel(TraceParams p) {
                     reinterpret_cast<float4*>(p.hits)[rid] =
                         make_float4(bt, 0.f, 0.f, __uint_as_float(RTBVH_MISS));
                 }
-                rid += stride;
+                sched.next();
             } else {
                 active = true;
                 oct = (__float_as_uint(ix) >> 31) | ((__float_as_uint(iy) >> 31) << 1) |
                       ((__float_as_uint(iz) >> 31) << 2);
+                flat = (fabsf(ix) > 3.4028235e38f ? 1u : 0u) |
+                       (fabsf(iy) > 3.4028235e38f ? 2u : 0u) |
+                       (fabsf(iz) > 3.4028235e38f ? 4u : 0u);
+                oct &= ~flat;  // a flat axis reads its low planes as near ones
                 // the root as a node group of one interior slot (slot 0), bound 0
                 ngx = BATCH ? scene_of_ray(p.ray_offsets, p.scene_count, uint32_t(rid)) : 0u;
                 ngy = (1u << (24 + oct)) | (1u << 16);
@@ -801,12 +835,30 @@ __global__ void __launch_bounds__(kMaxBlock) cw8_kernel(TraceParams p) {
                 const uint4* q = reinterpret_cast<const uint4*>(p.cnodes + idx);
                 const uint4 h0 = q[0], h1 = q[1], qa = q[2], qb = q[3], qc = q[4];
                 // grid step times 1/d, and the grid origin's distance
-                const float sx = __uint_as_float((h0.w & 0xFFu) << 23) * ix;
-                const float sy = __uint_as_float(((h0.w >> 8) & 0xFFu) << 23) * iy;
-                const float sz = __uint_as_float(((h0.w >> 16) & 0xFFu) << 23) * iz;
-                const float bx = (__uint_as_float(h0.x) - ox) * ix;
-                const float by = (__uint_as_float(h0.y) - oy) * iy;
-                const float bz = (__uint_as_float(h0.z) - oz) * iz;
+                const float stx = __uint_as_float((h0.w & 0xFFu) << 23);
+                const float sty = __uint_as_float(((h0.w >> 8) & 0xFFu) << 23);
+                const float stz = __uint_as_float(((h0.w >> 16) & 0xFFu) << 23);
+                float sx = stx * ix, sy = sty * iy, sz = stz * iz;
+                float bx = (__uint_as_float(h0.x) - ox) * ix;
+                float by = (__uint_as_float(h0.y) - oy) * iy;
+                float bz = (__uint_as_float(h0.z) - oz) * iz;
+                if (flat != 0u) {
+                    // along a zero direction component: plane distance H (q - u), u the
+                    // ray's origin in grid steps, so that a plane a step or more beyond
+                    // the origin is out of reach and one behind it is -H or lower
+                    if (flat & 1u) {
+                        sx = kFlatScale;
+                        bx = flat_offset((ox - __uint_as_float(h0.x)) / stx);
+                    }
+                    if (flat & 2u) {
+                        sy = kFlatScale;
+                        by = flat_offset((oy - __uint_as_float(h0.y)) / sty);
+                    }
+                    if (flat & 4u) {
+                        sz = kFlatScale;
+                        bz = flat_offset((oz - __uint_as_float(h0.z)) / stz);
+                    }
+                }
                 // near and far planes: low planes first along a positive direction
                 const bool nx = (oct & 1u) != 0u, ny = (oct & 2u) != 0u, nz = (oct & 4u) != 0u;
                 const uint32_t nx0 = nx ? qb.z : qa.x, nx1 = nx ? qb.w : qa.y;
@@ -883,38 +935,13 @@ __global__ void __launch_bounds__(kMaxBlock) cw8_kernel(TraceParams p) {
             active = false;
             ngy = 0;
             tgy = 0;
-            rid += stride;
+            sched.next();
         }
     }
     if (STATS) {
         atomicAdd(reinterpret_cast<unsigned long long*>(p.stats), (unsigned long long)n_nodes);
         atomicAdd(reinterpret_cast<unsigned long long*>(p.stats) + 1, (unsigned long long)n_tris);
     }
-}
-
-// Counter blocks of the dynamic schedule: a ring per device, one block per
-// launch (cleared by a kernel in the launch's stream).  A launch that brings
-// its own block (p.work) uses that instead.
-constexpr uint32_t kWorkChunks = 256;
-constexpr uint32_t kWorkSlots = 256;
-__device__ uint32_t g_work_ring[kWorkSlots * kWorkChunks];
-std::atomic<uint32_t> g_work_next{0};
-
-uint32_t* work_ring_ptr() {
-    static uint32_t* ptrs[64] = {nullptr};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-    if (ptrs[dev] == nullptr) {
-        void* q = nullptr;
-        if (hipGetSymbolAddress(&q, HIP_SYMBOL(g_work_ring)) != hipSuccess) return nullptr;
-        ptrs[dev] = static_cast<uint32_t*>(q);
-    }
-    return ptrs[dev];
-}
-
-__global__ void clear_words_kernel(uint32_t* w, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) w[i] = 0;
 }
 
 // Launch shape per device: block size, LDS stack depth and the share of the
@@ -953,32 +980,38 @@ hipError_t launch_fast(const TraceParams& p0, hipStream_t stream) {
     }
     uint64_t blocks = uint64_t(cache_blocks[dev]) * cfg.occupancy_pct / 100;
     if (blocks == 0) blocks = 1;
+    // Wave ranges need a few rays per lane: with r rays per resident lane
+    // they run with refill min(refill, r / 2) and enough waves to fill the
+    // device, but no more than leave each lane max(refill, 8) rays; launches
+    // with fewer than 2 rays per resident lane (a scene of a few thousand
+    // triangles) run the static schedule with refill 1.
+    const uint64_t resident = uint64_t(cache_blocks[dev]);
+    const uint64_t rays_per_lane = p0.ray_count / (resident * cfg.block);
+    const bool ranges = (ITEM || CW8) && cfg.schedule == 1 && rays_per_lane >= 2;
+    uint32_t refill = cfg.schedule == 1 ? 1u : cfg.refill;
+    if (ranges) {
+        refill = uint32_t(rays_per_lane / 2 < cfg.refill ? rays_per_lane / 2 : cfg.refill);
+        if (refill == 0) refill = 1;
+        const uint64_t per_block = uint64_t(cfg.block) * (refill > 8 ? refill : 8u);
+        uint64_t most = (p0.ray_count + per_block - 1) / per_block;
+        if (most < resident) most = resident;
+        if (most < blocks) blocks = most;
+    }
     const uint64_t need = (p0.ray_count + cfg.block - 1) / cfg.block;
     const int grid = int(need < blocks ? need : blocks);
     if (grid == 0) return hipSuccess;
     TraceParams p = p0;
     p.lds_depth = cfg.lds_depth;
-    p.refill = cfg.refill;
+    p.refill = refill;
     p.postpone = cfg.postpone;
-    if (ITEM && !CW8 && cfg.schedule == 1) {
-        if (!p.work) {
-            uint32_t* ring = work_ring_ptr();
-            if (ring == nullptr) return hipErrorInvalidSymbol;
-            p.work = ring + size_t(g_work_next.fetch_add(1) % kWorkSlots) * kWorkChunks;
-        }
-        p.work_chunks = kWorkChunks;
-        hipLaunchKernelGGL(clear_words_kernel, dim3((kWorkChunks + 255) / 256), dim3(256), 0,
-                           stream, p.work, kWorkChunks);
-    } else {
-        p.work = nullptr;
-    }
+    p.wave_ranges = ranges ? 1u : 0u;
     hipLaunchKernelGGL(k, dim3(grid), dim3(cfg.block), lds, stream, p);
     return hipGetLastError();
 }
 
 }  // namespace
 
-TraceConfig default_trace_config() { return TraceConfig{64, 8, 800, 0, 1, 0, 0}; }
+TraceConfig default_trace_config() { return TraceConfig{64, 8, 800, 0, 16, 1, 0}; }
 
 bool valid_trace_config(const TraceConfig& c) {
     return (c.block == 64 || c.block == 128 || c.block == 256) && c.lds_depth <= 32 &&

@@ -394,3 +394,10 @@ def test_trace_tree8_flat_scene_axis_rays(offset):
             print(f"  {name} ray {i}: {rays8[i].tolist()} got {got[i]} want {want[i]}", flush=True)
         assert same.all(), name
     assert (want["prim"] != 0xFFFFFFFF).sum() > n // 8
+    # a zero direction component must still cull along its axis: the
+    # compressed tree visits no more nodes than the 4-wide one on these rays
+    # (NaN plane distances would open every slot along that axis)
+    _, n8, _ = tt.intersect_stats(dr)
+    _, n4, _ = bvh.trace_tree(dv, width=4).intersect_stats(dr)
+    print(f"  nodes/ray 8-wide {n8 / n:.2f}, 4-wide {n4 / n:.2f}", flush=True)
+    assert n8 <= n4

@@ -220,29 +220,36 @@ def test_launch_shapes_and_autotune_keep_results():
 
 def test_compressed_tree_launch_shapes_and_autotune():
     """The same for the compressed 8-wide tree's kernel: every launch shape
-    (LDS stack depth 0 puts the whole group stack in private memory) and the
-    autotuner's pick give the 4-wide tree's hit records and occlusion flags."""
+    (LDS stack depth 0 puts the whole group stack in private memory; both
+    ray schedules -- 1.2M rays, so that wave ranges engage with refill 1;
+    triangle postponing) and the autotuner's pick give the 4-wide tree's hit records
+    and occlusion flags."""
     print("[launch_shapes_cw8]", flush=True)
     verts = np.concatenate([scenes.noisy_sphere(100, 80, seed=12),
                             scenes.random_soup(5000, 0.02, seed=13)])
     aabbs, centers = prep(verts)
     bvh = rt.Builder(_dev(centers), aabbs=_dev(flat(aabbs))).construct_locally_ordered_clustered()
     dv = _dev(verts)
-    rays = _dev(ray_set(verts, 50000, seed=22, kind="box"))
+    # 1.2M rays: about 2 per resident lane, so the wave-range schedule engages
+    rays = _dev(ray_set(verts, 1_200_000, seed=22, kind="box"))
+    rt.set_trace_config(64, 8, 100, 0, 1, 0)
     want = bvh.trace_tree(dv, width=4).intersect(rays).cpu().numpy()
+    rt.set_trace_config(None)
     want_occ = (want[:, 3].view(np.uint32) != 0xFFFFFFFF).astype(np.uint8)
     tt = bvh.trace_tree(dv)
     assert tt.width == 8
     default = rt.get_trace_config()
     try:
-        for block, depth, occ, refill in ((64, 0, 100, 1), (128, 4, 50, 8), (256, 24, 200, 64),
-                                          (64, 32, 1600, 16), (64, 12, 3200, 1)):
-            rt.set_trace_config(block, depth, occ, 0, refill, 0)
+        for block, depth, occ, refill, sched, post in (
+                (64, 0, 100, 1, 0, 0), (128, 4, 50, 8, 0, 16), (256, 24, 200, 64, 0, 0),
+                (64, 32, 1600, 16, 0, 64), (64, 12, 3200, 1, 0, 0), (64, 8, 400, 1, 1, 0),
+                (128, 0, 100, 2, 1, 32), (256, 8, 3200, 1, 1, 0), (64, 8, 800, 64, 1, 0)):
+            rt.set_trace_config(block, depth, occ, 0, refill, sched, post)
             got = tt.intersect(rays).cpu().numpy()
             assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (block, depth, occ)
             assert np.array_equal(tt.occluded(rays).cpu().numpy(), want_occ), (block, depth, occ)
         best = rt.autotune(tt, rays)
-        assert rt.get_trace_config() == best and best["schedule"] == 0
+        assert rt.get_trace_config() == best
         got = tt.intersect(rays).cpu().numpy()
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
         tt.status()
