@@ -290,10 +290,12 @@ ResultCode rtbvh_gpu_create_trace_tree(RTBvh bvh, const float *vertices, size_t 
                                        uint32_t max_leaf, void *stream, RTTraceTree *result);
 /* The compressed 8-wide traversal tree: 80-byte nodes whose eight slot boxes
  * are 8-bit steps on a per-node power-of-two grid, rounded outward with one
- * step to spare; slots placed by direction octant so that a ray visits the
+ * step to spare; the binary tree spread over the slots by an SAH-optimal
+ * distribution; slots placed by direction octant so that a ray visits the
  * near side first; one triangle per leaf slot.  Same hits, bit for bit, as
  * the 4-wide tree; traced and freed by the same calls.  Fails with Error on
- * non-finite bounds (the 4-wide tree accepts them). */
+ * non-finite bounds or binary leaves of more than 8 primitives (the 4-wide
+ * tree accepts both). */
 ResultCode rtbvh_gpu_create_trace_tree8(RTBvh bvh, const float *vertices, size_t prim_count,
                                         void *stream, RTTraceTree *result);
 ResultCode rtbvh_gpu_trace_tree_intersect(RTTraceTree tree, const RTRay *rays, size_t ray_count,

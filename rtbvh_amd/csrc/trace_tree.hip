@@ -132,6 +132,7 @@ struct LevelArgs {
     uint32_t max_leaf;
     DevMbvhNode* wnodes;
     DevCw8Node* cnodes;  // width 8: compressed nodes instead of wnodes
+    const uint32_t* dpd;  // width 8: slot-distribution decisions per binary node
     uint32_t wcap;
     float4* tris;
     uint32_t* counters;  // [0] wide nodes, [1] triangles, [2] fail
@@ -318,11 +319,6 @@ __device__ bool cand_binary(const LevelArgs& a, int32_t i, Cand* c, int& ns) {
     return n.extra2 >= 0;
 }
 
-__device__ __forceinline__ float cand_area(const Cand& c) {
-    const float dx = c.hi[0] - c.lo[0], dy = c.hi[1] - c.lo[1], dz = c.hi[2] - c.lo[2];
-    return (dx + dy) * dz + dx * dy;
-}
-
 // Grid of one axis: origin p at least one step below lo, step 2^(e-127) a
 // power of two with (hi - lo) / step <= 252 and step >= 2^-22 * max(|lo|,
 // |hi|) (so the spare step outlasts float rounding in the traversal).
@@ -354,11 +350,23 @@ __device__ __forceinline__ uint32_t quant_hi(float v, float p, double step) {
     return q <= 0.0 ? 0u : (q >= 255.0 ? 255u : uint32_t(q));
 }
 
-// One compressed node from queue item (b, w).  Slots are opened by largest
-// area (as the 4-wide tree does) up to eight, then placed so that slot s
-// lies towards the octant s (bit k of s set: the + side of axis k) of the
-// node's centre, greedily by alignment; a ray whose direction has signs O
-// visits keys s ^ O in increasing order, the near side first.
+// Decision nibble j (1..8) of binary node k: 0 the node takes one slot (an
+// interior wide node, or a triangle for a one-primitive leaf), kDissolve a
+// leaf of several primitives spreads them over their own slots, otherwise k
+// in 1..j-1 gives its left child k slots and its right child j - k.
+constexpr uint32_t kDissolve = 15u;
+
+__device__ __forceinline__ uint32_t dp_decision(const uint32_t* dpd, uint32_t k, uint32_t j) {
+    return (dpd[k] >> (4u * (j - 1u))) & 15u;
+}
+
+// One compressed node from queue item (b, w).  Its slots come from the
+// SAH-optimal distribution of b's subtree over eight slots (dp_level_kernel):
+// b's children get k and 8 - k slots, and each part either stays one slot
+// or spreads further.  The slots are then placed so that slot s lies towards
+// the octant s (bit k of s set: the + side of axis k) of the node's centre,
+// greedily by alignment; a ray whose direction has signs O visits keys
+// s ^ O in increasing order, the near side first.
 __device__ void make_node8(const LevelArgs& a, uint32_t b, uint32_t w, QItem* qout,
                            uint32_t* qout_count) {
     Cand c[kWide];
@@ -371,31 +379,46 @@ __device__ void make_node8(const LevelArgs& a, uint32_t b, uint32_t w, QItem* qo
         for (int32_t j = 0; ok && j < bn.extra1; ++j)
             ok = cand_prim(a, uint32_t(bn.extra2 + j), c[ns++]);
     } else {
-        ok = cand_binary(a, bn.extra2, c, ns) && cand_binary(a, bn.extra2 + 1, c, ns);
-        while (ok) {
-            int best = -1;
-            float best_area = -1.0f;
-            for (int k = 0; k < ns; ++k) {
-                const bool fits = c[k].kind == 0 ? ns + 1 <= kWide
-                                                 : (c[k].kind == 1 && ns - 1 + int(c[k].n) <= kWide);
-                if (!fits) continue;
-                const float ar = cand_area(c[k]);
-                if (!(ar <= best_area)) {
-                    best_area = ar;
-                    best = k;
-                }
+        // (binary node, slots) pairs still to place; at most 8 are pending
+        uint32_t st_node[kWide], st_slots[kWide];
+        int sp = 0;
+        const uint32_t k8 = dp_decision(a.dpd, b, 8u);
+        if (bn.extra2 < 0 || k8 == 0u || k8 >= 8u) {
+            ok = false;
+        } else {
+            st_node[sp] = uint32_t(bn.extra2) + 1u;
+            st_slots[sp++] = 8u - k8;
+            st_node[sp] = uint32_t(bn.extra2);
+            st_slots[sp++] = k8;
+        }
+        while (ok && sp > 0) {
+            --sp;
+            const uint32_t n = st_node[sp], j = st_slots[sp];
+            if (n >= a.node_count || j < 1u || j > 8u) {
+                ok = false;
+                break;
             }
-            if (best < 0) break;
-            const Cand open = c[best];
-            c[best] = c[ns - 1];
-            --ns;
-            DevAabb on;
-            ldn(a.nodes + open.id, on);
-            if (open.kind == 0) {
-                ok = cand_binary(a, on.extra2, c, ns) && cand_binary(a, on.extra2 + 1, c, ns);
-            } else {
-                for (uint32_t j = 0; ok && j < open.n; ++j)
-                    ok = ns < kWide && cand_prim(a, uint32_t(on.extra2) + j, c[ns++]);
+            DevAabb nd;
+            ldn(a.nodes + n, nd);
+            const uint32_t d = dp_decision(a.dpd, n, j);
+            if (nd.extra1 >= 0 && d == kDissolve) {  // its primitives, one slot each
+                if (nd.extra2 < 0 || uint32_t(nd.extra1) > j) {
+                    ok = false;
+                    break;
+                }
+                for (int32_t q = 0; ok && q < nd.extra1; ++q)
+                    ok = ns < kWide && cand_prim(a, uint32_t(nd.extra2 + q), c[ns++]);
+            } else if (nd.extra1 >= 0 || d == 0u) {  // one slot
+                ok = cand_binary(a, int32_t(n), c, ns);
+            } else {  // spread over j slots: k to the left child, j - k to the right
+                if (nd.extra2 < 0 || d >= j || sp + 2 > kWide) {
+                    ok = false;
+                    break;
+                }
+                st_node[sp] = uint32_t(nd.extra2) + 1u;
+                st_slots[sp++] = j - d;
+                st_node[sp] = uint32_t(nd.extra2);
+                st_slots[sp++] = d;
             }
         }
     }
@@ -516,6 +539,157 @@ __device__ void make_node8(const LevelArgs& a, uint32_t b, uint32_t w, QItem* qo
     a.cnodes[w] = m;
 }
 
+// ---- SAH-optimal distribution of binary subtrees over compressed slots ---
+//
+// cost(n, j): expected cost (surface area times visit cost) of binary node
+// n's subtree when it may occupy up to j slots of its parent's node:
+//   leaf of c primitives: j >= c: the c triangles, each in its own slot
+//   (sum of C_tri * area); j < c (c > 1): one slot holding a node of its own,
+//   C_node * area + the triangles;
+//   interior: min(one slot holding a node of its own: C_node * area +
+//   dist(n, 8); dist(n, j)), dist(n, j) = min over k of cost(left, k) +
+//   cost(right, j - k).
+// Computed level by level from the deepest binary level up, so that every
+// child is final before its parent.  The costs are relative; C_tri / C_node
+// is the measured cost of a triangle test against a node visit.
+constexpr float kDpNode = 1.0f;
+constexpr float kDpTri = 0.4f;
+constexpr uint32_t kMaxBinLevels = 8192;
+
+__device__ __forceinline__ float prim_area(const LevelArgs& a, uint32_t pos) {
+    if (pos >= a.index_count) return 0.0f;
+    const uint32_t prim = a.indices[pos];
+    if (prim >= a.prim_count) return 0.0f;
+    const float* v = a.verts + size_t(prim) * 9;
+    float d[3];
+    for (int k = 0; k < 3; ++k)
+        d[k] = fmaxf(fmaxf(v[k], v[3 + k]), v[6 + k]) - fminf(fminf(v[k], v[3 + k]), v[6 + k]);
+    return (d[0] + d[1]) * d[2] + d[0] * d[1];
+}
+
+// The binary tree breadth-first: level 0 holds the roots; level L is
+// order[off[L] .. off[L] + cnt[L]).  Cleared before each build.
+__global__ void bin_init_kernel(uint32_t roots, uint32_t* order, uint32_t* off, uint32_t* cnt,
+                                uint32_t count) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kMaxBinLevels + 2;
+         i += gridDim.x * blockDim.x) {
+        cnt[i] = i == 0 ? roots : 0u;
+        off[i] = 0u;
+        if (i < roots && i < count) order[i] = i;
+    }
+    for (uint32_t i = kMaxBinLevels + 2 + blockIdx.x * blockDim.x + threadIdx.x; i < roots;
+         i += gridDim.x * blockDim.x)
+        if (i < count) order[i] = i;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    bin_level_kernel(const DevAabb* nodes, uint32_t count, uint32_t* order, uint32_t* off,
+                     uint32_t* cnt, uint32_t level, uint32_t* fail) {
+    const uint32_t begin = off[level];
+    const uint32_t n = cnt[level];
+    const uint32_t end = begin + n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) off[level + 1] = end;
+    const uint32_t lane = threadIdx.x & 63u;
+    // whole waves iterate together, so that the slots of the next level are
+    // claimed with one atomic per wave
+    const uint32_t span = (n + 63u) & ~63u;
+    for (uint32_t w = blockIdx.x * kBlock + threadIdx.x - lane; w < span;
+         w += gridDim.x * kBlock) {
+        const uint32_t i = begin + w + lane;
+        bool interior = false;
+        DevAabb nd;
+        if (w + lane < n) {
+            if (i >= count) {
+                atomicOr(fail, 1u);
+            } else {
+                ldn(nodes + order[i], nd);
+                interior = nd.extra1 < 0;
+            }
+        }
+        const uint64_t mask = __ballot(interior);
+        if (mask == 0) continue;
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        uint32_t base = 0;
+        if (int(lane) == leader) base = atomicAdd(&cnt[level + 1], 2u * uint32_t(__popcll(mask)));
+        base = uint32_t(__shfl(int(base), leader));
+        if (!interior) continue;
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+            uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+        const uint32_t pos = end + base + 2u * below;
+        if (nd.extra2 < 0 || uint32_t(nd.extra2) + 1u >= count || pos + 1u >= count) {
+            atomicOr(fail, 1u);
+            continue;
+        }
+        order[pos] = uint32_t(nd.extra2);
+        order[pos + 1] = uint32_t(nd.extra2) + 1u;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    dp_level_kernel(LevelArgs a, const uint32_t* order, const uint32_t* off, const uint32_t* cnt,
+                    uint32_t level, float* cost, uint32_t* dec) {
+    const uint32_t begin = off[level];
+    const uint32_t end = begin + cnt[level];
+    for (uint32_t i = begin + blockIdx.x * kBlock + threadIdx.x; i < end;
+         i += gridDim.x * kBlock) {
+        const uint32_t k = order[i];
+        if (k >= a.node_count) {
+            atomicOr(&a.counters[2], 1u);
+            continue;
+        }
+        DevAabb nd;
+        ldn(a.nodes + k, nd);
+        float c[9];
+        uint32_t d = 0;
+        if (nd.extra1 >= 0) {
+            const uint32_t np = uint32_t(nd.extra1);
+            if (np > uint32_t(kWide)) atomicOr(&a.counters[2], 1u);
+            float tri = 0.0f;
+            for (uint32_t q = 0; q < np && q < uint32_t(kWide); ++q)
+                tri += kDpTri * prim_area(a, uint32_t(nd.extra2) + q);
+            const float one = np <= 1u ? tri : kDpNode * half_area(nd) + tri;
+            for (uint32_t j = 1; j <= 8u; ++j) {
+                c[j] = j >= np ? tri : one;
+                if (j >= np && np > 1u) d |= kDissolve << (4u * (j - 1u));
+            }
+        } else {
+            const uint32_t l = uint32_t(nd.extra2), r = l + 1u;
+            if (nd.extra2 < 0 || r >= a.node_count) {
+                atomicOr(&a.counters[2], 1u);
+                continue;
+            }
+            float cl[9], cr[9];
+            for (uint32_t j = 1; j <= 8u; ++j) {
+                cl[j] = cost[size_t(l) * 8 + j - 1];
+                cr[j] = cost[size_t(r) * 8 + j - 1];
+            }
+            float dist[9];
+            uint32_t dk[9];
+            for (uint32_t j = 2; j <= 8u; ++j) {
+                float best = 3.0e38f;
+                uint32_t bk = 1;
+                for (uint32_t q = 1; q < j; ++q) {
+                    const float v = cl[q] + cr[j - q];
+                    if (v < best) {
+                        best = v;
+                        bk = q;
+                    }
+                }
+                dist[j] = best;
+                dk[j] = bk;
+            }
+            c[1] = kDpNode * half_area(nd) + dist[8];
+            for (uint32_t j = 2; j <= 8u; ++j) {
+                const bool spread = j == 8u || dist[j] < c[1];
+                c[j] = spread ? dist[j] : c[1];
+                if (spread) d |= dk[j] << (4u * (j - 1u));
+            }
+        }
+        for (uint32_t j = 1; j <= 8u; ++j) cost[size_t(k) * 8 + j - 1] = c[j];
+        dec[k] = d;
+    }
+}
+
 __global__ void __launch_bounds__(kBlock)
     level8_kernel(LevelArgs a, const QItem* qin, const uint32_t* qin_count, QItem* qout,
                   uint32_t* qout_count) {
@@ -576,9 +750,16 @@ __global__ void finish_kernel(const uint32_t* last_q, uint32_t* counters, uint32
 
 struct TraceTreePlan {
     uint32_t count = 0, roots = 1, n = 0, wcap = 0, max_leaf = 1, levels = 0, width = 4;
+    uint32_t bin_levels = 0;    // width 8: binary-tree depth seen by the eager build
     uint32_t* cnt = nullptr;
     QItem* q[2] = {nullptr, nullptr};
     uint32_t* words = nullptr;  // [0..1] queue counts, [2..4] counters
+    // width 8: breadth-first binary order, its level offsets and counts, and
+    // the distribution costs (8 per node) and decisions (8 nibbles per node)
+    uint32_t* bord = nullptr;
+    uint32_t* blev = nullptr;
+    float* dpc = nullptr;
+    uint32_t* dpd = nullptr;
 };
 
 void trace_tree_plan_destroy(TraceTreePlan* t) {
@@ -587,6 +768,10 @@ void trace_tree_plan_destroy(TraceTreePlan* t) {
     if (t->q[0]) (void)hipFree(t->q[0]);
     if (t->q[1]) (void)hipFree(t->q[1]);
     if (t->words) (void)hipFree(t->words);
+    if (t->bord) (void)hipFree(t->bord);
+    if (t->blev) (void)hipFree(t->blev);
+    if (t->dpc) (void)hipFree(t->dpc);
+    if (t->dpd) (void)hipFree(t->dpd);
     delete t;
 }
 
@@ -608,6 +793,7 @@ LevelArgs level_args(const TraceTreePlan* t, const DeviceBvh& tree, const float*
     a.max_leaf = t->max_leaf;
     a.wnodes = out->nodes;
     a.cnodes = out->cnodes;
+    a.dpd = t->dpd;
     a.wcap = t->wcap;
     a.tris = out->tris;
     a.counters = t->words + 2;
@@ -621,10 +807,33 @@ hipError_t enqueue_setup(const TraceTreePlan* t, const DeviceBvh& tree, hipStrea
     if (t->width == 4)  // the compressed tree never collapses subtrees into leaves
         hipLaunchKernelGGL(capped_counts_kernel, dim3(g), dim3(kBlock), 0, stream, tree.nodes,
                            count, t->max_leaf, t->cnt);
+    else
+        hipLaunchKernelGGL(bin_init_kernel, dim3(64), dim3(kBlock), 0, stream, t->roots, t->bord,
+                           t->blev, t->blev + kMaxBinLevels + 2, count);
     const uint32_t gr = (t->roots + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(roots_kernel, dim3(gr), dim3(kBlock), 0, stream, t->roots, t->q[0],
                        t->words + 0, t->words + 2);
     return hipGetLastError();
+}
+
+constexpr uint32_t kBinGrid = 2048;  // persistent blocks per binary-level kernel
+
+void enqueue_bin_level(const TraceTreePlan* t, const DeviceBvh& tree, uint32_t level,
+                       hipStream_t stream) {
+    hipLaunchKernelGGL(bin_level_kernel, dim3(kBinGrid), dim3(kBlock), 0, stream, tree.nodes,
+                       t->count, t->bord, t->blev, t->blev + kMaxBinLevels + 2, level,
+                       t->words + 4);
+}
+
+void enqueue_dp_levels(const TraceTreePlan* t, const LevelArgs& a, uint32_t levels,
+                       hipStream_t stream) {
+    for (uint32_t l = levels; l-- > 0;)
+        hipLaunchKernelGGL(dp_level_kernel, dim3(kBinGrid), dim3(kBlock), 0, stream, a, t->bord,
+                           t->blev, t->blev + kMaxBinLevels + 2, l, t->dpc, t->dpd);
+}
+
+__global__ void bin_depth_check_kernel(const uint32_t* cnt, uint32_t levels, uint32_t* fail) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && cnt[levels] != 0) *fail |= 1u;
 }
 
 void enqueue_level(const TraceTreePlan* t, const LevelArgs& a, uint32_t level, uint32_t grid,
@@ -662,7 +871,12 @@ ResultCode trace_tree_plan_create(const DeviceBvh& tree, uint32_t max_leaf, uint
     if ((e = hipMalloc(&t->cnt, size_t(t->count) * 4)) != hipSuccess ||
         (e = hipMalloc(&t->q[0], size_t(t->wcap) * sizeof(QItem))) != hipSuccess ||
         (e = hipMalloc(&t->q[1], size_t(t->wcap) * sizeof(QItem))) != hipSuccess ||
-        (e = hipMalloc(&t->words, 8 * 4)) != hipSuccess) {
+        (e = hipMalloc(&t->words, 8 * 4)) != hipSuccess ||
+        (width == 8 &&
+         ((e = hipMalloc(&t->bord, size_t(t->count) * 4)) != hipSuccess ||
+          (e = hipMalloc(&t->blev, size_t(kMaxBinLevels + 2) * 2 * 4)) != hipSuccess ||
+          (e = hipMalloc(&t->dpc, size_t(t->count) * 8 * sizeof(float))) != hipSuccess ||
+          (e = hipMalloc(&t->dpd, size_t(t->count) * 4)) != hipSuccess))) {
         trace_tree_plan_destroy(t);
         return hip_fail(e, "trace tree workspace");
     }
@@ -693,6 +907,28 @@ ResultCode trace_tree_build(TraceTreePlan* t, const DeviceBvh& tree, const float
     hipError_t e = enqueue_setup(t, tree, stream);
     if (e != hipSuccess) return hip_fail(e, "trace tree setup");
     const LevelArgs a = level_args(t, tree, verts, prim_count, scene_off, scenes, out);
+    if (t->width == 8) {  // binary levels until one is empty, then the distribution
+        uint32_t bl = 0;
+        std::vector<uint32_t> counts;
+        for (;;) {
+            const uint32_t chunk = 24u;
+            if (bl + chunk > kMaxBinLevels) {
+                set_error("binary tree deeper than %u levels", kMaxBinLevels);
+                return Error;
+            }
+            for (uint32_t k = 0; k < chunk; ++k, ++bl) enqueue_bin_level(t, tree, bl, stream);
+            counts.resize(bl + 1);
+            if ((e = hipMemcpyAsync(counts.data(), t->blev + kMaxBinLevels + 2, (bl + 1) * 4,
+                                    hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+                (e = hipStreamSynchronize(stream)) != hipSuccess)
+                return hip_fail(e, "binary levels");
+            if (counts[bl] == 0) break;
+        }
+        uint32_t depth = bl;
+        while (depth > 0 && counts[depth - 1] == 0) --depth;
+        t->bin_levels = depth;
+        enqueue_dp_levels(t, a, depth, stream);
+    }
     uint32_t level = 0;
     uint32_t host_words[8];
     constexpr uint32_t kChunk = 4;
@@ -733,6 +969,14 @@ hipError_t trace_tree_enqueue(const TraceTreePlan* t, const DeviceBvh& tree, con
     hipError_t e = enqueue_setup(t, tree, stream);
     if (e != hipSuccess) return e;
     const LevelArgs a = level_args(t, tree, verts, prim_count, scene_off, scenes, out);
+    if (t->width == 8) {  // the eager build's binary depth plus headroom
+        uint32_t bl = t->bin_levels + (t->bin_levels / 4 > 8 ? t->bin_levels / 4 : 8);
+        if (bl > kMaxBinLevels) bl = kMaxBinLevels;
+        for (uint32_t l = 0; l < bl; ++l) enqueue_bin_level(t, tree, l, stream);
+        hipLaunchKernelGGL(bin_depth_check_kernel, dim3(1), dim3(1), 0, stream,
+                           t->blev + kMaxBinLevels + 2, bl, t->words + 4);
+        enqueue_dp_levels(t, a, bl, stream);
+    }
     for (uint32_t level = 0; level < levels; ++level) enqueue_level(t, a, level, kLevelGrid, stream);
     hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1), 0, stream, t->words + (levels & 1),
                        t->words + 2, t->n, fail);
