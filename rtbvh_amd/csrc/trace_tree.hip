@@ -582,23 +582,30 @@ __global__ void bin_init_kernel(uint32_t roots, uint32_t* order, uint32_t* off, 
         if (i < count) order[i] = i;
 }
 
-__global__ void __launch_bounds__(kBlock)
+// Children of level `level` appended as level + 1.  Whole blocks iterate
+// together, so that the next level's entries are claimed with one atomic per
+// block and tile: same-address atomics serialize, and one per wave was the
+// kernel's bottleneck on wide levels.
+constexpr int kBinBlock = 1024;
+constexpr int kBinWaves = kBinBlock / 64;
+
+__global__ void __launch_bounds__(kBinBlock)
     bin_level_kernel(const DevAabb* nodes, uint32_t count, uint32_t* order, uint32_t* off,
                      uint32_t* cnt, uint32_t level, uint32_t* fail) {
+    __shared__ uint32_t s_cnt[kBinWaves];
+    __shared__ uint32_t s_base[kBinWaves];
     const uint32_t begin = off[level];
     const uint32_t n = cnt[level];
     const uint32_t end = begin + n;
     if (blockIdx.x == 0 && threadIdx.x == 0) off[level + 1] = end;
     const uint32_t lane = threadIdx.x & 63u;
-    // whole waves iterate together, so that the slots of the next level are
-    // claimed with one atomic per wave
-    const uint32_t span = (n + 63u) & ~63u;
-    for (uint32_t w = blockIdx.x * kBlock + threadIdx.x - lane; w < span;
-         w += gridDim.x * kBlock) {
-        const uint32_t i = begin + w + lane;
+    const uint32_t wave = threadIdx.x >> 6;
+    for (uint32_t t = blockIdx.x * kBinBlock; t < n; t += gridDim.x * kBinBlock) {
+        const uint32_t idx = t + threadIdx.x;
+        const uint32_t i = begin + idx;
         bool interior = false;
         DevAabb nd;
-        if (w + lane < n) {
+        if (idx < n) {
             if (i >= count) {
                 atomicOr(fail, 1u);
             } else {
@@ -607,15 +614,22 @@ __global__ void __launch_bounds__(kBlock)
             }
         }
         const uint64_t mask = __ballot(interior);
-        if (mask == 0) continue;
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        uint32_t base = 0;
-        if (int(lane) == leader) base = atomicAdd(&cnt[level + 1], 2u * uint32_t(__popcll(mask)));
-        base = uint32_t(__shfl(int(base), leader));
+        if (lane == 0) s_cnt[wave] = 2u * uint32_t(__popcll(mask));
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t total = 0;
+            for (int k = 0; k < kBinWaves; ++k) {
+                s_base[k] = total;
+                total += s_cnt[k];
+            }
+            const uint32_t base = total ? atomicAdd(&cnt[level + 1], total) : 0u;
+            for (int k = 0; k < kBinWaves; ++k) s_base[k] += base;
+        }
+        __syncthreads();
         if (!interior) continue;
         const uint32_t below = __builtin_amdgcn_mbcnt_hi(
             uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
-        const uint32_t pos = end + base + 2u * below;
+        const uint32_t pos = end + s_base[wave] + 2u * below;
         if (nd.extra2 < 0 || uint32_t(nd.extra2) + 1u >= count || pos + 1u >= count) {
             atomicOr(fail, 1u);
             continue;
@@ -816,12 +830,13 @@ hipError_t enqueue_setup(const TraceTreePlan* t, const DeviceBvh& tree, hipStrea
     return hipGetLastError();
 }
 
-constexpr uint32_t kBinGrid = 2048;  // persistent blocks per binary-level kernel
+constexpr uint32_t kBinGrid = 2048;   // persistent blocks per distribution-level kernel
+constexpr uint32_t kBinLevelGrid = 512;  // blocks of kBinBlock per breadth-first level
 
 void enqueue_bin_level(const TraceTreePlan* t, const DeviceBvh& tree, uint32_t level,
                        hipStream_t stream) {
-    hipLaunchKernelGGL(bin_level_kernel, dim3(kBinGrid), dim3(kBlock), 0, stream, tree.nodes,
-                       t->count, t->bord, t->blev, t->blev + kMaxBinLevels + 2, level,
+    hipLaunchKernelGGL(bin_level_kernel, dim3(kBinLevelGrid), dim3(kBinBlock), 0, stream,
+                       tree.nodes, t->count, t->bord, t->blev, t->blev + kMaxBinLevels + 2, level,
                        t->words + 4);
 }
 
