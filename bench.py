@@ -377,6 +377,9 @@ def main():
                          "(and the copy calibration) for profiler runs; no JSON line")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape instead of autotuning")
+    ap.add_argument("--shape", default="",
+                    help="launch shape block,lds_stack,occupancy_pct,kernel,refill,schedule,"
+                         "postpone instead of autotuning (to profile the shape a run picked)")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="GiB copied by one plain device copy at the end (scales rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -515,7 +518,14 @@ def main():
     # launch shape chosen by the autotuner on the bounce rays themselves
     # (untimed set-up, like the builds; a smaller sample favours smaller
     # persistent grids than the full launch wants)
-    if args.no_autotune:
+    if args.shape:
+        keys = ("block", "lds_stack", "occupancy_pct", "kernel", "refill", "schedule", "postpone")
+        vals = [int(x) for x in args.shape.split(",")]
+        if len(vals) != len(keys):
+            raise SystemExit(f"--shape takes {len(keys)} integers: {','.join(keys)}")
+        rt.set_trace_config(**dict(zip(keys, vals)))
+        tuned = rt.get_trace_config()
+    elif args.no_autotune:
         tuned = rt.get_trace_config()
     else:
         tuned = rt.autotune(ttree, bounce)
