@@ -401,3 +401,59 @@ def test_trace_tree8_flat_scene_axis_rays(offset):
     _, n4, _ = bvh.trace_tree(dv, width=4).intersect_stats(dr)
     print(f"  nodes/ray 8-wide {n8 / n:.2f}, 4-wide {n4 / n:.2f}", flush=True)
     assert n8 <= n4
+
+
+@pytest.mark.parametrize("scale,offset", [(0.1, 0.0), (1.0, 1.0e6), (1.0e12, 0.0),
+                                          (1.0, -3.0e4)])
+def test_trace_tree8_scales_and_offsets(scale, offset):
+    """The compressed tree's per-node grids (a power-of-two step no finer
+    than 2^-22 of the coordinate magnitude, origin a step below the node)
+    stay conservative for small, huge and far-off scenes: closest hits and
+    occlusion equal brute force bit for bit, t_min scaled with the scene."""
+    print(f"[trace_tree8_scales] scale={scale} offset={offset}", flush=True)
+    base = np.concatenate([scenes.noisy_sphere(40, 30, seed=61),
+                           scenes.random_soup(1500, 0.04, seed=62)])
+    verts = (base.astype(np.float64) * scale + offset).astype(np.float32)
+    rays8 = ray_set(verts, 8000, seed=63, kind="box")
+    rays8[:, 3] = np.float32(1e-4 * scale)
+    rays8[1::4, 7] = np.float32(0.3 * scale)
+    bvh, _, _ = _build(verts, "ploc")
+    dv, dr = _dev(verts), _dev(rays8)
+    tt = bvh.trace_tree(dv, width=8)
+    _check_cw8(tt, verts)
+    want = ref.trace_brute(verts, as_rays(rays8), threads=8)
+    got = tt.intersect(dr).cpu().numpy().view(ref.HIT).reshape(-1)
+    same = hits_equal(got, want)
+    for i in np.flatnonzero(~same)[:4]:
+        print(f"  ray {i}: {rays8[i].tolist()} got {got[i]} want {want[i]}", flush=True)
+    assert same.all()
+    assert (want["prim"] != 0xFFFFFFFF).sum() > len(rays8) // 8
+    occ = tt.occluded(dr).cpu().numpy().astype(bool)
+    assert np.array_equal(occ, want["prim"] != 0xFFFFFFFF)
+
+
+def test_trace_tree8_duplicates_and_degenerates():
+    """Exact duplicates (equal t for several ids), zero-area triangles and a
+    cluster of triangles sharing one point: the compressed tree returns the
+    lowest id among equal t, as brute force in id order does."""
+    print("[trace_tree8_duplicates]", flush=True)
+    rng = np.random.default_rng(71)
+    soup = scenes.random_soup(600, 0.05, seed=72)
+    dup = np.repeat(soup[:100], 5, axis=0)            # five copies of each
+    line = soup[100:160].copy()
+    line[:, 6:9] = line[:, 3:6]                        # two equal vertices: zero area
+    point = np.tile(np.float32([0.5, 0.5, 0.5]), (40, 3))  # all at one point
+    verts = np.concatenate([dup, soup[160:], line, point])
+    verts = verts[rng.permutation(len(verts))].astype(np.float32)
+    rays8 = ray_set(verts, 12000, seed=73, kind="box")
+    bvh, _, _ = _build(verts, "ploc")
+    dv, dr = _dev(verts), _dev(rays8)
+    tt = bvh.trace_tree(dv, width=8)
+    _check_cw8(tt, verts)
+    want = ref.trace_brute(verts, as_rays(rays8), threads=8)
+    got = tt.intersect(dr).cpu().numpy().view(ref.HIT).reshape(-1)
+    assert hits_equal(got, want).all()
+    m = rt.Mbvh.from_bvh(bvh)
+    assert np.array_equal(m.intersect(dv, dr).cpu().numpy().view(ref.HIT).reshape(-1)["prim"],
+                          want["prim"])
+    assert (want["prim"] != 0xFFFFFFFF).sum() > len(rays8) // 20
