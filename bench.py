@@ -96,9 +96,14 @@ def run_batched(args, dev, rt, scenes, torch):
         torch.cuda.synchronize()
         return b, t1
 
-    # warm-up: kernels loaded, allocator primed
+    # warm-up: kernels loaded, allocator primed; the launch shape is chosen
+    # by the autotuner on the batch's own rays (untimed set-up, like the
+    # builds) and serves the per-scene launches too
     plain(0, 4, False)
     b, _ = batched()
+    outer_shape = rt.get_trace_config()
+    shape = rt.autotune(b, rays, roff_d)
+    log(f"batch launch shape {shape}")
     del b
     plain_ms, batch_ms, batch_build_ms, batch_trace_ms = [], [], [], []
     trees = None
@@ -141,6 +146,7 @@ def run_batched(args, dev, rt, scenes, torch):
     plan.status()
     same_graph = bool(torch.equal(hits_graph.view(torch.int32), hits_batch.view(torch.int32)))
     del hits_graph, plan
+    rt.set_trace_config(**outer_shape)
     p_ms, b_ms = statistics.median(plain_ms), statistics.median(batch_ms)
     g_ms = statistics.median(graph_ms)
     bt_ms = statistics.median(batch_trace_ms)
@@ -151,6 +157,7 @@ def run_batched(args, dev, rt, scenes, torch):
         "plain_ms": round(p_ms, 2),
         "batched_ms": round(b_ms, 2),
         "plain_trace_ms": round(plain_trace_ms, 2),
+        "launch_shape": shape,
         "batched_build_ms": round(statistics.median(batch_build_ms), 2),
         "batched_trace_ms": round(bt_ms, 2),
         "batched_mrays_s": round(n_rays / (bt_ms * 1e-3) / 1e6, 1),

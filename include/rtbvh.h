@@ -233,8 +233,8 @@ ResultCode rtbvh_gpu_intersect_mbvh_stats(RTMbvh mbvh, const float *vertices, si
 
 /* Attaches to an MBVH made by rtbvh_gpu_create_mbvh a device copy of its
  * triangles in leaf order (v0 and the edges v1 - v0, v2 - v0 per index
- * entry, 48 bytes each), taken from `vertices` (device, 9 floats per
- * primitive).  The copy is a snapshot: attach again after the vertices
+ * entry, 64 bytes each: 48 used, 16 padding), taken from `vertices`
+ * (device, 9 floats per primitive).  The copy is a snapshot: attach again after the vertices
  * change.  Freed with the MBVH. */
 ResultCode rtbvh_gpu_mbvh_attach_triangles(RTMbvh mbvh, const float *vertices, size_t prim_count,
                                            void *stream);
@@ -268,7 +268,7 @@ ResultCode rtbvh_gpu_rays_from_packets(const RTRayPacket4 *packets, size_t packe
  * only (no counterpart in the reference): each node opens the child with the
  * largest surface area until it has four slots, slot bounds are the
  * children's own, and subtrees of at most max_leaf primitives become one
- * leaf whose triangles are stored contiguously (48 bytes each).  Hits are
+ * leaf whose triangles are stored contiguously (64 bytes each).  Hits are
  * identical to rtbvh_gpu_intersect_mbvh's default mode on the same
  * triangles; the tree is a snapshot of `vertices`. */
 typedef struct RTTraceTree {
@@ -280,8 +280,9 @@ typedef struct RTTraceTree {
     uint32_t pad_;
     const void *nodes;       /* device: width 4, 128 B RTMbvhNode-layout records; width 8,
                                 80 B compressed records (rtbvh_amd/csrc/device_math.hpp) */
-    const float *triangles;  /* device: 12 floats per primitive, leaf order: v0 with the
-                                primitive id's bits in [3], e1 = v1 - v0, e2 = v2 - v0 */
+    const float *triangles;  /* device: 16 floats per primitive, leaf order: v0 with the
+                                primitive id's bits in [3], e1 = v1 - v0, e2 = v2 - v0,
+                                4 floats of padding */
 } RTTraceTree;
 
 /* bvh: made by rtbvh_gpu_create_bvh; vertices: device, 9 floats per
@@ -411,6 +412,12 @@ ResultCode rtbvh_gpu_intersect_batch(RTBatch batch, const RTRay *rays, const uin
 ResultCode rtbvh_gpu_occluded_batch(RTBatch batch, const RTRay *rays, const uint32_t *ray_offsets,
                                     size_t ray_count, uint8_t *occluded, uint32_t flags,
                                     void *stream);
+/* rtbvh_gpu_autotune_trace_tree over a batch with attached triangles,
+ * timing rtbvh_gpu_intersect_batch on `rays` with per-scene `ray_offsets`;
+ * the shape list follows the batch's tree (compressed or 4-wide). */
+ResultCode rtbvh_gpu_autotune_batch(RTBatch batch, const RTRay *rays, const uint32_t *ray_offsets,
+                                    size_t ray_count, RTHit *hits, void *stream,
+                                    RTTraceConfig *best);
 void rtbvh_gpu_free_batch(RTBatch batch);
 
 /* ---- Capturable batch build + trace ----------------------------------------

@@ -109,6 +109,8 @@ SIGNATURES = {
     "rtbvh_gpu_autotune_mbvh": (_rc, [RTMbvh, _vp, _sz, _vp, _vp, ctypes.POINTER(RTTraceConfig)]),
     "rtbvh_gpu_autotune_trace_tree": (_rc, [RTTraceTree, _vp, _sz, _vp, _vp,
                                            ctypes.POINTER(RTTraceConfig)]),
+    "rtbvh_gpu_autotune_batch": (_rc, [RTBatch, _vp, _vp, _sz, _vp, _vp,
+                                      ctypes.POINTER(RTTraceConfig)]),
     "rtbvh_gpu_adjoint": (_rc, [_vp, _sz, _vp, _vp, _sz, _vp, _vp, _vp, _vp, _vp]),
     "rtbvh_gpu_batch_plan_create": (_rc, [_vp, _vp, _sz, _vp, _vp, _sz, _vp,
                                          ctypes.POINTER(RTBatchPlan)]),

@@ -286,9 +286,10 @@ class TraceTree:
         return _device_bytes(self._h.nodes, self.node_count * rec, self.device)
 
     def triangles(self) -> torch.Tensor:
-        """Copy of the leaf-ordered triangle records, (prim_count, 12) float32."""
-        b = _device_bytes(self._h.triangles, int(self._h.prim_count) * 48, self.device)
-        return b.view(torch.float32).reshape(-1, 12)
+        """Copy of the leaf-ordered triangle records, (prim_count, 12) float32
+        (v0 with the primitive id, e1, e2; the records' padding dropped)."""
+        b = _device_bytes(self._h.triangles, int(self._h.prim_count) * 64, self.device)
+        return b.view(torch.float32).reshape(-1, 16)[:, :12].contiguous()
 
     def intersect_stats(self, rays: torch.Tensor, out=None):
         """intersect() plus (nodes visited, triangles tested) over all rays."""
@@ -662,17 +663,26 @@ def set_trace_config(block: Optional[int] = None, lds_stack: int = 8, occupancy_
     L.check(L.lib().rtbvh_gpu_set_trace_config(ctypes.byref(c)), "rtbvh_gpu_set_trace_config")
 
 
-def autotune(tree, rays: torch.Tensor) -> dict:
+def autotune(tree, rays: torch.Tensor, ray_offsets: Optional[torch.Tensor] = None) -> dict:
     """Times the candidate launch shapes on `rays` against `tree` (an Mbvh with
-    attached triangles, or a TraceTree) and keeps the fastest for the current
-    device."""
+    attached triangles, a TraceTree, or a SceneBatch with its per-scene
+    `ray_offsets`) and keeps the fastest for the current device."""
     _require_device(rays, "rays")
     hits = torch.empty((rays.shape[0], HIT_FLOATS), dtype=torch.float32, device=rays.device)
     c = L.RTTraceConfig()
-    fn = (L.lib().rtbvh_gpu_autotune_trace_tree if isinstance(tree, TraceTree)
-          else L.lib().rtbvh_gpu_autotune_mbvh)
-    L.check(fn(tree._h, _ptr(rays), rays.shape[0], _ptr(hits), _stream(rays), ctypes.byref(c)),
-            fn.__name__)
+    if isinstance(tree, SceneBatch):
+        if ray_offsets is None or ray_offsets.device != rays.device \
+                or ray_offsets.element_size() != 4 or ray_offsets.numel() != tree.scene_count + 1:
+            raise ValueError("a SceneBatch needs ray_offsets: a 32-bit device tensor of "
+                             "scene_count + 1 entries")
+        fn = L.lib().rtbvh_gpu_autotune_batch
+        L.check(fn(tree._h, _ptr(rays), _ptr(ray_offsets.contiguous()), rays.shape[0],
+                   _ptr(hits), _stream(rays), ctypes.byref(c)), fn.__name__)
+    else:
+        fn = (L.lib().rtbvh_gpu_autotune_trace_tree if isinstance(tree, TraceTree)
+              else L.lib().rtbvh_gpu_autotune_mbvh)
+        L.check(fn(tree._h, _ptr(rays), rays.shape[0], _ptr(hits), _stream(rays),
+                   ctypes.byref(c)), fn.__name__)
     return {"block": c.block, "lds_stack": c.lds_stack, "occupancy_pct": c.occupancy_pct,
             "kernel": c.kernel, "refill": c.refill, "schedule": c.schedule,
             "postpone": c.postpone}

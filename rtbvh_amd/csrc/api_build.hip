@@ -423,7 +423,7 @@ ResultCode rtbvh_gpu_mbvh_attach_triangles(RTMbvh mbvh, const float* vertices, s
     }
     const uint32_t entries = me->dev.prim_count;
     if (me->dev.tris == nullptr && entries > 0) {
-        hipError_t e = hipMalloc(&me->dev.tris, size_t(entries) * 3 * sizeof(float4));
+        hipError_t e = hipMalloc(&me->dev.tris, size_t(entries) * kTriRecord * sizeof(float4));
         if (e != hipSuccess) {
             me->dev.tris = nullptr;
             return hip_fail(e, "hipMalloc(triangles)");

@@ -595,6 +595,19 @@ ResultCode rtbvh_gpu_autotune_trace_tree(RTTraceTree tree, const RTRay* rays, si
     });
 }
 
+ResultCode rtbvh_gpu_autotune_batch(RTBatch batch, const RTRay* rays, const uint32_t* ray_offsets,
+                                    size_t ray_count, RTHit* hits, void* stream,
+                                    RTTraceConfig* best) {
+    BatchView v;
+    if (!batch_view(batch.id, &v)) {
+        set_error("unknown batch id %u", batch.id);
+        return Error;
+    }
+    return autotune(rays, ray_count, hits, stream, best, v.cnodes != nullptr, [&]() {
+        return batch_entry(batch, rays, ray_offsets, ray_count, hits, nullptr, 0, stream);
+    });
+}
+
 ResultCode rtbvh_gpu_status(void* stream) {
     hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");

@@ -160,7 +160,8 @@ ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t st
 ResultCode gpu_build_mbvh(DeviceBvh& bvh, hipStream_t stream, DeviceMbvh* out);
 ResultCode gpu_refit(DeviceBvh& bvh, const RTAabb* aabbs, hipStream_t stream);
 ResultCode ensure_parents(DeviceBvh& bvh, hipStream_t stream);
-// Writes tris[3*pos..] for every index entry pos (leaf order).
+// Writes the record of every index entry pos (leaf order) at
+// tris[kTriRecord * pos].
 // scene_off (device, scenes + 1 entries) makes the stored ids scene-local.
 hipError_t launch_prepare_triangles(const uint32_t* indices, uint32_t index_count,
                                     const float* verts, uint32_t prim_count, float4* tris,

@@ -343,6 +343,14 @@ __device__ __forceinline__ int32_t slot_entry(int32_t c, int32_t n) {
 // idle lanes draw the next ones in lane order as they free up, so a wave's
 // lanes stay busy until its range is used up (no atomics: the cursor is
 // wave-uniform).
+// A value known to be equal in every lane of the wave, read from the first
+// active lane so that the compiler keeps it in scalar registers.
+__device__ __forceinline__ uint64_t wave_uniform(uint64_t v) {
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(v))));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(v >> 32))));
+    return (uint64_t(hi) << 32) | lo;
+}
+
 struct RaySchedule {
     uint64_t rid;
     uint64_t stride;
@@ -358,8 +366,10 @@ struct RaySchedule {
         const uint64_t waves = (stride + 63) >> 6;
         const uint64_t per = (p.ray_count + waves - 1) / waves;
         const uint64_t b = (tid >> 6) * per;
-        cursor = b < p.ray_count ? b : p.ray_count;
-        end = b + per < p.ray_count ? b + per : p.ray_count;
+        // the same for every lane of the wave (blocks are whole waves): held
+        // in scalar registers
+        cursor = wave_uniform(b < p.ray_count ? b : p.ray_count);
+        end = wave_uniform(b + per < p.ray_count ? b + per : p.ray_count);
     }
 
     // This lane is idle and rays may be left for it.
@@ -630,20 +640,24 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
 // in hand, which pushes the rest of that group) and tests at most one
 // triangle; with `postpone` set, the triangle test waits until that many
 // lanes of the wave have one or a lane has no node group left to visit
-// (a newer triangle group then pushes the waiting one).  A slot is entered when max(t_near, t_min) <= min(t_far, t_best)
-// with t_near lowered and t_far raised by 2^-20 of their magnitude; with the
-// grid step kept free around every box, this never rejects a box that the
-// exact slab test would enter.  Along an axis whose 1/d is infinite (a zero
-// direction component) the plane distances would be infinite or NaN (0 * inf),
-// and a NaN dropping out of min/max would leave that axis unculled; such an
-// axis uses the distance H (q - u) instead, u the ray's origin in grid steps
-// and H = 2^118: a slot whose planes lie a step or more beyond the origin is
-// out of reach (entry >= H, beyond any t_max up to 3e35), one that holds the
-// origin is open along that axis (entry <= -H, exit >= H).  The spare grid
-// step makes every decision within a step of a plane safe either way.  The
-// offset -H u is clamped to +-2^126, so that H q - H u stays finite (an
-// infinite distance would turn NaN in the 2^-20 widening).  Which triangles are tested can therefore
-// only grow, and the lowest (t, id) is that of brute force.
+// (a newer triangle group then pushes the waiting one).  A slot is entered
+// when max(t_near, t_min) <= min(t_far, t_best), the near planes computed
+// from the grid origin's distance b lowered and the far planes from b raised
+// by e = 2^-20 (|b| + 255 |s|), s the grid step over d: twice the rounding of
+// s, b (1-ulp reciprocal, two products) and the plane fma, so that no
+// computed near plane lies beyond the exact one and no far plane before it;
+// this never rejects a box that the exact slab test would enter.  Along an
+// axis whose 1/d is infinite (a zero direction component) the plane
+// distances would be infinite or NaN (0 * inf), and a NaN dropping out of
+// min/max would leave that axis unculled; such an axis uses the distance
+// H (q - u) instead, u the ray's origin in grid steps and H = 2^118: a slot
+// whose planes lie a step or more beyond the origin is out of reach
+// (entry >= H, beyond any t_max up to 3e35), one that holds the origin is
+// open along that axis (entry <= -H, exit >= H).  The spare grid step makes
+// every decision within a step of a plane safe either way.  The offset -H u
+// is clamped to +-2^126, so that H q - H u and the slack stay finite.  Which
+// triangles are tested can therefore only grow, and the lowest (t, id) is
+// that of brute force.
 constexpr uint32_t kNodeGroup = 1u << 24;
 constexpr float kBoxEps = 0x1p-20f;
 constexpr float kFlatScale = 0x1p118f;
@@ -716,8 +730,8 @@ cw8_kernel(TraceParams p) {
     float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f,
           iz = 0.f, tmin = 0.f;
     float bt = 0.f, bu = 0.f, bv = 0.f;
-    uint32_t bp = RTBVH_MISS, oct = 0;
-    uint32_t flat = 0;          // axes whose 1/d is infinite
+    uint32_t bp = RTBVH_MISS;
+    uint32_t oct = 0;           // bits 0..2 direction signs, 3..5 axes whose 1/d is infinite
     uint32_t ngx = 0, ngy = 0;  // node group in hand
     uint32_t tgx = 0, tgy = 0;  // triangle group in hand
     bool active = false;
@@ -772,15 +786,17 @@ cw8_kernel(TraceParams p) {
                 sched.next();
             } else {
                 active = true;
-                oct = (__float_as_uint(ix) >> 31) | ((__float_as_uint(iy) >> 31) << 1) |
-                      ((__float_as_uint(iz) >> 31) << 2);
-                flat = (fabsf(ix) > 3.4028235e38f ? 1u : 0u) |
-                       (fabsf(iy) > 3.4028235e38f ? 2u : 0u) |
-                       (fabsf(iz) > 3.4028235e38f ? 4u : 0u);
-                oct &= ~flat;  // a flat axis reads its low planes as near ones
+                const uint32_t signs = (__float_as_uint(ix) >> 31) |
+                                       ((__float_as_uint(iy) >> 31) << 1) |
+                                       ((__float_as_uint(iz) >> 31) << 2);
+                const uint32_t flat = (fabsf(ix) > 3.4028235e38f ? 1u : 0u) |
+                                      (fabsf(iy) > 3.4028235e38f ? 2u : 0u) |
+                                      (fabsf(iz) > 3.4028235e38f ? 4u : 0u);
+                // a flat axis reads its low planes as near ones
+                oct = (signs & ~flat) | (flat << 3);
                 // the root as a node group of one interior slot (slot 0), bound 0
                 ngx = BATCH ? scene_of_ray(p.ray_offsets, p.scene_count, uint32_t(rid)) : 0u;
-                ngy = (1u << (24 + oct)) | (1u << 16);
+                ngy = (1u << (24 + (oct & 7u))) | (1u << 16);
                 tgy = 0;
                 sp = 0;
             }
@@ -826,7 +842,7 @@ cw8_kernel(TraceParams p) {
         if (ngy >= kNodeGroup) {
             const uint32_t key = uint32_t(__builtin_ctz(ngy >> 24));
             ngy &= ~(1u << (24 + key));
-            const uint32_t slot = key ^ oct;
+            const uint32_t slot = key ^ (oct & 7u);
             const uint32_t idx = ngx + uint32_t(__popc((ngy >> 16) & ((1u << slot) - 1u)));
             if (ngy >= kNodeGroup) overflow = !push(ngx, ngy);
             ngy = 0;
@@ -842,23 +858,32 @@ cw8_kernel(TraceParams p) {
                 float bx = (__uint_as_float(h0.x) - ox) * ix;
                 float by = (__uint_as_float(h0.y) - oy) * iy;
                 float bz = (__uint_as_float(h0.z) - oz) * iz;
-                if (flat != 0u) {
+                if (oct >= 8u) {
                     // along a zero direction component: plane distance H (q - u), u the
                     // ray's origin in grid steps, so that a plane a step or more beyond
                     // the origin is out of reach and one behind it is -H or lower
-                    if (flat & 1u) {
+                    if (oct & 8u) {
                         sx = kFlatScale;
                         bx = flat_offset((ox - __uint_as_float(h0.x)) / stx);
                     }
-                    if (flat & 2u) {
+                    if (oct & 16u) {
                         sy = kFlatScale;
                         by = flat_offset((oy - __uint_as_float(h0.y)) / sty);
                     }
-                    if (flat & 4u) {
+                    if (oct & 32u) {
                         sz = kFlatScale;
                         bz = flat_offset((oz - __uint_as_float(h0.z)) / stz);
                     }
                 }
+                // outward slack: near planes use b - e, far planes b + e, with
+                // e = 2^-20 (|b| + 255 |s|), twice the rounding of s, b and the
+                // plane fma
+                const float ex = fmaf(fabsf(sx), 255.0f, fabsf(bx)) * kBoxEps;
+                const float ey = fmaf(fabsf(sy), 255.0f, fabsf(by)) * kBoxEps;
+                const float ez = fmaf(fabsf(sz), 255.0f, fabsf(bz)) * kBoxEps;
+                const float bxn = bx - ex, bxf = bx + ex;
+                const float byn = by - ey, byf = by + ey;
+                const float bzn = bz - ez, bzf = bz + ez;
                 // near and far planes: low planes first along a positive direction
                 const bool nx = (oct & 1u) != 0u, ny = (oct & 2u) != 0u, nz = (oct & 4u) != 0u;
                 const uint32_t nx0 = nx ? qb.z : qa.x, nx1 = nx ? qb.w : qa.y;
@@ -872,20 +897,16 @@ cw8_kernel(TraceParams p) {
                 float gmin = 3.0e38f;  // nearest entry among the interior hits
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
-                    const float tnx = fmaf(qbyte(i < 4 ? nx0 : nx1, i), sx, bx);
-                    const float tfx = fmaf(qbyte(i < 4 ? fx0 : fx1, i), sx, bx);
-                    const float tny = fmaf(qbyte(i < 4 ? ny0 : ny1, i), sy, by);
-                    const float tfy = fmaf(qbyte(i < 4 ? fy0 : fy1, i), sy, by);
-                    const float tnz = fmaf(qbyte(i < 4 ? nz0 : nz1, i), sz, bz);
-                    const float tfz = fmaf(qbyte(i < 4 ? fz0 : fz1, i), sz, bz);
-                    const float tn = fmaxf(fmaxf(tnx, tny), tnz);
-                    const float tf = fminf(fminf(tfx, tfy), tfz);
-                    const float tnc = fmaf(fabsf(tn), -kBoxEps, tn);
-                    const float tfc = fmaf(fabsf(tf), kBoxEps, tf);
-                    const float en = fmaxf(tnc, tmin);
-                    const bool hit = en <= fminf(tfc, bt);
+                    const float tnx = fmaf(qbyte(i < 4 ? nx0 : nx1, i), sx, bxn);
+                    const float tfx = fmaf(qbyte(i < 4 ? fx0 : fx1, i), sx, bxf);
+                    const float tny = fmaf(qbyte(i < 4 ? ny0 : ny1, i), sy, byn);
+                    const float tfy = fmaf(qbyte(i < 4 ? fy0 : fy1, i), sy, byf);
+                    const float tnz = fmaf(qbyte(i < 4 ? nz0 : nz1, i), sz, bzn);
+                    const float tfz = fmaf(qbyte(i < 4 ? fz0 : fz1, i), sz, bzf);
+                    const float en = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
+                    const bool hit = en <= fminf(fminf(fminf(tfx, tfy), tfz), bt);
                     hm |= hit ? (1u << i) : 0u;
-                    gmin = (hit && ((imask >> i) & 1u)) ? fminf(gmin, en) : gmin;
+                    gmin = fminf(gmin, (hit && ((imask >> i) & 1u)) ? en : 3.0e38f);
                 }
                 const uint32_t ih = hm & imask, th = hm & lmask;
                 if (ih) {
@@ -1087,9 +1108,9 @@ __global__ void prepare_tris_kernel(const uint32_t* indices, uint32_t index_coun
         e1 = make_float4(v[3] - v[0], v[4] - v[1], v[5] - v[2], 0.f);
         e2 = make_float4(v[6] - v[0], v[7] - v[1], v[8] - v[2], 0.f);
     }
-    tris[size_t(pos) * 3] = a;
-    tris[size_t(pos) * 3 + 1] = e1;
-    tris[size_t(pos) * 3 + 2] = e2;
+    tris[size_t(pos) * kTriRecord] = a;
+    tris[size_t(pos) * kTriRecord + 1] = e1;
+    tris[size_t(pos) * kTriRecord + 2] = e2;
 }
 }  // namespace
 

@@ -7,7 +7,7 @@
 // opens the child with the largest surface area, until it has four slots or
 // nothing is left to open.  A subtree of at most `max_leaf` primitives is not
 // opened: it becomes one leaf slot with its own (tight) bounds, and its
-// triangles are stored contiguously, in leaf order, as three float4 each
+// triangles are stored contiguously, in leaf order, as one 64-byte record each
 // (v0 with the primitive id, e1, e2).  The closest hit under the lowest
 // (t, id) rule does not depend on the tree, so tracing this tree gives the
 // same hit records as tracing the reference-layout MBVH.
@@ -114,9 +114,9 @@ __device__ __forceinline__ void write_tri(float4* tris, uint32_t pos, const floa
         e1 = make_float4(v[3] - v[0], v[4] - v[1], v[5] - v[2], 0.f);
         e2 = make_float4(v[6] - v[0], v[7] - v[1], v[8] - v[2], 0.f);
     }
-    tris[size_t(pos) * 3] = a;
-    tris[size_t(pos) * 3 + 1] = e1;
-    tris[size_t(pos) * 3 + 2] = e2;
+    tris[size_t(pos) * kTriRecord] = a;
+    tris[size_t(pos) * kTriRecord + 1] = e1;
+    tris[size_t(pos) * kTriRecord + 2] = e2;
 }
 
 struct LevelArgs {
@@ -541,31 +541,19 @@ __device__ void make_node8(const LevelArgs& a, uint32_t b, uint32_t w, QItem* qo
 
 // ---- SAH-optimal distribution of binary subtrees over compressed slots ---
 //
-// cost(n, j): expected cost (surface area times visit cost) of binary node
-// n's subtree when it may occupy up to j slots of its parent's node:
-//   leaf of c primitives: j >= c: the c triangles, each in its own slot
-//   (sum of C_tri * area); j < c (c > 1): one slot holding a node of its own,
-//   C_node * area + the triangles;
-//   interior: min(one slot holding a node of its own: C_node * area +
-//   dist(n, 8); dist(n, j)), dist(n, j) = min over k of cost(left, k) +
+// cost(n, j): expected cost (surface area of the compressed nodes a ray may
+// enter) of binary node n's subtree when it may occupy up to j slots of its
+// parent's node:
+//   leaf of c primitives: j >= c: 0, the c triangles each take a slot of the
+//   parent; j < c (c > 1): one slot holding a node of its own, its area;
+//   interior: min(one slot holding a node of its own: area + dist(n, 8);
+//   dist(n, j)), dist(n, j) = min over k of cost(left, k) +
 //   cost(right, j - k).
-// Computed level by level from the deepest binary level up, so that every
-// child is final before its parent.  The costs are relative; C_tri / C_node
-// is the measured cost of a triangle test against a node visit.
-constexpr float kDpNode = 1.0f;
-constexpr float kDpTri = 0.4f;
+// Every triangle ends in a slot of its own, entered through its own box, so
+// the triangle tests cost the same for every distribution and only the nodes
+// count.  Computed level by level from the deepest binary level up, so that
+// every child is final before its parent.
 constexpr uint32_t kMaxBinLevels = 8192;
-
-__device__ __forceinline__ float prim_area(const LevelArgs& a, uint32_t pos) {
-    if (pos >= a.index_count) return 0.0f;
-    const uint32_t prim = a.indices[pos];
-    if (prim >= a.prim_count) return 0.0f;
-    const float* v = a.verts + size_t(prim) * 9;
-    float d[3];
-    for (int k = 0; k < 3; ++k)
-        d[k] = fmaxf(fmaxf(v[k], v[3 + k]), v[6 + k]) - fminf(fminf(v[k], v[3 + k]), v[6 + k]);
-    return (d[0] + d[1]) * d[2] + d[0] * d[1];
-}
 
 // The binary tree breadth-first: level 0 holds the roots; level L is
 // order[off[L] .. off[L] + cnt[L]).  Cleared before each build.
@@ -658,12 +646,9 @@ __global__ void __launch_bounds__(kBlock)
         if (nd.extra1 >= 0) {
             const uint32_t np = uint32_t(nd.extra1);
             if (np > uint32_t(kWide)) atomicOr(&a.counters[2], 1u);
-            float tri = 0.0f;
-            for (uint32_t q = 0; q < np && q < uint32_t(kWide); ++q)
-                tri += kDpTri * prim_area(a, uint32_t(nd.extra2) + q);
-            const float one = np <= 1u ? tri : kDpNode * half_area(nd) + tri;
+            const float one = np <= 1u ? 0.0f : half_area(nd);
             for (uint32_t j = 1; j <= 8u; ++j) {
-                c[j] = j >= np ? tri : one;
+                c[j] = j >= np ? 0.0f : one;
                 if (j >= np && np > 1u) d |= kDissolve << (4u * (j - 1u));
             }
         } else {
@@ -692,7 +677,7 @@ __global__ void __launch_bounds__(kBlock)
                 dist[j] = best;
                 dk[j] = bk;
             }
-            c[1] = kDpNode * half_area(nd) + dist[8];
+            c[1] = half_area(nd) + dist[8];
             for (uint32_t j = 2; j <= 8u; ++j) {
                 const bool spread = j == 8u || dist[j] < c[1];
                 c[j] = spread ? dist[j] : c[1];
@@ -900,7 +885,7 @@ ResultCode trace_tree_plan_create(const DeviceBvh& tree, uint32_t max_leaf, uint
          (e = hipMalloc(&res.nodes, size_t(t->wcap) * sizeof(DevMbvhNode))) != hipSuccess) ||
         (width == 8 &&
          (e = hipMalloc(&res.cnodes, size_t(t->wcap) * sizeof(DevCw8Node))) != hipSuccess) ||
-        (e = hipMalloc(&res.tris, size_t(t->n) * 3 * sizeof(float4))) != hipSuccess) {
+        (e = hipMalloc(&res.tris, size_t(t->n) * kTriRecord * sizeof(float4))) != hipSuccess) {
         if (res.nodes) (void)hipFree(res.nodes);
         if (res.cnodes) (void)hipFree(res.cnodes);
         trace_tree_plan_destroy(t);

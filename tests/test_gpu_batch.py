@@ -128,6 +128,30 @@ def test_batch_trace_equals_per_scene_trace():
     m.status()
 
 
+def test_batch_autotune_keeps_results():
+    """rtbvh_gpu_autotune_batch times the launch shapes on the batch's own
+    rays; the shape it keeps returns the same hits as the default shape."""
+    print("[batch_autotune]", flush=True)
+    vl = _scene_list()
+    batch, verts, offs = _build_batch(vl)
+    batch.attach_triangles(_dev(verts))
+    rays_l = [ray_set(v, 2000, seed=40 + s, kind="box") for s, v in enumerate(vl)]
+    rays = _dev(np.concatenate(rays_l))
+    roff = rt.SceneBatch.ray_offsets([len(r) for r in rays_l], "cuda")
+    rt.set_trace_config()
+    try:
+        want = batch.intersect(rays, roff).cpu().numpy()
+        best = rt.autotune(batch, rays, roff)
+        assert best == rt.get_trace_config()
+        assert best["block"] in (64, 128) and best["kernel"] == 0
+        got = batch.intersect(rays, roff).cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+        with pytest.raises(ValueError):
+            rt.autotune(batch, rays)
+    finally:
+        rt.set_trace_config()
+
+
 def test_batch_errors():
     print("[batch_errors]", flush=True)
     v = scenes.random_soup(50, 0.1, seed=3)
