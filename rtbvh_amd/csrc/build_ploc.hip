@@ -20,9 +20,9 @@
 //             locb.rs:170-176) and the next iteration's bookkeeping;
 //   merge     parents at the upper index's compacted slot, children written
 //             to their final positions end-2M.. (locb.rs:178-215).
-// Once N <= kTailMax the remaining iterations run in one workgroup with the
-// clusters resident in LDS (the long tail of small iterations costs one
-// launch).
+// Once N <= max(kTailEnter, scenes) the remaining iterations run in one
+// workgroup with the clusters resident in LDS (the long tail of small
+// iterations costs one launch).
 #include <vector>
 
 #include "internal.hpp"
@@ -40,6 +40,15 @@ constexpr int kTile = 1024;            // clusters per tile in the multi-block p
 constexpr int kBlock = 256;
 constexpr int kTailThreads = 1024;
 constexpr int kTailMax = 4096;         // clusters held in LDS by the tail kernel
+// The tail takes over at this many clusters (or at one per scene, when a
+// batch has more scenes): measured on the 1M scene, 2048 builds 0.3 ms
+// faster than 4096 (the single workgroup's share shrinks) and 1024 0.9 ms
+// slower (the multi-block iterations' launches cost more than they save).
+constexpr uint32_t kTailEnter = 2048;
+
+__host__ __device__ __forceinline__ uint32_t tail_threshold(uint32_t roots) {
+    return roots > kTailEnter ? roots : kTailEnter;
+}
 constexpr int kSortBlock = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortBlock * kSortItems;
@@ -416,7 +425,8 @@ struct PlocState {
     uint32_t end;
     uint32_t merged;    // pairs merged by this iteration (M)
     uint32_t buf;       // cluster buffer holding the active clusters
-    uint32_t pad[3];
+    uint32_t tail_at;   // the tail kernel takes over at this many clusters
+    uint32_t pad[2];
 };
 
 __device__ __forceinline__ void nn_window(uint32_t i, uint32_t n, uint32_t& sb, uint32_t& se) {
@@ -440,7 +450,7 @@ __global__ void __launch_bounds__(kBlock)
     __shared__ uint32_t s_nb[kTile + 2 * kRadius];
     __shared__ uint32_t s_total;
     const PlocState stt = *state;
-    if (stt.n_active <= uint32_t(kTailMax)) return;
+    if (stt.n_active <= stt.tail_at) return;
     const uint32_t n = stt.n_active;
     const DevAabb* cl = stt.buf ? cl1 : cl0;
     const uint32_t* cs = stt.buf ? cs1 : cs0;
@@ -505,7 +515,7 @@ __global__ void __launch_bounds__(kBlock)
 __global__ void __launch_bounds__(1024) tile_scan_kernel(PlocState* states, uint32_t* tile_count) {
     __shared__ uint32_t sums[1024];
     const PlocState stt = states[0];
-    if (stt.n_active <= uint32_t(kTailMax)) {
+    if (stt.n_active <= stt.tail_at) {
         if (threadIdx.x == 0) {
             PlocState nx = stt;
             nx.merged = 0;
@@ -543,7 +553,8 @@ __global__ void __launch_bounds__(1024) tile_scan_kernel(PlocState* states, uint
         nx.end = stt.end - 2 * m;
         nx.merged = 0;
         nx.buf = stt.buf ^ 1u;
-        nx.pad[0] = nx.pad[1] = nx.pad[2] = 0;
+        nx.tail_at = stt.tail_at;
+        nx.pad[0] = nx.pad[1] = 0;
         states[1] = nx;
     }
 }
@@ -558,7 +569,7 @@ __global__ void __launch_bounds__(kBlock)
     __shared__ uint32_t s_m[kRadius + kTile];
     __shared__ uint32_t s_scan[kBlock];
     const PlocState stt = *state;
-    if (stt.n_active <= uint32_t(kTailMax)) return;
+    if (stt.n_active <= stt.tail_at) return;
     const uint32_t n = stt.n_active;
     const DevAabb* cl = stt.buf ? cl1 : cl0;
     DevAabb* cl_out = stt.buf ? const_cast<DevAabb*>(cl0) : cl1;
@@ -633,18 +644,21 @@ __global__ void __launch_bounds__(kBlock)
 // six bound arrays plus the two node fields.  Thread t owns clusters
 // t, t + 1024, t + 2048, t + 3072, so lanes touch consecutive LDS words; the
 // mark prefix comes from wave ballots.
-constexpr int kTailPer = kTailMax / kTailThreads;
 constexpr int kTailWaves = kTailThreads / 64;
 
-template <bool SEG>
+// CAP clusters at most: kTailEnter for the common case, kTailMax for a
+// batch of more scenes (the smaller instance keeps half the registers per
+// thread and builds the 1M scene 0.3 ms faster).
+template <bool SEG, int CAP>
 struct TailShared {
-    float b[6][kTailMax];
-    int32_t e1[kTailMax];
-    int32_t e2[kTailMax];
-    uint16_t nb[kTailMax];
-    uint16_t m[kTailMax];
-    uint16_t sc[SEG ? kTailMax : 2];  // scene of each cluster (batches)
-    uint32_t wsum[kTailPer * kTailWaves];
+    static constexpr int kPer = CAP / kTailThreads;  // clusters per thread
+    float b[6][CAP];
+    int32_t e1[CAP];
+    int32_t e2[CAP];
+    uint16_t nb[CAP];
+    uint16_t m[CAP];
+    uint16_t sc[SEG ? CAP : 2];  // scene of each cluster (batches)
+    uint32_t wsum[kPer * kTailWaves];
 };
 
 template <typename TS>
@@ -675,17 +689,19 @@ __device__ __forceinline__ void tail_store(TS& sh, uint32_t i, const DevAabb& a)
 
 // Runs until n_final clusters remain (1, or one per scene of a batch), stores
 // them at nodes[0, n_final) and reports the count it reached in *result.
-template <bool SEG>
+template <bool SEG, int CAP>
 __global__ void __launch_bounds__(kTailThreads)
     tail_kernel(const PlocState* state, const DevAabb* cl0, const DevAabb* cl1,
                 const uint32_t* cs0, const uint32_t* cs1, DevAabb* nodes, uint32_t n_final,
                 PlocState* result) {
     extern __shared__ unsigned char smem_raw[];
-    TailShared<SEG>& sh = *reinterpret_cast<TailShared<SEG>*>(smem_raw);
+    using TS = TailShared<SEG, CAP>;
+    constexpr int kTailPer = TS::kPer;
+    TS& sh = *reinterpret_cast<TS*>(smem_raw);
     const PlocState stt = *state;
     uint32_t n = stt.n_active;
     uint32_t end = stt.end;
-    if (n > uint32_t(kTailMax)) {  // the iterations before did not get here
+    if (n > uint32_t(CAP)) {  // the iterations before did not get here
         if (threadIdx.x == 0) *result = stt;
         return;
     }
@@ -827,7 +843,8 @@ __global__ void init_state_kernel(PlocState* st, uint32_t n, uint32_t roots) {
     s.end = 2 * n - roots;
     s.merged = 0;
     s.buf = 0;
-    s.pad[0] = s.pad[1] = s.pad[2] = 0;
+    s.tail_at = tail_threshold(roots);
+    s.pad[0] = s.pad[1] = 0;
     st[0] = s;
 }
 
@@ -1018,9 +1035,13 @@ hipError_t set_tail_lds() {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
     if (done[dev]) return hipSuccess;
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(tail_kernel<SEG>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             int(sizeof(TailShared<SEG>)));
+    hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(tail_kernel<SEG, int(kTailEnter)>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(TailShared<SEG, int(kTailEnter)>)));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(tail_kernel<SEG, kTailMax>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                int(sizeof(TailShared<SEG, kTailMax>)));
     if (e == hipSuccess) done[dev] = true;
     return e;
 }
@@ -1030,9 +1051,14 @@ hipError_t set_tail_lds() {
 template <bool SEG>
 hipError_t enqueue_tail(const PlocPlanImpl* p, uint32_t iter, hipStream_t stream) {
     const uint32_t roots = SEG ? p->scenes : 1u;
-    hipLaunchKernelGGL(tail_kernel<SEG>, dim3(1), dim3(kTailThreads), sizeof(TailShared<SEG>),
-                       stream, p->st + iter, p->cl0, p->cl1, p->cs0, p->cs1, p->nodes, roots,
-                       p->st + iter + 1);
+    if (tail_threshold(roots) <= kTailEnter)
+        hipLaunchKernelGGL((tail_kernel<SEG, int(kTailEnter)>), dim3(1), dim3(kTailThreads),
+                           sizeof(TailShared<SEG, int(kTailEnter)>), stream, p->st + iter, p->cl0,
+                           p->cl1, p->cs0, p->cs1, p->nodes, roots, p->st + iter + 1);
+    else
+        hipLaunchKernelGGL((tail_kernel<SEG, kTailMax>), dim3(1), dim3(kTailThreads),
+                           sizeof(TailShared<SEG, kTailMax>), stream, p->st + iter, p->cl0,
+                           p->cl1, p->cs0, p->cs1, p->nodes, roots, p->st + iter + 1);
     hipLaunchKernelGGL(check_kernel, dim3(1), dim3(1), 0, stream, p->st + iter + 1, roots,
                        p->fail);
     return hipGetLastError();
@@ -1055,7 +1081,8 @@ ResultCode run_eager(PlocPlanImpl* p, const RTAabb* aabbs, const float* centers,
     uint32_t known = p->n;
     uint32_t iter = 0;
     PlocState host_state;
-    while (known > uint32_t(kTailMax)) {
+    const uint32_t tail_at = tail_threshold(SEG ? p->scenes : 1u);
+    while (known > tail_at) {
         const uint32_t tiles = (known + kTile - 1) / kTile;
         for (int bt = 0; bt < kBatch; ++bt) {
             if (iter + 1 >= kMaxIterations) {
@@ -1095,7 +1122,7 @@ ResultCode run_eager(PlocPlanImpl* p, const RTAabb* aabbs, const float* centers,
             if ((e = hipMemcpy(states.data(), p->st, sizeof(PlocState) * (iter + 1),
                                hipMemcpyDeviceToHost)) != hipSuccess)
                 return hip_fail(e, "state read");
-            while (need < iter && states[need].n_active > uint32_t(kTailMax)) ++need;
+            while (need < iter && states[need].n_active > tail_at) ++need;
         }
         *iterations = need;
     }

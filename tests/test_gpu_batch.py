@@ -128,6 +128,38 @@ def test_batch_trace_equals_per_scene_trace():
     m.status()
 
 
+@pytest.mark.parametrize("n_scenes", [2500, 4096])
+def test_batch_many_small_scenes(n_scenes):
+    """More scenes than the clustering's usual tail threshold (2048), up to
+    RTBVH_MAX_BATCH_SCENES: the tail then starts at one cluster per scene
+    and holds up to 4096 clusters.  Sampled scenes' subtrees equal the
+    oracle's LOCB and their batched hits equal brute force."""
+    print(f"[batch_many] scenes={n_scenes}", flush=True)
+    rng = np.random.default_rng(n_scenes)
+    sizes = rng.integers(3, 13, n_scenes)
+    vl = [scenes.random_soup(int(c), 0.2, seed=1000 + i) for i, c in enumerate(sizes)]
+    batch, verts, offs = _build_batch(vl)
+    assert batch.scene_count == n_scenes
+    nodes = batch.bvh_nodes().cpu().numpy().view(ref.AABB).reshape(-1)
+    idx = batch.indices().cpu().numpy().view(np.uint32)
+    assert sorted(idx.tolist()) == list(range(len(verts)))
+    sample = list(range(0, n_scenes, 97)) + [n_scenes - 1]
+    for s in sample:
+        aabbs, centers = prep(vl[s])
+        want_n, want_i = ref.build_locb(aabbs, centers)
+        ok, info = _subtree_equal(nodes, idx, s, int(offs[s]), want_n, want_i, 0)
+        assert ok, f"scene {s}: {info}"
+    batch.attach_triangles(_dev(verts))
+    rays_l = [ray_set(v, 40, seed=s, kind="box") for s, v in enumerate(vl)]
+    roff = rt.SceneBatch.ray_offsets([len(r) for r in rays_l], "cuda")
+    got = batch.intersect(_dev(np.concatenate(rays_l)), roff).cpu().numpy()
+    starts = np.concatenate([[0], np.cumsum([len(r) for r in rays_l])])
+    for s in sample:
+        g = got[starts[s]:starts[s + 1]].view(ref.HIT).reshape(-1)
+        brute = ref.trace_brute(vl[s], as_rays(rays_l[s]), threads=1)
+        assert hits_equal(g, brute).all(), f"scene {s}"
+
+
 def test_batch_autotune_keeps_results():
     """rtbvh_gpu_autotune_batch times the launch shapes on the batch's own
     rays; the shape it keeps returns the same hits as the default shape."""
