@@ -23,6 +23,8 @@
 // Once N <= max(kTailEnter, scenes) the remaining iterations run in one
 // workgroup with the clusters resident in LDS (the long tail of small
 // iterations costs one launch).
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "internal.hpp"
@@ -831,7 +833,7 @@ __global__ void __launch_bounds__(kTailThreads)
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 constexpr uint32_t kMaxIterations = 1u << 14;
-constexpr int kBatch = 6;  // iterations enqueued between host checks
+constexpr uint32_t kBatch = 6;  // iterations before the first host check
 
 // The first iteration's state (a kernel, so that the whole build can be
 // recorded into a graph).
@@ -1082,9 +1084,10 @@ ResultCode run_eager(PlocPlanImpl* p, const RTAabb* aabbs, const float* centers,
     uint32_t iter = 0;
     PlocState host_state;
     const uint32_t tail_at = tail_threshold(SEG ? p->scenes : 1u);
+    uint32_t batch = kBatch;
     while (known > tail_at) {
         const uint32_t tiles = (known + kTile - 1) / kTile;
-        for (int bt = 0; bt < kBatch; ++bt) {
+        for (uint32_t bt = 0; bt < batch; ++bt) {
             if (iter + 1 >= kMaxIterations) {
                 set_error("clustering needed more than %u iterations", kMaxIterations);
                 return Error;
@@ -1101,7 +1104,16 @@ ResultCode run_eager(PlocPlanImpl* p, const RTAabb* aabbs, const float* centers,
             set_error("clustering made no progress (non-finite bounds?)");
             return Error;
         }
+        // the next batch: as many iterations as the shrink rate of this one
+        // predicts to reach the tail, plus one (surplus iterations return at
+        // once), so that a build needs few host round trips
+        const double rate = std::pow(double(host_state.n_active) / double(known), 1.0 / batch);
         known = host_state.n_active;
+        batch = kBatch;
+        if (known > tail_at && rate > 0.0 && rate < 0.999) {
+            const double need = std::log(double(tail_at) / double(known)) / std::log(rate);
+            batch = uint32_t(std::min(64.0, std::max(1.0, std::ceil(need)))) + 1u;
+        }
     }
     if ((e = enqueue_tail<SEG>(p, iter, stream)) != hipSuccess) return hip_fail(e, "tail launch");
     uint32_t fail = 1;
