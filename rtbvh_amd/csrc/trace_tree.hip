@@ -340,13 +340,15 @@ __device__ bool grid_axis(float lo, float hi, float& p, uint32_t& ebyte, double&
     return true;
 }
 
-__device__ __forceinline__ uint32_t quant_lo(float v, float p, double step) {
-    const double q = floor((double(v) - double(p)) / step) - 1.0;
+// inv_step: 1 / step, exact (a power of two), so the product equals the
+// quotient
+__device__ __forceinline__ uint32_t quant_lo(float v, float p, double inv_step) {
+    const double q = floor((double(v) - double(p)) * inv_step) - 1.0;
     return q <= 0.0 ? 0u : (q >= 255.0 ? 255u : uint32_t(q));
 }
 
-__device__ __forceinline__ uint32_t quant_hi(float v, float p, double step) {
-    const double q = ceil((double(v) - double(p)) / step) + 1.0;
+__device__ __forceinline__ uint32_t quant_hi(float v, float p, double inv_step) {
+    const double q = ceil((double(v) - double(p)) * inv_step) + 1.0;
     return q <= 0.0 ? 0u : (q >= 255.0 ? 255u : uint32_t(q));
 }
 
@@ -477,6 +479,8 @@ __device__ void make_node8(const LevelArgs& a, uint32_t b, uint32_t w, QItem* qo
             atomicOr(&a.counters[2], 2u);
             return;
         }
+    double inv_step[3];
+    for (int k = 0; k < 3; ++k) inv_step[k] = 1.0 / step[k];
     uint32_t n_inner = 0, n_tris = 0;
     for (int j = 0; j < ns; ++j) {
         if (c[j].kind == 2)
@@ -507,8 +511,8 @@ __device__ void make_node8(const LevelArgs& a, uint32_t b, uint32_t w, QItem* qo
             continue;
         }
         for (int ax = 0; ax < 3; ++ax) {
-            qlo[ax][word] |= quant_lo(c[k].lo[ax], p[ax], step[ax]) << sh;
-            qhi[ax][word] |= quant_hi(c[k].hi[ax], p[ax], step[ax]) << sh;
+            qlo[ax][word] |= quant_lo(c[k].lo[ax], p[ax], inv_step[ax]) << sh;
+            qhi[ax][word] |= quant_hi(c[k].hi[ax], p[ax], inv_step[ax]) << sh;
         }
         if (c[k].kind == 2) {
             lmask |= 1u << s;
