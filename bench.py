@@ -419,7 +419,10 @@ def main():
     build_ms = []
     bvh = None
     reps = 2 if args.quick else 6
+    # (each timed region starts after the previous result is released: a
+    # release frees device memory, which waits for the device)
     for i in range(reps):
+        bvh = None
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         bvh = rt.Builder(c_d, aabbs=a_d).construct_locally_ordered_clustered()
@@ -429,6 +432,7 @@ def main():
     collapse_ms = []
     mbvh = None
     for i in range(2 if args.quick else 4):
+        mbvh = None
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         mbvh = rt.Mbvh.from_bvh(bvh)
@@ -439,10 +443,11 @@ def main():
     for i in range(0 if args.quick else 3):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        rt.Builder(c_d, aabbs=a_d).construct_binned_sah()
+        sah = rt.Builder(c_d, aabbs=a_d).construct_binned_sah()
         torch.cuda.synchronize()
         if i > 0:
             binned_ms.append((time.perf_counter() - t0) * 1e3)
+        del sah
     if len(binned_ms) > 1:
         binned_ms = binned_ms[1:]
     log(f"PLOC {statistics.median(build_ms):.2f} ms, collapse {statistics.median(collapse_ms):.2f} ms,"
@@ -479,6 +484,7 @@ def main():
     tree_ms = []
     ttree = None
     for i in range(3):
+        ttree = None
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ttree = bvh.trace_tree(v_d, max_leaf=args.max_leaf if args.width == 4 else 1,

@@ -763,18 +763,12 @@ struct TraceTreePlan {
     uint32_t* blev = nullptr;
     float* dpc = nullptr;
     uint32_t* dpd = nullptr;
+    void* ws = nullptr;  // one allocation holding all of the above
 };
 
 void trace_tree_plan_destroy(TraceTreePlan* t) {
     if (!t) return;
-    if (t->cnt) (void)hipFree(t->cnt);
-    if (t->q[0]) (void)hipFree(t->q[0]);
-    if (t->q[1]) (void)hipFree(t->q[1]);
-    if (t->words) (void)hipFree(t->words);
-    if (t->bord) (void)hipFree(t->bord);
-    if (t->blev) (void)hipFree(t->blev);
-    if (t->dpc) (void)hipFree(t->dpc);
-    if (t->dpd) (void)hipFree(t->dpd);
+    if (t->ws) (void)hipFree(t->ws);
     delete t;
 }
 
@@ -871,18 +865,41 @@ ResultCode trace_tree_plan_create(const DeviceBvh& tree, uint32_t max_leaf, uint
     t->max_leaf = max_leaf;
     t->width = width;
     t->wcap = tree.prim_count + tree.root_count;
-    hipError_t e;
-    if ((e = hipMalloc(&t->cnt, size_t(t->count) * 4)) != hipSuccess ||
-        (e = hipMalloc(&t->q[0], size_t(t->wcap) * sizeof(QItem))) != hipSuccess ||
-        (e = hipMalloc(&t->q[1], size_t(t->wcap) * sizeof(QItem))) != hipSuccess ||
-        (e = hipMalloc(&t->words, 8 * 4)) != hipSuccess ||
-        (width == 8 &&
-         ((e = hipMalloc(&t->bord, size_t(t->count) * 4)) != hipSuccess ||
-          (e = hipMalloc(&t->blev, size_t(kMaxBinLevels + 2) * 2 * 4)) != hipSuccess ||
-          (e = hipMalloc(&t->dpc, size_t(t->count) * 8 * sizeof(float))) != hipSuccess ||
-          (e = hipMalloc(&t->dpd, size_t(t->count) * 4)) != hipSuccess))) {
+    // one workspace allocation (allocations and frees cost far more than
+    // the kernels of a small tree), carved at 256-byte offsets
+    size_t bytes = 0;
+    const auto carve = [&bytes](size_t n) {
+        const size_t at = bytes;
+        bytes += (n + 255) & ~size_t(255);
+        return at;
+    };
+    const size_t o_cnt = carve(size_t(t->count) * 4);
+    const size_t o_q0 = carve(size_t(t->wcap) * sizeof(QItem));
+    const size_t o_q1 = carve(size_t(t->wcap) * sizeof(QItem));
+    const size_t o_words = carve(8 * 4);
+    size_t o_bord = 0, o_blev = 0, o_dpc = 0, o_dpd = 0;
+    if (width == 8) {
+        o_bord = carve(size_t(t->count) * 4);
+        o_blev = carve(size_t(kMaxBinLevels + 2) * 2 * 4);
+        o_dpc = carve(size_t(t->count) * 8 * sizeof(float));
+        o_dpd = carve(size_t(t->count) * 4);
+    }
+    hipError_t e = hipMalloc(&t->ws, bytes);
+    if (e != hipSuccess) {
+        t->ws = nullptr;
         trace_tree_plan_destroy(t);
         return hip_fail(e, "trace tree workspace");
+    }
+    char* base = static_cast<char*>(t->ws);
+    t->cnt = reinterpret_cast<uint32_t*>(base + o_cnt);
+    t->q[0] = reinterpret_cast<QItem*>(base + o_q0);
+    t->q[1] = reinterpret_cast<QItem*>(base + o_q1);
+    t->words = reinterpret_cast<uint32_t*>(base + o_words);
+    if (width == 8) {
+        t->bord = reinterpret_cast<uint32_t*>(base + o_bord);
+        t->blev = reinterpret_cast<uint32_t*>(base + o_blev);
+        t->dpc = reinterpret_cast<float*>(base + o_dpc);
+        t->dpd = reinterpret_cast<uint32_t*>(base + o_dpd);
     }
     DeviceMbvh res;
     if ((width == 4 &&
