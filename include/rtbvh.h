@@ -203,6 +203,20 @@ ResultCode rtbvh_gpu_create_bvh(const RTAabb *aabbs, size_t prim_count, const fl
                                 size_t center_stride, size_t prims_per_leaf, BvhType bvh_type,
                                 void *stream, RTBvh *result);
 
+/* GPU spatial-split SAH build, the capability of create_spatial_Bvh
+ * (rtbvh_ffi/src/lib.rs:337-389, SpatialSahBuilder,
+ * src/builders/spatial_sah.rs:247-1034): nodes are split by reference
+ * centroids or by planes that cut straddling triangles into one reference
+ * per side, bounded by the clipped triangle; up to 0.75 * prim_count extra
+ * references.  Arguments as rtbvh_gpu_create_bvh plus `vertices` (device, 9
+ * floats per primitive, the triangles the references clip).  result->index_count
+ * counts the references (at least prim_count; a primitive may sit in several
+ * leaves).  Synchronises the stream. */
+ResultCode rtbvh_gpu_create_spatial_bvh(const RTAabb *aabbs, size_t prim_count,
+                                        const float *centers, size_t center_stride,
+                                        const float *vertices, size_t prims_per_leaf,
+                                        void *stream, RTBvh *result);
+
 /* GPU MBVH collapse (Mbvh::construct, src/bvh.rs:381-404) of a tree made by
  * rtbvh_gpu_create_bvh. */
 ResultCode rtbvh_gpu_create_mbvh(RTBvh bvh, void *stream, RTMbvh *result);

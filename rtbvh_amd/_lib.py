@@ -89,6 +89,8 @@ SIGNATURES = {
     "free_bvh": (None, [RTBvh]),
     "free_mbvh": (None, [RTMbvh]),
     "rtbvh_gpu_create_bvh": (_rc, [_vp, _sz, _vp, _sz, _sz, _u32, _vp, ctypes.POINTER(RTBvh)]),
+    "rtbvh_gpu_create_spatial_bvh": (_rc, [_vp, _sz, _vp, _sz, _vp, _sz, _vp,
+                                          ctypes.POINTER(RTBvh)]),
     "rtbvh_gpu_create_mbvh": (_rc, [RTBvh, _vp, ctypes.POINTER(RTMbvh)]),
     "rtbvh_gpu_refit": (_rc, [_vp, RTBvh, _vp]),
     "rtbvh_gpu_intersect": (_rc, [RTBvh, _vp, _sz, _vp, _sz, _vp, _u32, _vp]),

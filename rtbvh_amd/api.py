@@ -208,12 +208,18 @@ class Bvh(_Tree):
         flags = L.TRACE_REFERENCE_ORDER if reference_order else 0
         return self._trace(L.lib().rtbvh_gpu_occluded, vertices, rays, out, flags)
 
+    @property
+    def primitive_count(self) -> int:
+        """Primitives the tree was built over (index_count counts references,
+        more than the primitives after spatial splits)."""
+        return int(getattr(self, "_prims", self.index_count))
+
     def refit(self, aabbs: torch.Tensor):
         """Bvh::refit (src/bvh.rs:176-205) on the GPU, in place."""
         if not self._owned:
             raise ValueError("refit needs a tree built by this library")
         _require_device(aabbs, "aabbs")
-        if aabbs.shape != (self.index_count, AABB_FLOATS):
+        if aabbs.shape != (self.primitive_count, AABB_FLOATS):
             raise ValueError("aabbs must be (prim_count, 8)")
         L.check(L.lib().rtbvh_gpu_refit(_ptr(aabbs), self._h, _stream(aabbs)), "rtbvh_gpu_refit")
 
@@ -642,6 +648,41 @@ class Builder:
     def construct_binned_sah(self) -> Bvh:
         """src/bvh.rs:87-111 — 16-bin SAH on the GPU."""
         return self._construct(L.BINNED_SAH)
+
+    def construct_spatial_sah(self, vertices: torch.Tensor) -> Bvh:
+        """Builder::construct_spatial_sah (src/bvh.rs, SpatialSahBuilder,
+        src/builders/spatial_sah.rs:247-1034) on the GPU: object and spatial
+        splits over `vertices` ((N, 9) or (N, 3, 3) float32, device), which
+        the straddling references are clipped against.  The tree's
+        index_count counts the references (a triangle may sit in several
+        leaves); primitive_count is N."""
+        n = 0 if self.primitives is None else int(self.primitives.shape[0])
+        if n == 0:
+            raise BuildError("NoPrimitives")
+        if self.aabbs is not None and int(self.aabbs.shape[0]) != n:
+            raise BuildError("InequalAabbsAndPrimitives", int(self.aabbs.shape[0]), n)
+        _require_device(self.primitives, "primitives")
+        _require_device(vertices, "vertices")
+        if self.primitives.dim() != 2 or self.primitives.shape[1] not in (3, 4):
+            raise ValueError("primitives must be (N, 3) or (N, 4) float32 centres")
+        verts = vertices.reshape(-1).contiguous()
+        if verts.numel() != 9 * n or verts.dtype != torch.float32:
+            raise ValueError("vertices must be (N, 9) float32 triangles")
+        if self.aabbs is not None:
+            _require_device(self.aabbs, "aabbs")
+            if self.aabbs.shape[1] != AABB_FLOATS:
+                raise ValueError("aabbs must be (N, 8) RTAabb records")
+        h = L.RTBvh()
+        stride = 4 * int(self.primitives.shape[1])
+        code = L.lib().rtbvh_gpu_create_spatial_bvh(
+            _ptr(self.aabbs), n, _ptr(self.primitives), stride, _ptr(verts),
+            self.primitives_per_leaf or 0, _stream(self.primitives), ctypes.byref(h))
+        if code == L.NO_PRIMITIVES:
+            raise BuildError("NoPrimitives")
+        L.check(code, "rtbvh_gpu_create_spatial_bvh")
+        bvh = Bvh(h, self.primitives.device, owned=True)
+        bvh._prims = n
+        return bvh
 
 
 def get_trace_config() -> dict:

@@ -371,6 +371,33 @@ ResultCode rtbvh_gpu_create_bvh(const RTAabb* aabbs, size_t prim_count, const fl
     return Ok;
 }
 
+ResultCode rtbvh_gpu_create_spatial_bvh(const RTAabb* aabbs, size_t prim_count,
+                                        const float* centers, size_t center_stride,
+                                        const float* vertices, size_t prims_per_leaf,
+                                        void* stream, RTBvh* result) {
+    if (centers == nullptr || vertices == nullptr || result == nullptr) {
+        set_error("centers, vertices and result must not be null");
+        return Error;
+    }
+    ResultCode rc = validate_build(prim_count, center_stride, LocallyOrderedClustered);
+    if (rc != Ok) return rc;
+    std::unique_ptr<BvhEntry> e(new BvhEntry());
+    rc = current_device(&e->device);
+    if (rc != Ok) return rc;
+    if (!is_device_pointer(centers) || !is_device_pointer(vertices) ||
+        (aabbs && !is_device_pointer(aabbs))) {
+        set_error("centers, vertices and aabbs must be device memory");
+        return InvalidArgument;
+    }
+    rc = gpu_build_spatial(aabbs, centers, center_stride / 4, vertices, prim_count, prims_per_leaf,
+                           static_cast<hipStream_t>(stream), &e->dev);
+    if (rc != Ok) return rc;
+    BvhEntry* raw = e.get();
+    const uint32_t id = register_bvh(std::move(e));
+    *result = handle_of(id, *raw, false);
+    return Ok;
+}
+
 ResultCode rtbvh_gpu_create_mbvh(RTBvh bvh, void* stream, RTMbvh* result) {
     if (result == nullptr) {
         set_error("result must not be null");

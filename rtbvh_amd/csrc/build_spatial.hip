@@ -1,0 +1,930 @@
+// GPU spatial-split SAH builder (SBVH) for gfx950, breadth-first.
+//
+// The capability of SpatialSahBuilder (src/builders/spatial_sah.rs:247-1034,
+// defaults :940-956: 16 bins, depth 64, leaf size from primitives_per_leaf,
+// traversal cost 1, alpha 1e-5, split factor 0.75): a node is split either by
+// its references' centroids (object split) or by a plane that cuts the
+// triangles straddling it into two references, one per side, each bounded by
+// the clipped triangle (spatial split).  The design is this build's own, one
+// tree level per step with every node of the level in parallel:
+//   node box      union of the node's reference boxes (padded by 1e-4, as
+//                 the reference pads a parent, :352)
+//   object split  16 centroid bins per axis over the node box, SAH sweep
+//                 (the reference sorts and sweeps every position, :318-363)
+//   spatial split only where the best object split's children overlap by
+//                 more than alpha times the root's surface (:768-775): 16
+//                 bins per axis, each reference clipped into every bin it
+//                 spans (chopped binning, :500-560; one pass instead of two),
+//                 entry and exit counts, SAH sweep over the 15 planes
+//   budget        the references may grow by split_factor * n (:1016); a
+//                 spatial split is taken when it is cheaper and all of its
+//                 duplicates still fit, deciding tasks in node order (the
+//                 reference falls back per reference when space runs out)
+//   leaf          one reference, depth 64, or no split cheaper than
+//                 A * (n - traversal cost) with at most max_leaf references;
+//                 otherwise an approximate median object split on the
+//                 longest axis, and an index median when every centroid
+//                 falls into one bin (:785-810)
+// Deliberate differences, each a reference defect: triangles are clipped
+// with a + t (b - a) (the reference's split_edge computes a * t * (b - a),
+// :91-94, so its clipped boxes can miss the triangle); exit counts go to the
+// last bin a reference spans (the reference adds both counts to the first,
+// :536-537); the straddle fallback keeps the node's own range (:636-645
+// index from 0).  A clipped box is widened on the two other axes by 2^-20
+// of the vertices' magnitude (the rounding of the crossings) and kept inside
+// the reference's box, so the references of a triangle always cover it and
+// every hit is found.
+#include <algorithm>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace rtbvh {
+
+hipError_t scan_exclusive_u32(uint32_t* data, uint32_t count, uint32_t* total, void* scratch,
+                              size_t scratch_bytes, hipStream_t stream);
+size_t scan_scratch_bytes(uint32_t count);
+
+namespace {
+
+constexpr int kBins = 16;
+constexpr uint32_t kMaxDepth = 64;
+constexpr float kSplitFactor = 0.75f;
+constexpr float kAlpha = 1e-5f;
+constexpr int kBlock = 256;
+constexpr uint32_t kMaxCand = 65536;  // spatial candidates evaluated per level
+
+// A reference: the part of triangle `prim` inside [mn, mx]; `task` is the
+// index of the current level's task that holds it.
+struct SRef {
+    float mn[3];
+    uint32_t prim;
+    float mx[3];
+    int32_t task;
+};
+static_assert(sizeof(SRef) == 32, "reference record is 32 bytes");
+
+struct STask {
+    uint32_t node, begin, end, depth;
+};
+
+// Box accumulator with order-preserving float encodings; c0/c1 are counts
+// (object bins: c0 references; spatial bins: c0 entries, c1 exits).
+struct Acc {
+    uint32_t c0;
+    uint32_t mn[3];
+    uint32_t mx[3];
+    uint32_t c1;
+};
+
+enum : uint32_t { kLeaf = 0, kObject = 1, kSpatial = 2, kIndex = 3 };
+
+struct SDec {
+    float mn[3], mx[3];   // padded node box
+    float org[3], inv[3];  // bins: b = (x - org) * inv, 16 per axis
+    float width[3];       // bin width
+    uint32_t kind, axis, k, left, right;  // split bin boundary k; child sizes
+    float cost;           // best cost so far
+    uint32_t cand;        // 1: spatial split evaluated
+    uint32_t s_axis, s_k, s_left, s_right, dup;  // best spatial split
+    float s_cost;
+};
+
+__device__ __forceinline__ uint32_t fenc(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float fdec(uint32_t e) {
+    return __uint_as_float((e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e);
+}
+
+__device__ __forceinline__ void acc_clear(Acc& a) {
+    a.c0 = a.c1 = 0;
+    for (int k = 0; k < 3; ++k) {
+        a.mn[k] = fenc(1e34f);
+        a.mx[k] = fenc(-1e34f);
+    }
+}
+
+__device__ __forceinline__ float half_area6(const float* mn, const float* mx) {
+    const float d0 = mx[0] - mn[0], d1 = mx[1] - mn[1], d2 = mx[2] - mn[2];
+    return (d0 + d1) * d2 + d0 * d1;
+}
+
+__device__ __forceinline__ uint32_t bin_of(float x, float org, float inv) {
+    const float b = (x - org) * inv;
+    if (!(b > 0.0f)) return 0u;  // also NaN
+    if (b >= float(kBins - 1)) return uint32_t(kBins - 1);
+    return uint32_t(b);
+}
+
+struct Box {
+    float mn[3], mx[3];
+};
+
+__device__ __forceinline__ void box_empty(Box& b) {
+    for (int k = 0; k < 3; ++k) {
+        b.mn[k] = 1e34f;
+        b.mx[k] = -1e34f;
+    }
+}
+
+__device__ __forceinline__ void box_grow_acc(Box& b, const Acc& a) {
+    for (int k = 0; k < 3; ++k) {
+        b.mn[k] = fminf(b.mn[k], fdec(a.mn[k]));
+        b.mx[k] = fmaxf(b.mx[k], fdec(a.mx[k]));
+    }
+}
+
+__device__ __forceinline__ bool box_valid(const Box& b) {
+    return b.mn[0] <= b.mx[0] && b.mn[1] <= b.mx[1] && b.mn[2] <= b.mx[2];
+}
+
+// Bounding box of the part of triangle v (9 floats) with lo <= x[ax] <= hi,
+// widened by rounding slack and kept inside [rmn, rmx]: the vertices inside
+// the slab and the edges' crossings of its two planes bound the clipped
+// polygon.  False when nothing is left.
+__device__ bool clip_box(const float* v, int ax, float lo, float hi, const float* rmn,
+                         const float* rmx, float* omn, float* omx) {
+    for (int k = 0; k < 3; ++k) {
+        omn[k] = 3.0e38f;
+        omx[k] = -3.0e38f;
+    }
+    float p[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) p[i][k] = v[i * 3 + k];
+    for (int i = 0; i < 3; ++i) {
+        if (p[i][ax] >= lo && p[i][ax] <= hi)
+            for (int k = 0; k < 3; ++k) {
+                omn[k] = fminf(omn[k], p[i][k]);
+                omx[k] = fmaxf(omx[k], p[i][k]);
+            }
+        const float* a = p[i];
+        const float* b = p[(i + 1) % 3];
+        for (int s = 0; s < 2; ++s) {
+            const float plane = s ? hi : lo;
+            if (!((a[ax] < plane && b[ax] > plane) || (a[ax] > plane && b[ax] < plane))) continue;
+            const float t = (plane - a[ax]) / (b[ax] - a[ax]);
+            for (int k = 0; k < 3; ++k) {
+                const float q = k == ax ? plane : a[k] + t * (b[k] - a[k]);
+                omn[k] = fminf(omn[k], q);
+                omx[k] = fmaxf(omx[k], q);
+            }
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (k != ax) {
+            // a crossing's other coordinates carry the rounding of t and of
+            // the edge, a few ulps of the vertices' magnitude
+            const float m = fmaxf(fmaxf(fabsf(p[0][k]), fabsf(p[1][k])), fabsf(p[2][k]));
+            const float e = m * 0x1p-20f + 1e-30f;
+            omn[k] -= e;
+            omx[k] += e;
+        } else {
+            omn[k] = fmaxf(omn[k], lo);
+            omx[k] = fminf(omx[k], hi);
+        }
+        omn[k] = fmaxf(omn[k], rmn[k]);
+        omx[k] = fminf(omx[k], rmx[k]);
+    }
+    return omn[0] <= omx[0] && omn[1] <= omx[1] && omn[2] <= omx[2];
+}
+
+// ------------------------------------------------------------- kernels --
+
+__global__ void init_refs_kernel(const DevAabb* aabbs, uint32_t n, SRef* refs) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevAabb a = aabbs[i];
+    SRef r;
+    for (int k = 0; k < 3; ++k) {
+        r.mn[k] = a.min[k];
+        r.mx[k] = a.max[k];
+    }
+    r.prim = i;
+    r.task = 0;
+    refs[i] = r;
+}
+
+__global__ void point_refs_kernel(const float* centers, uint32_t stride, uint32_t n,
+                                  SRef* refs) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    SRef r;
+    for (int k = 0; k < 3; ++k) r.mn[k] = r.mx[k] = centers[size_t(i) * stride + k];
+    r.prim = i;
+    r.task = 0;
+    refs[i] = r;
+}
+
+// Per task: node box and object bins cleared (block t, 64 threads).
+__global__ void task_clear_kernel(uint32_t num_tasks, Acc* tbox, Acc* obins) {
+    const uint32_t t = blockIdx.x;
+    if (t >= num_tasks) return;
+    if (threadIdx.x == 0) acc_clear(tbox[t]);
+    if (threadIdx.x < 3 * kBins) {
+        Acc a;
+        acc_clear(a);
+        obins[size_t(t) * 3 * kBins + threadIdx.x] = a;
+    }
+}
+
+__global__ void cand_clear_kernel(uint32_t count, Acc* sbins) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    Acc a;
+    acc_clear(a);
+    sbins[i] = a;
+}
+
+// Node boxes: the union of each task's reference boxes.  A block whose
+// references all belong to one task reduces in LDS first.
+__global__ void __launch_bounds__(kBlock)
+    ref_bounds_kernel(uint32_t nrefs, const SRef* refs, Acc* tbox) {
+    __shared__ uint32_t s_mn[3], s_mx[3];
+    __shared__ int32_t s_uni;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t first = blockIdx.x * kBlock;
+    const uint32_t last = (first + kBlock < nrefs ? first + kBlock : nrefs) - 1;
+    if (threadIdx.x == 0) {
+        const int32_t a = refs[first].task;
+        s_uni = (a >= 0 && refs[last].task == a) ? a : -1;
+        for (int k = 0; k < 3; ++k) {
+            s_mn[k] = fenc(1e34f);
+            s_mx[k] = fenc(-1e34f);
+        }
+    }
+    __syncthreads();
+    const int32_t uni = s_uni;
+    if (i < nrefs) {
+        const SRef r = refs[i];
+        if (r.task >= 0)
+            for (int k = 0; k < 3; ++k) {
+                if (!(r.mn[k] <= r.mx[k])) continue;  // empty or NaN
+                if (uni >= 0) {
+                    atomicMin(&s_mn[k], fenc(r.mn[k]));
+                    atomicMax(&s_mx[k], fenc(r.mx[k]));
+                } else {
+                    atomicMin(&tbox[r.task].mn[k], fenc(r.mn[k]));
+                    atomicMax(&tbox[r.task].mx[k], fenc(r.mx[k]));
+                }
+            }
+    }
+    __syncthreads();
+    if (uni >= 0 && threadIdx.x < 3) {
+        atomicMin(&tbox[uni].mn[threadIdx.x], s_mn[threadIdx.x]);
+        atomicMax(&tbox[uni].mx[threadIdx.x], s_mx[threadIdx.x]);
+    }
+}
+
+// Padded node box and bin parameters; the root's surface at level 0.
+__global__ void task_setup_kernel(uint32_t num_tasks, const Acc* tbox, SDec* dec,
+                                  float* root_half_area, uint32_t level) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tasks) return;
+    SDec d;
+    const Acc b = tbox[t];
+    float mn[3], mx[3];
+    for (int k = 0; k < 3; ++k) {
+        mn[k] = fdec(b.mn[k]);
+        mx[k] = fdec(b.mx[k]);
+        if (!(mn[k] <= mx[k])) mn[k] = mx[k] = 0.0f;  // no valid reference box
+        d.mn[k] = mn[k] - 0.0001f;
+        d.mx[k] = mx[k] + 0.0001f;
+        d.org[k] = d.mn[k];
+        const float ext = d.mx[k] - d.mn[k];
+        d.width[k] = ext / float(kBins);
+        d.inv[k] = ext > 0.0f ? float(kBins) / ext : 0.0f;
+    }
+    d.kind = kLeaf;
+    d.axis = d.k = d.left = d.right = 0;
+    d.cost = 3.0e38f;
+    d.cand = 0;
+    d.s_axis = d.s_k = d.s_left = d.s_right = d.dup = 0;
+    d.s_cost = 3.0e38f;
+    dec[t] = d;
+    if (level == 0 && t == 0) *root_half_area = half_area6(mn, mx);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    obj_bin_kernel(uint32_t nrefs, const SRef* refs, const STask* tasks, const SDec* dec,
+                   Acc* obins) {
+    __shared__ Acc s_b[3 * kBins];
+    __shared__ int32_t s_uni;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t first = blockIdx.x * kBlock;
+    const uint32_t last = (first + kBlock < nrefs ? first + kBlock : nrefs) - 1;
+    if (threadIdx.x == 0) {
+        const int32_t a = refs[first].task;
+        s_uni = (a >= 0 && refs[last].task == a) ? a : -1;
+    }
+    if (threadIdx.x < 3 * kBins) acc_clear(s_b[threadIdx.x]);
+    __syncthreads();
+    const int32_t uni = s_uni;
+    if (i < nrefs) {
+        const SRef r = refs[i];
+        const int32_t t = r.task;
+        if (t >= 0) {
+            const STask tk = tasks[t];
+            if (tk.end - tk.begin > 1 && tk.depth < kMaxDepth) {
+                const SDec& d = dec[t];
+                for (int ax = 0; ax < 3; ++ax) {
+                    const float c = 0.5f * (r.mn[ax] + r.mx[ax]);
+                    const uint32_t slot = ax * kBins + bin_of(c, d.org[ax], d.inv[ax]);
+                    Acc* g = uni >= 0 ? &s_b[slot] : &obins[size_t(t) * 3 * kBins + slot];
+                    atomicAdd(&g->c0, 1u);
+                    for (int k = 0; k < 3; ++k) {
+                        if (!(r.mn[k] <= r.mx[k])) continue;
+                        atomicMin(&g->mn[k], fenc(r.mn[k]));
+                        atomicMax(&g->mx[k], fenc(r.mx[k]));
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (uni >= 0 && threadIdx.x < 3 * kBins && s_b[threadIdx.x].c0 > 0) {
+        Acc* g = &obins[size_t(uni) * 3 * kBins + threadIdx.x];
+        atomicAdd(&g->c0, s_b[threadIdx.x].c0);
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&g->mn[k], s_b[threadIdx.x].mn[k]);
+            atomicMax(&g->mx[k], s_b[threadIdx.x].mx[k]);
+        }
+    }
+}
+
+// Best object split and whether a spatial split is worth evaluating.
+__global__ void obj_split_kernel(uint32_t num_tasks, const STask* tasks, SDec* dec,
+                                 const Acc* obins, const float* root_half_area,
+                                 uint32_t* cand_flag) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tasks) return;
+    const STask tk = tasks[t];
+    SDec d = dec[t];
+    const uint32_t work = tk.end - tk.begin;
+    uint32_t cand = 0;
+    if (work > 1 && tk.depth < kMaxDepth) {
+        const Acc* tb = obins + size_t(t) * 3 * kBins;
+        Box lbest, rbest;
+        box_empty(lbest);
+        box_empty(rbest);
+        for (int ax = 0; ax < 3; ++ax) {
+            const Acc* b = tb + ax * kBins;
+            float rcost[kBins];
+            Box rb[kBins];
+            Box cur;
+            box_empty(cur);
+            uint32_t cnt = 0;
+            for (int i = kBins - 1; i > 0; --i) {
+                box_grow_acc(cur, b[i]);
+                cnt += b[i].c0;
+                rcost[i] = half_area6(cur.mn, cur.mx) * float(cnt);
+                rb[i] = cur;
+            }
+            box_empty(cur);
+            cnt = 0;
+            for (int i = 0; i < kBins - 1; ++i) {
+                box_grow_acc(cur, b[i]);
+                cnt += b[i].c0;
+                if (cnt == 0 || cnt == work) continue;
+                const float cost = half_area6(cur.mn, cur.mx) * float(cnt) + rcost[i + 1];
+                if (cost < d.cost) {
+                    d.cost = cost;
+                    d.kind = kObject;
+                    d.axis = uint32_t(ax);
+                    d.k = uint32_t(i + 1);
+                    d.left = cnt;
+                    d.right = work - cnt;
+                    lbest = cur;
+                    rbest = rb[i + 1];
+                }
+            }
+        }
+        if (d.kind == kObject) {
+            Box ov;
+            for (int k = 0; k < 3; ++k) {
+                ov.mn[k] = fmaxf(lbest.mn[k], rbest.mn[k]);
+                ov.mx[k] = fminf(lbest.mx[k], rbest.mx[k]);
+            }
+            if (box_valid(ov) && half_area6(ov.mn, ov.mx) > kAlpha * *root_half_area) cand = 1;
+        } else {
+            cand = 1;  // every centroid in one bin: a plane may still separate them
+        }
+    }
+    d.cand = cand;
+    dec[t] = d;
+    cand_flag[t] = cand;
+}
+
+// Chopped spatial binning of the references of candidate tasks.
+__global__ void __launch_bounds__(kBlock)
+    sp_bin_kernel(uint32_t nrefs, const SRef* refs, const SDec* dec, const uint32_t* cand_excl,
+                  const float* verts, uint32_t prim_count, Acc* sbins) {
+    __shared__ Acc s_b[3 * kBins];
+    __shared__ int32_t s_uni;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t first = blockIdx.x * kBlock;
+    const uint32_t last = (first + kBlock < nrefs ? first + kBlock : nrefs) - 1;
+    if (threadIdx.x == 0) {
+        const int32_t a = refs[first].task;
+        s_uni = (a >= 0 && refs[last].task == a && dec[a].cand && cand_excl[a] < kMaxCand) ? a
+                                                                                          : -1;
+    }
+    if (threadIdx.x < 3 * kBins) acc_clear(s_b[threadIdx.x]);
+    __syncthreads();
+    const int32_t uni = s_uni;
+    if (i < nrefs) {
+        const SRef r = refs[i];
+        const int32_t t = r.task;
+        if (t >= 0 && dec[t].cand && cand_excl[t] < kMaxCand && r.prim < prim_count) {
+            const SDec& d = dec[t];
+            Acc* base = uni >= 0 ? s_b : sbins + size_t(cand_excl[t]) * 3 * kBins;
+            const float* v = verts + size_t(r.prim) * 9;
+            for (int ax = 0; ax < 3; ++ax) {
+                const uint32_t b0 = bin_of(r.mn[ax], d.org[ax], d.inv[ax]);
+                const uint32_t b1 = max(b0, bin_of(r.mx[ax], d.org[ax], d.inv[ax]));
+                atomicAdd(&base[ax * kBins + b0].c0, 1u);
+                atomicAdd(&base[ax * kBins + b1].c1, 1u);
+                for (uint32_t b = b0; b <= b1 && b < uint32_t(kBins); ++b) {
+                    float mn[3], mx[3];
+                    if (b0 == b1) {
+                        for (int k = 0; k < 3; ++k) {
+                            mn[k] = r.mn[k];
+                            mx[k] = r.mx[k];
+                        }
+                    } else {
+                        const float lo = b == b0 ? -3.0e38f : d.org[ax] + float(b) * d.width[ax];
+                        const float hi =
+                            b == b1 ? 3.0e38f : d.org[ax] + float(b + 1) * d.width[ax];
+                        if (!clip_box(v, ax, lo, hi, r.mn, r.mx, mn, mx)) continue;
+                    }
+                    Acc* g = &base[ax * kBins + b];
+                    for (int k = 0; k < 3; ++k) {
+                        if (!(mn[k] <= mx[k])) continue;
+                        atomicMin(&g->mn[k], fenc(mn[k]));
+                        atomicMax(&g->mx[k], fenc(mx[k]));
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (uni >= 0 && threadIdx.x < 3 * kBins) {
+        const Acc& s = s_b[threadIdx.x];
+        if (s.c0 > 0 || s.c1 > 0) {
+            Acc* g = &sbins[size_t(cand_excl[uni]) * 3 * kBins + threadIdx.x];
+            atomicAdd(&g->c0, s.c0);
+            atomicAdd(&g->c1, s.c1);
+            for (int k = 0; k < 3; ++k) {
+                atomicMin(&g->mn[k], s.mn[k]);
+                atomicMax(&g->mx[k], s.mx[k]);
+            }
+        }
+    }
+}
+
+// Best spatial split of each evaluated candidate; dup[t] = its duplicates
+// when it beats the object split (0 otherwise).
+__global__ void sp_split_kernel(uint32_t num_tasks, const STask* tasks, SDec* dec,
+                                const uint32_t* cand_excl, const Acc* sbins, uint32_t* dup) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tasks) return;
+    SDec d = dec[t];
+    uint32_t want = 0;
+    if (d.cand && cand_excl[t] < kMaxCand) {
+        const STask tk = tasks[t];
+        const uint32_t work = tk.end - tk.begin;
+        const Acc* tb = sbins + size_t(cand_excl[t]) * 3 * kBins;
+        for (int ax = 0; ax < 3; ++ax) {
+            const Acc* b = tb + ax * kBins;
+            float rcost[kBins];
+            uint32_t rcnt[kBins];
+            Box cur;
+            box_empty(cur);
+            uint32_t exits = 0;
+            for (int i = kBins - 1; i > 0; --i) {
+                box_grow_acc(cur, b[i]);
+                exits += b[i].c1;
+                rcnt[i] = exits;
+                rcost[i] = half_area6(cur.mn, cur.mx) * float(exits);
+            }
+            box_empty(cur);
+            uint32_t entries = 0;
+            for (int i = 0; i < kBins - 1; ++i) {
+                box_grow_acc(cur, b[i]);
+                entries += b[i].c0;
+                const uint32_t l = entries, r = rcnt[i + 1];
+                if (l == 0 || r == 0 || l + r < work) continue;
+                const float cost = half_area6(cur.mn, cur.mx) * float(l) + rcost[i + 1];
+                if (cost < d.s_cost) {
+                    d.s_cost = cost;
+                    d.s_axis = uint32_t(ax);
+                    d.s_k = uint32_t(i + 1);
+                    d.s_left = l;
+                    d.s_right = r;
+                }
+            }
+        }
+        if (d.s_cost < d.cost) {
+            d.dup = d.s_left + d.s_right - work;
+            want = 1;
+        }
+    }
+    dec[t] = d;
+    dup[t] = want ? d.dup : 0u;
+}
+
+// Final decision per task; counts for the scans.
+__global__ void decide_kernel(uint32_t num_tasks, const STask* tasks, SDec* dec,
+                              const uint32_t* dup_excl, uint32_t remaining, uint32_t max_leaf,
+                              const Acc* obins, uint32_t* split_flag, uint32_t* out_cnt,
+                              uint32_t* leaf_cnt) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tasks) return;
+    const STask tk = tasks[t];
+    SDec d = dec[t];
+    const uint32_t work = tk.end - tk.begin;
+    if (work > 1 && tk.depth < kMaxDepth) {
+        float best = d.cost;
+        if (d.s_cost < d.cost && d.dup <= remaining && dup_excl[t] <= remaining - d.dup) {
+            d.kind = kSpatial;
+            d.axis = d.s_axis;
+            d.k = d.s_k;
+            d.left = d.s_left;
+            d.right = d.s_right;
+            best = d.s_cost;
+        }
+        const float max_split_cost = half_area6(d.mn, d.mx) * (float(work) - 1.0f);
+        if (d.kind == kLeaf || !(best < max_split_cost)) {
+            if (work > max_leaf) {
+                // approximate median on the longest axis; an index median
+                // when the centroids share one bin
+                uint32_t ax = 0;
+                if (d.mx[1] - d.mn[1] > d.mx[0] - d.mn[0]) ax = 1;
+                if (d.mx[2] - d.mn[2] > d.mx[ax] - d.mn[ax]) ax = 2;
+                const Acc* b = obins + size_t(t) * 3 * kBins + ax * kBins;
+                uint32_t c = 0, k = 0;
+                for (int i = 0; i < kBins - 1; ++i) {
+                    c += b[i].c0;
+                    if (c >= work * 2 / 5 + 1) {
+                        k = uint32_t(i + 1);
+                        break;
+                    }
+                }
+                if (k > 0 && c > 0 && c < work) {
+                    d.kind = kObject;
+                    d.axis = ax;
+                    d.k = k;
+                    d.left = c;
+                    d.right = work - c;
+                } else {
+                    d.kind = kIndex;
+                    d.left = work / 2;
+                    d.right = work - work / 2;
+                }
+            } else {
+                d.kind = kLeaf;
+            }
+        }
+    } else {
+        d.kind = kLeaf;
+    }
+    dec[t] = d;
+    split_flag[t] = d.kind != kLeaf ? 1u : 0u;
+    out_cnt[t] = d.kind != kLeaf ? d.left + d.right : 0u;
+    leaf_cnt[t] = d.kind == kLeaf ? work : 0u;
+}
+
+__global__ void children_kernel(uint32_t num_tasks, const STask* tasks, const SDec* dec,
+                                const uint32_t* split_excl, const uint32_t* out_excl,
+                                const uint32_t* leaf_excl, uint32_t counter, uint32_t leaf_base,
+                                uint32_t node_cap, DevAabb* nodes, STask* next,
+                                uint32_t* fail) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tasks) return;
+    const STask tk = tasks[t];
+    const SDec& d = dec[t];
+    DevAabb nd;
+    for (int k = 0; k < 3; ++k) {
+        nd.min[k] = d.mn[k];
+        nd.max[k] = d.mx[k];
+    }
+    if (d.kind != kLeaf) {
+        const uint32_t child = counter + 2 * split_excl[t];
+        if (child + 1 >= node_cap) {
+            atomicOr(fail, 1u);
+            return;
+        }
+        nd.extra1 = -1;
+        nd.extra2 = int32_t(child);
+        STask l, r;
+        l.node = child;
+        r.node = child + 1;
+        l.begin = out_excl[t];
+        l.end = l.begin + d.left;
+        r.begin = l.end;
+        r.end = r.begin + d.right;
+        l.depth = r.depth = tk.depth + 1;
+        next[2 * split_excl[t]] = l;
+        next[2 * split_excl[t] + 1] = r;
+    } else {
+        nd.extra1 = int32_t(tk.end - tk.begin);
+        nd.extra2 = int32_t(leaf_base + leaf_excl[t]);
+    }
+    if (tk.node < node_cap) nodes[tk.node] = nd;
+}
+
+// Which child (or both) each reference goes to.
+__global__ void side_kernel(uint32_t nrefs, const SRef* refs, const STask* tasks,
+                            const SDec* dec, uint32_t* fl, uint32_t* fr) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrefs) return;
+    const SRef r = refs[i];
+    uint32_t l = 0, rr = 0;
+    if (r.task >= 0) {
+        const SDec& d = dec[r.task];
+        const uint32_t ax = d.axis;
+        if (d.kind == kObject) {
+            l = bin_of(0.5f * (r.mn[ax] + r.mx[ax]), d.org[ax], d.inv[ax]) < d.k ? 1u : 0u;
+            rr = 1u - l;
+        } else if (d.kind == kSpatial) {  // the spans of the spatial binning
+            const uint32_t b0 = bin_of(r.mn[ax], d.org[ax], d.inv[ax]);
+            const uint32_t b1 = max(b0, bin_of(r.mx[ax], d.org[ax], d.inv[ax]));
+            l = b0 < d.k ? 1u : 0u;
+            rr = b1 >= d.k ? 1u : 0u;
+        } else if (d.kind == kIndex) {
+            l = i - tasks[r.task].begin < d.left ? 1u : 0u;
+            rr = 1u - l;
+        }
+    }
+    fl[i] = l;
+    fr[i] = rr;
+}
+
+__global__ void scatter_kernel(uint32_t nrefs, const SRef* refs, const STask* tasks,
+                               const SDec* dec, const uint32_t* split_excl,
+                               const uint32_t* out_excl, const uint32_t* leaf_excl,
+                               uint32_t leaf_base, const uint32_t* fl, const uint32_t* fr,
+                               const uint32_t* fl_excl, const uint32_t* fr_excl,
+                               const float* verts, uint32_t prim_count, uint32_t cap,
+                               SRef* out, uint32_t* indices, uint32_t* fail) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrefs) return;
+    const SRef r = refs[i];
+    if (r.task < 0) return;
+    const STask tk = tasks[r.task];
+    const SDec& d = dec[r.task];
+    if (d.kind == kLeaf) {
+        const uint32_t pos = leaf_base + leaf_excl[r.task] + (i - tk.begin);
+        if (pos < cap)
+            indices[pos] = r.prim;
+        else
+            atomicOr(fail, 1u);
+        return;
+    }
+    const bool l = fl[i] != 0, rr = fr[i] != 0;
+    SRef a = r, b = r;
+    if (l && rr) {  // a straddling reference: one clipped part per side
+        const int ax = int(d.axis);
+        const float plane = d.org[ax] + float(d.k) * d.width[ax];
+        bool ok_l = false, ok_r = false;
+        if (r.prim < prim_count) {
+            const float* v = verts + size_t(r.prim) * 9;
+            ok_l = clip_box(v, ax, -3.0e38f, plane, r.mn, r.mx, a.mn, a.mx);
+            ok_r = clip_box(v, ax, plane, 3.0e38f, r.mn, r.mx, b.mn, b.mx);
+        }
+        // keep coverage when clipping leaves nothing on a side: the
+        // reference's own box cut at the plane
+        if (!ok_l) {
+            for (int k = 0; k < 3; ++k) {
+                a.mn[k] = r.mn[k];
+                a.mx[k] = r.mx[k];
+            }
+            a.mx[ax] = fmaxf(fminf(r.mx[ax], plane), r.mn[ax]);
+        }
+        if (!ok_r) {
+            for (int k = 0; k < 3; ++k) {
+                b.mn[k] = r.mn[k];
+                b.mx[k] = r.mx[k];
+            }
+            b.mn[ax] = fminf(fmaxf(r.mn[ax], plane), r.mx[ax]);
+        }
+    }
+    if (l) {
+        const uint32_t pos = out_excl[r.task] + (fl_excl[i] - fl_excl[tk.begin]);
+        a.task = int32_t(2 * split_excl[r.task]);
+        if (pos < cap)
+            out[pos] = a;
+        else
+            atomicOr(fail, 1u);
+    }
+    if (rr) {
+        const uint32_t pos = out_excl[r.task] + d.left + (fr_excl[i] - fr_excl[tk.begin]);
+        b.task = int32_t(2 * split_excl[r.task] + 1);
+        if (pos < cap)
+            out[pos] = b;
+        else
+            atomicOr(fail, 1u);
+    }
+}
+
+__global__ void root_task_kernel(STask* task, uint32_t n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        STask t;
+        t.node = 0;
+        t.begin = 0;
+        t.end = n;
+        t.depth = 0;
+        *task = t;
+    }
+}
+
+template <typename T>
+struct Buf {
+    T* p = nullptr;
+    size_t cap = 0;
+    ~Buf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t ensure(size_t count) {
+        if (count <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, count * sizeof(T) + 16);
+        if (e == hipSuccess) cap = count;
+        return e;
+    }
+};
+
+}  // namespace
+
+ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t stride_floats,
+                             const float* verts, size_t n64, size_t max_leaf, hipStream_t stream,
+                             DeviceBvh* out) {
+    if (n64 == 0) return NoPrimitives;
+    if (n64 > (1u << 29)) {
+        set_error("prim_count %zu too large for the spatial builder", n64);
+        return InvalidArgument;
+    }
+    const uint32_t n = uint32_t(n64);
+    const uint32_t cap = n + uint32_t(float(n) * kSplitFactor);  // references at most
+    const uint32_t node_cap = 2 * cap;
+    const uint32_t leaf = max_leaf == 0 ? 1u : uint32_t(std::min<size_t>(max_leaf, 0xFFFFFFFFu));
+    const uint32_t stride = uint32_t(stride_floats);
+    hipError_t e;
+    DeviceBvh res;
+    if ((e = hipMalloc(&res.nodes, size_t(node_cap) * sizeof(DevAabb))) != hipSuccess)
+        return hip_fail(e, "hipMalloc(nodes)");
+    if ((e = hipMalloc(&res.indices, size_t(cap) * 4)) != hipSuccess) {
+        (void)hipFree(res.nodes);
+        return hip_fail(e, "hipMalloc(indices)");
+    }
+    auto fail = [&](hipError_t err, const char* what) {
+        (void)hipFree(res.nodes);
+        (void)hipFree(res.indices);
+        return hip_fail(err, what);
+    };
+    auto fail_msg = [&]() {  // the message is already set
+        (void)hipFree(res.nodes);
+        (void)hipFree(res.indices);
+        return Error;
+    };
+    Buf<SRef> r0, r1;
+    Buf<STask> t0, t1;
+    Buf<SDec> dec;
+    Buf<Acc> tbox, obins, sbins;
+    Buf<uint32_t> split_flag, out_cnt, leaf_cnt, cand, dup, fl, fr, fle, fre, scratch, words;
+    uint32_t* host = nullptr;  // scan totals: splits, next refs, leaf refs, candidates, dups
+    if ((e = hipHostMalloc(&host, 8 * 4)) != hipSuccess) return fail(e, "hipHostMalloc");
+    struct HostFree {
+        uint32_t* p;
+        ~HostFree() { (void)hipHostFree(p); }
+    } hf{host};
+    const size_t sbytes = scan_scratch_bytes(cap) + 64;
+    if ((e = r0.ensure(cap)) != hipSuccess || (e = r1.ensure(cap)) != hipSuccess ||
+        (e = t0.ensure(1)) != hipSuccess || (e = fl.ensure(cap)) != hipSuccess ||
+        (e = fr.ensure(cap)) != hipSuccess || (e = fle.ensure(cap)) != hipSuccess ||
+        (e = fre.ensure(cap)) != hipSuccess || (e = scratch.ensure(sbytes / 4 + 1)) != hipSuccess ||
+        (e = words.ensure(4)) != hipSuccess ||
+        (e = sbins.ensure(size_t(std::min<uint32_t>(cap, kMaxCand)) * 3 * kBins)) != hipSuccess)
+        return fail(e, "spatial builder workspace");
+    float* root_half_area = reinterpret_cast<float*>(words.p);
+    uint32_t* fail_word = words.p + 1;
+    if ((e = hipMemsetAsync(words.p, 0, 16, stream)) != hipSuccess)
+        return fail(e, "spatial builder state");
+    const uint32_t g0 = (n + kBlock - 1) / kBlock;
+    if (aabbs)
+        hipLaunchKernelGGL(init_refs_kernel, dim3(g0), dim3(kBlock), 0, stream,
+                           reinterpret_cast<const DevAabb*>(aabbs), n, r0.p);
+    else
+        hipLaunchKernelGGL(point_refs_kernel, dim3(g0), dim3(kBlock), 0, stream, centers, stride,
+                           n, r0.p);
+    hipLaunchKernelGGL(root_task_kernel, dim3(1), dim3(1), 0, stream, t0.p, n);
+    SRef* ref_cur = r0.p;
+    SRef* ref_nxt = r1.p;
+    STask* t_cur = t0.p;
+    uint32_t nrefs = n, num_tasks = 1, counter = 1, leaf_base = 0, dups = 0;
+    const uint32_t budget = cap - n;
+    int flip = 0;
+    for (uint32_t level = 0; num_tasks > 0; ++level) {
+        if (level > kMaxDepth + 1) {
+            set_error("spatial builder did not finish in %u levels", kMaxDepth + 1);
+            return fail_msg();
+        }
+        if ((e = dec.ensure(num_tasks)) != hipSuccess || (e = tbox.ensure(num_tasks)) != hipSuccess ||
+            (e = obins.ensure(size_t(num_tasks) * 3 * kBins)) != hipSuccess ||
+            (e = split_flag.ensure(num_tasks)) != hipSuccess ||
+            (e = out_cnt.ensure(num_tasks)) != hipSuccess ||
+            (e = leaf_cnt.ensure(num_tasks)) != hipSuccess ||
+            (e = cand.ensure(num_tasks)) != hipSuccess || (e = dup.ensure(num_tasks)) != hipSuccess)
+            return fail(e, "spatial level workspace");
+        Buf<STask>& nb = flip ? t0 : t1;
+        if ((e = nb.ensure(size_t(2) * num_tasks)) != hipSuccess) return fail(e, "spatial tasks");
+        STask* t_nxt = nb.p;
+        const uint32_t tg = (num_tasks + kBlock - 1) / kBlock;
+        const uint32_t rg = (nrefs + kBlock - 1) / kBlock;
+        const uint32_t ncand = std::min<uint32_t>(num_tasks, kMaxCand);
+        hipLaunchKernelGGL(task_clear_kernel, dim3(num_tasks), dim3(64), 0, stream, num_tasks,
+                           tbox.p, obins.p);
+        hipLaunchKernelGGL(ref_bounds_kernel, dim3(rg), dim3(kBlock), 0, stream, nrefs, ref_cur,
+                           tbox.p);
+        hipLaunchKernelGGL(task_setup_kernel, dim3(tg), dim3(kBlock), 0, stream, num_tasks,
+                           tbox.p, dec.p, root_half_area, level);
+        hipLaunchKernelGGL(obj_bin_kernel, dim3(rg), dim3(kBlock), 0, stream, nrefs, ref_cur,
+                           t_cur, dec.p, obins.p);
+        hipLaunchKernelGGL(obj_split_kernel, dim3(tg), dim3(kBlock), 0, stream, num_tasks, t_cur,
+                           dec.p, obins.p, root_half_area, cand.p);
+        if ((e = scan_exclusive_u32(cand.p, num_tasks, host + 3, scratch.p, scratch.cap * 4,
+                                    stream)) != hipSuccess)
+            return fail(e, "spatial candidate scan");
+        const uint32_t ccl = ncand * 3 * kBins;
+        hipLaunchKernelGGL(cand_clear_kernel, dim3((ccl + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                           stream, ccl, sbins.p);
+        hipLaunchKernelGGL(sp_bin_kernel, dim3(rg), dim3(kBlock), 0, stream, nrefs, ref_cur, dec.p,
+                           cand.p, verts, n, sbins.p);
+        hipLaunchKernelGGL(sp_split_kernel, dim3(tg), dim3(kBlock), 0, stream, num_tasks, t_cur,
+                           dec.p, cand.p, sbins.p, dup.p);
+        if ((e = scan_exclusive_u32(dup.p, num_tasks, host + 4, scratch.p, scratch.cap * 4,
+                                    stream)) != hipSuccess)
+            return fail(e, "spatial budget scan");
+        hipLaunchKernelGGL(decide_kernel, dim3(tg), dim3(kBlock), 0, stream, num_tasks, t_cur,
+                           dec.p, dup.p, budget - dups, leaf, obins.p, split_flag.p, out_cnt.p,
+                           leaf_cnt.p);
+        if ((e = scan_exclusive_u32(split_flag.p, num_tasks, host + 0, scratch.p,
+                                    scratch.cap * 4, stream)) != hipSuccess ||
+            (e = scan_exclusive_u32(out_cnt.p, num_tasks, host + 1, scratch.p, scratch.cap * 4,
+                                    stream)) != hipSuccess ||
+            (e = scan_exclusive_u32(leaf_cnt.p, num_tasks, host + 2, scratch.p, scratch.cap * 4,
+                                    stream)) != hipSuccess)
+            return fail(e, "spatial task scans");
+        hipLaunchKernelGGL(children_kernel, dim3(tg), dim3(kBlock), 0, stream, num_tasks, t_cur,
+                           dec.p, split_flag.p, out_cnt.p, leaf_cnt.p, counter, leaf_base,
+                           node_cap, res.nodes, t_nxt, fail_word);
+        hipLaunchKernelGGL(side_kernel, dim3(rg), dim3(kBlock), 0, stream, nrefs, ref_cur, t_cur,
+                           dec.p, fl.p, fr.p);
+        if ((e = hipMemcpyAsync(fle.p, fl.p, size_t(nrefs) * 4, hipMemcpyDeviceToDevice,
+                                stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(fre.p, fr.p, size_t(nrefs) * 4, hipMemcpyDeviceToDevice,
+                                stream)) != hipSuccess)
+            return fail(e, "spatial side flags");
+        if ((e = scan_exclusive_u32(fle.p, nrefs, nullptr, scratch.p, scratch.cap * 4,
+                                    stream)) != hipSuccess ||
+            (e = scan_exclusive_u32(fre.p, nrefs, nullptr, scratch.p, scratch.cap * 4,
+                                    stream)) != hipSuccess)
+            return fail(e, "spatial partition scans");
+        hipLaunchKernelGGL(scatter_kernel, dim3(rg), dim3(kBlock), 0, stream, nrefs, ref_cur,
+                           t_cur, dec.p, split_flag.p, out_cnt.p, leaf_cnt.p, leaf_base, fl.p,
+                           fr.p, fle.p, fre.p, verts, n, cap, ref_nxt, res.indices, fail_word);
+        if ((e = hipGetLastError()) != hipSuccess) return fail(e, "spatial level");
+        if ((e = hipMemcpyAsync(&host[5], fail_word, 4, hipMemcpyDeviceToHost, stream)) !=
+                hipSuccess ||
+            (e = hipStreamSynchronize(stream)) != hipSuccess)
+            return fail(e, "spatial level");
+        if (host[5]) {
+            set_error("spatial builder ran out of reference or node space");
+            return fail_msg();
+        }
+        const uint32_t splits = host[0], next_refs = host[1], leaf_refs = host[2];
+        if (next_refs + leaf_refs < nrefs || next_refs > cap || leaf_base + leaf_refs > cap) {
+            set_error("spatial builder lost references (%u + %u of %u)", next_refs, leaf_refs,
+                      nrefs);
+            return fail_msg();
+        }
+        dups += next_refs + leaf_refs - nrefs;
+        leaf_base += leaf_refs;
+        counter += 2 * splits;
+        num_tasks = 2 * splits;
+        nrefs = next_refs;
+        std::swap(ref_cur, ref_nxt);
+        t_cur = t_nxt;
+        flip ^= 1;
+    }
+    res.node_count = counter;
+    res.prim_count = leaf_base;  // index entries: the primitives and their duplicates
+    res.root_count = 1;
+    *out = res;
+    return Ok;
+}
+
+}  // namespace rtbvh

@@ -80,7 +80,7 @@ struct DeviceBvh {
     int32_t* parents = nullptr;  // lazily built by ensure_parents()
     uint32_t* order = nullptr;   // breadth-first node order, lazily built for the refit
     uint32_t node_count = 0;
-    uint32_t prim_count = 0;
+    uint32_t prim_count = 0;  // entries of indices (above the primitives after spatial splits)
     uint32_t root_count = 1;
     std::vector<uint32_t> level_off;  // order[level_off[L], level_off[L+1]) is depth L
 };
@@ -157,6 +157,11 @@ void trace_tree_plan_destroy(TraceTreePlan* t);
 
 ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t stride_floats,
                             size_t n, size_t max_leaf, hipStream_t stream, DeviceBvh* out);
+// Spatial-split SAH; verts: device, 9 floats per primitive.  The result's
+// prim_count is its number of index entries (primitives and duplicates).
+ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t stride_floats,
+                             const float* verts, size_t n, size_t max_leaf, hipStream_t stream,
+                             DeviceBvh* out);
 ResultCode gpu_build_mbvh(DeviceBvh& bvh, hipStream_t stream, DeviceMbvh* out);
 ResultCode gpu_refit(DeviceBvh& bvh, const RTAabb* aabbs, hipStream_t stream);
 ResultCode ensure_parents(DeviceBvh& bvh, hipStream_t stream);
