@@ -83,6 +83,7 @@ struct SDec {
     float mn[3], mx[3];   // padded node box
     float org[3], inv[3];  // bins: b = (x - org) * inv, 16 per axis
     float width[3];       // bin width
+    float area;           // half area of the unpadded node box
     uint32_t kind, axis, k, left, right;  // split bin boundary k; child sizes
     float cost;           // best cost so far
     uint32_t cand;        // 1: spatial split evaluated
@@ -302,8 +303,9 @@ __global__ void task_setup_kernel(uint32_t num_tasks, const Acc* tbox, SDec* dec
     d.cand = 0;
     d.s_axis = d.s_k = d.s_left = d.s_right = d.dup = 0;
     d.s_cost = 3.0e38f;
+    d.area = half_area6(mn, mx);
     dec[t] = d;
-    if (level == 0 && t == 0) *root_half_area = half_area6(mn, mx);
+    if (level == 0 && t == 0) *root_half_area = d.area;
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -554,7 +556,10 @@ __global__ void decide_kernel(uint32_t num_tasks, const STask* tasks, SDec* dec,
             d.right = d.s_right;
             best = d.s_cost;
         }
-        const float max_split_cost = half_area6(d.mn, d.mx) * (float(work) - 1.0f);
+        // against the unpadded box, as the reference compares before it pads
+        // a parent (spatial_sah.rs:781-782, :352): the padding would make
+        // every split of a small node look cheap
+        const float max_split_cost = d.area * (float(work) - 1.0f);
         if (d.kind == kLeaf || !(best < max_split_cost)) {
             if (work > max_leaf) {
                 // approximate median on the longest axis; an index median

@@ -282,7 +282,11 @@ struct Cand {
     float lo[3], hi[3];
 };
 
-__device__ bool cand_prim(const LevelArgs& a, uint32_t pos, Cand& c) {
+// A triangle slot: the box of its three vertices cut to its binary leaf's
+// box.  The cut changes nothing where the leaf box holds the whole triangle
+// (every tree without spatial splits); after a spatial split the leaf holds
+// the part of the triangle inside it, and the cut keeps the slot as tight.
+__device__ bool cand_prim(const LevelArgs& a, uint32_t pos, Cand& c, const DevAabb& leaf) {
     if (pos >= a.index_count) return false;
     c.id = int32_t(pos);
     c.kind = 2;
@@ -294,6 +298,18 @@ __device__ bool cand_prim(const LevelArgs& a, uint32_t pos, Cand& c) {
             c.lo[k] = fminf(fminf(v[k], v[3 + k]), v[6 + k]);
             c.hi[k] = fmaxf(fmaxf(v[k], v[3 + k]), v[6 + k]);
         }
+        float lo[3], hi[3];
+        bool cut = true;
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fmaxf(c.lo[k], leaf.min[k]);
+            hi[k] = fminf(c.hi[k], leaf.max[k]);
+            cut = cut && lo[k] <= hi[k];
+        }
+        if (cut)
+            for (int k = 0; k < 3; ++k) {
+                c.lo[k] = lo[k];
+                c.hi[k] = hi[k];
+            }
     } else {  // stored as a triangle that no ray hits
         for (int k = 0; k < 3; ++k) c.lo[k] = c.hi[k] = 0.0f;
     }
@@ -307,7 +323,7 @@ __device__ bool cand_binary(const LevelArgs& a, int32_t i, Cand* c, int& ns) {
     ldn(a.nodes + i, n);
     if (n.extra1 == 0) return true;
     if (ns >= kWide) return false;
-    if (n.extra1 == 1) return cand_prim(a, uint32_t(n.extra2), c[ns++]);
+    if (n.extra1 == 1) return cand_prim(a, uint32_t(n.extra2), c[ns++], n);
     Cand& d = c[ns++];
     d.id = i;
     d.kind = n.extra1 > 1 ? 1 : 0;
@@ -379,7 +395,7 @@ __device__ void make_node8(const LevelArgs& a, uint32_t b, uint32_t w, QItem* qo
     if (bn.extra1 >= 0) {  // a root that is a leaf, or a leaf of several primitives
         if (bn.extra1 > kWide || bn.extra2 < 0) ok = false;
         for (int32_t j = 0; ok && j < bn.extra1; ++j)
-            ok = cand_prim(a, uint32_t(bn.extra2 + j), c[ns++]);
+            ok = cand_prim(a, uint32_t(bn.extra2 + j), c[ns++], bn);
     } else {
         // (binary node, slots) pairs still to place; at most 8 are pending
         uint32_t st_node[kWide], st_slots[kWide];
@@ -409,7 +425,7 @@ __device__ void make_node8(const LevelArgs& a, uint32_t b, uint32_t w, QItem* qo
                     break;
                 }
                 for (int32_t q = 0; ok && q < nd.extra1; ++q)
-                    ok = ns < kWide && cand_prim(a, uint32_t(nd.extra2 + q), c[ns++]);
+                    ok = ns < kWide && cand_prim(a, uint32_t(nd.extra2 + q), c[ns++], nd);
             } else if (nd.extra1 >= 0 || d == 0u) {  // one slot
                 ok = cand_binary(a, int32_t(n), c, ns);
             } else {  // spread over j slots: k to the left child, j - k to the right
