@@ -14,9 +14,11 @@ identical to brute force; a sample is checked against the CPU oracle).  The
 headline value is bounce-ray Mrays/s of the traversal tree; the JSON line
 also carries the build times (PLOC, collapse, traversal tree, binned SAH),
 the primary-ray rate, the 4-wide traversal tree's and the reference-layout
-MBVH's rates on the same rays (same hits), the roofline of the trace kernel, the CPU oracle's rate on a sample of the same
-rays (host threads stated), the batched-vs-plain comparison over 256 scenes
-of uneven size, and the animated 20M-triangle any-hit frame.
+MBVH's rates on the same rays (same hits), the spatial-split SAH tree's build
+time and rates (same hits), the roofline of the trace kernel, the CPU
+oracle's rate on a sample of the same rays (host threads stated), the
+batched-vs-plain comparison over 256 scenes of uneven size, and the animated
+20M-triangle any-hit frame.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -377,6 +379,8 @@ def main():
     ap.add_argument("--large", type=int, default=1,
                     help="1: also run the animated 20M-triangle any-hit frame (0 skips it)")
     ap.add_argument("--large-rays-log2", type=int, default=28)
+    ap.add_argument("--spatial", type=int, default=1,
+                    help="1: also build the spatial-split SAH tree and trace the same rays")
     ap.add_argument("--adjoint", type=int, default=1,
                     help="1: also time forward + adjoint backward on 100k triangles (0 skips it)")
     ap.add_argument("--trace-only", action="store_true",
@@ -578,6 +582,44 @@ def main():
     same_other_hits = bool(torch.equal(hits_o.view(torch.int32), hits.view(torch.int32)))
     del hits_o
     log(f"width-{other_width} traversal tree {other_mrays:.1f} Mrays/s (same {same_other_hits})")
+    del otree
+
+    # ---- spatial-split SAH tree (src/builders/spatial_sah.rs), same rays ----
+    spatial = None
+    if args.spatial:
+        sp_ms = []
+        sp = None
+        for i in range(3):
+            sp = None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sp = rt.Builder(c_d, aabbs=a_d).construct_spatial_sah(v_d)
+            torch.cuda.synchronize()
+            if i > 0:
+                sp_ms.append((time.perf_counter() - t0) * 1e3)
+        sp_tree = sp.trace_tree(v_d, width=args.width)
+
+        def trace_spatial(rays, out):
+            sp_tree.intersect(rays, out=out)
+
+        hits_s, per_s, _ = timed(bounce, 3, 1, fn=trace_spatial)
+        sp_bounce = n_rays / (statistics.median(per_s) * 1e-3) / 1e6
+        same_sp = bool(torch.equal(hits_s.view(torch.int32), hits.view(torch.int32)))
+        del hits_s
+        hits_s, per_s, _ = timed(primary, 3, 1, fn=trace_spatial)
+        sp_primary = n_rays / (statistics.median(per_s) * 1e-3) / 1e6
+        del hits_s
+        spatial = {
+            "build_ms": round(statistics.median(sp_ms), 2),
+            "references": sp.index_count,
+            "trace_tree_nodes": sp_tree.node_count,
+            "bounce_mrays_s": round(sp_bounce, 1),
+            "primary_mrays_s": round(sp_primary, 1),
+            "same_hits": same_sp,
+        }
+        log(f"spatial SAH: build {spatial['build_ms']} ms, {sp.index_count} references, "
+            f"bounce {sp_bounce:.1f} Mrays/s (same {same_sp}), primary {sp_primary:.1f} Mrays/s")
+        del sp_tree, sp
 
     # ---- kernel work counters (nodes and triangles per ray) ----------------
     import ctypes
@@ -686,6 +728,7 @@ def main():
         "trace_tree_nodes": ttree.node_count,
         f"width{other_width}_tree_mrays_s": round(other_mrays, 2),
         f"width{other_width}_tree_same_hits": same_other_hits,
+        "spatial_sah": spatial,
         "trace_launch_shape": tuned,
         "binned_sah_build_ms": round(statistics.median(binned_ms), 3),
         "primary_mrays_s": round(primary_mrays, 2),
