@@ -721,7 +721,9 @@ ResultCode refit(const RTAabb* aabbs, RTBvh bvh) {
         return Error;
     }
     DeviceGuard g(be->device);
-    const size_t n = be->dev.prim_count;  // Bvh::prim_count (src/bvh.rs:172-174)
+    // Bvh::prim_count (src/bvh.rs:172-174): one box per primitive, also for
+    // a spatial-split tree whose index array holds more entries
+    const size_t n = be->dev.source_prims ? be->dev.source_prims : be->dev.prim_count;
     void* d_aabbs = nullptr;
     hipError_t err = hipMalloc(&d_aabbs, n * sizeof(RTAabb) + 16);
     if (err == hipSuccess && n)

@@ -927,6 +927,7 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
     }
     res.node_count = counter;
     res.prim_count = leaf_base;  // index entries: the primitives and their duplicates
+    res.source_prims = n;
     res.root_count = 1;
     *out = res;
     return Ok;

@@ -81,6 +81,7 @@ struct DeviceBvh {
     uint32_t* order = nullptr;   // breadth-first node order, lazily built for the refit
     uint32_t node_count = 0;
     uint32_t prim_count = 0;  // entries of indices (above the primitives after spatial splits)
+    uint32_t source_prims = 0;  // primitives built over, when that differs from prim_count
     uint32_t root_count = 1;
     std::vector<uint32_t> level_off;  // order[level_off[L], level_off[L+1]) is depth L
 };
