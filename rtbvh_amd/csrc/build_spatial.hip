@@ -35,7 +35,6 @@
 // the reference's box, so the references of a triangle always cover it and
 // every hit is found.
 #include <algorithm>
-#include <vector>
 
 #include "internal.hpp"
 
