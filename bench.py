@@ -383,6 +383,9 @@ def main():
                     help="1: also build the spatial-split SAH tree and trace the same rays")
     ap.add_argument("--adjoint", type=int, default=1,
                     help="1: also time forward + adjoint backward on 100k triangles (0 skips it)")
+    ap.add_argument("--build-only", action="store_true",
+                    help="only the timed PLOC builds of the 1M scene, for profiler runs; no JSON "
+                         "line")
     ap.add_argument("--trace-only", action="store_true",
                     help="build, then only the timed bounce-ray steps of the headline trace "
                          "(and the copy calibration) for profiler runs; no JSON line")
@@ -433,6 +436,9 @@ def main():
         torch.cuda.synchronize()
         if i > 0:
             build_ms.append((time.perf_counter() - t0) * 1e3)
+    if args.build_only:
+        log(f"build only: PLOC {statistics.median(build_ms):.3f} ms (median of {len(build_ms)})")
+        return
     collapse_ms = []
     mbvh = None
     for i in range(2 if args.quick else 4):
