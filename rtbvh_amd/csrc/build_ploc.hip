@@ -47,7 +47,7 @@ constexpr int kTailThreads = 1024;
 constexpr int kTailMax = 4096;         // clusters held in LDS by the tail kernel
 // The tail takes over at this many clusters (or at one per scene, when a
 // batch has more scenes): measured on the 1M scene, 2048 builds 0.3 ms
-// faster than 4096 (the single workgroup's share shrinks) and 1024 0.9 ms
+// faster than 4096 (the single workgroup's share shrinks) and 1024 0.3 ms
 // slower (the multi-block iterations' launches cost more than they save).
 constexpr uint32_t kTailEnter = 2048;
 
