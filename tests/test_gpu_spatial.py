@@ -17,7 +17,7 @@ import pytest
 
 from oracle import refcpu as ref
 from rtbvh_amd import scenes
-from tests.helpers import as_rays, flat, hits_equal, prep, ray_set
+from tests.helpers import as_rays, flat, flat_grid, flat_grid_rays, hits_equal, prep, ray_set
 
 torch = pytest.importorskip("torch")
 import rtbvh_amd as rt  # noqa: E402
@@ -168,6 +168,29 @@ def test_spatial_leaf_size_and_refit():
     want = ref.trace_brute(verts, as_rays(rays8), threads=8)
     got = bvh.intersect(dv, dr).cpu().numpy().view(ref.HIT).reshape(-1)
     assert hits_equal(got, want).all()
+
+
+def test_spatial_duplicates_degenerates_and_flat():
+    """Exact duplicates, zero-area triangles, triangles collapsed to a point,
+    and a flat scene (one axis of zero extent) hit by axis-parallel rays:
+    the spatial build stays valid and its hits equal brute force."""
+    print("[spatial_degenerate]", flush=True)
+    rng = np.random.default_rng(81)
+    soup = slivers(400, length=0.4, seed=82)
+    dup = np.repeat(soup[:60], 4, axis=0)
+    line = soup[60:100].copy()
+    line[:, 6:9] = line[:, 3:6]
+    point = np.tile(np.float32([0.5, 0.5, 0.5]), (30, 3))
+    verts = np.concatenate([dup, soup[100:], line, point])
+    verts = verts[rng.permutation(len(verts))].astype(np.float32)
+    bvh = _build(verts, "spatial")
+    _check_structure(bvh, len(verts))
+    _hits_all_paths(bvh, verts, ray_set(verts, 8000, seed=83, kind="box"))
+    flatv = flat_grid(g=30, offset=0.25)
+    bvh = _build(flatv, "spatial")
+    _check_structure(bvh, len(flatv))
+    want = _hits_all_paths(bvh, flatv, flat_grid_rays(flatv, 4000))
+    assert (want["prim"] != 0xFFFFFFFF).sum() > 400
 
 
 def test_spatial_errors():
