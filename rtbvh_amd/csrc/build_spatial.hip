@@ -237,33 +237,47 @@ __global__ void cand_clear_kernel(uint32_t count, Acc* sbins) {
     sbins[i] = a;
 }
 
-// Node boxes: the union of each task's reference boxes.  A block whose
-// references all belong to one task reduces in LDS first.
-__global__ void __launch_bounds__(kBlock)
-    ref_bounds_kernel(uint32_t nrefs, const SRef* refs, Acc* tbox) {
-    __shared__ uint32_t s_mn[3], s_mx[3];
-    __shared__ int32_t s_uni;
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+// A level's references are ordered by task (the scatter writes each split
+// task's children, left then right, at the exclusive scan of the counts), so
+// a block's references belong to the tasks refs[first].task ..
+// refs[last].task.  When those are at most kLdsTasks, the block reduces in
+// LDS and merges one value per task and slot; returns the first task, or -1
+// for the global-atomic path.
+constexpr int kLdsTasks = 4;
+
+__device__ __forceinline__ int32_t block_tasks(const SRef* refs, uint32_t nrefs, int32_t* count) {
     const uint32_t first = blockIdx.x * kBlock;
     const uint32_t last = (first + kBlock < nrefs ? first + kBlock : nrefs) - 1;
+    const int32_t a = refs[first].task, b = refs[last].task;
+    *count = b - a + 1;
+    return (a >= 0 && b >= a && b - a < kLdsTasks) ? a : -1;
+}
+
+// Node boxes: the union of each task's reference boxes.
+__global__ void __launch_bounds__(kBlock)
+    ref_bounds_kernel(uint32_t nrefs, const SRef* refs, Acc* tbox) {
+    __shared__ uint32_t s_mn[kLdsTasks][3], s_mx[kLdsTasks][3];
+    __shared__ int32_t s_t0, s_nt;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (threadIdx.x == 0) {
-        const int32_t a = refs[first].task;
-        s_uni = (a >= 0 && refs[last].task == a) ? a : -1;
-        for (int k = 0; k < 3; ++k) {
-            s_mn[k] = fenc(1e34f);
-            s_mx[k] = fenc(-1e34f);
-        }
+        int32_t nt = 0;
+        s_t0 = block_tasks(refs, nrefs, &nt);
+        s_nt = nt;
+    }
+    if (threadIdx.x < kLdsTasks * 3) {
+        s_mn[threadIdx.x / 3][threadIdx.x % 3] = fenc(1e34f);
+        s_mx[threadIdx.x / 3][threadIdx.x % 3] = fenc(-1e34f);
     }
     __syncthreads();
-    const int32_t uni = s_uni;
+    const int32_t t0 = s_t0;
     if (i < nrefs) {
         const SRef r = refs[i];
         if (r.task >= 0)
             for (int k = 0; k < 3; ++k) {
                 if (!(r.mn[k] <= r.mx[k])) continue;  // empty or NaN
-                if (uni >= 0) {
-                    atomicMin(&s_mn[k], fenc(r.mn[k]));
-                    atomicMax(&s_mx[k], fenc(r.mx[k]));
+                if (t0 >= 0) {
+                    atomicMin(&s_mn[r.task - t0][k], fenc(r.mn[k]));
+                    atomicMax(&s_mx[r.task - t0][k], fenc(r.mx[k]));
                 } else {
                     atomicMin(&tbox[r.task].mn[k], fenc(r.mn[k]));
                     atomicMax(&tbox[r.task].mx[k], fenc(r.mx[k]));
@@ -271,9 +285,10 @@ __global__ void __launch_bounds__(kBlock)
             }
     }
     __syncthreads();
-    if (uni >= 0 && threadIdx.x < 3) {
-        atomicMin(&tbox[uni].mn[threadIdx.x], s_mn[threadIdx.x]);
-        atomicMax(&tbox[uni].mx[threadIdx.x], s_mx[threadIdx.x]);
+    if (t0 >= 0 && threadIdx.x < uint32_t(s_nt) * 3) {
+        const int tt = threadIdx.x / 3, k = threadIdx.x % 3;
+        atomicMin(&tbox[t0 + tt].mn[k], s_mn[tt][k]);
+        atomicMax(&tbox[t0 + tt].mx[k], s_mx[tt][k]);
     }
 }
 
@@ -310,18 +325,18 @@ __global__ void task_setup_kernel(uint32_t num_tasks, const Acc* tbox, SDec* dec
 __global__ void __launch_bounds__(kBlock)
     obj_bin_kernel(uint32_t nrefs, const SRef* refs, const STask* tasks, const SDec* dec,
                    Acc* obins) {
-    __shared__ Acc s_b[3 * kBins];
-    __shared__ int32_t s_uni;
+    __shared__ Acc s_b[kLdsTasks][3 * kBins];
+    __shared__ int32_t s_t0, s_nt;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t first = blockIdx.x * kBlock;
-    const uint32_t last = (first + kBlock < nrefs ? first + kBlock : nrefs) - 1;
     if (threadIdx.x == 0) {
-        const int32_t a = refs[first].task;
-        s_uni = (a >= 0 && refs[last].task == a) ? a : -1;
+        int32_t nt = 0;
+        s_t0 = block_tasks(refs, nrefs, &nt);
+        s_nt = nt;
     }
-    if (threadIdx.x < 3 * kBins) acc_clear(s_b[threadIdx.x]);
+    for (uint32_t k = threadIdx.x; k < uint32_t(kLdsTasks * 3 * kBins); k += kBlock)
+        acc_clear(s_b[k / (3 * kBins)][k % (3 * kBins)]);
     __syncthreads();
-    const int32_t uni = s_uni;
+    const int32_t t0 = s_t0;
     if (i < nrefs) {
         const SRef r = refs[i];
         const int32_t t = r.task;
@@ -332,7 +347,7 @@ __global__ void __launch_bounds__(kBlock)
                 for (int ax = 0; ax < 3; ++ax) {
                     const float c = 0.5f * (r.mn[ax] + r.mx[ax]);
                     const uint32_t slot = ax * kBins + bin_of(c, d.org[ax], d.inv[ax]);
-                    Acc* g = uni >= 0 ? &s_b[slot] : &obins[size_t(t) * 3 * kBins + slot];
+                    Acc* g = t0 >= 0 ? &s_b[t - t0][slot] : &obins[size_t(t) * 3 * kBins + slot];
                     atomicAdd(&g->c0, 1u);
                     for (int k = 0; k < 3; ++k) {
                         if (!(r.mn[k] <= r.mx[k])) continue;
@@ -344,14 +359,18 @@ __global__ void __launch_bounds__(kBlock)
         }
     }
     __syncthreads();
-    if (uni >= 0 && threadIdx.x < 3 * kBins && s_b[threadIdx.x].c0 > 0) {
-        Acc* g = &obins[size_t(uni) * 3 * kBins + threadIdx.x];
-        atomicAdd(&g->c0, s_b[threadIdx.x].c0);
-        for (int k = 0; k < 3; ++k) {
-            atomicMin(&g->mn[k], s_b[threadIdx.x].mn[k]);
-            atomicMax(&g->mx[k], s_b[threadIdx.x].mx[k]);
+    if (t0 >= 0)
+        for (uint32_t k = threadIdx.x; k < uint32_t(s_nt * 3 * kBins); k += kBlock) {
+            const uint32_t tt = k / (3 * kBins), slot = k % (3 * kBins);
+            const Acc& a = s_b[tt][slot];
+            if (a.c0 == 0) continue;
+            Acc* g = &obins[size_t(t0 + tt) * 3 * kBins + slot];
+            atomicAdd(&g->c0, a.c0);
+            for (int q = 0; q < 3; ++q) {
+                atomicMin(&g->mn[q], a.mn[q]);
+                atomicMax(&g->mx[q], a.mx[q]);
+            }
         }
-    }
 }
 
 // Best object split and whether a spatial split is worth evaluating.
