@@ -122,71 +122,83 @@ __global__ void level_init_kernel(const BTask* tasks, uint32_t num_tasks, BDecis
     }
 }
 
-// Bin every primitive of every task with >= 2 primitives.  Blocks whose
-// primitives all belong to one task accumulate in LDS first.
+// Bin every primitive of every task with >= 2 primitives.  The live task
+// ids along the index array never decrease (the scatter places each split
+// task's children, left then right, inside the parent's range, and tasks are
+// numbered in range order); finished primitives carry -1 and sit between
+// them.  A block whose live primitives span at most kLdsTasks tasks
+// accumulates in LDS and merges one set of atomics per task and bin.
+constexpr int kLdsTasks = 4;
+
 __global__ void __launch_bounds__(kBlock)
     bin_kernel(uint32_t n, const int32_t* ptask, const uint32_t* idx, const BTask* tasks,
                const BDecision* dec, const float* centers, uint32_t stride,
                const DevAabb* aabbs, BinAcc* bins) {
-    __shared__ uint32_t s_cnt[3 * kBins];
-    __shared__ uint32_t s_mn[3 * kBins][3];
-    __shared__ uint32_t s_mx[3 * kBins][3];
-    __shared__ int32_t s_uniform;
+    constexpr int kSlots = kLdsTasks * 3 * kBins;
+    __shared__ uint32_t s_cnt[kSlots];
+    __shared__ uint32_t s_mn[kSlots][3];
+    __shared__ uint32_t s_mx[kSlots][3];
+    __shared__ int32_t s_lo, s_hi;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t first = blockIdx.x * kBlock;
-    const uint32_t last = (first + kBlock < n ? first + kBlock : n) - 1;
     if (threadIdx.x == 0) {
-        const int32_t a = ptask[first];
-        s_uniform = (a >= 0 && ptask[last] == a) ? a : -1;
+        s_lo = 0x7fffffff;
+        s_hi = -1;
     }
-    if (threadIdx.x < 3 * kBins) {
-        s_cnt[threadIdx.x] = 0;
-        for (int k = 0; k < 3; ++k) {
-            s_mn[threadIdx.x][k] = fenc(1e34f);
-            s_mx[threadIdx.x][k] = fenc(-1e34f);
+    for (uint32_t k = threadIdx.x; k < uint32_t(kSlots); k += kBlock) {
+        s_cnt[k] = 0;
+        for (int q = 0; q < 3; ++q) {
+            s_mn[k][q] = fenc(1e34f);
+            s_mx[k][q] = fenc(-1e34f);
         }
     }
     __syncthreads();
-    const int32_t uni = s_uniform;
-    if (i < n) {
-        const int32_t t = ptask[i];
-        if (t >= 0) {
-            const BTask tk = tasks[t];
-            if (tk.end - tk.begin > 1 && tk.depth < kMaxDepth) {
-                const BDecision& d = dec[t];
-                const uint32_t p = idx[i];
-                const float* c = centers + size_t(p) * stride;
-                const DevAabb a = aabbs[p];
-                for (int ax = 0; ax < 3; ++ax) {
-                    const uint32_t b = bin_index(c[ax], d.ctb[ax], d.bo[ax]);
-                    const uint32_t slot = ax * kBins + b;
-                    if (uni >= 0) {
-                        atomicAdd(&s_cnt[slot], 1u);
-                        for (int k = 0; k < 3; ++k) {
-                            if (a.min[k] == a.min[k]) atomicMin(&s_mn[slot][k], fenc(a.min[k]));
-                            if (a.max[k] == a.max[k]) atomicMax(&s_mx[slot][k], fenc(a.max[k]));
-                        }
-                    } else {
-                        BinAcc* g = bins + size_t(t) * 3 * kBins + slot;
-                        atomicAdd(&g->cnt, 1u);
-                        for (int k = 0; k < 3; ++k) {
-                            if (a.min[k] == a.min[k]) atomicMin(&g->mn[k], fenc(a.min[k]));
-                            if (a.max[k] == a.max[k]) atomicMax(&g->mx[k], fenc(a.max[k]));
-                        }
+    const int32_t t = i < n ? ptask[i] : -1;
+    if (t >= 0) {
+        atomicMin(&s_lo, t);
+        atomicMax(&s_hi, t);
+    }
+    __syncthreads();
+    const int32_t lo = s_lo, hi = s_hi;
+    const bool local = hi >= 0 && hi - lo < kLdsTasks;
+    if (t >= 0) {
+        const BTask tk = tasks[t];
+        if (tk.end - tk.begin > 1 && tk.depth < kMaxDepth) {
+            const BDecision& d = dec[t];
+            const uint32_t p = idx[i];
+            const float* c = centers + size_t(p) * stride;
+            const DevAabb a = aabbs[p];
+            for (int ax = 0; ax < 3; ++ax) {
+                const uint32_t b = bin_index(c[ax], d.ctb[ax], d.bo[ax]);
+                const uint32_t slot = ax * kBins + b;
+                if (local) {
+                    const uint32_t ls = uint32_t(t - lo) * 3 * kBins + slot;
+                    atomicAdd(&s_cnt[ls], 1u);
+                    for (int k = 0; k < 3; ++k) {
+                        if (a.min[k] == a.min[k]) atomicMin(&s_mn[ls][k], fenc(a.min[k]));
+                        if (a.max[k] == a.max[k]) atomicMax(&s_mx[ls][k], fenc(a.max[k]));
+                    }
+                } else {
+                    BinAcc* g = bins + size_t(t) * 3 * kBins + slot;
+                    atomicAdd(&g->cnt, 1u);
+                    for (int k = 0; k < 3; ++k) {
+                        if (a.min[k] == a.min[k]) atomicMin(&g->mn[k], fenc(a.min[k]));
+                        if (a.max[k] == a.max[k]) atomicMax(&g->mx[k], fenc(a.max[k]));
                     }
                 }
             }
         }
     }
     __syncthreads();
-    if (uni >= 0 && threadIdx.x < 3 * kBins && s_cnt[threadIdx.x] > 0) {
-        BinAcc* g = bins + size_t(uni) * 3 * kBins + threadIdx.x;
-        atomicAdd(&g->cnt, s_cnt[threadIdx.x]);
-        for (int k = 0; k < 3; ++k) {
-            atomicMin(&g->mn[k], s_mn[threadIdx.x][k]);
-            atomicMax(&g->mx[k], s_mx[threadIdx.x][k]);
+    if (local)
+        for (uint32_t k = threadIdx.x; k < uint32_t(hi - lo + 1) * 3 * kBins; k += kBlock) {
+            if (s_cnt[k] == 0) continue;
+            BinAcc* g = bins + size_t(lo) * 3 * kBins + k;
+            atomicAdd(&g->cnt, s_cnt[k]);
+            for (int q = 0; q < 3; ++q) {
+                atomicMin(&g->mn[q], s_mn[k][q]);
+                atomicMax(&g->mx[q], s_mx[k][q]);
+            }
         }
-    }
 }
 
 struct Box {
@@ -408,32 +420,56 @@ __global__ void point_aabbs_kernel(const float* centers, uint32_t stride, uint32
 
 // Root task: Aabb::union_of_list (binned_sah.rs:361), sequential fold order
 // per block then across blocks.
-__global__ void root_bounds_kernel(const DevAabb* aabbs, uint32_t n, BTask* task) {
+// Root box in two passes: kRootParts blocks read the boxes coalesced
+// (grid stride) and leave one partial box each; one block reduces those.
+// The minimum and maximum ignore NaN as Rust's f32::min/max do and do not
+// depend on the order, up to the sign of a zero bound, which the 0.0001
+// padding below removes.
+constexpr uint32_t kRootParts = 512;
+
+__device__ __forceinline__ void block_box(float (&acc)[6], float (*sh)[6]) {
+    for (int k = 0; k < 6; ++k) sh[threadIdx.x][k] = acc[k];
+    __syncthreads();
+    for (uint32_t h = kBlock / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int k = 0; k < 3; ++k) {
+                sh[threadIdx.x][k] = rs_min(sh[threadIdx.x][k], sh[threadIdx.x + h][k]);
+                sh[threadIdx.x][3 + k] = rs_max(sh[threadIdx.x][3 + k], sh[threadIdx.x + h][3 + k]);
+            }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    root_partial_kernel(const DevAabb* aabbs, uint32_t n, float* parts) {
     __shared__ float sh[kBlock][6];
     float acc[6] = {1e34f, 1e34f, 1e34f, -1e34f, -1e34f, -1e34f};
-    const uint32_t per = (n + kBlock - 1) / kBlock;
-    const uint32_t b = threadIdx.x * per;
-    const uint32_t e = b + per < n ? b + per : n;
-    for (uint32_t i = b; i < e; ++i) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kRootParts * kBlock) {
         const DevAabb a = aabbs[i];
         for (int k = 0; k < 3; ++k) {
             acc[k] = rs_min(acc[k], a.min[k]);
             acc[3 + k] = rs_max(acc[3 + k], a.max[k]);
         }
     }
-    for (int k = 0; k < 6; ++k) sh[threadIdx.x][k] = acc[k];
-    __syncthreads();
+    block_box(acc, sh);
+    if (threadIdx.x < 6) parts[size_t(blockIdx.x) * 6 + threadIdx.x] = sh[0][threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    root_bounds_kernel(const float* parts, uint32_t n, BTask* task) {
+    __shared__ float sh[kBlock][6];
+    float acc[6] = {1e34f, 1e34f, 1e34f, -1e34f, -1e34f, -1e34f};
+    for (uint32_t p = threadIdx.x; p < kRootParts; p += kBlock)
+        for (int k = 0; k < 3; ++k) {
+            acc[k] = rs_min(acc[k], parts[p * 6 + k]);
+            acc[3 + k] = rs_max(acc[3 + k], parts[p * 6 + 3 + k]);
+        }
+    block_box(acc, sh);
     if (threadIdx.x == 0) {
-        float r[6] = {1e34f, 1e34f, 1e34f, -1e34f, -1e34f, -1e34f};
-        for (int t = 0; t < kBlock; ++t)
-            for (int k = 0; k < 3; ++k) {
-                r[k] = rs_min(r[k], sh[t][k]);
-                r[3 + k] = rs_max(r[3 + k], sh[t][3 + k]);
-            }
         BTask tk;
         for (int k = 0; k < 3; ++k) {
-            tk.mn[k] = r[k] - 0.0001f;
-            tk.mx[k] = r[3 + k] + 0.0001f;
+            tk.mn[k] = sh[0][k] - 0.0001f;
+            tk.mx[k] = sh[0][3 + k] + 0.0001f;
         }
         tk.node = 0;
         tk.begin = 0;
@@ -503,6 +539,7 @@ ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t st
     DevBuf<BTask> tasks0, tasks1;
     DevBuf<BDecision> dec;
     DevBuf<BinAcc> bins;
+    DevBuf<float> root_parts;
     uint32_t* host_total = nullptr;
     if ((e = hipHostMalloc(&host_total, 8)) != hipSuccess) return fail(e, "hipHostMalloc");
     struct HostFree {
@@ -512,14 +549,17 @@ ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t st
     if ((e = idx1.ensure(n)) != hipSuccess || (e = flag.ensure(n)) != hipSuccess ||
         (e = lscan.ensure(n)) != hipSuccess || (e = pt0.ensure(n)) != hipSuccess ||
         (e = pt1.ensure(n)) != hipSuccess || (e = tasks0.ensure(1)) != hipSuccess ||
-        (e = scan_scratch.ensure(scan_scratch_bytes(n) / 4 + 1)) != hipSuccess)
+        (e = scan_scratch.ensure(scan_scratch_bytes(n) / 4 + 1)) != hipSuccess ||
+        (e = root_parts.ensure(size_t(kRootParts) * 6)) != hipSuccess)
         return fail(e, "binned workspace");
     uint32_t* idx_cur = res.indices;
     uint32_t* idx_nxt = idx1.p;
     int32_t* pt_cur = pt0.p;
     int32_t* pt_nxt = pt1.p;
     hipLaunchKernelGGL(iota_kernel, dim3(g), dim3(kBlock), 0, stream, idx_cur, pt_cur, n);
-    hipLaunchKernelGGL(root_bounds_kernel, dim3(1), dim3(kBlock), 0, stream, in_aabbs, n,
+    hipLaunchKernelGGL(root_partial_kernel, dim3(kRootParts), dim3(kBlock), 0, stream, in_aabbs, n,
+                       root_parts.p);
+    hipLaunchKernelGGL(root_bounds_kernel, dim3(1), dim3(kBlock), 0, stream, root_parts.p, n,
                        tasks0.p);
     BTask* t_cur = tasks0.p;
     uint32_t num_tasks = 1;
