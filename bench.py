@@ -359,6 +359,24 @@ def run_large(args, dev, rt, scenes, torch):
     return res
 
 
+def run_build_only(kind, rt, torch, a_d, c_d, v_d, reps):
+    """Timed binned-SAH or spatial-split builds of the 1M scene, each from a
+    synchronised device after the previous tree was released."""
+    log(f"{kind} builds")
+    ms, tree = [], None
+    for i in range(reps):
+        tree = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b = rt.Builder(c_d, aabbs=a_d)
+        tree = b.construct_binned_sah() if kind == "binned" else b.construct_spatial_sah(v_d)
+        torch.cuda.synchronize()
+        if i > 0:
+            ms.append((time.perf_counter() - t0) * 1e3)
+    log(f"build only: {kind} {statistics.median(ms):.3f} ms (median of {len(ms)}), "
+        f"{tree.index_count} references, {tree.node_count} nodes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -383,9 +401,10 @@ def main():
                     help="1: also build the spatial-split SAH tree and trace the same rays")
     ap.add_argument("--adjoint", type=int, default=1,
                     help="1: also time forward + adjoint backward on 100k triangles (0 skips it)")
-    ap.add_argument("--build-only", action="store_true",
-                    help="only the timed PLOC builds of the 1M scene, for profiler runs; no JSON "
-                         "line")
+    ap.add_argument("--build-only", nargs="?", const="ploc", default=None,
+                    choices=("ploc", "binned", "spatial"),
+                    help="only the timed builds of the 1M scene with one builder (default PLOC), "
+                         "for profiler runs; no JSON line")
     ap.add_argument("--trace-only", action="store_true",
                     help="build, then only the timed bounce-ray steps of the headline trace "
                          "(and the copy calibration) for profiler runs; no JSON line")
@@ -420,6 +439,10 @@ def main():
     a_d = torch.from_numpy(aabbs).to(dev)
     c_d = torch.from_numpy(centers).to(dev)
     n_tris = len(verts)
+
+    if args.build_only in ("binned", "spatial"):
+        run_build_only(args.build_only, rt, torch, a_d, c_d, v_d, 2 if args.quick else 6)
+        return
 
     # ---- build: PLOC + collapse ------------------------------------------
     log("PLOC builds")

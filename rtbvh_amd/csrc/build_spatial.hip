@@ -436,29 +436,36 @@ __global__ void obj_split_kernel(uint32_t num_tasks, const STask* tasks, SDec* d
     cand_flag[t] = cand;
 }
 
-// Chopped spatial binning of the references of candidate tasks.
+// Chopped spatial binning of the references of candidate tasks.  A
+// reference counts one entry (c0) in its first bin and one exit (c1) in its
+// last, and grows the box of every bin it spans with the triangle clipped to
+// that bin, so a bin can hold a box without counts.  Blocks spanning at most
+// kLdsTasks tasks bin in LDS and merge every touched bin.
+__device__ __forceinline__ bool sp_evaluated(const SDec* dec, const uint32_t* cand_excl, int32_t t) {
+    return dec[t].cand && cand_excl[t] < kMaxCand;
+}
+
 __global__ void __launch_bounds__(kBlock)
     sp_bin_kernel(uint32_t nrefs, const SRef* refs, const SDec* dec, const uint32_t* cand_excl,
                   const float* verts, uint32_t prim_count, Acc* sbins) {
-    __shared__ Acc s_b[3 * kBins];
-    __shared__ int32_t s_uni;
+    __shared__ Acc s_b[kLdsTasks][3 * kBins];
+    __shared__ int32_t s_t0, s_nt;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t first = blockIdx.x * kBlock;
-    const uint32_t last = (first + kBlock < nrefs ? first + kBlock : nrefs) - 1;
     if (threadIdx.x == 0) {
-        const int32_t a = refs[first].task;
-        s_uni = (a >= 0 && refs[last].task == a && dec[a].cand && cand_excl[a] < kMaxCand) ? a
-                                                                                          : -1;
+        int32_t nt = 0;
+        s_t0 = block_tasks(refs, nrefs, &nt);
+        s_nt = nt;
     }
-    if (threadIdx.x < 3 * kBins) acc_clear(s_b[threadIdx.x]);
+    for (uint32_t k = threadIdx.x; k < uint32_t(kLdsTasks * 3 * kBins); k += kBlock)
+        acc_clear(s_b[k / (3 * kBins)][k % (3 * kBins)]);
     __syncthreads();
-    const int32_t uni = s_uni;
+    const int32_t t0 = s_t0;
     if (i < nrefs) {
         const SRef r = refs[i];
         const int32_t t = r.task;
-        if (t >= 0 && dec[t].cand && cand_excl[t] < kMaxCand && r.prim < prim_count) {
+        if (t >= 0 && sp_evaluated(dec, cand_excl, t) && r.prim < prim_count) {
             const SDec& d = dec[t];
-            Acc* base = uni >= 0 ? s_b : sbins + size_t(cand_excl[t]) * 3 * kBins;
+            Acc* base = t0 >= 0 ? s_b[t - t0] : sbins + size_t(cand_excl[t]) * 3 * kBins;
             const float* v = verts + size_t(r.prim) * 9;
             for (int ax = 0; ax < 3; ++ax) {
                 const uint32_t b0 = bin_of(r.mn[ax], d.org[ax], d.inv[ax]);
@@ -489,18 +496,23 @@ __global__ void __launch_bounds__(kBlock)
         }
     }
     __syncthreads();
-    if (uni >= 0 && threadIdx.x < 3 * kBins) {
-        const Acc& s = s_b[threadIdx.x];
-        if (s.c0 > 0 || s.c1 > 0) {
-            Acc* g = &sbins[size_t(cand_excl[uni]) * 3 * kBins + threadIdx.x];
-            atomicAdd(&g->c0, s.c0);
-            atomicAdd(&g->c1, s.c1);
-            for (int k = 0; k < 3; ++k) {
-                atomicMin(&g->mn[k], s.mn[k]);
-                atomicMax(&g->mx[k], s.mx[k]);
+    if (t0 >= 0)
+        for (uint32_t k = threadIdx.x; k < uint32_t(s_nt * 3 * kBins); k += kBlock) {
+            const uint32_t tt = k / (3 * kBins), slot = k % (3 * kBins);
+            const int32_t t = t0 + int32_t(tt);
+            if (!sp_evaluated(dec, cand_excl, t)) continue;
+            const Acc& a = s_b[tt][slot];
+            const uint32_t empty = fenc(1e34f);
+            if ((a.c0 | a.c1) == 0 && a.mn[0] == empty && a.mn[1] == empty && a.mn[2] == empty)
+                continue;
+            Acc* g = &sbins[size_t(cand_excl[t]) * 3 * kBins + slot];
+            if (a.c0) atomicAdd(&g->c0, a.c0);
+            if (a.c1) atomicAdd(&g->c1, a.c1);
+            for (int q = 0; q < 3; ++q) {
+                atomicMin(&g->mn[q], a.mn[q]);
+                atomicMax(&g->mx[q], a.mx[q]);
             }
         }
-    }
 }
 
 // Best spatial split of each evaluated candidate; dup[t] = its duplicates
