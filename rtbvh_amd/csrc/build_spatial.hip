@@ -242,7 +242,8 @@ __global__ void cand_clear_kernel(uint32_t count, Acc* sbins) {
 // a block's references belong to the tasks refs[first].task ..
 // refs[last].task.  When those are at most kLdsTasks, the block reduces in
 // LDS and merges one value per task and slot; returns the first task, or -1
-// for the global-atomic path.
+// for the global-atomic path.  A reference whose task lies outside the run
+// (never, by the ordering) takes the global path as well (in_run).
 constexpr int kLdsTasks = 16;  // 4: 41 ms spatial, 22 ms binned at 1M; 16: 36, 19.5; 32: no better
 
 __device__ __forceinline__ int32_t block_tasks(const SRef* refs, uint32_t nrefs, int32_t* count) {
@@ -251,6 +252,10 @@ __device__ __forceinline__ int32_t block_tasks(const SRef* refs, uint32_t nrefs,
     const int32_t a = refs[first].task, b = refs[last].task;
     *count = b - a + 1;
     return (a >= 0 && b >= a && b - a < kLdsTasks) ? a : -1;
+}
+
+__device__ __forceinline__ bool in_run(int32_t t, int32_t t0, int32_t nt) {
+    return t0 >= 0 && uint32_t(t - t0) < uint32_t(nt);
 }
 
 // Node boxes: the union of each task's reference boxes.
@@ -272,10 +277,11 @@ __global__ void __launch_bounds__(kBlock)
     const int32_t t0 = s_t0;
     if (i < nrefs) {
         const SRef r = refs[i];
+        const bool lds = in_run(r.task, t0, s_nt);
         if (r.task >= 0)
             for (int k = 0; k < 3; ++k) {
                 if (!(r.mn[k] <= r.mx[k])) continue;  // empty or NaN
-                if (t0 >= 0) {
+                if (lds) {
                     atomicMin(&s_mn[r.task - t0][k], fenc(r.mn[k]));
                     atomicMax(&s_mx[r.task - t0][k], fenc(r.mx[k]));
                 } else {
@@ -347,7 +353,8 @@ __global__ void __launch_bounds__(kBlock)
                 for (int ax = 0; ax < 3; ++ax) {
                     const float c = 0.5f * (r.mn[ax] + r.mx[ax]);
                     const uint32_t slot = ax * kBins + bin_of(c, d.org[ax], d.inv[ax]);
-                    Acc* g = t0 >= 0 ? &s_b[t - t0][slot] : &obins[size_t(t) * 3 * kBins + slot];
+                    Acc* g = in_run(t, t0, s_nt) ? &s_b[t - t0][slot]
+                                                 : &obins[size_t(t) * 3 * kBins + slot];
                     atomicAdd(&g->c0, 1u);
                     for (int k = 0; k < 3; ++k) {
                         if (!(r.mn[k] <= r.mx[k])) continue;
@@ -465,7 +472,8 @@ __global__ void __launch_bounds__(kBlock)
         const int32_t t = r.task;
         if (t >= 0 && sp_evaluated(dec, cand_excl, t) && r.prim < prim_count) {
             const SDec& d = dec[t];
-            Acc* base = t0 >= 0 ? s_b[t - t0] : sbins + size_t(cand_excl[t]) * 3 * kBins;
+            Acc* base = in_run(t, t0, s_nt) ? s_b[t - t0]
+                                            : sbins + size_t(cand_excl[t]) * 3 * kBins;
             const float* v = verts + size_t(r.prim) * 9;
             for (int ax = 0; ax < 3; ++ax) {
                 const uint32_t b0 = bin_of(r.mn[ax], d.org[ax], d.inv[ax]);
