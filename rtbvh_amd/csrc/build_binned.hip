@@ -20,6 +20,8 @@
 // inside multi-primitive leaves may differ; the sets, the topology and the
 // bounds do not.  Nodes are numbered breadth-first (the reference's numbering
 // depends on thread timing, builders/mod.rs:59-76).
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace rtbvh {
@@ -565,13 +567,22 @@ ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t st
     uint32_t num_tasks = 1;
     uint32_t counter = 1;  // AtomicNodeStack starts at 1 (builders/mod.rs:51-56)
     int flip = 0;
+    // Per-task buffers hold tcap tasks: at least 65536 (at most n, as every
+    // task has a primitive), doubled when a level needs more, so the levels
+    // near the root, whose task count doubles, do not free and allocate
+    // every buffer each time (a hipFree and a hipMalloc cost more than the
+    // level's kernels there).
+    size_t tcap = 0;
     while (num_tasks > 0) {
-        if ((e = dec.ensure(num_tasks)) != hipSuccess ||
-            (e = bins.ensure(size_t(num_tasks) * 3 * kBins)) != hipSuccess ||
-            (e = split_flag.ensure(num_tasks)) != hipSuccess)
+        if (num_tasks > tcap)
+            tcap = std::max<size_t>(
+                num_tasks, std::min<size_t>(n, std::max<size_t>(65536, 2 * size_t(num_tasks))));
+        if ((e = dec.ensure(tcap)) != hipSuccess ||
+            (e = bins.ensure(tcap * 3 * kBins)) != hipSuccess ||
+            (e = split_flag.ensure(tcap)) != hipSuccess)
             return fail(e, "binned level workspace");
         DevBuf<BTask>& nxt_buf = flip ? tasks0 : tasks1;
-        if ((e = nxt_buf.ensure(size_t(2) * num_tasks)) != hipSuccess)
+        if ((e = nxt_buf.ensure(2 * tcap)) != hipSuccess)
             return fail(e, "binned tasks");
         BTask* t_nxt = nxt_buf.p;
         const uint32_t tg = (num_tasks + kBlock - 1) / kBlock;

@@ -448,7 +448,8 @@ __global__ void obj_split_kernel(uint32_t num_tasks, const STask* tasks, SDec* d
 // last, and grows the box of every bin it spans with the triangle clipped to
 // that bin, so a bin can hold a box without counts.  Blocks spanning at most
 // kLdsTasks tasks bin in LDS and merge every touched bin.
-__device__ __forceinline__ bool sp_evaluated(const SDec* dec, const uint32_t* cand_excl, int32_t t) {
+__device__ __forceinline__ bool sp_evaluated(const SDec* dec, const uint32_t* cand_excl,
+                                             int32_t t) {
     return dec[t].cand && cand_excl[t] < kMaxCand;
 }
 
@@ -870,20 +871,27 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
     uint32_t nrefs = n, num_tasks = 1, counter = 1, leaf_base = 0, dups = 0;
     const uint32_t budget = cap - n;
     int flip = 0;
+    // Per-task buffers hold tcap tasks: at least 65536 (at most cap, as every
+    // task has a reference), doubled when a level needs more, so the levels
+    // near the root, whose task count doubles, do not free and allocate
+    // every buffer each time.
+    size_t tcap = 0;
     for (uint32_t level = 0; num_tasks > 0; ++level) {
         if (level > kMaxDepth + 1) {
             set_error("spatial builder did not finish in %u levels", kMaxDepth + 1);
             return fail_msg();
         }
-        if ((e = dec.ensure(num_tasks)) != hipSuccess || (e = tbox.ensure(num_tasks)) != hipSuccess ||
-            (e = obins.ensure(size_t(num_tasks) * 3 * kBins)) != hipSuccess ||
-            (e = split_flag.ensure(num_tasks)) != hipSuccess ||
-            (e = out_cnt.ensure(num_tasks)) != hipSuccess ||
-            (e = leaf_cnt.ensure(num_tasks)) != hipSuccess ||
-            (e = cand.ensure(num_tasks)) != hipSuccess || (e = dup.ensure(num_tasks)) != hipSuccess)
+        if (num_tasks > tcap)
+            tcap = std::max<size_t>(
+                num_tasks, std::min<size_t>(cap, std::max<size_t>(65536, 2 * size_t(num_tasks))));
+        if ((e = dec.ensure(tcap)) != hipSuccess || (e = tbox.ensure(tcap)) != hipSuccess ||
+            (e = obins.ensure(tcap * 3 * kBins)) != hipSuccess ||
+            (e = split_flag.ensure(tcap)) != hipSuccess ||
+            (e = out_cnt.ensure(tcap)) != hipSuccess || (e = leaf_cnt.ensure(tcap)) != hipSuccess ||
+            (e = cand.ensure(tcap)) != hipSuccess || (e = dup.ensure(tcap)) != hipSuccess)
             return fail(e, "spatial level workspace");
         Buf<STask>& nb = flip ? t0 : t1;
-        if ((e = nb.ensure(size_t(2) * num_tasks)) != hipSuccess) return fail(e, "spatial tasks");
+        if ((e = nb.ensure(2 * tcap)) != hipSuccess) return fail(e, "spatial tasks");
         STask* t_nxt = nb.p;
         const uint32_t tg = (num_tasks + kBlock - 1) / kBlock;
         const uint32_t rg = (nrefs + kBlock - 1) / kBlock;
