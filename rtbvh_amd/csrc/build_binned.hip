@@ -203,64 +203,125 @@ __global__ void __launch_bounds__(kBlock)
         }
 }
 
-struct Box {
+// A run of bins: its box and its primitive count.
+struct GBox {
     float mn[3], mx[3];
+    uint32_t cnt;
 };
 
-__device__ __forceinline__ void box_empty(Box& b) {
+// lo := lo joined with hi, where hi holds higher bins.  The minimum keeps
+// lo's bound unless hi's is strictly lower (rs_min/rs_max keep the first
+// argument on equal values), which is what a fold in increasing bin order
+// gives: joins in any grouping return the same bits.
+__device__ __forceinline__ void gbox_join(GBox& lo, const GBox& hi) {
     for (int k = 0; k < 3; ++k) {
-        b.mn[k] = 1e34f;
-        b.mx[k] = -1e34f;
+        lo.mn[k] = rs_min(lo.mn[k], hi.mn[k]);
+        lo.mx[k] = rs_max(lo.mx[k], hi.mx[k]);
     }
+    lo.cnt += hi.cnt;
 }
 
-__device__ __forceinline__ void box_grow(Box& b, const BinAcc& a) {
-    // grow_bb with the bin's box (src/aabb.rs:264-273)
+__device__ __forceinline__ GBox gbox_shfl_up(const GBox& b, int off) {
+    GBox r;
     for (int k = 0; k < 3; ++k) {
-        b.mn[k] = rs_min(b.mn[k], fdec(a.mn[k]));
-        b.mx[k] = rs_max(b.mx[k], fdec(a.mx[k]));
+        r.mn[k] = __shfl_up(b.mn[k], off, kBins);
+        r.mx[k] = __shfl_up(b.mx[k], off, kBins);
     }
+    r.cnt = __shfl_up(b.cnt, off, kBins);
+    return r;
 }
 
-// find_split, binned_sah.rs:80-114
-__device__ void find_split(const BinAcc* bins, float& best_cost, uint32_t& best_count) {
-    float right_cost[kBins];
-    Box cur;
-    box_empty(cur);
-    uint64_t cc = 0;
-    for (int i = kBins - 1; i > 0; --i) {
-        box_grow(cur, bins[i]);
-        cc += bins[i].cnt;
-        right_cost[i] = half_area6(cur.mn, cur.mx) * float(cc);
+__device__ __forceinline__ GBox gbox_shfl_down(const GBox& b, int off) {
+    GBox r;
+    for (int k = 0; k < 3; ++k) {
+        r.mn[k] = __shfl_down(b.mn[k], off, kBins);
+        r.mx[k] = __shfl_down(b.mx[k], off, kBins);
     }
-    box_empty(cur);
-    cc = 0;
-    best_cost = 3.40282347e38f;
-    best_count = kBins;
-    for (int i = 0; i < kBins - 1; ++i) {
-        box_grow(cur, bins[i]);
-        cc += bins[i].cnt;
-        const float cost = half_area6(cur.mn, cur.mx) * float(cc) + right_cost[i + 1];
-        if (cost < best_cost) {
-            best_cost = cost;
-            best_count = uint32_t(i + 1);
-        }
-    }
+    r.cnt = __shfl_down(b.cnt, off, kBins);
+    return r;
 }
 
-__global__ void split_kernel(const BTask* tasks, uint32_t num_tasks, BDecision* dec,
-                             const BinAcc* bins, uint32_t max_leaf, uint32_t* split_flag) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= num_tasks) return;
+__device__ __forceinline__ GBox gbox_shfl(const GBox& b, int src) {
+    GBox r;
+    for (int k = 0; k < 3; ++k) {
+        r.mn[k] = __shfl(b.mn[k], src, kBins);
+        r.mx[k] = __shfl(b.mx[k], src, kBins);
+    }
+    r.cnt = __shfl(b.cnt, src, kBins);
+    return r;
+}
+
+__device__ __forceinline__ float gbox_half_area(const GBox& b) { return half_area6(b.mn, b.mx); }
+
+// Per task: the best SAH plane per axis (find_split, binned_sah.rs:80-114),
+// the axis, the fallback and the child boxes (:150-230).  One group of 16
+// lanes per task, lane i holding bin i: inclusive prefix (bins 0..i) and
+// suffix (bins i..15) runs by shuffle scans, the cost of the plane after bin
+// i as the one-thread sweep computes it, half_area(left) * left count +
+// half_area(right) * right count, and the first lowest cost below FLT_MAX
+// by a reduction.  The bins' bounds hold no NaN (the binning skips NaN
+// bounds), so every bound equals the sequential fold's, up to the sign of a
+// zero in the right runs of the sweep, which no comparison tells apart; the
+// child boxes come from increasing-order runs, as the sweep builds them.
+__global__ void __launch_bounds__(kBlock)
+    split_kernel(const BTask* tasks, uint32_t num_tasks, BDecision* dec, const BinAcc* bins,
+                 uint32_t max_leaf, uint32_t* split_flag) {
+    const uint32_t t = (blockIdx.x * kBlock + threadIdx.x) / kBins;
+    const int lane = int(threadIdx.x % kBins);
+    if (t >= num_tasks) return;  // whole groups: kBlock is a multiple of kBins
     const BTask tk = tasks[t];
     BDecision d = dec[t];
     const uint32_t work = tk.end - tk.begin;
     d.split = 0;
     if (work > 1 && tk.depth < kMaxDepth) {
         const BinAcc* tb = bins + size_t(t) * 3 * kBins;
+        GBox pre[3], suf[3];
         float cost[3];
         uint32_t cnt[3];
-        for (int ax = 0; ax < 3; ++ax) find_split(tb + ax * kBins, cost[ax], cnt[ax]);
+        for (int ax = 0; ax < 3; ++ax) {
+            const BinAcc a = tb[ax * kBins + lane];
+            GBox own;
+            for (int k = 0; k < 3; ++k) {
+                own.mn[k] = fdec(a.mn[k]);
+                own.mx[k] = fdec(a.mx[k]);
+            }
+            own.cnt = a.cnt;
+            GBox p = own, q = own;
+            for (int off = 1; off < kBins; off <<= 1) {
+                GBox lower = gbox_shfl_up(p, off);
+                const GBox higher = gbox_shfl_down(q, off);
+                if (lane >= off) {
+                    gbox_join(lower, p);
+                    p = lower;
+                }
+                if (lane + off < kBins) gbox_join(q, higher);
+            }
+            pre[ax] = p;
+            suf[ax] = q;
+            const GBox right = gbox_shfl_down(q, 1);
+            float key = __builtin_inff();
+            if (lane < kBins - 1) {
+                const float c = gbox_half_area(p) * float(p.cnt) +
+                                gbox_half_area(right) * float(right.cnt);
+                if (c < 3.40282347e38f) key = c;
+            }
+            int idx = lane;
+            for (int off = kBins / 2; off > 0; off >>= 1) {
+                const float ok = __shfl_xor(key, off, kBins);
+                const int oi = __shfl_xor(idx, off, kBins);
+                if (ok < key || (ok == key && oi < idx)) {
+                    key = ok;
+                    idx = oi;
+                }
+            }
+            if (key == __builtin_inff()) {
+                cost[ax] = 3.40282347e38f;
+                cnt[ax] = kBins;
+            } else {
+                cost[ax] = key;
+                cnt[ax] = uint32_t(idx + 1);
+            }
+        }
         uint32_t axis = 0;
         if (cost[0] > cost[1]) axis = 1;
         if (cost[axis] > cost[2]) axis = 2;
@@ -273,28 +334,27 @@ __global__ void split_kernel(const BTask* tasks, uint32_t num_tasks, BDecision* 
                 axis = 0;
                 if (d.mx[1] - d.mn[1] > d.mx[0] - d.mn[0]) axis = 1;
                 if (d.mx[2] - d.mn[2] > d.mx[axis] - d.mn[axis]) axis = 2;
-                uint64_t c = 0;
-                for (int i = 0; i < kBins - 1; ++i) {
-                    c += tb[axis * kBins + i].cnt;
-                    if (c >= uint64_t(work * 2 / 5 + 1)) {
-                        split_index = uint32_t(i + 1);
-                        break;
-                    }
-                }
+                // the first bin whose prefix count reaches 2/5 of the work
+                const bool reach =
+                    lane < kBins - 1 && uint64_t(pre[axis].cnt) >= uint64_t(work * 2 / 5 + 1);
+                const uint32_t base = threadIdx.x & 63u & ~uint32_t(kBins - 1);  // group's lane 0
+                const uint32_t m = uint32_t(__ballot(reach) >> base) & 0xFFFFu;
+                if (m) split_index = uint32_t(__builtin_ctz(m) + 1);
             } else {
                 make_leaf = true;
             }
         }
         if (!make_leaf) {
-            uint32_t left = 0;
             const uint32_t lim = split_index < uint32_t(kBins) ? split_index : uint32_t(kBins);
-            for (uint32_t i = 0; i < lim; ++i) left += tb[axis * kBins + i].cnt;
+            const GBox lb = gbox_shfl(pre[axis], lim > 0 ? int(lim) - 1 : 0);
+            GBox rb = gbox_shfl(suf[axis], lim < uint32_t(kBins) ? int(lim) : 0);
+            const uint32_t left = lim > 0 ? lb.cnt : 0u;
+            if (lim >= uint32_t(kBins))
+                for (int k = 0; k < 3; ++k) {
+                    rb.mn[k] = 1e34f;  // Aabb::empty
+                    rb.mx[k] = -1e34f;
+                }
             if (left > 0 && left < work) {
-                Box lb, rb;
-                box_empty(lb);
-                box_empty(rb);
-                for (uint32_t i = 0; i < lim; ++i) box_grow(lb, tb[axis * kBins + i]);
-                for (uint32_t i = lim; i < uint32_t(kBins); ++i) box_grow(rb, tb[axis * kBins + i]);
                 d.split = 1;
                 d.axis = axis;
                 d.split_index = split_index;
@@ -308,8 +368,10 @@ __global__ void split_kernel(const BTask* tasks, uint32_t num_tasks, BDecision* 
             }
         }
     }
-    dec[t] = d;
-    split_flag[t] = d.split;
+    if (lane == 0) {
+        dec[t] = d;
+        split_flag[t] = d.split;
+    }
 }
 
 // Writes each task's node (leaf or interior) and the next level's tasks.
@@ -590,7 +652,8 @@ ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t st
                            num_tasks, dec.p, bins.p);
         hipLaunchKernelGGL(bin_kernel, dim3(g), dim3(kBlock), 0, stream, n, pt_cur, idx_cur, t_cur,
                            dec.p, centers, stride, in_aabbs, bins.p);
-        hipLaunchKernelGGL(split_kernel, dim3(tg), dim3(kBlock), 0, stream, t_cur, num_tasks, dec.p,
+        hipLaunchKernelGGL(split_kernel, dim3((num_tasks + kBlock / kBins - 1) / (kBlock / kBins)),
+                           dim3(kBlock), 0, stream, t_cur, num_tasks, dec.p,
                            bins.p, leaf, split_flag.p);
         if ((e = scan_exclusive_u32(split_flag.p, num_tasks, host_total, scan_scratch.p,
                                     scan_scratch.cap * 4, stream)) != hipSuccess)
