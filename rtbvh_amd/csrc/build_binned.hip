@@ -35,6 +35,9 @@ namespace {
 constexpr int kBins = 16;          // binned_sah.rs:314
 constexpr uint32_t kMaxDepth = 64;  // binned_sah.rs:313
 constexpr int kBlock = 256;
+// Tasks of at most kSmallTask primitives are binned by their split group
+// (one lane per primitive) instead of by bin_kernel's atomics.
+constexpr uint32_t kSmallTask = 16;
 
 struct BTask {
     float mn[3];
@@ -112,7 +115,7 @@ __global__ void level_init_kernel(const BTask* tasks, uint32_t num_tasks, BDecis
         d.left_count = 0;
         dec[t] = d;
     }
-    if (threadIdx.x < 3 * kBins) {
+    if (threadIdx.x < 3 * kBins && tk.end - tk.begin > kSmallTask) {
         BinAcc a;
         a.cnt = 0;
         for (int k = 0; k < 3; ++k) {
@@ -124,7 +127,8 @@ __global__ void level_init_kernel(const BTask* tasks, uint32_t num_tasks, BDecis
     }
 }
 
-// Bin every primitive of every task with >= 2 primitives.  The live task
+// Bin every primitive of every task with more than kSmallTask primitives
+// (smaller tasks with two or more are binned in split_kernel).  The live task
 // ids along the index array never decrease (the scatter places each split
 // task's children, left then right, inside the parent's range, and tasks are
 // numbered in range order); finished primitives carry -1 and sit between
@@ -164,7 +168,7 @@ __global__ void __launch_bounds__(kBlock)
     const bool local = hi >= 0 && hi - lo < kLdsTasks;
     if (t >= 0) {
         const BTask tk = tasks[t];
-        if (tk.end - tk.begin > 1 && tk.depth < kMaxDepth) {
+        if (tk.end - tk.begin > kSmallTask && tk.depth < kMaxDepth) {
             const BDecision& d = dec[t];
             const uint32_t p = idx[i];
             const float* c = centers + size_t(p) * stride;
@@ -265,7 +269,8 @@ __device__ __forceinline__ float gbox_half_area(const GBox& b) { return half_are
 // child boxes come from increasing-order runs, as the sweep builds them.
 __global__ void __launch_bounds__(kBlock)
     split_kernel(const BTask* tasks, uint32_t num_tasks, BDecision* dec, const BinAcc* bins,
-                 uint32_t max_leaf, uint32_t* split_flag) {
+                 uint32_t max_leaf, uint32_t* split_flag, const uint32_t* idx,
+                 const float* centers, uint32_t stride, const DevAabb* aabbs) {
     const uint32_t t = (blockIdx.x * kBlock + threadIdx.x) / kBins;
     const int lane = int(threadIdx.x % kBins);
     if (t >= num_tasks) return;  // whole groups: kBlock is a multiple of kBins
@@ -275,11 +280,55 @@ __global__ void __launch_bounds__(kBlock)
     d.split = 0;
     if (work > 1 && tk.depth < kMaxDepth) {
         const BinAcc* tb = bins + size_t(t) * 3 * kBins;
+        BinAcc sb[3];  // this lane's bin of each axis, for a small task
+        if (work <= kSmallTask) {
+            // bin_kernel's accumulation without atomics: lane q holds
+            // primitive q, lane j gathers bin j; the minimum and maximum of
+            // the same ordered encodings, NaN bounds skipped
+            uint32_t pb[3] = {0, 0, 0}, pmn[3], pmx[3];
+            for (int k = 0; k < 3; ++k) {
+                pmn[k] = fenc(1e34f);
+                pmx[k] = fenc(-1e34f);
+            }
+            if (uint32_t(lane) < work) {
+                const uint32_t p = idx[tk.begin + lane];
+                const float* c = centers + size_t(p) * stride;
+                const DevAabb a = aabbs[p];
+                for (int k = 0; k < 3; ++k) {
+                    pb[k] = bin_index(c[k], d.ctb[k], d.bo[k]);
+                    if (a.min[k] == a.min[k]) pmn[k] = fenc(a.min[k]);
+                    if (a.max[k] == a.max[k]) pmx[k] = fenc(a.max[k]);
+                }
+            }
+            for (int ax = 0; ax < 3; ++ax) {
+                sb[ax].cnt = 0;
+                for (int k = 0; k < 3; ++k) {
+                    sb[ax].mn[k] = fenc(1e34f);
+                    sb[ax].mx[k] = fenc(-1e34f);
+                }
+            }
+            for (uint32_t q = 0; q < work; ++q) {
+                uint32_t qb[3], qmn[3], qmx[3];
+                for (int k = 0; k < 3; ++k) {
+                    qb[k] = __shfl(pb[k], int(q), kBins);
+                    qmn[k] = __shfl(pmn[k], int(q), kBins);
+                    qmx[k] = __shfl(pmx[k], int(q), kBins);
+                }
+                for (int ax = 0; ax < 3; ++ax)
+                    if (qb[ax] == uint32_t(lane)) {
+                        ++sb[ax].cnt;
+                        for (int k = 0; k < 3; ++k) {
+                            sb[ax].mn[k] = min(sb[ax].mn[k], qmn[k]);
+                            sb[ax].mx[k] = max(sb[ax].mx[k], qmx[k]);
+                        }
+                    }
+            }
+        }
         GBox pre[3], suf[3];
         float cost[3];
         uint32_t cnt[3];
         for (int ax = 0; ax < 3; ++ax) {
-            const BinAcc a = tb[ax * kBins + lane];
+            const BinAcc a = work <= kSmallTask ? sb[ax] : tb[ax * kBins + lane];
             GBox own;
             for (int k = 0; k < 3; ++k) {
                 own.mn[k] = fdec(a.mn[k]);
@@ -653,8 +702,8 @@ ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t st
         hipLaunchKernelGGL(bin_kernel, dim3(g), dim3(kBlock), 0, stream, n, pt_cur, idx_cur, t_cur,
                            dec.p, centers, stride, in_aabbs, bins.p);
         hipLaunchKernelGGL(split_kernel, dim3((num_tasks + kBlock / kBins - 1) / (kBlock / kBins)),
-                           dim3(kBlock), 0, stream, t_cur, num_tasks, dec.p,
-                           bins.p, leaf, split_flag.p);
+                           dim3(kBlock), 0, stream, t_cur, num_tasks, dec.p, bins.p, leaf,
+                           split_flag.p, idx_cur, centers, stride, in_aabbs);
         if ((e = scan_exclusive_u32(split_flag.p, num_tasks, host_total, scan_scratch.p,
                                     scan_scratch.cap * 4, stream)) != hipSuccess)
             return fail(e, "binned task scan");
