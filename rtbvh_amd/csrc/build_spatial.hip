@@ -52,6 +52,9 @@ constexpr float kSplitFactor = 0.75f;
 constexpr float kAlpha = 1e-5f;
 constexpr int kBlock = 256;
 constexpr uint32_t kMaxCand = 65536;  // spatial candidates evaluated per level
+// Tasks of at most kSmallTask references get their object bins from their
+// split group (one lane per reference) instead of obj_bin_kernel's atomics.
+constexpr uint32_t kSmallTask = 16;
 
 // A reference: the part of triangle `prim` inside [mn, mx]; `task` is the
 // index of the current level's task that holds it.
@@ -217,12 +220,13 @@ __global__ void point_refs_kernel(const float* centers, uint32_t stride, uint32_
     refs[i] = r;
 }
 
-// Per task: node box and object bins cleared (block t, 64 threads).
-__global__ void task_clear_kernel(uint32_t num_tasks, Acc* tbox, Acc* obins) {
+// Per task: node box and object bins cleared (block t, 64 threads; small
+// tasks' bins are written whole by obj_split_kernel).
+__global__ void task_clear_kernel(uint32_t num_tasks, const STask* tasks, Acc* tbox, Acc* obins) {
     const uint32_t t = blockIdx.x;
     if (t >= num_tasks) return;
     if (threadIdx.x == 0) acc_clear(tbox[t]);
-    if (threadIdx.x < 3 * kBins) {
+    if (threadIdx.x < 3 * kBins && tasks[t].end - tasks[t].begin > kSmallTask) {
         Acc a;
         acc_clear(a);
         obins[size_t(t) * 3 * kBins + threadIdx.x] = a;
@@ -348,7 +352,7 @@ __global__ void __launch_bounds__(kBlock)
         const int32_t t = r.task;
         if (t >= 0) {
             const STask tk = tasks[t];
-            if (tk.end - tk.begin > 1 && tk.depth < kMaxDepth) {
+            if (tk.end - tk.begin > kSmallTask && tk.depth < kMaxDepth) {
                 const SDec& d = dec[t];
                 for (int ax = 0; ax < 3; ++ax) {
                     const float c = 0.5f * (r.mn[ax] + r.mx[ax]);
@@ -380,54 +384,166 @@ __global__ void __launch_bounds__(kBlock)
         }
 }
 
-// Best object split and whether a spatial split is worth evaluating.
-__global__ void obj_split_kernel(uint32_t num_tasks, const STask* tasks, SDec* dec,
-                                 const Acc* obins, const float* root_half_area,
-                                 uint32_t* cand_flag) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= num_tasks) return;
+// A run of bins: its box and its reference count.
+struct GRun {
+    float mn[3], mx[3];
+    uint32_t cnt;
+};
+
+__device__ __forceinline__ void grun_join(GRun& a, const GRun& b) {
+    for (int k = 0; k < 3; ++k) {
+        a.mn[k] = fminf(a.mn[k], b.mn[k]);
+        a.mx[k] = fmaxf(a.mx[k], b.mx[k]);
+    }
+    a.cnt += b.cnt;
+}
+
+__device__ __forceinline__ GRun grun_shfl(const GRun& b, int src) {
+    GRun r;
+    for (int k = 0; k < 3; ++k) {
+        r.mn[k] = __shfl(b.mn[k], src, kBins);
+        r.mx[k] = __shfl(b.mx[k], src, kBins);
+    }
+    r.cnt = __shfl(b.cnt, src, kBins);
+    return r;
+}
+
+__device__ __forceinline__ GRun grun_shfl_up(const GRun& b, int off) {
+    GRun r;
+    for (int k = 0; k < 3; ++k) {
+        r.mn[k] = __shfl_up(b.mn[k], off, kBins);
+        r.mx[k] = __shfl_up(b.mx[k], off, kBins);
+    }
+    r.cnt = __shfl_up(b.cnt, off, kBins);
+    return r;
+}
+
+__device__ __forceinline__ GRun grun_shfl_down(const GRun& b, int off) {
+    GRun r;
+    for (int k = 0; k < 3; ++k) {
+        r.mn[k] = __shfl_down(b.mn[k], off, kBins);
+        r.mx[k] = __shfl_down(b.mx[k], off, kBins);
+    }
+    r.cnt = __shfl_down(b.cnt, off, kBins);
+    return r;
+}
+
+__device__ __forceinline__ GRun grun_pick(const GRun (&r)[3], uint32_t ax) {
+    return ax == 0 ? r[0] : (ax == 1 ? r[1] : r[2]);
+}
+
+// Best object split and whether a spatial split is worth evaluating.  One
+// group of 16 lanes per task, lane i holding object bin i of each axis:
+// prefix runs (bins 0..i) and suffix runs (i..15) by shuffle scans, the cost
+// of the plane after bin i as a sweep computes it, half_area(left) * left
+// count + half_area(right) * right count, skipping planes with an empty
+// side, and the lowest cost below the node's current one, the first in
+// (axis, plane) order, by a reduction.  Bin bounds hold no NaN, so the runs
+// equal a sweep's up to the sign of a zero, which no comparison tells apart.
+// A task of at most kSmallTask references is binned here (lane q takes
+// reference q, lane j gathers bin j with obj_bin_kernel's ordered-encoding
+// minimum and maximum) and its bins are stored for decide_kernel.
+__global__ void __launch_bounds__(kBlock)
+    obj_split_kernel(uint32_t num_tasks, const STask* tasks, SDec* dec, Acc* obins,
+                     const SRef* refs, const float* root_half_area, uint32_t* cand_flag) {
+    const uint32_t t = (blockIdx.x * kBlock + threadIdx.x) / kBins;
+    const int lane = int(threadIdx.x % kBins);
+    if (t >= num_tasks) return;  // whole groups: kBlock is a multiple of kBins
     const STask tk = tasks[t];
     SDec d = dec[t];
     const uint32_t work = tk.end - tk.begin;
     uint32_t cand = 0;
     if (work > 1 && tk.depth < kMaxDepth) {
-        const Acc* tb = obins + size_t(t) * 3 * kBins;
-        Box lbest, rbest;
-        box_empty(lbest);
-        box_empty(rbest);
-        for (int ax = 0; ax < 3; ++ax) {
-            const Acc* b = tb + ax * kBins;
-            float rcost[kBins];
-            Box rb[kBins];
-            Box cur;
-            box_empty(cur);
-            uint32_t cnt = 0;
-            for (int i = kBins - 1; i > 0; --i) {
-                box_grow_acc(cur, b[i]);
-                cnt += b[i].c0;
-                rcost[i] = half_area6(cur.mn, cur.mx) * float(cnt);
-                rb[i] = cur;
+        Acc* tb = obins + size_t(t) * 3 * kBins;
+        Acc own[3];
+        if (work <= kSmallTask) {
+            uint32_t pb[3] = {0, 0, 0}, pmn[3], pmx[3];
+            for (int k = 0; k < 3; ++k) {
+                pmn[k] = fenc(1e34f);
+                pmx[k] = fenc(-1e34f);
             }
-            box_empty(cur);
-            cnt = 0;
-            for (int i = 0; i < kBins - 1; ++i) {
-                box_grow_acc(cur, b[i]);
-                cnt += b[i].c0;
-                if (cnt == 0 || cnt == work) continue;
-                const float cost = half_area6(cur.mn, cur.mx) * float(cnt) + rcost[i + 1];
-                if (cost < d.cost) {
-                    d.cost = cost;
-                    d.kind = kObject;
-                    d.axis = uint32_t(ax);
-                    d.k = uint32_t(i + 1);
-                    d.left = cnt;
-                    d.right = work - cnt;
-                    lbest = cur;
-                    rbest = rb[i + 1];
+            if (uint32_t(lane) < work) {
+                const SRef r = refs[tk.begin + lane];
+                for (int k = 0; k < 3; ++k) {
+                    pb[k] = bin_of(0.5f * (r.mn[k] + r.mx[k]), d.org[k], d.inv[k]);
+                    if (r.mn[k] <= r.mx[k]) {
+                        pmn[k] = fenc(r.mn[k]);
+                        pmx[k] = fenc(r.mx[k]);
+                    }
+                }
+            }
+            for (int ax = 0; ax < 3; ++ax) acc_clear(own[ax]);
+            for (uint32_t q = 0; q < work; ++q) {
+                uint32_t qb[3], qmn[3], qmx[3];
+                for (int k = 0; k < 3; ++k) {
+                    qb[k] = __shfl(pb[k], int(q), kBins);
+                    qmn[k] = __shfl(pmn[k], int(q), kBins);
+                    qmx[k] = __shfl(pmx[k], int(q), kBins);
+                }
+                for (int ax = 0; ax < 3; ++ax)
+                    if (qb[ax] == uint32_t(lane)) {
+                        ++own[ax].c0;
+                        for (int k = 0; k < 3; ++k) {
+                            own[ax].mn[k] = min(own[ax].mn[k], qmn[k]);
+                            own[ax].mx[k] = max(own[ax].mx[k], qmx[k]);
+                        }
+                    }
+            }
+            for (int ax = 0; ax < 3; ++ax) tb[ax * kBins + lane] = own[ax];
+        } else {
+            for (int ax = 0; ax < 3; ++ax) own[ax] = tb[ax * kBins + lane];
+        }
+        GRun pre[3], suf[3];
+        float key = __builtin_inff();
+        int idx = 3 * kBins;
+        for (int ax = 0; ax < 3; ++ax) {
+            GRun p;
+            for (int k = 0; k < 3; ++k) {
+                p.mn[k] = fdec(own[ax].mn[k]);
+                p.mx[k] = fdec(own[ax].mx[k]);
+            }
+            p.cnt = own[ax].c0;
+            GRun q = p;
+            for (int off = 1; off < kBins; off <<= 1) {
+                GRun lower = grun_shfl_up(p, off);
+                const GRun higher = grun_shfl_down(q, off);
+                if (lane >= off) {
+                    grun_join(lower, p);
+                    p = lower;
+                }
+                if (lane + off < kBins) grun_join(q, higher);
+            }
+            pre[ax] = p;
+            suf[ax] = q;
+            const GRun right = grun_shfl_down(q, 1);
+            if (lane < kBins - 1 && p.cnt != 0 && p.cnt != work) {
+                const float c = half_area6(p.mn, p.mx) * float(p.cnt) +
+                                half_area6(right.mn, right.mx) * float(right.cnt);
+                if (c < d.cost && c < key) {  // first axis wins a tie
+                    key = c;
+                    idx = ax * kBins + lane;
                 }
             }
         }
-        if (d.kind == kObject) {
+        for (int off = kBins / 2; off > 0; off >>= 1) {
+            const float ok = __shfl_xor(key, off, kBins);
+            const int oi = __shfl_xor(idx, off, kBins);
+            if (ok < key || (ok == key && oi < idx)) {
+                key = ok;
+                idx = oi;
+            }
+        }
+        if (idx < 3 * kBins) {
+            const uint32_t ax = uint32_t(idx / kBins);
+            const int i = idx % kBins;
+            const GRun lbest = grun_shfl(grun_pick(pre, ax), i);
+            const GRun rbest = grun_shfl(grun_pick(suf, ax), i + 1);
+            d.cost = key;
+            d.kind = kObject;
+            d.axis = ax;
+            d.k = uint32_t(i + 1);
+            d.left = lbest.cnt;
+            d.right = work - lbest.cnt;
             Box ov;
             for (int k = 0; k < 3; ++k) {
                 ov.mn[k] = fmaxf(lbest.mn[k], rbest.mn[k]);
@@ -438,9 +554,11 @@ __global__ void obj_split_kernel(uint32_t num_tasks, const STask* tasks, SDec* d
             cand = 1;  // every centroid in one bin: a plane may still separate them
         }
     }
-    d.cand = cand;
-    dec[t] = d;
-    cand_flag[t] = cand;
+    if (lane == 0) {
+        d.cand = cand;
+        dec[t] = d;
+        cand_flag[t] = cand;
+    }
 }
 
 // Chopped spatial binning of the references of candidate tasks.  A
@@ -897,15 +1015,16 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
         const uint32_t rg = (nrefs + kBlock - 1) / kBlock;
         const uint32_t ncand = std::min<uint32_t>(num_tasks, kMaxCand);
         hipLaunchKernelGGL(task_clear_kernel, dim3(num_tasks), dim3(64), 0, stream, num_tasks,
-                           tbox.p, obins.p);
+                           t_cur, tbox.p, obins.p);
         hipLaunchKernelGGL(ref_bounds_kernel, dim3(rg), dim3(kBlock), 0, stream, nrefs, ref_cur,
                            tbox.p);
         hipLaunchKernelGGL(task_setup_kernel, dim3(tg), dim3(kBlock), 0, stream, num_tasks,
                            tbox.p, dec.p, root_half_area, level);
         hipLaunchKernelGGL(obj_bin_kernel, dim3(rg), dim3(kBlock), 0, stream, nrefs, ref_cur,
                            t_cur, dec.p, obins.p);
-        hipLaunchKernelGGL(obj_split_kernel, dim3(tg), dim3(kBlock), 0, stream, num_tasks, t_cur,
-                           dec.p, obins.p, root_half_area, cand.p);
+        const uint32_t gg = (num_tasks + kBlock / kBins - 1) / (kBlock / kBins);  // task groups
+        hipLaunchKernelGGL(obj_split_kernel, dim3(gg), dim3(kBlock), 0, stream, num_tasks, t_cur,
+                           dec.p, obins.p, ref_cur, root_half_area, cand.p);
         if ((e = scan_exclusive_u32(cand.p, num_tasks, host + 3, scratch.p, scratch.cap * 4,
                                     stream)) != hipSuccess)
             return fail(e, "spatial candidate scan");
