@@ -262,9 +262,10 @@ __device__ __forceinline__ bool in_run(int32_t t, int32_t t0, int32_t nt) {
     return t0 >= 0 && uint32_t(t - t0) < uint32_t(nt);
 }
 
-// Node boxes: the union of each task's reference boxes.
+// Node boxes: the union of each task's reference boxes (tasks of at most
+// kSmallTask references are left to task_setup_kernel).
 __global__ void __launch_bounds__(kBlock)
-    ref_bounds_kernel(uint32_t nrefs, const SRef* refs, Acc* tbox) {
+    ref_bounds_kernel(uint32_t nrefs, const SRef* refs, const STask* tasks, Acc* tbox) {
     __shared__ uint32_t s_mn[kLdsTasks][3], s_mx[kLdsTasks][3];
     __shared__ int32_t s_t0, s_nt;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -282,7 +283,7 @@ __global__ void __launch_bounds__(kBlock)
     if (i < nrefs) {
         const SRef r = refs[i];
         const bool lds = in_run(r.task, t0, s_nt);
-        if (r.task >= 0)
+        if (r.task >= 0 && tasks[r.task].end - tasks[r.task].begin > kSmallTask)
             for (int k = 0; k < 3; ++k) {
                 if (!(r.mn[k] <= r.mx[k])) continue;  // empty or NaN
                 if (lds) {
@@ -303,12 +304,29 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Padded node box and bin parameters; the root's surface at level 0.
-__global__ void task_setup_kernel(uint32_t num_tasks, const Acc* tbox, SDec* dec,
-                                  float* root_half_area, uint32_t level) {
+__global__ void task_setup_kernel(uint32_t num_tasks, const STask* tasks, const SRef* refs,
+                                  const Acc* tbox, SDec* dec, float* root_half_area,
+                                  uint32_t level) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= num_tasks) return;
     SDec d;
-    const Acc b = tbox[t];
+    const STask tk = tasks[t];
+    Acc b;
+    if (tk.end - tk.begin <= kSmallTask) {
+        // ref_bounds_kernel's union for a small task: the same ordered
+        // encodings, empty and NaN bounds skipped
+        acc_clear(b);
+        for (uint32_t i = tk.begin; i < tk.end; ++i) {
+            const SRef r = refs[i];
+            for (int k = 0; k < 3; ++k) {
+                if (!(r.mn[k] <= r.mx[k])) continue;
+                b.mn[k] = min(b.mn[k], fenc(r.mn[k]));
+                b.mx[k] = max(b.mx[k], fenc(r.mx[k]));
+            }
+        }
+    } else {
+        b = tbox[t];
+    }
     float mn[3], mx[3];
     for (int k = 0; k < 3; ++k) {
         mn[k] = fdec(b.mn[k]);
@@ -1017,9 +1035,9 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
         hipLaunchKernelGGL(task_clear_kernel, dim3(num_tasks), dim3(64), 0, stream, num_tasks,
                            t_cur, tbox.p, obins.p);
         hipLaunchKernelGGL(ref_bounds_kernel, dim3(rg), dim3(kBlock), 0, stream, nrefs, ref_cur,
-                           tbox.p);
-        hipLaunchKernelGGL(task_setup_kernel, dim3(tg), dim3(kBlock), 0, stream, num_tasks,
-                           tbox.p, dec.p, root_half_area, level);
+                           t_cur, tbox.p);
+        hipLaunchKernelGGL(task_setup_kernel, dim3(tg), dim3(kBlock), 0, stream, num_tasks, t_cur,
+                           ref_cur, tbox.p, dec.p, root_half_area, level);
         hipLaunchKernelGGL(obj_bin_kernel, dim3(rg), dim3(kBlock), 0, stream, nrefs, ref_cur,
                            t_cur, dec.p, obins.p);
         const uint32_t gg = (num_tasks + kBlock / kBins - 1) / (kBlock / kBins);  // task groups
