@@ -460,9 +460,9 @@ __device__ __forceinline__ GRun grun_pick(const GRun (&r)[3], uint32_t ax) {
 // equal a sweep's up to the sign of a zero, which no comparison tells apart.
 // A task of at most kSmallTask references is binned here (lane q takes
 // reference q, lane j gathers bin j with obj_bin_kernel's ordered-encoding
-// minimum and maximum) and its bins are stored for decide_kernel.
+// minimum and maximum); decide_kernel counts its references itself.
 __global__ void __launch_bounds__(kBlock)
-    obj_split_kernel(uint32_t num_tasks, const STask* tasks, SDec* dec, Acc* obins,
+    obj_split_kernel(uint32_t num_tasks, const STask* tasks, SDec* dec, const Acc* obins,
                      const SRef* refs, const float* root_half_area, uint32_t* cand_flag) {
     const uint32_t t = (blockIdx.x * kBlock + threadIdx.x) / kBins;
     const int lane = int(threadIdx.x % kBins);
@@ -472,7 +472,7 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t work = tk.end - tk.begin;
     uint32_t cand = 0;
     if (work > 1 && tk.depth < kMaxDepth) {
-        Acc* tb = obins + size_t(t) * 3 * kBins;
+        const Acc* tb = obins + size_t(t) * 3 * kBins;
         Acc own[3];
         if (work <= kSmallTask) {
             uint32_t pb[3] = {0, 0, 0}, pmn[3], pmx[3];
@@ -507,7 +507,6 @@ __global__ void __launch_bounds__(kBlock)
                         }
                     }
             }
-            for (int ax = 0; ax < 3; ++ax) tb[ax * kBins + lane] = own[ax];
         } else {
             for (int ax = 0; ax < 3; ++ax) own[ax] = tb[ax * kBins + lane];
         }
@@ -714,8 +713,8 @@ __global__ void sp_split_kernel(uint32_t num_tasks, const STask* tasks, SDec* de
 // Final decision per task; counts for the scans.
 __global__ void decide_kernel(uint32_t num_tasks, const STask* tasks, SDec* dec,
                               const uint32_t* dup_excl, uint32_t remaining, uint32_t max_leaf,
-                              const Acc* obins, uint32_t* split_flag, uint32_t* out_cnt,
-                              uint32_t* leaf_cnt) {
+                              const Acc* obins, const SRef* refs, uint32_t* split_flag,
+                              uint32_t* out_cnt, uint32_t* leaf_cnt) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= num_tasks) return;
     const STask tk = tasks[t];
@@ -745,7 +744,19 @@ __global__ void decide_kernel(uint32_t num_tasks, const STask* tasks, SDec* dec,
                 const Acc* b = obins + size_t(t) * 3 * kBins + ax * kBins;
                 uint32_t c = 0, k = 0;
                 for (int i = 0; i < kBins - 1; ++i) {
-                    c += b[i].c0;
+                    if (work > kSmallTask) {
+                        c += b[i].c0;
+                    } else {
+                        // a small task's bins were never stored: count its
+                        // references' centroid bins (obj_bin_kernel's count)
+                        for (uint32_t q = tk.begin; q < tk.end; ++q) {
+                            const SRef& r = refs[q];
+                            c += bin_of(0.5f * (r.mn[ax] + r.mx[ax]), d.org[ax], d.inv[ax]) ==
+                                         uint32_t(i)
+                                     ? 1u
+                                     : 0u;
+                        }
+                    }
                     if (c >= work * 2 / 5 + 1) {
                         k = uint32_t(i + 1);
                         break;
@@ -1057,8 +1068,8 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
                                     stream)) != hipSuccess)
             return fail(e, "spatial budget scan");
         hipLaunchKernelGGL(decide_kernel, dim3(tg), dim3(kBlock), 0, stream, num_tasks, t_cur,
-                           dec.p, dup.p, budget - dups, leaf, obins.p, split_flag.p, out_cnt.p,
-                           leaf_cnt.p);
+                           dec.p, dup.p, budget - dups, leaf, obins.p, ref_cur, split_flag.p,
+                           out_cnt.p, leaf_cnt.p);
         if ((e = scan_exclusive_u32(split_flag.p, num_tasks, host + 0, scratch.p,
                                     scratch.cap * 4, stream)) != hipSuccess ||
             (e = scan_exclusive_u32(out_cnt.p, num_tasks, host + 1, scratch.p, scratch.cap * 4,
