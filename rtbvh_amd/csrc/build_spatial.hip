@@ -220,8 +220,9 @@ __global__ void point_refs_kernel(const float* centers, uint32_t stride, uint32_
     refs[i] = r;
 }
 
-// Per task: node box and object bins cleared (block t, 64 threads; small
-// tasks' bins are written whole by obj_split_kernel).
+// Per task: node box and object bins cleared (block t, 64 threads; a task of
+// at most kSmallTask references has no stored object bins: obj_split_kernel
+// bins it in registers).
 __global__ void task_clear_kernel(uint32_t num_tasks, const STask* tasks, Acc* tbox, Acc* obins) {
     const uint32_t t = blockIdx.x;
     if (t >= num_tasks) return;
