@@ -464,9 +464,10 @@ __global__ void children_kernel(const BTask* tasks, uint32_t num_tasks, const BD
     nodes[tk.node] = nd;
 }
 
+// Left-side flags, written twice: kept in flag, scanned in place in lscan.
 __global__ void flag_kernel(uint32_t n, const int32_t* ptask, const uint32_t* idx,
                             const BDecision* dec, const float* centers, uint32_t stride,
-                            uint32_t* flag) {
+                            uint32_t* flag, uint32_t* lscan) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int32_t t = ptask[i];
@@ -479,6 +480,7 @@ __global__ void flag_kernel(uint32_t n, const int32_t* ptask, const uint32_t* id
         }
     }
     flag[i] = f;
+    lscan[i] = f;
 }
 
 __global__ void scatter_kernel(uint32_t n, const int32_t* ptask, const uint32_t* idx,
@@ -710,10 +712,7 @@ ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t st
         hipLaunchKernelGGL(children_kernel, dim3(tg), dim3(kBlock), 0, stream, t_cur, num_tasks,
                            dec.p, split_flag.p, counter, res.nodes, t_nxt);
         hipLaunchKernelGGL(flag_kernel, dim3(g), dim3(kBlock), 0, stream, n, pt_cur, idx_cur, dec.p,
-                           centers, stride, flag.p);
-        if ((e = hipMemcpyAsync(lscan.p, flag.p, size_t(n) * 4, hipMemcpyDeviceToDevice,
-                                stream)) != hipSuccess)
-            return fail(e, "binned flag copy");
+                           centers, stride, flag.p, lscan.p);
         if ((e = scan_exclusive_u32(lscan.p, n, nullptr, scan_scratch.p, scan_scratch.cap * 4,
                                     stream)) != hipSuccess)
             return fail(e, "binned partition scan");
