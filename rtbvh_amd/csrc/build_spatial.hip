@@ -827,8 +827,11 @@ __global__ void children_kernel(uint32_t num_tasks, const STask* tasks, const SD
 }
 
 // Which child (or both) each reference goes to.
+// Side flags of each reference, written twice: kept in fl/fr, scanned in
+// place in fle/fre.
 __global__ void side_kernel(uint32_t nrefs, const SRef* refs, const STask* tasks,
-                            const SDec* dec, uint32_t* fl, uint32_t* fr) {
+                            const SDec* dec, uint32_t* fl, uint32_t* fr, uint32_t* fle,
+                            uint32_t* fre) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nrefs) return;
     const SRef r = refs[i];
@@ -851,6 +854,8 @@ __global__ void side_kernel(uint32_t nrefs, const SRef* refs, const STask* tasks
     }
     fl[i] = l;
     fr[i] = rr;
+    fle[i] = l;
+    fre[i] = rr;
 }
 
 __global__ void scatter_kernel(uint32_t nrefs, const SRef* refs, const STask* tasks,
@@ -1082,12 +1087,7 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
                            dec.p, split_flag.p, out_cnt.p, leaf_cnt.p, counter, leaf_base,
                            node_cap, res.nodes, t_nxt, fail_word);
         hipLaunchKernelGGL(side_kernel, dim3(rg), dim3(kBlock), 0, stream, nrefs, ref_cur, t_cur,
-                           dec.p, fl.p, fr.p);
-        if ((e = hipMemcpyAsync(fle.p, fl.p, size_t(nrefs) * 4, hipMemcpyDeviceToDevice,
-                                stream)) != hipSuccess ||
-            (e = hipMemcpyAsync(fre.p, fr.p, size_t(nrefs) * 4, hipMemcpyDeviceToDevice,
-                                stream)) != hipSuccess)
-            return fail(e, "spatial side flags");
+                           dec.p, fl.p, fr.p, fle.p, fre.p);
         if ((e = scan_exclusive_u32(fle.p, nrefs, nullptr, scratch.p, scratch.cap * 4,
                                     stream)) != hipSuccess ||
             (e = scan_exclusive_u32(fre.p, nrefs, nullptr, scratch.p, scratch.cap * 4,
