@@ -19,6 +19,17 @@
  * Codes 0-4 keep the reference's values (rtbvh_ffi/src/lib.rs:17-25); codes
  * 5-9 are appended.  rtbvh_last_error() describes the last failure on the
  * calling thread.
+ *
+ * Threading (as rtbvh_ffi's StructureManager, rtbvh_ffi/src/lib.rs:8-15):
+ * handles live in one registry guarded by a reader-writer lock.  Creation
+ * and free_* take it exclusively; the traversals that look a handle up (the
+ * prepared, traversal-tree and batch calls) hold it shared until their
+ * kernels are enqueued, so a concurrent free waits for them and then for the
+ * device before releasing memory.  Calls that take raw node/index pointers
+ * (intersect*, rtbvh_gpu_intersect*, rtbvh_gpu_occluded*) read them without a
+ * lock, as the reference's do: freeing or refitting a tree while another
+ * thread traverses it is the caller's race.  Calls on one batch plan must not
+ * overlap, its free included.
  */
 #ifndef RTBVH_H
 #define RTBVH_H

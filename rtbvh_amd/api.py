@@ -264,6 +264,11 @@ class TraceTree:
         _require_device(vertices, "vertices")
         verts = vertices.reshape(-1)
         self.device = vertices.device
+        # the tree holds a snapshot of these vertices: remember which tensor
+        # and which version of it (in-place writes bump _version), and keep it
+        # alive so that its memory cannot be handed to another tensor
+        self._source = vertices
+        self._source_version = vertices._version
         self._owned = False
         self._h = L.RTTraceTree()
         if width == 8:
@@ -276,6 +281,14 @@ class TraceTree:
                                                         ctypes.byref(self._h)),
                     "rtbvh_gpu_create_trace_tree")
         self._owned = True
+
+    def built_from(self, vertices: torch.Tensor) -> bool:
+        """True when `vertices` is the tensor (or a view of the same memory
+        and shape, e.g. a .detach()) this tree was built from, unmodified
+        since."""
+        src = self._source
+        return (vertices.data_ptr() == src.data_ptr() and vertices.numel() == src.numel()
+                and vertices.device == src.device and vertices._version == self._source_version)
 
     @property
     def node_count(self) -> int:

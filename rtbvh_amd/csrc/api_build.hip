@@ -295,9 +295,10 @@ ResultCode collapse_entry(uint32_t id, void* stream, bool host, RTMbvh* out) {
 
 // Attached leaf-ordered triangles of an MBVH made by this library (null if
 // none); *count receives the number of index entries they cover.
-const float4* attached_triangles(uint32_t id, uint32_t* count, int* device, bool* small) {
+const float4* attached_triangles(uint32_t id, uint32_t* count, int* device, bool* small,
+                                 RegistryHold& hold) {
     Registry& r = reg();
-    std::shared_lock<std::shared_mutex> lk(r.mu);
+    if (!hold.owns_lock()) hold = RegistryHold(r.mu);
     MbvhEntry* me = find_mbvh(id);
     if (me == nullptr || me->dev.tris == nullptr) return nullptr;
     *count = me->dev.prim_count;
@@ -308,9 +309,9 @@ const float4* attached_triangles(uint32_t id, uint32_t* count, int* device, bool
 
 // The arrays a batched traversal needs (false if the id is unknown); tris
 // is null until triangles are attached.
-bool batch_view(uint32_t id, BatchView* out) {
+bool batch_view(uint32_t id, BatchView* out, RegistryHold& hold) {
     Registry& r = reg();
-    std::shared_lock<std::shared_mutex> lk(r.mu);
+    if (!hold.owns_lock()) hold = RegistryHold(r.mu);
     BatchEntry* be = find_batch(id);
     if (be == nullptr) return false;
     out->nodes = be->ttree.nodes;
@@ -324,9 +325,9 @@ bool batch_view(uint32_t id, BatchView* out) {
     return true;
 }
 
-bool trace_tree_view(uint32_t id, BatchView* out) {
+bool trace_tree_view(uint32_t id, BatchView* out, RegistryHold& hold) {
     Registry& r = reg();
-    std::shared_lock<std::shared_mutex> lk(r.mu);
+    if (!hold.owns_lock()) hold = RegistryHold(r.mu);
     if (id >= r.ttrees.size() || !r.ttrees[id]) return false;
     const TraceTreeEntry& te = *r.ttrees[id];
     out->nodes = te.t.nodes;

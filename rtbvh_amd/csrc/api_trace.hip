@@ -132,7 +132,8 @@ ResultCode prepared_entry(RTMbvh mbvh, const RTRay* rays, size_t ray_count, RTHi
     uint32_t count = 0;
     int device = 0;
     bool small = false;
-    const float4* tris = attached_triangles(mbvh.id, &count, &device, &small);
+    RegistryHold hold;  // held until the traversal is enqueued
+    const float4* tris = attached_triangles(mbvh.id, &count, &device, &small, hold);
     if (tris == nullptr) {
         set_error("mbvh %u has no attached triangles (rtbvh_gpu_mbvh_attach_triangles)", mbvh.id);
         return Error;
@@ -218,7 +219,8 @@ ResultCode batch_entry(RTBatch batch, const RTRay* rays, const uint32_t* ray_off
                        size_t ray_count, RTHit* hits, uint8_t* occluded, uint32_t flags,
                        void* stream) {
     BatchView v;
-    if (!batch_view(batch.id, &v)) {
+    RegistryHold hold;  // held until the traversal is enqueued
+    if (!batch_view(batch.id, &v, hold)) {
         set_error("unknown batch id %u", batch.id);
         return Error;
     }
@@ -233,7 +235,8 @@ ResultCode batch_entry(RTBatch batch, const RTRay* rays, const uint32_t* ray_off
 ResultCode tree_entry(RTTraceTree tree, const RTRay* rays, size_t ray_count, RTHit* hits,
                       uint8_t* occluded, uint32_t flags, void* stream, uint64_t* stats = nullptr) {
     BatchView v;
-    if (!trace_tree_view(tree.id, &v)) {
+    RegistryHold hold;  // held until the traversal is enqueued
+    if (!trace_tree_view(tree.id, &v, hold)) {
         set_error("unknown trace tree id %u", tree.id);
         return Error;
     }
@@ -586,9 +589,12 @@ ResultCode rtbvh_gpu_autotune_mbvh(RTMbvh mbvh, const RTRay* rays, size_t ray_co
 ResultCode rtbvh_gpu_autotune_trace_tree(RTTraceTree tree, const RTRay* rays, size_t ray_count,
                                          RTHit* hits, void* stream, RTTraceConfig* best) {
     BatchView v;
-    if (!trace_tree_view(tree.id, &v)) {
-        set_error("unknown trace tree id %u", tree.id);
-        return Error;
+    {
+        RegistryHold hold;  // released before the timed calls take their own
+        if (!trace_tree_view(tree.id, &v, hold)) {
+            set_error("unknown trace tree id %u", tree.id);
+            return Error;
+        }
     }
     return autotune(rays, ray_count, hits, stream, best, v.cnodes != nullptr, [&]() {
         return tree_entry(tree, rays, ray_count, hits, nullptr, 0, stream);
@@ -599,9 +605,12 @@ ResultCode rtbvh_gpu_autotune_batch(RTBatch batch, const RTRay* rays, const uint
                                     size_t ray_count, RTHit* hits, void* stream,
                                     RTTraceConfig* best) {
     BatchView v;
-    if (!batch_view(batch.id, &v)) {
-        set_error("unknown batch id %u", batch.id);
-        return Error;
+    {
+        RegistryHold hold;  // released before the timed calls take their own
+        if (!batch_view(batch.id, &v, hold)) {
+            set_error("unknown batch id %u", batch.id);
+            return Error;
+        }
     }
     return autotune(rays, ray_count, hits, stream, best, v.cnodes != nullptr, [&]() {
         return batch_entry(batch, rays, ray_offsets, ray_count, hits, nullptr, 0, stream);

@@ -268,7 +268,14 @@ ResultCode rtbvh_gpu_batch_plan_launch(RTBatchPlan plan, const RTAabb* aabbs, co
         check_args(*pe, aabbs, centers, vertices, rays, ray_offsets, ray_count, hits, occluded);
     if (rc != Ok) return rc;
     const GraphKey key{aabbs, centers, vertices, rays, ray_offsets, ray_count, hits, occluded};
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    // everything below runs on the plan's stream after the caller's prior
+    // work: the inputs may still be written there, and the outputs may be
+    // memory the caller's stream is still using
     hipError_t e;
+    if ((e = hipEventRecord(pe->ev_in, user)) != hipSuccess ||
+        (e = hipStreamWaitEvent(pe->stream, pe->ev_in, 0)) != hipSuccess)
+        return hip_fail(e, "order after the caller's stream");
     if (pe->exec == nullptr || !(pe->key == key)) {
         if (pe->exec) {
             (void)hipGraphExecDestroy(pe->exec);
@@ -298,10 +305,7 @@ ResultCode rtbvh_gpu_batch_plan_launch(RTBatchPlan plan, const RTAabb* aabbs, co
         }
         pe->key = key;
     }
-    hipStream_t user = static_cast<hipStream_t>(stream);
-    if ((e = hipEventRecord(pe->ev_in, user)) != hipSuccess ||
-        (e = hipStreamWaitEvent(pe->stream, pe->ev_in, 0)) != hipSuccess ||
-        (e = hipGraphLaunch(pe->exec, pe->stream)) != hipSuccess ||
+    if ((e = hipGraphLaunch(pe->exec, pe->stream)) != hipSuccess ||
         (e = hipEventRecord(pe->ev_out, pe->stream)) != hipSuccess ||
         (e = hipStreamWaitEvent(user, pe->ev_out, 0)) != hipSuccess)
         return hip_fail(e, "graph launch");

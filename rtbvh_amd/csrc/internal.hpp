@@ -1,6 +1,7 @@
 // Internal interfaces shared by the translation units of librtbvh_amd.so.
 #pragma once
 
+#include <shared_mutex>
 #include <vector>
 
 #include <hip/hip_runtime.h>
@@ -183,8 +184,16 @@ hipError_t launch_adjoint(const float* verts, uint32_t prim_count, const DevRay*
 hipError_t mbvh_max_leaf(const DevMbvhNode* nodes, uint32_t count, uint32_t* result,
                          hipStream_t stream);
 
+// A shared lock on the library's handle registry.  The lookups below take it
+// into the caller's hold, and the caller keeps it until its kernels are
+// enqueued: every free_* takes the registry's exclusive lock, so the arrays a
+// lookup returned stay allocated until then (a free then waits for the
+// device before releasing them).
+using RegistryHold = std::shared_lock<std::shared_mutex>;
+
 // small: every leaf of the MBVH holds at most 16 triangles.
-const float4* attached_triangles(uint32_t id, uint32_t* count, int* device, bool* small);
+const float4* attached_triangles(uint32_t id, uint32_t* count, int* device, bool* small,
+                                 RegistryHold& hold);
 
 struct BatchView {
     const DevMbvhNode* nodes;
@@ -196,8 +205,8 @@ struct BatchView {
     uint32_t scenes;
     int device;
 };
-bool batch_view(uint32_t id, BatchView* out);
-bool trace_tree_view(uint32_t id, BatchView* out);
+bool batch_view(uint32_t id, BatchView* out, RegistryHold& hold);
+bool trace_tree_view(uint32_t id, BatchView* out, RegistryHold& hold);
 
 constexpr uint32_t kDefaultMaxLeaf = 1;
 // Traversal trees of batches and batch plans: 4 or 8 (the compressed tree).

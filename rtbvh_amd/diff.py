@@ -46,7 +46,13 @@ class TraceTUV(torch.autograd.Function):
         verts = vertices.detach().contiguous()
         rays = rays_tensor(origins.detach().contiguous(), directions.detach().contiguous(),
                            t_min, t_max)
-        if isinstance(tree, TraceTree):  # built over these vertices
+        if isinstance(tree, TraceTree):
+            # a traversal tree traces its own snapshot of the triangles: the
+            # gradients are only those of `vertices` if the snapshot is theirs
+            if not tree.built_from(vertices):
+                raise ValueError("the TraceTree was built from other vertices, or these were "
+                                 "modified since: rebuild it (Bvh.trace_tree) from the vertices "
+                                 "passed here")
             hits = tree.intersect(rays)
         else:
             hits = tree.intersect(verts.reshape(-1, 9), rays)

@@ -195,3 +195,34 @@ def test_gpu_gradient_matches_gpu_central_differences():
     err = _rel(np.asarray(an), np.asarray(fd))
     print(f"[adjoint_fd_gpu] relative error {err:.2e} over {len(fd)} shared vertices", flush=True)
     assert err <= 5e-2
+
+
+def test_trace_tree_snapshot_guard():
+    """A TraceTree traces the snapshot of the vertices it was built from;
+    trace_tuv refuses vertices that are another tensor or were modified in
+    place since, because its gradients would belong to other geometry."""
+    print("[adjoint_snapshot_guard]", flush=True)
+    verts_np = scenes.noisy_sphere(60, 40, seed=5).astype(np.float32)
+    rays = scenes.rays_sphere_to_box(2000, (0.42, 0.42, 0.42), (0.58, 0.58, 0.58), seed=6)
+    dev = torch.device("cuda:0")
+    aabbs, centers = scenes.triangle_aabbs_centers(verts_np)
+    verts = torch.from_numpy(verts_np).to(dev)
+    bvh = rt.Builder(torch.from_numpy(centers).to(dev),
+                     aabbs=torch.from_numpy(aabbs).to(dev)).construct_locally_ordered_clustered()
+    tree = bvh.trace_tree(verts)
+    o = torch.from_numpy(np.ascontiguousarray(rays[:, 0:3])).to(dev)
+    d = torch.from_numpy(np.ascontiguousarray(rays[:, 4:7])).to(dev)
+    t, _, _, prim = trace_tuv(tree, verts.detach().requires_grad_(True), o, d)
+    assert (prim >= 0).float().mean().item() > 0.5
+    with pytest.raises(ValueError):
+        trace_tuv(tree, verts.clone(), o, d)  # equal values, another tensor
+    verts[:, 1] += 0.01                       # moved in place after the build
+    with pytest.raises(ValueError):
+        trace_tuv(tree, verts, o, d)
+    # a tree rebuilt over the moved vertices is accepted
+    a2, c2 = scenes.triangle_aabbs_centers(verts.cpu().numpy())
+    bvh2 = rt.Builder(torch.from_numpy(c2).to(dev),
+                      aabbs=torch.from_numpy(a2).to(dev)).construct_locally_ordered_clustered()
+    tree2 = bvh2.trace_tree(verts)
+    t2, _, _, _ = trace_tuv(tree2, verts, o, d)
+    assert not torch.equal(t, t2)

@@ -8,7 +8,9 @@ replayed as a hipGraph.
   * replays are checked one by one with eager work and synchronisation in
     between, and after the vertices move in place (an animated frame) the
     replay equals an eager build of the moved scenes;
-  * a one-scene plan's any-hit flags equal Mbvh.occluded_prepared.
+  * a one-scene plan's any-hit flags equal Mbvh.occluded_prepared;
+  * the first launch (warm-up and capture) is ordered after the caller's
+    stream.
 """
 import numpy as np
 import pytest
@@ -110,3 +112,33 @@ def test_single_scene_plan_any_hit():
     want = m.occluded_prepared(r_d).cpu().numpy()
     assert np.array_equal(occ, want)
     assert 0 < occ.mean() < 1
+
+
+def test_first_launch_waits_for_the_callers_stream():
+    """The first launch of a plan (eager warm-up, then capture) runs after the
+    work already queued on the caller's stream: a copy of `out` queued behind
+    slow work still sees the old contents, and the result equals the eager
+    path, with no synchronisation between producing the inputs and the
+    launch."""
+    print("[plan_stream_order]", flush=True)
+    vl, offs, rays_l = _batch()
+    verts = np.concatenate(vl)
+    rays = np.concatenate(rays_l)
+    aabbs, centers = prep(verts)
+    a_d, c_d, v_d, r_src = _dev(flat(aabbs)), _dev(centers), _dev(verts), _dev(rays)
+    roff = rt.SceneBatch.ray_offsets([len(r) for r in rays_l], "cuda")
+    want = _eager(verts, offs, rays, roff)
+    plan = rt.BatchPlan(c_d, offs, v_d, aabbs=a_d)
+    out = torch.full((len(rays), 4), -1.0, dtype=torch.float32, device="cuda")
+    big = torch.randn((6144, 6144), device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(6):  # tens of milliseconds of work ahead of the copy
+        big = big @ big
+        big = big / big.norm()
+    seen = out.clone()            # queued behind the slow work
+    r_d = r_src * 1.0             # the rays, produced asynchronously
+    plan.launch(c_d, v_d, r_d, roff, aabbs=a_d, out=out)
+    plan.status()
+    torch.cuda.synchronize()
+    assert bool((seen == -1.0).all())
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), want.view(np.uint32))
