@@ -359,6 +359,178 @@ def run_large(args, dev, rt, scenes, torch):
     return res
 
 
+def cpu_info():
+    """Host CPU model, logical CPUs, and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, allowed
+
+
+def cpu_threads():
+    """Threads for the CPU oracle: the CPUs this process may use, at most 16
+    (one GPU's share of the host on the benchmark machines)."""
+    return max(1, min(16, cpu_info()[2]))
+
+
+def oracle_rate(fn, rays, threads, target_s, reps=5, probe=20_000):
+    """Median-of-`reps` rays/s of the oracle traversal fn(rays, threads) on a
+    sample sized for about target_s seconds per run."""
+    probe = min(probe, len(rays))
+    t0 = time.perf_counter()
+    fn(rays[:probe], threads)
+    rate = probe / max(time.perf_counter() - t0, 1e-9)
+    n = int(min(len(rays), max(probe, rate * target_s)))
+    sample = rays[:n]
+    runs = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn(sample, threads)
+        runs.append(time.perf_counter() - t0)
+    return n / statistics.median(runs), n
+
+
+def gpu_launch_ms(torch, stream, fn, reps=20):
+    """Median per-launch time (HIP events on the launch stream) of fn()."""
+    fn()
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        fn()
+        b.record(stream)
+        b.synchronize()
+        ms.append(a.elapsed_time(b))
+    return statistics.median(ms)
+
+
+def run_c1(args, dev, rt, scenes, torch, stream):
+    """The reference's own benchmark shape, small meshes with the binned SAH
+    builder (examples/benchmark.rs): the teapot (6,320 triangles, the
+    reference's fixture) with the benchmark's 1000 x 1000 camera, and the
+    80 x 40 torus (6,400 triangles) with 1M rays from a radius-3 sphere aimed
+    at points in its box.  GPU: binned SAH build, MBVH collapse, the
+    reference-order traversals (BVH2 and MBVH, the reference's visit order and
+    tie rule) and the lowest-(t, id) compressed tree, each checked against the
+    CPU oracle's traversal of the same tree; CPU oracle: the same build and
+    reference-order traversals at 1 thread and at the host share."""
+    from oracle import refcpu as ref
+
+    log("C1: teapot and torus, binned SAH")
+    golden = os.path.join(ROOT, "tests", "golden", "teapot.obj")
+    threads = cpu_threads()
+    out = {}
+    for name in ("teapot", "torus"):
+        if name == "teapot":
+            verts = scenes.load_obj(golden)
+            rays_np = scenes.reference_camera_rays(1000, 1000)
+        else:
+            verts = scenes.torus(80, 40)
+            v3 = verts.reshape(-1, 3)
+            rays_np = scenes.rays_sphere_to_box(1_000_000, v3.min(0), v3.max(0), radius=3.0,
+                                                seed=args.seed)
+        aabbs, centers = scenes.triangle_aabbs_centers(verts)
+        v_d = torch.from_numpy(verts).to(dev)
+        a_d = torch.from_numpy(aabbs).to(dev)
+        c_d = torch.from_numpy(centers).to(dev)
+        r_d = torch.from_numpy(rays_np).to(dev)
+        build_ms, coll_ms = [], []
+        bvh = m = None
+        for i in range(6):
+            bvh = m = None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            bvh = rt.Builder(c_d, aabbs=a_d).construct_binned_sah()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            m = rt.Mbvh.from_bvh(bvh)
+            torch.cuda.synchronize()
+            if i > 0:
+                build_ms.append((t1 - t0) * 1e3)
+                coll_ms.append((time.perf_counter() - t1) * 1e3)
+        tree = bvh.trace_tree(v_d)
+        m.attach_triangles(v_d)
+        n = r_d.shape[0]
+        hits = torch.empty((n, 4), dtype=torch.float32, device=dev)
+        gpu = {}
+        for key, fn in (("bvh2_reference_order", lambda: bvh.intersect(v_d, r_d, True, out=hits)),
+                        ("mbvh_reference_order",
+                         lambda: m.intersect_prepared(r_d, True, out=hits)),
+                        ("compressed_tree_lowest_id", lambda: tree.intersect(r_d, out=hits))):
+            ms = gpu_launch_ms(torch, stream, fn)
+            gpu[key] = round(n / (ms * 1e-3) / 1e6, 1)
+        # parity on the GPU-built tree: reference order against the oracle's
+        # reference-order traversal, lowest id against the oracle's lowest id
+        nodes = bvh.nodes().cpu().numpy().view(ref.AABB).reshape(-1)
+        idx = bvh.indices().cpu().numpy().view(np.uint32)
+        mn = m.quad_nodes().cpu().numpy().view(ref.MBVH).reshape(-1)
+        rays = rays_np.view(ref.RAY).reshape(-1)
+        same = {}
+        for key, got, want in (
+                ("bvh2_reference_order", bvh.intersect(v_d, r_d, True),
+                 ref.trace_bvh(nodes, idx, verts, rays, mode=ref.MODE_VISIT_ORDER,
+                               threads=threads)),
+                ("mbvh_reference_order", m.intersect_prepared(r_d, True),
+                 ref.trace_mbvh(mn, idx, verts, rays, mode=ref.MODE_VISIT_ORDER,
+                                threads=threads)),
+                ("compressed_tree_lowest_id", tree.intersect(r_d),
+                 ref.trace_mbvh(mn, idx, verts, rays, threads=threads))):
+            g = got.cpu().numpy().view(np.uint32).reshape(-1, 4)
+            same[key] = bool(np.array_equal(g, want.view(np.uint32).reshape(-1, 4)))
+        # the CPU oracle: the same binned build (1 thread) and traversals
+        a_rec = aabbs.view(ref.AABB).reshape(-1)
+        cpu_build = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            ref.build_binned_sah(a_rec, centers)
+            cpu_build.append((time.perf_counter() - t0) * 1e3)
+        t0 = time.perf_counter()
+        ref.build_mbvh(nodes)
+        cpu_collapse_ms = (time.perf_counter() - t0) * 1e3
+        cpu = {}
+        for key, fn in (("bvh2", lambda r, t: ref.trace_bvh(nodes, idx, verts, r,
+                                                           mode=ref.MODE_VISIT_ORDER, threads=t)),
+                        ("mbvh", lambda r, t: ref.trace_mbvh(mn, idx, verts, r,
+                                                            mode=ref.MODE_VISIT_ORDER, threads=t))):
+            r1, _ = oracle_rate(fn, rays, 1, 0.4)
+            rn, _ = oracle_rate(fn, rays, threads, 0.4)
+            cpu[f"{key}_1t_mrays_s"] = round(r1 / 1e6, 3)
+            cpu[f"{key}_{threads}t_mrays_s"] = round(rn / 1e6, 3)
+        out[name] = {
+            "tris": len(verts),
+            "rays": n,
+            "rays_kind": ("examples/benchmark.rs camera, 1000 x 1000" if name == "teapot" else
+                          "radius-3 sphere aimed at uniform points in the mesh box"),
+            "gpu_binned_sah_build_ms": round(statistics.median(build_ms), 3),
+            "gpu_mbvh_collapse_ms": round(statistics.median(coll_ms), 3),
+            "gpu_mrays_s": gpu,
+            "gpu_same_as_oracle": same,
+            "hit_fraction": round(float((hits[:, 3].view(torch.int32) >= 0).float().mean()), 4),
+            "cpu_oracle": dict(cpu, binned_sah_build_ms_1t=round(statistics.median(cpu_build), 3),
+                               mbvh_collapse_ms_1t=round(cpu_collapse_ms, 3), threads=threads),
+        }
+        log(f"C1 {name}: GPU build {out[name]['gpu_binned_sah_build_ms']} ms, {gpu}, same {same}; "
+            f"CPU {cpu}")
+        del tree, m, bvh, v_d, a_d, c_d, r_d, hits
+    out["reference_published"] = {
+        "note": "README.md:31-41, teapot, AMD 5950X, 32 threads, coherent camera rays",
+        "binned_sah_build_ms": 3.81, "mbvh_collapse_ms": 1.156,
+        "bvh2_1t_mrays_s": 4.704701, "bvh2_32t_mrays_s": 77.38225,
+        "mbvh_1t_mrays_s": 5.970805, "mbvh_32t_mrays_s": 125.44958}
+    return out
+
+
 def run_build_only(kind, rt, torch, a_d, c_d, v_d, reps):
     """Timed binned-SAH or spatial-split builds of the 1M scene, each from a
     synchronised device after the previous tree was released."""
@@ -383,7 +555,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--image", type=int, default=8192, help="primary image side")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="budget for the CPU oracle baseline (0 disables it)")
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--quick", action="store_true",
@@ -399,6 +571,9 @@ def main():
     ap.add_argument("--large-rays-log2", type=int, default=28)
     ap.add_argument("--spatial", type=int, default=1,
                     help="1: also build the spatial-split SAH tree and trace the same rays")
+    ap.add_argument("--c1", type=int, default=1,
+                    help="1: also run the reference's own benchmark shape (teapot and torus, "
+                         "binned SAH, 1M rays) on the GPU and the CPU oracle")
     ap.add_argument("--adjoint", type=int, default=1,
                     help="1: also time forward + adjoint backward on 100k triangles (0 skips it)")
     ap.add_argument("--build-only", nargs="?", const="ploc", default=None,
@@ -677,46 +852,67 @@ def main():
     node_bytes = 80 if args.width == 8 else 128
     b_ray = 32 + 16 + node_bytes * nodes_per_ray + 48 * tris_per_ray
     achieved = value * 1e6 * b_ray / 1e9
-    traffic = None
+    # HBM bytes per launch, measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    # of this kernel (scaled by a plain copy), kept beside the kernel trace
+    traffic, traffic_shape = None, None
     if os.path.exists(PMC_SUMMARY):
         with open(PMC_SUMMARY) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            pmc = json.load(fh)
+        traffic, traffic_shape = pmc.get("hbm_bytes_per_launch"), pmc.get("launch_shape")
 
     # ---- CPU oracle on a sample of the same rays ----------------------------
+    # the reference's algorithm restated in C++ (oracle/refcpu.cpp): its
+    # BVH2 and MBVH iterators in their visit order with the first-accepted
+    # closest hit (examples/benchmark.rs:15-41), over the same PLOC tree (the
+    # oracle's LOCB, which the GPU build equals bit for bit) and its MBVH, at
+    # 1 thread and at the host's share; median of 5
     cpu = None
     parity = None
     if args.cpu_seconds > 0:
         from oracle import refcpu as ref
 
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
+        model, nproc, allowed = cpu_info()
         m_nodes = mbvh.quad_nodes().cpu().numpy().view(ref.MBVH).reshape(-1)
         m_idx = mbvh.indices().cpu().numpy().view(np.uint32)
-        probe = 100_000
-        sample = bounce[:probe].cpu().numpy().view(ref.RAY).reshape(-1)
-        t0 = time.perf_counter()
-        ref.trace_mbvh(m_nodes, m_idx, verts, sample, threads=threads)
-        rate = probe / (time.perf_counter() - t0)
-        n_sample = int(min(n_rays, max(probe, rate * args.cpu_seconds / 3)))
-        n_sample -= n_sample % 1000
-        sample = bounce[:n_sample].cpu().numpy().view(ref.RAY).reshape(-1)
-        runs = []
-        want = None
-        for _ in range(3):
-            t0 = time.perf_counter()
-            want = ref.trace_mbvh(m_nodes, m_idx, verts, sample, threads=threads)
-            runs.append(time.perf_counter() - t0)
-        cpu_mrays = n_sample / statistics.median(runs) / 1e6
-        got = hits[:n_sample].cpu().numpy().view(np.uint32).reshape(-1, 4)
+        b_nodes = bvh.nodes().cpu().numpy().view(ref.AABB).reshape(-1)
+        b_idx = bvh.indices().cpu().numpy().view(np.uint32)
+        budget = args.cpu_seconds / 20.0  # per timed run: 2 trees x 2 thread counts x 5
+        pool = bounce[:4_000_000].cpu().numpy().view(ref.RAY).reshape(-1)
+        rates = {}
+        for key, fn in (("mbvh", lambda r, t: ref.trace_mbvh(m_nodes, m_idx, verts, r,
+                                                            mode=ref.MODE_VISIT_ORDER, threads=t)),
+                        ("bvh2", lambda r, t: ref.trace_bvh(b_nodes, b_idx, verts, r,
+                                                           mode=ref.MODE_VISIT_ORDER, threads=t))):
+            for t in (threads, 1):
+                rates[(key, t)] = oracle_rate(fn, pool, t, budget)
+        cpu_mrays = rates[("mbvh", threads)][0] / 1e6
+        # parity of the headline's lowest-(t, id) hits on a 1M-ray sample
+        n_par = min(n_rays, 1_000_000)
+        want = ref.trace_mbvh(m_nodes, m_idx, verts, pool[:n_par], threads=threads)
+        got = hits[:n_par].cpu().numpy().view(np.uint32).reshape(-1, 4)
         mism = int(np.count_nonzero(np.any(got != want.view(np.uint32).reshape(-1, 4), axis=1)))
-        parity = {"rays_checked": n_sample, "mismatches": mism}
+        parity = {"rays_checked": n_par, "mismatches": mism}
         t0 = time.perf_counter()
         ref.build_locb(aabbs.view(ref.AABB).reshape(-1), centers)
         cpu_build_ms = (time.perf_counter() - t0) * 1e3
         cpu = {"value": round(cpu_mrays, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-               "sample": f"first {n_sample} of the {n_rays} bounce rays, same MBVH and triangles, "
-                         f"oracle lowest-id closest hit, median of 3; oracle LOCB build of the "
-                         f"1M scene {cpu_build_ms:.0f} ms on 1 thread"}
-        log(f"CPU oracle {cpu_mrays:.2f} Mrays/s on {threads} threads; parity {parity}")
+               "sample": f"first {rates[('mbvh', threads)][1]} of the {n_rays} bounce rays "
+                         f"({rates[('mbvh', 1)][1]} at 1 thread), the oracle's MBVH iterator in "
+                         f"the reference's visit order with the first-accepted closest hit "
+                         f"(examples/benchmark.rs:15-41) over the same PLOC tree, median of 5, "
+                         f"{threads} threads (this process may use {allowed} of {nproc} host "
+                         f"CPUs; one GPU's share is 16)",
+               "cpu_model": model,
+               "mbvh_1t_mrays_s": round(rates[("mbvh", 1)][0] / 1e6, 3),
+               "bvh2_mrays_s": round(rates[("bvh2", threads)][0] / 1e6, 3),
+               "bvh2_1t_mrays_s": round(rates[("bvh2", 1)][0] / 1e6, 3),
+               "locb_build_ms_1t": round(cpu_build_ms, 1)}
+        log(f"CPU oracle ({model}, {threads} threads): MBVH {cpu_mrays:.2f} Mrays/s "
+            f"(1T {cpu['mbvh_1t_mrays_s']}), BVH2 {cpu['bvh2_mrays_s']} "
+            f"(1T {cpu['bvh2_1t_mrays_s']}); parity {parity}")
+
+    c1_info = run_c1(args, dev, rt, scenes, torch, stream) if args.c1 else None
 
     batch_info, batch_ratio = None, None
     if args.batch_scenes > 0:
@@ -775,6 +971,7 @@ def main():
         "batched": batch_info,
         "large_animated": large_info,
         "differentiable": adjoint_info,
+        "plain_small_meshes": c1_info,
         "parity_vs_cpu_oracle": parity,
         "roofline": {
             "bound": "hbm",
@@ -784,6 +981,12 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "bytes_per_ray": round(b_ray, 1),
+            # the measured HBM bytes over this run's launch time: the share of
+            # HBM bandwidth the kernel really draws (most node fetches hit L2)
+            "hbm_frac": (None if traffic is None else
+                         round(traffic / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)),
+            "traffic_launch_shape": traffic_shape,
+            "bytes_per_ray_compulsory": 48,
         },
         "cpu_baseline": cpu,
     }

@@ -210,6 +210,34 @@ def pinhole_rays(width: int, height: int, eye=(0.5, 0.5, -1.5), fov_deg: float =
     return make_rays(o.astype(np.float32), d.astype(np.float32))
 
 
+def reference_camera_rays(width: int = 1000, height: int = 1000) -> np.ndarray:
+    """The camera of the reference's benchmark (examples/benchmark.rs:72-95,
+    CameraView3D::generate_ray in shared/src/lib.rs:157-164), in float32: eye
+    (0, 1.5, -100) looking along +z, pixel (x, y) through p1 + (x / width)
+    right + (y / height) up.  Its field of view is converted to radians twice
+    (tan(pi/4 / (180/pi)) = 0.0137 half-extent at distance 1), so these rays
+    form a narrow, very coherent bundle around the teapot's middle."""
+    f = np.float32
+    fov = f(np.pi / 2)
+    screen = f(np.tan(f(fov * f(0.5)) / f(180.0 / np.pi)))
+    pos = np.array([0.0, 1.5, -100.0], f)
+    center = pos + np.array([0.0, 0.0, 1.0], f)
+    up = np.array([0.0, 1.0, 0.0], f)
+    right = np.array([-1.0, 0.0, 0.0], f)  # Vec3::Z.cross(Vec3::Y)
+    aspect = f(width / height)
+    p1 = center - screen * right * aspect + screen * up
+    p2 = center + screen * right * aspect + screen * up
+    p3 = center - screen * right * aspect - screen * up
+    r_vec, u_vec = p2 - p1, p3 - p1
+    xs = np.arange(width, dtype=f) * f(1.0 / width)
+    ys = np.arange(height, dtype=f) * f(1.0 / height)
+    u, v = np.meshgrid(xs, ys)  # row-major: y outer, x inner
+    pts = p1 + u.reshape(-1, 1) * r_vec + v.reshape(-1, 1) * u_vec
+    d = (pts - pos).astype(f)
+    d /= np.sqrt((d * d).sum(1, keepdims=True)).astype(f)
+    return make_rays(np.broadcast_to(pos, d.shape).astype(f), d.astype(f))
+
+
 # ------------------------------------------------------- device ray sets --
 def pinhole_rays_torch(width: int, height: int, device, eye=(0.5, 0.5, -1.5),
                        fov_deg: float = 40.0, row0: int = 0, rows: int = None):
