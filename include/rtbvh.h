@@ -501,6 +501,25 @@ ResultCode rtbvh_gpu_occluded_mbvh(RTMbvh mbvh, const float *vertices, size_t pr
                                    const RTRay *rays, size_t ray_count, uint8_t *occluded,
                                    uint32_t flags, void *stream);
 
+/* ---- Allocation-free build ---------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops, code that must not call hipMalloc): the PLOC build of
+ * rtbvh_gpu_create_bvh (LocallyOrderedClustered; the same nodes and indices,
+ * bit for bit) into caller arrays, its workspace taken from a static buffer of
+ * RTBVH_NOALLOC_SCRATCH_BYTES per device that the library holds until the
+ * process ends.  aabbs (nullable), centers (stride 12 or 16): device, as
+ * rtbvh_gpu_create_bvh; nodes: device, at least max(1, 2 * prim_count - 1)
+ * RTBvhNode records (for prim_count <= 2 the tree is the single root leaf in
+ * nodes[0]); indices: device, prim_count entries.  prim_count 0 ->
+ * NoPrimitives; above RTBVH_NOALLOC_MAX_PRIMS -> InvalidArgument.  Calls no
+ * hipMalloc/hipFree.  Synchronises `stream` (host reads between clustering
+ * batches).  Calls that overlap on one device race on the shared buffer. */
+#define RTBVH_NOALLOC_MAX_PRIMS 262144u
+#define RTBVH_NOALLOC_SCRATCH_BYTES (40u << 20)
+ResultCode rtbvh_gpu_create_bvh_noalloc(const RTAabb *aabbs, size_t prim_count,
+                                        const float *centers, size_t center_stride,
+                                        RTBvhNode *nodes, uint32_t *indices, void *stream);
+
 /* Synchronises `stream`, then returns and clears the device status of the
  * current device: Ok, or StackOverflow when a traversal ran out of stack
  * since the last call. */

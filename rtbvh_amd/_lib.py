@@ -26,6 +26,8 @@ TRACE_REFERENCE_ORDER = 0x1
 MISS = 0xFFFFFFFF
 MAX_STACK = 256
 MAX_BATCH_SCENES = 4096
+NOALLOC_MAX_PRIMS = 262144
+NOALLOC_SCRATCH_BYTES = 40 << 20
 
 
 class RTAabb(ctypes.Structure):
@@ -129,6 +131,7 @@ SIGNATURES = {
     "rtbvh_gpu_free_batch": (None, [RTBatch]),
     "rtbvh_gpu_occluded": (_rc, [RTBvh, _vp, _sz, _vp, _sz, _vp, _u32, _vp]),
     "rtbvh_gpu_occluded_mbvh": (_rc, [RTMbvh, _vp, _sz, _vp, _sz, _vp, _u32, _vp]),
+    "rtbvh_gpu_create_bvh_noalloc": (_rc, [_vp, _sz, _vp, _sz, _vp, _vp, _vp]),
     "rtbvh_gpu_status": (_rc, [_vp]),
     "rtbvh_gpu_free_bvh": (None, [RTBvh]),
     "rtbvh_gpu_free_mbvh": (None, [RTMbvh]),

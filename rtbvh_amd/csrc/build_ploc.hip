@@ -900,9 +900,8 @@ void free_plan(PlocPlanImpl* p, bool keep_tree) {
     delete p;
 }
 
-ResultCode alloc_plan(uint32_t n, uint32_t scenes, bool point_aabbs, size_t stride_floats,
-                      PlocPlanImpl** out) {
-    PlocPlanImpl* p = new PlocPlanImpl();
+void init_plan(PlocPlanImpl* p, uint32_t n, uint32_t scenes, bool point_aabbs,
+               size_t stride_floats) {
     p->n = n;
     p->scenes = scenes;
     p->seg = scenes > 1;
@@ -913,13 +912,19 @@ ResultCode alloc_plan(uint32_t n, uint32_t scenes, bool point_aabbs, size_t stri
     p->parts = p->seg ? 16u : 256u;  // bounds partials (per scene when batched)
     p->count_entries = 256u * p->sort_tiles;
     p->scan_bytes = scan_scratch_bytes(p->count_entries);
+}
+
+// Carves the build's workspace out of `base` (when not null) and returns its
+// size in bytes.
+size_t carve_plan(PlocPlanImpl* p, char* base) {
+    const uint32_t n = p->n, scenes = p->scenes;
     size_t off = 0;
     auto carve = [&](size_t bytes) {
         const size_t o = off;
         off += align_up(bytes);
         return o;
     };
-    const size_t o_aabb = point_aabbs ? carve(size_t(n) * sizeof(DevAabb)) : 0;
+    const size_t o_aabb = p->point_aabbs ? carve(size_t(n) * sizeof(DevAabb)) : 0;
     const size_t o_part = carve(size_t(p->parts) * scenes * sizeof(Bounds6));
     const size_t o_world = carve(size_t(scenes) * sizeof(Bounds6));
     const size_t o_k0 = carve(size_t(n) * 4), o_k1 = carve(size_t(n) * 4);
@@ -936,15 +941,9 @@ ResultCode alloc_plan(uint32_t n, uint32_t scenes, bool point_aabbs, size_t stri
     const size_t o_cs0 = p->seg ? carve(size_t(n) * 4) : 0;
     const size_t o_cs1 = p->seg ? carve(size_t(n) * 4) : 0;
     const size_t o_fail = carve(4);
-    hipError_t e;
-    if ((e = hipMalloc(&p->nodes, size_t(p->node_count()) * sizeof(DevAabb))) != hipSuccess ||
-        (e = hipMalloc(&p->indices, size_t(n) * sizeof(uint32_t))) != hipSuccess ||
-        (e = hipMalloc(&p->ws, off)) != hipSuccess) {
-        free_plan(p, false);
-        return hip_fail(e, "hipMalloc(ploc)");
-    }
-    char* b = static_cast<char*>(p->ws);
-    p->in_aabbs = point_aabbs ? reinterpret_cast<DevAabb*>(b + o_aabb) : nullptr;
+    if (base == nullptr) return off;
+    char* b = base;
+    p->in_aabbs = p->point_aabbs ? reinterpret_cast<DevAabb*>(b + o_aabb) : nullptr;
     p->part = reinterpret_cast<Bounds6*>(b + o_part);
     p->world = reinterpret_cast<Bounds6*>(b + o_world);
     p->k0 = reinterpret_cast<uint32_t*>(b + o_k0);
@@ -963,6 +962,22 @@ ResultCode alloc_plan(uint32_t n, uint32_t scenes, bool point_aabbs, size_t stri
     p->cs0 = p->seg ? reinterpret_cast<uint32_t*>(b + o_cs0) : nullptr;
     p->cs1 = p->seg ? reinterpret_cast<uint32_t*>(b + o_cs1) : nullptr;
     p->fail = reinterpret_cast<uint32_t*>(b + o_fail);
+    return off;
+}
+
+ResultCode alloc_plan(uint32_t n, uint32_t scenes, bool point_aabbs, size_t stride_floats,
+                      PlocPlanImpl** out) {
+    PlocPlanImpl* p = new PlocPlanImpl();
+    init_plan(p, n, scenes, point_aabbs, stride_floats);
+    const size_t bytes = carve_plan(p, nullptr);
+    hipError_t e;
+    if ((e = hipMalloc(&p->nodes, size_t(p->node_count()) * sizeof(DevAabb))) != hipSuccess ||
+        (e = hipMalloc(&p->indices, size_t(n) * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&p->ws, bytes)) != hipSuccess) {
+        free_plan(p, false);
+        return hip_fail(e, "hipMalloc(ploc)");
+    }
+    carve_plan(p, static_cast<char*>(p->ws));
     *out = p;
     return Ok;
 }
@@ -1144,23 +1159,29 @@ ResultCode run_eager(PlocPlanImpl* p, const RTAabb* aabbs, const float* centers,
     return Ok;
 }
 
-// n <= 2, single scene: one root leaf with the world bounds.
+// n <= 2, single scene: one root leaf with the world bounds, into p->nodes
+// and p->indices.
+hipError_t enqueue_small(const PlocPlanImpl* p, const RTAabb* aabbs, const float* centers,
+                         hipStream_t stream) {
+    const DevAabb* in_aabbs = aabbs ? reinterpret_cast<const DevAabb*>(aabbs) : p->in_aabbs;
+    if (!aabbs)
+        hipLaunchKernelGGL(point_aabbs_kernel, dim3(1), dim3(kBlock), 0, stream, centers,
+                           p->stride, p->n, p->in_aabbs);
+    hipLaunchKernelGGL(bounds_partial_kernel, dim3(1), dim3(kBlock), 0, stream, in_aabbs, p->n,
+                       p->part);
+    hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(kBlock), 0, stream, p->part, 1u,
+                       p->world);
+    hipLaunchKernelGGL(small_root_kernel, dim3(1), dim3(1), 0, stream, p->world, p->n, p->nodes,
+                       p->indices);
+    return hipGetLastError();
+}
+
 ResultCode build_small(const RTAabb* aabbs, const float* centers, size_t stride_floats,
                        uint32_t n, hipStream_t stream, DeviceBvh* out) {
     PlocPlanImpl* p = nullptr;
     ResultCode rc = alloc_plan(n, 1, aabbs == nullptr, stride_floats, &p);
     if (rc != Ok) return rc;
-    const DevAabb* in_aabbs = aabbs ? reinterpret_cast<const DevAabb*>(aabbs) : p->in_aabbs;
-    if (!aabbs)
-        hipLaunchKernelGGL(point_aabbs_kernel, dim3(1), dim3(kBlock), 0, stream, centers,
-                           p->stride, n, p->in_aabbs);
-    hipLaunchKernelGGL(bounds_partial_kernel, dim3(1), dim3(kBlock), 0, stream, in_aabbs, n,
-                       p->part);
-    hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(kBlock), 0, stream, p->part, 1u,
-                       p->world);
-    hipLaunchKernelGGL(small_root_kernel, dim3(1), dim3(1), 0, stream, p->world, n, p->nodes,
-                       p->indices);
-    hipError_t e = hipGetLastError();
+    hipError_t e = enqueue_small(p, aabbs, centers, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) {
         free_plan(p, false);
@@ -1268,6 +1289,34 @@ ResultCode gpu_build_ploc(const RTAabb* aabbs, const float* centers, size_t stri
         return InvalidArgument;
     }
     return build_eager(aabbs, centers, stride_floats, uint32_t(n64), nullptr, 1u, stream, out);
+}
+
+size_t ploc_workspace_bytes(size_t n, bool point_aabbs) {
+    PlocPlanImpl p;
+    init_plan(&p, uint32_t(n), 1, point_aabbs, 3);
+    return carve_plan(&p, nullptr);
+}
+
+ResultCode gpu_build_ploc_into(const RTAabb* aabbs, const float* centers, size_t stride_floats,
+                               uint32_t n, void* ws, size_t ws_bytes, RTBvhNode* nodes,
+                               uint32_t* indices, hipStream_t stream) {
+    PlocPlanImpl p;  // every buffer is the caller's: nothing to free
+    init_plan(&p, n, 1, aabbs == nullptr, stride_floats);
+    const size_t need = carve_plan(&p, nullptr);
+    if (need > ws_bytes) {
+        set_error("the build of %u primitives needs %zu bytes of workspace (%zu given)", n, need,
+                  ws_bytes);
+        return InvalidArgument;
+    }
+    carve_plan(&p, static_cast<char*>(ws));
+    p.nodes = reinterpret_cast<DevAabb*>(nodes);
+    p.indices = indices;
+    if (n <= 2) {
+        hipError_t e = enqueue_small(&p, aabbs, centers, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        return e == hipSuccess ? Ok : hip_fail(e, "small build");
+    }
+    return run_eager<false>(&p, aabbs, centers, nullptr, stream, nullptr);
 }
 
 ResultCode check_batch_offsets(const uint32_t* scene_off, uint32_t scenes) {

@@ -110,6 +110,14 @@ ResultCode gpu_build_ploc_batch(const RTAabb* aabbs, const float* centers, size_
                                 const uint32_t* scene_off, uint32_t scenes, hipStream_t stream,
                                 DeviceBvh* out);
 ResultCode check_batch_offsets(const uint32_t* scene_off, uint32_t scenes);
+// Workspace bytes of one single-scene PLOC build of n primitives.
+size_t ploc_workspace_bytes(size_t n, bool point_aabbs);
+// The same build as gpu_build_ploc (radius 14, 1 <= n) into caller arrays
+// (nodes: max(1, 2n - 1) records, indices: n) with a caller workspace of
+// ws_bytes; allocates nothing.  Synchronises the stream.
+ResultCode gpu_build_ploc_into(const RTAabb* aabbs, const float* centers, size_t stride_floats,
+                               uint32_t n, void* ws, size_t ws_bytes, RTBvhNode* nodes,
+                               uint32_t* indices, hipStream_t stream);
 
 // Capturable PLOC build: every buffer allocated up front, a fixed number of
 // clustering iterations (sized by one eager build at creation), no host
