@@ -137,6 +137,9 @@ __device__ __forceinline__ MtResult mt_prepared(const float4* tris, uint32_t pos
     const float4 a = tris[size_t(pos) * kTriRecord];
     const float4 e1 = tris[size_t(pos) * kTriRecord + 1];
     const float4 e2 = tris[size_t(pos) * kTriRecord + 2];
+    // all 48 bytes in one round trip: without this use the compiler moves the
+    // v0 load behind the determinant test, a second dependent memory access
+    asm volatile("" : : "v"(a.x), "v"(a.y), "v"(a.z));
     *prim = __float_as_uint(a.w);
     return mt_candidate_e(a.x, a.y, a.z, e1.x, e1.y, e1.z, e2.x, e2.y, e2.z, ox, oy, oz, dx, dy,
                           dz, t_min);

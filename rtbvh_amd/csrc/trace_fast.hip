@@ -894,7 +894,10 @@ cw8_kernel(TraceParams p) {
                 const uint32_t fz0 = nz ? qb.x : qc.z, fz1 = nz ? qb.y : qc.w;
                 const uint32_t imask = h0.w >> 24, lmask = h1.z & 0xFFu;
                 uint32_t hm = 0;
-                float gmin = 3.0e38f;  // nearest entry among the interior hits
+                // nearest entry among all hit slots: a lower bound for the interior
+                // hits that costs one select per slot (the exact minimum over the
+                // interior slots alone took a mask test per slot and pruned no more)
+                float gmin = 3.0e38f;
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     const float tnx = fmaf(qbyte(i < 4 ? nx0 : nx1, i), sx, bxn);
@@ -906,7 +909,7 @@ cw8_kernel(TraceParams p) {
                     const float en = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
                     const bool hit = en <= fminf(fminf(fminf(tfx, tfy), tfz), bt);
                     hm |= hit ? (1u << i) : 0u;
-                    gmin = fminf(gmin, (hit && ((imask >> i) & 1u)) ? en : 3.0e38f);
+                    gmin = fminf(gmin, hit ? en : 3.0e38f);
                 }
                 const uint32_t ih = hm & imask, th = hm & lmask;
                 if (ih) {
