@@ -20,51 +20,103 @@
 // inside multi-primitive leaves may differ; the sets, the topology and the
 // bounds do not.  Nodes are numbered breadth-first (the reference's numbering
 // depends on thread timing, builders/mod.rs:59-76).
+//
+// A level is four kernels.  Its task count, first node id and number of
+// binned tasks sit in a device record that the level before writes, so the
+// host enqueues levels in batches and reads one record per batch:
+//   bin_kernel        primitives of tasks with more than 16 into the task's
+//                     48 bins (LDS atomics per block, then one global merge)
+//   decide_kernel     per task the SAH decision (a 16-lane group per task)
+//   children_kernel   per tile of 1024 tasks the splitting tasks and binned
+//                     children before the tile (a decoupled look-back over
+//                     tile words), then node records, child tasks and the
+//                     next level's record
+//   partition_kernel  the stable partition of every splitting task's range:
+//                     ranks of left primitives from a segmented scan over
+//                     tiles of 1024 primitives (decoupled look-back across
+//                     tiles of one task); the primitives of a leaf are
+//                     written to both index buffers and drop out
+// Look-back waits are bounded: tile ids come from a counter taken by running
+// blocks, a tile waits only on lower ids, and every tile publishes a word
+// before it waits on any.
 #include <algorithm>
 
 #include "internal.hpp"
 
 namespace rtbvh {
 
-hipError_t scan_exclusive_u32(uint32_t* data, uint32_t count, uint32_t* total, void* scratch,
-                              size_t scratch_bytes, hipStream_t stream);
-size_t scan_scratch_bytes(uint32_t count);
-
 namespace {
 
-constexpr int kBins = 16;          // binned_sah.rs:314
+constexpr int kBins = 16;           // binned_sah.rs:314
 constexpr uint32_t kMaxDepth = 64;  // binned_sah.rs:313
+// Levels that hold tasks: depths 0..64 (a task at depth 64 is a leaf).
+constexpr uint32_t kMaxLevels = kMaxDepth + 1;
 constexpr int kBlock = 256;
 // Tasks of at most kSmallTask primitives are binned by their split group
-// (one lane per primitive) instead of by bin_kernel's atomics.
-constexpr uint32_t kSmallTask = 16;
+// (through LDS) instead of by bin_kernel's atomics.
+constexpr uint32_t kSmallTask = 64;
+// The first kHotSlots slots (the levels near the root, where every block of
+// bin_kernel merges into the same few bins) have kCopies copies, one per
+// XCD's blocks, so that fewer atomics queue on one address; the decision
+// folds the copies.
+constexpr uint32_t kHotSlots = 64;
+constexpr uint32_t kCopies = 8;
+constexpr uint32_t kDecideGrid = 4096;      // most decide_kernel blocks (grid stride)
+constexpr uint32_t kTaskTile = 1024;        // children_kernel: tasks per tile, one per thread
+constexpr uint32_t kChildGrid = 512;        // most children_kernel blocks
+constexpr int kPartBlock = 1024;            // partition_kernel
+constexpr uint32_t kChunks = 4;             // partition tile = kChunks runs of kPartBlock
+constexpr uint32_t kPrimTile = kChunks * kPartBlock;
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+constexpr uint32_t kLeaf = 0xFFFFFFFFu;
 
+// A task of one level: its node's padded box and bin parameters.
 struct BTask {
-    float mn[3];
-    uint32_t node;
+    float mn[3];     // padded node box, binned_sah.rs:133
+    uint32_t begin;  // primitives [begin, end) of the index array
     float mx[3];
-    uint32_t begin;
     uint32_t end;
+    float ctb[3];    // (1 / extent) * 16, :146
     uint32_t depth;
-    uint32_t pad[2];
+    float bo[3];     // -min * ctb, :147
+    uint32_t slot;   // bins of bin_kernel, or kNoSlot
 };
 
-struct BinAcc {
-    uint32_t cnt;
-    uint32_t mn[3];  // order-preserving encodings of floats
-    uint32_t mx[3];
+// Device record of one level, written by the level before (the root's by
+// root_bounds_kernel); records past the last level hold no tasks and the
+// final node count.
+struct BLevel {
+    uint32_t num_tasks;
+    uint32_t node_base;     // task t is node node_base + t
+    uint32_t slots;         // tasks binned by bin_kernel: slots [0, slots)
+    uint32_t split_ticket;  // children_kernel's next tile
+    uint32_t part_ticket;   // partition_kernel's next tile
+    uint32_t pad[3];
+};
+
+// What partition_kernel needs of a task.
+struct BPart {
+    uint32_t begin;
+    uint32_t left_count;
+    uint32_t axis;  // kLeaf: the task is a leaf
+    uint32_t split_index;
+    float ctb, bo;   // of the split axis
+    uint32_t child;  // next-level task of the left child; the right one is child + 1
     uint32_t pad;
 };
 
-struct BDecision {
-    float mn[3];      // padded node box
-    uint32_t split;   // 1 = split, 0 = leaf
-    float mx[3];
-    uint32_t axis;
-    float ctb[3];
-    uint32_t split_index;
-    float bo[3];
-    uint32_t left_count;
+// One bin, all zero when empty: the count and the bounds as order-preserving
+// encodings, the minimum complemented so that both fold by maximum.
+struct BinAcc {
+    uint32_t cnt;
+    uint32_t mn[3];  // ~fenc(min)
+    uint32_t mx[3];  // fenc(max)
+    uint32_t pad;
+};
+
+// A task's decision, held in LDS until the tile's prefix is known.
+struct SDec {
+    uint32_t split, axis, split_index, left_count;
     float lmn[3], lmx[3], rmn[3], rmx[3];
 };
 
@@ -74,6 +126,16 @@ __device__ __forceinline__ uint32_t fenc(float f) {
 }
 __device__ __forceinline__ float fdec(uint32_t e) {
     return __uint_as_float((e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e);
+}
+// Aabb::empty (1e34, -1e34) joined with the bounds folded into a word
+// (min and max over the encodings, NaN bounds never folded in).
+__device__ __forceinline__ float dec_min(uint32_t w) {
+    const float v = fdec(~w);
+    return (w != 0 && v < 1e34f) ? v : 1e34f;
+}
+__device__ __forceinline__ float dec_max(uint32_t w) {
+    const float v = fdec(w);
+    return (w != 0 && v > -1e34f) ? v : -1e34f;
 }
 
 __device__ __forceinline__ uint32_t bin_index(float c, float ctb, float bo) {
@@ -95,116 +157,263 @@ __device__ __forceinline__ float half_area6(const float* mn, const float* mx) {
     return (d0 + d1) * d2 + d0 * d1;
 }
 
-// Per task: padded box, bin parameters, cleared bins.
-__global__ void level_init_kernel(const BTask* tasks, uint32_t num_tasks, BDecision* dec,
-                                  BinAcc* bins) {
-    const uint32_t t = blockIdx.x;
-    if (t >= num_tasks) return;
-    const BTask tk = tasks[t];
-    if (threadIdx.x == 0) {
-        BDecision d;
-        for (int k = 0; k < 3; ++k) {
-            d.mn[k] = tk.mn[k] - 0.0001f;  // binned_sah.rs:133
-            d.mx[k] = tk.mx[k] + 0.0001f;
-            d.ctb[k] = (1.0f / (d.mx[k] - d.mn[k])) * float(kBins);  // :146
-            d.bo[k] = (-d.mn[k]) * d.ctb[k];                        // :147
-        }
-        d.split = 0;
-        d.axis = 0;
-        d.split_index = 0;
-        d.left_count = 0;
-        dec[t] = d;
-    }
-    if (threadIdx.x < 3 * kBins && tk.end - tk.begin > kSmallTask) {
-        BinAcc a;
-        a.cnt = 0;
-        for (int k = 0; k < 3; ++k) {
-            a.mn[k] = fenc(1e34f);  // Aabb::empty
-            a.mx[k] = fenc(-1e34f);
-        }
-        a.pad = 0;
-        bins[size_t(t) * 3 * kBins + threadIdx.x] = a;
-    }
+// a[i] for i in 0..2 by selects (an indexed register array would go to
+// scratch memory)
+template <typename T>
+__device__ __forceinline__ T pick3(const T (&a)[3], uint32_t i) {
+    return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]);
 }
 
-// Bin every primitive of every task with more than kSmallTask primitives
-// (smaller tasks with two or more are binned in split_kernel).  The live task
-// ids along the index array never decrease (the scatter places each split
-// task's children, left then right, inside the parent's range, and tasks are
-// numbered in range order); finished primitives carry -1 and sit between
-// them.  A block whose live primitives span at most kLdsTasks tasks
-// accumulates in LDS and merges one set of atomics per task and bin.
+__device__ __forceinline__ bool binned_by_atomics(uint32_t count, uint32_t depth) {
+    return count > kSmallTask && depth < kMaxDepth;
+}
+
+// The 48 bins of a slot (copy: the caller's copy of a hot slot).
+__device__ __forceinline__ BinAcc* slot_bins(BinAcc* bins, BinAcc* hot, uint32_t slot,
+                                             uint32_t copy) {
+    return slot < kHotSlots ? hot + (size_t(slot) * kCopies + copy) * 3 * kBins
+                            : bins + size_t(slot) * 3 * kBins;
+}
+
+// A task from its unpadded box: the padding and bin parameters of
+// binned_sah.rs:133,146-147.
+__device__ __forceinline__ BTask make_task(const float* mn, const float* mx, uint32_t begin,
+                                          uint32_t end, uint32_t depth, uint32_t slot) {
+    BTask t;
+    for (int k = 0; k < 3; ++k) {
+        t.mn[k] = mn[k] - 0.0001f;
+        t.mx[k] = mx[k] + 0.0001f;
+        t.ctb[k] = (1.0f / (t.mx[k] - t.mn[k])) * float(kBins);
+        t.bo[k] = (-t.mn[k]) * t.ctb[k];
+    }
+    t.begin = begin;
+    t.end = end;
+    t.depth = depth;
+    t.slot = slot;
+    return t;
+}
+
+// Row (16-lane) moves by DPP: a group of 16 lanes is one DPP row, so the
+// scans and reductions of a group stay in the VALU.  row_shr:k gives lane i
+// the value of lane i - k, row_shl:k that of lane i + k (lanes without a
+// source keep their own value, which the callers never use);
+// quad_perm [1,0,3,2] and [2,3,0,1] pair lanes i ^ 1 and i ^ 2,
+// row_half_mirror lanes i and 7 - i of each half, row_mirror i and 15 - i.
+constexpr int kDppShl = 0x100, kDppShr = 0x110, kDppXor1 = 0xB1, kDppXor2 = 0x4E,
+              kDppHalfMirror = 0x141, kDppMirror = 0x140;
+
+template <int kCtrl>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_update_dpp(int(v), int(v), kCtrl, 0xF, 0xF, false));
+}
+template <int kCtrl>
+__device__ __forceinline__ float dpp_f32(float v) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), kCtrl, 0xF, 0xF, false));
+}
+// Maximum and minimum over a row of 16 lanes (every lane of the row gets it).
+__device__ __forceinline__ uint32_t row_max(uint32_t v) {
+    v = max(v, dpp_u32<kDppXor1>(v));
+    v = max(v, dpp_u32<kDppXor2>(v));
+    v = max(v, dpp_u32<kDppHalfMirror>(v));
+    return max(v, dpp_u32<kDppMirror>(v));
+}
+__device__ __forceinline__ uint32_t row_min(uint32_t v) {
+    v = min(v, dpp_u32<kDppXor1>(v));
+    v = min(v, dpp_u32<kDppXor2>(v));
+    v = min(v, dpp_u32<kDppHalfMirror>(v));
+    return min(v, dpp_u32<kDppMirror>(v));
+}
+
+// Look-back tile words: bits 63:62 the status (0 not yet written, 1 the
+// tile's own aggregate, 2 the inclusive prefix through the tile), the rest
+// the payload.  Agent-scope atomics: tiles run on every XCD.
+constexpr uint64_t kAggregate = 1ull << 62;
+constexpr uint64_t kInclusive = 2ull << 62;
+constexpr uint64_t kPayload = (1ull << 62) - 1;
+
+__device__ __forceinline__ void publish(uint64_t* w, uint64_t v) {
+    __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t wait_word(uint64_t* w) {
+    uint64_t v;
+    while (((v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 62) == 0)
+        __builtin_amdgcn_s_sleep(1);
+    return v;
+}
+
+// A block's next tile of a persistent loop.
+__device__ __forceinline__ uint32_t next_tile(uint32_t* ticket, uint32_t* s_tile) {
+    if (threadIdx.x == 0) *s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    return *s_tile;
+}
+
+// Wave-wide sum (all 64 lanes take part).
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Exclusive prefix of tile `tile`: the payloads of the words before it down
+// to the nearest inclusive one, by the 64 lanes of one wave (lane k looks at
+// word end - 1 - k of each window of 64).
+__device__ uint64_t lookback(uint64_t* words, uint32_t tile, uint32_t lane) {
+    uint64_t excl = 0;
+    int64_t end = tile;
+    while (end > 0) {
+        const int64_t p = end - 1 - int64_t(lane);
+        const uint64_t w = p >= 0 ? wait_word(&words[p]) : kInclusive;
+        const uint64_t inc = __ballot((w >> 62) == 2);
+        const uint32_t first = inc ? uint32_t(__builtin_ctzll(inc)) : 64u;
+        excl += wave_sum(lane <= first && p >= 0 ? (w & kPayload) : 0);
+        if (inc) break;
+        end -= 64;
+    }
+    return excl;
+}
+
+// Bin every primitive of every task with a bin slot (more than kSmallTask
+// primitives; smaller tasks with two or more are binned in decide_kernel).
+// Slots are numbered in task order and tasks in range order, so the slots
+// met along the index array never decrease; finished primitives carry -1
+// and sit between them.  A block walks its run of the index array in chunks
+// of kBlock and accumulates in an LDS window of kLdsTasks slots, merged into
+// the global bins (one set of atomics per slot and bin) when a chunk reaches
+// past it and at the end; a chunk that alone spans more slots than the
+// window goes to the global bins directly.
 constexpr int kLdsTasks = 16;  // 4: 41 ms spatial, 22 ms binned at 1M; 16: 36, 19.5; 32: no better
+constexpr uint32_t kBinGrid = 1024;  // blocks of bin_kernel (runs of n / 1024)
+constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 __global__ void __launch_bounds__(kBlock)
-    bin_kernel(uint32_t n, const int32_t* ptask, const uint32_t* idx, const BTask* tasks,
-               const BDecision* dec, const float* centers, uint32_t stride,
-               const DevAabb* aabbs, BinAcc* bins) {
+    bin_kernel(const BLevel* levels, uint32_t level, uint32_t n, uint32_t run,
+               const int32_t* ptask, const uint32_t* idx, const BTask* tasks,
+               const float* centers, uint32_t stride, const DevAabb* aabbs, BinAcc* bins,
+               BinAcc* hot) {
+    if (levels[level].slots == 0) return;
     constexpr int kSlots = kLdsTasks * 3 * kBins;
     __shared__ uint32_t s_cnt[kSlots];
     __shared__ uint32_t s_mn[kSlots][3];
     __shared__ uint32_t s_mx[kSlots][3];
-    __shared__ int32_t s_lo, s_hi;
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (threadIdx.x == 0) {
-        s_lo = 0x7fffffff;
-        s_hi = -1;
-    }
-    for (uint32_t k = threadIdx.x; k < uint32_t(kSlots); k += kBlock) {
-        s_cnt[k] = 0;
-        for (int q = 0; q < 3; ++q) {
-            s_mn[k][q] = fenc(1e34f);
-            s_mx[k][q] = fenc(-1e34f);
+    // first and last slot of each wave's slotted lanes, two chunks in turn
+    __shared__ uint32_t s_first[2][kBlock / 64], s_last[2][kBlock / 64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    auto clear = [&]() {
+        for (uint32_t k = threadIdx.x; k < uint32_t(kSlots); k += kBlock) {
+            s_cnt[k] = 0;
+            for (int q = 0; q < 3; ++q) {
+                s_mn[k][q] = 0;
+                s_mx[k][q] = 0;
+            }
         }
-    }
+    };
+    const uint32_t copy = blockIdx.x % kCopies;
+    auto merge = [&](uint32_t wlo) {
+        for (uint32_t k = threadIdx.x; k < uint32_t(kSlots); k += kBlock) {
+            if (s_cnt[k] == 0) continue;
+            BinAcc* g = slot_bins(bins, hot, wlo + k / (3 * kBins), copy) + k % (3 * kBins);
+            atomicAdd(&g->cnt, s_cnt[k]);
+            for (int q = 0; q < 3; ++q) {
+                if (s_mn[k][q]) atomicMax(&g->mn[q], s_mn[k][q]);
+                if (s_mx[k][q]) atomicMax(&g->mx[q], s_mx[k][q]);
+            }
+        }
+    };
+    clear();
     __syncthreads();
-    const int32_t t = i < n ? ptask[i] : -1;
-    if (t >= 0) {
-        atomicMin(&s_lo, t);
-        atomicMax(&s_hi, t);
-    }
-    __syncthreads();
-    const int32_t lo = s_lo, hi = s_hi;
-    const bool local = hi >= 0 && hi - lo < kLdsTasks;
-    if (t >= 0) {
-        const BTask tk = tasks[t];
-        if (tk.end - tk.begin > kSmallTask && tk.depth < kMaxDepth) {
-            const BDecision& d = dec[t];
+    uint32_t wlo = kNone;  // first slot of the LDS window (kNone: empty)
+    const uint32_t b0 = blockIdx.x * run;
+    const uint32_t b1 = b0 + run < n ? b0 + run : n;
+    for (uint32_t c0 = b0; c0 < b1; c0 += kBlock) {
+        const uint32_t i = c0 + threadIdx.x;
+        const int32_t t = i < b1 ? ptask[i] : -1;
+        BTask tk;
+        uint32_t slot = kNoSlot;
+        if (t >= 0) {
+            tk = tasks[t];
+            slot = tk.slot;
+        }
+        // the chunk's slot range, from the first and last slotted lane of
+        // each wave (slots never decrease along the array)
+        const uint64_t m = __ballot(slot != kNoSlot);
+        const uint32_t f = __shfl(slot, m ? int(__builtin_ctzll(m)) : 0, 64);
+        const uint32_t l = __shfl(slot, m ? int(63 - __builtin_clzll(m)) : 0, 64);
+        const uint32_t par = ((c0 - b0) / kBlock) & 1u;
+        if (lane == 0) {
+            s_first[par][wave] = m ? f : kNone;
+            s_last[par][wave] = m ? l : kNone;
+        }
+        __syncthreads();
+        uint32_t lo = kNone, hi = kNone;
+        for (uint32_t w = 0; w < kBlock / 64; ++w) {
+            if (lo == kNone) lo = s_first[par][w];
+            if (s_last[par][w] != kNone) hi = s_last[par][w];
+        }
+        if (lo == kNone) continue;  // no slotted primitive in the chunk
+        if (wlo != kNone && hi - wlo >= uint32_t(kLdsTasks)) {
+            merge(wlo);
+            __syncthreads();
+            clear();
+            __syncthreads();
+            wlo = kNone;
+        }
+        if (wlo == kNone && hi - lo < uint32_t(kLdsTasks)) wlo = lo;
+        const bool local = wlo != kNone;  // else the chunk alone spans more than the window
+        // a primitive's bounds as bin words (0: nothing, NaN bounds skipped)
+        uint32_t emn[3] = {0, 0, 0}, emx[3] = {0, 0, 0};
+        float cc[3] = {0.0f, 0.0f, 0.0f};
+        if (slot != kNoSlot) {
             const uint32_t p = idx[i];
             const float* c = centers + size_t(p) * stride;
             const DevAabb a = aabbs[p];
-            for (int ax = 0; ax < 3; ++ax) {
-                const uint32_t b = bin_index(c[ax], d.ctb[ax], d.bo[ax]);
-                const uint32_t slot = ax * kBins + b;
+            for (int k = 0; k < 3; ++k) {
+                cc[k] = c[k];
+                if (a.min[k] == a.min[k]) emn[k] = ~fenc(a.min[k]);
+                if (a.max[k] == a.max[k]) emx[k] = fenc(a.max[k]);
+            }
+        }
+        // A row of 16 lanes whose primitives share slot and bin on an axis
+        // (neighbours in the array often do near the root) adds its row
+        // totals with one lane; other rows add per lane.
+        uint32_t rmn[3], rmx[3];
+        for (int k = 0; k < 3; ++k) {
+            rmn[k] = row_max(emn[k]);
+            rmx[k] = row_max(emx[k]);
+        }
+        const uint32_t rowm = uint32_t(m >> (lane & 48u)) & 0xFFFFu;  // slotted lanes of the row
+        const uint32_t rlead = rowm ? uint32_t(__builtin_ctz(rowm)) : 16u;
+        for (int ax = 0; ax < 3; ++ax) {
+            const uint32_t b = slot != kNoSlot ? bin_index(cc[ax], tk.ctb[ax], tk.bo[ax]) : 0u;
+            const uint32_t key = slot != kNoSlot ? (slot << 4) | b : 0u;  // slot < 2^24
+            const bool uni = rowm != 0 &&
+                             row_min(slot != kNoSlot ? key : 0xFFFFFFFFu) == row_max(key);
+            const bool one = uni ? (lane & 15u) == rlead : slot != kNoSlot;
+            if (one) {
+                const uint32_t add = uni ? uint32_t(__builtin_popcount(rowm)) : 1u;
+                const uint32_t* wmn = uni ? rmn : emn;
+                const uint32_t* wmx = uni ? rmx : emx;
+                const uint32_t s = ax * kBins + b;
                 if (local) {
-                    const uint32_t ls = uint32_t(t - lo) * 3 * kBins + slot;
-                    atomicAdd(&s_cnt[ls], 1u);
+                    const uint32_t ls = (slot - wlo) * 3 * kBins + s;
+                    atomicAdd(&s_cnt[ls], add);
                     for (int k = 0; k < 3; ++k) {
-                        if (a.min[k] == a.min[k]) atomicMin(&s_mn[ls][k], fenc(a.min[k]));
-                        if (a.max[k] == a.max[k]) atomicMax(&s_mx[ls][k], fenc(a.max[k]));
+                        if (wmn[k]) atomicMax(&s_mn[ls][k], wmn[k]);
+                        if (wmx[k]) atomicMax(&s_mx[ls][k], wmx[k]);
                     }
                 } else {
-                    BinAcc* g = bins + size_t(t) * 3 * kBins + slot;
-                    atomicAdd(&g->cnt, 1u);
+                    BinAcc* g = slot_bins(bins, hot, slot, copy) + s;
+                    atomicAdd(&g->cnt, add);
                     for (int k = 0; k < 3; ++k) {
-                        if (a.min[k] == a.min[k]) atomicMin(&g->mn[k], fenc(a.min[k]));
-                        if (a.max[k] == a.max[k]) atomicMax(&g->mx[k], fenc(a.max[k]));
+                        if (wmn[k]) atomicMax(&g->mn[k], wmn[k]);
+                        if (wmx[k]) atomicMax(&g->mx[k], wmx[k]);
                     }
                 }
             }
         }
     }
     __syncthreads();
-    if (local)
-        for (uint32_t k = threadIdx.x; k < uint32_t(hi - lo + 1) * 3 * kBins; k += kBlock) {
-            if (s_cnt[k] == 0) continue;
-            BinAcc* g = bins + size_t(lo) * 3 * kBins + k;
-            atomicAdd(&g->cnt, s_cnt[k]);
-            for (int q = 0; q < 3; ++q) {
-                atomicMin(&g->mn[q], s_mn[k][q]);
-                atomicMax(&g->mx[q], s_mx[k][q]);
-            }
-        }
+    if (wlo != kNone) merge(wlo);
 }
 
 // A run of bins: its box and its primitive count.
@@ -225,24 +434,39 @@ __device__ __forceinline__ void gbox_join(GBox& lo, const GBox& hi) {
     lo.cnt += hi.cnt;
 }
 
-__device__ __forceinline__ GBox gbox_shfl_up(const GBox& b, int off) {
+template <int kCtrl>
+__device__ __forceinline__ GBox gbox_dpp(const GBox& b) {
     GBox r;
     for (int k = 0; k < 3; ++k) {
-        r.mn[k] = __shfl_up(b.mn[k], off, kBins);
-        r.mx[k] = __shfl_up(b.mx[k], off, kBins);
+        r.mn[k] = dpp_f32<kCtrl>(b.mn[k]);
+        r.mx[k] = dpp_f32<kCtrl>(b.mx[k]);
     }
-    r.cnt = __shfl_up(b.cnt, off, kBins);
+    r.cnt = dpp_u32<kCtrl>(b.cnt);
     return r;
 }
 
-__device__ __forceinline__ GBox gbox_shfl_down(const GBox& b, int off) {
-    GBox r;
-    for (int k = 0; k < 3; ++k) {
-        r.mn[k] = __shfl_down(b.mn[k], off, kBins);
-        r.mx[k] = __shfl_down(b.mx[k], off, kBins);
+// One step of the prefix (p, bins 0..lane) and suffix (q, bins lane..15)
+// scans of a group.
+template <int kOff>
+__device__ __forceinline__ void scan_step(GBox& p, GBox& q, int lane) {
+    GBox lower = gbox_dpp<kDppShr + kOff>(p);
+    const GBox higher = gbox_dpp<kDppShl + kOff>(q);
+    if (lane >= kOff) {
+        gbox_join(lower, p);
+        p = lower;
     }
-    r.cnt = __shfl_down(b.cnt, off, kBins);
-    return r;
+    if (lane + kOff < kBins) gbox_join(q, higher);
+}
+
+// One step of the group's (cost, bin) minimum, ties to the lower bin.
+template <int kCtrl>
+__device__ __forceinline__ void argmin_step(float& key, int& at) {
+    const float ok = dpp_f32<kCtrl>(key);
+    const int oi = int(dpp_u32<kCtrl>(uint32_t(at)));
+    if (ok < key || (ok == key && oi < at)) {
+        key = ok;
+        at = oi;
+    }
 }
 
 __device__ __forceinline__ GBox gbox_shfl(const GBox& b, int src) {
@@ -257,257 +481,456 @@ __device__ __forceinline__ GBox gbox_shfl(const GBox& b, int src) {
 
 __device__ __forceinline__ float gbox_half_area(const GBox& b) { return half_area6(b.mn, b.mx); }
 
-// Per task: the best SAH plane per axis (find_split, binned_sah.rs:80-114),
-// the axis, the fallback and the child boxes (:150-230).  One group of 16
-// lanes per task, lane i holding bin i: inclusive prefix (bins 0..i) and
-// suffix (bins i..15) runs by shuffle scans, the cost of the plane after bin
-// i as the one-thread sweep computes it, half_area(left) * left count +
-// half_area(right) * right count, and the first lowest cost below FLT_MAX
-// by a reduction.  The bins' bounds hold no NaN (the binning skips NaN
-// bounds), so every bound equals the sequential fold's, up to the sign of a
-// zero in the right runs of the sweep, which no comparison tells apart; the
-// child boxes come from increasing-order runs, as the sweep builds them.
-__global__ void __launch_bounds__(kBlock)
-    split_kernel(const BTask* tasks, uint32_t num_tasks, BDecision* dec, const BinAcc* bins,
-                 uint32_t max_leaf, uint32_t* split_flag, const uint32_t* idx,
-                 const float* centers, uint32_t stride, const DevAabb* aabbs) {
-    const uint32_t t = (blockIdx.x * kBlock + threadIdx.x) / kBins;
-    const int lane = int(threadIdx.x % kBins);
-    if (t >= num_tasks) return;  // whole groups: kBlock is a multiple of kBins
-    const BTask tk = tasks[t];
-    BDecision d = dec[t];
-    const uint32_t work = tk.end - tk.begin;
+// The decision for one task by its group of 16 lanes (every lane returns
+// it): the best SAH plane per axis (find_split, binned_sah.rs:80-114), the
+// axis, the fallback and the child boxes (:150-230).  Lane i holds bin i:
+// inclusive prefix (bins 0..i) and suffix (bins i..15) runs by DPP row
+// scans, the cost of the plane after bin i as the one-thread sweep computes
+// it, half_area(left) * left count + half_area(right) * right count, and the
+// first lowest cost below FLT_MAX by a reduction.  The bins' bounds hold no
+// NaN (the binning skips NaN bounds), so every bound equals the sequential
+// fold's, up to the sign of a zero in the right runs of the sweep, which no
+// comparison tells apart; the child boxes come from increasing-order runs,
+// as the sweep builds them.  A task's bins are left empty for the slot's
+// next owner.
+__device__ __forceinline__ SDec decide(const BTask& tk, int lane, BinAcc* bins, BinAcc* hot,
+                                       uint32_t max_leaf, const uint32_t* idx,
+                                       const float* centers, uint32_t stride,
+                                       const DevAabb* aabbs, uint4* gather) {
+    SDec d;
     d.split = 0;
-    if (work > 1 && tk.depth < kMaxDepth) {
-        const BinAcc* tb = bins + size_t(t) * 3 * kBins;
-        BinAcc sb[3];  // this lane's bin of each axis, for a small task
-        if (work <= kSmallTask) {
-            // bin_kernel's accumulation without atomics: lane q holds
-            // primitive q, lane j gathers bin j; the minimum and maximum of
-            // the same ordered encodings, NaN bounds skipped
-            uint32_t pb[3] = {0, 0, 0}, pmn[3], pmx[3];
-            for (int k = 0; k < 3; ++k) {
-                pmn[k] = fenc(1e34f);
-                pmx[k] = fenc(-1e34f);
-            }
-            if (uint32_t(lane) < work) {
-                const uint32_t p = idx[tk.begin + lane];
-                const float* c = centers + size_t(p) * stride;
-                const DevAabb a = aabbs[p];
-                for (int k = 0; k < 3; ++k) {
-                    pb[k] = bin_index(c[k], d.ctb[k], d.bo[k]);
-                    if (a.min[k] == a.min[k]) pmn[k] = fenc(a.min[k]);
-                    if (a.max[k] == a.max[k]) pmx[k] = fenc(a.max[k]);
-                }
-            }
-            for (int ax = 0; ax < 3; ++ax) {
-                sb[ax].cnt = 0;
-                for (int k = 0; k < 3; ++k) {
-                    sb[ax].mn[k] = fenc(1e34f);
-                    sb[ax].mx[k] = fenc(-1e34f);
-                }
-            }
-            for (uint32_t q = 0; q < work; ++q) {
-                uint32_t qb[3], qmn[3], qmx[3];
-                for (int k = 0; k < 3; ++k) {
-                    qb[k] = __shfl(pb[k], int(q), kBins);
-                    qmn[k] = __shfl(pmn[k], int(q), kBins);
-                    qmx[k] = __shfl(pmx[k], int(q), kBins);
-                }
-                for (int ax = 0; ax < 3; ++ax)
-                    if (qb[ax] == uint32_t(lane)) {
-                        ++sb[ax].cnt;
-                        for (int k = 0; k < 3; ++k) {
-                            sb[ax].mn[k] = min(sb[ax].mn[k], qmn[k]);
-                            sb[ax].mx[k] = max(sb[ax].mx[k], qmx[k]);
-                        }
-                    }
-            }
+    d.axis = 0;
+    d.split_index = 0;
+    d.left_count = 0;
+    const uint32_t work = tk.end - tk.begin;
+    if (!(work > 1 && tk.depth < kMaxDepth)) return d;
+    BinAcc own[3];  // this lane's bin of each axis
+    if (tk.slot == kNoSlot) {
+        // at most kSmallTask primitives: bin_kernel's accumulation without
+        // atomics; the group stages its primitives in LDS (encoded minimum
+        // with the three bins packed in w, encoded maximum) and lane j
+        // gathers bin j
+        for (uint32_t q = uint32_t(lane); q < work; q += kBins) {
+            const uint32_t p = idx[tk.begin + q];
+            const float* c = centers + size_t(p) * stride;
+            const DevAabb a = aabbs[p];
+            uint4 pmn = make_uint4(0, 0, 0, 0), pmx = pmn;
+            pmn.w = bin_index(c[0], tk.ctb[0], tk.bo[0]) |
+                    (bin_index(c[1], tk.ctb[1], tk.bo[1]) << 4) |
+                    (bin_index(c[2], tk.ctb[2], tk.bo[2]) << 8);
+            if (a.min[0] == a.min[0]) pmn.x = ~fenc(a.min[0]);
+            if (a.min[1] == a.min[1]) pmn.y = ~fenc(a.min[1]);
+            if (a.min[2] == a.min[2]) pmn.z = ~fenc(a.min[2]);
+            if (a.max[0] == a.max[0]) pmx.x = fenc(a.max[0]);
+            if (a.max[1] == a.max[1]) pmx.y = fenc(a.max[1]);
+            if (a.max[2] == a.max[2]) pmx.z = fenc(a.max[2]);
+            gather[2 * q] = pmn;
+            gather[2 * q + 1] = pmx;
         }
-        GBox pre[3], suf[3];
-        float cost[3];
-        uint32_t cnt[3];
+        // the group's lanes are one wave's: its LDS operations run in order
+        __builtin_amdgcn_wave_barrier();
         for (int ax = 0; ax < 3; ++ax) {
-            const BinAcc a = work <= kSmallTask ? sb[ax] : tb[ax * kBins + lane];
-            GBox own;
+            own[ax].cnt = 0;
             for (int k = 0; k < 3; ++k) {
-                own.mn[k] = fdec(a.mn[k]);
-                own.mx[k] = fdec(a.mx[k]);
-            }
-            own.cnt = a.cnt;
-            GBox p = own, q = own;
-            for (int off = 1; off < kBins; off <<= 1) {
-                GBox lower = gbox_shfl_up(p, off);
-                const GBox higher = gbox_shfl_down(q, off);
-                if (lane >= off) {
-                    gbox_join(lower, p);
-                    p = lower;
-                }
-                if (lane + off < kBins) gbox_join(q, higher);
-            }
-            pre[ax] = p;
-            suf[ax] = q;
-            const GBox right = gbox_shfl_down(q, 1);
-            float key = __builtin_inff();
-            if (lane < kBins - 1) {
-                const float c = gbox_half_area(p) * float(p.cnt) +
-                                gbox_half_area(right) * float(right.cnt);
-                if (c < 3.40282347e38f) key = c;
-            }
-            int idx = lane;
-            for (int off = kBins / 2; off > 0; off >>= 1) {
-                const float ok = __shfl_xor(key, off, kBins);
-                const int oi = __shfl_xor(idx, off, kBins);
-                if (ok < key || (ok == key && oi < idx)) {
-                    key = ok;
-                    idx = oi;
-                }
-            }
-            if (key == __builtin_inff()) {
-                cost[ax] = 3.40282347e38f;
-                cnt[ax] = kBins;
-            } else {
-                cost[ax] = key;
-                cnt[ax] = uint32_t(idx + 1);
+                own[ax].mn[k] = 0;
+                own[ax].mx[k] = 0;
             }
         }
-        uint32_t axis = 0;
-        if (cost[0] > cost[1]) axis = 1;
-        if (cost[axis] > cost[2]) axis = 2;
-        uint32_t split_index = cnt[axis];
-        const float max_split_cost = half_area6(d.mn, d.mx) * (float(work) - 1.0f);
-        bool make_leaf = false;
-        if (cnt[axis] == uint32_t(kBins) || cost[axis] >= max_split_cost) {
-            if (work > max_leaf) {
-                // Aabb::longest_axis of the padded box, aabb.rs:354-363
-                axis = 0;
-                if (d.mx[1] - d.mn[1] > d.mx[0] - d.mn[0]) axis = 1;
-                if (d.mx[2] - d.mn[2] > d.mx[axis] - d.mn[axis]) axis = 2;
-                // the first bin whose prefix count reaches 2/5 of the work
-                const bool reach =
-                    lane < kBins - 1 && uint64_t(pre[axis].cnt) >= uint64_t(work * 2 / 5 + 1);
-                const uint32_t base = threadIdx.x & 63u & ~uint32_t(kBins - 1);  // group's lane 0
-                const uint32_t m = uint32_t(__ballot(reach) >> base) & 0xFFFFu;
-                if (m) split_index = uint32_t(__builtin_ctz(m) + 1);
-            } else {
-                make_leaf = true;
-            }
-        }
-        if (!make_leaf) {
-            const uint32_t lim = split_index < uint32_t(kBins) ? split_index : uint32_t(kBins);
-            const GBox lb = gbox_shfl(pre[axis], lim > 0 ? int(lim) - 1 : 0);
-            GBox rb = gbox_shfl(suf[axis], lim < uint32_t(kBins) ? int(lim) : 0);
-            const uint32_t left = lim > 0 ? lb.cnt : 0u;
-            if (lim >= uint32_t(kBins))
-                for (int k = 0; k < 3; ++k) {
-                    rb.mn[k] = 1e34f;  // Aabb::empty
-                    rb.mx[k] = -1e34f;
+        for (uint32_t q = 0; q < work; ++q) {
+            const uint4 qmn = gather[2 * q], qmx = gather[2 * q + 1];
+            const uint32_t qmnv[3] = {qmn.x, qmn.y, qmn.z};
+            const uint32_t qmxv[3] = {qmx.x, qmx.y, qmx.z};
+            for (int ax = 0; ax < 3; ++ax)
+                if (((qmn.w >> (4 * ax)) & 15u) == uint32_t(lane)) {
+                    ++own[ax].cnt;
+                    for (int k = 0; k < 3; ++k) {
+                        own[ax].mn[k] = max(own[ax].mn[k], qmnv[k]);
+                        own[ax].mx[k] = max(own[ax].mx[k], qmxv[k]);
+                    }
                 }
-            if (left > 0 && left < work) {
-                d.split = 1;
-                d.axis = axis;
-                d.split_index = split_index;
-                d.left_count = left;
-                for (int k = 0; k < 3; ++k) {
-                    d.lmn[k] = lb.mn[k];
-                    d.lmx[k] = lb.mx[k];
-                    d.rmn[k] = rb.mn[k];
-                    d.rmx[k] = rb.mx[k];
-                }
-            }
         }
-    }
-    if (lane == 0) {
-        dec[t] = d;
-        split_flag[t] = d.split;
-    }
-}
-
-// Writes each task's node (leaf or interior) and the next level's tasks.
-__global__ void children_kernel(const BTask* tasks, uint32_t num_tasks, const BDecision* dec,
-                                const uint32_t* split_excl, uint32_t counter, DevAabb* nodes,
-                                BTask* next) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= num_tasks) return;
-    const BTask tk = tasks[t];
-    const BDecision& d = dec[t];
-    DevAabb nd;
-    for (int k = 0; k < 3; ++k) {
-        nd.min[k] = d.mn[k];
-        nd.max[k] = d.mx[k];
-    }
-    if (d.split) {
-        const uint32_t child = counter + 2 * split_excl[t];
-        nd.extra1 = -1;
-        nd.extra2 = int32_t(child);
-        BTask l, r;
-        for (int k = 0; k < 3; ++k) {
-            l.mn[k] = d.lmn[k];
-            l.mx[k] = d.lmx[k];
-            r.mn[k] = d.rmn[k];
-            r.mx[k] = d.rmx[k];
-        }
-        l.node = child;
-        r.node = child + 1;
-        l.begin = tk.begin;
-        l.end = tk.begin + d.left_count;
-        r.begin = l.end;
-        r.end = tk.end;
-        l.depth = r.depth = tk.depth + 1;
-        l.pad[0] = l.pad[1] = r.pad[0] = r.pad[1] = 0;
-        next[2 * split_excl[t]] = l;
-        next[2 * split_excl[t] + 1] = r;
+        __builtin_amdgcn_wave_barrier();
     } else {
-        nd.extra1 = int32_t(tk.end - tk.begin);
-        nd.extra2 = int32_t(tk.begin);
-    }
-    nodes[tk.node] = nd;
-}
-
-// Left-side flags, written twice: kept in flag, scanned in place in lscan.
-__global__ void flag_kernel(uint32_t n, const int32_t* ptask, const uint32_t* idx,
-                            const BDecision* dec, const float* centers, uint32_t stride,
-                            uint32_t* flag, uint32_t* lscan) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int32_t t = ptask[i];
-    uint32_t f = 0;
-    if (t >= 0) {
-        const BDecision& d = dec[t];
-        if (d.split) {
-            const float c = centers[size_t(idx[i]) * stride + d.axis];
-            f = bin_index(c, d.ctb[d.axis], d.bo[d.axis]) < d.split_index ? 1u : 0u;
+        // fold the slot's copies; leave them empty for the slot's next owner
+        BinAcc empty;
+        empty.cnt = 0;
+        for (int k = 0; k < 3; ++k) empty.mn[k] = empty.mx[k] = 0;
+        empty.pad = 0;
+        for (int ax = 0; ax < 3; ++ax) own[ax] = empty;
+        const uint32_t copies = tk.slot < kHotSlots ? kCopies : 1u;
+        for (uint32_t cp = 0; cp < copies; ++cp) {
+            BinAcc* tb = slot_bins(bins, hot, tk.slot, cp);
+            for (int ax = 0; ax < 3; ++ax) {
+                const BinAcc v = tb[ax * kBins + lane];
+                own[ax].cnt += v.cnt;
+                for (int k = 0; k < 3; ++k) {
+                    own[ax].mn[k] = max(own[ax].mn[k], v.mn[k]);
+                    own[ax].mx[k] = max(own[ax].mx[k], v.mx[k]);
+                }
+                tb[ax * kBins + lane] = empty;
+            }
         }
     }
-    flag[i] = f;
-    lscan[i] = f;
+    // per axis: prefix and suffix runs, best plane cost and bin count
+    auto scan_axis = [&](const BinAcc& own_bin, GBox& pre, GBox& suf, float& cost,
+                         uint32_t& cnt) {
+        GBox b;
+        for (int k = 0; k < 3; ++k) {
+            b.mn[k] = dec_min(own_bin.mn[k]);
+            b.mx[k] = dec_max(own_bin.mx[k]);
+        }
+        b.cnt = own_bin.cnt;
+        GBox p = b, q = b;
+        scan_step<1>(p, q, lane);
+        scan_step<2>(p, q, lane);
+        scan_step<4>(p, q, lane);
+        scan_step<8>(p, q, lane);
+        pre = p;
+        suf = q;
+        const GBox right = gbox_dpp<kDppShl + 1>(q);
+        float key = __builtin_inff();
+        if (lane < kBins - 1) {
+            const float c = gbox_half_area(p) * float(p.cnt) +
+                            gbox_half_area(right) * float(right.cnt);
+            if (c < 3.40282347e38f) key = c;
+        }
+        int at = lane;
+        argmin_step<kDppXor1>(key, at);
+        argmin_step<kDppXor2>(key, at);
+        argmin_step<kDppHalfMirror>(key, at);
+        argmin_step<kDppMirror>(key, at);
+        if (key == __builtin_inff()) {
+            cost = 3.40282347e38f;
+            cnt = kBins;
+        } else {
+            cost = key;
+            cnt = uint32_t(at + 1);
+        }
+    };
+    GBox pre0, pre1, pre2, suf0, suf1, suf2;
+    float cost[3];
+    uint32_t cnt[3];
+    scan_axis(own[0], pre0, suf0, cost[0], cnt[0]);
+    scan_axis(own[1], pre1, suf1, cost[1], cnt[1]);
+    scan_axis(own[2], pre2, suf2, cost[2], cnt[2]);
+    // member-wise selects keep the runs in registers
+    auto gsel = [](uint32_t i, const GBox& a0, const GBox& a1, const GBox& a2) {
+        GBox r;
+        for (int k = 0; k < 3; ++k) {
+            r.mn[k] = i == 0 ? a0.mn[k] : (i == 1 ? a1.mn[k] : a2.mn[k]);
+            r.mx[k] = i == 0 ? a0.mx[k] : (i == 1 ? a1.mx[k] : a2.mx[k]);
+        }
+        r.cnt = i == 0 ? a0.cnt : (i == 1 ? a1.cnt : a2.cnt);
+        return r;
+    };
+    uint32_t axis = 0;
+    if (cost[0] > cost[1]) axis = 1;
+    if (pick3(cost, axis) > cost[2]) axis = 2;
+    uint32_t split_index = pick3(cnt, axis);
+    const float max_split_cost = half_area6(tk.mn, tk.mx) * (float(work) - 1.0f);
+    if (pick3(cnt, axis) == uint32_t(kBins) || pick3(cost, axis) >= max_split_cost) {
+        if (work <= max_leaf) return d;
+        // Aabb::longest_axis of the padded box, aabb.rs:354-363
+        const float e0 = tk.mx[0] - tk.mn[0], e1 = tk.mx[1] - tk.mn[1], e2 = tk.mx[2] - tk.mn[2];
+        axis = 0;
+        if (e1 > e0) axis = 1;
+        if (e2 > (axis == 1 ? e1 : e0)) axis = 2;
+        // the first bin whose prefix count reaches 2/5 of the work
+        const bool reach =
+            lane < kBins - 1 && uint64_t(axis == 0 ? pre0.cnt : (axis == 1 ? pre1.cnt : pre2.cnt)) >=
+                uint64_t(work * 2 / 5 + 1);
+        const uint32_t base = threadIdx.x & 63u & ~uint32_t(kBins - 1);  // group's lane 0
+        const uint32_t m = uint32_t(__ballot(reach) >> base) & 0xFFFFu;
+        if (m) split_index = uint32_t(__builtin_ctz(m) + 1);
+    }
+    const uint32_t lim = split_index < uint32_t(kBins) ? split_index : uint32_t(kBins);
+    const GBox lb = gbox_shfl(gsel(axis, pre0, pre1, pre2), lim > 0 ? int(lim) - 1 : 0);
+    GBox rb = gbox_shfl(gsel(axis, suf0, suf1, suf2), lim < uint32_t(kBins) ? int(lim) : 0);
+    const uint32_t left = lim > 0 ? lb.cnt : 0u;
+    if (lim >= uint32_t(kBins))
+        for (int k = 0; k < 3; ++k) {
+            rb.mn[k] = 1e34f;  // Aabb::empty
+            rb.mx[k] = -1e34f;
+        }
+    if (left > 0 && left < work) {
+        d.split = 1;
+        d.axis = axis;
+        d.split_index = split_index;
+        d.left_count = left;
+        for (int k = 0; k < 3; ++k) {
+            d.lmn[k] = lb.mn[k];
+            d.lmx[k] = lb.mx[k];
+            d.rmn[k] = rb.mn[k];
+            d.rmx[k] = rb.mx[k];
+        }
+    }
+    return d;
 }
 
-__global__ void scatter_kernel(uint32_t n, const int32_t* ptask, const uint32_t* idx,
-                               const BTask* tasks, const BDecision* dec,
-                               const uint32_t* split_excl, const uint32_t* flag,
-                               const uint32_t* lscan, uint32_t* idx_out, int32_t* ptask_out) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int32_t t = ptask[i];
-    if (t < 0 || !dec[t].split) {
-        idx_out[i] = idx[i];
-        ptask_out[i] = -1;
+// The decision of every task of the level, one group of 16 lanes per task
+// (grid stride: decisions are independent of each other).
+__global__ void __launch_bounds__(kBlock)
+    decide_kernel(const BLevel* levels, uint32_t level, const BTask* tasks, BinAcc* bins,
+                  BinAcc* hot, uint32_t max_leaf, const uint32_t* idx, const float* centers, uint32_t stride,
+                  const DevAabb* aabbs, SDec* decs) {
+    __shared__ uint4 s_gather[kBlock / kBins][2 * kSmallTask];  // per group
+    const uint32_t num = levels[level].num_tasks;
+    const uint32_t group = threadIdx.x / kBins;
+    const int lane = int(threadIdx.x % kBins);
+    // whole groups: kBlock is a multiple of kBins
+    for (uint32_t t = blockIdx.x * (kBlock / kBins) + group; t < num;
+         t += gridDim.x * (kBlock / kBins)) {
+        const SDec d = decide(tasks[t], lane, bins, hot, max_leaf, idx, centers, stride, aabbs,
+                              s_gather[group]);
+        if (lane == 0) decs[t] = d;
+    }
+}
+
+// Per tile of kTaskTile tasks, one per thread: the prefix of splitting tasks
+// and of binned children over the tiles before (payload: splits in bits
+// 61:31, binned children in bits 30:0), each task's node record, its
+// children as tasks of the next level and what the partition needs; the last
+// tile writes the next level's record.  A block takes one tile when the grid
+// covers the tiles, else tiles until none is left.
+__global__ void __launch_bounds__(kTaskTile)
+    children_kernel(BLevel* levels, uint32_t level, const BTask* tasks, const SDec* decs,
+                    uint64_t* tile_words, DevAabb* nodes, BTask* next, BPart* parts) {
+    constexpr uint32_t kWaves = kTaskTile / 64;
+    __shared__ uint32_t s_wsplit[kWaves], s_wslot[kWaves];
+    __shared__ uint32_t s_tile, s_split_excl, s_slot_excl;
+    const BLevel lv = levels[level];
+    const uint32_t num = lv.num_tasks;
+    if (num == 0) {
+        // past the last level: carry the node count forward
+        if (blockIdx.x == 0 && threadIdx.x == 0) levels[level + 1].node_base = lv.node_base;
         return;
     }
-    const BTask tk = tasks[t];
-    const uint32_t lefts_before = lscan[i] - lscan[tk.begin];
-    uint32_t pos;
-    int32_t nt;
-    if (flag[i]) {
-        pos = tk.begin + lefts_before;
-        nt = int32_t(2 * split_excl[t]);
-    } else {
-        pos = tk.begin + dec[t].left_count + (i - tk.begin - lefts_before);
-        nt = int32_t(2 * split_excl[t] + 1);
+    const uint32_t tiles = (num + kTaskTile - 1) / kTaskTile;
+    if (blockIdx.x >= tiles) return;  // no ticket taken: nothing would be left for it
+    // this level's words: after every earlier level's, as levels occupy
+    // consecutive node ids
+    uint64_t* words = tile_words + lv.node_base / kTaskTile + level;
+    const uint32_t wlane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (;;) {
+        const uint32_t tile = next_tile(&levels[level].split_ticket, &s_tile);
+        if (tile >= tiles) break;
+        const uint32_t t = tile * kTaskTile + threadIdx.x;
+        const bool valid = t < num;
+        BTask tk;
+        SDec d;
+        d.split = 0;
+        bool lbin = false, rbin = false;
+        if (valid) {
+            tk = tasks[t];
+            d = decs[t];
+            if (d.split) {
+                lbin = binned_by_atomics(d.left_count, tk.depth + 1);
+                rbin = binned_by_atomics(tk.end - tk.begin - d.left_count, tk.depth + 1);
+            }
+        }
+        const bool sp = valid && d.split;
+        const uint64_t ms = __ballot(sp), ml = __ballot(lbin), mr = __ballot(rbin);
+        const uint64_t below = (1ull << wlane) - 1;
+        uint32_t below_split = uint32_t(__popcll(ms & below));
+        uint32_t below_slots = uint32_t(__popcll(ml & below) + __popcll(mr & below));
+        if (wlane == 0) {
+            s_wsplit[wave] = uint32_t(__popcll(ms));
+            s_wslot[wave] = uint32_t(__popcll(ml) + __popcll(mr));
+        }
+        __syncthreads();
+        if (wave == 0) {
+            uint32_t ts = 0, tl = 0;
+            for (uint32_t w = 0; w < kWaves; ++w) {
+                ts += s_wsplit[w];
+                tl += s_wslot[w];
+            }
+            const uint64_t agg = (uint64_t(ts) << 31) | uint64_t(tl);
+            uint64_t excl = 0;
+            if (tile == 0) {
+                if (wlane == 0) publish(&words[0], kInclusive | agg);
+            } else {
+                if (wlane == 0) publish(&words[tile], kAggregate | agg);
+                excl = lookback(words, tile, wlane);
+                if (wlane == 0) publish(&words[tile], kInclusive | (excl + agg));
+            }
+            if (wlane == 0) {
+                s_split_excl = uint32_t(excl >> 31);
+                s_slot_excl = uint32_t(excl & 0x7FFFFFFFu);
+                if (tile == tiles - 1) {
+                    const uint64_t total = excl + agg;
+                    BLevel* nx = &levels[level + 1];
+                    nx->num_tasks = 2 * uint32_t(total >> 31);
+                    nx->node_base = lv.node_base + num;
+                    nx->slots = uint32_t(total & 0x7FFFFFFFu);
+                }
+            }
+        }
+        __syncthreads();
+        if (valid) {
+            for (uint32_t w = 0; w < wave; ++w) {
+                below_split += s_wsplit[w];
+                below_slots += s_wslot[w];
+            }
+            const uint32_t work = tk.end - tk.begin;
+            DevAabb nd;
+            for (int k = 0; k < 3; ++k) {
+                nd.min[k] = tk.mn[k];
+                nd.max[k] = tk.mx[k];
+            }
+            BPart pt;
+            pt.begin = tk.begin;
+            pt.left_count = 0;
+            pt.axis = kLeaf;
+            pt.split_index = 0;
+            pt.ctb = 0.0f;
+            pt.bo = 0.0f;
+            pt.child = 0;
+            pt.pad = 0;
+            if (d.split) {
+                const uint32_t child = 2 * (s_split_excl + below_split);  // next-level task
+                const uint32_t slot = s_slot_excl + below_slots;
+                nd.extra1 = -1;
+                nd.extra2 = int32_t(lv.node_base + num + child);
+                next[child] = make_task(d.lmn, d.lmx, tk.begin, tk.begin + d.left_count,
+                                        tk.depth + 1, lbin ? slot : kNoSlot);
+                next[child + 1] = make_task(d.rmn, d.rmx, tk.begin + d.left_count, tk.end,
+                                            tk.depth + 1, rbin ? slot + (lbin ? 1u : 0u) : kNoSlot);
+                pt.left_count = d.left_count;
+                pt.axis = d.axis;
+                pt.split_index = d.split_index;
+                pt.ctb = pick3(tk.ctb, d.axis);
+                pt.bo = pick3(tk.bo, d.axis);
+                pt.child = child;
+            } else {
+                nd.extra1 = int32_t(work);
+                nd.extra2 = int32_t(tk.begin);
+            }
+            nodes[lv.node_base + t] = nd;
+            parts[t] = pt;
+        }
+        if (gridDim.x >= tiles) break;  // one tile per block
+        __syncthreads();  // s_w* and s_tile are rewritten for the next tile
     }
-    idx_out[pos] = idx[i];
-    ptask_out[pos] = nt;
+}
+
+// Stable partition of every splitting task's range, one tile of kPrimTile
+// primitives per block (kChunks runs of kPartBlock; lane order = index
+// order).  A left primitive goes to begin + (lefts before it in the task), a
+// right one to begin + left_count + (rights before it).  Lefts before come
+// from the tile's own ranks and, for the one task that may begin in an
+// earlier tile, a carry over the tiles before: a tile's word holds the lefts
+// of its last task from the task's first primitive in the tile (aggregate)
+// or in the index array (inclusive); a tile whose last task begins inside
+// it, or whose last primitive is no splitting task's, publishes inclusive at
+// once.
+__global__ void __launch_bounds__(kPartBlock)
+    partition_kernel(BLevel* levels, uint32_t level, uint32_t n, int32_t* ptask,
+                     const uint32_t* idx, const BPart* parts, const float* centers,
+                     uint32_t stride, uint64_t* tile_words, uint32_t* idx_out,
+                     int32_t* ptask_out) {
+    constexpr uint32_t kWaves = kPartBlock / 64;
+    constexpr uint32_t kRuns = kChunks * kWaves;  // (chunk, wave) runs of 64, in index order
+    static_assert(kRuns == 64, "one lane per run");
+    __shared__ uint32_t s_rank[kPrimTile];  // lefts before each primitive in the tile
+    __shared__ uint32_t s_run[kRuns];
+    __shared__ uint32_t s_tile, s_carry, s_total;
+    __shared__ int32_t s_last;
+    __shared__ uint32_t s_last_begin, s_last_split;
+    if (levels[level].num_tasks == 0) return;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t tiles = (n + kPrimTile - 1) / kPrimTile;
+    uint64_t* words = tile_words + size_t(level) * tiles;
+    const uint32_t tile = next_tile(&levels[level].part_ticket, &s_tile);  // grid = tiles
+    const uint32_t base = tile * kPrimTile;
+    const uint32_t last = (base + kPrimTile < n ? base + kPrimTile : n) - 1;
+    int32_t tt[kChunks];
+    uint32_t pp[kChunks];
+    bool ff[kChunks];
+    BPart pb[kChunks];
+    for (uint32_t c = 0; c < kChunks; ++c) {
+        const uint32_t i = base + c * kPartBlock + threadIdx.x;
+        const int32_t t = i < n ? ptask[i] : -1;
+        uint32_t p = 0;
+        bool f = false;
+        BPart pt;
+        pt.axis = kLeaf;
+        pt.begin = 0;
+        if (t >= 0) {
+            p = idx[i];
+            pt = parts[t];
+            if (pt.axis != kLeaf)
+                f = bin_index(centers[size_t(p) * stride + pt.axis], pt.ctb, pt.bo) <
+                    pt.split_index;
+        }
+        if (i == last) {
+            s_last = t;
+            s_last_begin = pt.begin;
+            s_last_split = pt.axis != kLeaf ? 1u : 0u;
+        }
+        tt[c] = t;
+        pp[c] = p;
+        ff[c] = f;
+        pb[c] = pt;
+        const uint64_t m = __ballot(f);
+        if (lane == 0) s_run[c * kWaves + wave] = uint32_t(__popcll(m));
+        s_rank[c * kPartBlock + threadIdx.x] = uint32_t(__popcll(m & ((1ull << lane) - 1)));
+    }
+    __syncthreads();
+    if (wave == 0) {
+        // exclusive prefix of the 64 run counts, lane k holding run k
+        const uint32_t v = s_run[lane];
+        uint32_t incl = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t u = __shfl_up(incl, off, 64);
+            if (lane >= uint32_t(off)) incl += u;
+        }
+        s_run[lane] = incl - v;
+        if (lane == 63) s_total = incl;
+    }
+    __syncthreads();
+    for (uint32_t c = 0; c < kChunks; ++c) s_rank[c * kPartBlock + threadIdx.x] += s_run[c * kWaves + wave];
+    __syncthreads();
+    if (wave == 0) {
+        // the tile's word: lefts of its last task within the tile
+        const uint32_t total = s_total;
+        uint64_t word = kInclusive;  // no splitting task continues past the tile
+        if (s_last >= 0 && s_last_split) {
+            const uint32_t b = s_last_begin;
+            word = b >= base ? kInclusive | (total - s_rank[b - base])
+                             : kAggregate | total;  // one task covers the tile
+        }
+        if (lane == 0) publish(&words[tile], word);
+        // carry: lefts of the first task in the tiles before (thread 0 holds
+        // primitive `base`)
+        const bool need = __shfl(tt[0] >= 0 && pb[0].axis != kLeaf && pb[0].begin < base ? 1 : 0,
+                                 0, 64) != 0;
+        const uint32_t carry = need ? uint32_t(lookback(words, tile, lane)) : 0u;
+        if (lane == 0) {
+            if ((word >> 62) == 1) publish(&words[tile], kInclusive | (carry + total));
+            s_carry = carry;
+        }
+    }
+    __syncthreads();
+    for (uint32_t c = 0; c < kChunks; ++c) {
+        const int32_t t = tt[c];
+        if (t < 0) continue;
+        const uint32_t i = base + c * kPartBlock + threadIdx.x;
+        const BPart& pt = pb[c];
+        if (pt.axis == kLeaf) {
+            // final place: both buffers, out of later levels
+            idx_out[i] = pp[c];
+            ptask_out[i] = -1;
+            ptask[i] = -1;
+            continue;
+        }
+        const uint32_t rank = s_rank[c * kPartBlock + threadIdx.x];
+        const uint32_t lefts = pt.begin >= base ? rank - s_rank[pt.begin - base] : s_carry + rank;
+        const uint32_t pos = ff[c] ? pt.begin + lefts
+                                   : pt.begin + pt.left_count + (i - pt.begin - lefts);
+        if (pos >= n) continue;  // never taken: ranks stay inside the task's range
+        idx_out[pos] = pp[c];
+        ptask_out[pos] = int32_t(pt.child + (ff[c] ? 0u : 1u));
+    }
 }
 
 __global__ void iota_kernel(uint32_t* idx, int32_t* ptask, uint32_t n) {
@@ -570,8 +993,10 @@ __global__ void __launch_bounds__(kBlock)
     if (threadIdx.x < 6) parts[size_t(blockIdx.x) * 6 + threadIdx.x] = sh[0][threadIdx.x];
 }
 
+// The root task (its box padded by 1e-4 before the node padding) and the
+// first level's record.
 __global__ void __launch_bounds__(kBlock)
-    root_bounds_kernel(const float* parts, uint32_t n, BTask* task) {
+    root_bounds_kernel(const float* parts, uint32_t n, BTask* task, BLevel* levels) {
     __shared__ float sh[kBlock][6];
     float acc[6] = {1e34f, 1e34f, 1e34f, -1e34f, -1e34f, -1e34f};
     for (uint32_t p = threadIdx.x; p < kRootParts; p += kBlock)
@@ -581,37 +1006,25 @@ __global__ void __launch_bounds__(kBlock)
         }
     block_box(acc, sh);
     if (threadIdx.x == 0) {
-        BTask tk;
+        float mn[3], mx[3];
         for (int k = 0; k < 3; ++k) {
-            tk.mn[k] = sh[0][k] - 0.0001f;
-            tk.mx[k] = sh[0][3 + k] + 0.0001f;
+            mn[k] = sh[0][k] - 0.0001f;
+            mx[k] = sh[0][3 + k] + 0.0001f;
         }
-        tk.node = 0;
-        tk.begin = 0;
-        tk.end = n;
-        tk.depth = 0;
-        tk.pad[0] = tk.pad[1] = 0;
-        *task = tk;
+        const bool binned = binned_by_atomics(n, 0);
+        *task = make_task(mn, mx, 0, n, 0, binned ? 0u : kNoSlot);
+        BLevel lv;
+        lv.num_tasks = 1;
+        lv.node_base = 0;
+        lv.slots = binned ? 1u : 0u;
+        lv.split_ticket = 0;
+        lv.part_ticket = 0;
+        lv.pad[0] = lv.pad[1] = lv.pad[2] = 0;
+        levels[0] = lv;
     }
 }
 
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t cap = 0;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    hipError_t ensure(size_t count) {
-        if (count <= cap) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        hipError_t e = hipMalloc(&p, count * sizeof(T) + 16);
-        if (e == hipSuccess) cap = count;
-        return e;
-    }
-};
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 }  // namespace
 
@@ -640,101 +1053,133 @@ ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t st
         (void)hipFree(res.indices);
         return hip_fail(err, what);
     };
-    DevBuf<DevAabb> pts;
+    // One workspace.  A level holds at most n tasks (disjoint non-empty
+    // ranges) and at most n / 65 of them have bins; the children tile words
+    // of all levels fit (2n - 1) / kTaskTile + levels + 1 (levels use
+    // consecutive node ids), the partition words one row per level.
+    const uint32_t ptiles = (n + kPrimTile - 1) / kPrimTile;
+    const size_t slot_cap = size_t(n) / (kSmallTask + 1) + 1;
+    const size_t split_words = size_t(2) * n / kTaskTile + kMaxLevels + 2;
+    const size_t sz_pts = aabbs ? 0 : align256(size_t(n) * sizeof(DevAabb));
+    const size_t sz_idx = align256(size_t(n) * 4);
+    const size_t sz_tasks = align256(size_t(n) * sizeof(BTask));
+    const size_t sz_parts = align256(size_t(n) * sizeof(BPart));
+    const size_t sz_decs = align256(size_t(n) * sizeof(SDec));
+    const size_t sz_root = align256(size_t(kRootParts) * 6 * 4);
+    // cleared together: level records, tile words, bins
+    const size_t sz_levels = align256(size_t(kMaxLevels + 1) * sizeof(BLevel));
+    const size_t sz_words = align256((split_words + size_t(kMaxLevels) * ptiles) * 8);
+    const size_t sz_bins = align256(slot_cap * 3 * kBins * sizeof(BinAcc));
+    const size_t sz_hot = align256(size_t(kHotSlots) * kCopies * 3 * kBins * sizeof(BinAcc));
+    const size_t total = sz_pts + 3 * sz_idx + 2 * sz_tasks + sz_parts + sz_decs + sz_root + sz_levels +
+                         sz_words + sz_bins + sz_hot;
+    char* ws = nullptr;
+    if ((e = hipMalloc(&ws, total)) != hipSuccess) return fail(e, "binned workspace");
+    struct WsFree {
+        char* p;
+        ~WsFree() { (void)hipFree(p); }
+    } wsf{ws};
+    BLevel* host_level = nullptr;
+    if ((e = hipHostMalloc(&host_level, sizeof(BLevel))) != hipSuccess)
+        return fail(e, "hipHostMalloc");
+    struct HostFree {
+        BLevel* p;
+        ~HostFree() { (void)hipHostFree(p); }
+    } hf{host_level};
+    char* cur = ws;
+    auto carve = [&](size_t bytes) {
+        char* p = cur;
+        cur += bytes;
+        return p;
+    };
+    DevAabb* pts = aabbs ? nullptr : reinterpret_cast<DevAabb*>(carve(sz_pts));
+    uint32_t* idx1 = reinterpret_cast<uint32_t*>(carve(sz_idx));
+    int32_t* pt0 = reinterpret_cast<int32_t*>(carve(sz_idx));
+    int32_t* pt1 = reinterpret_cast<int32_t*>(carve(sz_idx));
+    BTask* tasks0 = reinterpret_cast<BTask*>(carve(sz_tasks));
+    BTask* tasks1 = reinterpret_cast<BTask*>(carve(sz_tasks));
+    BPart* parts = reinterpret_cast<BPart*>(carve(sz_parts));
+    SDec* decs = reinterpret_cast<SDec*>(carve(sz_decs));
+    float* root_parts = reinterpret_cast<float*>(carve(sz_root));
+    char* cleared = cur;
+    BLevel* levels = reinterpret_cast<BLevel*>(carve(sz_levels));
+    uint64_t* split_w = reinterpret_cast<uint64_t*>(carve(sz_words));
+    uint64_t* part_w = split_w + split_words;
+    BinAcc* bins = reinterpret_cast<BinAcc*>(carve(sz_bins));
+    BinAcc* hot = reinterpret_cast<BinAcc*>(carve(sz_hot));
+    if ((e = hipMemsetAsync(cleared, 0, sz_levels + sz_words + sz_bins + sz_hot, stream)) !=
+        hipSuccess)
+        return fail(e, "binned clear");
+
     const DevAabb* in_aabbs = reinterpret_cast<const DevAabb*>(aabbs);
     const uint32_t g = (n + kBlock - 1) / kBlock;
     if (!aabbs) {
-        if ((e = pts.ensure(n)) != hipSuccess) return fail(e, "hipMalloc(aabbs)");
         hipLaunchKernelGGL(point_aabbs_kernel, dim3(g), dim3(kBlock), 0, stream, centers, stride,
-                           n, pts.p);
-        in_aabbs = pts.p;
+                           n, pts);
+        in_aabbs = pts;
     }
-    DevBuf<uint32_t> idx1, flag, lscan, split_flag, scan_scratch;
-    DevBuf<int32_t> pt0, pt1;
-    DevBuf<BTask> tasks0, tasks1;
-    DevBuf<BDecision> dec;
-    DevBuf<BinAcc> bins;
-    DevBuf<float> root_parts;
-    uint32_t* host_total = nullptr;
-    if ((e = hipHostMalloc(&host_total, 8)) != hipSuccess) return fail(e, "hipHostMalloc");
-    struct HostFree {
-        uint32_t* p;
-        ~HostFree() { (void)hipHostFree(p); }
-    } hf{host_total};
-    if ((e = idx1.ensure(n)) != hipSuccess || (e = flag.ensure(n)) != hipSuccess ||
-        (e = lscan.ensure(n)) != hipSuccess || (e = pt0.ensure(n)) != hipSuccess ||
-        (e = pt1.ensure(n)) != hipSuccess || (e = tasks0.ensure(1)) != hipSuccess ||
-        (e = scan_scratch.ensure(scan_scratch_bytes(n) / 4 + 1)) != hipSuccess ||
-        (e = root_parts.ensure(size_t(kRootParts) * 6)) != hipSuccess)
-        return fail(e, "binned workspace");
-    uint32_t* idx_cur = res.indices;
-    uint32_t* idx_nxt = idx1.p;
-    int32_t* pt_cur = pt0.p;
-    int32_t* pt_nxt = pt1.p;
-    hipLaunchKernelGGL(iota_kernel, dim3(g), dim3(kBlock), 0, stream, idx_cur, pt_cur, n);
+    uint32_t* idx[2] = {res.indices, idx1};
+    int32_t* ptk[2] = {pt0, pt1};
+    BTask* tasks[2] = {tasks0, tasks1};
+    hipLaunchKernelGGL(iota_kernel, dim3(g), dim3(kBlock), 0, stream, idx[0], ptk[0], n);
     hipLaunchKernelGGL(root_partial_kernel, dim3(kRootParts), dim3(kBlock), 0, stream, in_aabbs, n,
-                       root_parts.p);
-    hipLaunchKernelGGL(root_bounds_kernel, dim3(1), dim3(kBlock), 0, stream, root_parts.p, n,
-                       tasks0.p);
-    BTask* t_cur = tasks0.p;
-    uint32_t num_tasks = 1;
-    uint32_t counter = 1;  // AtomicNodeStack starts at 1 (builders/mod.rs:51-56)
-    int flip = 0;
-    // Per-task buffers hold tcap tasks: at least 65536 (at most n, as every
-    // task has a primitive), doubled when a level needs more, so the levels
-    // near the root, whose task count doubles, do not free and allocate
-    // every buffer each time (a hipFree and a hipMalloc cost more than the
-    // level's kernels there).
-    size_t tcap = 0;
-    while (num_tasks > 0) {
-        if (num_tasks > tcap)
-            tcap = std::max<size_t>(
-                num_tasks, std::min<size_t>(n, std::max<size_t>(65536, 2 * size_t(num_tasks))));
-        if ((e = dec.ensure(tcap)) != hipSuccess ||
-            (e = bins.ensure(tcap * 3 * kBins)) != hipSuccess ||
-            (e = split_flag.ensure(tcap)) != hipSuccess)
-            return fail(e, "binned level workspace");
-        DevBuf<BTask>& nxt_buf = flip ? tasks0 : tasks1;
-        if ((e = nxt_buf.ensure(2 * tcap)) != hipSuccess)
-            return fail(e, "binned tasks");
-        BTask* t_nxt = nxt_buf.p;
-        const uint32_t tg = (num_tasks + kBlock - 1) / kBlock;
-        hipLaunchKernelGGL(level_init_kernel, dim3(num_tasks), dim3(64), 0, stream, t_cur,
-                           num_tasks, dec.p, bins.p);
-        hipLaunchKernelGGL(bin_kernel, dim3(g), dim3(kBlock), 0, stream, n, pt_cur, idx_cur, t_cur,
-                           dec.p, centers, stride, in_aabbs, bins.p);
-        hipLaunchKernelGGL(split_kernel, dim3((num_tasks + kBlock / kBins - 1) / (kBlock / kBins)),
-                           dim3(kBlock), 0, stream, t_cur, num_tasks, dec.p, bins.p, leaf,
-                           split_flag.p, idx_cur, centers, stride, in_aabbs);
-        if ((e = scan_exclusive_u32(split_flag.p, num_tasks, host_total, scan_scratch.p,
-                                    scan_scratch.cap * 4, stream)) != hipSuccess)
-            return fail(e, "binned task scan");
-        hipLaunchKernelGGL(children_kernel, dim3(tg), dim3(kBlock), 0, stream, t_cur, num_tasks,
-                           dec.p, split_flag.p, counter, res.nodes, t_nxt);
-        hipLaunchKernelGGL(flag_kernel, dim3(g), dim3(kBlock), 0, stream, n, pt_cur, idx_cur, dec.p,
-                           centers, stride, flag.p, lscan.p);
-        if ((e = scan_exclusive_u32(lscan.p, n, nullptr, scan_scratch.p, scan_scratch.cap * 4,
-                                    stream)) != hipSuccess)
-            return fail(e, "binned partition scan");
-        hipLaunchKernelGGL(scatter_kernel, dim3(g), dim3(kBlock), 0, stream, n, pt_cur, idx_cur,
-                           t_cur, dec.p, split_flag.p, flag.p, lscan.p, idx_nxt, pt_nxt);
+                       root_parts);
+    hipLaunchKernelGGL(root_bounds_kernel, dim3(1), dim3(kBlock), 0, stream, root_parts, n,
+                       tasks[0], levels);
+    const uint32_t child_cap = std::min<uint32_t>((n + kTaskTile - 1) / kTaskTile, kChildGrid);
+    const uint32_t decide_cap = std::min<uint32_t>((n + kBins - 1) / kBins, kDecideGrid);
+    // bin_kernel: runs of whole chunks, about kBinGrid of them
+    const uint32_t run = ((n + kBinGrid - 1) / kBinGrid + kBlock - 1) / kBlock * kBlock;
+    const uint32_t bgrid = (n + run - 1) / run;
+    // The first batch covers a tree twice as deep as a balanced one (levels
+    // past the last cost four launches that return at once); then batches
+    // of 8 until a level has no task.
+    uint32_t lg = 0;
+    while (lg < 31 && (1u << lg) < n) ++lg;
+    uint32_t batch = std::min<uint32_t>(kMaxLevels, 2 * lg + 2);
+    uint32_t level = 0;
+    uint64_t known = 1;  // tasks at `level`
+    for (;;) {
+        const uint32_t end = std::min(kMaxLevels, level + batch);
+        for (uint32_t k = level; k < end; ++k) {
+            // a level holds at most twice the tasks of the one before
+            const uint32_t sh = std::min<uint32_t>(k - level, 32);
+            const uint64_t bound = std::min<uint64_t>(n, known << sh);
+            const uint32_t cgrid = uint32_t(
+                std::min<uint64_t>(child_cap, (bound + kTaskTile - 1) / kTaskTile));
+            const uint32_t dgrid = uint32_t(std::min<uint64_t>(
+                decide_cap, (bound + kBlock / kBins - 1) / (kBlock / kBins)));
+            const uint32_t a = k & 1u, b = a ^ 1u;
+            hipLaunchKernelGGL(bin_kernel, dim3(bgrid), dim3(kBlock), 0, stream, levels, k, n, run,
+                               ptk[a], idx[a], tasks[a], centers, stride, in_aabbs, bins, hot);
+            hipLaunchKernelGGL(decide_kernel, dim3(dgrid), dim3(kBlock), 0, stream, levels, k,
+                               tasks[a], bins, hot, leaf, idx[a], centers, stride, in_aabbs,
+                               decs);
+            hipLaunchKernelGGL(children_kernel, dim3(cgrid), dim3(kTaskTile), 0, stream, levels, k,
+                               tasks[a], decs, split_w, res.nodes, tasks[b], parts);
+            // one tile per block: the grid is the tile count
+            hipLaunchKernelGGL(partition_kernel, dim3(ptiles), dim3(kPartBlock), 0, stream, levels,
+                               k, n, ptk[a], idx[a], parts, centers, stride, part_w, idx[b],
+                               ptk[b]);
+        }
         if ((e = hipGetLastError()) != hipSuccess) return fail(e, "binned level");
-        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return fail(e, "binned level");
-        const uint32_t splits = *host_total;
-        counter += 2 * splits;
-        num_tasks = 2 * splits;
-        std::swap(idx_cur, idx_nxt);
-        std::swap(pt_cur, pt_nxt);
-        t_cur = t_nxt;
-        flip ^= 1;
+        level = end;
+        if ((e = hipMemcpyAsync(host_level, levels + level, sizeof(BLevel), hipMemcpyDeviceToHost,
+                                stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(stream)) != hipSuccess)
+            return fail(e, "binned levels");
+        if (host_level->num_tasks == 0) break;
+        if (level >= kMaxLevels) {  // a task at depth 64 is a leaf
+            set_error("binned SAH: tasks left past depth %u", kMaxDepth);
+            (void)hipFree(res.nodes);
+            (void)hipFree(res.indices);
+            return Error;
+        }
+        known = host_level->num_tasks;
+        batch = 8;
     }
-    if (idx_cur != res.indices) {
-        if ((e = hipMemcpyAsync(res.indices, idx_cur, size_t(n) * 4, hipMemcpyDeviceToDevice,
-                                stream)) != hipSuccess)
-            return fail(e, "binned indices");
-        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return fail(e, "binned indices");
-    }
-    res.node_count = counter;
+    // every primitive was written to both index buffers when its leaf was made
+    res.node_count = host_level->node_base;
     *out = res;
     return Ok;
 }
