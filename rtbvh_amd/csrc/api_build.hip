@@ -547,8 +547,8 @@ ResultCode rtbvh_gpu_batch_attach_triangles(RTBatch* batch, const float* vertice
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
     TraceTreePlan* plan = nullptr;
-    ResultCode rc =
-        trace_tree_plan_create(be->bvh, kDefaultMaxLeaf, kBatchTreeWidth, &plan, &be->ttree);
+    ResultCode rc = trace_tree_plan_create(be->bvh, kDefaultMaxLeaf, kBatchTreeWidth, &plan,
+                                           &be->ttree, true);
     if (rc != Ok) return rc;
     rc = trace_tree_build(plan, be->bvh, vertices, uint32_t(prim_count), be->scene_off,
                           be->scenes, &be->ttree, st);
@@ -590,7 +590,7 @@ static ResultCode create_trace_tree(RTBvh bvh, const float* vertices, size_t pri
         te->device = be->device;
         hipStream_t st = static_cast<hipStream_t>(stream);
         TraceTreePlan* plan = nullptr;
-        ResultCode rc = trace_tree_plan_create(be->dev, max_leaf, width, &plan, &te->t);
+        ResultCode rc = trace_tree_plan_create(be->dev, max_leaf, width, &plan, &te->t, true);
         if (rc != Ok) return rc;
         rc = trace_tree_build(plan, be->dev, vertices, uint32_t(prim_count), nullptr, 0, &te->t,
                               st);

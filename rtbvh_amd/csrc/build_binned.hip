@@ -636,9 +636,8 @@ __device__ __forceinline__ SDec decide(const BTask& tk, int lane, BinAcc* bins, 
         if (e1 > e0) axis = 1;
         if (e2 > (axis == 1 ? e1 : e0)) axis = 2;
         // the first bin whose prefix count reaches 2/5 of the work
-        const bool reach =
-            lane < kBins - 1 && uint64_t(axis == 0 ? pre0.cnt : (axis == 1 ? pre1.cnt : pre2.cnt)) >=
-                uint64_t(work * 2 / 5 + 1);
+        const uint32_t run = axis == 0 ? pre0.cnt : (axis == 1 ? pre1.cnt : pre2.cnt);
+        const bool reach = lane < kBins - 1 && uint64_t(run) >= uint64_t(work * 2 / 5 + 1);
         const uint32_t base = threadIdx.x & 63u & ~uint32_t(kBins - 1);  // group's lane 0
         const uint32_t m = uint32_t(__ballot(reach) >> base) & 0xFFFFu;
         if (m) split_index = uint32_t(__builtin_ctz(m) + 1);
@@ -671,8 +670,8 @@ __device__ __forceinline__ SDec decide(const BTask& tk, int lane, BinAcc* bins, 
 // (grid stride: decisions are independent of each other).
 __global__ void __launch_bounds__(kBlock)
     decide_kernel(const BLevel* levels, uint32_t level, const BTask* tasks, BinAcc* bins,
-                  BinAcc* hot, uint32_t max_leaf, const uint32_t* idx, const float* centers, uint32_t stride,
-                  const DevAabb* aabbs, SDec* decs) {
+                  BinAcc* hot, uint32_t max_leaf, const uint32_t* idx, const float* centers,
+                  uint32_t stride, const DevAabb* aabbs, SDec* decs) {
     __shared__ uint4 s_gather[kBlock / kBins][2 * kSmallTask];  // per group
     const uint32_t num = levels[level].num_tasks;
     const uint32_t group = threadIdx.x / kBins;
@@ -888,7 +887,8 @@ __global__ void __launch_bounds__(kPartBlock)
         if (lane == 63) s_total = incl;
     }
     __syncthreads();
-    for (uint32_t c = 0; c < kChunks; ++c) s_rank[c * kPartBlock + threadIdx.x] += s_run[c * kWaves + wave];
+    for (uint32_t c = 0; c < kChunks; ++c)
+        s_rank[c * kPartBlock + threadIdx.x] += s_run[c * kWaves + wave];
     __syncthreads();
     if (wave == 0) {
         // the tile's word: lefts of its last task within the tile
@@ -1037,7 +1037,8 @@ ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t st
     }
     const uint32_t n = uint32_t(n64);
     const uint32_t stride = uint32_t(stride_floats);
-    const uint32_t leaf = max_leaf == 0 ? 1u : uint32_t(max_leaf > 0xFFFFFFFFu ? 0xFFFFFFFFu : max_leaf);
+    const uint32_t leaf =
+        max_leaf == 0 ? 1u : uint32_t(max_leaf > 0xFFFFFFFFu ? 0xFFFFFFFFu : max_leaf);
     hipError_t e;
     DeviceBvh res;
     const uint32_t cap_nodes = 2 * n - 1;
@@ -1071,14 +1072,18 @@ ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t st
     const size_t sz_words = align256((split_words + size_t(kMaxLevels) * ptiles) * 8);
     const size_t sz_bins = align256(slot_cap * 3 * kBins * sizeof(BinAcc));
     const size_t sz_hot = align256(size_t(kHotSlots) * kCopies * 3 * kBins * sizeof(BinAcc));
-    const size_t total = sz_pts + 3 * sz_idx + 2 * sz_tasks + sz_parts + sz_decs + sz_root + sz_levels +
-                         sz_words + sz_bins + sz_hot;
-    char* ws = nullptr;
-    if ((e = hipMalloc(&ws, total)) != hipSuccess) return fail(e, "binned workspace");
-    struct WsFree {
-        char* p;
-        ~WsFree() { (void)hipFree(p); }
-    } wsf{ws};
+    const size_t total = sz_pts + 3 * sz_idx + 2 * sz_tasks + sz_parts + sz_decs + sz_root +
+                         sz_levels + sz_words + sz_bins + sz_hot;
+    Workspace wsh;
+    if ((e = workspace_acquire(kWsBinned, total, &wsh)) != hipSuccess)
+        return fail(e, "binned workspace");
+    // released on every return, after the stream's work that used it
+    struct WsRelease {
+        Workspace* w;
+        hipStream_t s;
+        ~WsRelease() { workspace_release(w, s, true); }
+    } wsr{&wsh, stream};
+    char* ws = static_cast<char*>(wsh.ptr);
     BLevel* host_level = nullptr;
     if ((e = hipHostMalloc(&host_level, sizeof(BLevel))) != hipSuccess)
         return fail(e, "hipHostMalloc");

@@ -871,6 +871,7 @@ struct PlocPlanImpl {
     DevAabb* nodes = nullptr;
     uint32_t* indices = nullptr;
     void* ws = nullptr;
+    Workspace wsh;  // ws from the workspace cache (eager builds)
     // carved from ws
     DevAabb* in_aabbs = nullptr;  // point boxes (aabbs == null)
     Bounds6* part = nullptr;
@@ -896,7 +897,11 @@ void free_plan(PlocPlanImpl* p, bool keep_tree) {
         if (p->nodes) (void)hipFree(p->nodes);
         if (p->indices) (void)hipFree(p->indices);
     }
-    if (p->ws) (void)hipFree(p->ws);
+    if (p->wsh.ptr)
+        // a failed build may have work in flight; a finished one has waited
+        workspace_release(&p->wsh, nullptr, !keep_tree);
+    else if (p->ws)
+        (void)hipFree(p->ws);
     delete p;
 }
 
@@ -965,18 +970,22 @@ size_t carve_plan(PlocPlanImpl* p, char* base) {
     return off;
 }
 
+// cached: the workspace comes from the per-device cache (eager builds, which
+// release it when they return; plans keep theirs for their lifetime).
 ResultCode alloc_plan(uint32_t n, uint32_t scenes, bool point_aabbs, size_t stride_floats,
-                      PlocPlanImpl** out) {
+                      PlocPlanImpl** out, bool cached) {
     PlocPlanImpl* p = new PlocPlanImpl();
     init_plan(p, n, scenes, point_aabbs, stride_floats);
     const size_t bytes = carve_plan(p, nullptr);
     hipError_t e;
     if ((e = hipMalloc(&p->nodes, size_t(p->node_count()) * sizeof(DevAabb))) != hipSuccess ||
         (e = hipMalloc(&p->indices, size_t(n) * sizeof(uint32_t))) != hipSuccess ||
-        (e = hipMalloc(&p->ws, bytes)) != hipSuccess) {
+        (e = cached ? workspace_acquire(kWsPloc, bytes, &p->wsh) : hipMalloc(&p->ws, bytes)) !=
+            hipSuccess) {
         free_plan(p, false);
         return hip_fail(e, "hipMalloc(ploc)");
     }
+    if (cached) p->ws = p->wsh.ptr;
     carve_plan(p, static_cast<char*>(p->ws));
     *out = p;
     return Ok;
@@ -1179,7 +1188,7 @@ hipError_t enqueue_small(const PlocPlanImpl* p, const RTAabb* aabbs, const float
 ResultCode build_small(const RTAabb* aabbs, const float* centers, size_t stride_floats,
                        uint32_t n, hipStream_t stream, DeviceBvh* out) {
     PlocPlanImpl* p = nullptr;
-    ResultCode rc = alloc_plan(n, 1, aabbs == nullptr, stride_floats, &p);
+    ResultCode rc = alloc_plan(n, 1, aabbs == nullptr, stride_floats, &p, true);
     if (rc != Ok) return rc;
     hipError_t e = enqueue_small(p, aabbs, centers, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -1203,7 +1212,7 @@ ResultCode build_eager(const RTAabb* aabbs, const float* centers, size_t stride_
                        DeviceBvh* out) {
     if (scenes == 1 && n <= 2) return build_small(aabbs, centers, stride_floats, n, stream, out);
     PlocPlanImpl* p = nullptr;
-    ResultCode rc = alloc_plan(n, scenes, aabbs == nullptr, stride_floats, &p);
+    ResultCode rc = alloc_plan(n, scenes, aabbs == nullptr, stride_floats, &p, true);
     if (rc != Ok) return rc;
     rc = p->seg ? run_eager<true>(p, aabbs, centers, scene_off, stream, nullptr)
                 : run_eager<false>(p, aabbs, centers, nullptr, stream, nullptr);
@@ -1229,7 +1238,7 @@ ResultCode ploc_plan_create(const RTAabb* aabbs, const float* centers, size_t st
                             PlocPlanImpl** out, DeviceBvh* tree) {
     const uint32_t n = scene_off[scenes];
     PlocPlanImpl* p = nullptr;
-    ResultCode rc = alloc_plan(n, scenes, aabbs == nullptr, stride_floats, &p);
+    ResultCode rc = alloc_plan(n, scenes, aabbs == nullptr, stride_floats, &p, false);
     if (rc != Ok) return rc;
     uint32_t need = 0;
     hipError_t e = hipMemcpy(p->soff, scene_off, size_t(scenes + 1) * 4, hipMemcpyHostToDevice);

@@ -1,6 +1,7 @@
 // Internal interfaces shared by the translation units of librtbvh_amd.so.
 #pragma once
 
+#include <algorithm>
 #include <shared_mutex>
 #include <vector>
 
@@ -151,10 +152,11 @@ hipError_t launch_parent_links(const DevAabb* nodes, uint32_t count, int32_t* pa
 // with one triangle per leaf slot (max_leaf must be 1).  Triangles are stored
 // in leaf order; traced by the lowest-id kernels with `tris`.  *out receives
 // newly allocated nodes or cnodes (capacity prim_count + roots) and tris;
-// indices is null.
+// indices is null.  cached: the plan's workspace comes from the workspace
+// cache (a plan destroyed right after its build).
 struct TraceTreePlan;
 ResultCode trace_tree_plan_create(const DeviceBvh& tree, uint32_t max_leaf, uint32_t width,
-                                  TraceTreePlan** out_plan, DeviceMbvh* out);
+                                  TraceTreePlan** out_plan, DeviceMbvh* out, bool cached = false);
 ResultCode trace_tree_build(TraceTreePlan* t, const DeviceBvh& tree, const float* verts,
                             uint32_t prim_count, const uint32_t* scene_off, uint32_t scenes,
                             DeviceMbvh* out, hipStream_t stream);
@@ -164,6 +166,25 @@ hipError_t trace_tree_enqueue(const TraceTreePlan* t, const DeviceBvh& tree, con
                               hipStream_t stream);
 uint32_t trace_tree_levels(const TraceTreePlan* t);
 void trace_tree_plan_destroy(TraceTreePlan* t);
+
+// Build workspaces.  Eager builds take their scratch from a per-device
+// cache that keeps one block per builder kind between builds (grown to the
+// largest request up to kWorkspaceCacheMax bytes, held until the process
+// ends): a hipMalloc and a hipFree of a few hundred MB cost more than a 1M
+// build's clustering iterations.  A block already in use, or a larger
+// request, is allocated and freed per build as before.  The work that used a
+// block must be complete when it is released (workspace_release waits for
+// the given stream, or for the device when stream is null).
+enum WorkspaceKind : int { kWsPloc = 0, kWsBinned = 1, kWsTree = 2, kWsKinds = 3 };
+constexpr size_t kWorkspaceCacheMax = size_t(1) << 30;
+struct Workspace {
+    void* ptr = nullptr;
+    bool cached = false;
+    int dev = -1;
+    int kind = 0;
+};
+hipError_t workspace_acquire(WorkspaceKind kind, size_t bytes, Workspace* out);
+void workspace_release(Workspace* w, hipStream_t stream, bool wait);
 
 ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t stride_floats,
                             size_t n, size_t max_leaf, hipStream_t stream, DeviceBvh* out);

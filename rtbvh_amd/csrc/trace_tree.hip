@@ -780,11 +780,16 @@ struct TraceTreePlan {
     float* dpc = nullptr;
     uint32_t* dpd = nullptr;
     void* ws = nullptr;  // one allocation holding all of the above
+    Workspace wsh;       // ws from the workspace cache
+    hipStream_t last_stream = nullptr;  // of the last build
 };
 
 void trace_tree_plan_destroy(TraceTreePlan* t) {
     if (!t) return;
-    if (t->ws) (void)hipFree(t->ws);
+    if (t->wsh.ptr)
+        workspace_release(&t->wsh, t->last_stream, true);  // after the build's work
+    else if (t->ws)
+        (void)hipFree(t->ws);
     delete t;
 }
 
@@ -865,7 +870,7 @@ void enqueue_level(const TraceTreePlan* t, const LevelArgs& a, uint32_t level, u
 }  // namespace
 
 ResultCode trace_tree_plan_create(const DeviceBvh& tree, uint32_t max_leaf, uint32_t width,
-                                  TraceTreePlan** out_plan, DeviceMbvh* out) {
+                                  TraceTreePlan** out_plan, DeviceMbvh* out, bool cached) {
     if (width != 4 && width != 8) {
         set_error("trace tree width must be 4 or 8");
         return InvalidArgument;
@@ -900,12 +905,14 @@ ResultCode trace_tree_plan_create(const DeviceBvh& tree, uint32_t max_leaf, uint
         o_dpc = carve(size_t(t->count) * 8 * sizeof(float));
         o_dpd = carve(size_t(t->count) * 4);
     }
-    hipError_t e = hipMalloc(&t->ws, bytes);
+    hipError_t e =
+        cached ? workspace_acquire(kWsTree, bytes, &t->wsh) : hipMalloc(&t->ws, bytes);
     if (e != hipSuccess) {
         t->ws = nullptr;
         trace_tree_plan_destroy(t);
         return hip_fail(e, "trace tree workspace");
     }
+    if (cached) t->ws = t->wsh.ptr;
     char* base = static_cast<char*>(t->ws);
     t->cnt = reinterpret_cast<uint32_t*>(base + o_cnt);
     t->q[0] = reinterpret_cast<QItem*>(base + o_q0);
@@ -941,6 +948,7 @@ ResultCode trace_tree_plan_create(const DeviceBvh& tree, uint32_t max_leaf, uint
 ResultCode trace_tree_build(TraceTreePlan* t, const DeviceBvh& tree, const float* verts,
                             uint32_t prim_count, const uint32_t* scene_off, uint32_t scenes,
                             DeviceMbvh* out, hipStream_t stream) {
+    t->last_stream = stream;
     hipError_t e = enqueue_setup(t, tree, stream);
     if (e != hipSuccess) return hip_fail(e, "trace tree setup");
     const LevelArgs a = level_args(t, tree, verts, prim_count, scene_off, scenes, out);
