@@ -30,6 +30,13 @@
  * lock, as the reference's do: freeing or refitting a tree while another
  * thread traverses it is the caller's race.  Calls on one batch plan must not
  * overlap, its free included.
+ *
+ * Memory: the builders behind create_bvh, rtbvh_gpu_create_bvh,
+ * rtbvh_gpu_create_batch and rtbvh_gpu_create_trace_tree* keep their scratch
+ * between calls, one block per builder kind (PLOC, binned SAH, traversal
+ * tree) and device, grown to the largest build up to 1 GiB each and held
+ * until the process ends; concurrent builds on one device and larger builds
+ * allocate and free their own.
  */
 #ifndef RTBVH_H
 #define RTBVH_H
