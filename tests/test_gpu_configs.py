@@ -162,6 +162,32 @@ def test_c2_work_counters_match_oracle(scene_1m):
     assert got[0] == stats[0] and got[1] == stats[1]
 
 
+def test_c2_reference_order_traversals_match_oracle(scene_1m):
+    """The reference's own benchmark loop at the headline size
+    (examples/benchmark.rs:15-41: the MBVH and BVH iterators in their visit
+    order, the first accepted candidate at the lowest t) on the GPU over the
+    1M tree, against the oracle's restatement of the same loop, on every
+    ninth bounce ray (262,144 rays), bit for bit."""
+    s = scene_1m
+    rays = s["bounce"][::9].contiguous()
+    print(f"[c2] reference-order traversals, {rays.shape[0]} rays", flush=True)
+    dev = rays.device
+    idx_d = torch.from_numpy(s["idx"].view(np.int32)).to(dev)
+    mbvh = rt.Mbvh.from_arrays(torch.from_numpy(flat(s["m"])).to(dev), idx_d)
+    bvh = rt.Bvh.from_arrays(torch.from_numpy(flat(s["nodes"])).to(dev), idx_d)
+    np_rays = _np_rays(rays)
+    for name, tree, want in (
+            ("mbvh", mbvh, ref.trace_mbvh(s["m"], s["idx"], s["verts"], np_rays,
+                                          mode=ref.MODE_VISIT_ORDER, threads=THREADS)),
+            ("bvh", bvh, ref.trace_bvh(s["nodes"], s["idx"], s["verts"], np_rays,
+                                       mode=ref.MODE_VISIT_ORDER, threads=THREADS))):
+        got = _np_hits(tree.intersect(s["v_d"], rays, reference_order=True))
+        tree.status()
+        bad = ~hits_equal(got, want)
+        assert not bad.any(), f"{name}: {bad.sum()} rays differ, first {np.nonzero(bad)[0][:5]}"
+        assert (want["prim"] != 0xFFFFFFFF).mean() > 0.3
+
+
 def _subtree_equal(na, ia, ra, off_a, nb, ib, rb):
     """Node-for-node equality of the subtree of (na, ia) at ra, whose leaves
     hold batch-wide ids offset by off_a, and the tree (nb, ib) at rb."""
