@@ -32,11 +32,12 @@
  * overlap, its free included.
  *
  * Memory: the builders behind create_bvh, rtbvh_gpu_create_bvh,
- * rtbvh_gpu_create_batch and rtbvh_gpu_create_trace_tree* keep their scratch
- * between calls, one block per builder kind (PLOC, binned SAH, traversal
- * tree) and device, grown to the largest build up to 1 GiB each and held
- * until the process ends; concurrent builds on one device and larger builds
- * allocate and free their own.
+ * rtbvh_gpu_create_spatial_bvh, rtbvh_gpu_create_batch and
+ * rtbvh_gpu_create_trace_tree* keep their scratch between calls, one block
+ * per builder kind (PLOC, binned SAH, spatial SAH, traversal tree) and
+ * device, grown to the largest build up to 1 GiB each and held until the
+ * process ends; concurrent builds on one device and larger builds allocate
+ * and free their own.
  */
 #ifndef RTBVH_H
 #define RTBVH_H

@@ -40,10 +40,6 @@
 
 namespace rtbvh {
 
-hipError_t scan_exclusive_u32(uint32_t* data, uint32_t count, uint32_t* total, void* scratch,
-                              size_t scratch_bytes, hipStream_t stream);
-size_t scan_scratch_bytes(uint32_t count);
-
 namespace {
 
 constexpr int kBins = 16;
@@ -68,7 +64,24 @@ static_assert(sizeof(SRef) == 32, "reference record is 32 bytes");
 
 struct STask {
     uint32_t node, begin, end, depth;
+    uint32_t slot;  // object bins in obins (more than kSmallTask references), else kNoSlot
+    uint32_t pad[3];
 };
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+
+// Device record of one level, written by the level before (the root's by
+// root_task_kernel; levels past the last carry the final counts): the
+// level's counts, then its scan totals.
+struct SLevel {
+    uint32_t num_tasks, nrefs, counter, leaf_base;
+    uint32_t dups;  // duplicate references made by the levels before
+    uint32_t pad0[3];
+    uint32_t cand_total, dup_total, splits, next_refs;
+    uint32_t leaf_refs, slots;
+    uint32_t fail;  // 1: out of reference, node or task space; 2: references lost
+    uint32_t pad1;
+};
+static_assert(sizeof(SLevel) == 64, "level record is 64 bytes");
 
 // Box accumulator with order-preserving float encodings; c0/c1 are counts
 // (object bins: c0 references; spatial bins: c0 entries, c1 exits).
@@ -223,23 +236,28 @@ __global__ void point_refs_kernel(const float* centers, uint32_t stride, uint32_
 // Per task: node box and object bins cleared (block t, 64 threads; a task of
 // at most kSmallTask references has no stored object bins: obj_split_kernel
 // bins it in registers).
-__global__ void task_clear_kernel(uint32_t num_tasks, const STask* tasks, Acc* tbox, Acc* obins) {
-    const uint32_t t = blockIdx.x;
-    if (t >= num_tasks) return;
-    if (threadIdx.x == 0) acc_clear(tbox[t]);
-    if (threadIdx.x < 3 * kBins && tasks[t].end - tasks[t].begin > kSmallTask) {
-        Acc a;
-        acc_clear(a);
-        obins[size_t(t) * 3 * kBins + threadIdx.x] = a;
+__global__ void task_clear_kernel(const SLevel* L, const STask* tasks, Acc* tbox, Acc* obins) {
+    const uint32_t num_tasks = L->num_tasks;
+    for (uint32_t t = blockIdx.x; t < num_tasks; t += gridDim.x) {
+        if (threadIdx.x == 0) acc_clear(tbox[t]);
+        const uint32_t slot = tasks[t].slot;
+        if (threadIdx.x < 3 * kBins && slot != kNoSlot) {
+            Acc a;
+            acc_clear(a);
+            obins[size_t(slot) * 3 * kBins + threadIdx.x] = a;
+        }
     }
 }
 
-__global__ void cand_clear_kernel(uint32_t count, Acc* sbins) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    Acc a;
-    acc_clear(a);
-    sbins[i] = a;
+// The spatial bins of the level's candidates (at most kMaxCand) cleared.
+__global__ void cand_clear_kernel(const SLevel* L, Acc* sbins) {
+    const uint32_t count = min(L->cand_total, kMaxCand) * 3 * kBins;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += gridDim.x * blockDim.x) {
+        Acc a;
+        acc_clear(a);
+        sbins[i] = a;
+    }
 }
 
 // A level's references are ordered by task (the scatter writes each split
@@ -266,9 +284,11 @@ __device__ __forceinline__ bool in_run(int32_t t, int32_t t0, int32_t nt) {
 // Node boxes: the union of each task's reference boxes (tasks of at most
 // kSmallTask references are left to task_setup_kernel).
 __global__ void __launch_bounds__(kBlock)
-    ref_bounds_kernel(uint32_t nrefs, const SRef* refs, const STask* tasks, Acc* tbox) {
+    ref_bounds_kernel(const SLevel* L, const SRef* refs, const STask* tasks, Acc* tbox) {
     __shared__ uint32_t s_mn[kLdsTasks][3], s_mx[kLdsTasks][3];
     __shared__ int32_t s_t0, s_nt;
+    const uint32_t nrefs = L->nrefs;
+    if (blockIdx.x * kBlock >= nrefs) return;  // whole block
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (threadIdx.x == 0) {
         int32_t nt = 0;
@@ -305,11 +325,11 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Padded node box and bin parameters; the root's surface at level 0.
-__global__ void task_setup_kernel(uint32_t num_tasks, const STask* tasks, const SRef* refs,
+__global__ void task_setup_kernel(const SLevel* L, const STask* tasks, const SRef* refs,
                                   const Acc* tbox, SDec* dec, float* root_half_area,
                                   uint32_t level) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= num_tasks) return;
+    if (t >= L->num_tasks) return;
     SDec d;
     const STask tk = tasks[t];
     Acc b;
@@ -352,10 +372,12 @@ __global__ void task_setup_kernel(uint32_t num_tasks, const STask* tasks, const 
 }
 
 __global__ void __launch_bounds__(kBlock)
-    obj_bin_kernel(uint32_t nrefs, const SRef* refs, const STask* tasks, const SDec* dec,
+    obj_bin_kernel(const SLevel* L, const SRef* refs, const STask* tasks, const SDec* dec,
                    Acc* obins) {
     __shared__ Acc s_b[kLdsTasks][3 * kBins];
     __shared__ int32_t s_t0, s_nt;
+    const uint32_t nrefs = L->nrefs;
+    if (blockIdx.x * kBlock >= nrefs) return;  // whole block
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (threadIdx.x == 0) {
         int32_t nt = 0;
@@ -377,7 +399,7 @@ __global__ void __launch_bounds__(kBlock)
                     const float c = 0.5f * (r.mn[ax] + r.mx[ax]);
                     const uint32_t slot = ax * kBins + bin_of(c, d.org[ax], d.inv[ax]);
                     Acc* g = in_run(t, t0, s_nt) ? &s_b[t - t0][slot]
-                                                 : &obins[size_t(t) * 3 * kBins + slot];
+                                                 : &obins[size_t(tk.slot) * 3 * kBins + slot];
                     atomicAdd(&g->c0, 1u);
                     for (int k = 0; k < 3; ++k) {
                         if (!(r.mn[k] <= r.mx[k])) continue;
@@ -394,7 +416,7 @@ __global__ void __launch_bounds__(kBlock)
             const uint32_t tt = k / (3 * kBins), slot = k % (3 * kBins);
             const Acc& a = s_b[tt][slot];
             if (a.c0 == 0) continue;
-            Acc* g = &obins[size_t(t0 + tt) * 3 * kBins + slot];
+            Acc* g = &obins[size_t(tasks[t0 + tt].slot) * 3 * kBins + slot];
             atomicAdd(&g->c0, a.c0);
             for (int q = 0; q < 3; ++q) {
                 atomicMin(&g->mn[q], a.mn[q]);
@@ -463,17 +485,16 @@ __device__ __forceinline__ GRun grun_pick(const GRun (&r)[3], uint32_t ax) {
 // reference q, lane j gathers bin j with obj_bin_kernel's ordered-encoding
 // minimum and maximum); decide_kernel counts its references itself.
 __global__ void __launch_bounds__(kBlock)
-    obj_split_kernel(uint32_t num_tasks, const STask* tasks, SDec* dec, const Acc* obins,
+    obj_split_kernel(const SLevel* L, const STask* tasks, SDec* dec, const Acc* obins,
                      const SRef* refs, const float* root_half_area, uint32_t* cand_flag) {
     const uint32_t t = (blockIdx.x * kBlock + threadIdx.x) / kBins;
     const int lane = int(threadIdx.x % kBins);
-    if (t >= num_tasks) return;  // whole groups: kBlock is a multiple of kBins
+    if (t >= L->num_tasks) return;  // whole groups: kBlock is a multiple of kBins
     const STask tk = tasks[t];
     SDec d = dec[t];
     const uint32_t work = tk.end - tk.begin;
     uint32_t cand = 0;
     if (work > 1 && tk.depth < kMaxDepth) {
-        const Acc* tb = obins + size_t(t) * 3 * kBins;
         Acc own[3];
         if (work <= kSmallTask) {
             uint32_t pb[3] = {0, 0, 0}, pmn[3], pmx[3];
@@ -509,6 +530,7 @@ __global__ void __launch_bounds__(kBlock)
                     }
             }
         } else {
+            const Acc* tb = obins + size_t(tk.slot) * 3 * kBins;
             for (int ax = 0; ax < 3; ++ax) own[ax] = tb[ax * kBins + lane];
         }
         GRun pre[3], suf[3];
@@ -590,10 +612,12 @@ __device__ __forceinline__ bool sp_evaluated(const SDec* dec, const uint32_t* ca
 }
 
 __global__ void __launch_bounds__(kBlock)
-    sp_bin_kernel(uint32_t nrefs, const SRef* refs, const SDec* dec, const uint32_t* cand_excl,
+    sp_bin_kernel(const SLevel* L, const SRef* refs, const SDec* dec, const uint32_t* cand_excl,
                   const float* verts, uint32_t prim_count, Acc* sbins) {
     __shared__ Acc s_b[kLdsTasks][3 * kBins];
     __shared__ int32_t s_t0, s_nt;
+    const uint32_t nrefs = L->nrefs;
+    if (blockIdx.x * kBlock >= nrefs || L->cand_total == 0) return;  // whole block
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (threadIdx.x == 0) {
         int32_t nt = 0;
@@ -662,10 +686,10 @@ __global__ void __launch_bounds__(kBlock)
 
 // Best spatial split of each evaluated candidate; dup[t] = its duplicates
 // when it beats the object split (0 otherwise).
-__global__ void sp_split_kernel(uint32_t num_tasks, const STask* tasks, SDec* dec,
+__global__ void sp_split_kernel(const SLevel* L, const STask* tasks, SDec* dec,
                                 const uint32_t* cand_excl, const Acc* sbins, uint32_t* dup) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= num_tasks) return;
+    if (t >= L->num_tasks) return;
     SDec d = dec[t];
     uint32_t want = 0;
     if (d.cand && cand_excl[t] < kMaxCand) {
@@ -711,13 +735,17 @@ __global__ void sp_split_kernel(uint32_t num_tasks, const STask* tasks, SDec* de
     dup[t] = want ? d.dup : 0u;
 }
 
-// Final decision per task; counts for the scans.
-__global__ void decide_kernel(uint32_t num_tasks, const STask* tasks, SDec* dec,
-                              const uint32_t* dup_excl, uint32_t remaining, uint32_t max_leaf,
+// Final decision per task; counts for the scans (big_cnt: children of more
+// than kSmallTask references, which get object-bin slots).  A spatial split
+// is taken while the duplicates of the wanted spatial splits up to and
+// including its own fit what the budget has left after the levels before.
+__global__ void decide_kernel(const SLevel* L, const STask* tasks, SDec* dec,
+                              const uint32_t* dup_excl, uint32_t budget, uint32_t max_leaf,
                               const Acc* obins, const SRef* refs, uint32_t* split_flag,
-                              uint32_t* out_cnt, uint32_t* leaf_cnt) {
+                              uint32_t* out_cnt, uint32_t* leaf_cnt, uint32_t* big_cnt) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= num_tasks) return;
+    if (t >= L->num_tasks) return;
+    const uint32_t remaining = L->dups < budget ? budget - L->dups : 0u;
     const STask tk = tasks[t];
     SDec d = dec[t];
     const uint32_t work = tk.end - tk.begin;
@@ -742,7 +770,9 @@ __global__ void decide_kernel(uint32_t num_tasks, const STask* tasks, SDec* dec,
                 uint32_t ax = 0;
                 if (d.mx[1] - d.mn[1] > d.mx[0] - d.mn[0]) ax = 1;
                 if (d.mx[2] - d.mn[2] > d.mx[ax] - d.mn[ax]) ax = 2;
-                const Acc* b = obins + size_t(t) * 3 * kBins + ax * kBins;
+                const Acc* b = work > kSmallTask
+                                   ? obins + size_t(tk.slot) * 3 * kBins + ax * kBins
+                                   : nullptr;
                 uint32_t c = 0, k = 0;
                 for (int i = 0; i < kBins - 1; ++i) {
                     if (work > kSmallTask) {
@@ -782,18 +812,23 @@ __global__ void decide_kernel(uint32_t num_tasks, const STask* tasks, SDec* dec,
         d.kind = kLeaf;
     }
     dec[t] = d;
-    split_flag[t] = d.kind != kLeaf ? 1u : 0u;
-    out_cnt[t] = d.kind != kLeaf ? d.left + d.right : 0u;
-    leaf_cnt[t] = d.kind == kLeaf ? work : 0u;
+    const bool split = d.kind != kLeaf;
+    split_flag[t] = split ? 1u : 0u;
+    out_cnt[t] = split ? d.left + d.right : 0u;
+    leaf_cnt[t] = split ? 0u : work;
+    big_cnt[t] = split ? (d.left > kSmallTask ? 1u : 0u) + (d.right > kSmallTask ? 1u : 0u) : 0u;
 }
 
-__global__ void children_kernel(uint32_t num_tasks, const STask* tasks, const SDec* dec,
+// Node records and the next level's tasks (the children of split task t
+// are tasks 2 split_excl[t] and the one after, nodes counter + the same).
+__global__ void children_kernel(const SLevel* L, const STask* tasks, const SDec* dec,
                                 const uint32_t* split_excl, const uint32_t* out_excl,
-                                const uint32_t* leaf_excl, uint32_t counter, uint32_t leaf_base,
-                                uint32_t node_cap, DevAabb* nodes, STask* next,
+                                const uint32_t* leaf_excl, const uint32_t* big_excl,
+                                uint32_t node_cap, uint32_t task_cap, DevAabb* nodes, STask* next,
                                 uint32_t* fail) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= num_tasks) return;
+    if (t >= L->num_tasks) return;
+    const uint32_t counter = L->counter, leaf_base = L->leaf_base;
     const STask tk = tasks[t];
     const SDec& d = dec[t];
     DevAabb nd;
@@ -803,7 +838,7 @@ __global__ void children_kernel(uint32_t num_tasks, const STask* tasks, const SD
     }
     if (d.kind != kLeaf) {
         const uint32_t child = counter + 2 * split_excl[t];
-        if (child + 1 >= node_cap) {
+        if (child + 1 >= node_cap || 2 * split_excl[t] + 1 >= task_cap) {
             atomicOr(fail, 1u);
             return;
         }
@@ -817,6 +852,9 @@ __global__ void children_kernel(uint32_t num_tasks, const STask* tasks, const SD
         r.begin = l.end;
         r.end = r.begin + d.right;
         l.depth = r.depth = tk.depth + 1;
+        l.slot = d.left > kSmallTask ? big_excl[t] : kNoSlot;
+        r.slot = d.right > kSmallTask ? big_excl[t] + (d.left > kSmallTask ? 1u : 0u) : kNoSlot;
+        l.pad[0] = l.pad[1] = l.pad[2] = r.pad[0] = r.pad[1] = r.pad[2] = 0;
         next[2 * split_excl[t]] = l;
         next[2 * split_excl[t] + 1] = r;
     } else {
@@ -829,11 +867,11 @@ __global__ void children_kernel(uint32_t num_tasks, const STask* tasks, const SD
 // Which child (or both) each reference goes to.
 // Side flags of each reference, written twice: kept in fl/fr, scanned in
 // place in fle/fre.
-__global__ void side_kernel(uint32_t nrefs, const SRef* refs, const STask* tasks,
+__global__ void side_kernel(const SLevel* L, const SRef* refs, const STask* tasks,
                             const SDec* dec, uint32_t* fl, uint32_t* fr, uint32_t* fle,
                             uint32_t* fre) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nrefs) return;
+    if (i >= L->nrefs) return;
     const SRef r = refs[i];
     uint32_t l = 0, rr = 0;
     if (r.task >= 0) {
@@ -858,15 +896,15 @@ __global__ void side_kernel(uint32_t nrefs, const SRef* refs, const STask* tasks
     fre[i] = rr;
 }
 
-__global__ void scatter_kernel(uint32_t nrefs, const SRef* refs, const STask* tasks,
+__global__ void scatter_kernel(const SLevel* L, const SRef* refs, const STask* tasks,
                                const SDec* dec, const uint32_t* split_excl,
                                const uint32_t* out_excl, const uint32_t* leaf_excl,
-                               uint32_t leaf_base, const uint32_t* fl, const uint32_t* fr,
-                               const uint32_t* fl_excl, const uint32_t* fr_excl,
-                               const float* verts, uint32_t prim_count, uint32_t cap,
-                               SRef* out, uint32_t* indices, uint32_t* fail) {
+                               const uint32_t* fl, const uint32_t* fr, const uint32_t* fl_excl,
+                               const uint32_t* fr_excl, const float* verts, uint32_t prim_count,
+                               uint32_t cap, SRef* out, uint32_t* indices, uint32_t* fail) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nrefs) return;
+    if (i >= L->nrefs) return;
+    const uint32_t leaf_base = L->leaf_base;
     const SRef r = refs[i];
     if (r.task < 0) return;
     const STask tk = tasks[r.task];
@@ -925,34 +963,57 @@ __global__ void scatter_kernel(uint32_t nrefs, const SRef* refs, const STask* ta
     }
 }
 
-__global__ void root_task_kernel(STask* task, uint32_t n) {
+// The root task and level 0's record (the records are zero before).
+__global__ void root_task_kernel(STask* task, SLevel* levels, uint32_t n) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         STask t;
         t.node = 0;
         t.begin = 0;
         t.end = n;
         t.depth = 0;
+        t.slot = n > kSmallTask ? 0u : kNoSlot;
+        t.pad[0] = t.pad[1] = t.pad[2] = 0;
         *task = t;
+        levels[0].num_tasks = 1;
+        levels[0].nrefs = n;
+        levels[0].counter = 1;
     }
 }
 
-template <typename T>
-struct Buf {
-    T* p = nullptr;
-    size_t cap = 0;
-    ~Buf() {
-        if (p) (void)hipFree(p);
+// levels[level + 1] from the counts and scan totals of levels[level]; a
+// level without tasks, or one that failed, leaves none to the next, so
+// every later level returns at once and carries the final counts.
+__global__ void level_end_kernel(SLevel* levels, uint32_t level, uint32_t cap,
+                                 const uint32_t* fail_word) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const SLevel L = levels[level];
+    SLevel nx;
+    nx.num_tasks = 0;
+    nx.nrefs = 0;
+    nx.counter = L.counter;
+    nx.leaf_base = L.leaf_base;
+    nx.dups = L.dups;
+    nx.pad0[0] = nx.pad0[1] = nx.pad0[2] = 0;
+    nx.cand_total = nx.dup_total = nx.splits = nx.next_refs = 0;
+    nx.leaf_refs = nx.slots = 0;
+    nx.fail = L.fail | *fail_word;
+    nx.pad1 = 0;
+    if (L.num_tasks > 0 && nx.fail == 0) {
+        if (L.next_refs + L.leaf_refs < L.nrefs || L.next_refs > cap ||
+            L.leaf_base + L.leaf_refs > cap) {
+            nx.fail = 2;
+        } else {
+            nx.num_tasks = 2 * L.splits;
+            nx.nrefs = L.next_refs;
+            nx.counter = L.counter + 2 * L.splits;
+            nx.leaf_base = L.leaf_base + L.leaf_refs;
+            nx.dups = L.dups + (L.next_refs + L.leaf_refs - L.nrefs);
+        }
     }
-    hipError_t ensure(size_t count) {
-        if (count <= cap) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        hipError_t e = hipMalloc(&p, count * sizeof(T) + 16);
-        if (e == hipSuccess) cap = count;
-        return e;
-    }
-};
+    levels[level + 1] = nx;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 }  // namespace
 
@@ -987,141 +1048,174 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
         (void)hipFree(res.indices);
         return Error;
     };
-    Buf<SRef> r0, r1;
-    Buf<STask> t0, t1;
-    Buf<SDec> dec;
-    Buf<Acc> tbox, obins, sbins;
-    Buf<uint32_t> split_flag, out_cnt, leaf_cnt, cand, dup, fl, fr, fle, fre, scratch, words;
-    uint32_t* host = nullptr;  // scan totals: splits, next refs, leaf refs, candidates, dups
-    if ((e = hipHostMalloc(&host, 8 * 4)) != hipSuccess) return fail(e, "hipHostMalloc");
-    struct HostFree {
-        uint32_t* p;
-        ~HostFree() { (void)hipHostFree(p); }
-    } hf{host};
-    const size_t sbytes = scan_scratch_bytes(cap) + 64;
-    if ((e = r0.ensure(cap)) != hipSuccess || (e = r1.ensure(cap)) != hipSuccess ||
-        (e = t0.ensure(1)) != hipSuccess || (e = fl.ensure(cap)) != hipSuccess ||
-        (e = fr.ensure(cap)) != hipSuccess || (e = fle.ensure(cap)) != hipSuccess ||
-        (e = fre.ensure(cap)) != hipSuccess || (e = scratch.ensure(sbytes / 4 + 1)) != hipSuccess ||
-        (e = words.ensure(4)) != hipSuccess ||
-        (e = sbins.ensure(size_t(std::min<uint32_t>(cap, kMaxCand)) * 3 * kBins)) != hipSuccess)
+    // One workspace.  A level holds at most cap references and at most cap
+    // tasks (each has a reference); at most cap / (kSmallTask + 1) tasks
+    // have object-bin slots.
+    constexpr uint32_t kLevels = kMaxDepth + 3;  // records of levels 0 .. kMaxDepth + 2
+    const size_t slot_cap = size_t(cap) / (kSmallTask + 1) + 1;
+    const size_t sz_refs = align256(size_t(cap) * sizeof(SRef));
+    const size_t sz_tasks = align256(size_t(cap) * sizeof(STask));
+    const size_t sz_dec = align256(size_t(cap) * sizeof(SDec));
+    const size_t sz_tbox = align256(size_t(cap) * sizeof(Acc));
+    const size_t sz_obins = align256(slot_cap * 3 * kBins * sizeof(Acc));
+    const size_t sz_sbins = align256(size_t(std::min(cap, kMaxCand)) * 3 * kBins * sizeof(Acc));
+    const size_t sz_u32 = align256(size_t(cap) * 4);
+    const size_t sz_scan = align256(scan_dev_scratch_bytes(cap, 4));
+    const size_t sz_levels = align256(size_t(kLevels) * sizeof(SLevel));
+    const size_t sz_words = 256;
+    const size_t total = 2 * sz_refs + 2 * sz_tasks + sz_dec + sz_tbox + sz_obins + sz_sbins +
+                         10 * sz_u32 + sz_scan + sz_levels + sz_words;
+    Workspace wsh;
+    if ((e = workspace_acquire(kWsSpatial, total, &wsh)) != hipSuccess)
         return fail(e, "spatial builder workspace");
-    float* root_half_area = reinterpret_cast<float*>(words.p);
-    uint32_t* fail_word = words.p + 1;
-    if ((e = hipMemsetAsync(words.p, 0, 16, stream)) != hipSuccess)
+    // released on every return, after the stream's work that used it
+    struct WsRelease {
+        Workspace* w;
+        hipStream_t s;
+        ~WsRelease() { workspace_release(w, s, true); }
+    } wsr{&wsh, stream};
+    SLevel* host_level = nullptr;
+    if ((e = hipHostMalloc(&host_level, sizeof(SLevel))) != hipSuccess)
+        return fail(e, "hipHostMalloc");
+    struct HostFree {
+        SLevel* p;
+        ~HostFree() { (void)hipHostFree(p); }
+    } hf{host_level};
+    char* cur = static_cast<char*>(wsh.ptr);
+    auto carve = [&](size_t bytes) {
+        char* p = cur;
+        cur += bytes;
+        return p;
+    };
+    SRef* refs[2] = {reinterpret_cast<SRef*>(carve(sz_refs)),
+                     reinterpret_cast<SRef*>(carve(sz_refs))};
+    STask* tasks[2] = {reinterpret_cast<STask*>(carve(sz_tasks)),
+                       reinterpret_cast<STask*>(carve(sz_tasks))};
+    SDec* dec = reinterpret_cast<SDec*>(carve(sz_dec));
+    Acc* tbox = reinterpret_cast<Acc*>(carve(sz_tbox));
+    Acc* obins = reinterpret_cast<Acc*>(carve(sz_obins));
+    Acc* sbins = reinterpret_cast<Acc*>(carve(sz_sbins));
+    uint32_t* u32[10];
+    for (int k = 0; k < 10; ++k) u32[k] = reinterpret_cast<uint32_t*>(carve(sz_u32));
+    uint32_t *split_flag = u32[0], *out_cnt = u32[1], *leaf_cnt = u32[2], *big_cnt = u32[3];
+    uint32_t *cand = u32[4], *dup = u32[5], *fl = u32[6], *fr = u32[7], *fle = u32[8],
+             *fre = u32[9];
+    void* scan_scratch = carve(sz_scan);
+    char* cleared = cur;
+    SLevel* levels = reinterpret_cast<SLevel*>(carve(sz_levels));
+    uint32_t* words = reinterpret_cast<uint32_t*>(carve(sz_words));
+    float* root_half_area = reinterpret_cast<float*>(words);
+    uint32_t* fail_word = words + 1;
+    if ((e = hipMemsetAsync(cleared, 0, sz_levels + sz_words, stream)) != hipSuccess)
         return fail(e, "spatial builder state");
     const uint32_t g0 = (n + kBlock - 1) / kBlock;
     if (aabbs)
         hipLaunchKernelGGL(init_refs_kernel, dim3(g0), dim3(kBlock), 0, stream,
-                           reinterpret_cast<const DevAabb*>(aabbs), n, r0.p);
+                           reinterpret_cast<const DevAabb*>(aabbs), n, refs[0]);
     else
         hipLaunchKernelGGL(point_refs_kernel, dim3(g0), dim3(kBlock), 0, stream, centers, stride,
-                           n, r0.p);
-    hipLaunchKernelGGL(root_task_kernel, dim3(1), dim3(1), 0, stream, t0.p, n);
-    SRef* ref_cur = r0.p;
-    SRef* ref_nxt = r1.p;
-    STask* t_cur = t0.p;
-    uint32_t nrefs = n, num_tasks = 1, counter = 1, leaf_base = 0, dups = 0;
+                           n, refs[0]);
+    hipLaunchKernelGGL(root_task_kernel, dim3(1), dim3(1), 0, stream, tasks[0], levels, n);
     const uint32_t budget = cap - n;
-    int flip = 0;
-    // Per-task buffers hold tcap tasks: at least 65536 (at most cap, as every
-    // task has a reference), doubled when a level needs more, so the levels
-    // near the root, whose task count doubles, do not free and allocate
-    // every buffer each time.
-    size_t tcap = 0;
-    for (uint32_t level = 0; num_tasks > 0; ++level) {
-        if (level > kMaxDepth + 1) {
-            set_error("spatial builder did not finish in %u levels", kMaxDepth + 1);
-            return fail_msg();
+    // Levels are enqueued in batches without waiting; the host reads one
+    // level record per batch.  Grids come from bounds on the counts of the
+    // record read last (level r: T tasks, N references, D duplicates): a
+    // later level holds at most N + (budget - D) references and at most
+    // T << (levels since r) tasks.  The first batch covers a tree twice as
+    // deep as a balanced one, then batches of 8 until a level has no task.
+    uint32_t lg = 0;
+    while (lg < 31 && (1u << lg) < n) ++lg;
+    uint32_t batch = std::min<uint32_t>(kMaxDepth + 2, 2 * lg + 2);
+    uint32_t level = 0;
+    uint64_t known_tasks = 1, known_refs = n, known_dups = 0;
+    for (;;) {
+        const uint32_t end = std::min<uint32_t>(kMaxDepth + 2, level + batch);
+        const uint32_t ref_bound =
+            uint32_t(std::min<uint64_t>(cap, known_refs + (budget - known_dups)));
+        const uint32_t rg = (ref_bound + kBlock - 1) / kBlock;
+        for (uint32_t k = level; k < end; ++k) {
+            const uint32_t sh = std::min<uint32_t>(k - level, 32);
+            const uint32_t tb = uint32_t(std::min<uint64_t>(ref_bound, known_tasks << sh));
+            const uint32_t tg = (tb + kBlock - 1) / kBlock;
+            const uint32_t gg = (tb + kBlock / kBins - 1) / (kBlock / kBins);  // task groups
+            const uint32_t ccl = std::min(tb, kMaxCand) * 3 * kBins;
+            const uint32_t cg = std::min<uint32_t>((ccl + kBlock - 1) / kBlock, 4096);
+            const SLevel* L = levels + k;
+            SLevel* Lw = levels + k;
+            const uint32_t a = k & 1u, b = a ^ 1u;
+            const SRef* rc = refs[a];
+            const STask* tc = tasks[a];
+            hipLaunchKernelGGL(task_clear_kernel, dim3(std::min<uint32_t>(tb, 4096)), dim3(64), 0,
+                               stream, L, tc, tbox, obins);
+            hipLaunchKernelGGL(ref_bounds_kernel, dim3(rg), dim3(kBlock), 0, stream, L, rc, tc,
+                               tbox);
+            hipLaunchKernelGGL(task_setup_kernel, dim3(tg), dim3(kBlock), 0, stream, L, tc, rc,
+                               tbox, dec, root_half_area, k);
+            hipLaunchKernelGGL(obj_bin_kernel, dim3(rg), dim3(kBlock), 0, stream, L, rc, tc, dec,
+                               obins);
+            hipLaunchKernelGGL(obj_split_kernel, dim3(gg), dim3(kBlock), 0, stream, L, tc, dec,
+                               obins, rc, root_half_area, cand);
+            if ((e = scan_exclusive_dev(ScanArrays{{cand}}, 1, &L->num_tasks, tb,
+                                        ScanTotals{{&Lw->cand_total}}, scan_scratch, sz_scan,
+                                        stream)) != hipSuccess)
+                return fail(e, "spatial candidate scan");
+            hipLaunchKernelGGL(cand_clear_kernel, dim3(cg), dim3(kBlock), 0, stream, L, sbins);
+            hipLaunchKernelGGL(sp_bin_kernel, dim3(rg), dim3(kBlock), 0, stream, L, rc, dec, cand,
+                               verts, n, sbins);
+            hipLaunchKernelGGL(sp_split_kernel, dim3(tg), dim3(kBlock), 0, stream, L, tc, dec, cand,
+                               sbins, dup);
+            if ((e = scan_exclusive_dev(ScanArrays{{dup}}, 1, &L->num_tasks, tb,
+                                        ScanTotals{{&Lw->dup_total}}, scan_scratch, sz_scan,
+                                        stream)) != hipSuccess)
+                return fail(e, "spatial budget scan");
+            hipLaunchKernelGGL(decide_kernel, dim3(tg), dim3(kBlock), 0, stream, L, tc, dec, dup,
+                               budget, leaf, obins, rc, split_flag, out_cnt, leaf_cnt, big_cnt);
+            if ((e = scan_exclusive_dev(
+                     ScanArrays{{split_flag, out_cnt, leaf_cnt, big_cnt}}, 4, &L->num_tasks, tb,
+                     ScanTotals{{&Lw->splits, &Lw->next_refs, &Lw->leaf_refs, &Lw->slots}},
+                     scan_scratch, sz_scan, stream)) != hipSuccess)
+                return fail(e, "spatial task scans");
+            hipLaunchKernelGGL(children_kernel, dim3(tg), dim3(kBlock), 0, stream, L, tc, dec,
+                               split_flag, out_cnt, leaf_cnt, big_cnt, node_cap, cap, res.nodes,
+                               tasks[b], fail_word);
+            hipLaunchKernelGGL(side_kernel, dim3(rg), dim3(kBlock), 0, stream, L, rc, tc, dec, fl,
+                               fr, fle, fre);
+            if ((e = scan_exclusive_dev(ScanArrays{{fle, fre}}, 2, &L->nrefs, ref_bound,
+                                        ScanTotals{{nullptr, nullptr}}, scan_scratch, sz_scan,
+                                        stream)) != hipSuccess)
+                return fail(e, "spatial partition scans");
+            hipLaunchKernelGGL(scatter_kernel, dim3(rg), dim3(kBlock), 0, stream, L, rc, tc, dec,
+                               split_flag, out_cnt, leaf_cnt, fl, fr, fle, fre, verts, n, cap,
+                               refs[b], res.indices, fail_word);
+            hipLaunchKernelGGL(level_end_kernel, dim3(1), dim3(64), 0, stream, levels, k, cap,
+                               fail_word);
         }
-        if (num_tasks > tcap)
-            tcap = std::max<size_t>(
-                num_tasks, std::min<size_t>(cap, std::max<size_t>(65536, 2 * size_t(num_tasks))));
-        if ((e = dec.ensure(tcap)) != hipSuccess || (e = tbox.ensure(tcap)) != hipSuccess ||
-            (e = obins.ensure(tcap * 3 * kBins)) != hipSuccess ||
-            (e = split_flag.ensure(tcap)) != hipSuccess ||
-            (e = out_cnt.ensure(tcap)) != hipSuccess || (e = leaf_cnt.ensure(tcap)) != hipSuccess ||
-            (e = cand.ensure(tcap)) != hipSuccess || (e = dup.ensure(tcap)) != hipSuccess)
-            return fail(e, "spatial level workspace");
-        Buf<STask>& nb = flip ? t0 : t1;
-        if ((e = nb.ensure(2 * tcap)) != hipSuccess) return fail(e, "spatial tasks");
-        STask* t_nxt = nb.p;
-        const uint32_t tg = (num_tasks + kBlock - 1) / kBlock;
-        const uint32_t rg = (nrefs + kBlock - 1) / kBlock;
-        const uint32_t ncand = std::min<uint32_t>(num_tasks, kMaxCand);
-        hipLaunchKernelGGL(task_clear_kernel, dim3(num_tasks), dim3(64), 0, stream, num_tasks,
-                           t_cur, tbox.p, obins.p);
-        hipLaunchKernelGGL(ref_bounds_kernel, dim3(rg), dim3(kBlock), 0, stream, nrefs, ref_cur,
-                           t_cur, tbox.p);
-        hipLaunchKernelGGL(task_setup_kernel, dim3(tg), dim3(kBlock), 0, stream, num_tasks, t_cur,
-                           ref_cur, tbox.p, dec.p, root_half_area, level);
-        hipLaunchKernelGGL(obj_bin_kernel, dim3(rg), dim3(kBlock), 0, stream, nrefs, ref_cur,
-                           t_cur, dec.p, obins.p);
-        const uint32_t gg = (num_tasks + kBlock / kBins - 1) / (kBlock / kBins);  // task groups
-        hipLaunchKernelGGL(obj_split_kernel, dim3(gg), dim3(kBlock), 0, stream, num_tasks, t_cur,
-                           dec.p, obins.p, ref_cur, root_half_area, cand.p);
-        if ((e = scan_exclusive_u32(cand.p, num_tasks, host + 3, scratch.p, scratch.cap * 4,
-                                    stream)) != hipSuccess)
-            return fail(e, "spatial candidate scan");
-        const uint32_t ccl = ncand * 3 * kBins;
-        hipLaunchKernelGGL(cand_clear_kernel, dim3((ccl + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                           stream, ccl, sbins.p);
-        hipLaunchKernelGGL(sp_bin_kernel, dim3(rg), dim3(kBlock), 0, stream, nrefs, ref_cur, dec.p,
-                           cand.p, verts, n, sbins.p);
-        hipLaunchKernelGGL(sp_split_kernel, dim3(tg), dim3(kBlock), 0, stream, num_tasks, t_cur,
-                           dec.p, cand.p, sbins.p, dup.p);
-        if ((e = scan_exclusive_u32(dup.p, num_tasks, host + 4, scratch.p, scratch.cap * 4,
-                                    stream)) != hipSuccess)
-            return fail(e, "spatial budget scan");
-        hipLaunchKernelGGL(decide_kernel, dim3(tg), dim3(kBlock), 0, stream, num_tasks, t_cur,
-                           dec.p, dup.p, budget - dups, leaf, obins.p, ref_cur, split_flag.p,
-                           out_cnt.p, leaf_cnt.p);
-        if ((e = scan_exclusive_u32(split_flag.p, num_tasks, host + 0, scratch.p,
-                                    scratch.cap * 4, stream)) != hipSuccess ||
-            (e = scan_exclusive_u32(out_cnt.p, num_tasks, host + 1, scratch.p, scratch.cap * 4,
-                                    stream)) != hipSuccess ||
-            (e = scan_exclusive_u32(leaf_cnt.p, num_tasks, host + 2, scratch.p, scratch.cap * 4,
-                                    stream)) != hipSuccess)
-            return fail(e, "spatial task scans");
-        hipLaunchKernelGGL(children_kernel, dim3(tg), dim3(kBlock), 0, stream, num_tasks, t_cur,
-                           dec.p, split_flag.p, out_cnt.p, leaf_cnt.p, counter, leaf_base,
-                           node_cap, res.nodes, t_nxt, fail_word);
-        hipLaunchKernelGGL(side_kernel, dim3(rg), dim3(kBlock), 0, stream, nrefs, ref_cur, t_cur,
-                           dec.p, fl.p, fr.p, fle.p, fre.p);
-        if ((e = scan_exclusive_u32(fle.p, nrefs, nullptr, scratch.p, scratch.cap * 4,
-                                    stream)) != hipSuccess ||
-            (e = scan_exclusive_u32(fre.p, nrefs, nullptr, scratch.p, scratch.cap * 4,
-                                    stream)) != hipSuccess)
-            return fail(e, "spatial partition scans");
-        hipLaunchKernelGGL(scatter_kernel, dim3(rg), dim3(kBlock), 0, stream, nrefs, ref_cur,
-                           t_cur, dec.p, split_flag.p, out_cnt.p, leaf_cnt.p, leaf_base, fl.p,
-                           fr.p, fle.p, fre.p, verts, n, cap, ref_nxt, res.indices, fail_word);
         if ((e = hipGetLastError()) != hipSuccess) return fail(e, "spatial level");
-        if ((e = hipMemcpyAsync(&host[5], fail_word, 4, hipMemcpyDeviceToHost, stream)) !=
-                hipSuccess ||
+        level = end;
+        if ((e = hipMemcpyAsync(host_level, levels + level, sizeof(SLevel), hipMemcpyDeviceToHost,
+                                stream)) != hipSuccess ||
             (e = hipStreamSynchronize(stream)) != hipSuccess)
-            return fail(e, "spatial level");
-        if (host[5]) {
+            return fail(e, "spatial levels");
+        if (host_level->fail & 1u) {
             set_error("spatial builder ran out of reference or node space");
             return fail_msg();
         }
-        const uint32_t splits = host[0], next_refs = host[1], leaf_refs = host[2];
-        if (next_refs + leaf_refs < nrefs || next_refs > cap || leaf_base + leaf_refs > cap) {
-            set_error("spatial builder lost references (%u + %u of %u)", next_refs, leaf_refs,
-                      nrefs);
+        if (host_level->fail) {
+            set_error("spatial builder lost references");
             return fail_msg();
         }
-        dups += next_refs + leaf_refs - nrefs;
-        leaf_base += leaf_refs;
-        counter += 2 * splits;
-        num_tasks = 2 * splits;
-        nrefs = next_refs;
-        std::swap(ref_cur, ref_nxt);
-        t_cur = t_nxt;
-        flip ^= 1;
+        if (host_level->num_tasks == 0) break;
+        if (level >= kMaxDepth + 2) {  // a task at depth kMaxDepth is a leaf
+            set_error("spatial builder did not finish in %u levels", kMaxDepth + 1);
+            return fail_msg();
+        }
+        known_tasks = host_level->num_tasks;
+        known_refs = host_level->nrefs;
+        known_dups = host_level->dups;
+        batch = 8;
     }
-    res.node_count = counter;
-    res.prim_count = leaf_base;  // index entries: the primitives and their duplicates
+    res.node_count = host_level->counter;
+    res.prim_count = host_level->leaf_base;  // index entries: primitives and duplicates
     res.source_prims = n;
     res.root_count = 1;
     *out = res;

@@ -167,6 +167,19 @@ hipError_t trace_tree_enqueue(const TraceTreePlan* t, const DeviceBvh& tree, con
 uint32_t trace_tree_levels(const TraceTreePlan* t);
 void trace_tree_plan_destroy(TraceTreePlan* t);
 
+// Exclusive scans whose length is read on the device (scan.hip): up to
+// four arrays at once, totals into device memory.
+struct ScanArrays {
+    uint32_t* p[4];
+};
+struct ScanTotals {
+    uint32_t* p[4];
+};
+size_t scan_dev_scratch_bytes(uint32_t max_count, int k);
+hipError_t scan_exclusive_dev(const ScanArrays& a, int k, const uint32_t* count,
+                              uint32_t max_count, const ScanTotals& totals, void* scratch,
+                              size_t scratch_bytes, hipStream_t stream);
+
 // Build workspaces.  Eager builds take their scratch from a per-device
 // cache that keeps one block per builder kind between builds (grown to the
 // largest request up to kWorkspaceCacheMax bytes, held until the process
@@ -175,7 +188,13 @@ void trace_tree_plan_destroy(TraceTreePlan* t);
 // request, is allocated and freed per build as before.  The work that used a
 // block must be complete when it is released (workspace_release waits for
 // the given stream, or for the device when stream is null).
-enum WorkspaceKind : int { kWsPloc = 0, kWsBinned = 1, kWsTree = 2, kWsKinds = 3 };
+enum WorkspaceKind : int {
+    kWsPloc = 0,
+    kWsBinned = 1,
+    kWsTree = 2,
+    kWsSpatial = 3,
+    kWsKinds = 4
+};
 constexpr size_t kWorkspaceCacheMax = size_t(1) << 30;
 struct Workspace {
     void* ptr = nullptr;

@@ -83,6 +83,95 @@ __global__ void __launch_bounds__(1024) sums_scan_kernel(uint32_t* data, uint32_
     if (threadIdx.x == 1023 && total) *total = sh[1023];
 }
 
+// Device-count variants (scan_exclusive_dev): K arrays scanned together,
+// their length read on the device; blocks past it return at once.
+template <int K>
+__global__ void __launch_bounds__(kThreads)
+    tile_sums_dev_kernel(ScanArrays a, const uint32_t* count_p, uint32_t* sums, uint32_t stride) {
+    __shared__ uint32_t sh[kThreads];
+    const uint32_t count = *count_p;
+    if (blockIdx.x * kTile >= count) return;  // whole block
+    const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPer;
+    for (int k = 0; k < K; ++k) {
+        uint32_t s = 0;
+        for (int q = 0; q < kPer; ++q)
+            if (base + q < count) s += a.p[k][base + q];
+        uint32_t tot = 0;
+        block_exclusive(s, sh, &tot);
+        if (threadIdx.x == 0) sums[k * stride + blockIdx.x] = tot;
+    }
+}
+
+template <int K>
+__global__ void __launch_bounds__(1024)
+    sums_scan_dev_kernel(uint32_t* sums, uint32_t stride, const uint32_t* count_p,
+                         ScanTotals totals) {
+    __shared__ uint32_t sh[1024];
+    const uint32_t count = *count_p;
+    const uint32_t tiles = (count + kTile - 1) / kTile;
+    const uint32_t per = (tiles + 1023) / 1024;
+    const uint32_t b = threadIdx.x * per;
+    const uint32_t e = b + per < tiles ? b + per : tiles;
+    for (int k = 0; k < K; ++k) {
+        uint32_t* data = sums + k * stride;
+        uint32_t s = 0;
+        for (uint32_t i = b; i < e; ++i) s += data[i];
+        sh[threadIdx.x] = s;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const uint32_t t = threadIdx.x >= uint32_t(off) ? sh[threadIdx.x - off] : 0u;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        uint32_t run = threadIdx.x ? sh[threadIdx.x - 1] : 0u;
+        for (uint32_t i = b; i < e; ++i) {
+            const uint32_t v = data[i];
+            data[i] = run;
+            run += v;
+        }
+        if (threadIdx.x == 1023 && totals.p[k]) *totals.p[k] = sh[1023];
+        __syncthreads();  // sh is reused by the next array
+    }
+}
+
+template <int K>
+__global__ void __launch_bounds__(kThreads)
+    tile_scan_dev_kernel(ScanArrays a, const uint32_t* count_p, const uint32_t* sums,
+                         uint32_t stride) {
+    __shared__ uint32_t sh[kThreads];
+    const uint32_t count = *count_p;
+    if (blockIdx.x * kTile >= count) return;  // whole block
+    const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPer;
+    for (int k = 0; k < K; ++k) {
+        uint32_t v[kPer];
+        uint32_t s = 0;
+        for (int q = 0; q < kPer; ++q) {
+            v[q] = base + q < count ? a.p[k][base + q] : 0u;
+            s += v[q];
+        }
+        uint32_t run = block_exclusive(s, sh, nullptr) + sums[k * stride + blockIdx.x];
+        for (int q = 0; q < kPer; ++q) {
+            if (base + q < count) a.p[k][base + q] = run;
+            run += v[q];
+        }
+    }
+}
+
+template <int K>
+hipError_t launch_scan_dev(const ScanArrays& a, const uint32_t* count, uint32_t max_count,
+                           const ScanTotals& totals, uint32_t* sums, hipStream_t stream) {
+    const uint32_t tiles = (max_count + kTile - 1) / kTile;
+    const uint32_t stride = tiles + 1;
+    hipLaunchKernelGGL(tile_sums_dev_kernel<K>, dim3(tiles), dim3(kThreads), 0, stream, a, count,
+                       sums, stride);
+    hipLaunchKernelGGL(sums_scan_dev_kernel<K>, dim3(1), dim3(1024), 0, stream, sums, stride,
+                       count, totals);
+    hipLaunchKernelGGL(tile_scan_dev_kernel<K>, dim3(tiles), dim3(kThreads), 0, stream, a, count,
+                       sums, stride);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 size_t scan_scratch_bytes(uint32_t count) {
@@ -102,6 +191,31 @@ hipError_t scan_exclusive_u32(uint32_t* data, uint32_t count, uint32_t* total, v
     hipLaunchKernelGGL(sums_scan_kernel, dim3(1), dim3(1024), 0, stream, sums, tiles, total);
     hipLaunchKernelGGL(tile_scan_kernel, dim3(tiles), dim3(kThreads), 0, stream, data, count, sums);
     return hipGetLastError();
+}
+
+size_t scan_dev_scratch_bytes(uint32_t max_count, int k) {
+    const size_t tiles = (size_t(max_count) + kTile - 1) / kTile;
+    return size_t(k) * (tiles + 1) * sizeof(uint32_t);
+}
+
+// In-place exclusive scans of the first k (1..4) arrays of a, each of
+// *count values (read on the device, at most max_count); *totals.p[j]
+// (device memory, or null) receives array j's sum.  scratch holds
+// scan_dev_scratch_bytes(max_count, k).  Nothing waits for the host, so a
+// sequence of them can be enqueued (or recorded) with counts still to come.
+hipError_t scan_exclusive_dev(const ScanArrays& a, int k, const uint32_t* count,
+                              uint32_t max_count, const ScanTotals& totals, void* scratch,
+                              size_t scratch_bytes, hipStream_t stream) {
+    if (max_count == 0) return hipSuccess;
+    if (k < 1 || k > 4 || scratch_bytes < scan_dev_scratch_bytes(max_count, k))
+        return hipErrorInvalidValue;
+    uint32_t* sums = static_cast<uint32_t*>(scratch);
+    switch (k) {
+        case 1: return launch_scan_dev<1>(a, count, max_count, totals, sums, stream);
+        case 2: return launch_scan_dev<2>(a, count, max_count, totals, sums, stream);
+        case 3: return launch_scan_dev<3>(a, count, max_count, totals, sums, stream);
+        default: return launch_scan_dev<4>(a, count, max_count, totals, sums, stream);
+    }
 }
 
 }  // namespace rtbvh
