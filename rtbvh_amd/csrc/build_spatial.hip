@@ -156,31 +156,42 @@ __device__ __forceinline__ bool box_valid(const Box& b) {
     return b.mn[0] <= b.mx[0] && b.mn[1] <= b.mx[1] && b.mn[2] <= b.mx[2];
 }
 
+// v[ax] by selects: a runtime index into a register array would place the
+// array in scratch memory.
+__device__ __forceinline__ float sel3(const float* v, int ax) {
+    return ax == 0 ? v[0] : (ax == 1 ? v[1] : v[2]);
+}
+
 // Bounding box of the part of triangle v (9 floats) with lo <= x[ax] <= hi,
 // widened by rounding slack and kept inside [rmn, rmx]: the vertices inside
 // the slab and the edges' crossings of its two planes bound the clipped
 // polygon.  False when nothing is left.
-__device__ bool clip_box(const float* v, int ax, float lo, float hi, const float* rmn,
+__device__ __forceinline__ bool clip_box(const float* v, int ax, float lo, float hi, const float* rmn,
                          const float* rmx, float* omn, float* omx) {
     for (int k = 0; k < 3; ++k) {
         omn[k] = 3.0e38f;
         omx[k] = -3.0e38f;
     }
-    float p[3][3];
-    for (int i = 0; i < 3; ++i)
-        for (int k = 0; k < 3; ++k) p[i][k] = v[i * 3 + k];
+    float p[3][3], pa[3];
     for (int i = 0; i < 3; ++i) {
-        if (p[i][ax] >= lo && p[i][ax] <= hi)
+        for (int k = 0; k < 3; ++k) p[i][k] = v[i * 3 + k];
+        pa[i] = sel3(p[i], ax);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (pa[i] >= lo && pa[i] <= hi)
             for (int k = 0; k < 3; ++k) {
                 omn[k] = fminf(omn[k], p[i][k]);
                 omx[k] = fmaxf(omx[k], p[i][k]);
             }
         const float* a = p[i];
         const float* b = p[(i + 1) % 3];
+        const float aa = pa[i], ba = pa[(i + 1) % 3];
+#pragma unroll
         for (int s = 0; s < 2; ++s) {
             const float plane = s ? hi : lo;
-            if (!((a[ax] < plane && b[ax] > plane) || (a[ax] > plane && b[ax] < plane))) continue;
-            const float t = (plane - a[ax]) / (b[ax] - a[ax]);
+            if (!((aa < plane && ba > plane) || (aa > plane && ba < plane))) continue;
+            const float t = (plane - aa) / (ba - aa);
             for (int k = 0; k < 3; ++k) {
                 const float q = k == ax ? plane : a[k] + t * (b[k] - a[k]);
                 omn[k] = fminf(omn[k], q);
@@ -249,11 +260,24 @@ __global__ void task_clear_kernel(const SLevel* L, const STask* tasks, Acc* tbox
     }
 }
 
-// The spatial bins of the level's candidates (at most kMaxCand) cleared.
-__global__ void cand_clear_kernel(const SLevel* L, Acc* sbins) {
-    const uint32_t count = min(L->cand_total, kMaxCand) * 3 * kBins;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count;
-         i += gridDim.x * blockDim.x) {
+// The spatial bins of the level's evaluated candidates (the first
+// kMaxCand) cleared, the list of their tasks (list[cand_excl[t]] = t), and
+// dup[t] = 0 for every other task.
+__global__ void cand_clear_kernel(const SLevel* L, const uint32_t* cand_excl, uint32_t* list,
+                                  uint32_t* dup, Acc* sbins) {
+    const uint32_t num_tasks = L->num_tasks, total = L->cand_total;
+    if (num_tasks == 0) return;
+    const uint32_t step = gridDim.x * blockDim.x;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < num_tasks; t += step) {
+        const uint32_t e = cand_excl[t];
+        const uint32_t next = t + 1 < num_tasks ? cand_excl[t + 1] : total;
+        if (next > e && e < kMaxCand)
+            list[e] = t;
+        else
+            dup[t] = 0;
+    }
+    const uint32_t count = min(total, kMaxCand) * 3 * kBins;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += step) {
         Acc a;
         acc_clear(a);
         sbins[i] = a;
@@ -469,8 +493,17 @@ __device__ __forceinline__ GRun grun_shfl_down(const GRun& b, int off) {
     return r;
 }
 
-__device__ __forceinline__ GRun grun_pick(const GRun (&r)[3], uint32_t ax) {
-    return ax == 0 ? r[0] : (ax == 1 ? r[1] : r[2]);
+// The run of axis ax, member by member: three named values and selects (a
+// runtime index into an array of runs would place it in scratch memory).
+__device__ __forceinline__ GRun grun_pick(uint32_t ax, const GRun& r0, const GRun& r1,
+                                          const GRun& r2) {
+    GRun g;
+    for (int k = 0; k < 3; ++k) {
+        g.mn[k] = ax == 0 ? r0.mn[k] : (ax == 1 ? r1.mn[k] : r2.mn[k]);
+        g.mx[k] = ax == 0 ? r0.mx[k] : (ax == 1 ? r1.mx[k] : r2.mx[k]);
+    }
+    g.cnt = ax == 0 ? r0.cnt : (ax == 1 ? r1.cnt : r2.cnt);
+    return g;
 }
 
 // Best object split and whether a spatial split is worth evaluating.  One
@@ -533,9 +566,10 @@ __global__ void __launch_bounds__(kBlock)
             const Acc* tb = obins + size_t(tk.slot) * 3 * kBins;
             for (int ax = 0; ax < 3; ++ax) own[ax] = tb[ax * kBins + lane];
         }
-        GRun pre[3], suf[3];
+        GRun pre0, pre1, pre2, suf0, suf1, suf2;
         float key = __builtin_inff();
         int idx = 3 * kBins;
+#pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
             GRun p;
             for (int k = 0; k < 3; ++k) {
@@ -553,8 +587,16 @@ __global__ void __launch_bounds__(kBlock)
                 }
                 if (lane + off < kBins) grun_join(q, higher);
             }
-            pre[ax] = p;
-            suf[ax] = q;
+            if (ax == 0) {
+                pre0 = p;
+                suf0 = q;
+            } else if (ax == 1) {
+                pre1 = p;
+                suf1 = q;
+            } else {
+                pre2 = p;
+                suf2 = q;
+            }
             const GRun right = grun_shfl_down(q, 1);
             if (lane < kBins - 1 && p.cnt != 0 && p.cnt != work) {
                 const float c = half_area6(p.mn, p.mx) * float(p.cnt) +
@@ -576,8 +618,8 @@ __global__ void __launch_bounds__(kBlock)
         if (idx < 3 * kBins) {
             const uint32_t ax = uint32_t(idx / kBins);
             const int i = idx % kBins;
-            const GRun lbest = grun_shfl(grun_pick(pre, ax), i);
-            const GRun rbest = grun_shfl(grun_pick(suf, ax), i + 1);
+            const GRun lbest = grun_shfl(grun_pick(ax, pre0, pre1, pre2), i);
+            const GRun rbest = grun_shfl(grun_pick(ax, suf0, suf1, suf2), i + 1);
             d.cost = key;
             d.kind = kObject;
             d.axis = ax;
@@ -637,8 +679,9 @@ __global__ void __launch_bounds__(kBlock)
                                             : sbins + size_t(cand_excl[t]) * 3 * kBins;
             const float* v = verts + size_t(r.prim) * 9;
             for (int ax = 0; ax < 3; ++ax) {
-                const uint32_t b0 = bin_of(r.mn[ax], d.org[ax], d.inv[ax]);
-                const uint32_t b1 = max(b0, bin_of(r.mx[ax], d.org[ax], d.inv[ax]));
+                const float org = sel3(d.org, ax), inv = sel3(d.inv, ax);
+                const uint32_t b0 = bin_of(sel3(r.mn, ax), org, inv);
+                const uint32_t b1 = max(b0, bin_of(sel3(r.mx, ax), org, inv));
                 atomicAdd(&base[ax * kBins + b0].c0, 1u);
                 atomicAdd(&base[ax * kBins + b1].c1, 1u);
                 for (uint32_t b = b0; b <= b1 && b < uint32_t(kBins); ++b) {
@@ -684,54 +727,87 @@ __global__ void __launch_bounds__(kBlock)
         }
 }
 
-// Best spatial split of each evaluated candidate; dup[t] = its duplicates
-// when it beats the object split (0 otherwise).
-__global__ void sp_split_kernel(const SLevel* L, const STask* tasks, SDec* dec,
-                                const uint32_t* cand_excl, const Acc* sbins, uint32_t* dup) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= L->num_tasks) return;
+// Best spatial split of each evaluated candidate (list: their tasks);
+// dup[t] = its duplicates when it beats the object split, else 0.  One group
+// of 16 lanes per candidate, lane i holding spatial bin i of each axis: the
+// boxes and entry counts of bins 0..i and the boxes and exit counts of bins
+// i+1..15 by shuffle scans, the cost of the plane after bin i as a sweep
+// computes it (left half area * entries + right half area * exits, planes
+// with an empty side or fewer references than the task skipped), and the
+// lowest cost below the task's, the first in (axis, plane) order.  Bin
+// bounds hold no NaN, so the unions equal a sweep's up to the sign of a
+// zero, which no comparison tells apart.
+__global__ void __launch_bounds__(kBlock)
+    sp_split_kernel(const SLevel* L, const uint32_t* list, const STask* tasks, SDec* dec,
+                    const Acc* sbins, uint32_t* dup) {
+    const uint32_t c = (blockIdx.x * kBlock + threadIdx.x) / kBins;
+    const int lane = int(threadIdx.x % kBins);
+    if (c >= min(L->cand_total, kMaxCand)) return;  // whole groups
+    const uint32_t t = list[c];
+    const STask tk = tasks[t];
     SDec d = dec[t];
-    uint32_t want = 0;
-    if (d.cand && cand_excl[t] < kMaxCand) {
-        const STask tk = tasks[t];
-        const uint32_t work = tk.end - tk.begin;
-        const Acc* tb = sbins + size_t(cand_excl[t]) * 3 * kBins;
-        for (int ax = 0; ax < 3; ++ax) {
-            const Acc* b = tb + ax * kBins;
-            float rcost[kBins];
-            uint32_t rcnt[kBins];
-            Box cur;
-            box_empty(cur);
-            uint32_t exits = 0;
-            for (int i = kBins - 1; i > 0; --i) {
-                box_grow_acc(cur, b[i]);
-                exits += b[i].c1;
-                rcnt[i] = exits;
-                rcost[i] = half_area6(cur.mn, cur.mx) * float(exits);
-            }
-            box_empty(cur);
-            uint32_t entries = 0;
-            for (int i = 0; i < kBins - 1; ++i) {
-                box_grow_acc(cur, b[i]);
-                entries += b[i].c0;
-                const uint32_t l = entries, r = rcnt[i + 1];
-                if (l == 0 || r == 0 || l + r < work) continue;
-                const float cost = half_area6(cur.mn, cur.mx) * float(l) + rcost[i + 1];
-                if (cost < d.s_cost) {
-                    d.s_cost = cost;
-                    d.s_axis = uint32_t(ax);
-                    d.s_k = uint32_t(i + 1);
-                    d.s_left = l;
-                    d.s_right = r;
-                }
-            }
+    const uint32_t work = tk.end - tk.begin;
+    const Acc* tb = sbins + size_t(c) * 3 * kBins;
+    float key = d.s_cost;
+    int idx = 3 * kBins;
+    uint32_t best_l = 0, best_r = 0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const Acc a = tb[ax * kBins + lane];
+        GRun p, q;
+        for (int k = 0; k < 3; ++k) {
+            p.mn[k] = q.mn[k] = fdec(a.mn[k]);
+            p.mx[k] = q.mx[k] = fdec(a.mx[k]);
         }
-        if (d.s_cost < d.cost) {
-            d.dup = d.s_left + d.s_right - work;
-            want = 1;
+        p.cnt = a.c0;  // entries, bins 0..lane
+        q.cnt = a.c1;  // exits, bins lane..15
+        for (int off = 1; off < kBins; off <<= 1) {
+            GRun lower = grun_shfl_up(p, off);
+            const GRun higher = grun_shfl_down(q, off);
+            if (lane >= off) {
+                grun_join(lower, p);
+                p = lower;
+            }
+            if (lane + off < kBins) grun_join(q, higher);
+        }
+        const GRun right = grun_shfl_down(q, 1);
+        const uint32_t l = p.cnt, r = right.cnt;
+        if (lane < kBins - 1 && l != 0 && r != 0 && l + r >= work) {
+            const float cost =
+                half_area6(p.mn, p.mx) * float(l) + half_area6(right.mn, right.mx) * float(r);
+            if (cost < key) {  // the first axis wins a tie
+                key = cost;
+                idx = ax * kBins + lane;
+                best_l = l;
+                best_r = r;
+            }
         }
     }
-    dec[t] = d;
+    for (int off = kBins / 2; off > 0; off >>= 1) {
+        const float ok = __shfl_xor(key, off, kBins);
+        const int oi = __shfl_xor(idx, off, kBins);
+        if (ok < key || (ok == key && oi < idx)) {
+            key = ok;
+            idx = oi;
+        }
+    }
+    // the winner's counts, from the lane that holds them
+    const int src = idx < 3 * kBins ? idx % kBins : 0;
+    const uint32_t wl = __shfl(best_l, src, kBins), wr = __shfl(best_r, src, kBins);
+    if (lane != 0) return;
+    uint32_t want = 0;
+    if (idx < 3 * kBins) {
+        d.s_cost = key;
+        d.s_axis = uint32_t(idx / kBins);
+        d.s_k = uint32_t(idx % kBins + 1);
+        d.s_left = wl;
+        d.s_right = wr;
+        if (d.s_cost < d.cost) {
+            d.dup = wl + wr - work;
+            want = 1;
+        }
+        dec[t] = d;
+    }
     dup[t] = want ? d.dup : 0u;
 }
 
@@ -767,9 +843,11 @@ __global__ void decide_kernel(const SLevel* L, const STask* tasks, SDec* dec,
             if (work > max_leaf) {
                 // approximate median on the longest axis; an index median
                 // when the centroids share one bin
-                uint32_t ax = 0;
-                if (d.mx[1] - d.mn[1] > d.mx[0] - d.mn[0]) ax = 1;
-                if (d.mx[2] - d.mn[2] > d.mx[ax] - d.mn[ax]) ax = 2;
+                const float e0 = d.mx[0] - d.mn[0], e1 = d.mx[1] - d.mn[1],
+                            e2 = d.mx[2] - d.mn[2];
+                uint32_t ax = e1 > e0 ? 1u : 0u;
+                if (e2 > (ax ? e1 : e0)) ax = 2;
+                const float org = sel3(d.org, int(ax)), inv = sel3(d.inv, int(ax));
                 const Acc* b = work > kSmallTask
                                    ? obins + size_t(tk.slot) * 3 * kBins + ax * kBins
                                    : nullptr;
@@ -782,8 +860,7 @@ __global__ void decide_kernel(const SLevel* L, const STask* tasks, SDec* dec,
                         // references' centroid bins (obj_bin_kernel's count)
                         for (uint32_t q = tk.begin; q < tk.end; ++q) {
                             const SRef& r = refs[q];
-                            c += bin_of(0.5f * (r.mn[ax] + r.mx[ax]), d.org[ax], d.inv[ax]) ==
-                                         uint32_t(i)
+                            c += bin_of(0.5f * (r.mn[ax] + r.mx[ax]), org, inv) == uint32_t(i)
                                      ? 1u
                                      : 0u;
                         }
@@ -930,20 +1007,16 @@ __global__ void scatter_kernel(const SLevel* L, const SRef* refs, const STask* t
         }
         // keep coverage when clipping leaves nothing on a side: the
         // reference's own box cut at the plane
-        if (!ok_l) {
+        if (!ok_l)
             for (int k = 0; k < 3; ++k) {
                 a.mn[k] = r.mn[k];
-                a.mx[k] = r.mx[k];
+                a.mx[k] = k == ax ? fmaxf(fminf(r.mx[k], plane), r.mn[k]) : r.mx[k];
             }
-            a.mx[ax] = fmaxf(fminf(r.mx[ax], plane), r.mn[ax]);
-        }
-        if (!ok_r) {
+        if (!ok_r)
             for (int k = 0; k < 3; ++k) {
-                b.mn[k] = r.mn[k];
+                b.mn[k] = k == ax ? fminf(fmaxf(r.mn[k], plane), r.mx[k]) : r.mn[k];
                 b.mx[k] = r.mx[k];
             }
-            b.mn[ax] = fminf(fmaxf(r.mn[ax], plane), r.mx[ax]);
-        }
     }
     if (l) {
         const uint32_t pos = out_excl[r.task] + (fl_excl[i] - fl_excl[tk.begin]);
@@ -1060,11 +1133,12 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
     const size_t sz_obins = align256(slot_cap * 3 * kBins * sizeof(Acc));
     const size_t sz_sbins = align256(size_t(std::min(cap, kMaxCand)) * 3 * kBins * sizeof(Acc));
     const size_t sz_u32 = align256(size_t(cap) * 4);
+    const size_t sz_list = align256(size_t(std::min(cap, kMaxCand)) * 4);
     const size_t sz_scan = align256(scan_dev_scratch_bytes(cap, 4));
     const size_t sz_levels = align256(size_t(kLevels) * sizeof(SLevel));
     const size_t sz_words = 256;
     const size_t total = 2 * sz_refs + 2 * sz_tasks + sz_dec + sz_tbox + sz_obins + sz_sbins +
-                         10 * sz_u32 + sz_scan + sz_levels + sz_words;
+                         10 * sz_u32 + sz_list + sz_scan + sz_levels + sz_words;
     Workspace wsh;
     if ((e = workspace_acquire(kWsSpatial, total, &wsh)) != hipSuccess)
         return fail(e, "spatial builder workspace");
@@ -1097,6 +1171,7 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
     Acc* sbins = reinterpret_cast<Acc*>(carve(sz_sbins));
     uint32_t* u32[10];
     for (int k = 0; k < 10; ++k) u32[k] = reinterpret_cast<uint32_t*>(carve(sz_u32));
+    uint32_t* cand_list = reinterpret_cast<uint32_t*>(carve(sz_list));
     uint32_t *split_flag = u32[0], *out_cnt = u32[1], *leaf_cnt = u32[2], *big_cnt = u32[3];
     uint32_t *cand = u32[4], *dup = u32[5], *fl = u32[6], *fr = u32[7], *fle = u32[8],
              *fre = u32[9];
@@ -1139,7 +1214,9 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
             const uint32_t tg = (tb + kBlock - 1) / kBlock;
             const uint32_t gg = (tb + kBlock / kBins - 1) / (kBlock / kBins);  // task groups
             const uint32_t ccl = std::min(tb, kMaxCand) * 3 * kBins;
-            const uint32_t cg = std::min<uint32_t>((ccl + kBlock - 1) / kBlock, 4096);
+            const uint32_t cg = std::min<uint32_t>(
+                std::max<uint32_t>((ccl + kBlock - 1) / kBlock, (tb + kBlock - 1) / kBlock), 4096);
+            const uint32_t sg = (std::min(tb, kMaxCand) * kBins + kBlock - 1) / kBlock;
             const SLevel* L = levels + k;
             SLevel* Lw = levels + k;
             const uint32_t a = k & 1u, b = a ^ 1u;
@@ -1159,11 +1236,12 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
                                         ScanTotals{{&Lw->cand_total}}, scan_scratch, sz_scan,
                                         stream)) != hipSuccess)
                 return fail(e, "spatial candidate scan");
-            hipLaunchKernelGGL(cand_clear_kernel, dim3(cg), dim3(kBlock), 0, stream, L, sbins);
+            hipLaunchKernelGGL(cand_clear_kernel, dim3(cg), dim3(kBlock), 0, stream, L, cand,
+                               cand_list, dup, sbins);
             hipLaunchKernelGGL(sp_bin_kernel, dim3(rg), dim3(kBlock), 0, stream, L, rc, dec, cand,
                                verts, n, sbins);
-            hipLaunchKernelGGL(sp_split_kernel, dim3(tg), dim3(kBlock), 0, stream, L, tc, dec, cand,
-                               sbins, dup);
+            hipLaunchKernelGGL(sp_split_kernel, dim3(sg), dim3(kBlock), 0, stream, L, cand_list,
+                               tc, dec, sbins, dup);
             if ((e = scan_exclusive_dev(ScanArrays{{dup}}, 1, &L->num_tasks, tb,
                                         ScanTotals{{&Lw->dup_total}}, scan_scratch, sz_scan,
                                         stream)) != hipSuccess)
