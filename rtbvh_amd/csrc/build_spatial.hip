@@ -166,8 +166,9 @@ __device__ __forceinline__ float sel3(const float* v, int ax) {
 // widened by rounding slack and kept inside [rmn, rmx]: the vertices inside
 // the slab and the edges' crossings of its two planes bound the clipped
 // polygon.  False when nothing is left.
-__device__ __forceinline__ bool clip_box(const float* v, int ax, float lo, float hi, const float* rmn,
-                         const float* rmx, float* omn, float* omx) {
+__device__ __forceinline__ bool clip_box(const float* v, int ax, float lo, float hi,
+                                         const float* rmn, const float* rmx, float* omn,
+                                         float* omx) {
     for (int k = 0; k < 3; ++k) {
         omn[k] = 3.0e38f;
         omx[k] = -3.0e38f;
@@ -286,17 +287,21 @@ __global__ void cand_clear_kernel(const SLevel* L, const uint32_t* cand_excl, ui
 
 // A level's references are ordered by task (the scatter writes each split
 // task's children, left then right, at the exclusive scan of the counts), so
-// a block's references belong to the tasks refs[first].task ..
-// refs[last].task.  When those are at most kLdsTasks, the block reduces in
-// LDS and merges one value per task and slot; returns the first task, or -1
-// for the global-atomic path.  A reference whose task lies outside the run
-// (never, by the ordering) takes the global path as well (in_run).
+// the run of references [first, end) of a block belongs to the tasks
+// refs[first].task .. refs[end - 1].task.  When those are at most kLdsTasks,
+// the block reduces in LDS and merges one value per task and slot; returns
+// the first task, or -1 for the global-atomic path.  A reference whose task
+// lies outside the run (never, by the ordering) takes the global path as
+// well (in_run).  A block's run is `run` references (a multiple of kBlock):
+// longer runs near the root, where a few tasks hold every reference, divide
+// the blocks that merge into each task's bins.
+constexpr uint32_t kSplitGrid = 8192;  // obj_split_kernel blocks at most (grid stride)
+constexpr uint32_t kRunLong = 8 * kBlock, kRunMid = 4 * kBlock, kRunMidTasks = 256;
 constexpr int kLdsTasks = 16;  // 4: 41 ms spatial, 22 ms binned at 1M; 16: 36, 19.5; 32: no better
 
-__device__ __forceinline__ int32_t block_tasks(const SRef* refs, uint32_t nrefs, int32_t* count) {
-    const uint32_t first = blockIdx.x * kBlock;
-    const uint32_t last = (first + kBlock < nrefs ? first + kBlock : nrefs) - 1;
-    const int32_t a = refs[first].task, b = refs[last].task;
+__device__ __forceinline__ int32_t block_tasks(const SRef* refs, uint32_t first, uint32_t end,
+                                               int32_t* count) {
+    const int32_t a = refs[first].task, b = refs[end - 1].task;
     *count = b - a + 1;
     return (a >= 0 && b >= a && b - a < kLdsTasks) ? a : -1;
 }
@@ -308,15 +313,17 @@ __device__ __forceinline__ bool in_run(int32_t t, int32_t t0, int32_t nt) {
 // Node boxes: the union of each task's reference boxes (tasks of at most
 // kSmallTask references are left to task_setup_kernel).
 __global__ void __launch_bounds__(kBlock)
-    ref_bounds_kernel(const SLevel* L, const SRef* refs, const STask* tasks, Acc* tbox) {
+    ref_bounds_kernel(const SLevel* L, const SRef* refs, const STask* tasks, Acc* tbox,
+                      uint32_t run) {
     __shared__ uint32_t s_mn[kLdsTasks][3], s_mx[kLdsTasks][3];
     __shared__ int32_t s_t0, s_nt;
     const uint32_t nrefs = L->nrefs;
-    if (blockIdx.x * kBlock >= nrefs) return;  // whole block
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t first = blockIdx.x * run;
+    if (first >= nrefs) return;  // whole block
+    const uint32_t end = first + run < nrefs ? first + run : nrefs;
     if (threadIdx.x == 0) {
         int32_t nt = 0;
-        s_t0 = block_tasks(refs, nrefs, &nt);
+        s_t0 = block_tasks(refs, first, end, &nt);
         s_nt = nt;
     }
     if (threadIdx.x < kLdsTasks * 3) {
@@ -325,7 +332,7 @@ __global__ void __launch_bounds__(kBlock)
     }
     __syncthreads();
     const int32_t t0 = s_t0;
-    if (i < nrefs) {
+    for (uint32_t i = first + threadIdx.x; i < end; i += kBlock) {
         const SRef r = refs[i];
         const bool lds = in_run(r.task, t0, s_nt);
         if (r.task >= 0 && tasks[r.task].end - tasks[r.task].begin > kSmallTask)
@@ -397,22 +404,23 @@ __global__ void task_setup_kernel(const SLevel* L, const STask* tasks, const SRe
 
 __global__ void __launch_bounds__(kBlock)
     obj_bin_kernel(const SLevel* L, const SRef* refs, const STask* tasks, const SDec* dec,
-                   Acc* obins) {
+                   Acc* obins, uint32_t run) {
     __shared__ Acc s_b[kLdsTasks][3 * kBins];
     __shared__ int32_t s_t0, s_nt;
     const uint32_t nrefs = L->nrefs;
-    if (blockIdx.x * kBlock >= nrefs) return;  // whole block
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t first = blockIdx.x * run;
+    if (first >= nrefs) return;  // whole block
+    const uint32_t end = first + run < nrefs ? first + run : nrefs;
     if (threadIdx.x == 0) {
         int32_t nt = 0;
-        s_t0 = block_tasks(refs, nrefs, &nt);
+        s_t0 = block_tasks(refs, first, end, &nt);
         s_nt = nt;
     }
     for (uint32_t k = threadIdx.x; k < uint32_t(kLdsTasks * 3 * kBins); k += kBlock)
         acc_clear(s_b[k / (3 * kBins)][k % (3 * kBins)]);
     __syncthreads();
     const int32_t t0 = s_t0;
-    if (i < nrefs) {
+    for (uint32_t i = first + threadIdx.x; i < end; i += kBlock) {
         const SRef r = refs[i];
         const int32_t t = r.task;
         if (t >= 0) {
@@ -520,126 +528,129 @@ __device__ __forceinline__ GRun grun_pick(uint32_t ax, const GRun& r0, const GRu
 __global__ void __launch_bounds__(kBlock)
     obj_split_kernel(const SLevel* L, const STask* tasks, SDec* dec, const Acc* obins,
                      const SRef* refs, const float* root_half_area, uint32_t* cand_flag) {
-    const uint32_t t = (blockIdx.x * kBlock + threadIdx.x) / kBins;
+    const uint32_t num_tasks = L->num_tasks;
     const int lane = int(threadIdx.x % kBins);
-    if (t >= L->num_tasks) return;  // whole groups: kBlock is a multiple of kBins
-    const STask tk = tasks[t];
-    SDec d = dec[t];
-    const uint32_t work = tk.end - tk.begin;
-    uint32_t cand = 0;
-    if (work > 1 && tk.depth < kMaxDepth) {
-        Acc own[3];
-        if (work <= kSmallTask) {
-            uint32_t pb[3] = {0, 0, 0}, pmn[3], pmx[3];
-            for (int k = 0; k < 3; ++k) {
-                pmn[k] = fenc(1e34f);
-                pmx[k] = fenc(-1e34f);
-            }
-            if (uint32_t(lane) < work) {
-                const SRef r = refs[tk.begin + lane];
+    // whole groups (kBlock is a multiple of kBins), grid-stride over tasks
+    for (uint32_t t = (blockIdx.x * kBlock + threadIdx.x) / kBins; t < num_tasks;
+         t += gridDim.x * (kBlock / kBins)) {
+        const STask tk = tasks[t];
+        SDec d = dec[t];
+        const uint32_t work = tk.end - tk.begin;
+        uint32_t cand = 0;
+        if (work > 1 && tk.depth < kMaxDepth) {
+            Acc own[3];
+            if (work <= kSmallTask) {
+                uint32_t pb[3] = {0, 0, 0}, pmn[3], pmx[3];
                 for (int k = 0; k < 3; ++k) {
-                    pb[k] = bin_of(0.5f * (r.mn[k] + r.mx[k]), d.org[k], d.inv[k]);
-                    if (r.mn[k] <= r.mx[k]) {
-                        pmn[k] = fenc(r.mn[k]);
-                        pmx[k] = fenc(r.mx[k]);
-                    }
+                    pmn[k] = fenc(1e34f);
+                    pmx[k] = fenc(-1e34f);
                 }
-            }
-            for (int ax = 0; ax < 3; ++ax) acc_clear(own[ax]);
-            for (uint32_t q = 0; q < work; ++q) {
-                uint32_t qb[3], qmn[3], qmx[3];
-                for (int k = 0; k < 3; ++k) {
-                    qb[k] = __shfl(pb[k], int(q), kBins);
-                    qmn[k] = __shfl(pmn[k], int(q), kBins);
-                    qmx[k] = __shfl(pmx[k], int(q), kBins);
-                }
-                for (int ax = 0; ax < 3; ++ax)
-                    if (qb[ax] == uint32_t(lane)) {
-                        ++own[ax].c0;
-                        for (int k = 0; k < 3; ++k) {
-                            own[ax].mn[k] = min(own[ax].mn[k], qmn[k]);
-                            own[ax].mx[k] = max(own[ax].mx[k], qmx[k]);
+                if (uint32_t(lane) < work) {
+                    const SRef r = refs[tk.begin + lane];
+                    for (int k = 0; k < 3; ++k) {
+                        pb[k] = bin_of(0.5f * (r.mn[k] + r.mx[k]), d.org[k], d.inv[k]);
+                        if (r.mn[k] <= r.mx[k]) {
+                            pmn[k] = fenc(r.mn[k]);
+                            pmx[k] = fenc(r.mx[k]);
                         }
                     }
-            }
-        } else {
-            const Acc* tb = obins + size_t(tk.slot) * 3 * kBins;
-            for (int ax = 0; ax < 3; ++ax) own[ax] = tb[ax * kBins + lane];
-        }
-        GRun pre0, pre1, pre2, suf0, suf1, suf2;
-        float key = __builtin_inff();
-        int idx = 3 * kBins;
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            GRun p;
-            for (int k = 0; k < 3; ++k) {
-                p.mn[k] = fdec(own[ax].mn[k]);
-                p.mx[k] = fdec(own[ax].mx[k]);
-            }
-            p.cnt = own[ax].c0;
-            GRun q = p;
-            for (int off = 1; off < kBins; off <<= 1) {
-                GRun lower = grun_shfl_up(p, off);
-                const GRun higher = grun_shfl_down(q, off);
-                if (lane >= off) {
-                    grun_join(lower, p);
-                    p = lower;
                 }
-                if (lane + off < kBins) grun_join(q, higher);
-            }
-            if (ax == 0) {
-                pre0 = p;
-                suf0 = q;
-            } else if (ax == 1) {
-                pre1 = p;
-                suf1 = q;
+                for (int ax = 0; ax < 3; ++ax) acc_clear(own[ax]);
+                for (uint32_t q = 0; q < work; ++q) {
+                    uint32_t qb[3], qmn[3], qmx[3];
+                    for (int k = 0; k < 3; ++k) {
+                        qb[k] = __shfl(pb[k], int(q), kBins);
+                        qmn[k] = __shfl(pmn[k], int(q), kBins);
+                        qmx[k] = __shfl(pmx[k], int(q), kBins);
+                    }
+                    for (int ax = 0; ax < 3; ++ax)
+                        if (qb[ax] == uint32_t(lane)) {
+                            ++own[ax].c0;
+                            for (int k = 0; k < 3; ++k) {
+                                own[ax].mn[k] = min(own[ax].mn[k], qmn[k]);
+                                own[ax].mx[k] = max(own[ax].mx[k], qmx[k]);
+                            }
+                        }
+                }
             } else {
-                pre2 = p;
-                suf2 = q;
+                const Acc* tb = obins + size_t(tk.slot) * 3 * kBins;
+                for (int ax = 0; ax < 3; ++ax) own[ax] = tb[ax * kBins + lane];
             }
-            const GRun right = grun_shfl_down(q, 1);
-            if (lane < kBins - 1 && p.cnt != 0 && p.cnt != work) {
-                const float c = half_area6(p.mn, p.mx) * float(p.cnt) +
-                                half_area6(right.mn, right.mx) * float(right.cnt);
-                if (c < d.cost && c < key) {  // first axis wins a tie
-                    key = c;
-                    idx = ax * kBins + lane;
+            GRun pre0, pre1, pre2, suf0, suf1, suf2;
+            float key = __builtin_inff();
+            int idx = 3 * kBins;
+    #pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                GRun p;
+                for (int k = 0; k < 3; ++k) {
+                    p.mn[k] = fdec(own[ax].mn[k]);
+                    p.mx[k] = fdec(own[ax].mx[k]);
+                }
+                p.cnt = own[ax].c0;
+                GRun q = p;
+                for (int off = 1; off < kBins; off <<= 1) {
+                    GRun lower = grun_shfl_up(p, off);
+                    const GRun higher = grun_shfl_down(q, off);
+                    if (lane >= off) {
+                        grun_join(lower, p);
+                        p = lower;
+                    }
+                    if (lane + off < kBins) grun_join(q, higher);
+                }
+                if (ax == 0) {
+                    pre0 = p;
+                    suf0 = q;
+                } else if (ax == 1) {
+                    pre1 = p;
+                    suf1 = q;
+                } else {
+                    pre2 = p;
+                    suf2 = q;
+                }
+                const GRun right = grun_shfl_down(q, 1);
+                if (lane < kBins - 1 && p.cnt != 0 && p.cnt != work) {
+                    const float c = half_area6(p.mn, p.mx) * float(p.cnt) +
+                                    half_area6(right.mn, right.mx) * float(right.cnt);
+                    if (c < d.cost && c < key) {  // first axis wins a tie
+                        key = c;
+                        idx = ax * kBins + lane;
+                    }
                 }
             }
-        }
-        for (int off = kBins / 2; off > 0; off >>= 1) {
-            const float ok = __shfl_xor(key, off, kBins);
-            const int oi = __shfl_xor(idx, off, kBins);
-            if (ok < key || (ok == key && oi < idx)) {
-                key = ok;
-                idx = oi;
+            for (int off = kBins / 2; off > 0; off >>= 1) {
+                const float ok = __shfl_xor(key, off, kBins);
+                const int oi = __shfl_xor(idx, off, kBins);
+                if (ok < key || (ok == key && oi < idx)) {
+                    key = ok;
+                    idx = oi;
+                }
+            }
+            if (idx < 3 * kBins) {
+                const uint32_t ax = uint32_t(idx / kBins);
+                const int i = idx % kBins;
+                const GRun lbest = grun_shfl(grun_pick(ax, pre0, pre1, pre2), i);
+                const GRun rbest = grun_shfl(grun_pick(ax, suf0, suf1, suf2), i + 1);
+                d.cost = key;
+                d.kind = kObject;
+                d.axis = ax;
+                d.k = uint32_t(i + 1);
+                d.left = lbest.cnt;
+                d.right = work - lbest.cnt;
+                Box ov;
+                for (int k = 0; k < 3; ++k) {
+                    ov.mn[k] = fmaxf(lbest.mn[k], rbest.mn[k]);
+                    ov.mx[k] = fminf(lbest.mx[k], rbest.mx[k]);
+                }
+                if (box_valid(ov) && half_area6(ov.mn, ov.mx) > kAlpha * *root_half_area) cand = 1;
+            } else {
+                cand = 1;  // every centroid in one bin: a plane may still separate them
             }
         }
-        if (idx < 3 * kBins) {
-            const uint32_t ax = uint32_t(idx / kBins);
-            const int i = idx % kBins;
-            const GRun lbest = grun_shfl(grun_pick(ax, pre0, pre1, pre2), i);
-            const GRun rbest = grun_shfl(grun_pick(ax, suf0, suf1, suf2), i + 1);
-            d.cost = key;
-            d.kind = kObject;
-            d.axis = ax;
-            d.k = uint32_t(i + 1);
-            d.left = lbest.cnt;
-            d.right = work - lbest.cnt;
-            Box ov;
-            for (int k = 0; k < 3; ++k) {
-                ov.mn[k] = fmaxf(lbest.mn[k], rbest.mn[k]);
-                ov.mx[k] = fminf(lbest.mx[k], rbest.mx[k]);
-            }
-            if (box_valid(ov) && half_area6(ov.mn, ov.mx) > kAlpha * *root_half_area) cand = 1;
-        } else {
-            cand = 1;  // every centroid in one bin: a plane may still separate them
+        if (lane == 0) {
+            d.cand = cand;
+            dec[t] = d;
+            cand_flag[t] = cand;
         }
-    }
-    if (lane == 0) {
-        d.cand = cand;
-        dec[t] = d;
-        cand_flag[t] = cand;
     }
 }
 
@@ -655,22 +666,23 @@ __device__ __forceinline__ bool sp_evaluated(const SDec* dec, const uint32_t* ca
 
 __global__ void __launch_bounds__(kBlock)
     sp_bin_kernel(const SLevel* L, const SRef* refs, const SDec* dec, const uint32_t* cand_excl,
-                  const float* verts, uint32_t prim_count, Acc* sbins) {
+                  const float* verts, uint32_t prim_count, Acc* sbins, uint32_t run) {
     __shared__ Acc s_b[kLdsTasks][3 * kBins];
     __shared__ int32_t s_t0, s_nt;
     const uint32_t nrefs = L->nrefs;
-    if (blockIdx.x * kBlock >= nrefs || L->cand_total == 0) return;  // whole block
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t first = blockIdx.x * run;
+    if (first >= nrefs || L->cand_total == 0) return;  // whole block
+    const uint32_t end = first + run < nrefs ? first + run : nrefs;
     if (threadIdx.x == 0) {
         int32_t nt = 0;
-        s_t0 = block_tasks(refs, nrefs, &nt);
+        s_t0 = block_tasks(refs, first, end, &nt);
         s_nt = nt;
     }
     for (uint32_t k = threadIdx.x; k < uint32_t(kLdsTasks * 3 * kBins); k += kBlock)
         acc_clear(s_b[k / (3 * kBins)][k % (3 * kBins)]);
     __syncthreads();
     const int32_t t0 = s_t0;
-    if (i < nrefs) {
+    for (uint32_t i = first + threadIdx.x; i < end; i += kBlock) {
         const SRef r = refs[i];
         const int32_t t = r.task;
         if (t >= 0 && sp_evaluated(dec, cand_excl, t) && r.prim < prim_count) {
@@ -851,20 +863,25 @@ __global__ void decide_kernel(const SLevel* L, const STask* tasks, SDec* dec,
                 const Acc* b = work > kSmallTask
                                    ? obins + size_t(tk.slot) * 3 * kBins + ax * kBins
                                    : nullptr;
+                // a small task's bins were never stored: count its
+                // references' centroid bins (obj_bin_kernel's count) in one
+                // pass, 8-bit counters packed eight to a word
+                uint64_t h_lo = 0, h_hi = 0;
+                if (work <= kSmallTask)
+                    for (uint32_t q = tk.begin; q < tk.end; ++q) {
+                        const SRef& r = refs[q];
+                        const uint32_t bq = bin_of(0.5f * (r.mn[ax] + r.mx[ax]), org, inv);
+                        if (bq < 8)
+                            h_lo += 1ull << (8 * bq);
+                        else
+                            h_hi += 1ull << (8 * (bq - 8));
+                    }
                 uint32_t c = 0, k = 0;
                 for (int i = 0; i < kBins - 1; ++i) {
-                    if (work > kSmallTask) {
+                    if (work > kSmallTask)
                         c += b[i].c0;
-                    } else {
-                        // a small task's bins were never stored: count its
-                        // references' centroid bins (obj_bin_kernel's count)
-                        for (uint32_t q = tk.begin; q < tk.end; ++q) {
-                            const SRef& r = refs[q];
-                            c += bin_of(0.5f * (r.mn[ax] + r.mx[ax]), org, inv) == uint32_t(i)
-                                     ? 1u
-                                     : 0u;
-                        }
-                    }
+                    else
+                        c += uint32_t((i < 8 ? h_lo >> (8 * i) : h_hi >> (8 * (i - 8))) & 0xFFu);
                     if (c >= work * 2 / 5 + 1) {
                         k = uint32_t(i + 1);
                         break;
@@ -1211,8 +1228,15 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
         for (uint32_t k = level; k < end; ++k) {
             const uint32_t sh = std::min<uint32_t>(k - level, 32);
             const uint32_t tb = uint32_t(std::min<uint64_t>(ref_bound, known_tasks << sh));
+            // binning runs: kRunLong references per block while the level has
+            // at most kLdsTasks tasks, kRunMid up to kRunMidTasks, else kBlock
+            const uint32_t run = tb <= uint32_t(kLdsTasks) ? kRunLong
+                                 : tb <= kRunMidTasks      ? kRunMid
+                                                           : uint32_t(kBlock);
+            const uint32_t bg = (ref_bound + run - 1) / run;
             const uint32_t tg = (tb + kBlock - 1) / kBlock;
-            const uint32_t gg = (tb + kBlock / kBins - 1) / (kBlock / kBins);  // task groups
+            const uint32_t gg = std::min<uint32_t>((tb + kBlock / kBins - 1) / (kBlock / kBins),
+                                                   kSplitGrid);  // task groups
             const uint32_t ccl = std::min(tb, kMaxCand) * 3 * kBins;
             const uint32_t cg = std::min<uint32_t>(
                 std::max<uint32_t>((ccl + kBlock - 1) / kBlock, (tb + kBlock - 1) / kBlock), 4096);
@@ -1224,12 +1248,12 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
             const STask* tc = tasks[a];
             hipLaunchKernelGGL(task_clear_kernel, dim3(std::min<uint32_t>(tb, 4096)), dim3(64), 0,
                                stream, L, tc, tbox, obins);
-            hipLaunchKernelGGL(ref_bounds_kernel, dim3(rg), dim3(kBlock), 0, stream, L, rc, tc,
-                               tbox);
+            hipLaunchKernelGGL(ref_bounds_kernel, dim3(bg), dim3(kBlock), 0, stream, L, rc, tc,
+                               tbox, run);
             hipLaunchKernelGGL(task_setup_kernel, dim3(tg), dim3(kBlock), 0, stream, L, tc, rc,
                                tbox, dec, root_half_area, k);
-            hipLaunchKernelGGL(obj_bin_kernel, dim3(rg), dim3(kBlock), 0, stream, L, rc, tc, dec,
-                               obins);
+            hipLaunchKernelGGL(obj_bin_kernel, dim3(bg), dim3(kBlock), 0, stream, L, rc, tc, dec,
+                               obins, run);
             hipLaunchKernelGGL(obj_split_kernel, dim3(gg), dim3(kBlock), 0, stream, L, tc, dec,
                                obins, rc, root_half_area, cand);
             if ((e = scan_exclusive_dev(ScanArrays{{cand}}, 1, &L->num_tasks, tb,
@@ -1238,8 +1262,8 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
                 return fail(e, "spatial candidate scan");
             hipLaunchKernelGGL(cand_clear_kernel, dim3(cg), dim3(kBlock), 0, stream, L, cand,
                                cand_list, dup, sbins);
-            hipLaunchKernelGGL(sp_bin_kernel, dim3(rg), dim3(kBlock), 0, stream, L, rc, dec, cand,
-                               verts, n, sbins);
+            hipLaunchKernelGGL(sp_bin_kernel, dim3(bg), dim3(kBlock), 0, stream, L, rc, dec, cand,
+                               verts, n, sbins, run);
             hipLaunchKernelGGL(sp_split_kernel, dim3(sg), dim3(kBlock), 0, stream, L, cand_list,
                                tc, dec, sbins, dup);
             if ((e = scan_exclusive_dev(ScanArrays{{dup}}, 1, &L->num_tasks, tb,

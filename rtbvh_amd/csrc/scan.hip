@@ -13,24 +13,44 @@ constexpr int kThreads = 256;
 constexpr int kPer = 8;
 constexpr uint32_t kTile = kThreads * kPer;  // values per block
 
-__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* sh, uint32_t* total) {
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < kThreads; off <<= 1) {
-        const uint32_t t = threadIdx.x >= uint32_t(off) ? sh[threadIdx.x - off] : 0u;
-        __syncthreads();
-        sh[threadIdx.x] += t;
-        __syncthreads();
+// Inclusive scan over the 64 lanes of a wave (shuffles, no barrier).
+__device__ __forceinline__ uint32_t wave_inclusive(uint32_t v) {
+    const uint32_t lane = threadIdx.x & 63u;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= uint32_t(off)) v += t;
     }
-    const uint32_t incl = sh[threadIdx.x];
-    if (total) *total = sh[kThreads - 1];
+    return v;
+}
+
+// Exclusive scan over a block of N threads: wave scans, then the wave
+// totals through sh (N / 64 words); *total (when not null) gets the sum.
+// Two barriers; sh may be reused as soon as it returns.
+template <int N>
+__device__ __forceinline__ uint32_t block_exclusive_n(uint32_t v, uint32_t* sh, uint32_t* total) {
+    constexpr int kWaves = N / 64;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t incl = wave_inclusive(v);
+    if (lane == 63) sh[wave] = incl;
     __syncthreads();
-    return incl - v;
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < kWaves; ++w) {
+        const uint32_t t = sh[w];
+        before += w < int(wave) ? t : 0u;
+        all += t;
+    }
+    if (total) *total = all;
+    __syncthreads();
+    return before + incl - v;
+}
+
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* sh, uint32_t* total) {
+    return block_exclusive_n<kThreads>(v, sh, total);
 }
 
 __global__ void __launch_bounds__(kThreads)
     tile_sums_kernel(const uint32_t* data, uint32_t count, uint32_t* sums) {
-    __shared__ uint32_t sh[kThreads];
+    __shared__ uint32_t sh[kThreads / 64];
     const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPer;
     uint32_t s = 0;
     for (int k = 0; k < kPer; ++k)
@@ -42,7 +62,7 @@ __global__ void __launch_bounds__(kThreads)
 
 __global__ void __launch_bounds__(kThreads)
     tile_scan_kernel(uint32_t* data, uint32_t count, const uint32_t* offsets) {
-    __shared__ uint32_t sh[kThreads];
+    __shared__ uint32_t sh[kThreads / 64];
     const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPer;
     uint32_t v[kPer];
     uint32_t s = 0;
@@ -60,27 +80,20 @@ __global__ void __launch_bounds__(kThreads)
 // Single-block exclusive scan for the tile sums; writes the grand total.
 __global__ void __launch_bounds__(1024) sums_scan_kernel(uint32_t* data, uint32_t count,
                                                          uint32_t* total) {
-    __shared__ uint32_t sh[1024];
+    __shared__ uint32_t sh[1024 / 64];
     const uint32_t per = (count + 1023) / 1024;
     const uint32_t b = threadIdx.x * per;
     const uint32_t e = b + per < count ? b + per : count;
     uint32_t s = 0;
     for (uint32_t i = b; i < e; ++i) s += data[i];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t t = threadIdx.x >= uint32_t(off) ? sh[threadIdx.x - off] : 0u;
-        __syncthreads();
-        sh[threadIdx.x] += t;
-        __syncthreads();
-    }
-    uint32_t run = threadIdx.x ? sh[threadIdx.x - 1] : 0u;
+    uint32_t all = 0;
+    uint32_t run = block_exclusive_n<1024>(s, sh, &all);
     for (uint32_t i = b; i < e; ++i) {
         const uint32_t v = data[i];
         data[i] = run;
         run += v;
     }
-    if (threadIdx.x == 1023 && total) *total = sh[1023];
+    if (threadIdx.x == 0 && total) *total = all;
 }
 
 // Device-count variants (scan_exclusive_dev): K arrays scanned together,
@@ -88,7 +101,7 @@ __global__ void __launch_bounds__(1024) sums_scan_kernel(uint32_t* data, uint32_
 template <int K>
 __global__ void __launch_bounds__(kThreads)
     tile_sums_dev_kernel(ScanArrays a, const uint32_t* count_p, uint32_t* sums, uint32_t stride) {
-    __shared__ uint32_t sh[kThreads];
+    __shared__ uint32_t sh[kThreads / 64];
     const uint32_t count = *count_p;
     if (blockIdx.x * kTile >= count) return;  // whole block
     const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPer;
@@ -106,7 +119,7 @@ template <int K>
 __global__ void __launch_bounds__(1024)
     sums_scan_dev_kernel(uint32_t* sums, uint32_t stride, const uint32_t* count_p,
                          ScanTotals totals) {
-    __shared__ uint32_t sh[1024];
+    __shared__ uint32_t sh[1024 / 64];
     const uint32_t count = *count_p;
     const uint32_t tiles = (count + kTile - 1) / kTile;
     const uint32_t per = (tiles + 1023) / 1024;
@@ -116,22 +129,14 @@ __global__ void __launch_bounds__(1024)
         uint32_t* data = sums + k * stride;
         uint32_t s = 0;
         for (uint32_t i = b; i < e; ++i) s += data[i];
-        sh[threadIdx.x] = s;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            const uint32_t t = threadIdx.x >= uint32_t(off) ? sh[threadIdx.x - off] : 0u;
-            __syncthreads();
-            sh[threadIdx.x] += t;
-            __syncthreads();
-        }
-        uint32_t run = threadIdx.x ? sh[threadIdx.x - 1] : 0u;
+        uint32_t all = 0;
+        uint32_t run = block_exclusive_n<1024>(s, sh, &all);
         for (uint32_t i = b; i < e; ++i) {
             const uint32_t v = data[i];
             data[i] = run;
             run += v;
         }
-        if (threadIdx.x == 1023 && totals.p[k]) *totals.p[k] = sh[1023];
-        __syncthreads();  // sh is reused by the next array
+        if (threadIdx.x == 0 && totals.p[k]) *totals.p[k] = all;
     }
 }
 
@@ -139,7 +144,7 @@ template <int K>
 __global__ void __launch_bounds__(kThreads)
     tile_scan_dev_kernel(ScanArrays a, const uint32_t* count_p, const uint32_t* sums,
                          uint32_t stride) {
-    __shared__ uint32_t sh[kThreads];
+    __shared__ uint32_t sh[kThreads / 64];
     const uint32_t count = *count_p;
     if (blockIdx.x * kTile >= count) return;  // whole block
     const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPer;
