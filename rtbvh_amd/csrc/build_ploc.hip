@@ -27,6 +27,7 @@
 #include <cmath>
 #include <vector>
 
+#include "block_scan.hpp"
 #include "internal.hpp"
 
 namespace rtbvh {
@@ -518,7 +519,7 @@ __global__ void __launch_bounds__(kBlock)
 // Exclusive scan of the tile counts; writes M into states[k] and the next
 // iteration's bookkeeping into states[k+1] (locb.rs:178-185).
 __global__ void __launch_bounds__(1024) tile_scan_kernel(PlocState* states, uint32_t* tile_count) {
-    __shared__ uint32_t sums[1024];
+    __shared__ uint32_t sums[1024 / 64];
     const PlocState stt = states[0];
     if (stt.n_active <= stt.tail_at) {
         if (threadIdx.x == 0) {
@@ -535,22 +536,14 @@ __global__ void __launch_bounds__(1024) tile_scan_kernel(PlocState* states, uint
     const uint32_t e = b + per < tiles ? b + per : tiles;
     uint32_t s = 0;
     for (uint32_t i = b; i < e; ++i) s += tile_count[i];
-    sums[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = threadIdx.x >= uint32_t(off) ? sums[threadIdx.x - off] : 0u;
-        __syncthreads();
-        sums[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = threadIdx.x ? sums[threadIdx.x - 1] : 0u;
+    uint32_t m = 0;
+    uint32_t run = block_exclusive_n<1024>(s, sums, &m);
     for (uint32_t i = b; i < e; ++i) {
         const uint32_t v = tile_count[i];
         tile_count[i] = run;
         run += v;
     }
-    if (threadIdx.x == 1023) {
-        const uint32_t m = sums[1023];
+    if (threadIdx.x == 0) {
         states[0].merged = m;
         PlocState nx;
         nx.n_active = n - m;
@@ -572,7 +565,7 @@ __global__ void __launch_bounds__(kBlock)
                  uint32_t* cs1, const uint32_t* nb, const uint32_t* mark,
                  const uint32_t* tile_offset, DevAabb* nodes) {
     __shared__ uint32_t s_m[kRadius + kTile];
-    __shared__ uint32_t s_scan[kBlock];
+    __shared__ uint32_t s_scan[kBlock / 64];
     const PlocState stt = *state;
     if (stt.n_active <= stt.tail_at) return;
     const uint32_t n = stt.n_active;
@@ -593,15 +586,7 @@ __global__ void __launch_bounds__(kBlock)
         v[k] = i < te ? mark[i] : 0u;
         s += v[k];
     }
-    s_scan[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < kBlock; off <<= 1) {
-        const uint32_t t = threadIdx.x >= uint32_t(off) ? s_scan[threadIdx.x - off] : 0u;
-        __syncthreads();
-        s_scan[threadIdx.x] += t;
-        __syncthreads();
-    }
-    uint32_t run = excl + (threadIdx.x ? s_scan[threadIdx.x - 1] : 0u);
+    uint32_t run = excl + block_exclusive_n<kBlock>(s, s_scan, nullptr);
     for (int k = 0; k < kPer; ++k) {
         run += v[k];
         s_m[kRadius + threadIdx.x * kPer + k] = run;
