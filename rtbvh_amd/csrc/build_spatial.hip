@@ -248,19 +248,6 @@ __global__ void point_refs_kernel(const float* centers, uint32_t stride, uint32_
 // Per task: node box and object bins cleared (block t, 64 threads; a task of
 // at most kSmallTask references has no stored object bins: obj_split_kernel
 // bins it in registers).
-__global__ void task_clear_kernel(const SLevel* L, const STask* tasks, Acc* tbox, Acc* obins) {
-    const uint32_t num_tasks = L->num_tasks;
-    for (uint32_t t = blockIdx.x; t < num_tasks; t += gridDim.x) {
-        if (threadIdx.x == 0) acc_clear(tbox[t]);
-        const uint32_t slot = tasks[t].slot;
-        if (threadIdx.x < 3 * kBins && slot != kNoSlot) {
-            Acc a;
-            acc_clear(a);
-            obins[size_t(slot) * 3 * kBins + threadIdx.x] = a;
-        }
-    }
-}
-
 // The spatial bins of the level's evaluated candidates (the first
 // kMaxCand) cleared, the list of their tasks (list[cand_excl[t]] = t), and
 // dup[t] = 0 for every other task.
@@ -995,8 +982,19 @@ __global__ void scatter_kernel(const SLevel* L, const SRef* refs, const STask* t
                                const uint32_t* out_excl, const uint32_t* leaf_excl,
                                const uint32_t* fl, const uint32_t* fr, const uint32_t* fl_excl,
                                const uint32_t* fr_excl, const float* verts, uint32_t prim_count,
-                               uint32_t cap, SRef* out, uint32_t* indices, uint32_t* fail) {
+                               uint32_t cap, SRef* out, uint32_t* indices, uint32_t* fail,
+                               Acc* tbox, Acc* obins, uint32_t slot_cap) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    // the next level's node boxes and object bins: this level's readers
+    // (task_setup_kernel, obj_split_kernel, decide_kernel) are done
+    if (L->num_tasks > 0) {
+        const uint32_t step = gridDim.x * blockDim.x;
+        // (bounded by the buffers: counts past them have failed the build)
+        const uint32_t next_tasks = min(2 * L->splits, cap);
+        const uint32_t bins = min(L->slots, slot_cap) * 3 * kBins;
+        for (uint32_t t = i; t < next_tasks; t += step) acc_clear(tbox[t]);
+        for (uint32_t q = i; q < bins; q += step) acc_clear(obins[q]);
+    }
     if (i >= L->nrefs) return;
     const uint32_t leaf_base = L->leaf_base;
     const SRef r = refs[i];
@@ -1053,9 +1051,15 @@ __global__ void scatter_kernel(const SLevel* L, const SRef* refs, const STask* t
     }
 }
 
-// The root task and level 0's record (the records are zero before).
-__global__ void root_task_kernel(STask* task, SLevel* levels, uint32_t n) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+// The root task, level 0's record (the records are zero before) and the
+// root's node box and object bins cleared (scatter_kernel clears them for
+// the levels after).
+__global__ void root_task_kernel(STask* task, SLevel* levels, uint32_t n, Acc* tbox,
+                                 Acc* obins) {
+    if (blockIdx.x != 0) return;
+    if (threadIdx.x < uint32_t(3 * kBins)) acc_clear(obins[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        acc_clear(tbox[0]);
         STask t;
         t.node = 0;
         t.begin = 0;
@@ -1151,7 +1155,7 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
     const size_t sz_sbins = align256(size_t(std::min(cap, kMaxCand)) * 3 * kBins * sizeof(Acc));
     const size_t sz_u32 = align256(size_t(cap) * 4);
     const size_t sz_list = align256(size_t(std::min(cap, kMaxCand)) * 4);
-    const size_t sz_scan = align256(scan_dev_scratch_bytes(cap, 4));
+    const size_t sz_scan = align256(scan_dev_scratch_bytes(cap, 6));
     const size_t sz_levels = align256(size_t(kLevels) * sizeof(SLevel));
     const size_t sz_words = 256;
     const size_t total = 2 * sz_refs + 2 * sz_tasks + sz_dec + sz_tbox + sz_obins + sz_sbins +
@@ -1207,7 +1211,8 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
     else
         hipLaunchKernelGGL(point_refs_kernel, dim3(g0), dim3(kBlock), 0, stream, centers, stride,
                            n, refs[0]);
-    hipLaunchKernelGGL(root_task_kernel, dim3(1), dim3(1), 0, stream, tasks[0], levels, n);
+    hipLaunchKernelGGL(root_task_kernel, dim3(1), dim3(64), 0, stream, tasks[0], levels, n, tbox,
+                       obins);
     const uint32_t budget = cap - n;
     // Levels are enqueued in batches without waiting; the host reads one
     // level record per batch.  Grids come from bounds on the counts of the
@@ -1246,8 +1251,6 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
             const uint32_t a = k & 1u, b = a ^ 1u;
             const SRef* rc = refs[a];
             const STask* tc = tasks[a];
-            hipLaunchKernelGGL(task_clear_kernel, dim3(std::min<uint32_t>(tb, 4096)), dim3(64), 0,
-                               stream, L, tc, tbox, obins);
             hipLaunchKernelGGL(ref_bounds_kernel, dim3(bg), dim3(kBlock), 0, stream, L, rc, tc,
                                tbox, run);
             hipLaunchKernelGGL(task_setup_kernel, dim3(tg), dim3(kBlock), 0, stream, L, tc, rc,
@@ -1256,7 +1259,7 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
                                obins, run);
             hipLaunchKernelGGL(obj_split_kernel, dim3(gg), dim3(kBlock), 0, stream, L, tc, dec,
                                obins, rc, root_half_area, cand);
-            if ((e = scan_exclusive_dev(ScanArrays{{cand}}, 1, &L->num_tasks, tb,
+            if ((e = scan_exclusive_dev(ScanArrays{{cand}}, 1, ScanCounts{{&L->num_tasks}}, tb,
                                         ScanTotals{{&Lw->cand_total}}, scan_scratch, sz_scan,
                                         stream)) != hipSuccess)
                 return fail(e, "spatial candidate scan");
@@ -1266,29 +1269,30 @@ ResultCode gpu_build_spatial(const RTAabb* aabbs, const float* centers, size_t s
                                verts, n, sbins, run);
             hipLaunchKernelGGL(sp_split_kernel, dim3(sg), dim3(kBlock), 0, stream, L, cand_list,
                                tc, dec, sbins, dup);
-            if ((e = scan_exclusive_dev(ScanArrays{{dup}}, 1, &L->num_tasks, tb,
+            if ((e = scan_exclusive_dev(ScanArrays{{dup}}, 1, ScanCounts{{&L->num_tasks}}, tb,
                                         ScanTotals{{&Lw->dup_total}}, scan_scratch, sz_scan,
                                         stream)) != hipSuccess)
                 return fail(e, "spatial budget scan");
             hipLaunchKernelGGL(decide_kernel, dim3(tg), dim3(kBlock), 0, stream, L, tc, dec, dup,
                                budget, leaf, obins, rc, split_flag, out_cnt, leaf_cnt, big_cnt);
+            hipLaunchKernelGGL(side_kernel, dim3(rg), dim3(kBlock), 0, stream, L, rc, tc, dec, fl,
+                               fr, fle, fre);
+            // the task counts (tb of them at most) and the side flags (ref_bound)
+            const ScanCounts cnt6{{&L->num_tasks, &L->num_tasks, &L->num_tasks, &L->num_tasks,
+                                   &L->nrefs, &L->nrefs}};
             if ((e = scan_exclusive_dev(
-                     ScanArrays{{split_flag, out_cnt, leaf_cnt, big_cnt}}, 4, &L->num_tasks, tb,
+                     ScanArrays{{split_flag, out_cnt, leaf_cnt, big_cnt, fle, fre}}, 6, cnt6,
+                     std::max(tb, ref_bound),
                      ScanTotals{{&Lw->splits, &Lw->next_refs, &Lw->leaf_refs, &Lw->slots}},
                      scan_scratch, sz_scan, stream)) != hipSuccess)
-                return fail(e, "spatial task scans");
+                return fail(e, "spatial task and partition scans");
             hipLaunchKernelGGL(children_kernel, dim3(tg), dim3(kBlock), 0, stream, L, tc, dec,
                                split_flag, out_cnt, leaf_cnt, big_cnt, node_cap, cap, res.nodes,
                                tasks[b], fail_word);
-            hipLaunchKernelGGL(side_kernel, dim3(rg), dim3(kBlock), 0, stream, L, rc, tc, dec, fl,
-                               fr, fle, fre);
-            if ((e = scan_exclusive_dev(ScanArrays{{fle, fre}}, 2, &L->nrefs, ref_bound,
-                                        ScanTotals{{nullptr, nullptr}}, scan_scratch, sz_scan,
-                                        stream)) != hipSuccess)
-                return fail(e, "spatial partition scans");
             hipLaunchKernelGGL(scatter_kernel, dim3(rg), dim3(kBlock), 0, stream, L, rc, tc, dec,
                                split_flag, out_cnt, leaf_cnt, fl, fr, fle, fre, verts, n, cap,
-                               refs[b], res.indices, fail_word);
+                               refs[b], res.indices, fail_word, tbox, obins,
+                               uint32_t(slot_cap));
             hipLaunchKernelGGL(level_end_kernel, dim3(1), dim3(64), 0, stream, levels, k, cap,
                                fail_word);
         }

@@ -167,16 +167,21 @@ hipError_t trace_tree_enqueue(const TraceTreePlan* t, const DeviceBvh& tree, con
 uint32_t trace_tree_levels(const TraceTreePlan* t);
 void trace_tree_plan_destroy(TraceTreePlan* t);
 
-// Exclusive scans whose length is read on the device (scan.hip): up to
-// four arrays at once, totals into device memory.
+// Exclusive scans whose lengths are read on the device (scan.hip): up to
+// kScanArraysMax arrays at once, each of its own length, totals into device
+// memory.
+constexpr int kScanArraysMax = 6;
 struct ScanArrays {
-    uint32_t* p[4];
+    uint32_t* p[kScanArraysMax];
+};
+struct ScanCounts {
+    const uint32_t* p[kScanArraysMax];
 };
 struct ScanTotals {
-    uint32_t* p[4];
+    uint32_t* p[kScanArraysMax];
 };
 size_t scan_dev_scratch_bytes(uint32_t max_count, int k);
-hipError_t scan_exclusive_dev(const ScanArrays& a, int k, const uint32_t* count,
+hipError_t scan_exclusive_dev(const ScanArrays& a, int k, const ScanCounts& counts,
                               uint32_t max_count, const ScanTotals& totals, void* scratch,
                               size_t scratch_bytes, hipStream_t stream);
 

@@ -67,15 +67,29 @@ __global__ void __launch_bounds__(1024) sums_scan_kernel(uint32_t* data, uint32_
 }
 
 // Device-count variants (scan_exclusive_dev): K arrays scanned together,
-// their length read on the device; blocks past it return at once.
+// each of its own length, read on the device; a block past every length
+// returns at once, and a block past one array's length skips that array
+// (a uniform branch: the block's barriers stay matched).
+template <int K>
+__device__ __forceinline__ uint32_t load_counts(const ScanCounts& c, uint32_t (&cnt)[K]) {
+    uint32_t most = 0;
+    for (int k = 0; k < K; ++k) {
+        cnt[k] = *c.p[k];
+        most = cnt[k] > most ? cnt[k] : most;
+    }
+    return most;
+}
+
 template <int K>
 __global__ void __launch_bounds__(kThreads)
-    tile_sums_dev_kernel(ScanArrays a, const uint32_t* count_p, uint32_t* sums, uint32_t stride) {
+    tile_sums_dev_kernel(ScanArrays a, ScanCounts counts, uint32_t* sums, uint32_t stride) {
     __shared__ uint32_t sh[kThreads / 64];
-    const uint32_t count = *count_p;
-    if (blockIdx.x * kTile >= count) return;  // whole block
+    uint32_t cnt[K];
+    if (blockIdx.x * kTile >= load_counts<K>(counts, cnt)) return;  // whole block
     const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPer;
     for (int k = 0; k < K; ++k) {
+        const uint32_t count = cnt[k];
+        if (blockIdx.x * kTile >= count) continue;
         uint32_t s = 0;
         for (int q = 0; q < kPer; ++q)
             if (base + q < count) s += a.p[k][base + q];
@@ -87,15 +101,15 @@ __global__ void __launch_bounds__(kThreads)
 
 template <int K>
 __global__ void __launch_bounds__(1024)
-    sums_scan_dev_kernel(uint32_t* sums, uint32_t stride, const uint32_t* count_p,
-                         ScanTotals totals) {
+    sums_scan_dev_kernel(uint32_t* sums, uint32_t stride, ScanCounts counts, ScanTotals totals) {
     __shared__ uint32_t sh[1024 / 64];
-    const uint32_t count = *count_p;
-    const uint32_t tiles = (count + kTile - 1) / kTile;
-    const uint32_t per = (tiles + 1023) / 1024;
-    const uint32_t b = threadIdx.x * per;
-    const uint32_t e = b + per < tiles ? b + per : tiles;
+    uint32_t cnt[K];
+    load_counts<K>(counts, cnt);
     for (int k = 0; k < K; ++k) {
+        const uint32_t tiles = (cnt[k] + kTile - 1) / kTile;
+        const uint32_t per = (tiles + 1023) / 1024;
+        const uint32_t b = threadIdx.x * per;
+        const uint32_t e = b + per < tiles ? b + per : tiles;
         uint32_t* data = sums + k * stride;
         uint32_t s = 0;
         for (uint32_t i = b; i < e; ++i) s += data[i];
@@ -112,13 +126,14 @@ __global__ void __launch_bounds__(1024)
 
 template <int K>
 __global__ void __launch_bounds__(kThreads)
-    tile_scan_dev_kernel(ScanArrays a, const uint32_t* count_p, const uint32_t* sums,
-                         uint32_t stride) {
+    tile_scan_dev_kernel(ScanArrays a, ScanCounts counts, const uint32_t* sums, uint32_t stride) {
     __shared__ uint32_t sh[kThreads / 64];
-    const uint32_t count = *count_p;
-    if (blockIdx.x * kTile >= count) return;  // whole block
+    uint32_t cnt[K];
+    if (blockIdx.x * kTile >= load_counts<K>(counts, cnt)) return;  // whole block
     const uint32_t base = blockIdx.x * kTile + threadIdx.x * kPer;
     for (int k = 0; k < K; ++k) {
+        const uint32_t count = cnt[k];
+        if (blockIdx.x * kTile >= count) continue;
         uint32_t v[kPer];
         uint32_t s = 0;
         for (int q = 0; q < kPer; ++q) {
@@ -134,15 +149,15 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 template <int K>
-hipError_t launch_scan_dev(const ScanArrays& a, const uint32_t* count, uint32_t max_count,
+hipError_t launch_scan_dev(const ScanArrays& a, const ScanCounts& counts, uint32_t max_count,
                            const ScanTotals& totals, uint32_t* sums, hipStream_t stream) {
     const uint32_t tiles = (max_count + kTile - 1) / kTile;
     const uint32_t stride = tiles + 1;
-    hipLaunchKernelGGL(tile_sums_dev_kernel<K>, dim3(tiles), dim3(kThreads), 0, stream, a, count,
+    hipLaunchKernelGGL(tile_sums_dev_kernel<K>, dim3(tiles), dim3(kThreads), 0, stream, a, counts,
                        sums, stride);
     hipLaunchKernelGGL(sums_scan_dev_kernel<K>, dim3(1), dim3(1024), 0, stream, sums, stride,
-                       count, totals);
-    hipLaunchKernelGGL(tile_scan_dev_kernel<K>, dim3(tiles), dim3(kThreads), 0, stream, a, count,
+                       counts, totals);
+    hipLaunchKernelGGL(tile_scan_dev_kernel<K>, dim3(tiles), dim3(kThreads), 0, stream, a, counts,
                        sums, stride);
     return hipGetLastError();
 }
@@ -173,23 +188,27 @@ size_t scan_dev_scratch_bytes(uint32_t max_count, int k) {
     return size_t(k) * (tiles + 1) * sizeof(uint32_t);
 }
 
-// In-place exclusive scans of the first k (1..4) arrays of a, each of
-// *count values (read on the device, at most max_count); *totals.p[j]
-// (device memory, or null) receives array j's sum.  scratch holds
-// scan_dev_scratch_bytes(max_count, k).  Nothing waits for the host, so a
-// sequence of them can be enqueued (or recorded) with counts still to come.
-hipError_t scan_exclusive_dev(const ScanArrays& a, int k, const uint32_t* count,
+// In-place exclusive scans of the first k (1..kScanArraysMax) arrays of a,
+// array j of *counts.p[j] values (read on the device, at most max_count);
+// *totals.p[j] (device memory, or null) receives array j's sum.  scratch
+// holds scan_dev_scratch_bytes(max_count, k).  Nothing waits for the host, so
+// a sequence of them can be enqueued (or recorded) with counts still to come.
+hipError_t scan_exclusive_dev(const ScanArrays& a, int k, const ScanCounts& counts,
                               uint32_t max_count, const ScanTotals& totals, void* scratch,
                               size_t scratch_bytes, hipStream_t stream) {
     if (max_count == 0) return hipSuccess;
-    if (k < 1 || k > 4 || scratch_bytes < scan_dev_scratch_bytes(max_count, k))
+    if (k < 1 || k > kScanArraysMax || scratch_bytes < scan_dev_scratch_bytes(max_count, k))
         return hipErrorInvalidValue;
+    for (int j = 0; j < k; ++j)
+        if (counts.p[j] == nullptr || a.p[j] == nullptr) return hipErrorInvalidValue;
     uint32_t* sums = static_cast<uint32_t*>(scratch);
     switch (k) {
-        case 1: return launch_scan_dev<1>(a, count, max_count, totals, sums, stream);
-        case 2: return launch_scan_dev<2>(a, count, max_count, totals, sums, stream);
-        case 3: return launch_scan_dev<3>(a, count, max_count, totals, sums, stream);
-        default: return launch_scan_dev<4>(a, count, max_count, totals, sums, stream);
+        case 1: return launch_scan_dev<1>(a, counts, max_count, totals, sums, stream);
+        case 2: return launch_scan_dev<2>(a, counts, max_count, totals, sums, stream);
+        case 3: return launch_scan_dev<3>(a, counts, max_count, totals, sums, stream);
+        case 4: return launch_scan_dev<4>(a, counts, max_count, totals, sums, stream);
+        case 5: return launch_scan_dev<5>(a, counts, max_count, totals, sums, stream);
+        default: return launch_scan_dev<6>(a, counts, max_count, totals, sums, stream);
     }
 }
 
