@@ -443,9 +443,17 @@ __device__ __forceinline__ void nn_window(uint32_t i, uint32_t n, uint32_t& sb, 
 
 // Nearest neighbour + mutual marks for one tile.  Bounds are staged in LDS
 // as six arrays so that lanes reading consecutive clusters hit consecutive
-// banks.
+// banks.  kNnBlock threads cover the tile and its halo (kTile + 2 kRadius
+// neighbour searches) in one pass, and each search loads its candidates four
+// at a time and compares them in index order (the first lowest distance
+// wins, as in the sequential loop): later iterations are short, and their
+// time is the latency of this chain.
+constexpr int kNnBlock = 320;
+static_assert(kNnBlock >= kTile + 2 * kRadius, "one neighbour search per thread");
+static_assert(kNnBlock >= kTile + 4 * kRadius, "staging in one pass");
+
 template <bool SEG>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kNnBlock)
     nn_mark_kernel(const PlocState* state, const DevAabb* cl0, const DevAabb* cl1,
                    const uint32_t* cs0, const uint32_t* cs1, uint32_t* nb_out,
                    uint32_t* mark_out, uint32_t* tile_count) {
@@ -466,7 +474,7 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t lb = ts > uint32_t(kHalo) ? ts - kHalo : 0u;
     const uint32_t le = te + kHalo < n ? te + kHalo : n;
     if (threadIdx.x == 0) s_total = 0;
-    for (uint32_t k = lb + threadIdx.x; k < le; k += kBlock) {
+    for (uint32_t k = lb + threadIdx.x; k < le; k += kNnBlock) {
         DevAabb a;
         load_node(cl + k, a);
         s_b[0][k - lb] = a.min[0];
@@ -480,7 +488,7 @@ __global__ void __launch_bounds__(kBlock)
     __syncthreads();
     const uint32_t nb0 = ts > uint32_t(kRadius) ? ts - kRadius : 0u;
     const uint32_t nb1 = te + kRadius < n ? te + kRadius : n;
-    for (uint32_t i = nb0 + threadIdx.x; i < nb1; i += kBlock) {
+    for (uint32_t i = nb0 + threadIdx.x; i < nb1; i += kNnBlock) {
         uint32_t s0, s1;
         nn_window(i, n, s0, s1);
         const uint32_t li = i - lb;
@@ -488,22 +496,31 @@ __global__ void __launch_bounds__(kBlock)
         const float a3 = s_b[3][li], a4 = s_b[4][li], a5 = s_b[5][li];
         float best = 3.40282347e38f;  // f32::MAX (locb.rs:95)
         uint32_t best_j = 0xFFFFFFFFu;
-        for (uint32_t j = s0; j < s1; ++j) {
-            if (j == i) continue;
-            const uint32_t lj = j - lb;
-            if (SEG && s_sc[lj] != s_sc[li]) continue;  // other scene
-            const float d = dist6(a0, a1, a2, a3, a4, a5, s_b[0][lj], s_b[1][lj], s_b[2][lj],
-                                  s_b[3][lj], s_b[4][lj], s_b[5][lj]);
-            if (d < best) {
-                best = d;
-                best_j = j;
+        for (uint32_t j0 = s0; j0 < s1; j0 += 4) {
+            float d[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t j = j0 + uint32_t(u);
+                // a position past the window reads a staged cluster and is
+                // dropped (NaN never compares less)
+                const uint32_t lj = (j < s1 ? j : s1 - 1) - lb;
+                const bool skip = j >= s1 || j == i || (SEG && s_sc[lj] != s_sc[li]);
+                const float dd = dist6(a0, a1, a2, a3, a4, a5, s_b[0][lj], s_b[1][lj],
+                                       s_b[2][lj], s_b[3][lj], s_b[4][lj], s_b[5][lj]);
+                d[u] = skip ? __builtin_nanf("") : dd;
             }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (d[u] < best) {
+                    best = d[u];
+                    best_j = j0 + uint32_t(u);
+                }
         }
         s_nb[i - nb0] = best_j;
     }
     __syncthreads();
     uint32_t cnt = 0;
-    for (uint32_t i = ts + threadIdx.x; i < te; i += kBlock) {
+    for (uint32_t i = ts + threadIdx.x; i < te; i += kNnBlock) {
         const uint32_t j = s_nb[i - nb0];
         uint32_t m = 0;
         if (j > i && j < nb1) m = (s_nb[j - nb0] == i) ? 1u : 0u;
@@ -1036,7 +1053,7 @@ hipError_t enqueue_front(const PlocPlanImpl* p, const RTAabb* aabbs, const float
 // the active set fits the tail kernel.
 template <bool SEG>
 void enqueue_iteration(const PlocPlanImpl* p, uint32_t iter, uint32_t tiles, hipStream_t stream) {
-    hipLaunchKernelGGL(nn_mark_kernel<SEG>, dim3(tiles), dim3(kBlock), 0, stream, p->st + iter,
+    hipLaunchKernelGGL(nn_mark_kernel<SEG>, dim3(tiles), dim3(kNnBlock), 0, stream, p->st + iter,
                        p->cl0, p->cl1, p->cs0, p->cs1, p->nbuf, p->mbuf, p->tcount);
     hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, stream, p->st + iter, p->tcount);
     hipLaunchKernelGGL(merge_kernel<SEG>, dim3(tiles), dim3(kBlock), 0, stream, p->st + iter,
