@@ -738,15 +738,28 @@ __global__ void __launch_bounds__(kTailThreads)
             const uint32_t si = SEG ? sh.sc[i] : 0u;
             float best = 3.40282347e38f;
             uint32_t best_j = 0xFFFFu;
-            for (uint32_t j = s0; j < s1; ++j) {
-                if (j == i) continue;
-                if (SEG && sh.sc[j] != si) continue;  // other scene
-                const float d = dist6(a0, a1, a2, a3, a4, a5, sh.b[0][j], sh.b[1][j], sh.b[2][j],
-                                      sh.b[3][j], sh.b[4][j], sh.b[5][j]);
-                if (d < best) {
-                    best = d;
-                    best_j = j;
+            // four candidates in flight, compared in index order (as
+            // nn_mark_kernel), where registers allow: the batch instances
+            // (scene ids, or 4096 clusters) would spill, and keep the
+            // one-at-a-time loop
+            constexpr uint32_t kStep = (!SEG && CAP <= int(kTailEnter)) ? 4u : 1u;
+            for (uint32_t j0 = s0; j0 < s1; j0 += kStep) {
+                float d[kStep];
+#pragma unroll
+                for (int u = 0; u < int(kStep); ++u) {
+                    const uint32_t j = j0 + uint32_t(u);
+                    const uint32_t jc = j < s1 ? j : s1 - 1;
+                    const bool skip = j >= s1 || j == i || (SEG && sh.sc[jc] != si);
+                    const float dd = dist6(a0, a1, a2, a3, a4, a5, sh.b[0][jc], sh.b[1][jc],
+                                           sh.b[2][jc], sh.b[3][jc], sh.b[4][jc], sh.b[5][jc]);
+                    d[u] = skip ? __builtin_nanf("") : dd;
                 }
+#pragma unroll
+                for (int u = 0; u < int(kStep); ++u)
+                    if (d[u] < best) {
+                        best = d[u];
+                        best_j = j0 + uint32_t(u);
+                    }
             }
             sh.nb[i] = uint16_t(best_j);
         }
