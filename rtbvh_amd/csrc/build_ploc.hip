@@ -594,56 +594,52 @@ __global__ void __launch_bounds__(kBlock)
     if (ts >= n) return;
     const uint32_t te = ts + kTile < n ? ts + kTile : n;
     const uint32_t children_begin = stt.end - 2 * stt.merged;
+    static_assert(kTile == kBlock, "one cluster of the tile per thread");
+    // every load that does not wait for another first: the cluster, its
+    // neighbour and mark, the marks of the left halo (one per lane of wave
+    // 0) and the tile's offset; then the neighbour's neighbour and box
+    const uint32_t i = ts + threadIdx.x;
+    const bool live = i < te;
+    const uint32_t mk = live ? mark[i] : 0u;
+    const uint32_t j = live ? nb[i] : 0xFFFFFFFFu;
+    DevAabb ci;
+    if (live) load_node(cl + i, ci);
+    const uint32_t hm =
+        threadIdx.x < uint32_t(kRadius) && ts > threadIdx.x ? mark[ts - 1 - threadIdx.x] : 0u;
     const uint32_t excl = tile_offset[blockIdx.x];  // marks in [0, ts)
-    constexpr int kPer = kTile / kBlock;
-    uint32_t v[kPer];
-    uint32_t s = 0;
-    for (int k = 0; k < kPer; ++k) {
-        const uint32_t i = ts + threadIdx.x * kPer + k;
-        v[k] = i < te ? mark[i] : 0u;
-        s += v[k];
-    }
-    uint32_t run = excl + block_exclusive_n<kBlock>(s, s_scan, nullptr);
-    for (int k = 0; k < kPer; ++k) {
-        run += v[k];
-        s_m[kRadius + threadIdx.x * kPer + k] = run;
-    }
-    // left halo: m[k] = m[ts-1] - sum(mark[k+1 .. ts-1])
-    if (threadIdx.x == 0) {
-        uint32_t acc = excl;
-        for (int h = 1; h <= kRadius; ++h) {
-            if (ts < uint32_t(h)) break;
-            s_m[kRadius - h] = acc;
-            acc -= mark[ts - h];
-        }
+    const uint32_t nbj = live && j < n ? nb[j] : 0xFFFFFFFFu;
+    const bool mutual = nbj == i;
+    const bool upper = mutual && j < i;  // i is the upper element of its pair
+    DevAabb cj;
+    if (upper) load_node(cl + j, cj);
+    // m = inclusive prefix of marks over [ts - r, te)
+    const uint32_t run = excl + block_exclusive_n<kBlock>(mk, s_scan, nullptr) + mk;
+    s_m[kRadius + threadIdx.x] = run;
+    if (threadIdx.x < 64) {
+        // left halo: m[ts - 1 - l] = m[ts - 1] - (mark[ts - 1] + .. + mark[ts - l])
+        const uint32_t hincl = wave_inclusive(hm);
+        if (threadIdx.x < uint32_t(kRadius) && ts > threadIdx.x)
+            s_m[kRadius - 1 - threadIdx.x] = excl - (hincl - hm);
     }
     __syncthreads();
-    for (uint32_t i = ts + threadIdx.x; i < te; i += kBlock) {
-        const uint32_t j = nb[i];
-        const bool mutual = j < n && nb[j] == i;
-        const uint32_t mi = s_m[kRadius + (i - ts)];
-        if (mutual) {
-            if (j < i) {
-                // i is the upper element: the parent takes its compacted slot
-                const uint32_t mj = s_m[kRadius + j - ts];  // j >= ts - r
-                const uint32_t first_child = children_begin + (mj - 1) * 2;
-                DevAabb ci, cj;
-                load_node(cl + i, ci);
-                load_node(cl + j, cj);
-                DevAabb parent = union_node(ci, cj);  // input[j].union_of(input[i]), locb.rs:202
-                parent.extra1 = -1;
-                parent.extra2 = int32_t(first_child);
-                store_node(cl_out + (i - mi), parent);
-                if (SEG) cs_out[i - mi] = cs[i];
-                store_node(nodes + first_child, cj);
-                store_node(nodes + first_child + 1, ci);
-            }
-        } else {
-            DevAabb ci;
-            load_node(cl + i, ci);
-            store_node(cl_out + (i - mi), ci);
+    if (!live) return;
+    const uint32_t mi = run;
+    if (mutual) {
+        if (upper) {
+            // the parent takes the upper element's compacted slot
+            const uint32_t mj = s_m[kRadius + j - ts];  // j >= ts - r
+            const uint32_t first_child = children_begin + (mj - 1) * 2;
+            DevAabb parent = union_node(ci, cj);  // input[j].union_of(input[i]), locb.rs:202
+            parent.extra1 = -1;
+            parent.extra2 = int32_t(first_child);
+            store_node(cl_out + (i - mi), parent);
             if (SEG) cs_out[i - mi] = cs[i];
+            store_node(nodes + first_child, cj);
+            store_node(nodes + first_child + 1, ci);
         }
+    } else {
+        store_node(cl_out + (i - mi), ci);
+        if (SEG) cs_out[i - mi] = cs[i];
     }
 }
 
