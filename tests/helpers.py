@@ -94,3 +94,30 @@ def flat_grid_rays(verts, n, seed=11):
     tmin = np.where(np.arange(n) % 5 == 0, np.float32(0.5), np.float32(1e-4)).astype(np.float32)
     tmax = np.where(np.arange(n) % 7 == 0, np.float32(1.0), np.float32(1e30)).astype(np.float32)
     return np.concatenate([o, tmin[:, None], dirs, tmax[:, None]], 1).astype(np.float32)
+
+
+def geometric_chain(n=80, ratio=1.5, seed=21):
+    """n triangles along x at ratio^k, each a tenth of its distance from the
+    origin in size: binned SAH peels the largest off a few per level, so the
+    tree (23 levels for the default 80) is deeper than the 2 lg n + 2 levels
+    of a builder's first batch, and PLOC's is a chain of n levels."""
+    rng = np.random.default_rng(seed)
+    x = ratio ** np.arange(n, dtype=np.float64)
+    c = np.stack([x, 0.01 * rng.random(n) * x, 0.01 * rng.random(n) * x], 1)
+    s = 0.1 * x
+    v0 = c + np.stack([-s, -s, 0 * s], 1)
+    v1 = c + np.stack([s, -s, 0 * s], 1)
+    v2 = c + np.stack([0 * s, s, s], 1)
+    return np.ascontiguousarray(np.concatenate([v0, v1, v2], 1), dtype=np.float32)
+
+
+def tree_depth(nodes) -> int:
+    """Deepest leaf of a binary node array (count >= 0 marks a leaf)."""
+    depth, stack = 0, [(0, 0)]
+    while stack:
+        i, d = stack.pop()
+        depth = max(depth, d)
+        if int(nodes[i]["count"]) < 0:
+            lf = int(nodes[i]["left_first"])
+            stack += [(lf, d + 1), (lf + 1, d + 1)]
+    return depth

@@ -14,8 +14,8 @@ import pytest
 
 from oracle import refcpu as ref
 from rtbvh_amd import scenes
-from tests.helpers import (as_rays, flat, flat_grid, flat_grid_rays, hits_equal, prep,
-                           ray_set, teapot)
+from tests.helpers import (as_rays, flat, flat_grid, flat_grid_rays, geometric_chain,
+                           hits_equal, prep, ray_set, teapot, tree_depth)
 
 torch = pytest.importorskip("torch")
 import rtbvh_amd as rt  # noqa: E402
@@ -58,6 +58,7 @@ SCENES = {
     "n5": lambda: scenes.random_soup(5, 0.2, seed=2),
     "n100": lambda: scenes.random_soup(100, 0.1, seed=4),
     "n5000_dups": lambda: np.repeat(scenes.random_soup(1000, 0.05, seed=6), 5, axis=0),
+    "chain": geometric_chain,  # deeper than a builder's first batch of levels
 }
 
 
@@ -131,7 +132,7 @@ def test_collapse_matches_oracle(scene, kind):
 
 
 @pytest.mark.parametrize("scene", ["teapot", "torus", "soup", "sphere", "n3", "n5", "n100",
-                                   "n5000_dups"])
+                                   "n5000_dups", "chain"])
 @pytest.mark.parametrize("ppl", [None, 4])
 def test_binned_matches_oracle(scene, ppl):
     print(f"[binned] scene={scene} ppl={ppl}", flush=True)
@@ -146,6 +147,8 @@ def test_binned_matches_oracle(scene, ppl):
     assert code == 0 and viol == 0
     ref_n, _ = ref.build_binned_sah(aabbs, centers, prims_per_leaf=ppl or 1)
     assert ref.sah_cost(got_n) <= 1.05 * ref.sah_cost(ref_n)
+    if scene == "chain":  # the levels past the first batch were built too
+        assert tree_depth(got_n) > 2 * int(np.ceil(np.log2(len(verts)))) + 2
 
 
 def test_gpu_trees_trace_brute_force():

@@ -17,7 +17,8 @@ import pytest
 
 from oracle import refcpu as ref
 from rtbvh_amd import scenes
-from tests.helpers import as_rays, flat, flat_grid, flat_grid_rays, hits_equal, prep, ray_set
+from tests.helpers import (as_rays, flat, flat_grid, flat_grid_rays, geometric_chain, hits_equal,
+                           prep, ray_set, tree_depth)
 
 torch = pytest.importorskip("torch")
 import rtbvh_amd as rt  # noqa: E402
@@ -191,6 +192,19 @@ def test_spatial_duplicates_degenerates_and_flat():
     _check_structure(bvh, len(flatv))
     want = _hits_all_paths(bvh, flatv, flat_grid_rays(flatv, 4000))
     assert (want["prim"] != 0xFFFFFFFF).sum() > 400
+
+
+@pytest.mark.parametrize("ppl", [None, 4])
+def test_spatial_deep_chain(ppl):
+    """A tree deeper than the first batch of levels the builder enqueues
+    (2 lg n + 2) before it reads a level record: the later batches take
+    their grids from the record read last."""
+    print(f"[spatial_chain] ppl={ppl}", flush=True)
+    verts = geometric_chain()
+    bvh = _build(verts, "spatial", ppl=ppl)
+    nodes, _ = _check_structure(bvh, len(verts))
+    assert tree_depth(nodes) > 2 * int(np.ceil(np.log2(len(verts)))) + 2
+    _hits_all_paths(bvh, verts, ray_set(verts, 4000, seed=91, kind="box"))
 
 
 def test_spatial_errors():
