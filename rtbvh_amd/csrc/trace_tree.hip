@@ -18,6 +18,7 @@
 // recorded into a graph.
 #include <vector>
 
+#include "block_scan.hpp"
 #include "internal.hpp"
 
 namespace rtbvh {
@@ -384,15 +385,23 @@ __device__ __forceinline__ uint32_t dp_decision(const uint32_t* dpd, uint32_t k,
 // or spreads further.  The slots are then placed so that slot s lies towards
 // the octant s (bit k of s set: the + side of axis k) of the node's centre,
 // greedily by alignment; a ray whose direction has signs O visits keys
-// s ^ O in increasing order, the near side first.
-__device__ void make_node8(const LevelArgs& a, uint32_t b, uint32_t w, QItem* qout,
+// s ^ O in increasing order, the near side first.  Called by all 64 lanes
+// of a wave (`live` false: no item): the wide-node, triangle and queue
+// counters take one atomic per wave, the lanes' offsets from wave scans.
+__device__ void make_node8(const LevelArgs& a, bool live, uint32_t b, uint32_t w, QItem* qout,
                            uint32_t* qout_count) {
     Cand c[kWide];
     int ns = 0;
+    bool ok = live;
+    if (ok && (b >= a.node_count || w >= a.wcap)) {
+        atomicOr(&a.counters[2], 1u);
+        ok = false;
+    }
     DevAabb bn;
-    ldn(a.nodes + b, bn);
-    bool ok = true;
-    if (bn.extra1 >= 0) {  // a root that is a leaf, or a leaf of several primitives
+    if (ok) ldn(a.nodes + b, bn);
+    if (!ok) {
+        // nothing to gather
+    } else if (bn.extra1 >= 0) {  // a root that is a leaf, or a leaf of several primitives
         if (bn.extra1 > kWide || bn.extra2 < 0) ok = false;
         for (int32_t j = 0; ok && j < bn.extra1; ++j)
             ok = cand_prim(a, uint32_t(bn.extra2 + j), c[ns++], bn);
@@ -440,31 +449,29 @@ __device__ void make_node8(const LevelArgs& a, uint32_t b, uint32_t w, QItem* qo
             }
         }
     }
-    if (!ok || ns == 0) {
-        atomicOr(&a.counters[2], 1u);
-        return;
-    }
+    if (live && (!ok || ns == 0)) atomicOr(&a.counters[2], 1u);
+    ok = ok && ns > 0;
     // bounds of the node and the centre of every candidate
-    float lo[3], hi[3];
-    for (int k = 0; k < 3; ++k) {
-        lo[k] = c[0].lo[k];
-        hi[k] = c[0].hi[k];
-    }
-    for (int j = 1; j < ns; ++j)
+    float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};
+    if (ok) {
         for (int k = 0; k < 3; ++k) {
-            lo[k] = fminf(lo[k], c[j].lo[k]);
-            hi[k] = fmaxf(hi[k], c[j].hi[k]);
+            lo[k] = c[0].lo[k];
+            hi[k] = c[0].hi[k];
         }
-    for (int k = 0; k < 3; ++k)
-        if (!(isfinite(lo[k]) && isfinite(hi[k]) && lo[k] <= hi[k])) {
-            atomicOr(&a.counters[2], 2u);
-            return;
-        }
+        for (int j = 1; j < ns; ++j)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = fminf(lo[k], c[j].lo[k]);
+                hi[k] = fmaxf(hi[k], c[j].hi[k]);
+            }
+        for (int k = 0; k < 3; ++k)
+            if (!(isfinite(lo[k]) && isfinite(hi[k]) && lo[k] <= hi[k])) ok = false;
+        if (!ok) atomicOr(&a.counters[2], 2u);
+    }
     // octant placement
     int slot_cand[kWide];
     for (int s = 0; s < kWide; ++s) slot_cand[s] = -1;
     uint32_t placed = 0;
-    for (int it = 0; it < ns; ++it) {
+    for (int it = 0; ok && it < ns; ++it) {
         float best = -3.4e38f;
         int bk = -1, bs = -1;
         for (int k = 0; k < ns; ++k) {
@@ -487,26 +494,38 @@ __device__ void make_node8(const LevelArgs& a, uint32_t b, uint32_t w, QItem* qo
         placed |= 1u << bk;
     }
     // grid, children and triangles
-    float p[3];
-    uint32_t eb[3];
-    double step[3];
-    for (int k = 0; k < 3; ++k)
+    float p[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t eb[3] = {0u, 0u, 0u};
+    double step[3] = {1.0, 1.0, 1.0};
+    for (int k = 0; ok && k < 3; ++k)
         if (!grid_axis(lo[k], hi[k], p[k], eb[k], step[k])) {
             atomicOr(&a.counters[2], 2u);
-            return;
+            ok = false;
         }
     double inv_step[3];
     for (int k = 0; k < 3; ++k) inv_step[k] = 1.0 / step[k];
     uint32_t n_inner = 0, n_tris = 0;
-    for (int j = 0; j < ns; ++j) {
+    for (int j = 0; ok && j < ns; ++j) {
         if (c[j].kind == 2)
             ++n_tris;
         else
             ++n_inner;
     }
-    const uint32_t wbase = n_inner ? atomicAdd(&a.counters[0], n_inner) : 0u;
-    const uint32_t tbase = n_tris ? atomicAdd(&a.counters[1], n_tris) : 0u;
-    const uint32_t qbase = n_inner ? atomicAdd(qout_count, n_inner) : 0u;
+    // offsets: wave scans of the lanes' counts, one atomic per counter and
+    // wave (the last lane's inclusive sums are the wave's totals)
+    const uint32_t inc_i = wave_inclusive(n_inner), inc_t = wave_inclusive(n_tris);
+    uint32_t wb = 0, tb = 0, qb = 0;
+    if ((threadIdx.x & 63u) == 63u) {
+        wb = inc_i ? atomicAdd(&a.counters[0], inc_i) : 0u;
+        tb = inc_t ? atomicAdd(&a.counters[1], inc_t) : 0u;
+        qb = inc_i ? atomicAdd(qout_count, inc_i) : 0u;
+    }
+    wb = __shfl(wb, 63, 64);
+    tb = __shfl(tb, 63, 64);
+    qb = __shfl(qb, 63, 64);
+    if (!ok) return;
+    const uint32_t wbase = wb + inc_i - n_inner, tbase = tb + inc_t - n_tris;
+    const uint32_t qbase = qb + inc_i - n_inner;
     if (wbase + n_inner > a.wcap || tbase + n_tris > a.index_count) {
         atomicOr(&a.counters[2], 1u);
         return;
@@ -709,24 +728,34 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// clear: the queue count two levels ahead (the level before's input, read
+// by nobody now) is zeroed for the level after this one to count into.
 __global__ void __launch_bounds__(kBlock)
     level8_kernel(LevelArgs a, const QItem* qin, const uint32_t* qin_count, QItem* qout,
-                  uint32_t* qout_count) {
+                  uint32_t* qout_count, uint32_t* clear, uint32_t level) {
     const uint32_t q = *qin_count;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < q; i += gridDim.x * kBlock) {
-        const QItem it = qin[i];
-        if (it.b >= a.node_count || it.w >= a.wcap) {
-            atomicOr(&a.counters[2], 1u);
-            continue;
-        }
-        make_node8(a, it.b, it.w, qout, qout_count);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *clear = 0;
+        if (q > 0) atomicMax(&a.counters[4], level + 1);  // levels with work
+    }
+    // whole blocks iterate together: make_node8 needs every lane of a wave
+    for (uint32_t base = blockIdx.x * kBlock; base < q; base += gridDim.x * kBlock) {
+        const uint32_t i = base + threadIdx.x;
+        QItem it;
+        it.b = it.w = 0;
+        if (i < q) it = qin[i];
+        make_node8(a, i < q, it.b, it.w, qout, qout_count);
     }
 }
 
 __global__ void __launch_bounds__(kBlock)
     level_kernel(LevelArgs a, const QItem* qin, const uint32_t* qin_count, QItem* qout,
-                 uint32_t* qout_count) {
+                 uint32_t* qout_count, uint32_t* clear, uint32_t level) {
     const uint32_t q = *qin_count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *clear = 0;
+        if (q > 0) atomicMax(&a.counters[4], level + 1);  // levels with work
+    }
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < q; i += gridDim.x * kBlock) {
         const QItem it = qin[i];
         if (it.b >= a.node_count || it.w >= a.wcap) {
@@ -738,13 +767,17 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Roots [0, roots) become wide nodes [0, roots); counters start after them.
-__global__ void roots_kernel(uint32_t roots, QItem* q, uint32_t* q_count, uint32_t* counters) {
+// q_count: level 0's input count, q_next: level 0's output count.
+__global__ void roots_kernel(uint32_t roots, QItem* q, uint32_t* q_count, uint32_t* q_next,
+                             uint32_t* counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         *q_count = roots;
+        *q_next = 0;
         counters[0] = roots;
         counters[1] = 0;
         counters[2] = 0;
+        counters[4] = 0;
     }
     if (i < roots) {
         q[i].b = i;
@@ -752,8 +785,12 @@ __global__ void roots_kernel(uint32_t roots, QItem* q, uint32_t* q_count, uint32
     }
 }
 
-__global__ void clear_word_kernel(uint32_t* w) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *w = 0;
+// Queue counts rotate over three words (level L counts its input in
+// qword(L) and its output in qword(L + 1), and clears qword(L + 2)):
+// words[0], words[1] and words[5]; words[2..4] are the counters (wide
+// nodes, triangles, failure) and words[6] the number of levels with work.
+__host__ __device__ constexpr uint32_t qword(uint32_t level) {
+    return level % 3 == 2 ? 5u : level % 3;
 }
 
 // fail when the last level still queued work (the level budget ran out) or
@@ -772,7 +809,7 @@ struct TraceTreePlan {
     uint32_t bin_levels = 0;    // width 8: binary-tree depth seen by the eager build
     uint32_t* cnt = nullptr;
     QItem* q[2] = {nullptr, nullptr};
-    uint32_t* words = nullptr;  // [0..1] queue counts, [2..4] counters
+    uint32_t* words = nullptr;  // [0, 1, 5] queue counts (qword), [2..4] counters, [6] depth
     // width 8: breadth-first binary order, its level offsets and counts, and
     // the distribution costs (8 per node) and decisions (8 nibbles per node)
     uint32_t* bord = nullptr;
@@ -830,7 +867,7 @@ hipError_t enqueue_setup(const TraceTreePlan* t, const DeviceBvh& tree, hipStrea
                            t->blev, t->blev + kMaxBinLevels + 2, count);
     const uint32_t gr = (t->roots + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(roots_kernel, dim3(gr), dim3(kBlock), 0, stream, t->roots, t->q[0],
-                       t->words + 0, t->words + 2);
+                       t->words + qword(0), t->words + qword(1), t->words + 2);
     return hipGetLastError();
 }
 
@@ -857,14 +894,16 @@ __global__ void bin_depth_check_kernel(const uint32_t* cnt, uint32_t levels, uin
 
 void enqueue_level(const TraceTreePlan* t, const LevelArgs& a, uint32_t level, uint32_t grid,
                    hipStream_t stream) {
-    const int in = level & 1, outq = in ^ 1;
-    hipLaunchKernelGGL(clear_word_kernel, dim3(1), dim3(1), 0, stream, t->words + outq);
+    const int in = level & 1, outq = in ^ 1;  // queue buffers alternate
+    uint32_t* w = t->words;
     if (t->width == 8)
         hipLaunchKernelGGL(level8_kernel, dim3(grid), dim3(kBlock), 0, stream, a, t->q[in],
-                           t->words + in, t->q[outq], t->words + outq);
+                           w + qword(level), t->q[outq], w + qword(level + 1),
+                           w + qword(level + 2), level);
     else
         hipLaunchKernelGGL(level_kernel, dim3(grid), dim3(kBlock), 0, stream, a, t->q[in],
-                           t->words + in, t->q[outq], t->words + outq);
+                           w + qword(level), t->q[outq], w + qword(level + 1),
+                           w + qword(level + 2), level);
 }
 
 }  // namespace
@@ -952,11 +991,16 @@ ResultCode trace_tree_build(TraceTreePlan* t, const DeviceBvh& tree, const float
     hipError_t e = enqueue_setup(t, tree, stream);
     if (e != hipSuccess) return hip_fail(e, "trace tree setup");
     const LevelArgs a = level_args(t, tree, verts, prim_count, scene_off, scenes, out);
+    uint32_t first_wide = 4;  // wide levels in the first batch
     if (t->width == 8) {  // binary levels until one is empty, then the distribution
+        // first batch: 3 lg n + 4 levels (a PLOC tree of 1M is about 61
+        // deep), then 24 at a time; levels past the last return at once
+        uint32_t lg = 0;
+        while (lg < 31 && (1u << lg) < t->count) ++lg;
+        uint32_t chunk = 3u * lg + 4u;
         uint32_t bl = 0;
         std::vector<uint32_t> counts;
-        for (;;) {
-            const uint32_t chunk = 24u;
+        for (;; chunk = 24u) {
             if (bl + chunk > kMaxBinLevels) {
                 set_error("binary tree deeper than %u levels", kMaxBinLevels);
                 return Error;
@@ -973,23 +1017,24 @@ ResultCode trace_tree_build(TraceTreePlan* t, const DeviceBvh& tree, const float
         while (depth > 0 && counts[depth - 1] == 0) --depth;
         t->bin_levels = depth;
         enqueue_dp_levels(t, a, depth, stream);
+        // a compressed node spans about three binary levels
+        first_wide = depth / 3 + 2 > 4 ? depth / 3 + 2 : 4;
     }
     uint32_t level = 0;
     uint32_t host_words[8];
-    constexpr uint32_t kChunk = 4;
-    for (;;) {
-        for (uint32_t k = 0; k < kChunk; ++k, ++level) enqueue_level(t, a, level, kLevelGrid, stream);
+    for (uint32_t chunk = first_wide;; chunk = 4) {
+        for (uint32_t k = 0; k < chunk; ++k, ++level) enqueue_level(t, a, level, kLevelGrid, stream);
         if ((e = hipMemcpyAsync(host_words, t->words, sizeof(host_words), hipMemcpyDeviceToHost,
                                 stream)) != hipSuccess ||
             (e = hipStreamSynchronize(stream)) != hipSuccess)
             return hip_fail(e, "trace tree level");
-        if (host_words[level & 1] == 0) break;  // nothing queued for the next level
+        if (host_words[qword(level)] == 0) break;  // nothing queued for the next level
         if (level > 4 * 4096) {
             set_error("trace tree deeper than expected");
             return Error;
         }
     }
-    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1), 0, stream, t->words + (level & 1),
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1), 0, stream, t->words + qword(level),
                        t->words + 2, t->n, nullptr);
     if ((e = hipMemcpyAsync(host_words, t->words, sizeof(host_words), hipMemcpyDeviceToHost,
                             stream)) != hipSuccess ||
@@ -1000,7 +1045,7 @@ ResultCode trace_tree_build(TraceTreePlan* t, const DeviceBvh& tree, const float
                                      : "trace tree construction failed (inconsistent binary tree?)");
         return Error;
     }
-    t->levels = level;
+    t->levels = host_words[6];  // the levels with work (the batches enqueue more)
     out->node_count = host_words[2];
     return Ok;
 }
@@ -1023,7 +1068,7 @@ hipError_t trace_tree_enqueue(const TraceTreePlan* t, const DeviceBvh& tree, con
         enqueue_dp_levels(t, a, bl, stream);
     }
     for (uint32_t level = 0; level < levels; ++level) enqueue_level(t, a, level, kLevelGrid, stream);
-    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1), 0, stream, t->words + (levels & 1),
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1), 0, stream, t->words + qword(levels),
                        t->words + 2, t->n, fail);
     return hipGetLastError();
 }
