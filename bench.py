@@ -532,10 +532,24 @@ def run_c1(args, dev, rt, scenes, torch, stream):
 
 
 def run_build_only(kind, rt, torch, a_d, c_d, v_d, reps):
-    """Timed binned-SAH or spatial-split builds of the 1M scene, each from a
-    synchronised device after the previous tree was released."""
+    """Timed binned-SAH, spatial-split or compressed traversal-tree builds of
+    the 1M scene (the last over one PLOC tree), each from a synchronised
+    device after the previous tree was released."""
     log(f"{kind} builds")
     ms, tree = [], None
+    if kind == "tree":
+        bvh = rt.Builder(c_d, aabbs=a_d).construct_locally_ordered_clustered()
+        for i in range(reps):
+            tree = None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            tree = bvh.trace_tree(v_d)
+            torch.cuda.synchronize()
+            if i > 0:
+                ms.append((time.perf_counter() - t0) * 1e3)
+        log(f"build only: tree {statistics.median(ms):.3f} ms (median of {len(ms)}), "
+            f"{tree.node_count} nodes")
+        return
     for i in range(reps):
         tree = None
         torch.cuda.synchronize()
@@ -577,7 +591,7 @@ def main():
     ap.add_argument("--adjoint", type=int, default=1,
                     help="1: also time forward + adjoint backward on 100k triangles (0 skips it)")
     ap.add_argument("--build-only", nargs="?", const="ploc", default=None,
-                    choices=("ploc", "binned", "spatial"),
+                    choices=("ploc", "binned", "spatial", "tree"),
                     help="only the timed builds of the 1M scene with one builder (default PLOC), "
                          "for profiler runs; no JSON line")
     ap.add_argument("--trace-only", action="store_true",
@@ -615,7 +629,7 @@ def main():
     c_d = torch.from_numpy(centers).to(dev)
     n_tris = len(verts)
 
-    if args.build_only in ("binned", "spatial"):
+    if args.build_only in ("binned", "spatial", "tree"):
         run_build_only(args.build_only, rt, torch, a_d, c_d, v_d, 2 if args.quick else 6)
         return
 
