@@ -467,22 +467,35 @@ __device__ void make_node8(const LevelArgs& a, bool live, uint32_t b, uint32_t w
             if (!(isfinite(lo[k]) && isfinite(hi[k]) && lo[k] <= hi[k])) ok = false;
         if (!ok) atomicOr(&a.counters[2], 2u);
     }
-    // octant placement
+    // octant placement: each round takes the first (candidate, free slot)
+    // pair of the highest alignment score.  The candidates' centre offsets
+    // are read once into registers and every loop over candidates and slots
+    // is unrolled, so the rounds index no array at run time (the candidate
+    // array itself sits in scratch memory).
+    float off0[kWide], off1[kWide], off2[kWide];
+#pragma unroll
+    for (int k = 0; k < kWide; ++k) {
+        const bool v = ok && k < ns;
+        off0[k] = v ? (c[k].lo[0] + c[k].hi[0]) - (lo[0] + hi[0]) : 0.0f;
+        off1[k] = v ? (c[k].lo[1] + c[k].hi[1]) - (lo[1] + hi[1]) : 0.0f;
+        off2[k] = v ? (c[k].lo[2] + c[k].hi[2]) - (lo[2] + hi[2]) : 0.0f;
+    }
     int slot_cand[kWide];
+#pragma unroll
     for (int s = 0; s < kWide; ++s) slot_cand[s] = -1;
-    uint32_t placed = 0;
+    uint32_t placed = 0, taken = 0;  // candidates placed, slots taken
     for (int it = 0; ok && it < ns; ++it) {
         float best = -3.4e38f;
         int bk = -1, bs = -1;
-        for (int k = 0; k < ns; ++k) {
-            if (placed & (1u << k)) continue;
-            float off[3];
-            for (int ax = 0; ax < 3; ++ax)
-                off[ax] = (c[k].lo[ax] + c[k].hi[ax]) - (lo[ax] + hi[ax]);
+#pragma unroll
+        for (int k = 0; k < kWide; ++k) {
+            if (k >= ns || (placed & (1u << k))) continue;
+#pragma unroll
             for (int s = 0; s < kWide; ++s) {
-                if (slot_cand[s] >= 0) continue;
-                const float sc = ((s & 1) ? off[0] : -off[0]) + ((s & 2) ? off[1] : -off[1]) +
-                                 ((s & 4) ? off[2] : -off[2]);
+                if (taken & (1u << s)) continue;
+                const float sc = ((s & 1) ? off0[k] : -off0[k]) +
+                                 ((s & 2) ? off1[k] : -off1[k]) +
+                                 ((s & 4) ? off2[k] : -off2[k]);
                 if (bk < 0 || sc > best) {
                     best = sc;
                     bk = k;
@@ -490,8 +503,11 @@ __device__ void make_node8(const LevelArgs& a, bool live, uint32_t b, uint32_t w
                 }
             }
         }
-        slot_cand[bs] = bk;
+#pragma unroll
+        for (int s = 0; s < kWide; ++s)
+            if (s == bs) slot_cand[s] = bk;
         placed |= 1u << bk;
+        taken |= 1u << bs;
     }
     // grid, children and triangles
     float p[3] = {0.0f, 0.0f, 0.0f};
