@@ -26,6 +26,21 @@ namespace rtbvh {
 
 namespace {
 
+// A hit record (t, u, v, primitive bits) written with a non-temporal store:
+// each record is written once and never read by the launch, so it should not
+// take L2 lines from the nodes and triangles (0.7 % faster bounce traces at
+// 1M in an A/B on one box).
+typedef float HitVec __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_hit(DevHit* hits, uint64_t rid, float t, float u, float v,
+                                          uint32_t prim) {
+    HitVec h;
+    h.x = t;
+    h.y = u;
+    h.z = v;
+    h.w = __uint_as_float(prim);
+    __builtin_nontemporal_store(h, reinterpret_cast<HitVec*>(hits) + rid);
+}
+
 constexpr int kMaxBlock = 256;
 
 // The stack's first `depth` entries live in LDS (entry k of lane l at
@@ -163,8 +178,7 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_fast_kernel(TraceParams p) {
                 if (ANY) {
                     p.occluded[rid] = 0;
                 } else {
-                    reinterpret_cast<float4*>(p.hits)[rid] =
-                        make_float4(bt, 0.f, 0.f, __uint_as_float(RTBVH_MISS));
+                    store_hit(p.hits, rid, bt, 0.f, 0.f, RTBVH_MISS);
                 }
                 rid += stride;
                 continue;
@@ -283,9 +297,8 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_fast_kernel(TraceParams p) {
             if (ANY) {
                 p.occluded[rid] = (bp != RTBVH_MISS) ? 1 : 0;
             } else {
-                reinterpret_cast<float4*>(p.hits)[rid] =
-                    make_float4(overflow ? __builtin_nanf("") : bt, bu, bv,
-                                __uint_as_float(overflow ? RTBVH_MISS : bp));
+                store_hit(p.hits, rid, overflow ? __builtin_nanf("") : bt, bu, bv,
+                          overflow ? RTBVH_MISS : bp);
             }
             active = false;
             cur = -1;
@@ -457,8 +470,7 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
                 if (ANY) {
                     p.occluded[rid] = 0;
                 } else {
-                    reinterpret_cast<float4*>(p.hits)[rid] =
-                        make_float4(bt, 0.f, 0.f, __uint_as_float(RTBVH_MISS));
+                    store_hit(p.hits, rid, bt, 0.f, 0.f, RTBVH_MISS);
                 }
                 sched.next();
             } else {
@@ -609,9 +621,8 @@ __global__ void __launch_bounds__(kMaxBlock) mbvh_item_kernel(TraceParams p) {
             if (ANY) {
                 p.occluded[rid] = (bp != RTBVH_MISS) ? 1 : 0;
             } else {
-                reinterpret_cast<float4*>(p.hits)[rid] =
-                    make_float4(overflow ? __builtin_nanf("") : bt, bu, bv,
-                                __uint_as_float(overflow ? RTBVH_MISS : bp));
+                store_hit(p.hits, rid, overflow ? __builtin_nanf("") : bt, bu, bv,
+                          overflow ? RTBVH_MISS : bp);
             }
             active = false;
             node = -1;
@@ -780,8 +791,7 @@ cw8_kernel(TraceParams p) {
                 if (ANY) {
                     p.occluded[rid] = 0;
                 } else {
-                    reinterpret_cast<float4*>(p.hits)[rid] =
-                        make_float4(bt, 0.f, 0.f, __uint_as_float(RTBVH_MISS));
+                    store_hit(p.hits, rid, bt, 0.f, 0.f, RTBVH_MISS);
                 }
                 sched.next();
             } else {
@@ -952,9 +962,8 @@ cw8_kernel(TraceParams p) {
             if (ANY) {
                 p.occluded[rid] = (bp != RTBVH_MISS) ? 1 : 0;
             } else {
-                reinterpret_cast<float4*>(p.hits)[rid] =
-                    make_float4(overflow ? __builtin_nanf("") : bt, bu, bv,
-                                __uint_as_float(overflow ? RTBVH_MISS : bp));
+                store_hit(p.hits, rid, overflow ? __builtin_nanf("") : bt, bu, bv,
+                          overflow ? RTBVH_MISS : bp);
             }
             active = false;
             ngy = 0;
