@@ -34,6 +34,20 @@
 // of the vertices' magnitude (the rounding of the crossings) and kept inside
 // the reference's box, so the references of a triangle always cover it and
 // every hit is found.
+//
+// The level loop runs on the device: each level's counts (tasks, references,
+// node counter, leaf base, duplicates so far) sit in a 64-byte record that
+// level_end_kernel writes from the level before's scan totals, every kernel
+// reads its counts there and returns at once past them, and the host
+// enqueues batches of levels with grids from bounds, reading one record per
+// batch.  A level: node boxes (ref_bounds_kernel, task_setup_kernel),
+// object bins and split (obj_bin_kernel, obj_split_kernel), a scan of the
+// spatial candidates, their bins and splits (cand_clear_kernel,
+// sp_bin_kernel, sp_split_kernel), a scan of the duplicates, the decisions
+// (decide_kernel), the side flags (side_kernel), one scan of the task
+// counts and side flags, then nodes and child tasks (children_kernel), the
+// references' scatter (scatter_kernel, which also clears the next level's
+// boxes and bins) and the next record.
 #include <algorithm>
 
 #include "internal.hpp"
