@@ -287,7 +287,8 @@ struct Cand {
 // box.  The cut changes nothing where the leaf box holds the whole triangle
 // (every tree without spatial splits); after a spatial split the leaf holds
 // the part of the triangle inside it, and the cut keeps the slot as tight.
-__device__ bool cand_prim(const LevelArgs& a, uint32_t pos, Cand& c, const DevAabb& leaf) {
+__device__ __forceinline__ bool cand_prim(const LevelArgs& a, uint32_t pos, Cand& c,
+                                          const DevAabb& leaf) {
     if (pos >= a.index_count) return false;
     c.id = int32_t(pos);
     c.kind = 2;
@@ -332,6 +333,61 @@ __device__ bool cand_binary(const LevelArgs& a, int32_t i, Cand* c, int& ns) {
     for (int k = 0; k < 3; ++k) {
         d.lo[k] = n.min[k];
         d.hi[k] = n.max[k];
+    }
+    return n.extra2 >= 0;
+}
+
+// The candidates of one compressed node held in registers: an append
+// shifts every entry down one place and writes the new one at the end, so
+// after ns appends they sit at [kWide - ns, kWide) in order, and every write
+// is by a compile-time index (an access by a run-time index would place
+// the arrays in scratch memory).
+struct CandRegs {
+    int32_t id[kWide], kind[kWide];
+    uint32_t n[kWide];
+    float lo0[kWide], lo1[kWide], lo2[kWide], hi0[kWide], hi1[kWide], hi2[kWide];
+
+    __device__ __forceinline__ void push(const Cand& t) {
+#pragma unroll
+        for (int q = 0; q + 1 < kWide; ++q) {
+            id[q] = id[q + 1];
+            kind[q] = kind[q + 1];
+            n[q] = n[q + 1];
+            lo0[q] = lo0[q + 1];
+            lo1[q] = lo1[q + 1];
+            lo2[q] = lo2[q + 1];
+            hi0[q] = hi0[q + 1];
+            hi1[q] = hi1[q + 1];
+            hi2[q] = hi2[q + 1];
+        }
+        id[kWide - 1] = t.id;
+        kind[kWide - 1] = t.kind;
+        n[kWide - 1] = t.n;
+        lo0[kWide - 1] = t.lo[0];
+        lo1[kWide - 1] = t.lo[1];
+        lo2[kWide - 1] = t.lo[2];
+        hi0[kWide - 1] = t.hi[0];
+        hi1[kWide - 1] = t.hi[1];
+        hi2[kWide - 1] = t.hi[2];
+    }
+};
+
+// Binary node i as one candidate t (add: whether it gives one; an empty
+// leaf gives none): cand_binary for a caller that stores the candidate.
+__device__ __forceinline__ bool cand_binary1(const LevelArgs& a, int32_t i, Cand& t, bool& add) {
+    add = false;
+    if (i < 0 || uint32_t(i) >= a.node_count) return false;
+    DevAabb n;
+    ldn(a.nodes + i, n);
+    if (n.extra1 == 0) return true;
+    add = true;
+    if (n.extra1 == 1) return cand_prim(a, uint32_t(n.extra2), t, n);
+    t.id = i;
+    t.kind = n.extra1 > 1 ? 1 : 0;
+    t.n = n.extra1 > 1 ? uint32_t(n.extra1) : 0u;
+    for (int k = 0; k < 3; ++k) {
+        t.lo[k] = n.min[k];
+        t.hi[k] = n.max[k];
     }
     return n.extra2 >= 0;
 }
@@ -390,8 +446,12 @@ __device__ __forceinline__ uint32_t dp_decision(const uint32_t* dpd, uint32_t k,
 // counters take one atomic per wave, the lanes' offsets from wave scans.
 __device__ void make_node8(const LevelArgs& a, bool live, uint32_t b, uint32_t w, QItem* qout,
                            uint32_t* qout_count) {
-    Cand c[kWide];
+    CandRegs cr;
     int ns = 0;
+    const auto append = [&](const Cand& t) {
+        cr.push(t);
+        ++ns;
+    };
     bool ok = live;
     if (ok && (b >= a.node_count || w >= a.wcap)) {
         atomicOr(&a.counters[2], 1u);
@@ -404,7 +464,11 @@ __device__ void make_node8(const LevelArgs& a, bool live, uint32_t b, uint32_t w
     } else if (bn.extra1 >= 0) {  // a root that is a leaf, or a leaf of several primitives
         if (bn.extra1 > kWide || bn.extra2 < 0) ok = false;
         for (int32_t j = 0; ok && j < bn.extra1; ++j)
-            ok = cand_prim(a, uint32_t(bn.extra2 + j), c[ns++], bn);
+        {
+            Cand t;
+            ok = cand_prim(a, uint32_t(bn.extra2 + j), t, bn);
+            append(t);
+        }
     } else {
         // (binary node, slots) pairs still to place; at most 8 are pending
         uint32_t st_node[kWide], st_slots[kWide];
@@ -433,10 +497,21 @@ __device__ void make_node8(const LevelArgs& a, bool live, uint32_t b, uint32_t w
                     ok = false;
                     break;
                 }
-                for (int32_t q = 0; ok && q < nd.extra1; ++q)
-                    ok = ns < kWide && cand_prim(a, uint32_t(nd.extra2 + q), c[ns++], nd);
+                for (int32_t q = 0; ok && q < nd.extra1; ++q) {
+                    Cand t;
+                    ok = ns < kWide && cand_prim(a, uint32_t(nd.extra2 + q), t, nd);
+                    if (ok) append(t);
+                }
             } else if (nd.extra1 >= 0 || d == 0u) {  // one slot
-                ok = cand_binary(a, int32_t(n), c, ns);
+                Cand t;
+                bool add = false;
+                ok = cand_binary1(a, int32_t(n), t, add);
+                if (ok && add) {
+                    if (ns >= kWide)
+                        ok = false;
+                    else
+                        append(t);
+                }
             } else {  // spread over j slots: k to the left child, j - k to the right
                 if (nd.extra2 < 0 || d >= j || sp + 2 > kWide) {
                     ok = false;
@@ -454,42 +529,47 @@ __device__ void make_node8(const LevelArgs& a, bool live, uint32_t b, uint32_t w
     // bounds of the node and the centre of every candidate
     float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};
     if (ok) {
+        // the candidates sit at [kWide - ns, kWide); fminf / fmaxf skip NaN as
+        // a fold from the first candidate would
         for (int k = 0; k < 3; ++k) {
-            lo[k] = c[0].lo[k];
-            hi[k] = c[0].hi[k];
+            lo[k] = __builtin_inff();
+            hi[k] = -__builtin_inff();
         }
-        for (int j = 1; j < ns; ++j)
-            for (int k = 0; k < 3; ++k) {
-                lo[k] = fminf(lo[k], c[j].lo[k]);
-                hi[k] = fmaxf(hi[k], c[j].hi[k]);
+#pragma unroll
+        for (int j = 0; j < kWide; ++j)
+            if (j >= kWide - ns) {
+                lo[0] = fminf(lo[0], cr.lo0[j]);
+                lo[1] = fminf(lo[1], cr.lo1[j]);
+                lo[2] = fminf(lo[2], cr.lo2[j]);
+                hi[0] = fmaxf(hi[0], cr.hi0[j]);
+                hi[1] = fmaxf(hi[1], cr.hi1[j]);
+                hi[2] = fmaxf(hi[2], cr.hi2[j]);
             }
         for (int k = 0; k < 3; ++k)
             if (!(isfinite(lo[k]) && isfinite(hi[k]) && lo[k] <= hi[k])) ok = false;
         if (!ok) atomicOr(&a.counters[2], 2u);
     }
     // octant placement: each round takes the first (candidate, free slot)
-    // pair of the highest alignment score.  The candidates' centre offsets
-    // are read once into registers and every loop over candidates and slots
-    // is unrolled, so the rounds index no array at run time (the candidate
-    // array itself sits in scratch memory).
+    // pair of the highest alignment score.  Every loop over candidates and
+    // slots is unrolled, so nothing is indexed at run time.
     float off0[kWide], off1[kWide], off2[kWide];
 #pragma unroll
     for (int k = 0; k < kWide; ++k) {
-        const bool v = ok && k < ns;
-        off0[k] = v ? (c[k].lo[0] + c[k].hi[0]) - (lo[0] + hi[0]) : 0.0f;
-        off1[k] = v ? (c[k].lo[1] + c[k].hi[1]) - (lo[1] + hi[1]) : 0.0f;
-        off2[k] = v ? (c[k].lo[2] + c[k].hi[2]) - (lo[2] + hi[2]) : 0.0f;
+        const bool v = ok && k >= kWide - ns;
+        off0[k] = v ? (cr.lo0[k] + cr.hi0[k]) - (lo[0] + hi[0]) : 0.0f;
+        off1[k] = v ? (cr.lo1[k] + cr.hi1[k]) - (lo[1] + hi[1]) : 0.0f;
+        off2[k] = v ? (cr.lo2[k] + cr.hi2[k]) - (lo[2] + hi[2]) : 0.0f;
     }
-    int slot_cand[kWide];
+    uint32_t cand_slot[kWide];  // the slot of each candidate
 #pragma unroll
-    for (int s = 0; s < kWide; ++s) slot_cand[s] = -1;
+    for (int k = 0; k < kWide; ++k) cand_slot[k] = 0;
     uint32_t placed = 0, taken = 0;  // candidates placed, slots taken
     for (int it = 0; ok && it < ns; ++it) {
         float best = -3.4e38f;
         int bk = -1, bs = -1;
 #pragma unroll
         for (int k = 0; k < kWide; ++k) {
-            if (k >= ns || (placed & (1u << k))) continue;
+            if (k < kWide - ns || (placed & (1u << k))) continue;
 #pragma unroll
             for (int s = 0; s < kWide; ++s) {
                 if (taken & (1u << s)) continue;
@@ -504,8 +584,8 @@ __device__ void make_node8(const LevelArgs& a, bool live, uint32_t b, uint32_t w
             }
         }
 #pragma unroll
-        for (int s = 0; s < kWide; ++s)
-            if (s == bs) slot_cand[s] = bk;
+        for (int k = 0; k < kWide; ++k)
+            if (k == bk) cand_slot[k] = uint32_t(bs);
         placed |= 1u << bk;
         taken |= 1u << bs;
     }
@@ -521,8 +601,10 @@ __device__ void make_node8(const LevelArgs& a, bool live, uint32_t b, uint32_t w
     double inv_step[3];
     for (int k = 0; k < 3; ++k) inv_step[k] = 1.0 / step[k];
     uint32_t n_inner = 0, n_tris = 0;
-    for (int j = 0; ok && j < ns; ++j) {
-        if (c[j].kind == 2)
+#pragma unroll
+    for (int j = 0; j < kWide; ++j) {
+        if (!(ok && j >= kWide - ns)) continue;
+        if (cr.kind[j] == 2)
             ++n_tris;
         else
             ++n_inner;
@@ -550,32 +632,47 @@ __device__ void make_node8(const LevelArgs& a, bool live, uint32_t b, uint32_t w
     m.px = p[0];
     m.py = p[1];
     m.pz = p[2];
+    // slot masks first: the triangles and the inner children are stored in
+    // slot order, each at its rank among the slots of its kind
     uint32_t imask = 0, lmask = 0;
+#pragma unroll
+    for (int k = 0; k < kWide; ++k) {
+        if (k < kWide - ns) continue;
+        if (cr.kind[k] == 2)
+            lmask |= 1u << cand_slot[k];
+        else
+            imask |= 1u << cand_slot[k];
+    }
     uint32_t qlo[3][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}};
     uint32_t qhi[3][2] = {{0u, 0u}, {0u, 0u}, {0u, 0u}};
-    uint32_t wi = wbase, ti = tbase, qi = qbase;
-    for (int s = 0; s < kWide; ++s) {
-        const int k = slot_cand[s];
-        const int word = s >> 2, sh = (s & 3) * 8;
-        if (k < 0) {  // empty: an inverted box, never entered
-            for (int ax = 0; ax < 3; ++ax) qlo[ax][word] |= 255u << sh;
-            continue;
-        }
+#pragma unroll
+    for (int s = 0; s < kWide; ++s)  // empty slots: an inverted box, never entered
+        if (!((imask | lmask) & (1u << s)))
+            for (int ax = 0; ax < 3; ++ax) qlo[ax][s >> 2] |= 255u << ((s & 3) * 8);
+#pragma unroll
+    for (int k = 0; k < kWide; ++k) {
+        if (k < kWide - ns) continue;
+        const uint32_t sl = cand_slot[k], sh = (sl & 3u) * 8u, below = (1u << sl) - 1u;
+        const float clo[3] = {cr.lo0[k], cr.lo1[k], cr.lo2[k]};
+        const float chi[3] = {cr.hi0[k], cr.hi1[k], cr.hi2[k]};
         for (int ax = 0; ax < 3; ++ax) {
-            qlo[ax][word] |= quant_lo(c[k].lo[ax], p[ax], inv_step[ax]) << sh;
-            qhi[ax][word] |= quant_hi(c[k].hi[ax], p[ax], inv_step[ax]) << sh;
+            const uint32_t ql = quant_lo(clo[ax], p[ax], inv_step[ax]) << sh;
+            const uint32_t qh = quant_hi(chi[ax], p[ax], inv_step[ax]) << sh;
+            if (sl >= 4u) {
+                qlo[ax][1] |= ql;
+                qhi[ax][1] |= qh;
+            } else {
+                qlo[ax][0] |= ql;
+                qhi[ax][0] |= qh;
+            }
         }
-        if (c[k].kind == 2) {
-            lmask |= 1u << s;
-            write_tri(a.tris, ti, a.verts, a.prim_count, a.indices[c[k].id], a.scene_off,
-                      a.scenes);
-            ++ti;
+        if (cr.kind[k] == 2) {
+            write_tri(a.tris, tbase + uint32_t(__popc(lmask & below)), a.verts, a.prim_count,
+                      a.indices[cr.id[k]], a.scene_off, a.scenes);
         } else {
-            imask |= 1u << s;
-            qout[qi].b = uint32_t(c[k].id);
-            qout[qi].w = wi;
-            ++wi;
-            ++qi;
+            const uint32_t r = uint32_t(__popc(imask & below));
+            qout[qbase + r].b = uint32_t(cr.id[k]);
+            qout[qbase + r].w = wbase + r;
         }
     }
     m.emask = eb[0] | (eb[1] << 8) | (eb[2] << 16) | (imask << 24);
