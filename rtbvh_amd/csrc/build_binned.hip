@@ -441,6 +441,82 @@ __device__ __forceinline__ GBox gbox_shfl(const GBox& b, int src) {
 
 __device__ __forceinline__ float gbox_half_area(const GBox& b) { return half_area6(b.mn, b.mx); }
 
+// decide() for a task of two primitives, restated for two bins without the
+// group's scans (every lane computes it; the values are those of the scans
+// bit for bit).  A primitive's bin box is its box, a NaN bound and one past
+// 1e34 in magnitude giving Aabb::empty's (dec_min / dec_max of a one-value
+// fold).  On an axis where the two bins differ, every plane between them
+// costs area(lower) + area(upper) and the first one, after the lower bin,
+// wins; where they share a bin no plane has both sides.  The fallback's
+// first bin whose prefix reaches 2/5 of two primitives (one) is the lower
+// bin on the longest axis, when it is below the last bin.
+__device__ __forceinline__ SDec decide_pair(const BTask& tk, uint32_t max_leaf,
+                                            const uint32_t* idx, const float* centers,
+                                            uint32_t stride, const DevAabb* aabbs) {
+    SDec d;
+    d.split = 0;
+    d.axis = 0;
+    d.split_index = 0;
+    d.left_count = 0;
+    uint32_t bin[2][3];
+    float mn[2][3], mx[2][3];
+    for (int q = 0; q < 2; ++q) {
+        const uint32_t p = idx[tk.begin + uint32_t(q)];
+        const float* c = centers + size_t(p) * stride;
+        const DevAabb a = aabbs[p];
+        for (int k = 0; k < 3; ++k) {
+            bin[q][k] = bin_index(c[k], tk.ctb[k], tk.bo[k]);
+            mn[q][k] = (a.min[k] == a.min[k] && a.min[k] < 1e34f) ? a.min[k] : 1e34f;
+            mx[q][k] = (a.max[k] == a.max[k] && a.max[k] > -1e34f) ? a.max[k] : -1e34f;
+        }
+    }
+    const float area0 = half_area6(mn[0], mx[0]), area1 = half_area6(mn[1], mx[1]);
+    float cost[3];
+    uint32_t cnt[3];
+    for (int k = 0; k < 3; ++k) {
+        cost[k] = 3.40282347e38f;
+        cnt[k] = kBins;
+        if (bin[0][k] != bin[1][k]) {
+            const bool first_low = bin[0][k] < bin[1][k];
+            const float c = first_low ? area0 * 1.0f + area1 * 1.0f : area1 * 1.0f + area0 * 1.0f;
+            if (c < 3.40282347e38f) {
+                cost[k] = c;
+                cnt[k] = (first_low ? bin[0][k] : bin[1][k]) + 1u;
+            }
+        }
+    }
+    uint32_t axis = 0;
+    if (cost[0] > cost[1]) axis = 1;
+    if (pick3(cost, axis) > cost[2]) axis = 2;
+    uint32_t split_index = pick3(cnt, axis);
+    const float max_split_cost = half_area6(tk.mn, tk.mx) * (2.0f - 1.0f);
+    if (pick3(cnt, axis) == uint32_t(kBins) || pick3(cost, axis) >= max_split_cost) {
+        if (2u <= max_leaf) return d;
+        // Aabb::longest_axis of the padded box, aabb.rs:354-363
+        const float e0 = tk.mx[0] - tk.mn[0], e1 = tk.mx[1] - tk.mn[1], e2 = tk.mx[2] - tk.mn[2];
+        axis = 0;
+        if (e1 > e0) axis = 1;
+        if (e2 > (axis == 1 ? e1 : e0)) axis = 2;
+        const uint32_t lowest = min(pick3(bin[0], axis), pick3(bin[1], axis));
+        if (lowest < uint32_t(kBins) - 1u) split_index = lowest + 1u;
+    }
+    const uint32_t lim = split_index < uint32_t(kBins) ? split_index : uint32_t(kBins);
+    const bool l0 = pick3(bin[0], axis) < lim, l1 = pick3(bin[1], axis) < lim;
+    if (l0 != l1) {  // one primitive on each side
+        d.split = 1;
+        d.axis = axis;
+        d.split_index = split_index;
+        d.left_count = 1;
+        for (int k = 0; k < 3; ++k) {
+            d.lmn[k] = l0 ? mn[0][k] : mn[1][k];
+            d.lmx[k] = l0 ? mx[0][k] : mx[1][k];
+            d.rmn[k] = l0 ? mn[1][k] : mn[0][k];
+            d.rmx[k] = l0 ? mx[1][k] : mx[0][k];
+        }
+    }
+    return d;
+}
+
 // The decision for one task by its group of 16 lanes (every lane returns
 // it): the best SAH plane per axis (find_split, binned_sah.rs:80-114), the
 // axis, the fallback and the child boxes (:150-230).  Lane i holds bin i:
@@ -464,6 +540,7 @@ __device__ __forceinline__ SDec decide(const BTask& tk, int lane, BinAcc* bins, 
     d.left_count = 0;
     const uint32_t work = tk.end - tk.begin;
     if (!(work > 1 && tk.depth < kMaxDepth)) return d;
+    if (work == 2) return decide_pair(tk, max_leaf, idx, centers, stride, aabbs);
     BinAcc own[3];  // this lane's bin of each axis
     if (tk.slot == kNoSlot) {
         // at most kSmallTask primitives: bin_kernel's accumulation without
