@@ -313,17 +313,29 @@ def run_large(args, dev, rt, scenes, torch):
     eager_build_ms = (time.perf_counter() - t0) * 1e3
     del bvh
     plan = rt.BatchPlan(centers, [0, n], verts, aabbs=aabbs)
-    frame_ms = []
-    for f in range(1, 5):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        scenes.animate_y_torch(base, f, verts)
-        scenes.triangle_aabbs_centers_torch(verts, aabbs, centers)
-        plan.launch(centers, verts, rays, None, aabbs=aabbs, any_hit=True, out=occ)
-        torch.cuda.synchronize()
-        if f > 1:  # the first launch records the graph
-            frame_ms.append((time.perf_counter() - t0) * 1e3)
-    plan.status()
+
+    def frames():
+        ms = []
+        for f in range(1, 5):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            scenes.animate_y_torch(base, f, verts)
+            scenes.triangle_aabbs_centers_torch(verts, aabbs, centers)
+            plan.launch(centers, verts, rays, None, aabbs=aabbs, any_hit=True, out=occ)
+            torch.cuda.synchronize()
+            if f > 1:  # the first launch records the graph
+                ms.append((time.perf_counter() - t0) * 1e3)
+        plan.status()
+        return ms
+
+    # the rays in the caller's (random) order, then grouped by the cells of
+    # their origins (sorted inside each frame's graph; same flags per ray)
+    caller_order_ms = frames()
+    occ_caller = occ.clone()
+    plan.order_rays(n_rays)
+    frame_ms = frames()
+    same_order = bool(torch.equal(occ, occ_caller))
+    del occ_caller
     occluded = float(occ.float().mean().item())
     # the last frame against an eager build + any-hit of the same vertices
     bvh = rt.Builder(centers, aabbs=aabbs).construct_locally_ordered_clustered()
@@ -337,11 +349,14 @@ def run_large(args, dev, rt, scenes, torch):
     torch.cuda.synchronize()
     trace_ms = (time.perf_counter() - t0) * 1e3
     fm = statistics.median(frame_ms)
+    fc = statistics.median(caller_order_ms)
     res = {
         "tris": n,
         "rays": n_rays,
         "frame_ms": round(fm, 2),
         "frame_mrays_s": round(n_rays / (fm * 1e-3) / 1e6, 1),
+        "frame_ms_caller_order": round(fc, 2),
+        "ordered_same_as_caller_order": same_order,
         "eager_ploc_build_ms": round(eager_build_ms, 2),
         "eager_mbvh_any_hit_ms": round(trace_ms, 2),
         "eager_mbvh_any_hit_mrays_s": round(n_rays / (trace_ms * 1e-3) / 1e6, 1),
@@ -350,10 +365,14 @@ def run_large(args, dev, rt, scenes, torch):
         "same_as_eager": same,
         "workload": "64x64 seeded displaced spheres (61x40 quads each); per frame y += "
                     "0.02 sin(2 pi (x + f/60)), boxes/centres, PLOC rebuild, compressed "
-                    "traversal tree and 2^28 any-hit shadow rays to a point light, one graph",
+                    "traversal tree and 2^28 any-hit shadow rays to a point light (random "
+                    "surface points, in random order), one graph; frame_ms with the rays "
+                    "grouped by the cells of their origins inside the graph "
+                    "(rtbvh_gpu_batch_plan_order_rays), frame_ms_caller_order without",
     }
-    log(f"large: frame {fm:.1f} ms ({res['frame_mrays_s']} Mrays/s incl. rebuild), eager "
-        f"MBVH any-hit {trace_ms:.1f} ms, eager PLOC {eager_build_ms:.1f} ms, same {same}")
+    log(f"large: frame {fm:.1f} ms ({res['frame_mrays_s']} Mrays/s incl. rebuild; caller's "
+        f"order {fc:.1f} ms, same {same_order}), eager MBVH any-hit {trace_ms:.1f} ms, eager "
+        f"PLOC {eager_build_ms:.1f} ms, same {same}")
     del plan, m, bvh, rays, occ, want, base, verts, aabbs, centers
     torch.cuda.empty_cache()
     return res

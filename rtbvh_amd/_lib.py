@@ -23,6 +23,7 @@ DEVICE_ERROR, OUT_OF_MEMORY, INVALID_ARGUMENT, STACK_OVERFLOW, NO_DEVICE = 5, 6,
 LOCALLY_ORDERED_CLUSTERED = 0
 BINNED_SAH = 1
 TRACE_REFERENCE_ORDER = 0x1
+TRACE_ORDER_RAYS = 0x2
 MISS = 0xFFFFFFFF
 MAX_STACK = 256
 MAX_BATCH_SCENES = 4096
@@ -123,6 +124,7 @@ SIGNATURES = {
     "rtbvh_gpu_batch_plan_launch": (_rc, [RTBatchPlan, _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp,
                                          _vp]),
     "rtbvh_gpu_batch_plan_status": (_rc, [RTBatchPlan, _vp]),
+    "rtbvh_gpu_batch_plan_order_rays": (_rc, [RTBatchPlan, _sz, _vp]),
     "rtbvh_gpu_batch_plan_free": (None, [RTBatchPlan]),
     "rtbvh_gpu_create_batch": (_rc, [_vp, _vp, _sz, _vp, _sz, _vp, ctypes.POINTER(RTBatch)]),
     "rtbvh_gpu_batch_attach_triangles": (_rc, [ctypes.POINTER(RTBatch), _vp, _sz, _vp]),

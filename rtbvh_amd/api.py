@@ -323,21 +323,27 @@ class TraceTree:
         c = counters.cpu()
         return out, int(c[0]), int(c[1])
 
-    def intersect(self, rays: torch.Tensor, out=None):
+    def intersect(self, rays: torch.Tensor, out=None, order_rays: bool = False):
+        """Closest hits; order_rays traces the rays grouped by the cell of
+        their origin (RTBVH_TRACE_ORDER_RAYS: same results, for large sets of
+        unrelated rays; the call then waits for the traversal)."""
         _require_device(rays, "rays")
         out = torch.empty((rays.shape[0], HIT_FLOATS), dtype=torch.float32,
                           device=rays.device) if out is None else out
+        flags = L.TRACE_ORDER_RAYS if order_rays else 0
         L.check(L.lib().rtbvh_gpu_trace_tree_intersect(self._h, _ptr(rays), rays.shape[0],
-                                                       _ptr(out), 0, _stream(rays)),
+                                                       _ptr(out), flags, _stream(rays)),
                 "rtbvh_gpu_trace_tree_intersect")
         return out
 
-    def occluded(self, rays: torch.Tensor, out=None):
+    def occluded(self, rays: torch.Tensor, out=None, order_rays: bool = False):
+        """Any hit per ray (1 occluded, 0 not); order_rays as for intersect."""
         _require_device(rays, "rays")
         out = torch.empty((rays.shape[0],), dtype=torch.uint8,
                           device=rays.device) if out is None else out
+        flags = L.TRACE_ORDER_RAYS if order_rays else 0
         L.check(L.lib().rtbvh_gpu_trace_tree_occluded(self._h, _ptr(rays), rays.shape[0],
-                                                      _ptr(out), 0, _stream(rays)),
+                                                      _ptr(out), flags, _stream(rays)),
                 "rtbvh_gpu_trace_tree_occluded")
         return out
 
@@ -610,6 +616,15 @@ class BatchPlan:
         out = self._out(rays, any_hit, out)
         return self._run(L.lib().rtbvh_gpu_batch_plan_launch, primitives, aabbs, vertices, rays,
                          ray_offsets, out, any_hit)
+
+    def order_rays(self, max_rays: int):
+        """Trace later frames (one scene, at most max_rays rays) grouped by
+        the cells of the ray origins, from plan-owned buffers of 52 bytes per
+        ray (rtbvh_gpu_batch_plan_order_rays); 0 returns to the caller's
+        order."""
+        L.check(L.lib().rtbvh_gpu_batch_plan_order_rays(
+            self._h, int(max_rays), _stream(torch.empty(0, device=self.device))),
+            "rtbvh_gpu_batch_plan_order_rays")
 
     def status(self):
         L.check(L.lib().rtbvh_gpu_batch_plan_status(self._h, _stream(torch.empty(0, device=self.device))),

@@ -158,8 +158,14 @@ ResultCode view_entry(const BatchView& v, const char* what, uint32_t id, const R
                       const uint32_t* ray_offsets, size_t ray_count, RTHit* hits,
                       uint8_t* occluded, uint32_t flags, void* stream,
                       uint64_t* stats = nullptr) {
-    if (flags != 0) {
-        set_error("this traversal supports the lowest-(t, id) rule only (flags 0)");
+    const bool order_rays = (flags & RTBVH_TRACE_ORDER_RAYS) != 0;
+    if ((flags & ~RTBVH_TRACE_ORDER_RAYS) != 0) {
+        set_error("this traversal supports the lowest-(t, id) rule only (flags 0 or "
+                  "RTBVH_TRACE_ORDER_RAYS)");
+        return InvalidArgument;
+    }
+    if (order_rays && (ray_offsets || stats)) {
+        set_error("RTBVH_TRACE_ORDER_RAYS applies to one traversal tree without counters");
         return InvalidArgument;
     }
     if (ray_count > 0xFFFFFFFFull) {
@@ -210,8 +216,26 @@ ResultCode view_entry(const BatchView& v, const char* what, uint32_t id, const R
         set_error("device status word unavailable");
         return DeviceError;
     }
-    const hipError_t e = launch_trace_fast(p, occluded != nullptr, static_cast<hipStream_t>(stream));
-    if (e != hipSuccess) return hip_fail(e, "trace launch");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (!order_rays) {
+        const hipError_t e = launch_trace_fast(p, occluded != nullptr, st);
+        if (e != hipSuccess) return hip_fail(e, "trace launch");
+        return Ok;
+    }
+    Workspace ws;
+    hipError_t e = workspace_acquire(kWsRayOrder, ray_order_bytes(ray_count), &ws);
+    if (e != hipSuccess) return hip_fail(e, "ray order workspace");
+    const RayOrder ro = ray_order_carve(ws.ptr, ray_count);
+    p.rays = ro.sorted;
+    p.hits = hits ? ro.results : nullptr;
+    p.occluded = occluded ? reinterpret_cast<uint8_t*>(ro.results) : nullptr;
+    e = ray_order_sort(ro, reinterpret_cast<const DevRay*>(rays), ray_count, st);
+    if (e == hipSuccess) e = launch_trace_fast(p, occluded != nullptr, st);
+    if (e == hipSuccess)
+        e = ray_order_gather(ro, ray_count, reinterpret_cast<DevHit*>(hits), occluded, nullptr,
+                             st);
+    workspace_release(&ws, st, true);  // the next call may reuse the block
+    if (e != hipSuccess) return hip_fail(e, "ordered trace");
     return Ok;
 }
 

@@ -50,8 +50,11 @@ struct PlanEntry {
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipGraphExec_t exec = nullptr;
     GraphKey key{};
+    void* order_buf = nullptr;  // rtbvh_gpu_batch_plan_order_rays buffers (null: caller's order)
+    RayOrder order{};
     ~PlanEntry() {
         if (exec) (void)hipGraphExecDestroy(exec);
+        if (order_buf) (void)hipFree(order_buf);
         if (ev_in) (void)hipEventDestroy(ev_in);
         if (ev_out) (void)hipEventDestroy(ev_out);
         if (stream) (void)hipStreamDestroy(stream);
@@ -109,6 +112,11 @@ ResultCode check_args(const PlanEntry& pe, const RTAabb* aabbs, const float* cen
         set_error("a batch traces at most 2^32-1 rays (got %zu)", ray_count);
         return InvalidArgument;
     }
+    if (pe.order_buf && ray_count > pe.order.capacity) {
+        set_error("the plan orders at most %llu rays (got %zu)",
+                  (unsigned long long)pe.order.capacity, ray_count);
+        return InvalidArgument;
+    }
     int cur = -1;
     if (hipGetDevice(&cur) != hipSuccess || cur != pe.device) {
         set_error("plan lives on device %d, current device is %d", pe.device, cur);
@@ -158,7 +166,19 @@ hipError_t enqueue_all(PlanEntry& pe, const RTAabb* aabbs, const float* centers,
     p.small_leaves = true;  // traversal-tree leaves
     p.status = device_status_ptr();
     if (p.status == nullptr) return hipErrorInvalidValue;
-    return launch_trace_fast(p, occluded != nullptr, stream);
+    if (!pe.order_buf) return launch_trace_fast(p, occluded != nullptr, stream);
+    // one scene: sorted rays in, results in sorted order, gathered back
+    // (the gather, like the traversal, does nothing while the fail word is set)
+    const RayOrder& ro = pe.order;
+    p.rays = ro.sorted;
+    p.hits = hits ? ro.results : nullptr;
+    p.occluded = occluded ? reinterpret_cast<uint8_t*>(ro.results) : nullptr;
+    if ((e = ray_order_sort(ro, reinterpret_cast<const DevRay*>(rays), ray_count, stream)) !=
+            hipSuccess ||
+        (e = launch_trace_fast(p, occluded != nullptr, stream)) != hipSuccess)
+        return e;
+    return ray_order_gather(ro, ray_count, reinterpret_cast<DevHit*>(hits), occluded, fail,
+                            stream);
 }
 
 }  // namespace
@@ -331,6 +351,49 @@ ResultCode rtbvh_gpu_batch_plan_status(RTBatchPlan plan, void* stream) {
         return Error;
     }
     return rtbvh_gpu_status(stream);
+}
+
+ResultCode rtbvh_gpu_batch_plan_order_rays(RTBatchPlan plan, size_t max_ray_count, void* stream) {
+    PlanEntry* pe = find_plan(plan.id);
+    if (pe == nullptr) {
+        set_error("unknown plan id %u", plan.id);
+        return Error;
+    }
+    if (max_ray_count > 0 && pe->scenes != 1) {
+        set_error("rays are ordered for one-scene plans only (this plan has %u scenes)",
+                  pe->scenes);
+        return InvalidArgument;
+    }
+    if (max_ray_count > 0xFFFFFFFFull) {
+        set_error("a batch traces at most 2^32-1 rays (got %zu)", max_ray_count);
+        return InvalidArgument;
+    }
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != pe->device) {
+        set_error("plan lives on device %d, current device is %d", pe->device, cur);
+        return InvalidArgument;
+    }
+    // earlier frames (graph replays on the plan's stream, eager enqueues on
+    // the caller's) may still use the old buffers
+    hipError_t e = hipStreamSynchronize(pe->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "order rays");
+    if (pe->exec) {
+        (void)hipGraphExecDestroy(pe->exec);
+        pe->exec = nullptr;
+    }
+    if (pe->order_buf) {
+        (void)hipFree(pe->order_buf);
+        pe->order_buf = nullptr;
+        pe->order = RayOrder{};
+    }
+    if (max_ray_count == 0) return Ok;
+    void* buf = nullptr;
+    if ((e = hipMalloc(&buf, ray_order_bytes(max_ray_count))) != hipSuccess)
+        return hip_fail(e, "ray order buffers");
+    pe->order_buf = buf;
+    pe->order = ray_order_carve(buf, max_ray_count);
+    return Ok;
 }
 
 void rtbvh_gpu_batch_plan_free(RTBatchPlan plan) {

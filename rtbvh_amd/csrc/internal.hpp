@@ -198,7 +198,8 @@ enum WorkspaceKind : int {
     kWsBinned = 1,
     kWsTree = 2,
     kWsSpatial = 3,
-    kWsKinds = 4
+    kWsRayOrder = 4,
+    kWsKinds = 5
 };
 constexpr size_t kWorkspaceCacheMax = size_t(1) << 30;
 struct Workspace {
@@ -209,6 +210,27 @@ struct Workspace {
 };
 hipError_t workspace_acquire(WorkspaceKind kind, size_t bytes, Workspace* out);
 void workspace_release(Workspace* w, hipStream_t stream, bool wait);
+
+// Rays traced in the order of their origins' cells (ray_order.hip): sort
+// rays into sorted[], trace sorted[] into results[], gather results[] back
+// into the caller's order.  One block of ray_order_bytes(capacity) holds
+// every buffer; the passes allocate nothing and never wait for the host.
+struct RayOrder {
+    float* frame;         // grid origin and cells per unit length
+    float* parts;         // per-block bounds of the origin sample
+    uint32_t* cells;      // per-cell counters, then their exclusive scan
+    void* scan_scratch;
+    size_t scan_bytes;
+    uint32_t* rank;       // place in the cell, then the ray's sorted slot
+    DevRay* sorted;       // capacity rays
+    DevHit* results;      // capacity hit records (or bytes, for any hit)
+    uint64_t capacity;
+};
+size_t ray_order_bytes(uint64_t ray_count);
+RayOrder ray_order_carve(void* base, uint64_t ray_count);
+hipError_t ray_order_sort(const RayOrder& r, const DevRay* rays, uint64_t n, hipStream_t stream);
+hipError_t ray_order_gather(const RayOrder& r, uint64_t n, DevHit* hits, uint8_t* occluded,
+                            const uint32_t* gate, hipStream_t stream);
 
 ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t stride_floats,
                             size_t n, size_t max_leaf, hipStream_t stream, DeviceBvh* out);
