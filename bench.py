@@ -616,6 +616,9 @@ def main():
     ap.add_argument("--trace-only", action="store_true",
                     help="build, then only the timed bounce-ray steps of the headline trace "
                          "(and the copy calibration) for profiler runs; no JSON line")
+    ap.add_argument("--large-only", action="store_true",
+                    help="only the animated 20M-triangle frames (caller's order, then ordered "
+                         "rays) and their eager comparison, for profiler runs; no JSON line")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape instead of autotuning")
     ap.add_argument("--shape", default="",
@@ -639,6 +642,10 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream(dev)
+
+    if args.large_only:
+        run_large(args, dev, rt, scenes, torch)
+        return
 
     log("generating the 1M-triangle scene")
     verts = scenes.scene_1m()
