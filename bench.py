@@ -18,7 +18,8 @@ MBVH's rates on the same rays (same hits), the spatial-split SAH tree's build
 time and rates (same hits), the roofline of the trace kernel, the CPU
 oracle's rate on a sample of the same rays (host threads stated), the
 batched-vs-plain comparison over 256 scenes of uneven size, and the animated
-20M-triangle any-hit frame.
+20M-triangle any-hit frame (rays in the caller's order, then grouped by the
+cells of their origins inside the frame).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
