@@ -218,10 +218,12 @@ typedef struct RTHit {
  * Morton order, and write each result at the
  * ray's own index — the same results as without the flag, bit for bit.  For
  * large sets of unrelated rays (2^28 shadow rays from random surface points
- * of a 20M-triangle scene: 117 ms in all against 189 ms); it costs three
- * more passes over the rays.  The call takes 52 bytes of device memory per ray
- * (sorted rays, their slots, results in sorted order) from a per-device
- * cache and waits for the traversal before it returns. */
+ * of a 20M-triangle scene: 117 ms in all against 189 ms); it costs two
+ * more passes over the rays.  The call takes device memory from a
+ * per-device cache — 8 bytes per ray for a compressed tree, whose kernel
+ * reads the rays through the order; 52 for a 4-wide tree, traced over a
+ * sorted copy whose results are gathered back — and waits for the
+ * traversal before it returns. */
 #define RTBVH_TRACE_ORDER_RAYS 0x2u
 
 /* GPU build, same arguments as create_bvh but every pointer is device memory
@@ -510,7 +512,7 @@ ResultCode rtbvh_gpu_batch_plan_launch(RTBatchPlan plan, const RTAabb *aabbs, co
  * else rtbvh_gpu_status. */
 ResultCode rtbvh_gpu_batch_plan_status(RTBatchPlan plan, void *stream);
 /* Traces later frames of a one-scene plan as RTBVH_TRACE_ORDER_RAYS does,
- * from buffers for up to max_ray_count rays (52 bytes per ray) that the plan
+ * from buffers for up to max_ray_count rays (8 bytes per ray) that the plan
  * allocates here and keeps; launches of more rays then fail with
  * InvalidArgument.  max_ray_count 0 returns to the caller's order and frees
  * the buffers.  A recorded graph is recorded again on the next launch.  No

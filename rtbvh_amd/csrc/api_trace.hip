@@ -222,18 +222,24 @@ ResultCode view_entry(const BatchView& v, const char* what, uint32_t id, const R
         if (e != hipSuccess) return hip_fail(e, "trace launch");
         return Ok;
     }
+    // compressed trees read each ray through the order; 4-wide trees trace
+    // a sorted copy whose results are gathered back
+    const bool copies = v.cnodes == nullptr;
     Workspace ws;
-    hipError_t e = workspace_acquire(kWsRayOrder, ray_order_bytes(ray_count), &ws);
+    hipError_t e = workspace_acquire(kWsRayOrder, ray_order_bytes(ray_count, copies), &ws);
     if (e != hipSuccess) return hip_fail(e, "ray order workspace");
-    const RayOrder ro = ray_order_carve(ws.ptr, ray_count);
-    p.rays = ro.sorted;
-    p.hits = hits ? ro.results : nullptr;
-    p.occluded = occluded ? reinterpret_cast<uint8_t*>(ro.results) : nullptr;
+    const RayOrder ro = ray_order_carve(ws.ptr, ray_count, copies);
+    if (copies) {
+        p.rays = ro.sorted;
+        p.hits = hits ? ro.results : nullptr;
+        p.occluded = occluded ? reinterpret_cast<uint8_t*>(ro.results) : nullptr;
+    } else {
+        p.ray_ids = ro.ids;
+    }
     e = ray_order_sort(ro, reinterpret_cast<const DevRay*>(rays), ray_count, st);
     if (e == hipSuccess) e = launch_trace_fast(p, occluded != nullptr, st);
-    if (e == hipSuccess)
-        e = ray_order_gather(ro, ray_count, reinterpret_cast<DevHit*>(hits), occluded, nullptr,
-                             st);
+    if (e == hipSuccess && copies)
+        e = ray_order_gather(ro, ray_count, reinterpret_cast<DevHit*>(hits), occluded, st);
     workspace_release(&ws, st, true);  // the next call may reuse the block
     if (e != hipSuccess) return hip_fail(e, "ordered trace");
     return Ok;

@@ -167,18 +167,14 @@ hipError_t enqueue_all(PlanEntry& pe, const RTAabb* aabbs, const float* centers,
     p.status = device_status_ptr();
     if (p.status == nullptr) return hipErrorInvalidValue;
     if (!pe.order_buf) return launch_trace_fast(p, occluded != nullptr, stream);
-    // one scene: sorted rays in, results in sorted order, gathered back
-    // (the gather, like the traversal, does nothing while the fail word is set)
-    const RayOrder& ro = pe.order;
-    p.rays = ro.sorted;
-    p.hits = hits ? ro.results : nullptr;
-    p.occluded = occluded ? reinterpret_cast<uint8_t*>(ro.results) : nullptr;
-    if ((e = ray_order_sort(ro, reinterpret_cast<const DevRay*>(rays), ray_count, stream)) !=
-            hipSuccess ||
-        (e = launch_trace_fast(p, occluded != nullptr, stream)) != hipSuccess)
+    // one scene, compressed tree: the traversal reads the rays through the
+    // order the sort leaves and writes each result at the ray's own index
+    static_assert(kBatchTreeWidth == 8, "ordered plans need the compressed tree's kernel");
+    if ((e = ray_order_sort(pe.order, reinterpret_cast<const DevRay*>(rays), ray_count,
+                            stream)) != hipSuccess)
         return e;
-    return ray_order_gather(ro, ray_count, reinterpret_cast<DevHit*>(hits), occluded, fail,
-                            stream);
+    p.ray_ids = pe.order.ids;
+    return launch_trace_fast(p, occluded != nullptr, stream);
 }
 
 }  // namespace
@@ -389,10 +385,10 @@ ResultCode rtbvh_gpu_batch_plan_order_rays(RTBatchPlan plan, size_t max_ray_coun
     }
     if (max_ray_count == 0) return Ok;
     void* buf = nullptr;
-    if ((e = hipMalloc(&buf, ray_order_bytes(max_ray_count))) != hipSuccess)
+    if ((e = hipMalloc(&buf, ray_order_bytes(max_ray_count, false))) != hipSuccess)
         return hip_fail(e, "ray order buffers");
     pe->order_buf = buf;
-    pe->order = ray_order_carve(buf, max_ray_count);
+    pe->order = ray_order_carve(buf, max_ray_count, false);
     return Ok;
 }
 

@@ -42,6 +42,8 @@ struct TraceParams {
                            // this many lanes of the wave have one (0: never wait)
     bool small_leaves;     // every leaf holds at most 16 triangles (one-item kernel)
     uint32_t wave_ranges;  // set by the launcher: 1 each wave draws from its own ray range
+    const uint32_t* ray_ids;  // non-null (compressed tree, one scene): position j traces ray
+                              // ray_ids[j] and writes its result at ray_ids[j]
 };
 
 
@@ -211,26 +213,29 @@ struct Workspace {
 hipError_t workspace_acquire(WorkspaceKind kind, size_t bytes, Workspace* out);
 void workspace_release(Workspace* w, hipStream_t stream, bool wait);
 
-// Rays traced in the order of their origins' cells (ray_order.hip): sort
-// rays into sorted[], trace sorted[] into results[], gather results[] back
-// into the caller's order.  One block of ray_order_bytes(capacity) holds
-// every buffer; the passes allocate nothing and never wait for the host.
+// Rays traced in the order of their origins' cells (ray_order.hip).  One
+// block of ray_order_bytes(capacity, copies) holds every buffer; the passes
+// allocate nothing and never wait for the host.  Without copies (compressed
+// trees) ray_order_sort leaves ids[] for TraceParams::ray_ids; with copies
+// (4-wide trees) it leaves the rays in sorted[], the traversal writes
+// results[] and ray_order_gather puts them in the caller's order.
 struct RayOrder {
-    float* frame;         // grid origin and cells per unit length
+    float* frame;         // grid origin, cells per unit length, cells per axis
     float* parts;         // per-block bounds of the origin sample
     uint32_t* cells;      // per-cell counters, then their exclusive scan
     void* scan_scratch;
     size_t scan_bytes;
-    uint32_t* rank;       // place in the cell, then the ray's sorted slot
-    DevRay* sorted;       // capacity rays
-    DevHit* results;      // capacity hit records (or bytes, for any hit)
+    uint32_t* rank;       // place in the cell (copies: then the ray's sorted slot)
+    uint32_t* ids;        // no copies: the caller's ray index at each sorted slot
+    DevRay* sorted;       // copies: capacity rays
+    DevHit* results;      // copies: capacity hit records (or bytes, for any hit)
     uint64_t capacity;
 };
-size_t ray_order_bytes(uint64_t ray_count);
-RayOrder ray_order_carve(void* base, uint64_t ray_count);
+size_t ray_order_bytes(uint64_t ray_count, bool copies);
+RayOrder ray_order_carve(void* base, uint64_t ray_count, bool copies);
 hipError_t ray_order_sort(const RayOrder& r, const DevRay* rays, uint64_t n, hipStream_t stream);
 hipError_t ray_order_gather(const RayOrder& r, uint64_t n, DevHit* hits, uint8_t* occluded,
-                            const uint32_t* gate, hipStream_t stream);
+                            hipStream_t stream);
 
 ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t stride_floats,
                             size_t n, size_t max_leaf, hipStream_t stream, DeviceBvh* out);

@@ -25,9 +25,15 @@
 //                 on the cell's counter (the order within a cell is
 //                 arbitrary)
 //   scan          exclusive, in place, over the 2^3b counters
+// then, for compressed 8-wide trees (whose traversal kernel can read rays
+// through an index):
+//   scatter_ids   ids[cell start + place] = i
+//   (the traversal takes position j's ray from rays[ids[j]] and writes its
+//   result at ids[j]: no ray is copied, no result gathered; 8 bytes per ray)
+// and for 4-wide trees:
 //   scatter       ray i to sorted[cell start + place]; rank[i] = that slot
 //   (the traversal of sorted[] into results[])
-//   gather        out[i] = results[rank[i]]
+//   gather        out[i] = results[rank[i]]  (52 bytes per ray in all)
 #include <math.h>
 
 #include "internal.hpp"
@@ -203,12 +209,18 @@ __global__ void __launch_bounds__(kBlock)
     rank[i] = slot;
 }
 
+__global__ void __launch_bounds__(kBlock)
+    scatter_ids_kernel(const float4* rays, uint64_t n, const float* frame, const uint32_t* cells,
+                       const uint32_t* rank, uint32_t* ids) {
+    const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    ids[cells[origin_cell(rays[2 * i], frame)] + rank[i]] = uint32_t(i);
+}
+
 typedef float HitVec __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(kBlock)
-    gather_hits_kernel(const HitVec* results, const uint32_t* rank, uint64_t n,
-                       const uint32_t* gate, HitVec* hits) {
-    if (gate && *gate) return;  // the traversal did nothing: leave the output alone
+    gather_hits_kernel(const HitVec* results, const uint32_t* rank, uint64_t n, HitVec* hits) {
     const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
     __builtin_nontemporal_store(results[rank[i]], hits + i);
@@ -216,8 +228,7 @@ __global__ void __launch_bounds__(kBlock)
 
 __global__ void __launch_bounds__(kBlock)
     gather_occluded_kernel(const uint8_t* results, const uint32_t* rank, uint64_t n,
-                           const uint32_t* gate, uint8_t* occluded) {
-    if (gate && *gate) return;
+                           uint8_t* occluded) {
     const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
     occluded[i] = results[rank[i]];
@@ -227,15 +238,19 @@ dim3 grid_for(uint64_t n) { return dim3(uint32_t((n + kBlock - 1) / kBlock)); }
 
 }  // namespace
 
-size_t ray_order_bytes(uint64_t ray_count) {
+size_t ray_order_bytes(uint64_t ray_count, bool copies) {
     const uint32_t cells = cell_count(ray_count);
-    return align256(8 * sizeof(float)) + align256(size_t(kParts) * 6 * sizeof(float)) +
-           align256(size_t(cells) * sizeof(uint32_t)) + align256(scan_scratch_bytes(cells)) +
-           align256(ray_count * sizeof(uint32_t)) + align256(ray_count * sizeof(DevRay)) +
-           align256(ray_count * sizeof(DevHit));
+    size_t b = align256(8 * sizeof(float)) + align256(size_t(kParts) * 6 * sizeof(float)) +
+               align256(size_t(cells) * sizeof(uint32_t)) + align256(scan_scratch_bytes(cells)) +
+               align256(ray_count * sizeof(uint32_t));
+    if (copies)
+        b += align256(ray_count * sizeof(DevRay)) + align256(ray_count * sizeof(DevHit));
+    else
+        b += align256(ray_count * sizeof(uint32_t));
+    return b;
 }
 
-RayOrder ray_order_carve(void* base, uint64_t ray_count) {
+RayOrder ray_order_carve(void* base, uint64_t ray_count, bool copies) {
     const uint32_t cells = cell_count(ray_count);
     RayOrder r{};
     char* p = static_cast<char*>(base);
@@ -250,16 +265,21 @@ RayOrder ray_order_carve(void* base, uint64_t ray_count) {
     p += r.scan_bytes;
     r.rank = reinterpret_cast<uint32_t*>(p);
     p += align256(ray_count * sizeof(uint32_t));
-    r.sorted = reinterpret_cast<DevRay*>(p);
-    p += align256(ray_count * sizeof(DevRay));
-    r.results = reinterpret_cast<DevHit*>(p);
+    if (copies) {
+        r.sorted = reinterpret_cast<DevRay*>(p);
+        p += align256(ray_count * sizeof(DevRay));
+        r.results = reinterpret_cast<DevHit*>(p);
+    } else {
+        r.ids = reinterpret_cast<uint32_t*>(p);
+    }
     r.capacity = ray_count;
     return r;
 }
 
 hipError_t ray_order_sort(const RayOrder& r, const DevRay* rays, uint64_t n, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    if (n > r.capacity || n > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    if (n > r.capacity || n > 0xFFFFFFFFull || (r.sorted == nullptr) == (r.ids == nullptr))
+        return hipErrorInvalidValue;
     const float4* rv = reinterpret_cast<const float4*>(rays);
     const uint64_t samples = uint64_t(kSampleRounds) * kParts * kBlock;
     const uint64_t step = (n + samples - 1) / samples;
@@ -274,23 +294,26 @@ hipError_t ray_order_sort(const RayOrder& r, const DevRay* rays, uint64_t n, hip
     hipError_t e = scan_exclusive_u32(r.cells, cells, nullptr, r.scan_scratch,
                                       align256(scan_scratch_bytes(cells)), stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(scatter_kernel, grid_for(n), dim3(kBlock), 0, stream, rv, n, r.frame,
-                       r.cells, r.rank, reinterpret_cast<float4*>(r.sorted));
+    if (r.ids)
+        hipLaunchKernelGGL(scatter_ids_kernel, grid_for(n), dim3(kBlock), 0, stream, rv, n,
+                           r.frame, r.cells, r.rank, r.ids);
+    else
+        hipLaunchKernelGGL(scatter_kernel, grid_for(n), dim3(kBlock), 0, stream, rv, n, r.frame,
+                           r.cells, r.rank, reinterpret_cast<float4*>(r.sorted));
     return hipGetLastError();
 }
 
 hipError_t ray_order_gather(const RayOrder& r, uint64_t n, DevHit* hits, uint8_t* occluded,
-                            const uint32_t* gate, hipStream_t stream) {
+                            hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    if (n > r.capacity) return hipErrorInvalidValue;
+    if (n > r.capacity || r.results == nullptr) return hipErrorInvalidValue;
     if (hits)
         hipLaunchKernelGGL(gather_hits_kernel, grid_for(n), dim3(kBlock), 0, stream,
-                           reinterpret_cast<const HitVec*>(r.results), r.rank, n, gate,
+                           reinterpret_cast<const HitVec*>(r.results), r.rank, n,
                            reinterpret_cast<HitVec*>(hits));
     else
         hipLaunchKernelGGL(gather_occluded_kernel, grid_for(n), dim3(kBlock), 0, stream,
-                           reinterpret_cast<const uint8_t*>(r.results), r.rank, n, gate,
-                           occluded);
+                           reinterpret_cast<const uint8_t*>(r.results), r.rank, n, occluded);
     return hipGetLastError();
 }
 

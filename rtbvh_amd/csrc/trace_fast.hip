@@ -722,7 +722,7 @@ __device__ __forceinline__ float qbyte(uint32_t w, int i) {
     return float((w >> ((i & 3) * 8)) & 0xFFu);
 }
 
-template <bool ANY, bool STATS, bool BATCH>
+template <bool ANY, bool STATS, bool BATCH, bool IDS = false>
 __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(8)))
 cw8_kernel(TraceParams p) {
     if (p.gate && *p.gate) return;  // the tree this launch would read is incomplete
@@ -746,6 +746,7 @@ cw8_kernel(TraceParams p) {
     uint32_t ngx = 0, ngy = 0;  // node group in hand
     uint32_t tgx = 0, tgy = 0;  // triangle group in hand
     bool active = false;
+    uint32_t cur = 0;  // IDS: the caller's index of the ray in hand
     const auto push = [&](uint32_t x, uint32_t y) -> bool {
         if (sp < ldepth) {
             lds_put(lbase + uint32_t(sp) * lstep, x, y);
@@ -768,7 +769,8 @@ cw8_kernel(TraceParams p) {
             take = sched.take(can_take, takers);
         const uint64_t rid = sched.rid;
         if (take) {
-            const float4* r4 = reinterpret_cast<const float4*>(p.rays + rid);
+            if (IDS) cur = p.ray_ids[rid];
+            const float4* r4 = reinterpret_cast<const float4*>(p.rays + (IDS ? cur : rid));
             const float4 ra = r4[0];
             const float4 rb = r4[1];
             ox = ra.x;
@@ -789,9 +791,9 @@ cw8_kernel(TraceParams p) {
                              (dy != dy) || (dz != dz);
             if (nan || p.node_count == 0) {
                 if (ANY) {
-                    p.occluded[rid] = 0;
+                    p.occluded[IDS ? cur : rid] = 0;
                 } else {
-                    store_hit(p.hits, rid, bt, 0.f, 0.f, RTBVH_MISS);
+                    store_hit(p.hits, IDS ? cur : rid, bt, 0.f, 0.f, RTBVH_MISS);
                 }
                 sched.next();
             } else {
@@ -960,9 +962,9 @@ cw8_kernel(TraceParams p) {
         if (overflow) atomicOr(p.status, kStatusStackOverflow);
         if (found || overflow || (ngy < kNodeGroup && (tgy & 0xFFu) == 0u && sp == 0)) {
             if (ANY) {
-                p.occluded[rid] = (bp != RTBVH_MISS) ? 1 : 0;
+                p.occluded[IDS ? cur : rid] = (bp != RTBVH_MISS) ? 1 : 0;
             } else {
-                store_hit(p.hits, rid, overflow ? __builtin_nanf("") : bt, bu, bv,
+                store_hit(p.hits, IDS ? cur : rid, overflow ? __builtin_nanf("") : bt, bu, bv,
                           overflow ? RTBVH_MISS : bp);
             }
             active = false;
@@ -988,9 +990,10 @@ int current_dev() {
     return dev;
 }
 
-template <bool ANY, bool STATS, bool PREP, bool BATCH = false, bool ITEM = false, bool CW8 = false>
+template <bool ANY, bool STATS, bool PREP, bool BATCH = false, bool ITEM = false, bool CW8 = false,
+          bool IDS = false>
 hipError_t launch_fast(const TraceParams& p0, hipStream_t stream) {
-    auto k = CW8 ? cw8_kernel<ANY, STATS, BATCH>
+    auto k = CW8 ? cw8_kernel<ANY, STATS, BATCH, IDS>
                  : (ITEM ? mbvh_item_kernel<ANY, STATS, BATCH>
                          : mbvh_fast_kernel<ANY, STATS, PREP, BATCH>);
     const int dev = current_dev();
@@ -1064,12 +1067,17 @@ TraceConfig get_trace_config() {
 }
 
 hipError_t launch_trace_fast(const TraceParams& p, bool any_hit, hipStream_t stream) {
+    // rays read through an index: compressed trees, one scene, no counters
+    if (p.ray_ids && (!p.cnodes || p.ray_offsets || p.stats)) return hipErrorInvalidValue;
     if (p.cnodes) {  // compressed 8-wide tree
         if (!p.tris || (p.stats && (any_hit || p.ray_offsets))) return hipErrorInvalidValue;
         if (p.ray_offsets)
             return any_hit ? launch_fast<true, false, true, true, false, true>(p, stream)
                            : launch_fast<false, false, true, true, false, true>(p, stream);
         if (p.stats) return launch_fast<false, true, true, false, false, true>(p, stream);
+        if (p.ray_ids)
+            return any_hit ? launch_fast<true, false, true, false, false, true, true>(p, stream)
+                           : launch_fast<false, false, true, false, false, true, true>(p, stream);
         return any_hit ? launch_fast<true, false, true, false, false, true>(p, stream)
                        : launch_fast<false, false, true, false, false, true>(p, stream);
     }

@@ -12,8 +12,10 @@ each ray's own index:
     cells of the grid);
   * ray counts of 1, 255, 257 and one past 2^20 (the origin sample's
     stride), and two calls in a row, the second smaller (the cached block
-    reused); one call of 2^24 + 2^22 rays, whose 52 bytes per ray exceed the
-    cache's 1 GiB (allocated and freed per call);
+    reused); compressed trees read the rays through the order, 4-wide trees
+    trace a sorted copy and gather the results back; one call of 2^24 + 2^22
+    rays on a 4-wide tree, whose 52 bytes per ray exceed the cache's 1 GiB
+    (allocated and freed per call);
   * closest hits of a small case against the oracle's brute force;
   * a plan with ordering on: eager enqueue and graph replays (moved vertices
     in between) equal the plan without it; more rays than it was set up for
@@ -104,12 +106,14 @@ def test_ordered_against_brute_force():
 
 
 def test_ordered_beyond_the_cached_block():
-    """2^24 + 2^22 rays need more than the 1 GiB the per-device cache holds
-    for this kind: the call allocates and frees its own block."""
+    """2^24 + 2^22 rays on a 4-wide tree (a sorted copy: 52 bytes per ray)
+    need more than the 1 GiB the per-device cache holds for this kind: the
+    call allocates and frees its own block."""
     print("[order_uncached]", flush=True)
     verts = scenes.random_soup(20000, 0.02, seed=6)
     a_d, c_d, v_d = _scene(verts)
-    tree = rt.Builder(c_d, aabbs=a_d).construct_locally_ordered_clustered().trace_tree(v_d)
+    bvh = rt.Builder(c_d, aabbs=a_d).construct_locally_ordered_clustered()
+    tree = bvh.trace_tree(v_d, width=4)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(7)
     n = (1 << 24) + (1 << 22)
