@@ -32,6 +32,21 @@ def test_library_exports_every_header_symbol():
     assert set(names) == set(L.SIGNATURES)
 
 
+def test_header_constants_match_the_binding():
+    """Every #define of rtbvh.h that the Python binding mirrors has the same
+    value there (trace flags, sentinels, limits)."""
+    text = open(HEADER).read()
+    defs = {m.group(1): int(m.group(2).rstrip("uU"), 0) for m in
+            re.finditer(r"^#define RTBVH_(\w+) \(?(0x[0-9A-Fa-f]+u?|\d+u?)", text, re.M)}
+    pairs = {"TRACE_REFERENCE_ORDER": L.TRACE_REFERENCE_ORDER,
+             "TRACE_ORDER_RAYS": L.TRACE_ORDER_RAYS, "MISS": L.MISS,
+             "MAX_STACK": L.MAX_STACK, "MAX_BATCH_SCENES": L.MAX_BATCH_SCENES,
+             "NOALLOC_MAX_PRIMS": L.NOALLOC_MAX_PRIMS}
+    for name, value in pairs.items():
+        assert defs.get(name) == value, name
+    assert L.TRACE_ORDER_RAYS & L.TRACE_REFERENCE_ORDER == 0
+
+
 def test_struct_layouts():
     """rtbvh_ffi `same_size` test (lib.rs:855-866): BvhNode = RTBvhNode = 32,
     MbvhNode = RTMbvhNode = 128, Aabb = RTAabb = 32; RTBvh/RTMbvh are 32 bytes
