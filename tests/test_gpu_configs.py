@@ -21,7 +21,8 @@ path):
   * 20M triangles animated, one frame as one recorded graph (PLOC rebuild +
     traversal tree + 2^28 any-hit shadow rays): the frame's build bit-identical
     to the oracle's LOCB, and the shadow flags of a 400k-ray sample equal to
-    the oracle's any-hit traversal.
+    the oracle's any-hit traversal; then the same frame with the rays ordered
+    by the cells of their origins inside the graph: all 2^28 flags equal.
 """
 import os
 
@@ -450,3 +451,11 @@ def test_c5_animated_20m_frame_matches_oracle():
         assert 0.2 < want.mean() < 0.95
         assert np.array_equal(got, want), f"frame {frame}: {(got != want).sum()} flags differ"
         del nodes, idx, m
+    # frame 2 again, the rays ordered by origin cell inside the graph
+    print("[c5] frame 2, ordered rays", flush=True)
+    plan.order_rays(n_rays)
+    occ_ordered = torch.empty_like(occ)
+    plan.launch(centers, verts, rays, None, aabbs=aabbs, any_hit=True, out=occ_ordered)
+    plan.status()
+    torch.cuda.synchronize()
+    assert torch.equal(occ_ordered, occ)
