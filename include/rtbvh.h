@@ -33,11 +33,11 @@
  *
  * Memory: the builders behind create_bvh, rtbvh_gpu_create_bvh,
  * rtbvh_gpu_create_spatial_bvh, rtbvh_gpu_create_batch and
- * rtbvh_gpu_create_trace_tree* keep their scratch between calls, one block
- * per builder kind (PLOC, binned SAH, spatial SAH, traversal tree) and
- * device, grown to the largest build up to 1 GiB each and held until the
- * process ends; concurrent builds on one device and larger builds allocate
- * and free their own.
+ * rtbvh_gpu_create_trace_tree*, and traversals with RTBVH_TRACE_ORDER_RAYS,
+ * keep their scratch between calls, one block per kind (PLOC, binned SAH,
+ * spatial SAH, traversal tree, ray order) and device, grown to the largest
+ * request up to 1 GiB each and held until the process ends; concurrent
+ * calls on one device and larger requests allocate and free their own.
  */
 #ifndef RTBVH_H
 #define RTBVH_H
@@ -218,12 +218,12 @@ typedef struct RTHit {
  * Morton order, and write each result at the
  * ray's own index — the same results as without the flag, bit for bit.  For
  * large sets of unrelated rays (2^28 shadow rays from random surface points
- * of a 20M-triangle scene: 117 ms in all against 189 ms); it costs two
- * more passes over the rays.  The call takes device memory from a
- * per-device cache — 8 bytes per ray for a compressed tree, whose kernel
- * reads the rays through the order; 52 for a 4-wide tree, traced over a
- * sorted copy whose results are gathered back — and waits for the
- * traversal before it returns. */
+ * of a 20M-triangle scene: 117 ms in all against 189 ms); it costs a
+ * stable radix sort of the rays' cells.  The call takes device memory from
+ * a per-device cache — about 16 bytes per ray for a compressed tree, whose
+ * kernel reads the rays through the order; 64 for a 4-wide tree, traced
+ * over a sorted copy whose results are gathered back — and waits for the
+ * traversal before it returns.  At most 2^32 - 4096 rays per call. */
 #define RTBVH_TRACE_ORDER_RAYS 0x2u
 
 /* GPU build, same arguments as create_bvh but every pointer is device memory
@@ -512,7 +512,8 @@ ResultCode rtbvh_gpu_batch_plan_launch(RTBatchPlan plan, const RTAabb *aabbs, co
  * else rtbvh_gpu_status. */
 ResultCode rtbvh_gpu_batch_plan_status(RTBatchPlan plan, void *stream);
 /* Traces later frames of a one-scene plan as RTBVH_TRACE_ORDER_RAYS does,
- * from buffers for up to max_ray_count rays (8 bytes per ray) that the plan
+ * from buffers for up to max_ray_count rays (about 16 bytes per ray, at
+ * most 2^32 - 4096 rays) that the plan
  * allocates here and keeps; launches of more rays then fail with
  * InvalidArgument.  max_ray_count 0 returns to the caller's order and frees
  * the buffers.  A recorded graph is recorded again on the next launch.  No

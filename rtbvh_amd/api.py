@@ -619,8 +619,8 @@ class BatchPlan:
 
     def order_rays(self, max_rays: int):
         """Trace later frames (one scene, at most max_rays rays) grouped by
-        the cells of the ray origins, from plan-owned buffers of 8 bytes per
-        ray (rtbvh_gpu_batch_plan_order_rays); 0 returns to the caller's
+        the cells of the ray origins, from plan-owned buffers of about 16
+        bytes per ray (rtbvh_gpu_batch_plan_order_rays); 0 returns to the caller's
         order."""
         L.check(L.lib().rtbvh_gpu_batch_plan_order_rays(
             self._h, int(max_rays), _stream(torch.empty(0, device=self.device))),

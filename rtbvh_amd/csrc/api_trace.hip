@@ -168,6 +168,11 @@ ResultCode view_entry(const BatchView& v, const char* what, uint32_t id, const R
         set_error("RTBVH_TRACE_ORDER_RAYS applies to one traversal tree without counters");
         return InvalidArgument;
     }
+    if (order_rays && ray_count > kRadixPairsMax) {
+        set_error("RTBVH_TRACE_ORDER_RAYS orders at most %u rays per call (got %zu)",
+                  kRadixPairsMax, ray_count);
+        return InvalidArgument;
+    }
     if (ray_count > 0xFFFFFFFFull) {
         set_error("at most 2^32-1 rays per call (got %zu)", ray_count);
         return InvalidArgument;
@@ -229,17 +234,18 @@ ResultCode view_entry(const BatchView& v, const char* what, uint32_t id, const R
     hipError_t e = workspace_acquire(kWsRayOrder, ray_order_bytes(ray_count, copies), &ws);
     if (e != hipSuccess) return hip_fail(e, "ray order workspace");
     const RayOrder ro = ray_order_carve(ws.ptr, ray_count, copies);
+    const uint32_t* ids = nullptr;
+    e = ray_order_sort(ro, reinterpret_cast<const DevRay*>(rays), ray_count, st, &ids);
     if (copies) {
         p.rays = ro.sorted;
         p.hits = hits ? ro.results : nullptr;
         p.occluded = occluded ? reinterpret_cast<uint8_t*>(ro.results) : nullptr;
     } else {
-        p.ray_ids = ro.ids;
+        p.ray_ids = ids;
     }
-    e = ray_order_sort(ro, reinterpret_cast<const DevRay*>(rays), ray_count, st);
     if (e == hipSuccess) e = launch_trace_fast(p, occluded != nullptr, st);
     if (e == hipSuccess && copies)
-        e = ray_order_gather(ro, ray_count, reinterpret_cast<DevHit*>(hits), occluded, st);
+        e = ray_order_gather(ro, ids, ray_count, reinterpret_cast<DevHit*>(hits), occluded, st);
     workspace_release(&ws, st, true);  // the next call may reuse the block
     if (e != hipSuccess) return hip_fail(e, "ordered trace");
     return Ok;

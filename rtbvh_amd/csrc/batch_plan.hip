@@ -170,10 +170,11 @@ hipError_t enqueue_all(PlanEntry& pe, const RTAabb* aabbs, const float* centers,
     // one scene, compressed tree: the traversal reads the rays through the
     // order the sort leaves and writes each result at the ray's own index
     static_assert(kBatchTreeWidth == 8, "ordered plans need the compressed tree's kernel");
-    if ((e = ray_order_sort(pe.order, reinterpret_cast<const DevRay*>(rays), ray_count,
-                            stream)) != hipSuccess)
+    const uint32_t* ids = nullptr;
+    if ((e = ray_order_sort(pe.order, reinterpret_cast<const DevRay*>(rays), ray_count, stream,
+                            &ids)) != hipSuccess)
         return e;
-    p.ray_ids = pe.order.ids;
+    p.ray_ids = ids;
     return launch_trace_fast(p, occluded != nullptr, stream);
 }
 
@@ -360,8 +361,8 @@ ResultCode rtbvh_gpu_batch_plan_order_rays(RTBatchPlan plan, size_t max_ray_coun
                   pe->scenes);
         return InvalidArgument;
     }
-    if (max_ray_count > 0xFFFFFFFFull) {
-        set_error("a batch traces at most 2^32-1 rays (got %zu)", max_ray_count);
+    if (max_ray_count > kRadixPairsMax) {
+        set_error("a plan orders at most %u rays (got %zu)", kRadixPairsMax, max_ray_count);
         return InvalidArgument;
     }
     int cur = -1;

@@ -872,6 +872,43 @@ __global__ void check_kernel(const PlocState* result, uint32_t n_final, uint32_t
 
 }  // namespace
 
+// Stable LSD radix sort of n (key, value) pairs by the low `bits` bits of
+// the keys, 8 bits a pass, with the kernels of the Morton-code sort.  The
+// pairs ping-pong between (k0, v0) and (k1, v1); *second is set when the
+// sorted pairs end in (k1, v1).  n <= kRadixPairsMax (the kernels index in
+// 32 bits, a tile past n included).
+size_t radix_pairs_scratch_bytes(uint32_t n) {
+    const size_t tiles = (size_t(n) + kSortTile - 1) / kSortTile;
+    return align_up(256 * tiles * 4) + align_up(scan_scratch_bytes(uint32_t(256 * tiles)));
+}
+
+hipError_t radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t n,
+                            uint32_t bits, void* scratch, size_t scratch_bytes,
+                            hipStream_t stream, bool* second) {
+    *second = false;
+    if (n == 0 || bits == 0) return hipSuccess;
+    if (n > kRadixPairsMax || scratch_bytes < radix_pairs_scratch_bytes(n))
+        return hipErrorInvalidValue;
+    const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
+    const size_t counts_bytes = align_up(size_t(256) * tiles * 4);
+    uint32_t* counts = static_cast<uint32_t*>(scratch);
+    void* scan = static_cast<char*>(scratch) + counts_bytes;
+    uint32_t *ki = k0, *vi = v0, *ko = k1, *vo = v1;
+    for (uint32_t shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(tiles), dim3(256), 0, stream, ki, n, shift,
+                           tiles, counts);
+        const hipError_t e = scan_exclusive_u32(counts, 256u * tiles, nullptr, scan,
+                                                scratch_bytes - counts_bytes, stream);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3(tiles), dim3(kSortBlock), 0, stream, ki,
+                           vi, n, shift, tiles, counts, ko, vo);
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+        *second = !*second;
+    }
+    return hipGetLastError();
+}
+
 // Every buffer of one PLOC build of n primitives in `scenes` scenes.  The
 // tree (nodes, indices) has allocations of its own so that an eager build
 // can hand it over and free the rest.

@@ -213,29 +213,38 @@ struct Workspace {
 hipError_t workspace_acquire(WorkspaceKind kind, size_t bytes, Workspace* out);
 void workspace_release(Workspace* w, hipStream_t stream, bool wait);
 
+// Stable radix sort of (key, value) pairs (build_ploc.hip): the low `bits`
+// bits of the keys, 8 a pass; *second: the result is in (k1, v1).
+constexpr uint32_t kRadixPairsMax = 0xFFFFF000u;
+size_t radix_pairs_scratch_bytes(uint32_t n);
+hipError_t radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t n,
+                            uint32_t bits, void* scratch, size_t scratch_bytes,
+                            hipStream_t stream, bool* second);
+
 // Rays traced in the order of their origins' cells (ray_order.hip).  One
 // block of ray_order_bytes(capacity, copies) holds every buffer; the passes
-// allocate nothing and never wait for the host.  Without copies (compressed
-// trees) ray_order_sort leaves ids[] for TraceParams::ray_ids; with copies
-// (4-wide trees) it leaves the rays in sorted[], the traversal writes
-// results[] and ray_order_gather puts them in the caller's order.
+// allocate nothing and never wait for the host.  ray_order_sort leaves the
+// caller's ray index of each sorted slot (*ids): compressed trees trace
+// through it (TraceParams::ray_ids); with copies (4-wide trees) it also
+// copies the rays into sorted[], the traversal writes results[] and
+// ray_order_gather puts them in the caller's order.
 struct RayOrder {
-    float* frame;         // grid origin, cells per unit length, cells per axis
-    float* parts;         // per-block bounds of the origin sample
-    uint32_t* cells;      // per-cell counters, then their exclusive scan
-    void* scan_scratch;
-    size_t scan_bytes;
-    uint32_t* rank;       // place in the cell (copies: then the ray's sorted slot)
-    uint32_t* ids;        // no copies: the caller's ray index at each sorted slot
-    DevRay* sorted;       // copies: capacity rays
-    DevHit* results;      // copies: capacity hit records (or bytes, for any hit)
+    float* frame;          // grid origin, cells per unit length, cells per axis
+    float* parts;          // per-block bounds of the origin sample
+    uint32_t *k0, *v0;     // (cell, ray) pairs
+    uint32_t *k1, *v1;     // the sort's other buffers
+    void* radix_scratch;
+    size_t radix_bytes;
+    DevRay* sorted;        // copies: capacity rays
+    DevHit* results;       // copies: capacity hit records (or bytes, for any hit)
     uint64_t capacity;
 };
 size_t ray_order_bytes(uint64_t ray_count, bool copies);
 RayOrder ray_order_carve(void* base, uint64_t ray_count, bool copies);
-hipError_t ray_order_sort(const RayOrder& r, const DevRay* rays, uint64_t n, hipStream_t stream);
-hipError_t ray_order_gather(const RayOrder& r, uint64_t n, DevHit* hits, uint8_t* occluded,
-                            hipStream_t stream);
+hipError_t ray_order_sort(const RayOrder& r, const DevRay* rays, uint64_t n, hipStream_t stream,
+                          const uint32_t** ids);
+hipError_t ray_order_gather(const RayOrder& r, const uint32_t* ids, uint64_t n, DevHit* hits,
+                            uint8_t* occluded, hipStream_t stream);
 
 ResultCode gpu_build_binned(const RTAabb* aabbs, const float* centers, size_t stride_floats,
                             size_t n, size_t max_leaf, hipStream_t stream, DeviceBvh* out);

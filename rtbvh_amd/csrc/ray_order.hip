@@ -15,34 +15,27 @@
 // written back in the caller's order.
 //
 // Passes (stream-ordered, no host reads, no allocation, capturable in a
-// graph: the cell counters are cleared by a kernel, not a memset):
+// graph):
 //   bounds_part   per-block min/max over a sample of at most 2^20 origins
 //                 spread evenly over the rays (the grid's frame is a
 //                 grouping heuristic: rays outside it fall into the border
-//                 cells); the blocks also clear the counters
+//                 cells)
 //   bounds_final  one block reduces the parts to the frame (lo, 2^b / extent)
-//   count         each ray's cell; its place in the cell from an atomic
-//                 on the cell's counter (the order within a cell is
-//                 arbitrary)
-//   scan          exclusive, in place, over the 2^3b counters
-// then, for compressed 8-wide trees (whose traversal kernel can read rays
-// through an index):
-//   scatter_ids   ids[cell start + place] = i
-//   (the traversal takes position j's ray from rays[ids[j]] and writes its
-//   result at ids[j]: no ray is copied, no result gathered; 8 bytes per ray)
-// and for 4-wide trees:
-//   scatter       ray i to sorted[cell start + place]; rank[i] = that slot
+//   keys          (cell of ray i, i) for every ray
+//   radix sort    stable, by the 3b bits of the cells, 8 a pass (the
+//                 Morton-code sort's kernels); the sorted values are the
+//                 caller's ray index of each slot, ids[j]
+// Compressed 8-wide trees trace through ids (the traversal takes position
+// j's ray from rays[ids[j]] and writes its result at ids[j]). For 4-wide
+// trees:
+//   copy          sorted[j] = rays[ids[j]]
 //   (the traversal of sorted[] into results[])
-//   gather        out[i] = results[rank[i]]  (52 bytes per ray in all)
+//   gather        out[ids[j]] = results[j]
 #include <math.h>
 
 #include "internal.hpp"
 
 namespace rtbvh {
-
-hipError_t scan_exclusive_u32(uint32_t* data, uint32_t count, uint32_t* total, void* scratch,
-                              size_t scratch_bytes, hipStream_t stream);
-size_t scan_scratch_bytes(uint32_t count);
 
 namespace {
 
@@ -61,7 +54,6 @@ uint32_t axis_bits(uint64_t n) {
     return b < kMinAxisBits ? kMinAxisBits : (b > kMaxAxisBits ? kMaxAxisBits : b);
 }
 
-uint32_t cell_count(uint64_t n) { return 1u << (3 * axis_bits(n)); }
 
 // bits 0..9 of x to bits 0, 3, 6, ..., 27
 __device__ __forceinline__ uint32_t spread3(uint32_t x) {
@@ -98,10 +90,7 @@ __device__ __forceinline__ float wave_max(float x) {
 
 // parts[b * 6 + k]: min x, y, z, max x, y, z of block b's sample.
 __global__ void __launch_bounds__(kBlock)
-    bounds_part_kernel(const float4* rays, uint64_t n, uint64_t step, float* parts,
-                       uint32_t* cells, uint32_t cells_n) {
-    for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < cells_n; c += kParts * kBlock)
-        cells[c] = 0u;
+    bounds_part_kernel(const float4* rays, uint64_t n, uint64_t step, float* parts) {
     float mn0 = INFINITY, mn1 = INFINITY, mn2 = INFINITY;
     float mx0 = -INFINITY, mx1 = -INFINITY, mx2 = -INFINITY;
     for (uint32_t r = 0; r < kSampleRounds; ++r) {
@@ -189,49 +178,38 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 __global__ void __launch_bounds__(kBlock)
-    count_kernel(const float4* rays, uint64_t n, const float* frame, uint32_t* cells,
-                 uint32_t* rank) {
+    keys_kernel(const float4* rays, uint64_t n, const float* frame, uint32_t* keys,
+                uint32_t* vals) {
     const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
-    rank[i] = atomicAdd(&cells[origin_cell(rays[2 * i], frame)], 1u);
+    keys[i] = origin_cell(rays[2 * i], frame);
+    vals[i] = uint32_t(i);
 }
 
 __global__ void __launch_bounds__(kBlock)
-    scatter_kernel(const float4* rays, uint64_t n, const float* frame, const uint32_t* cells,
-                   uint32_t* rank, float4* sorted) {
-    const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const float4 a = rays[2 * i];
-    const float4 b = rays[2 * i + 1];
-    const uint32_t slot = cells[origin_cell(a, frame)] + rank[i];
-    sorted[2 * uint64_t(slot)] = a;
-    sorted[2 * uint64_t(slot) + 1] = b;
-    rank[i] = slot;
-}
-
-__global__ void __launch_bounds__(kBlock)
-    scatter_ids_kernel(const float4* rays, uint64_t n, const float* frame, const uint32_t* cells,
-                       const uint32_t* rank, uint32_t* ids) {
-    const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (i >= n) return;
-    ids[cells[origin_cell(rays[2 * i], frame)] + rank[i]] = uint32_t(i);
+    copy_kernel(const float4* rays, const uint32_t* ids, uint64_t n, float4* sorted) {
+    const uint64_t j = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (j >= n) return;
+    const uint64_t i = ids[j];
+    sorted[2 * j] = rays[2 * i];
+    sorted[2 * j + 1] = rays[2 * i + 1];
 }
 
 typedef float HitVec __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(kBlock)
-    gather_hits_kernel(const HitVec* results, const uint32_t* rank, uint64_t n, HitVec* hits) {
-    const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (i >= n) return;
-    __builtin_nontemporal_store(results[rank[i]], hits + i);
+    gather_hits_kernel(const HitVec* results, const uint32_t* ids, uint64_t n, HitVec* hits) {
+    const uint64_t j = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (j >= n) return;
+    __builtin_nontemporal_store(results[j], hits + ids[j]);
 }
 
 __global__ void __launch_bounds__(kBlock)
-    gather_occluded_kernel(const uint8_t* results, const uint32_t* rank, uint64_t n,
+    gather_occluded_kernel(const uint8_t* results, const uint32_t* ids, uint64_t n,
                            uint8_t* occluded) {
-    const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (i >= n) return;
-    occluded[i] = results[rank[i]];
+    const uint64_t j = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (j >= n) return;
+    occluded[ids[j]] = results[j];
 }
 
 dim3 grid_for(uint64_t n) { return dim3(uint32_t((n + kBlock - 1) / kBlock)); }
@@ -239,81 +217,76 @@ dim3 grid_for(uint64_t n) { return dim3(uint32_t((n + kBlock - 1) / kBlock)); }
 }  // namespace
 
 size_t ray_order_bytes(uint64_t ray_count, bool copies) {
-    const uint32_t cells = cell_count(ray_count);
+    const uint32_t n32 = uint32_t(ray_count < kRadixPairsMax ? ray_count : kRadixPairsMax);
     size_t b = align256(8 * sizeof(float)) + align256(size_t(kParts) * 6 * sizeof(float)) +
-               align256(size_t(cells) * sizeof(uint32_t)) + align256(scan_scratch_bytes(cells)) +
-               align256(ray_count * sizeof(uint32_t));
+               4 * align256(ray_count * sizeof(uint32_t)) +
+               align256(radix_pairs_scratch_bytes(n32));
     if (copies)
         b += align256(ray_count * sizeof(DevRay)) + align256(ray_count * sizeof(DevHit));
-    else
-        b += align256(ray_count * sizeof(uint32_t));
     return b;
 }
 
 RayOrder ray_order_carve(void* base, uint64_t ray_count, bool copies) {
-    const uint32_t cells = cell_count(ray_count);
+    const uint32_t n32 = uint32_t(ray_count < kRadixPairsMax ? ray_count : kRadixPairsMax);
     RayOrder r{};
     char* p = static_cast<char*>(base);
     r.frame = reinterpret_cast<float*>(p);
     p += align256(8 * sizeof(float));
     r.parts = reinterpret_cast<float*>(p);
     p += align256(size_t(kParts) * 6 * sizeof(float));
-    r.cells = reinterpret_cast<uint32_t*>(p);
-    p += align256(size_t(cells) * sizeof(uint32_t));
-    r.scan_scratch = p;
-    r.scan_bytes = align256(scan_scratch_bytes(cells));
-    p += r.scan_bytes;
-    r.rank = reinterpret_cast<uint32_t*>(p);
-    p += align256(ray_count * sizeof(uint32_t));
+    uint32_t** words[4] = {&r.k0, &r.v0, &r.k1, &r.v1};
+    for (uint32_t** w : words) {
+        *w = reinterpret_cast<uint32_t*>(p);
+        p += align256(ray_count * sizeof(uint32_t));
+    }
+    r.radix_scratch = p;
+    r.radix_bytes = align256(radix_pairs_scratch_bytes(n32));
+    p += r.radix_bytes;
     if (copies) {
         r.sorted = reinterpret_cast<DevRay*>(p);
         p += align256(ray_count * sizeof(DevRay));
         r.results = reinterpret_cast<DevHit*>(p);
-    } else {
-        r.ids = reinterpret_cast<uint32_t*>(p);
     }
     r.capacity = ray_count;
     return r;
 }
 
-hipError_t ray_order_sort(const RayOrder& r, const DevRay* rays, uint64_t n, hipStream_t stream) {
+hipError_t ray_order_sort(const RayOrder& r, const DevRay* rays, uint64_t n, hipStream_t stream,
+                          const uint32_t** ids) {
+    *ids = r.v0;
     if (n == 0) return hipSuccess;
-    if (n > r.capacity || n > 0xFFFFFFFFull || (r.sorted == nullptr) == (r.ids == nullptr))
-        return hipErrorInvalidValue;
+    if (n > r.capacity || n > kRadixPairsMax) return hipErrorInvalidValue;
     const float4* rv = reinterpret_cast<const float4*>(rays);
     const uint64_t samples = uint64_t(kSampleRounds) * kParts * kBlock;
     const uint64_t step = (n + samples - 1) / samples;
-    // fewer rays than the buffers were carved for: fewer cells (n <= capacity)
-    const uint32_t cells = cell_count(n);
     hipLaunchKernelGGL(bounds_part_kernel, dim3(kParts), dim3(kBlock), 0, stream, rv, n, step,
-                       r.parts, r.cells, cells);
+                       r.parts);
     hipLaunchKernelGGL(bounds_final_kernel, dim3(1), dim3(kBlock), 0, stream, r.parts,
                        float(1u << axis_bits(n)), r.frame);
-    hipLaunchKernelGGL(count_kernel, grid_for(n), dim3(kBlock), 0, stream, rv, n, r.frame,
-                       r.cells, r.rank);
-    hipError_t e = scan_exclusive_u32(r.cells, cells, nullptr, r.scan_scratch,
-                                      align256(scan_scratch_bytes(cells)), stream);
+    hipLaunchKernelGGL(keys_kernel, grid_for(n), dim3(kBlock), 0, stream, rv, n, r.frame, r.k0,
+                       r.v0);
+    bool second = false;
+    hipError_t e = radix_sort_pairs(r.k0, r.v0, r.k1, r.v1, uint32_t(n), 3 * axis_bits(n),
+                                    r.radix_scratch, r.radix_bytes, stream, &second);
     if (e != hipSuccess) return e;
-    if (r.ids)
-        hipLaunchKernelGGL(scatter_ids_kernel, grid_for(n), dim3(kBlock), 0, stream, rv, n,
-                           r.frame, r.cells, r.rank, r.ids);
-    else
-        hipLaunchKernelGGL(scatter_kernel, grid_for(n), dim3(kBlock), 0, stream, rv, n, r.frame,
-                           r.cells, r.rank, reinterpret_cast<float4*>(r.sorted));
+    *ids = second ? r.v1 : r.v0;
+    if (r.sorted)
+        hipLaunchKernelGGL(copy_kernel, grid_for(n), dim3(kBlock), 0, stream, rv, *ids, n,
+                           reinterpret_cast<float4*>(r.sorted));
     return hipGetLastError();
 }
 
-hipError_t ray_order_gather(const RayOrder& r, uint64_t n, DevHit* hits, uint8_t* occluded,
-                            hipStream_t stream) {
+hipError_t ray_order_gather(const RayOrder& r, const uint32_t* ids, uint64_t n, DevHit* hits,
+                            uint8_t* occluded, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     if (n > r.capacity || r.results == nullptr) return hipErrorInvalidValue;
     if (hits)
         hipLaunchKernelGGL(gather_hits_kernel, grid_for(n), dim3(kBlock), 0, stream,
-                           reinterpret_cast<const HitVec*>(r.results), r.rank, n,
+                           reinterpret_cast<const HitVec*>(r.results), ids, n,
                            reinterpret_cast<HitVec*>(hits));
     else
         hipLaunchKernelGGL(gather_occluded_kernel, grid_for(n), dim3(kBlock), 0, stream,
-                           reinterpret_cast<const uint8_t*>(r.results), r.rank, n, occluded);
+                           reinterpret_cast<const uint8_t*>(r.results), ids, n, occluded);
     return hipGetLastError();
 }
 

@@ -14,7 +14,7 @@ each ray's own index:
     stride), and two calls in a row, the second smaller (the cached block
     reused); compressed trees read the rays through the order, 4-wide trees
     trace a sorted copy and gather the results back; one call of 2^24 + 2^22
-    rays on a 4-wide tree, whose 52 bytes per ray exceed the cache's 1 GiB
+    rays on a 4-wide tree, whose 64 bytes per ray exceed the cache's 1 GiB
     (allocated and freed per call);
   * closest hits of a small case against the oracle's brute force;
   * a plan with ordering on: eager enqueue and graph replays (moved vertices
@@ -106,7 +106,7 @@ def test_ordered_against_brute_force():
 
 
 def test_ordered_beyond_the_cached_block():
-    """2^24 + 2^22 rays on a 4-wide tree (a sorted copy: 52 bytes per ray)
+    """2^24 + 2^22 rays on a 4-wide tree (a sorted copy: 64 bytes per ray)
     need more than the 1 GiB the per-device cache holds for this kind: the
     call allocates and frees its own block."""
     print("[order_uncached]", flush=True)
