@@ -333,18 +333,37 @@ __global__ void radix_hist_kernel(const uint32_t* keys, uint32_t n, uint32_t shi
     counts[threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
 }
 
+// A tile's pairs are ranked into LDS in digit order (stable: rounds, then
+// waves, then lanes in order), then written out so that consecutive threads
+// write consecutive slots of one digit's run.  The tile's count of digit d is
+// the difference of consecutive entries of the scanned counts (digit-major,
+// the last entry's end is n).
 __global__ void __launch_bounds__(kSortBlock)
     radix_scatter_kernel(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t n,
                          uint32_t shift, uint32_t tiles, const uint32_t* offsets,
                          uint32_t* keys_out, uint32_t* vals_out) {
     constexpr int kWaves = kSortBlock / 64;
-    __shared__ uint32_t run[256];
+    static_assert(kSortBlock == 256, "one thread per digit");
+    __shared__ uint32_t run[256];    // next local slot of each digit
+    __shared__ uint32_t start[256];  // first local slot of each digit
+    __shared__ uint32_t gbase[256];  // first global slot of each digit's run of this tile
     __shared__ uint32_t wcnt[kWaves][256];
+    __shared__ uint32_t scan_sh[kWaves];
+    __shared__ uint32_t skeys[kSortTile];
+    __shared__ uint32_t svals[kSortTile];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = threadIdx.x >> 6;
-    run[threadIdx.x] = offsets[threadIdx.x * tiles + blockIdx.x];
-    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     const uint32_t base = blockIdx.x * kSortTile;
+    {
+        const uint32_t idx = threadIdx.x * tiles + blockIdx.x;
+        const uint32_t first = offsets[idx];
+        const uint32_t next = idx + 1 < 256u * tiles ? offsets[idx + 1] : n;
+        gbase[threadIdx.x] = first;
+        const uint32_t s = block_exclusive_n<kSortBlock>(next - first, scan_sh, nullptr);
+        start[threadIdx.x] = s;
+        run[threadIdx.x] = s;
+    }
+    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     for (int r = 0; r < kSortItems; ++r) {
         for (int w = 0; w < kWaves; ++w) wcnt[w][threadIdx.x] = 0;
         __syncthreads();
@@ -376,10 +395,18 @@ __global__ void __launch_bounds__(kSortBlock)
             run[threadIdx.x] += add;
         }
         if (valid) {
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+            skeys[pos] = key;
+            svals[pos] = val;
         }
         __syncthreads();
+    }
+    const uint32_t count = n - base < uint32_t(kSortTile) ? n - base : uint32_t(kSortTile);
+    for (uint32_t k = threadIdx.x; k < count; k += kSortBlock) {
+        const uint32_t key = skeys[k];
+        const uint32_t d = (key >> shift) & 255u;
+        const uint32_t g = gbase[d] + (k - start[d]);
+        keys_out[g] = key;
+        vals_out[g] = svals[k];
     }
 }
 
