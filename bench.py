@@ -895,11 +895,15 @@ def main():
     achieved = value * 1e6 * b_ray / 1e9
     # HBM bytes per launch, measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs
     # of this kernel (scaled by a plain copy), kept beside the kernel trace
+    # (the entry of the launch shape this run picked when there is one)
     traffic, traffic_shape = None, None
     if os.path.exists(PMC_SUMMARY):
         with open(PMC_SUMMARY) as fh:
             pmc = json.load(fh)
         traffic, traffic_shape = pmc.get("hbm_bytes_per_launch"), pmc.get("launch_shape")
+        for entry in pmc.get("by_shape", []):
+            if entry.get("launch_shape") == dict(tuned):
+                traffic, traffic_shape = entry["hbm_bytes_per_launch"], entry["launch_shape"]
 
     # ---- CPU oracle on a sample of the same rays ----------------------------
     # the reference's algorithm restated in C++ (oracle/refcpu.cpp): its
