@@ -54,7 +54,6 @@ uint32_t axis_bits(uint64_t n) {
     return b < kMinAxisBits ? kMinAxisBits : (b > kMaxAxisBits ? kMaxAxisBits : b);
 }
 
-
 // bits 0..9 of x to bits 0, 3, 6, ..., 27
 __device__ __forceinline__ uint32_t spread3(uint32_t x) {
     x = (x | (x << 16)) & 0x030000FFu;
