@@ -222,8 +222,11 @@ typedef struct RTHit {
  * stable radix sort of the rays' cells.  The call takes device memory from
  * a per-device cache — about 16 bytes per ray for a compressed tree, whose
  * kernel reads the rays through the order; 64 for a 4-wide tree, traced
- * over a sorted copy whose results are gathered back — and waits for the
- * traversal before it returns.  At most 2^32 - 4096 rays per call. */
+ * over a sorted copy whose results are gathered back.  The call is
+ * stream-ordered like the others: a later call that reuses the cached block
+ * waits for this one on the device; a request beyond the cache's 1 GiB is
+ * allocated and freed per call, and then the call waits for its work.  At
+ * most 2^32 - 4096 rays per call. */
 #define RTBVH_TRACE_ORDER_RAYS 0x2u
 
 /* GPU build, same arguments as create_bvh but every pointer is device memory

@@ -326,7 +326,7 @@ class TraceTree:
     def intersect(self, rays: torch.Tensor, out=None, order_rays: bool = False):
         """Closest hits; order_rays traces the rays grouped by the cell of
         their origin (RTBVH_TRACE_ORDER_RAYS: same results, for large sets of
-        unrelated rays; the call then waits for the traversal)."""
+        unrelated rays)."""
         _require_device(rays, "rays")
         out = torch.empty((rays.shape[0], HIT_FLOATS), dtype=torch.float32,
                           device=rays.device) if out is None else out

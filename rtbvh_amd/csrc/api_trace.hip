@@ -231,7 +231,7 @@ ResultCode view_entry(const BatchView& v, const char* what, uint32_t id, const R
     // a sorted copy whose results are gathered back
     const bool copies = v.cnodes == nullptr;
     Workspace ws;
-    hipError_t e = workspace_acquire(kWsRayOrder, ray_order_bytes(ray_count, copies), &ws);
+    hipError_t e = workspace_acquire_on(kWsRayOrder, ray_order_bytes(ray_count, copies), st, &ws);
     if (e != hipSuccess) return hip_fail(e, "ray order workspace");
     const RayOrder ro = ray_order_carve(ws.ptr, ray_count, copies);
     const uint32_t* ids = nullptr;
@@ -246,7 +246,7 @@ ResultCode view_entry(const BatchView& v, const char* what, uint32_t id, const R
     if (e == hipSuccess) e = launch_trace_fast(p, occluded != nullptr, st);
     if (e == hipSuccess && copies)
         e = ray_order_gather(ro, ids, ray_count, reinterpret_cast<DevHit*>(hits), occluded, st);
-    workspace_release(&ws, st, true);  // the next call may reuse the block
+    workspace_release_on(&ws, st);  // the next user waits for this call's work
     if (e != hipSuccess) return hip_fail(e, "ordered trace");
     return Ok;
 }

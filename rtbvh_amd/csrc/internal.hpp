@@ -212,6 +212,10 @@ struct Workspace {
 };
 hipError_t workspace_acquire(WorkspaceKind kind, size_t bytes, Workspace* out);
 void workspace_release(Workspace* w, hipStream_t stream, bool wait);
+// Stream-ordered use (workspace.hip): no host wait for a cached block.
+hipError_t workspace_acquire_on(WorkspaceKind kind, size_t bytes, hipStream_t stream,
+                                Workspace* out);
+void workspace_release_on(Workspace* w, hipStream_t stream);
 
 // Stable radix sort of (key, value) pairs (build_ploc.hip): the low `bits`
 // bits of the keys, 8 a pass; *second: the result is in (k1, v1).

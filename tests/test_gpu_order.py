@@ -21,7 +21,10 @@ each ray's own index:
     in between) equal the plan without it; more rays than it was set up for
     and multi-scene plans are refused; order_rays(0) returns to the caller's
     order;
-  * the flag is refused where it does not apply (batches, MBVH calls).
+  * the flag is refused where it does not apply (batches, MBVH calls);
+  * eager ordered calls are stream-ordered: several in a row on one stream
+    and on a second stream, with no synchronisation between them, reuse the
+    cached block one after the other and each returns its own results.
 """
 import numpy as np
 import pytest
@@ -89,6 +92,33 @@ def test_ordered_equals_unordered(width):
     assert torch.equal(got, want)
     assert 0 < float(want.float().mean()) < 1
     assert _same_bits(tree.intersect(shadow, order_rays=True), tree.intersect(shadow))
+
+
+def test_ordered_calls_in_stream_order():
+    print("[order_streams]", flush=True)
+    verts = scenes.noisy_sphere(90, 70, seed=14)
+    a_d, c_d, v_d = _scene(verts)
+    tree = rt.Builder(c_d, aabbs=a_d).construct_locally_ordered_clustered().trace_tree(v_d)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(21)
+    sets = [scenes.shadow_rays_torch(v_d, m, (2.0 + k, 3.0, 1.0 - k), "cuda", gen)
+            for k, m in enumerate((300000, 120000, 300000, 50000))]
+    want_occ = [tree.occluded(r) for r in sets]
+    want_hit = [tree.intersect(r) for r in sets]
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    got = []
+    for k, r in enumerate(sets):  # no synchronisation between the calls
+        if k == 2:
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                got.append((tree.occluded(r, order_rays=True), tree.intersect(r, order_rays=True)))
+        else:
+            got.append((tree.occluded(r, order_rays=True), tree.intersect(r, order_rays=True)))
+    torch.cuda.synchronize()
+    for k in range(len(sets)):
+        assert torch.equal(got[k][0], want_occ[k]), k
+        assert _same_bits(got[k][1], want_hit[k]), k
 
 
 def test_ordered_against_brute_force():
