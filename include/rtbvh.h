@@ -218,8 +218,8 @@ typedef struct RTHit {
  * Morton order, and write each result at the
  * ray's own index — the same results as without the flag, bit for bit.  For
  * large sets of unrelated rays (2^28 shadow rays from random surface points
- * of a 20M-triangle scene: 117 ms in all against 189 ms); it costs a
- * stable radix sort of the rays' cells.  The call takes device memory from
+ * of a 20M-triangle scene: ordering and any-hit traversal 85 ms against
+ * 157 ms); it costs a stable radix sort of the rays' cells.  The call takes device memory from
  * a per-device cache — about 16 bytes per ray for a compressed tree, whose
  * kernel reads the rays through the order; 64 for a 4-wide tree, traced
  * over a sorted copy whose results are gathered back.  The call is
